@@ -1,0 +1,279 @@
+"""Python view of the C ABI: contexts, device graphs, phase 1 and phase 2.
+
+Every call goes through libdmolp_hip.so; errors raise DmolpError with the
+library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+
+GEN_RMAT, GEN_CHUNG_LU, GEN_GRID = 1, 2, 3
+EXIT_NAMES = {0: "max_rounds", 1: "empty_boundary", 2: "no_change", 3: "converged"}
+
+
+class DmolpError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"dmolp status {status}: {message}")
+        self.status = status
+
+
+def _p(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _check(rc: int, ctx=None):
+    if rc != 0:
+        msg = _abi.lib().dmolp_last_error(ctx)
+        raise DmolpError(rc, (msg or b"").decode(errors="replace"))
+
+
+def rmat_spec(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1) -> _abi.GenSpec:
+    return _abi.GenSpec(kind=GEN_RMAT, scale=scale, edge_factor=edge_factor, a=a, b=b, c=c,
+                        seed=seed)
+
+
+def chung_lu_spec(n, m, gamma=2.0, dmax=3.0e6, seed=3) -> _abi.GenSpec:
+    return _abi.GenSpec(kind=GEN_CHUNG_LU, n=n, m=m, gamma=gamma, dmax=dmax, seed=seed)
+
+
+def grid_spec(rows, cols) -> _abi.GenSpec:
+    return _abi.GenSpec(kind=GEN_GRID, rows=rows, cols=cols)
+
+
+def _take_csr(csr: _abi.Csr):
+    n, nnz = csr.n, csr.nnz
+    off = np.ctypeslib.as_array(csr.off, shape=(n + 1,)).copy()
+    col = (np.ctypeslib.as_array(csr.col, shape=(nnz,)).copy() if nnz > 0
+           else np.zeros(0, np.int32))
+    _abi.lib().dmolp_csr_free(C.byref(csr))
+    return off, col
+
+
+def gen_host(spec: _abi.GenSpec):
+    """Generate a graph on the host: returns (off int64[n+1], col int32[nnz])."""
+    csr = _abi.Csr()
+    _check(_abi.lib().dmolp_gen_host(C.byref(spec), C.byref(csr)))
+    return _take_csr(csr)
+
+
+def read_adjacency_text(path: str):
+    csr = _abi.Csr()
+    _check(_abi.lib().dmolp_read_adjacency_text(str(path).encode(), C.byref(csr)))
+    return _take_csr(csr)
+
+
+def write_adjacency_text(path: str, off: np.ndarray, col: np.ndarray):
+    off = np.ascontiguousarray(off, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    _check(_abi.lib().dmolp_write_adjacency_text(str(path).encode(), len(off) - 1,
+                                                 _p(off, C.c_int64), _p(col, C.c_int32)))
+
+
+@dataclass
+class Phase1Result:
+    labels: np.ndarray
+    seeds: np.ndarray
+    vc: np.ndarray
+    ec: np.ndarray
+    metrics: _abi.Phase1Metrics
+
+
+@dataclass
+class Phase2Result:
+    labels: np.ndarray | None
+    vc: np.ndarray
+    ic: np.ndarray
+    metrics: _abi.Phase2Metrics
+    trace: dict
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(_abi.lib().dmolp_ctx_create(device, C.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _abi.lib().dmolp_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tuning(self) -> _abi.Tuning:
+        t = _abi.Tuning()
+        _check(_abi.lib().dmolp_ctx_get_tuning(self._h, C.byref(t)), self._h)
+        return t
+
+    def set_tuning(self, heavy_degree=0, chunk_edges=0, grid_blocks=0):
+        t = _abi.Tuning(heavy_degree=heavy_degree, chunk_edges=chunk_edges, block_threads=0,
+                        grid_blocks=grid_blocks)
+        _check(_abi.lib().dmolp_ctx_set_tuning(self._h, C.byref(t)), self._h)
+
+    def set_profiling(self, on: bool):
+        _check(_abi.lib().dmolp_ctx_set_profiling(self._h, 1 if on else 0), self._h)
+
+    def round_times(self, group: int) -> np.ndarray:
+        n = C.c_int()
+        _check(_abi.lib().dmolp_ctx_round_times(self._h, group, None, 0, C.byref(n)), self._h)
+        out = np.zeros(n.value, np.float64)
+        if n.value:
+            _check(_abi.lib().dmolp_ctx_round_times(self._h, group, _p(out, C.c_double), n.value,
+                                                    C.byref(n)), self._h)
+        return out
+
+    def upload(self, off: np.ndarray, col: np.ndarray) -> "Graph":
+        off = np.ascontiguousarray(off)
+        col = np.ascontiguousarray(col, np.int32)
+        ob = 8 if off.dtype == np.int64 else 4
+        if ob == 4:
+            off = off.astype(np.int32, copy=False)
+        h = C.c_void_p()
+        _check(_abi.lib().dmolp_graph_upload(self._h, len(off) - 1, len(col),
+                                             off.ctypes.data_as(C.c_void_p), ob,
+                                             _p(col, C.c_int32), C.byref(h)), self._h)
+        return Graph(self, h)
+
+    def generate(self, spec: _abi.GenSpec) -> "Graph":
+        h = C.c_void_p()
+        _check(_abi.lib().dmolp_graph_generate(self._h, C.byref(spec), C.byref(h)), self._h)
+        return Graph(self, h)
+
+    def penalties(self, k, vc, ic, t):
+        vc = np.ascontiguousarray(vc, np.int64)
+        ic = np.ascontiguousarray(ic, np.int64)
+        pen = np.zeros(k, np.float64)
+        re = np.zeros(k, np.float64)
+        _check(_abi.lib().dmolp_penalties(self._h, k, _p(vc, C.c_int64), _p(ic, C.c_int64), t,
+                                          _p(pen, C.c_double), _p(re, C.c_double)), self._h)
+        return pen, re
+
+
+class Graph:
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self._h = h
+        n, nnz = C.c_int64(), C.c_int64()
+        _check(_abi.lib().dmolp_graph_info(h, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def free(self):
+        if self._h:
+            _abi.lib().dmolp_graph_free(self.ctx.handle, self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.ctx.handle:
+                self.free()
+        except Exception:
+            pass
+
+    def download(self):
+        off = np.zeros(self.n + 1, np.int64)
+        col = np.zeros(self.nnz, np.int32)
+        _check(_abi.lib().dmolp_graph_download(self.ctx.handle, self._h, _p(off, C.c_int64),
+                                               _p(col, C.c_int32)), self.ctx.handle)
+        return off, col
+
+    def set_labels(self, labels):
+        labels = np.ascontiguousarray(labels, np.int32)
+        assert labels.shape == (self.n,)
+        _check(_abi.lib().dmolp_graph_set_labels(self.ctx.handle, self._h, _p(labels, C.c_int32)),
+               self.ctx.handle)
+
+    def checkpoint_labels(self, restore: bool = False):
+        _check(_abi.lib().dmolp_graph_labels_checkpoint(self.ctx.handle, self._h,
+                                                        1 if restore else 0), self.ctx.handle)
+
+    def get_labels(self) -> np.ndarray:
+        out = np.zeros(self.n, np.int32)
+        _check(_abi.lib().dmolp_graph_get_labels(self.ctx.handle, self._h, _p(out, C.c_int32)),
+               self.ctx.handle)
+        return out
+
+    def phase1(self, k: int, theta: int, want_labels=True) -> Phase1Result:
+        labels = np.zeros(self.n, np.int32) if want_labels else None
+        seeds = np.zeros(k, np.int32)
+        vc = np.zeros(k, np.int64)
+        ec = np.zeros(k, np.int64)
+        m = _abi.Phase1Metrics()
+        _check(_abi.lib().dmolp_phase1(
+            self.ctx.handle, self._h, k, theta,
+            _p(labels, C.c_int32) if want_labels else None, _p(seeds, C.c_int32),
+            _p(vc, C.c_int64), _p(ec, C.c_int64), C.byref(m)), self.ctx.handle)
+        return Phase1Result(labels, seeds, vc, ec, m)
+
+    def phase2(self, k: int, max_rounds: int, epsilon: float, want_labels=True,
+               want_trace=True) -> Phase2Result:
+        vc = np.zeros(k, np.int64)
+        ic = np.zeros(k, np.int64)
+        m = _abi.Phase2Metrics()
+        tr_arrays = {}
+        tr = None
+        if want_trace and max_rounds > 0:
+            tr = _abi.Trace()
+            for name, _ in _abi.Trace._fields_:
+                a = np.full(max_rounds, -7, np.int64)
+                tr_arrays[name] = a
+                setattr(tr, name, _p(a, C.c_int64))
+        _check(_abi.lib().dmolp_phase2(self.ctx.handle, self._h, k, max_rounds, epsilon,
+                                       _p(vc, C.c_int64), _p(ic, C.c_int64), C.byref(m),
+                                       C.byref(tr) if tr is not None else None),
+               self.ctx.handle)
+        trace = {name: a[:m.rounds].copy() for name, a in tr_arrays.items()}
+        labels = self.get_labels() if want_labels else None
+        return Phase2Result(labels, vc, ic, m, trace)
+
+    def stats(self, k, labels):
+        labels = np.ascontiguousarray(labels, np.int32)
+        vc = np.zeros(k, np.int64)
+        ic = np.zeros(k, np.int64)
+        cut = C.c_int64()
+        _check(_abi.lib().dmolp_stats(self.ctx.handle, self._h, k, _p(labels, C.c_int32),
+                                      _p(vc, C.c_int64), _p(ic, C.c_int64), C.byref(cut)),
+               self.ctx.handle)
+        return vc, ic, cut.value
+
+    def select_boundary(self, labels, prev, re, k, t):
+        labels = np.ascontiguousarray(labels, np.int32)
+        prev = np.ascontiguousarray(prev, np.int32)
+        re = np.ascontiguousarray(re, np.float64)
+        out = np.zeros(self.n, np.int32)
+        nout = C.c_int64()
+        _check(_abi.lib().dmolp_select_boundary(
+            self.ctx.handle, self._h, _p(labels, C.c_int32), _p(prev, C.c_int32), len(prev),
+            _p(re, C.c_double), k, t, _p(out, C.c_int32), C.byref(nout)), self.ctx.handle)
+        return out[:nout.value].copy()
+
+    def lp_round(self, labels, b, pen, k):
+        labels = np.ascontiguousarray(labels, np.int32)
+        b = np.ascontiguousarray(b, np.int32)
+        pen = np.ascontiguousarray(pen, np.float64)
+        out = np.zeros(self.n, np.int32)
+        _check(_abi.lib().dmolp_lp_round(self.ctx.handle, self._h, _p(labels, C.c_int32),
+                                         _p(b, C.c_int32), len(b), _p(pen, C.c_double), k,
+                                         _p(out, C.c_int32)), self.ctx.handle)
+        return out

@@ -1,0 +1,601 @@
+// api.hip -- the C ABI of include/dmolp_hip.h: contexts, graphs, phase 1,
+// phase 2 and the test hooks.  Every entry point catches library errors
+// and returns a status; nothing throws across the boundary.
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gen_host.hpp"
+#include "internal.hpp"
+#include "phase1.hpp"
+#include "phase2.hpp"
+#include "rows.hpp"
+
+namespace dmolp {
+void gen_device(dmolp_ctx *ctx, const dmolp_gen_spec *spec, dmolp_graph *g);
+void graph_widen_offsets(dmolp_ctx *ctx, const int32_t *d_src, int64_t *d_dst, int64_t count);
+void phase2_penalties_device(dmolp_ctx *ctx, int k, const unsigned long long *d_vc,
+                             const unsigned long long *d_ic, int t, double *d_pen, double *d_re);
+void phase2_select_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t);
+void phase2_lp_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
+
+namespace {
+
+__global__ void k_bin_ids(GraphDev g, const int32_t *ids, int64_t count, ListDev out, int H,
+                          int CH, unsigned long long *bad) {
+    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+        const int64_t i = b0 + lane_id();
+        bool valid = i < count;
+        int32_t v = valid ? ids[i] : 0;
+        if (valid && (v < 0 || v >= g.n)) { atomicAdd(bad, 1ull); valid = false; }
+        const int64_t d = valid ? g.off[v + 1] - g.off[v] : 0;
+        list_append(out, valid, v, d, H, CH);
+    }
+}
+
+thread_local std::string g_tls_error;
+
+int fail(dmolp_ctx *ctx, int status, const std::string &msg) {
+    if (ctx) ctx->last_error = msg;
+    g_tls_error = msg;
+    return status;
+}
+
+template <typename F>
+int guarded(dmolp_ctx *ctx, F &&f) {
+    try {
+        f();
+        return DMOLP_OK;
+    } catch (const Error &e) {
+        return fail(ctx, e.status, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, DMOLP_E_NOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(ctx, DMOLP_E_HIP, e.what());
+    }
+}
+
+// Upload an id list into a binned vertex list.
+void upload_list(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *ids, int64_t count, List &L) {
+    hipStream_t s = ctx->stream;
+    DMOLP_HIP_CHECK(hipMemsetAsync(L.cnt.p, 0, L.cnt.bytes, s));
+    if (count == 0) return;
+    DevMem d(sizeof(int32_t) * (size_t)count);
+    DevMem bad(sizeof(unsigned long long));
+    DMOLP_HIP_CHECK(hipMemsetAsync(bad.p, 0, bad.bytes, s));
+    DMOLP_HIP_CHECK(hipMemcpyAsync(d.p, ids, d.bytes, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_bin_ids, dim3(ctx->grid_blocks()), dim3(256), 0, s, g->dev(),
+                       d.as<int32_t>(), count, L.dev, ctx->tune.heavy_degree, ctx->tune.chunk_edges,
+                       bad.as<unsigned long long>());
+    DMOLP_HIP_CHECK(hipGetLastError());
+    unsigned long long nb = 0;
+    DMOLP_HIP_CHECK(hipMemcpyAsync(&nb, bad.p, sizeof(nb), hipMemcpyDeviceToHost, s));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    DMOLP_REQUIRE(nb == 0, DMOLP_E_ARG, "vertex id outside [0, n) in list");
+}
+
+// Download a binned list as sorted ids.
+std::vector<int32_t> download_list(dmolp_ctx *ctx, const List &L) {
+    hipStream_t s = ctx->stream;
+    unsigned long long c[8];
+    DMOLP_HIP_CHECK(hipMemcpyAsync(c, L.cnt.p, sizeof(c), hipMemcpyDeviceToHost, s));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<int32_t> out((size_t)(c[LC_LIGHT] + c[LC_SLOTS]));
+    if (c[LC_LIGHT])
+        DMOLP_HIP_CHECK(hipMemcpyAsync(out.data(), L.light.p, sizeof(int32_t) * c[LC_LIGHT],
+                                       hipMemcpyDeviceToHost, s));
+    if (c[LC_SLOTS])
+        DMOLP_HIP_CHECK(hipMemcpyAsync(out.data() + c[LC_LIGHT], L.hv.p,
+                                       sizeof(int32_t) * c[LC_SLOTS], hipMemcpyDeviceToHost, s));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    std::sort(out.begin(), out.end());
+    return out;
+}
+
+void set_labels(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels) {
+    if (!g->labels.p) g->labels.alloc(sizeof(int32_t) * (size_t)g->n);
+    if (g->n > 0)
+        DMOLP_HIP_CHECK(hipMemcpyAsync(g->labels.p, labels, sizeof(int32_t) * (size_t)g->n,
+                                       hipMemcpyHostToDevice, ctx->stream));
+    g->labels_valid = true;
+}
+
+void get_labels(dmolp_ctx *ctx, const dmolp_graph *g, int32_t *labels) {
+    DMOLP_REQUIRE(g->labels_valid, DMOLP_E_STATE, "graph has no resident labels");
+    if (g->n > 0)
+        DMOLP_HIP_CHECK(hipMemcpyAsync(labels, g->labels.p, sizeof(int32_t) * (size_t)g->n,
+                                       hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+// Full stats sweep on the resident labels: vc, ic, cut (phase2.cpp:55-117,
+// 214-253).  Returns the device error code (0 when fine).
+int stats_sweep(dmolp_ctx *ctx, dmolp_graph *g, int k, std::vector<int64_t> &vc,
+                std::vector<int64_t> &ic, int64_t &cut) {
+    Phase2Work &w = phase2_work(ctx, g, k);
+    phase2_init_enqueue(ctx, g, w);
+    Ctl c{};
+    vc.assign((size_t)k, 0);
+    ic.assign((size_t)k, 0);
+    DMOLP_HIP_CHECK(hipMemcpyAsync(&c, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipMemcpyAsync(vc.data(), w.vc.p, sizeof(int64_t) * (size_t)k,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipMemcpyAsync(ic.data(), w.ic.p, sizeof(int64_t) * (size_t)k,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    cut = (int64_t)c.v[C_CUT];
+    return c.err;
+}
+
+}  // namespace
+}  // namespace dmolp
+
+using namespace dmolp;
+
+dmolp_graph::~dmolp_graph() = default;
+
+hipEvent_t dmolp_ctx::prof_event() {
+    if (ev_used == ev_pool.size()) {
+        hipEvent_t e;
+        DMOLP_HIP_CHECK(hipEventCreate(&e));
+        ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+}
+
+extern "C" {
+
+int dmolp_abi_version(void) { return DMOLP_ABI_VERSION; }
+
+const char *dmolp_status_string(int status) {
+    switch (status) {
+        case DMOLP_OK: return "ok";
+        case DMOLP_E_ARG: return "invalid argument";
+        case DMOLP_E_SEEDS: return "k larger than the vertex count";
+        case DMOLP_E_INVALID_LABEL: return "a -1 label reached the cut";
+        case DMOLP_E_ODD_CUT: return "odd edge cut (asymmetric CSR)";
+        case DMOLP_E_HIP: return "HIP runtime failure";
+        case DMOLP_E_NOMEM: return "out of memory";
+        case DMOLP_E_UNSUPPORTED: return "unsupported request";
+        case DMOLP_E_STATE: return "call out of order";
+        case DMOLP_E_GRAPH: return "CSR failed validation";
+        default: return "unknown status";
+    }
+}
+
+const char *dmolp_last_error(const dmolp_ctx *ctx) {
+    if (ctx) return ctx->last_error.c_str();
+    return g_tls_error.c_str();
+}
+
+int dmolp_ctx_create(int device, dmolp_ctx **out) {
+    if (!out) return fail(nullptr, DMOLP_E_ARG, "out is null");
+    *out = nullptr;
+    dmolp_ctx *ctx = new (std::nothrow) dmolp_ctx();
+    if (!ctx) return fail(nullptr, DMOLP_E_NOMEM, "host allocation failed");
+    int rc = guarded(ctx, [&] {
+        int count = 0;
+        DMOLP_HIP_CHECK(hipGetDeviceCount(&count));
+        DMOLP_REQUIRE(device >= 0 && device < count, DMOLP_E_ARG, "device index out of range");
+        DMOLP_HIP_CHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        DMOLP_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        ctx->device = device;
+        ctx->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        DMOLP_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        DMOLP_HIP_CHECK(hipEventCreate(&ctx->ev0));
+        DMOLP_HIP_CHECK(hipEventCreate(&ctx->ev1));
+    });
+    if (rc != DMOLP_OK) {
+        g_tls_error = ctx->last_error;
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return DMOLP_OK;
+}
+
+int dmolp_ctx_destroy(dmolp_ctx *ctx) {
+    if (!ctx) return DMOLP_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->scratch.release();
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DMOLP_OK;
+}
+
+void *dmolp_ctx_stream(dmolp_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int dmolp_ctx_set_profiling(dmolp_ctx *ctx, int on) {
+    if (!ctx) return fail(nullptr, DMOLP_E_ARG, "null context");
+    ctx->profiling = on != 0;
+    return DMOLP_OK;
+}
+
+int dmolp_ctx_round_times(dmolp_ctx *ctx, int group, double *ms_out, int cap, int *n_out) {
+    if (!ctx || group < 0 || group > 5 || cap < 0 || (cap > 0 && !ms_out))
+        return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    const std::vector<double> empty;
+    const std::vector<double> &v =
+        (size_t)group < ctx->prof_ms.size() ? ctx->prof_ms[(size_t)group] : empty;
+    const int n = (int)v.size();
+    for (int i = 0; i < n && i < cap; ++i) ms_out[i] = v[(size_t)i];
+    if (n_out) *n_out = n;
+    return DMOLP_OK;
+}
+
+int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out) {
+    if (!ctx || !out) return fail(nullptr, DMOLP_E_ARG, "null argument");
+    out->heavy_degree = ctx->tune.heavy_degree;
+    out->chunk_edges = ctx->tune.chunk_edges;
+    out->block_threads = ctx->tune.block_threads;
+    out->grid_blocks = ctx->grid_blocks();
+    return DMOLP_OK;
+}
+
+int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
+    if (!ctx || !t) return fail(ctx, DMOLP_E_ARG, "null argument");
+    if (t->heavy_degree < 0 || t->heavy_degree > (1 << 20) || t->chunk_edges < 0 ||
+        t->grid_blocks < 0 || (t->block_threads != 0 && t->block_threads != 256))
+        return fail(ctx, DMOLP_E_ARG, "tuning value out of range");
+    if (t->heavy_degree) ctx->tune.heavy_degree = t->heavy_degree;
+    if (t->chunk_edges) ctx->tune.chunk_edges = t->chunk_edges < 64 ? 64 : t->chunk_edges;
+    if (t->grid_blocks) ctx->tune.grid_blocks = t->grid_blocks;
+    return DMOLP_OK;
+}
+
+int dmolp_gen_host(const dmolp_gen_spec *spec, dmolp_csr *out) {
+    return guarded(nullptr, [&] { gen_host(spec, out); });
+}
+
+void dmolp_csr_free(dmolp_csr *csr) {
+    if (!csr) return;
+    std::free(csr->off);
+    std::free(csr->col);
+    csr->off = nullptr;
+    csr->col = nullptr;
+    csr->n = csr->nnz = 0;
+}
+
+int dmolp_graph_upload(dmolp_ctx *ctx, int64_t n, int64_t nnz, const void *offsets,
+                       int offset_bytes, const int32_t *cols, dmolp_graph **out) {
+    if (!ctx || !out) return fail(ctx, DMOLP_E_ARG, "null argument");
+    *out = nullptr;
+    if (n <= 0 || n >= (1ll << 31) || nnz < 0 || !offsets || (nnz > 0 && !cols) ||
+        (offset_bytes != 4 && offset_bytes != 8))
+        return fail(ctx, DMOLP_E_ARG, "bad CSR arguments");
+    if (offset_bytes == 4 && nnz >= (1ll << 31))
+        return fail(ctx, DMOLP_E_ARG, "int32 offsets cannot address nnz >= 2^31");
+    dmolp_graph *g = new (std::nothrow) dmolp_graph();
+    if (!g) return fail(ctx, DMOLP_E_NOMEM, "host allocation failed");
+    int rc = guarded(ctx, [&] {
+        DMOLP_HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        g->n = n;
+        g->nnz = nnz;
+        g->off.alloc(sizeof(int64_t) * (size_t)(n + 1));
+        g->col.alloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+        if (offset_bytes == 8) {
+            DMOLP_HIP_CHECK(hipMemcpyAsync(g->off.p, offsets, sizeof(int64_t) * (size_t)(n + 1),
+                                           hipMemcpyHostToDevice, s));
+        } else {
+            DevMem tmp(sizeof(int32_t) * (size_t)(n + 1));
+            DMOLP_HIP_CHECK(hipMemcpyAsync(tmp.p, offsets, tmp.bytes, hipMemcpyHostToDevice, s));
+            graph_widen_offsets(ctx, tmp.as<int32_t>(), g->off.as<int64_t>(), n + 1);
+            DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        if (nnz > 0)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(g->col.p, cols, sizeof(int32_t) * (size_t)nnz,
+                                           hipMemcpyHostToDevice, s));
+        int64_t first = 0, last = 0;
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&first, g->off.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&last, g->off.as<int64_t>() + n, sizeof(int64_t),
+                                       hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+        DMOLP_REQUIRE(first == 0 && last == nnz, DMOLP_E_GRAPH, "offsets must run from 0 to nnz");
+        graph_validate(ctx, g);
+    });
+    if (rc != DMOLP_OK) { delete g; return rc; }
+    *out = g;
+    return DMOLP_OK;
+}
+
+int dmolp_graph_generate(dmolp_ctx *ctx, const dmolp_gen_spec *spec, dmolp_graph **out) {
+    if (!ctx || !out || !spec) return fail(ctx, DMOLP_E_ARG, "null argument");
+    *out = nullptr;
+    dmolp_graph *g = new (std::nothrow) dmolp_graph();
+    if (!g) return fail(ctx, DMOLP_E_NOMEM, "host allocation failed");
+    int rc = guarded(ctx, [&] {
+        DMOLP_HIP_CHECK(hipSetDevice(ctx->device));
+        gen_device(ctx, spec, g);
+        graph_validate(ctx, g);
+    });
+    if (rc != DMOLP_OK) { delete g; return rc; }
+    *out = g;
+    return DMOLP_OK;
+}
+
+int dmolp_graph_info(const dmolp_graph *g, int64_t *n, int64_t *nnz) {
+    if (!g) return fail(nullptr, DMOLP_E_ARG, "null graph");
+    if (n) *n = g->n;
+    if (nnz) *nnz = g->nnz;
+    return DMOLP_OK;
+}
+
+int dmolp_graph_download(dmolp_ctx *ctx, const dmolp_graph *g, int64_t *off, int32_t *col) {
+    if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    return guarded(ctx, [&] {
+        if (off)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(off, g->off.p, sizeof(int64_t) * (size_t)(g->n + 1),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+        if (col && g->nnz > 0)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(col, g->col.p, sizeof(int32_t) * (size_t)g->nnz,
+                                           hipMemcpyDeviceToHost, ctx->stream));
+        DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dmolp_graph_free(dmolp_ctx *ctx, dmolp_graph *g) {
+    if (!g) return DMOLP_OK;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    delete g;
+    return DMOLP_OK;
+}
+
+int dmolp_graph_set_labels(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels) {
+    if (!ctx || !g || !labels) return fail(ctx, DMOLP_E_ARG, "null argument");
+    return guarded(ctx, [&] {
+        set_labels(ctx, g, labels);
+        DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dmolp_graph_get_labels(dmolp_ctx *ctx, const dmolp_graph *g, int32_t *labels) {
+    if (!ctx || !g || !labels) return fail(ctx, DMOLP_E_ARG, "null argument");
+    return guarded(ctx, [&] { get_labels(ctx, g, labels); });
+}
+
+int dmolp_graph_labels_checkpoint(dmolp_ctx *ctx, dmolp_graph *g, int op) {
+    if (!ctx || !g || (op != 0 && op != 1)) return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    return guarded(ctx, [&] {
+        const size_t bytes = sizeof(int32_t) * (size_t)g->n;
+        if (op == 0) {
+            DMOLP_REQUIRE(g->labels_valid, DMOLP_E_STATE, "graph has no resident labels");
+            if (!g->saved.p) g->saved.alloc(bytes);
+            DMOLP_HIP_CHECK(hipMemcpyAsync(g->saved.p, g->labels.p, bytes, hipMemcpyDeviceToDevice,
+                                           ctx->stream));
+        } else {
+            DMOLP_REQUIRE(g->saved.p, DMOLP_E_STATE, "no label checkpoint saved");
+            DMOLP_HIP_CHECK(hipMemcpyAsync(g->labels.p, g->saved.p, bytes, hipMemcpyDeviceToDevice,
+                                           ctx->stream));
+            g->labels_valid = true;
+        }
+    });
+}
+
+int dmolp_phase1(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t theta, int32_t *labels_out,
+                 int32_t *seeds_out, int64_t *vc, int64_t *ec, dmolp_phase1_metrics *m) {
+    if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
+    if (theta <= 0) return fail(ctx, DMOLP_E_ARG, "theta must be positive");
+    if ((int64_t)k > g->n) return fail(ctx, DMOLP_E_SEEDS, "k larger than the vertex count");
+    return guarded(ctx, [&] {
+        std::vector<int32_t> seeds;
+        dmolp_phase1_metrics mm{};
+        phase1_run(ctx, g, k, theta, seeds, &mm);
+        // Metrics (phase1.cpp:268-339): counts from the stats sweep.
+        std::vector<int64_t> cv, ce;
+        int64_t cut = 0;
+        const int err = stats_sweep(ctx, g, k, cv, ce, cut);
+        DMOLP_REQUIRE(err == 0, err, "phase-1 labels failed the stats sweep");
+        mm.initial_edge_cut = cut;
+        const double expected_vertices = static_cast<double>(g->n) / k;
+        double max_vr = 0;
+        for (int p = 0; p < k; ++p) max_vr = std::max(max_vr, (double)cv[(size_t)p] / expected_vertices);
+        double total_pe = 0;
+        for (int p = 0; p < k; ++p) total_pe += (double)ce[(size_t)p];
+        const double expected_edges = total_pe / k;
+        double max_er = 0;
+        for (int p = 0; p < k; ++p) max_er = std::max(max_er, (double)ce[(size_t)p] / expected_edges);
+        mm.initial_vertex_balance = max_vr;
+        mm.initial_edge_balance = max_er;
+        mm.total_vertices = g->n;
+        mm.total_edges = g->nnz / 2;
+        if (m) *m = mm;
+        if (vc) std::copy(cv.begin(), cv.end(), vc);
+        if (ec) std::copy(ce.begin(), ce.end(), ec);
+        if (seeds_out) std::copy(seeds.begin(), seeds.end(), seeds_out);
+        if (labels_out) get_labels(ctx, g, labels_out);
+    });
+}
+
+int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, double epsilon,
+                 int64_t *vc, int64_t *ic, dmolp_phase2_metrics *m, dmolp_trace *trace) {
+    if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
+    if (max_rounds < 0) return fail(ctx, DMOLP_E_ARG, "max_rounds must be >= 0");
+    if (!g->labels_valid) return fail(ctx, DMOLP_E_STATE, "graph has no resident labels");
+    return guarded(ctx, [&] {
+        Phase2Work &w = phase2_work(ctx, g, k);
+        if (w.trace_cap < max_rounds) {
+            w.trace.alloc(sizeof(int64_t) * (size_t)(max_rounds > 0 ? max_rounds : 1) * T_NCOL);
+            w.trace_cap = max_rounds;
+        }
+        hipStream_t s = ctx->stream;
+        ctx->ev_used = 0;
+        ctx->prof_marks.clear();
+        ctx->prof_ms.clear();
+        DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, s));
+        phase2_init_enqueue(ctx, g, w);
+        Ctl c{};
+        const int batch = 8;
+        for (int t = 1; t <= max_rounds; ++t) {
+            phase2_round_enqueue(ctx, g, w, t, max_rounds, epsilon);
+            if (t % batch == 0 && t < max_rounds) {
+                DMOLP_HIP_CHECK(hipMemcpyAsync(&c, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+                DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+                if (c.done) break;
+            }
+        }
+        DMOLP_HIP_CHECK(hipEventRecord(ctx->ev1, s));
+        std::vector<int64_t> cv((size_t)k), ci((size_t)k);
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&c, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(cv.data(), w.vc.p, sizeof(int64_t) * (size_t)k,
+                                       hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(ci.data(), w.ic.p, sizeof(int64_t) * (size_t)k,
+                                       hipMemcpyDeviceToHost, s));
+        std::vector<int64_t> tr((size_t)(max_rounds > 0 ? max_rounds : 1) * T_NCOL, 0);
+        if (trace && max_rounds > 0)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(tr.data(), w.trace.p,
+                                           sizeof(int64_t) * (size_t)max_rounds * T_NCOL,
+                                           hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+        DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+        DMOLP_REQUIRE(c.err == 0, c.err, "phase 2 rejected the labels (see status)");
+        float ms = 0.f;
+        DMOLP_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (ctx->profiling) {
+            ctx->prof_ms.assign(ctx->prof_marks.size(), {});
+            for (size_t gi = 0; gi < ctx->prof_marks.size(); ++gi) {
+                for (int t = 0; t < c.rounds && t < (int)ctx->prof_marks[gi].size(); ++t) {
+                    const auto &sp = ctx->prof_marks[gi][(size_t)t];
+                    float e = 0.f;
+                    DMOLP_HIP_CHECK(hipEventElapsedTime(&e, ctx->ev_pool[sp.first],
+                                                        ctx->ev_pool[sp.second]));
+                    ctx->prof_ms[gi].push_back(e);
+                }
+            }
+        }
+        // Final statistics (phase2.cpp:424-441).
+        int64_t tv = 0, te = 0;
+        for (int i = 0; i < k; ++i) { tv += cv[(size_t)i]; te += ci[(size_t)i]; }
+        const double exp_v = static_cast<double>(tv) / k;
+        const double exp_e = (te > 0) ? static_cast<double>(te) / k : 1.0;
+        double max_v = 0.0, max_e = 0.0;
+        for (int i = 0; i < k; ++i) {
+            const double rv = (exp_v > 0) ? static_cast<double>(cv[(size_t)i]) / exp_v : 1.0;
+            const double r_e = (exp_e > 0) ? static_cast<double>(ci[(size_t)i]) / exp_e : 1.0;
+            max_v = std::max(max_v, rv);
+            max_e = std::max(max_e, r_e);
+        }
+        if (m) {
+            std::memset(m, 0, sizeof(*m));
+            m->edge_cut = (int64_t)c.v[C_PREV_CUT];
+            m->vertex_balance = max_v;
+            m->edge_balance = max_e;
+            m->total_vertices = tv;
+            m->total_edges = te;
+            m->rounds = c.rounds;
+            m->exit_reason = max_rounds == 0 ? DMOLP_EXIT_MAX_ROUNDS : c.reason;
+            m->lp_edges = (int64_t)c.v[C_LP_EDGES];
+            m->initial_cut = (int64_t)c.v[C_INIT_CUT];
+            m->device_ms = ms;
+        }
+        if (vc) std::copy(cv.begin(), cv.end(), vc);
+        if (ic) std::copy(ci.begin(), ci.end(), ic);
+        if (trace) {
+            const int R = c.rounds;
+            int64_t *cols[] = {trace->prev_count, trace->prev_deg, trace->cand_count,
+                               trace->bd_count,   trace->b_count,  trace->b_deg,
+                               trace->chg_count,  trace->chg_deg,  trace->cut,
+                               trace->b_light_count, trace->b_light_deg};
+            const int map[] = {T_PREV_N, T_PREV_DEG, T_CAND, T_BD, T_B, T_BDEG, T_CHG, T_CHGDEG, T_CUT,
+                               T_BL, T_BLDEG};
+            for (int j = 0; j < 11; ++j) {
+                if (!cols[j]) continue;
+                for (int t = 0; t < R; ++t) cols[j][t] = tr[(size_t)t * T_NCOL + map[j]];
+            }
+        }
+    });
+}
+
+int dmolp_stats(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, const int32_t *labels, int64_t *vc,
+                int64_t *ic, int64_t *cut) {
+    if (!ctx || !g || !labels) return fail(ctx, DMOLP_E_ARG, "null argument");
+    if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
+    return guarded(ctx, [&] {
+        set_labels(ctx, g, labels);
+        std::vector<int64_t> cv, ci;
+        int64_t c = 0;
+        const int err = stats_sweep(ctx, g, k, cv, ci, c);
+        DMOLP_REQUIRE(err == 0, err, "labels failed the stats sweep");
+        if (vc) std::copy(cv.begin(), cv.end(), vc);
+        if (ic) std::copy(ci.begin(), ci.end(), ic);
+        if (cut) *cut = c;
+    });
+}
+
+int dmolp_penalties(dmolp_ctx *ctx, int32_t k, const int64_t *vc, const int64_t *ic, int32_t t,
+                    double *pen, double *re) {
+    if (!ctx || !vc || !ic || k <= 0 || k > DMOLP_MAX_K || t < 1)
+        return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    return guarded(ctx, [&] {
+        hipStream_t s = ctx->stream;
+        DevMem dv(sizeof(int64_t) * (size_t)k), di(sizeof(int64_t) * (size_t)k);
+        DevMem dp(sizeof(double) * (size_t)k), dr(sizeof(double) * (size_t)k);
+        DMOLP_HIP_CHECK(hipMemcpyAsync(dv.p, vc, dv.bytes, hipMemcpyHostToDevice, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(di.p, ic, di.bytes, hipMemcpyHostToDevice, s));
+        phase2_penalties_device(ctx, k, dv.as<unsigned long long>(), di.as<unsigned long long>(), t,
+                                dp.as<double>(), dr.as<double>());
+        if (pen) DMOLP_HIP_CHECK(hipMemcpyAsync(pen, dp.p, dp.bytes, hipMemcpyDeviceToHost, s));
+        if (re) DMOLP_HIP_CHECK(hipMemcpyAsync(re, dr.p, dr.bytes, hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int dmolp_select_boundary(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels,
+                          const int32_t *prev, int64_t n_prev, const double *re, int32_t k,
+                          int32_t t, int32_t *out, int64_t *n_out) {
+    if (!ctx || !g || !labels || !re || !out || !n_out || n_prev < 0 || (n_prev > 0 && !prev))
+        return fail(ctx, DMOLP_E_ARG, "null argument");
+    if (k <= 0 || k > DMOLP_MAX_K || t < 1) return fail(ctx, DMOLP_E_ARG, "bad k or t");
+    return guarded(ctx, [&] {
+        set_labels(ctx, g, labels);
+        Phase2Work &w = phase2_work(ctx, g, k);
+        phase2_init_enqueue(ctx, g, w);  // differing-neighbour counts
+        Ctl c{};
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&c, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+        DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        DMOLP_REQUIRE(c.err == 0, c.err, "labels failed the stats sweep");
+        upload_list(ctx, g, prev, n_prev, w.B[0]);
+        DMOLP_HIP_CHECK(hipMemcpyAsync(w.re.p, re, sizeof(double) * (size_t)k, hipMemcpyHostToDevice,
+                                       ctx->stream));
+        phase2_select_enqueue(ctx, g, w, t);
+        const std::vector<int32_t> b = download_list(ctx, w.B[1]);
+        std::copy(b.begin(), b.end(), out);
+        *n_out = (int64_t)b.size();
+    });
+}
+
+int dmolp_lp_round(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels, const int32_t *b,
+                   int64_t nb, const double *pen, int32_t k, int32_t *labels_out) {
+    if (!ctx || !g || !labels || !pen || !labels_out || nb < 0 || (nb > 0 && !b))
+        return fail(ctx, DMOLP_E_ARG, "null argument");
+    if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
+    return guarded(ctx, [&] {
+        set_labels(ctx, g, labels);
+        Phase2Work &w = phase2_work(ctx, g, k);
+        std::vector<int32_t> ids(b, b + nb);
+        std::sort(ids.begin(), ids.end());
+        DMOLP_REQUIRE(std::adjacent_find(ids.begin(), ids.end()) == ids.end(), DMOLP_E_ARG,
+                      "boundary list has duplicates");
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(Ctl), ctx->stream));
+        upload_list(ctx, g, ids.data(), nb, w.B[1]);
+        DMOLP_HIP_CHECK(hipMemcpyAsync(w.pen.p, pen, sizeof(double) * (size_t)k,
+                                       hipMemcpyHostToDevice, ctx->stream));
+        phase2_lp_enqueue(ctx, g, w);
+        get_labels(ctx, g, labels_out);
+    });
+}
+
+}  // extern "C"

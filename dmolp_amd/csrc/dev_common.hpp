@@ -1,0 +1,80 @@
+// dev_common.hpp -- device helpers shared by the gfx950 kernels.
+//
+// Wave size is 64 (CDNA4).  Work lists are appended with one atomic per
+// wave (ballot + mbcnt), never one per lane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace dmolp {
+
+constexpr int kWave = 64;
+constexpr int kLpCap = 32;          // gpu_lp_boundary.cu:360
+constexpr int kSelBins = 4096;      // level-1 selection bins: min(deg, 4095)
+constexpr int kDigitBits = 12;      // radix-select digit width
+constexpr int kDigitBins = 1 << kDigitBits;
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Number of active lanes below this one in `mask`.
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_bcast(T v, int src) {
+    return __shfl(v, src, kWave);
+}
+
+// Inclusive scan over the 64 lanes (all lanes must participate).
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t x) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int64_t y = __shfl_up(x, o, kWave);
+        if (lane_id() >= o) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int64_t wave_sum(int64_t x) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+    return x;
+}
+
+// Append the lanes with `pred` to list[*count ...]: one atomic per wave.
+// Returns this lane's slot (or -1).  All lanes must call it.
+__device__ __forceinline__ int64_t wave_append(bool pred, unsigned long long *count) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return -1;
+    int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane_id() == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader, kWave);
+    return pred ? (int64_t)(base + (unsigned long long)lanes_below(mask)) : -1;
+}
+
+// Add a wave-reduced value to a global counter with one atomic.
+__device__ __forceinline__ void wave_add(unsigned long long *dst, int64_t v) {
+    int64_t s = wave_sum(v);
+    if (lane_id() == 0 && s != 0) atomicAdd(dst, (unsigned long long)s);
+}
+
+__device__ __forceinline__ int bitrev5(int x) { return (int)(__brev((unsigned)x) >> 27); }
+
+// Global wave index and wave count of a 1-D launch.
+__device__ __forceinline__ int64_t global_wave() {
+    return ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+}
+__device__ __forceinline__ int64_t wave_count() {
+    return ((int64_t)gridDim.x * blockDim.x) / kWave;
+}
+
+}  // namespace dmolp

@@ -1,0 +1,114 @@
+// graph.hip -- device-resident CSR: upload, validation, degree binning,
+// download, labels.
+#include "rows.hpp"
+
+namespace dmolp {
+
+int64_t bits_for(int64_t v) {
+    int64_t b = 0;
+    while (b < 63 && (v >> b) != 0) ++b;
+    return b;
+}
+
+namespace {
+
+__global__ void k_widen_offsets(const int32_t *src, int64_t *dst, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (int64_t)src[i];
+}
+
+// Flags: [0] bad offsets, [1] column out of range, [2] unsorted or
+// duplicate, [3] self-loop; [4] max degree.
+__global__ void k_validate(GraphDev g, unsigned long long *flags) {
+    unsigned long long bad_off = 0, bad_col = 0, bad_ord = 0, loops = 0, maxd = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = g.off[v], t = g.off[v + 1];
+        if (s > t || s < 0 || t > g.nnz) { ++bad_off; continue; }
+        if ((unsigned long long)(t - s) > maxd) maxd = (unsigned long long)(t - s);
+        int32_t prev = -1;
+        for (int64_t e = s; e < t; ++e) {
+            const int32_t u = g.col[e];
+            if (u < 0 || u >= g.n) ++bad_col;
+            if (u <= prev) ++bad_ord;
+            if (u == v) ++loops;
+            prev = u;
+        }
+    }
+    if (bad_off) atomicAdd(&flags[0], bad_off);
+    if (bad_col) atomicAdd(&flags[1], bad_col);
+    if (bad_ord) atomicAdd(&flags[2], bad_ord);
+    if (loops) atomicAdd(&flags[3], loops);
+    if (maxd) atomicMax(&flags[4], maxd);
+}
+
+__global__ void k_bin_all(GraphDev g, ListDev out, int heavy_degree, int chunk_edges) {
+    for (int64_t b0 = global_wave() * kWave; b0 < g.n; b0 += wave_count() * kWave) {
+        const int64_t v = b0 + lane_id();
+        const bool valid = v < g.n;
+        const int64_t d = valid ? g.off[v + 1] - g.off[v] : 0;
+        list_append(out, valid, (int32_t)v, d, heavy_degree, chunk_edges);
+    }
+}
+
+__global__ void k_count_heavy(GraphDev g, int heavy_degree, int chunk_edges,
+                              unsigned long long *acc) {
+    unsigned long long nh = 0, nc = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = g.off[v + 1] - g.off[v];
+        if (d > heavy_degree) { ++nh; nc += (unsigned long long)((d + chunk_edges - 1) / chunk_edges); }
+    }
+    if (nh) { atomicAdd(&acc[0], nh); atomicAdd(&acc[1], nc); }
+}
+
+}  // namespace
+
+void graph_validate(dmolp_ctx *ctx, dmolp_graph *g) {
+    DevMem flags(sizeof(unsigned long long) * 8);
+    DMOLP_HIP_CHECK(hipMemsetAsync(flags.p, 0, flags.bytes, ctx->stream));
+    const int blocks = ctx->cu_count * 4;
+    if (g->n > 0)
+        hipLaunchKernelGGL(k_validate, dim3(blocks), dim3(256), 0, ctx->stream, g->dev(),
+                           flags.as<unsigned long long>());
+    DMOLP_HIP_CHECK(hipGetLastError());
+    unsigned long long h[8] = {0};
+    DMOLP_HIP_CHECK(hipMemcpyAsync(h, flags.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    DMOLP_REQUIRE(h[0] == 0, DMOLP_E_GRAPH, "CSR offsets are not monotone within [0, nnz]");
+    DMOLP_REQUIRE(h[1] == 0, DMOLP_E_GRAPH, "CSR column outside [0, n)");
+    DMOLP_REQUIRE(h[2] == 0, DMOLP_E_GRAPH, "CSR row not strictly ascending (unsorted or duplicate)");
+    DMOLP_REQUIRE(h[3] == 0, DMOLP_E_GRAPH, "CSR has a self-loop");
+    g->max_degree = (int32_t)h[4];
+}
+
+void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g) {
+    const int H = ctx->tune.heavy_degree, CH = ctx->tune.chunk_edges;
+    if (g->all_heavy_degree == H && g->all_chunk_edges == CH) return;
+    DevMem acc(sizeof(unsigned long long) * 2);
+    DMOLP_HIP_CHECK(hipMemsetAsync(acc.p, 0, acc.bytes, ctx->stream));
+    hipLaunchKernelGGL(k_count_heavy, dim3(ctx->cu_count * 4), dim3(256), 0, ctx->stream, g->dev(),
+                       H, CH, acc.as<unsigned long long>());
+    DMOLP_HIP_CHECK(hipGetLastError());
+    unsigned long long h[2];
+    DMOLP_HIP_CHECK(hipMemcpyAsync(h, acc.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    g->n_heavy = (int64_t)h[0];
+    g->n_heavy_chunks = (int64_t)h[1];
+    g->all.alloc(g->n, g->n_heavy, g->n_heavy_chunks);
+    DMOLP_HIP_CHECK(hipMemsetAsync(g->all.cnt.p, 0, g->all.cnt.bytes, ctx->stream));
+    hipLaunchKernelGGL(k_bin_all, dim3(ctx->grid_blocks()), dim3(256), 0, ctx->stream, g->dev(),
+                       g->all.dev, H, CH);
+    DMOLP_HIP_CHECK(hipGetLastError());
+    g->all_heavy_degree = H;
+    g->all_chunk_edges = CH;
+}
+
+void graph_widen_offsets(dmolp_ctx *ctx, const int32_t *d_src, int64_t *d_dst, int64_t count) {
+    hipLaunchKernelGGL(k_widen_offsets, dim3(ctx->cu_count * 4), dim3(256), 0, ctx->stream, d_src,
+                       d_dst, count);
+    DMOLP_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace dmolp

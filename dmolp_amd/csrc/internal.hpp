@@ -1,0 +1,158 @@
+// internal.hpp -- context, graph handle and device-view structures behind
+// the C ABI in include/dmolp_hip.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dmolp_hip.h"
+
+namespace dmolp {
+
+// Error carried from deep inside the library to the ABI boundary, where it
+// becomes a status code and the context's last-error message.
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string &msg) : std::runtime_error(msg), status(s) {}
+};
+
+#define DMOLP_HIP_CHECK(expr)                                                              \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            throw ::dmolp::Error(e_ == hipErrorOutOfMemory ? DMOLP_E_NOMEM : DMOLP_E_HIP,  \
+                                 std::string(#expr) + ": " + hipGetErrorString(e_) +      \
+                                     " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+        }                                                                                  \
+    } while (0)
+
+#define DMOLP_REQUIRE(cond, status, msg)                              \
+    do {                                                              \
+        if (!(cond)) throw ::dmolp::Error((status), std::string(msg)); \
+    } while (0)
+
+// Owning device allocation.
+struct DevMem {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    explicit DevMem(size_t b) { alloc(b); }
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    DevMem(DevMem &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevMem &operator=(DevMem &&o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevMem() { release(); }
+    void alloc(size_t b) {
+        release();
+        if (b == 0) b = 256;
+        DMOLP_HIP_CHECK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    void release() {
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Device view of a vertex work list.  Rows of at most `heavy_degree` edges
+// are "light" and processed 64 list entries per wave; longer rows are
+// "heavy": they get a slot and are split into chunks of `chunk_edges`
+// edges, one chunk per wave.
+enum ListCount { LC_LIGHT = 0, LC_SLOTS = 1, LC_CHUNKS = 2, LC_DEG = 3, LC_HDEG = 4, LC_N = 5 };
+struct ListDev {
+    int32_t *light;            // light vertex ids
+    int32_t *hv;               // heavy slot -> vertex
+    int32_t *hn;               // heavy slot -> chunk count
+    int64_t *hc;               // chunk items: (slot << 32) | chunk index
+    unsigned long long *cnt;   // ListCount counters (device)
+};
+
+struct List {
+    DevMem light, hv, hn, hc, cnt;
+    ListDev dev{};
+    void alloc(int64_t cap_light, int64_t cap_slots, int64_t cap_chunks) {
+        light.alloc(sizeof(int32_t) * (size_t)(cap_light > 0 ? cap_light : 1));
+        hv.alloc(sizeof(int32_t) * (size_t)(cap_slots > 0 ? cap_slots : 1));
+        hn.alloc(sizeof(int32_t) * (size_t)(cap_slots > 0 ? cap_slots : 1));
+        hc.alloc(sizeof(int64_t) * (size_t)(cap_chunks > 0 ? cap_chunks : 1));
+        cnt.alloc(sizeof(unsigned long long) * 8);
+        dev = ListDev{light.as<int32_t>(), hv.as<int32_t>(), hn.as<int32_t>(), hc.as<int64_t>(),
+                      cnt.as<unsigned long long>()};
+    }
+};
+
+struct Tuning {
+    int heavy_degree = 1024;
+    int chunk_edges = 4096;
+    int block_threads = 256;
+    int grid_blocks = 0;  // 0: derived from the CU count
+};
+
+struct GraphDev {
+    int64_t n;
+    int64_t nnz;
+    const int64_t *off;
+    const int32_t *col;
+};
+
+// Per-graph phase-2 state, allocated on first use for a given k.
+struct Phase2Work;
+struct Phase1Work;
+
+}  // namespace dmolp
+
+struct dmolp_ctx {
+    int device = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    dmolp::Tuning tune;
+    dmolp::DevMem scratch;  // small per-call scratch (host<->device staging)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Kernel-group timing of the last phase-2 run (profiling on): events
+    // recorded around each group every round.
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::vector<std::pair<size_t, size_t>>> prof_marks;  // [group] -> (ev_a, ev_b) per round
+    std::vector<std::vector<double>> prof_ms;                         // [group] -> ms per round
+    hipEvent_t prof_event();
+    int grid_blocks() const { return tune.grid_blocks > 0 ? tune.grid_blocks : cu_count * 8; }
+};
+
+struct dmolp_graph {
+    int64_t n = 0;
+    int64_t nnz = 0;
+    int32_t max_degree = 0;
+    dmolp::DevMem off;     // int64[n + 1]
+    dmolp::DevMem col;     // int32[nnz]
+    dmolp::DevMem labels;  // int32[n], resident labels
+    dmolp::DevMem saved;   // int32[n], label checkpoint
+    bool labels_valid = false;
+    // All vertices binned by degree (for full sweeps), built for one
+    // (heavy_degree, chunk_edges) pair.
+    dmolp::List all;
+    int all_heavy_degree = -1, all_chunk_edges = -1;
+    int64_t n_heavy = 0, n_heavy_chunks = 0;
+    std::unique_ptr<dmolp::Phase2Work> p2;
+    std::unique_ptr<dmolp::Phase1Work> p1;
+    dmolp::GraphDev dev() const {
+        return dmolp::GraphDev{n, nnz, off.as<int64_t>(), col.as<int32_t>()};
+    }
+    ~dmolp_graph();
+};
+
+namespace dmolp {
+// Implemented in the .hip files.
+void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g);  // builds g->all for ctx->tune
+void graph_validate(dmolp_ctx *ctx, dmolp_graph *g);       // offsets, sorted rows, no self-loops
+int64_t bits_for(int64_t v);                                // bits needed to hold 0..v
+}  // namespace dmolp

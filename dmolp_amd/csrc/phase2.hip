@@ -1,0 +1,752 @@
+// phase2.hip -- boundary label propagation (phase2.cpp:256-449 with
+// expandBoundaryNodes gpu_lp_boundary.cu:185-305 and the decision of
+// boundaryLPKernel_unified gpu_lp_boundary.cu:327-427), device-resident.
+//
+// Per round t the host enqueues, without waiting:
+//   round_begin   penalties (phase2.cpp:158-209) from the device counts
+//   candidates    B_{t-1} u N(B_{t-1}), first touch by stamp; a candidate is
+//                 a boundary vertex iff its differing-neighbour count > 0
+//   select        per label top ceil(ratio*c) by (degree, id)  -> B_t
+//   lp            decision for every b in B_t (Jacobi)        -> Delta
+//   delta         incremental vc, ic, cut and differing-neighbour counts
+//   apply         commit Delta to the labels
+//   round_end     exit tests (phase2.cpp:309-312, 384-415), trace
+// Every kernel returns at once when the control block says "done", so a
+// finished run costs empty launches only.
+#include <cmath>
+#include <cstring>
+
+#include "phase2.hpp"
+#include "rows.hpp"
+
+namespace dmolp {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / kWave;
+constexpr int kHistStride = kLpCap + 1;  // odd stride: no LDS bank conflicts across slots
+constexpr int kMaxKLds = DMOLP_MAX_K;
+
+__device__ __forceinline__ bool is_done(const Ctl *c) { return c->done != 0; }
+
+// ---------------------------------------------------------------- init sweep
+
+// phase2.cpp:55-117 and 214-253 over all vertices, plus the per-vertex count
+// of differing neighbours used as the boundary test.
+__global__ __launch_bounds__(kThreads) void k_init_light(GraphDev g, ListDev all, const int32_t *L,
+                                                         int k, int32_t *diff, Ctl *ctl,
+                                                         unsigned long long *vc,
+                                                         unsigned long long *ic) {
+    __shared__ BatchLds s_b[kWavesPerBlock];
+    __shared__ int32_t s_cnt[kWavesPerBlock][kWave];
+    __shared__ unsigned long long s_vc[kMaxKLds], s_ic[kMaxKLds];
+    for (int i = threadIdx.x; i < k; i += blockDim.x) { s_vc[i] = 0; s_ic[i] = 0; }
+    __syncthreads();
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    int64_t invalid = 0, cut2 = 0;
+    const int64_t count = (int64_t)all.cnt[LC_LIGHT];
+    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+        int32_t v, deg;
+        bool valid;
+        const int32_t total = batch_load(g, all.light, count, b0, s_b[w], v, deg, valid);
+        s_cnt[w][lane] = 0;
+        wave_sync();
+        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
+            const int32_t q = q0 + lane;
+            if (q < total) {
+                const int slot = batch_slot(s_b[w], q);
+                const int64_t e = s_b[w].base[slot] + q;
+                const int32_t vv = s_b[w].v[slot];
+                const int32_t lv = L[vv], lu = L[g.col[e]];
+                if (lv == -1 || lu == -1) {
+                    ++invalid;
+                } else if (lv != lu) {
+                    ++cut2;
+                    atomicAdd(&s_cnt[w][slot], 1);
+                } else if (lv >= 0 && lv < k) {
+                    atomicAdd(&s_ic[lv], 1ull);
+                }
+            }
+        }
+        wave_sync();
+        if (valid) {
+            diff[v] = s_cnt[w][lane];
+            const int32_t lv = L[v];
+            if (lv >= 0 && lv < k) atomicAdd(&s_vc[lv], 1ull);
+            else if (lv != -1) atomicAdd(&ctl->v[C_KERNEL_ERR], 1ull);
+        }
+        wave_sync();
+    }
+    wave_add(&ctl->v[C_INVALID], invalid);
+    wave_add(&ctl->v[C_CUT2], cut2);
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        if (s_vc[i]) atomicAdd(&vc[i], s_vc[i]);
+        if (s_ic[i]) atomicAdd(&ic[i], s_ic[i]);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_init_heavy(GraphDev g, ListDev all, const int32_t *L,
+                                                         int k, int chunk_edges, int32_t *diff,
+                                                         Ctl *ctl, unsigned long long *vc,
+                                                         unsigned long long *ic) {
+    const int lane = lane_id();
+    int64_t invalid = 0, cut2 = 0;
+    const int64_t nch = (int64_t)all.cnt[LC_CHUNKS];
+    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+        int32_t slot, v;
+        int64_t e0, e1, deg;
+        const int64_t item = all.hc[j];
+        chunk_bounds(g, all, item, chunk_edges, slot, v, e0, e1, deg);
+        const int32_t lv = L[v];
+        int64_t dcount = 0, intra = 0;
+        for (int64_t e = e0 + lane; e < e1; e += kWave) {
+            const int32_t lu = L[g.col[e]];
+            if (lv == -1 || lu == -1) ++invalid;
+            else if (lv != lu) { ++cut2; ++dcount; }
+            else if (lv >= 0 && lv < k) ++intra;
+        }
+        dcount = wave_sum(dcount);
+        intra = wave_sum(intra);
+        if (lane == 0) {
+            if (dcount) atomicAdd(&diff[v], (int32_t)dcount);
+            if (intra) atomicAdd(&ic[lv], (unsigned long long)intra);
+            if ((item & 0xffffffffll) == 0) {
+                if (lv >= 0 && lv < k) atomicAdd(&vc[lv], 1ull);
+                else if (lv != -1) atomicAdd(&ctl->v[C_KERNEL_ERR], 1ull);
+            }
+        }
+    }
+    wave_add(&ctl->v[C_INVALID], invalid);
+    wave_add(&ctl->v[C_CUT2], cut2);
+}
+
+__global__ void k_init_finish(Ctl *ctl, unsigned long long *ic, int k) {
+    for (int i = threadIdx.x; i < k; i += blockDim.x) ic[i] = ic[i] / 2;  // phase2.cpp:110
+    if (threadIdx.x == 0) {
+        const unsigned long long cut2 = ctl->v[C_CUT2];
+        if (ctl->v[C_KERNEL_ERR] != 0) { ctl->err = DMOLP_E_ARG; ctl->done = 1; }
+        else if (ctl->v[C_INVALID] != 0) { ctl->err = DMOLP_E_INVALID_LABEL; ctl->done = 1; }
+        else if (cut2 & 1ull) { ctl->err = DMOLP_E_ODD_CUT; ctl->done = 1; }
+        ctl->v[C_CUT] = cut2 >> 1;
+        ctl->v[C_PREV_CUT] = cut2 >> 1;
+        ctl->v[C_INIT_CUT] = cut2 >> 1;
+    }
+}
+
+// ---------------------------------------------------------------- round begin
+
+// phase2.cpp:158-209 in source order; lg = 1 + log(t) from the host's libm.
+__device__ void penalties(const unsigned long long *vc, const unsigned long long *ic, int k,
+                          double lg, double *pen, double *re) {
+    int64_t tv = 0, te = 0;
+    for (int i = 0; i < k; ++i) { tv += (int64_t)vc[i]; te += (int64_t)ic[i]; }
+    const double exp_v = static_cast<double>(tv) / k;
+    const double exp_e = (te > 0) ? static_cast<double>(te) / k : 1.0;
+    for (int i = 0; i < k; ++i) {
+        pen[i] = (exp_v > 0) ? static_cast<double>((int64_t)vc[i]) / exp_v : 1.0;  // RV
+        re[i] = (exp_e > 0) ? static_cast<double>((int64_t)ic[i]) / exp_e : 1.0;
+    }
+    double rv_mean = 0.0, re_mean = 0.0;
+    for (int i = 0; i < k; ++i) { rv_mean += pen[i]; re_mean += re[i]; }
+    rv_mean /= k;
+    re_mean /= k;
+    double rv_var = 0.0, re_var = 0.0;
+    for (int i = 0; i < k; ++i) {
+        rv_var += (pen[i] - rv_mean) * (pen[i] - rv_mean);
+        re_var += (re[i] - re_mean) * (re[i] - re_mean);
+    }
+    rv_var /= k;
+    re_var /= k;
+    const double total_var = rv_var + re_var;
+    const double imb_rv = (total_var > 0) ? rv_var / total_var : 0.0;
+    const double imb_re = (total_var > 0) ? re_var / total_var : 0.0;
+    for (int i = 0; i < k; ++i) {
+        const double g_rv = (1.0 - pen[i]) * lg;
+        const double g_re = (1.0 - re[i]) * lg;
+        pen[i] = imb_rv * g_rv + imb_re * g_re;
+    }
+}
+
+__global__ void k_penalties(const unsigned long long *vc, const unsigned long long *ic, int k,
+                            double lg, double *pen, double *re) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) penalties(vc, ic, k, lg, pen, re);
+}
+
+__global__ void k_round_begin(Ctl *ctl, const unsigned long long *vc, const unsigned long long *ic,
+                              int k, double lg, double *pen, double *re, ListDev bcur, ListDev dl,
+                              RecDev bd) {
+    if (is_done(ctl)) return;
+    if (threadIdx.x < LC_N) { bcur.cnt[threadIdx.x] = 0; dl.cnt[threadIdx.x] = 0; }
+    if (threadIdx.x == 0) {
+        *bd.cnt = 0;
+        ctl->v[C_CAND] = 0;
+        penalties(vc, ic, k, lg, pen, re);
+    }
+}
+
+// ---------------------------------------------------------------- candidates
+
+// Round 1: B_0 is every vertex, so the candidates are every vertex.
+__global__ __launch_bounds__(kThreads) void k_cand_all(GraphDev g, const int32_t *L,
+                                                       const int32_t *diff, RecDev bd, Ctl *ctl) {
+    if (is_done(ctl)) return;
+    for (int64_t b0 = global_wave() * kWave; b0 < g.n; b0 += wave_count() * kWave) {
+        const int64_t v = b0 + lane_id();
+        const bool is_bd = v < g.n && diff[v] > 0;
+        const int64_t pos = wave_append(is_bd, bd.cnt);
+        if (is_bd) {
+            bd.x[pos] = (int32_t)v;
+            bd.g[pos] = L[v];
+            bd.d[pos] = (int32_t)(g.off[v + 1] - g.off[v]);
+        }
+    }
+}
+
+// First touch of u in this pass; a touched boundary vertex becomes a record.
+__device__ __forceinline__ void touch(const GraphDev &g, bool pred, int32_t u, int32_t *stamp,
+                                      int32_t epoch, const int32_t *L, const int32_t *diff,
+                                      const RecDev &bd, int64_t &cand) {
+    bool first = false;
+    if (pred && stamp[u] != epoch) first = atomicExch(&stamp[u], epoch) != epoch;
+    if (first) ++cand;
+    const bool rec = first && diff[u] > 0;
+    const int64_t pos = wave_append(rec, bd.cnt);
+    if (rec) {
+        bd.x[pos] = u;
+        bd.g[pos] = L[u];
+        bd.d[pos] = (int32_t)(g.off[u + 1] - g.off[u]);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_cand_light(GraphDev g, ListDev prev, const int32_t *L,
+                                                         const int32_t *diff, int32_t *stamp,
+                                                         int32_t epoch, RecDev bd, Ctl *ctl) {
+    if (is_done(ctl)) return;
+    __shared__ BatchLds s_b[kWavesPerBlock];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    int64_t cand = 0;
+    const int64_t count = (int64_t)prev.cnt[LC_LIGHT];
+    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+        int32_t v, deg;
+        bool valid;
+        const int32_t total = batch_load(g, prev.light, count, b0, s_b[w], v, deg, valid);
+        touch(g, valid, v, stamp, epoch, L, diff, bd, cand);
+        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
+            const int32_t q = q0 + lane;
+            const bool on = q < total;
+            int32_t u = 0;
+            if (on) u = g.col[s_b[w].base[batch_slot(s_b[w], q)] + q];
+            touch(g, on, u, stamp, epoch, L, diff, bd, cand);
+        }
+        wave_sync();
+    }
+    wave_add(&ctl->v[C_CAND], cand);
+}
+
+__global__ __launch_bounds__(kThreads) void k_cand_heavy(GraphDev g, ListDev prev, const int32_t *L,
+                                                         const int32_t *diff, int32_t *stamp,
+                                                         int32_t epoch, int chunk_edges, RecDev bd,
+                                                         Ctl *ctl) {
+    if (is_done(ctl)) return;
+    const int lane = lane_id();
+    int64_t cand = 0;
+    const int64_t nch = (int64_t)prev.cnt[LC_CHUNKS];
+    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+        int32_t slot, v;
+        int64_t e0, e1, deg;
+        const int64_t item = prev.hc[j];
+        chunk_bounds(g, prev, item, chunk_edges, slot, v, e0, e1, deg);
+        touch(g, lane == 0 && (item & 0xffffffffll) == 0, v, stamp, epoch, L, diff, bd, cand);
+        for (int64_t eb = e0; eb < e1; eb += kWave) {
+            const int64_t e = eb + lane;
+            const bool on = e < e1;
+            touch(g, on, on ? g.col[e] : 0, stamp, epoch, L, diff, bd, cand);
+        }
+    }
+    wave_add(&ctl->v[C_CAND], cand);
+}
+
+// ---------------------------------------------------------------- LP decision
+
+// gpu_lp_boundary.cu:388-421 for one vertex: scores h*(1+pen) over labels
+// below E; among equal best scores the smallest 5-bit bit-reversed label
+// wins (the order the 32-lane shuffle-down tree yields); the winner must
+// beat the raw own-label count.
+__device__ __forceinline__ int32_t lp_decide(const int32_t *h, int stride, int E, const double *pen,
+                                             int32_t own) {
+    const double base = (own >= 0 && own < E) ? (double)h[own * stride] : -1e300;
+    int best_l = 0;
+    double best_s = 0.0;
+    for (int l = 0; l < E; ++l) {
+        double s = (double)h[l * stride];
+        if (s > 0.0) s *= (1.0 + pen[l]);
+        if (l == 0 || s > best_s || (s == best_s && bitrev5(l) < bitrev5(best_l))) {
+            best_l = l;
+            best_s = s;
+        }
+    }
+    return (best_s > base) ? best_l : own;
+}
+
+__global__ __launch_bounds__(kThreads) void k_lp_light(GraphDev g, const int32_t *L, ListDev B,
+                                                       const double *pen, int k, int32_t *chg,
+                                                       ListDev D, int heavy_degree,
+                                                       int chunk_edges, const Ctl *ctl) {
+    if (is_done(ctl)) return;
+    __shared__ BatchLds s_b[kWavesPerBlock];
+    __shared__ int32_t s_h[kWavesPerBlock][kWave * kHistStride];
+    __shared__ double s_pen[kLpCap];
+    const int E = k < kLpCap ? k : kLpCap;
+    if ((int)threadIdx.x < E) s_pen[threadIdx.x] = pen[threadIdx.x];
+    __syncthreads();
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    int32_t *h = s_h[w];
+    const int64_t count = (int64_t)B.cnt[LC_LIGHT];
+    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+        int32_t v, deg;
+        bool valid;
+        for (int l = 0; l < E; ++l) h[lane * kHistStride + l] = 0;
+        const int32_t total = batch_load(g, B.light, count, b0, s_b[w], v, deg, valid);
+        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
+            const int32_t q = q0 + lane;
+            if (q < total) {
+                const int slot = batch_slot(s_b[w], q);
+                const int32_t lab = L[g.col[s_b[w].base[slot] + q]];
+                if (lab >= 0 && lab < E) atomicAdd(&h[slot * kHistStride + lab], 1);
+            }
+        }
+        wave_sync();
+        bool changed = false;
+        int32_t nl = 0;
+        if (valid) {
+            const int32_t own = L[v];
+            nl = lp_decide(&h[lane * kHistStride], 1, E, s_pen, own);
+            changed = nl != own;
+            if (changed) chg[v] = nl;
+        }
+        list_append(D, changed, v, deg, heavy_degree, chunk_edges);
+        wave_sync();
+    }
+}
+
+// Heavy rows: each wave counts one chunk with a peel loop over the distinct
+// labels present in each 64-edge step, adds the counts to the row's global
+// histogram, and the wave finishing the row's last chunk decides.
+__global__ __launch_bounds__(kThreads) void k_lp_heavy(GraphDev g, const int32_t *L, ListDev B,
+                                                       const double *pen, int k, int32_t *chg,
+                                                       ListDev D, int heavy_degree,
+                                                       int chunk_edges, int32_t *hub_hist,
+                                                       int32_t *hub_done, const Ctl *ctl) {
+    if (is_done(ctl)) return;
+    __shared__ int32_t s_fin[kWavesPerBlock][kLpCap];
+    __shared__ double s_pen[kLpCap];
+    const int E = k < kLpCap ? k : kLpCap;
+    if ((int)threadIdx.x < E) s_pen[threadIdx.x] = pen[threadIdx.x];
+    __syncthreads();
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    const int64_t nch = (int64_t)B.cnt[LC_CHUNKS];
+    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+        int32_t slot, v;
+        int64_t e0, e1, deg;
+        chunk_bounds(g, B, B.hc[j], chunk_edges, slot, v, e0, e1, deg);
+        int32_t mycnt = 0;  // lane l counts label l
+        for (int64_t eb = e0; eb < e1; eb += kWave) {
+            const int64_t e = eb + lane;
+            const int32_t lab = e < e1 ? L[g.col[e]] : -1;
+            uint64_t active = __ballot(lab >= 0 && lab < E);
+            while (active) {
+                const int leader = __ffsll((long long)active) - 1;
+                const int32_t l0 = __shfl(lab, leader, kWave);
+                const uint64_t m = __ballot(lab == l0) & active;
+                if (lane == l0) mycnt += __popcll(m);
+                active &= ~m;
+            }
+        }
+        if (lane < E && mycnt) atomicAdd(&hub_hist[slot * kLpCap + lane], mycnt);
+        __threadfence();
+        int prev = 0;
+        if (lane == 0) prev = atomicAdd(&hub_done[slot], 1);
+        prev = __shfl(prev, 0, kWave);
+        const bool last = prev == B.hn[slot] - 1;
+        bool changed = false;
+        int32_t nl = 0;
+        if (last) {
+            __threadfence();
+            if (lane < kLpCap) s_fin[w][lane] = lane < E ? atomicExch(&hub_hist[slot * kLpCap + lane], 0) : 0;
+            if (lane == 0) hub_done[slot] = 0;
+            wave_sync();
+            if (lane == 0) {
+                const int32_t own = L[v];
+                nl = lp_decide(s_fin[w], 1, E, s_pen, own);
+                changed = nl != own;
+                if (changed) chg[v] = nl;
+            }
+        }
+        list_append(D, changed, v, deg, heavy_degree, chunk_edges);
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- delta stats
+
+// For a changed vertex v (old a, new c) and neighbour u (old lu, new lu2):
+// the pair is counted once, from the smaller id when both changed.
+struct PairDelta {
+    int32_t dd;     // change of "labels differ" (-1, 0, +1)
+};
+
+__device__ __forceinline__ int32_t pair_update(int32_t v, int32_t a, int32_t c, int32_t u,
+                                               const int32_t *L, const int32_t *chg, int k,
+                                               int32_t *diff, unsigned long long *s_ic,
+                                               int64_t &cut_delta, bool &counted) {
+    const int32_t cu = chg[u];
+    counted = !(cu >= 0 && u < v);
+    if (!counted) return 0;
+    const int32_t lu = L[u];
+    const int32_t lu2 = cu >= 0 ? cu : lu;
+    const int32_t od = a != lu ? 1 : 0, nd = c != lu2 ? 1 : 0;
+    const int32_t dd = nd - od;
+    if (dd) { atomicAdd(&diff[u], dd); cut_delta += dd; }
+    if (a == lu && a >= 0 && a < k) atomicAdd(&s_ic[a], ~0ull);          // -1
+    if (c == lu2 && c >= 0 && c < k) atomicAdd(&s_ic[c], 1ull);
+    return dd;
+}
+
+__global__ __launch_bounds__(kThreads) void k_delta_light(GraphDev g, ListDev D, const int32_t *L,
+                                                          const int32_t *chg, int k, int32_t *diff,
+                                                          Ctl *ctl, unsigned long long *vc,
+                                                          unsigned long long *ic) {
+    if (is_done(ctl)) return;
+    __shared__ BatchLds s_b[kWavesPerBlock];
+    __shared__ int32_t s_dv[kWavesPerBlock][kWave];
+    __shared__ unsigned long long s_vc[kMaxKLds], s_ic[kMaxKLds];
+    for (int i = threadIdx.x; i < k; i += blockDim.x) { s_vc[i] = 0; s_ic[i] = 0; }
+    __syncthreads();
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    int64_t cut_delta = 0;
+    const int64_t count = (int64_t)D.cnt[LC_LIGHT];
+    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+        int32_t v, deg;
+        bool valid;
+        const int32_t total = batch_load(g, D.light, count, b0, s_b[w], v, deg, valid);
+        s_dv[w][lane] = 0;
+        wave_sync();
+        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
+            const int32_t q = q0 + lane;
+            if (q < total) {
+                const int slot = batch_slot(s_b[w], q);
+                const int32_t vv = s_b[w].v[slot];
+                const int32_t a = L[vv], c = chg[vv];
+                const int32_t u = g.col[s_b[w].base[slot] + q];
+                bool counted;
+                const int32_t dd = pair_update(vv, a, c, u, L, chg, k, diff, s_ic, cut_delta, counted);
+                if (dd) atomicAdd(&s_dv[w][slot], dd);
+            }
+        }
+        wave_sync();
+        if (valid) {
+            if (s_dv[w][lane]) atomicAdd(&diff[v], s_dv[w][lane]);
+            const int32_t a = L[v], c = chg[v];
+            if (a >= 0 && a < k) atomicAdd(&s_vc[a], ~0ull);
+            if (c >= 0 && c < k) atomicAdd(&s_vc[c], 1ull);
+        }
+        wave_sync();
+    }
+    wave_add(&ctl->v[C_CUT], cut_delta);
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        if (s_vc[i]) atomicAdd(&vc[i], s_vc[i]);
+        if (s_ic[i]) atomicAdd(&ic[i], s_ic[i]);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_delta_heavy(GraphDev g, ListDev D, const int32_t *L,
+                                                          const int32_t *chg, int k,
+                                                          int chunk_edges, int32_t *diff, Ctl *ctl,
+                                                          unsigned long long *vc,
+                                                          unsigned long long *ic) {
+    if (is_done(ctl)) return;
+    __shared__ unsigned long long s_ic[kMaxKLds];
+    for (int i = threadIdx.x; i < k; i += blockDim.x) s_ic[i] = 0;
+    __syncthreads();
+    const int lane = lane_id();
+    int64_t cut_delta = 0;
+    const int64_t nch = (int64_t)D.cnt[LC_CHUNKS];
+    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+        int32_t slot, v;
+        int64_t e0, e1, deg;
+        const int64_t item = D.hc[j];
+        chunk_bounds(g, D, item, chunk_edges, slot, v, e0, e1, deg);
+        const int32_t a = L[v], c = chg[v];
+        int64_t dv = 0;
+        for (int64_t e = e0 + lane; e < e1; e += kWave) {
+            bool counted;
+            dv += pair_update(v, a, c, g.col[e], L, chg, k, diff, s_ic, cut_delta, counted);
+        }
+        dv = wave_sum(dv);
+        if (lane == 0) {
+            if (dv) atomicAdd(&diff[v], (int32_t)dv);
+            if ((item & 0xffffffffll) == 0) {
+                if (a >= 0 && a < k) atomicAdd(&vc[a], ~0ull);
+                if (c >= 0 && c < k) atomicAdd(&vc[c], 1ull);
+            }
+        }
+    }
+    wave_add(&ctl->v[C_CUT], cut_delta);
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += blockDim.x)
+        if (s_ic[i]) atomicAdd(&ic[i], s_ic[i]);
+}
+
+__global__ void k_apply(ListDev D, int32_t *L, int32_t *chg, const Ctl *ctl) {
+    if (is_done(ctl)) return;
+    const int64_t nl = (int64_t)D.cnt[LC_LIGHT], ns = (int64_t)D.cnt[LC_SLOTS];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl + ns;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = i < nl ? D.light[i] : D.hv[i - nl];
+        L[v] = chg[v];
+        chg[v] = -1;
+    }
+}
+
+// ---------------------------------------------------------------- round end
+
+__global__ void k_round_end(Ctl *ctl, ListDev prev, ListDev cur, ListDev D, RecDev bd, int t,
+                            int max_rounds, double epsilon, int64_t *trace, int trace_cap,
+                            int64_t n, int64_t nnz) {
+    if (is_done(ctl)) return;
+    const int64_t nprev = t == 1 ? n : (int64_t)(prev.cnt[LC_LIGHT] + prev.cnt[LC_SLOTS]);
+    const int64_t dprev = t == 1 ? nnz : (int64_t)prev.cnt[LC_DEG];
+    const int64_t ncand = t == 1 ? n : (int64_t)ctl->v[C_CAND];
+    const int64_t nb = (int64_t)(cur.cnt[LC_LIGHT] + cur.cnt[LC_SLOTS]);
+    const int64_t bdeg = (int64_t)cur.cnt[LC_DEG];
+    const int64_t nchg = (int64_t)(D.cnt[LC_LIGHT] + D.cnt[LC_SLOTS]);
+    const int64_t cdeg = (int64_t)D.cnt[LC_DEG];
+    ctl->rounds = t;
+    ctl->v[C_LP_EDGES] += (unsigned long long)bdeg;
+    int64_t cut_rec = -1;
+    if (nb == 0) {
+        ctl->done = 1;
+        ctl->reason = DMOLP_EXIT_EMPTY_BOUNDARY;
+    } else if (nchg == 0) {
+        ctl->done = 1;
+        ctl->reason = DMOLP_EXIT_NO_CHANGE;
+    } else {
+        const int64_t curc = (int64_t)ctl->v[C_CUT];
+        const int64_t prevc = (int64_t)ctl->v[C_PREV_CUT];
+        const double delta =
+            (prevc > 0) ? fabs((double)(curc - prevc) / (double)prevc) : 1.0;  // phase2.cpp:397
+        if (delta < epsilon) ctl->cc += 1; else ctl->cc = 0;
+        ctl->v[C_PREV_CUT] = (unsigned long long)curc;
+        cut_rec = curc;
+        if (ctl->cc >= 10) { ctl->done = 1; ctl->reason = DMOLP_EXIT_CONVERGED; }
+    }
+    if (!ctl->done && t >= max_rounds) { ctl->done = 1; ctl->reason = DMOLP_EXIT_MAX_ROUNDS; }
+    if (trace && t <= trace_cap) {
+        int64_t *r = trace + (int64_t)(t - 1) * T_NCOL;
+        r[T_PREV_N] = nprev;
+        r[T_PREV_DEG] = dprev;
+        r[T_CAND] = ncand;
+        r[T_BD] = (int64_t)*bd.cnt;
+        r[T_B] = nb;
+        r[T_BDEG] = bdeg;
+        r[T_CHG] = nb == 0 ? 0 : nchg;
+        r[T_CHGDEG] = nb == 0 ? 0 : cdeg;
+        r[T_CUT] = cut_rec;
+        r[T_BL] = (int64_t)cur.cnt[LC_LIGHT];
+        r[T_BLDEG] = (int64_t)(cur.cnt[LC_DEG] - cur.cnt[LC_HDEG]);
+    }
+}
+
+inline unsigned grid_for(dmolp_ctx *ctx) { return (unsigned)ctx->grid_blocks(); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+
+Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
+    DMOLP_REQUIRE(k >= 1 && k <= DMOLP_MAX_K, DMOLP_E_ARG, "k outside [1, 1024]");
+    graph_prepare_bins(ctx, g);
+    if (!g->p2) g->p2.reset(new Phase2Work());
+    Phase2Work &w = *g->p2;
+    const int64_t n = g->n;
+    if (!w.chg.p) {
+        w.chg.alloc(sizeof(int32_t) * (size_t)n);
+        w.diff.alloc(sizeof(int32_t) * (size_t)n);
+        w.stamp.alloc(sizeof(int32_t) * (size_t)n);
+        w.bd.alloc(n);
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, ctx->stream));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.chg.p, 0xff, w.chg.bytes, ctx->stream));
+        w.ctl.alloc(sizeof(Ctl));
+    }
+    if (w.cap_heavy != g->n_heavy || w.cap_chunks != g->n_heavy_chunks) {
+        for (auto &b : w.B) b.alloc(n, g->n_heavy, g->n_heavy_chunks);
+        w.D.alloc(n, g->n_heavy, g->n_heavy_chunks);
+        w.tmp.alloc(n, g->n_heavy, g->n_heavy_chunks);
+        w.hub_hist.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1) * kLpCap);
+        w.hub_done.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.hub_hist.p, 0, w.hub_hist.bytes, ctx->stream));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.hub_done.p, 0, w.hub_done.bytes, ctx->stream));
+        for (auto &b : w.B) DMOLP_HIP_CHECK(hipMemsetAsync(b.cnt.p, 0, b.cnt.bytes, ctx->stream));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.D.cnt.p, 0, w.D.cnt.bytes, ctx->stream));
+        w.cap_heavy = g->n_heavy;
+        w.cap_chunks = g->n_heavy_chunks;
+    }
+    if (w.k != k) {
+        w.k = k;
+        w.vc.alloc(sizeof(unsigned long long) * (size_t)k);
+        w.ic.alloc(sizeof(unsigned long long) * (size_t)k);
+        w.pen.alloc(sizeof(double) * (size_t)k);
+        w.re.alloc(sizeof(double) * (size_t)k);
+    }
+    w.sel.prepare(ctx, k, n, g->max_degree);
+    return w;
+}
+
+void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
+    hipStream_t s = ctx->stream;
+    DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(Ctl), s));
+    DMOLP_HIP_CHECK(hipMemsetAsync(w.vc.p, 0, w.vc.bytes, s));
+    DMOLP_HIP_CHECK(hipMemsetAsync(w.ic.p, 0, w.ic.bytes, s));
+    DMOLP_HIP_CHECK(hipMemsetAsync(w.diff.p, 0, w.diff.bytes, s));
+    Ctl *ctl = w.ctl.as<Ctl>();
+    const int32_t *L = g->labels.as<int32_t>();
+    hipLaunchKernelGGL(k_init_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, g->dev(), g->all.dev,
+                       L, w.k, w.diff.as<int32_t>(), ctl, w.vc.as<unsigned long long>(),
+                       w.ic.as<unsigned long long>());
+    if (g->n_heavy > 0)
+        hipLaunchKernelGGL(k_init_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, g->dev(),
+                           g->all.dev, L, w.k, ctx->tune.chunk_edges, w.diff.as<int32_t>(), ctl,
+                           w.vc.as<unsigned long long>(), w.ic.as<unsigned long long>());
+    hipLaunchKernelGGL(k_init_finish, dim3(1), dim3(256), 0, s, ctl, w.ic.as<unsigned long long>(),
+                       w.k);
+    DMOLP_HIP_CHECK(hipGetLastError());
+}
+
+void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t, int max_rounds,
+                          double epsilon) {
+    hipStream_t s = ctx->stream;
+    Ctl *ctl = w.ctl.as<Ctl>();
+    const int32_t *done = &ctl->done;
+    const GraphDev gd = g->dev();
+    int32_t *L = g->labels.as<int32_t>();
+    List &bprev = w.B[(t - 1) & 1];
+    List &bcur = w.B[t & 1];
+    const int H = ctx->tune.heavy_degree, CH = ctx->tune.chunk_edges;
+    const unsigned grid = grid_for(ctx);
+    const double lg = 1.0 + log((double)t);  // phase2.cpp:199-200
+
+    auto mark = [&]() -> size_t {
+        if (!ctx->profiling) return 0;
+        hipEvent_t e = ctx->prof_event();
+        DMOLP_HIP_CHECK(hipEventRecord(e, s));
+        return ctx->ev_used - 1;
+    };
+    const size_t m0 = mark();
+    hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, s, ctl, w.vc.as<unsigned long long>(),
+                       w.ic.as<unsigned long long>(), w.k, lg, w.pen.as<double>(),
+                       w.re.as<double>(), bcur.dev, w.D.dev, w.bd.dev);
+    if (t == 1) {
+        hipLaunchKernelGGL(k_cand_all, dim3(grid), dim3(kThreads), 0, s, gd, L, w.diff.as<int32_t>(),
+                           w.bd.dev, ctl);
+    } else {
+        const int32_t epoch = ++w.epoch;
+        hipLaunchKernelGGL(k_cand_light, dim3(grid), dim3(kThreads), 0, s, gd, bprev.dev, L,
+                           w.diff.as<int32_t>(), w.stamp.as<int32_t>(), epoch, w.bd.dev, ctl);
+        if (g->n_heavy > 0)
+            hipLaunchKernelGGL(k_cand_heavy, dim3(grid), dim3(kThreads), 0, s, gd, bprev.dev, L,
+                               w.diff.as<int32_t>(), w.stamp.as<int32_t>(), epoch, CH, w.bd.dev,
+                               ctl);
+    }
+    const size_t m1 = mark();
+    TakeRule rule{};
+    rule.kind = 0;
+    rule.ratio = exp((-t) / (5.0));  // gpu_lp_boundary.cu:244
+    rule.re = w.re.as<double>();
+    select_enqueue(ctx, w.sel, gd, w.bd.dev, rule, done, bcur.dev, nullptr);
+    const size_t m2 = mark();
+    hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, gd, L, bcur.dev,
+                       w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH, ctl);
+    const size_t m3 = mark();
+    if (g->n_heavy > 0)
+        hipLaunchKernelGGL(k_lp_heavy, dim3(grid), dim3(kThreads), 0, s, gd, L, bcur.dev,
+                           w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH,
+                           w.hub_hist.as<int32_t>(), w.hub_done.as<int32_t>(), ctl);
+    const size_t m4 = mark();
+    hipLaunchKernelGGL(k_delta_light, dim3(grid), dim3(kThreads), 0, s, gd, w.D.dev, L,
+                       w.chg.as<int32_t>(), w.k, w.diff.as<int32_t>(), ctl,
+                       w.vc.as<unsigned long long>(), w.ic.as<unsigned long long>());
+    if (g->n_heavy > 0)
+        hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, gd, w.D.dev, L,
+                           w.chg.as<int32_t>(), w.k, CH, w.diff.as<int32_t>(), ctl,
+                           w.vc.as<unsigned long long>(), w.ic.as<unsigned long long>());
+    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, w.D.dev, L, w.chg.as<int32_t>(),
+                       ctl);
+    hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ctl, bprev.dev, bcur.dev, w.D.dev,
+                       w.bd.dev, t, max_rounds, epsilon, w.trace.as<int64_t>(), w.trace_cap, g->n,
+                       g->nnz);
+    DMOLP_HIP_CHECK(hipGetLastError());
+    if (ctx->profiling) {
+        const size_t m5 = mark();
+        // Groups: 0 lp_light, 1 lp_heavy, 2 candidates, 3 select, 4 delta+apply+end, 5 round.
+        const std::pair<size_t, size_t> spans[6] = {{m2, m3}, {m3, m4}, {m0, m1}, {m1, m2},
+                                                    {m4, m5}, {m0, m5}};
+        if (ctx->prof_marks.size() < 6) ctx->prof_marks.resize(6);
+        for (int i = 0; i < 6; ++i) ctx->prof_marks[(size_t)i].push_back(spans[i]);
+    }
+}
+
+// Device penalties for the test hook.
+void phase2_penalties_device(dmolp_ctx *ctx, int k, const unsigned long long *d_vc,
+                             const unsigned long long *d_ic, int t, double *d_pen, double *d_re) {
+    const double lg = 1.0 + log((double)t);
+    hipLaunchKernelGGL(k_penalties, dim3(1), dim3(64), 0, ctx->stream, d_vc, d_ic, k, lg, d_pen,
+                       d_re);
+    DMOLP_HIP_CHECK(hipGetLastError());
+}
+
+// Test hook pieces: run the candidate + select steps of round t against an
+// explicit previous boundary already placed in w.B[0] (as light/heavy
+// lists), with re already on the device.  Output in w.B[1].
+void phase2_select_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t) {
+    hipStream_t s = ctx->stream;
+    Ctl *ctl = w.ctl.as<Ctl>();
+    const GraphDev gd = g->dev();
+    const int32_t *L = g->labels.as<int32_t>();
+    DMOLP_HIP_CHECK(hipMemsetAsync(w.bd.cnt.p, 0, w.bd.cnt.bytes, s));
+    DMOLP_HIP_CHECK(hipMemsetAsync(w.B[1].cnt.p, 0, w.B[1].cnt.bytes, s));
+    const int32_t epoch = ++w.epoch;
+    hipLaunchKernelGGL(k_cand_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, w.B[0].dev, L,
+                       w.diff.as<int32_t>(), w.stamp.as<int32_t>(), epoch, w.bd.dev, ctl);
+    if (g->n_heavy > 0)
+        hipLaunchKernelGGL(k_cand_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, w.B[0].dev,
+                           L, w.diff.as<int32_t>(), w.stamp.as<int32_t>(), epoch,
+                           ctx->tune.chunk_edges, w.bd.dev, ctl);
+    TakeRule rule{};
+    rule.kind = 0;
+    rule.ratio = exp((-t) / (5.0));
+    rule.re = w.re.as<double>();
+    select_enqueue(ctx, w.sel, gd, w.bd.dev, rule, &ctl->done, w.B[1].dev, nullptr);
+}
+
+// Test hook: decisions for the list in w.B[1]; Delta applied to g's labels.
+void phase2_lp_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
+    hipStream_t s = ctx->stream;
+    Ctl *ctl = w.ctl.as<Ctl>();
+    const GraphDev gd = g->dev();
+    int32_t *L = g->labels.as<int32_t>();
+    const int H = ctx->tune.heavy_degree, CH = ctx->tune.chunk_edges;
+    DMOLP_HIP_CHECK(hipMemsetAsync(w.D.cnt.p, 0, w.D.cnt.bytes, s));
+    hipLaunchKernelGGL(k_lp_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L, w.B[1].dev,
+                       w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH, ctl);
+    if (g->n_heavy > 0)
+        hipLaunchKernelGGL(k_lp_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L,
+                           w.B[1].dev, w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H,
+                           CH, w.hub_hist.as<int32_t>(), w.hub_done.as<int32_t>(), ctl);
+    hipLaunchKernelGGL(k_apply, dim3(grid_for(ctx)), dim3(kThreads), 0, s, w.D.dev, L,
+                       w.chg.as<int32_t>(), ctl);
+    DMOLP_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace dmolp

@@ -1,0 +1,63 @@
+// phase2.hpp -- device state of the boundary label-propagation phase.
+#pragma once
+#include "internal.hpp"
+#include "select.hpp"
+
+namespace dmolp {
+
+// 64-bit control counters (device).
+enum CtlIndex {
+    C_CUT = 0,       // current cut (undirected)
+    C_PREV_CUT,      // cut of the previous completed round
+    C_INIT_CUT,      // cut before round 1
+    C_CAND,          // first-touched candidates this round
+    C_LP_EDGES,      // sum over rounds of sum deg(B_t)
+    C_INVALID,       // entries with a -1 label (initial sweep)
+    C_CUT2,          // differing entries (initial sweep)
+    C_PREV_N,        // |B_{t-1}|
+    C_PREV_DEG,      // sum deg over B_{t-1}
+    C_KERNEL_ERR,    // a kernel found an impossible state
+    C_NCTL = 16
+};
+
+struct Ctl {
+    unsigned long long v[C_NCTL];
+    int32_t done;
+    int32_t reason;
+    int32_t rounds;
+    int32_t cc;
+    int32_t err;  // DMOLP_E_* raised on the device
+    int32_t pad[3];
+};
+
+// Trace columns written by the round-end kernel.
+enum TraceCol {
+    T_PREV_N = 0, T_PREV_DEG, T_CAND, T_BD, T_B, T_BDEG, T_CHG, T_CHGDEG, T_CUT, T_BL, T_BLDEG, T_NCOL
+};
+
+struct Phase2Work {
+    int k = 0;
+    int64_t cap_heavy = -1, cap_chunks = -1;  // heavy capacities the lists were sized for
+    int32_t epoch = 0;  // stamp value of the current candidate pass
+    DevMem chg, diff, stamp;
+    List B[2], D, tmp;
+    Recs bd;
+    SelectWork sel;
+    DevMem hub_hist, hub_done;
+    DevMem ctl;
+    DevMem vc, ic;  // unsigned long long [k]
+    DevMem pen, re; // double [k]
+    DevMem trace;   // int64 [trace_cap * T_NCOL]
+    int trace_cap = 0;
+};
+
+Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
+
+// Enqueue: reset state, full sweep of the labels (vc, ic, cut, per-vertex
+// differing-neighbour counts).  Labels are g's resident labels.
+void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
+// Enqueue one LP round t (1-based).
+void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t, int max_rounds,
+                          double epsilon);
+
+}  // namespace dmolp

@@ -1,0 +1,87 @@
+// rows.hpp -- device-side traversal of CSR rows named by a work list.
+//
+// Light rows (degree <= heavy_degree) are processed 64 list entries per
+// wave: each lane loads one entry's row bounds, a wave scan concatenates the
+// rows, and the wave then walks the concatenated edge range 64 edges at a
+// time, so short rows do not leave lanes idle.  Heavy rows are split into
+// chunks of chunk_edges edges, one chunk per wave.
+#pragma once
+#include "dev_common.hpp"
+#include "internal.hpp"
+
+namespace dmolp {
+
+// LDS block of one wave's batch: inclusive degree prefix and the value
+// e - q for each slot, so edge index = s_base[slot] + q.
+struct BatchLds {
+    int32_t incl[kWave];
+    int32_t v[kWave];
+    int32_t aux[kWave];  // free per-slot word for the caller
+    int64_t base[kWave];
+};
+
+// Load a batch of up to 64 list entries starting at b0.  `ids == nullptr`
+// means the identity list 0..count-1.  Returns the batch's total edge count.
+__device__ __forceinline__ int32_t batch_load(const GraphDev &g, const int32_t *ids, int64_t count,
+                                              int64_t b0, BatchLds &lds, int32_t &v, int32_t &deg,
+                                              bool &valid) {
+    const int lane = lane_id();
+    int64_t i = b0 + lane;
+    valid = i < count;
+    v = valid ? (ids ? ids[i] : (int32_t)i) : 0;
+    int64_t s = valid ? g.off[v] : 0;
+    deg = valid ? (int32_t)(g.off[v + 1] - s) : 0;
+    int64_t incl = wave_incl_scan((int64_t)deg);
+    lds.incl[lane] = (int32_t)incl;
+    lds.v[lane] = v;
+    lds.base[lane] = s - (incl - deg);
+    int32_t total = (int32_t)wave_bcast(incl, kWave - 1);
+    wave_sync();
+    return total;
+}
+
+// Slot owning concatenated position q (q < total): the first slot whose
+// inclusive prefix exceeds q.
+__device__ __forceinline__ int batch_slot(const BatchLds &lds, int32_t q) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+        if (lds.incl[lo + step - 1] <= q) lo += step;
+    return lo;
+}
+
+// Append v to a list (all lanes of the wave must call; pred selects lanes).
+__device__ __forceinline__ void list_append(const ListDev &L, bool pred, int32_t v, int64_t deg,
+                                            int heavy_degree, int chunk_edges) {
+    const bool light = pred && deg <= heavy_degree;
+    int64_t pos = wave_append(light, &L.cnt[LC_LIGHT]);
+    if (light) L.light[pos] = v;
+    const bool heavy = pred && deg > heavy_degree;
+    if (__ballot(heavy)) {
+        if (heavy) {
+            int64_t slot = (int64_t)atomicAdd(&L.cnt[LC_SLOTS], 1ull);
+            int64_t nch = (deg + chunk_edges - 1) / chunk_edges;
+            int64_t base = (int64_t)atomicAdd(&L.cnt[LC_CHUNKS], (unsigned long long)nch);
+            L.hv[slot] = v;
+            L.hn[slot] = (int32_t)nch;
+            atomicAdd(&L.cnt[LC_HDEG], (unsigned long long)deg);
+            for (int64_t c = 0; c < nch; ++c) L.hc[base + c] = (slot << 32) | c;
+        }
+    }
+    wave_add(&L.cnt[LC_DEG], pred ? deg : 0);
+}
+
+// Bounds of one heavy chunk item.
+__device__ __forceinline__ void chunk_bounds(const GraphDev &g, const ListDev &L, int64_t item,
+                                             int chunk_edges, int32_t &slot, int32_t &v,
+                                             int64_t &e0, int64_t &e1, int64_t &deg) {
+    slot = (int32_t)(item >> 32);
+    const int64_t chunk = item & 0xffffffffll;
+    v = L.hv[slot];
+    const int64_t s = g.off[v], t = g.off[v + 1];
+    deg = t - s;
+    e0 = s + chunk * chunk_edges;
+    e1 = e0 + chunk_edges < t ? e0 + chunk_edges : t;
+}
+
+}  // namespace dmolp

@@ -1,0 +1,282 @@
+// select.hip -- per-group top-m selection by (degree, id), see select.hpp.
+#include <cmath>
+
+#include "rows.hpp"
+#include "select.hpp"
+
+namespace dmolp {
+namespace {
+
+constexpr int kSelThreads = 256;
+constexpr int kHist1LdsInts = 8192;  // 32 KB of LDS for the level-1 histogram
+
+__device__ __forceinline__ uint64_t sel_key(int32_t x, int32_t d, int desc, int id_bits,
+                                            int deg_bits) {
+    const uint64_t dmax = (deg_bits >= 63) ? ~0ull : ((1ull << deg_bits) - 1);
+    const uint64_t dk = desc ? dmax - (uint64_t)d : (uint64_t)d;
+    return (dk << id_bits) | (uint64_t)(uint32_t)x;
+}
+
+__device__ __forceinline__ int bin1_of(int32_t d, int b1_bins) {
+    return d < b1_bins - 1 ? d : b1_bins - 1;
+}
+
+// Running count over bins in scan order (reverse: from the last bin down);
+// finds the first bin where it reaches target (1 <= target <= total).
+// Returns through LDS variables; all threads must call.
+__device__ void block_crossing(const int32_t *bins, int nb, bool reverse, int64_t target,
+                               int *s_bin, int64_t *s_before, int64_t *s_tmp) {
+    const int t = threadIdx.x, T = blockDim.x;
+    const int per = (nb + T - 1) / T;
+    const int j0 = t * per, j1 = min(nb, j0 + per);
+    int64_t sum = 0;
+    for (int j = j0; j < j1; ++j) sum += bins[reverse ? nb - 1 - j : j];
+    s_tmp[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < T; o <<= 1) {
+        int64_t add = t >= o ? s_tmp[t - o] : 0;
+        __syncthreads();
+        s_tmp[t] += add;
+        __syncthreads();
+    }
+    const int64_t incl = s_tmp[t], excl = incl - sum;
+    if (excl < target && target <= incl) {
+        int64_t run = excl;
+        for (int j = j0; j < j1; ++j) {
+            const int b = reverse ? nb - 1 - j : j;
+            const int64_t c = bins[b];
+            if (run + c >= target) { *s_bin = b; *s_before = run; break; }
+            run += c;
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_sel_hist1(RecDev in, int32_t *h1, int groups,
+                                                           int b1_bins, const int32_t *done) {
+    if (done && *done) return;
+    __shared__ int32_t s_h[kHist1LdsInts];
+    const int nb = groups * b1_bins;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s_h[i] = 0;
+    __syncthreads();
+    const int64_t count = (int64_t)*in.cnt;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t gg = in.g[i];
+        atomicAdd(&s_h[gg * b1_bins + bin1_of(in.d[i], b1_bins)], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        if (s_h[i]) atomicAdd(&h1[i], s_h[i]);
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_sel_pick1(int32_t *h1, int b1_bins, SelState *st,
+                                                           TakeRule rule, int id_bits, int deg_bits,
+                                                           int32_t *refining, const int32_t *done) {
+    if (done && *done) return;
+    __shared__ int64_t s_tmp[kSelThreads];
+    __shared__ int s_bin;
+    __shared__ int64_t s_before;
+    const int g = blockIdx.x;
+    int32_t *bins = h1 + (int64_t)g * b1_bins;
+    // Group size.
+    int64_t part = 0;
+    for (int j = threadIdx.x; j < b1_bins; j += blockDim.x) part += bins[j];
+    s_tmp[threadIdx.x] = part;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s_tmp[threadIdx.x] += s_tmp[threadIdx.x + o];
+        __syncthreads();
+    }
+    const int64_t c = s_tmp[0];
+    __syncthreads();
+    int64_t m;
+    int desc;
+    if (rule.kind == 0) {
+        // gpu_lp_boundary.cu:282,296-297.
+        desc = rule.re[g] > 1.0 ? 1 : 0;
+        m = c > 0 ? (int64_t)(int32_t)ceil(rule.ratio * (double)c) : 0;
+        if (m > c) m = c;
+    } else {
+        // partition.cpp:426-430.
+        desc = 0;
+        m = c < (int64_t)rule.theta ? c : (int64_t)rule.theta;
+    }
+    SelState s{};
+    s.desc = desc;
+    s.take = m;
+    s.count = c;
+    if (m <= 0) {
+        s.mode = SEL_NONE;
+    } else if (m == c) {
+        s.mode = SEL_ALL;
+    } else {
+        block_crossing(bins, b1_bins, desc != 0, m, &s_bin, &s_before, s_tmp);
+        const int b = s_bin;
+        const int64_t before = s_before;
+        s.b1 = b;
+        if (before + bins[b] == m) {
+            s.mode = SEL_THROUGH;
+        } else {
+            s.mode = SEL_REFINE;
+            s.need = m - before;
+            if (b < b1_bins - 1) {
+                const uint64_t dmax = (1ull << deg_bits) - 1;
+                const uint64_t dk = desc ? dmax - (uint64_t)b : (uint64_t)b;
+                s.prefix = dk << id_bits;
+                s.r = id_bits;
+            } else {
+                s.prefix = 0;
+                s.r = id_bits + deg_bits;
+            }
+            if (threadIdx.x == 0) atomicAdd(refining, 1);
+        }
+    }
+    if (threadIdx.x == 0) st[g] = s;
+    __syncthreads();
+    for (int j = threadIdx.x; j < b1_bins; j += blockDim.x) bins[j] = 0;
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_sel_hist2(RecDev in, int32_t *h2,
+                                                           const SelState *st, int b1_bins,
+                                                           int id_bits, int deg_bits,
+                                                           const int32_t *refining,
+                                                           const int32_t *done) {
+    if (done && *done) return;
+    if (*refining == 0) return;
+    const int64_t count = (int64_t)*in.cnt;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t gg = in.g[i];
+        const SelState s = st[gg];
+        if (s.mode != SEL_REFINE) continue;
+        const int32_t d = in.d[i];
+        if (bin1_of(d, b1_bins) != s.b1) continue;
+        const uint64_t K = sel_key(in.x[i], d, s.desc, id_bits, deg_bits);
+        if ((K >> s.r) != (s.prefix >> s.r)) continue;
+        const int w = s.r < kDigitBits ? s.r : kDigitBits;
+        const int digit = (int)((K >> (s.r - w)) & ((1ull << w) - 1));
+        atomicAdd(&h2[(int64_t)gg * kDigitBins + digit], 1);
+    }
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_sel_pick2(int32_t *h2, SelState *st,
+                                                           int32_t *refining,
+                                                           const int32_t *done) {
+    if (done && *done) return;
+    __shared__ int64_t s_tmp[kSelThreads];
+    __shared__ int s_bin;
+    __shared__ int64_t s_before;
+    const int g = blockIdx.x;
+    SelState s = st[g];
+    if (s.mode != SEL_REFINE) return;  // uniform per block
+    const int w = s.r < kDigitBits ? s.r : kDigitBits;
+    const int nb = 1 << w;
+    int32_t *bins = h2 + (int64_t)g * kDigitBins;
+    block_crossing(bins, nb, false, s.need, &s_bin, &s_before, s_tmp);
+    const int digit = s_bin;
+    const int64_t cnt = bins[digit];
+    s.need -= s_before;
+    s.prefix |= (uint64_t)digit << (s.r - w);
+    s.r -= w;
+    if (s.need == cnt) {
+        s.mode = SEL_KEY;
+        if (threadIdx.x == 0) atomicSub(refining, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) st[g] = s;
+    for (int j = threadIdx.x; j < nb; j += blockDim.x) bins[j] = 0;
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_sel_compact(RecDev in, const SelState *st,
+                                                             int b1_bins, int id_bits,
+                                                             int deg_bits, ListDev sel,
+                                                             RecDev carry, int has_carry,
+                                                             int heavy_degree, int chunk_edges,
+                                                             const int32_t *done) {
+    if (done && *done) return;
+    const int64_t count = (int64_t)*in.cnt;
+    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+        const int64_t i = b0 + lane_id();
+        const bool valid = i < count;
+        int32_t x = 0, gg = 0, d = 0;
+        bool take = false;
+        if (valid) {
+            x = in.x[i];
+            gg = in.g[i];
+            d = in.d[i];
+            const SelState s = st[gg];
+            const int b = bin1_of(d, b1_bins);
+            const bool earlier = s.desc ? b > s.b1 : b < s.b1;
+            switch (s.mode) {
+                case SEL_ALL: take = true; break;
+                case SEL_THROUGH: take = earlier || b == s.b1; break;
+                case SEL_KEY: {
+                    if (earlier) { take = true; break; }
+                    if (b != s.b1) break;
+                    const uint64_t K = sel_key(x, d, s.desc, id_bits, deg_bits);
+                    const uint64_t kmax = s.prefix | ((s.r >= 64) ? ~0ull : ((1ull << s.r) - 1));
+                    take = K <= kmax;
+                    break;
+                }
+                default: break;
+            }
+        }
+        list_append(sel, take, x, d, heavy_degree, chunk_edges);
+        if (has_carry) {
+            const bool keep = valid && !take;
+            const int64_t pos = wave_append(keep, carry.cnt);
+            if (keep) { carry.x[pos] = x; carry.g[pos] = gg; carry.d[pos] = d; }
+        }
+    }
+}
+
+}  // namespace
+
+void SelectWork::prepare(dmolp_ctx *ctx, int ngroups, int64_t n, int32_t max_degree) {
+    const int ib = (int)(bits_for(n > 1 ? n - 1 : 1));
+    const int db = (int)(bits_for(max_degree > 1 ? max_degree : 1));
+    if (groups == ngroups && id_bits == ib && deg_bits == db) return;
+    groups = ngroups;
+    id_bits = ib;
+    deg_bits = db;
+    int b1 = kHist1LdsInts / ngroups;
+    if (b1 > kSelBins) b1 = kSelBins;
+    DMOLP_REQUIRE(b1 >= 2, DMOLP_E_UNSUPPORTED, "too many groups for the selection histogram");
+    b1_bins = b1;
+    passes = (id_bits + deg_bits + kDigitBits - 1) / kDigitBits;
+    hist1_blocks = ctx->cu_count * 2;
+    h1.alloc(sizeof(int32_t) * (size_t)groups * b1_bins);
+    h2.alloc(sizeof(int32_t) * (size_t)groups * kDigitBins);
+    st.alloc(sizeof(SelState) * (size_t)groups);
+    refining.alloc(sizeof(int32_t) * 4);
+    DMOLP_HIP_CHECK(hipMemsetAsync(h1.p, 0, h1.bytes, ctx->stream));
+    DMOLP_HIP_CHECK(hipMemsetAsync(h2.p, 0, h2.bytes, ctx->stream));
+    DMOLP_HIP_CHECK(hipMemsetAsync(refining.p, 0, refining.bytes, ctx->stream));
+}
+
+void select_enqueue(dmolp_ctx *ctx, SelectWork &w, const GraphDev &g, const RecDev &in,
+                    const TakeRule &rule, const int32_t *done, const ListDev &sel,
+                    const RecDev *carry) {
+    (void)g;
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_sel_hist1, dim3(w.hist1_blocks), dim3(kSelThreads), 0, s, in,
+                       w.h1.as<int32_t>(), w.groups, w.b1_bins, done);
+    hipLaunchKernelGGL(k_sel_pick1, dim3(w.groups), dim3(kSelThreads), 0, s, w.h1.as<int32_t>(),
+                       w.b1_bins, w.st.as<SelState>(), rule, w.id_bits, w.deg_bits,
+                       w.refining.as<int32_t>(), done);
+    for (int p = 0; p < w.passes; ++p) {
+        hipLaunchKernelGGL(k_sel_hist2, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, in,
+                           w.h2.as<int32_t>(), w.st.as<SelState>(), w.b1_bins, w.id_bits,
+                           w.deg_bits, w.refining.as<int32_t>(), done);
+        hipLaunchKernelGGL(k_sel_pick2, dim3(w.groups), dim3(kSelThreads), 0, s, w.h2.as<int32_t>(),
+                           w.st.as<SelState>(), w.refining.as<int32_t>(), done);
+    }
+    RecDev c = carry ? *carry : RecDev{nullptr, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL(k_sel_compact, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, in,
+                       w.st.as<SelState>(), w.b1_bins, w.id_bits, w.deg_bits, sel, c,
+                       carry ? 1 : 0, ctx->tune.heavy_degree, ctx->tune.chunk_edges, done);
+    DMOLP_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace dmolp

@@ -1,0 +1,82 @@
+// select.hpp -- per-group deterministic top-m selection by (degree, id).
+//
+// Used for the boundary selection of gpu_lp_boundary.cu:277-302 (per label,
+// the first ceil(ratio*|group|) by degree descending when RE > 1.0 else
+// ascending, ties by id) and for the frontier throttle of
+// partition.cpp:400-431 (per partition, the theta smallest (degree, id)).
+//
+// No sort: a level-1 histogram over min(degree, B1-1) finds the degree bin
+// where the cut falls; if that bin must be split, 12-bit MSD digit passes
+// over the key K = (dk << IB) | id (dk = degree, or maxdeg - degree for the
+// descending order) find the exact threshold key.  Every pass is a
+// histogram, so the selected set does not depend on record order.
+#pragma once
+#include "internal.hpp"
+
+namespace dmolp {
+
+struct RecDev {
+    int32_t *x;  // vertex id
+    int32_t *g;  // group (label / partition)
+    int32_t *d;  // degree
+    unsigned long long *cnt;
+};
+
+struct Recs {
+    DevMem x, g, d, cnt;
+    RecDev dev{};
+    void alloc(int64_t cap) {
+        x.alloc(sizeof(int32_t) * (size_t)(cap > 0 ? cap : 1));
+        g.alloc(sizeof(int32_t) * (size_t)(cap > 0 ? cap : 1));
+        d.alloc(sizeof(int32_t) * (size_t)(cap > 0 ? cap : 1));
+        cnt.alloc(sizeof(unsigned long long) * 2);
+        dev = RecDev{x.as<int32_t>(), g.as<int32_t>(), d.as<int32_t>(), cnt.as<unsigned long long>()};
+    }
+};
+
+enum SelMode : int32_t {
+    SEL_NONE = 0,     // take nothing
+    SEL_ALL = 1,      // take the whole group
+    SEL_THROUGH = 2,  // take every record up to and including level-1 bin b1
+    SEL_REFINE = 3,   // threshold inside bin b1 still being refined
+    SEL_KEY = 4       // take records before bin b1, and in b1 those with K <= prefix|(2^r-1)
+};
+
+struct SelState {
+    int32_t mode;
+    int32_t b1;
+    int32_t r;
+    int32_t desc;
+    int64_t need;
+    uint64_t prefix;
+    int64_t take;
+    int64_t count;
+};
+
+// How the per-group take count is derived from the group size c.
+struct TakeRule {
+    int32_t kind;          // 0: ceil(ratio * c) capped at c, desc = re[g] > 1.0; 1: min(theta, c), asc
+    int32_t theta;
+    double ratio;
+    const double *re;      // device, groups entries (kind 0)
+};
+
+struct SelectWork {
+    int groups = 0;
+    int b1_bins = 0;
+    int id_bits = 0;
+    int deg_bits = 0;
+    int passes = 0;
+    int hist1_blocks = 0;
+    DevMem h1, h2, st, refining;
+    void prepare(dmolp_ctx *ctx, int groups, int64_t n, int32_t max_degree);
+};
+
+// Enqueue the whole selection: level-1 histogram, pick, digit passes,
+// compaction.  Selected records go to `sel` (a vertex list); when `carry`
+// is non-null the unselected records are appended to it.
+void select_enqueue(dmolp_ctx *ctx, SelectWork &w, const GraphDev &g, const RecDev &in,
+                    const TakeRule &rule, const int32_t *done, const ListDev &sel,
+                    const RecDev *carry);
+
+}  // namespace dmolp

@@ -1,0 +1,289 @@
+/*
+ * dmolp_hip.h -- C ABI of the MI355X (gfx950) implementation of DMOLP's
+ * two-phase partitioner path.
+ *
+ * The reference exposes its path as C++ functions; each entry point below
+ * names the reference interface it replaces.  The C++ wrappers with the
+ * reference's own signatures (run_phase1 / run_phase2 over graph_types.h)
+ * live in include/phase1/phase1.h and include/phase2/phase2.h and call this
+ * ABI.
+ *
+ * Conventions
+ *  - Every function returns DMOLP_OK (0) or a negative dmolp_status; no C++
+ *    exception crosses the ABI.  dmolp_last_error(ctx) gives the message of
+ *    the last failure on that context.
+ *  - Host buffers are owned by the caller.  Device memory is owned by the
+ *    context and graph handles.
+ *  - One context per host thread; a context is not re-entrant.  Several
+ *    contexts may share a device.
+ *  - Graph layout: symmetric CSR, rows sorted ascending with no duplicates
+ *    and no self-loops; offsets int32 or int64 (offset_bytes 4 or 8),
+ *    columns int32, labels int32.  This is graph_types.h's Graph
+ *    (row_ptr / col_indices / vertex_labels, graph_types.h:126-132) with
+ *    offsets widened to int64 where int32 would overflow.
+ */
+#ifndef DMOLP_HIP_H
+#define DMOLP_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMOLP_ABI_VERSION 1
+
+typedef enum {
+    DMOLP_OK = 0,
+    DMOLP_E_ARG = -1,           /* invalid argument */
+    DMOLP_E_SEEDS = -2,         /* k larger than the vertex count */
+    DMOLP_E_INVALID_LABEL = -3, /* a -1 label reached the cut (phase2.cpp:239-244) */
+    DMOLP_E_ODD_CUT = -4,       /* odd cut: asymmetric CSR (phase2.cpp:246-250) */
+    DMOLP_E_HIP = -5,           /* HIP runtime failure */
+    DMOLP_E_NOMEM = -6,         /* device or host allocation failed */
+    DMOLP_E_UNSUPPORTED = -7,   /* valid request outside this build's limits */
+    DMOLP_E_STATE = -8,         /* call out of order (e.g. no labels resident) */
+    DMOLP_E_GRAPH = -9          /* CSR fails validation (self-loop, unsorted row, bad offset) */
+} dmolp_status;
+
+/* Phase-2 exit reasons (phase2.cpp:309-312, 387-390, 409-415). */
+#define DMOLP_EXIT_MAX_ROUNDS 0
+#define DMOLP_EXIT_EMPTY_BOUNDARY 1
+#define DMOLP_EXIT_NO_CHANGE 2
+#define DMOLP_EXIT_CONVERGED 3
+
+/* Labels at or above this value are invisible to the LP decision
+ * (gpu_lp_boundary.cu:360). */
+#define DMOLP_LP_LABEL_CAP 32
+/* Largest k this build accepts. */
+#define DMOLP_MAX_K 1024
+
+typedef struct dmolp_ctx dmolp_ctx;
+typedef struct dmolp_graph dmolp_graph;
+
+/* ---------------- context ---------------- */
+
+int dmolp_abi_version(void);
+const char *dmolp_status_string(int status);
+/* Message of the last failure on ctx (never NULL). */
+const char *dmolp_last_error(const dmolp_ctx *ctx);
+
+/* Owns the device binding, one HIP stream, kernel scratch and the
+ * autotuned launch parameters.  Replaces allocateGPU (main.cpp:25-121) for
+ * one GPU and the per-call stream of gpu_lp_boundary.cu:448-449. */
+int dmolp_ctx_create(int device, dmolp_ctx **out);
+int dmolp_ctx_destroy(dmolp_ctx *ctx);
+/* Stream the context launches on (a hipStream_t), for callers that time or
+ * order work around it. */
+void *dmolp_ctx_stream(dmolp_ctx *ctx);
+
+/* Per-round device times of kernel groups, recorded with HIP events on the
+ * context stream during dmolp_phase2 while profiling is on (no counterpart
+ * in the reference, which times whole GPU calls with std::chrono,
+ * phase2.cpp:324-339).  group: 0 light-row LP decision, 1 heavy-row LP
+ * decision, 2 round start + candidates, 3 selection, 4 delta statistics +
+ * apply + round end, 5 whole round. */
+#define DMOLP_PROF_LP_LIGHT 0
+#define DMOLP_PROF_LP_HEAVY 1
+#define DMOLP_PROF_CANDIDATES 2
+#define DMOLP_PROF_SELECT 3
+#define DMOLP_PROF_DELTA 4
+#define DMOLP_PROF_ROUND 5
+int dmolp_ctx_set_profiling(dmolp_ctx *ctx, int on);
+int dmolp_ctx_round_times(dmolp_ctx *ctx, int group, double *ms_out, int cap, int *n_out);
+
+/* Launch parameters the autotuner chooses; 0 means "keep". */
+typedef struct {
+    int32_t heavy_degree;  /* rows longer than this are split into chunks */
+    int32_t chunk_edges;   /* edges per chunk of a heavy row */
+    int32_t block_threads; /* threads per workgroup for the row kernels */
+    int32_t grid_blocks;   /* workgroups of the persistent row kernels */
+} dmolp_tuning;
+int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out);
+int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t);
+
+/* ---------------- graphs ---------------- */
+
+/* Generator description (no counterpart in the reference, whose inputs are
+ * WebGraph text files; post-processing matches tool/convert.sh:76-79,
+ * 106-116: symmetrise, drop self-loops, dedupe, sort rows). */
+#define DMOLP_GEN_RMAT 1     /* R-MAT, no noise: scale, edge_factor, a, b, c */
+#define DMOLP_GEN_CHUNG_LU 2 /* n, m directed samples, gamma, dmax */
+#define DMOLP_GEN_GRID 3     /* 2-D 5-point grid: rows x cols */
+typedef struct {
+    int32_t kind;
+    int32_t scale;
+    int32_t edge_factor;
+    int32_t rows;
+    int32_t cols;
+    int32_t reserved;
+    double a, b, c;
+    double gamma;
+    double dmax;
+    int64_t n;
+    int64_t m;
+    uint64_t seed;
+} dmolp_gen_spec;
+
+/* Host CSR owned by the library (generator and file-reader output). */
+typedef struct {
+    int64_t n;
+    int64_t nnz;
+    int64_t *off; /* n + 1 */
+    int32_t *col; /* nnz */
+} dmolp_csr;
+
+/* Generate on the host (threads); free with dmolp_csr_free. */
+int dmolp_gen_host(const dmolp_gen_spec *spec, dmolp_csr *out);
+void dmolp_csr_free(dmolp_csr *csr);
+
+/* Reference text adjacency format (init.cpp:25-100; tool/convert.sh:132-176):
+ * first line V, then one line per vertex listing its neighbours.  Rows are
+ * sorted and deduplicated on load, self-loops dropped, as
+ * build_local_graph_and_ghosts does (partition.cpp:231-232). */
+int dmolp_read_adjacency_text(const char *path, dmolp_csr *out);
+int dmolp_write_adjacency_text(const char *path, int64_t n, const int64_t *off, const int32_t *col);
+
+/* Make a CSR device-resident (replaces the per-round H2D copies of
+ * gpu_lp_boundary.cu:456-471).  Validates offsets, sorted rows and the
+ * absence of self-loops on the device. */
+int dmolp_graph_upload(dmolp_ctx *ctx, int64_t n, int64_t nnz, const void *offsets,
+                       int offset_bytes, const int32_t *cols, dmolp_graph **out);
+/* Generate directly on the device; same graph as dmolp_gen_host. */
+int dmolp_graph_generate(dmolp_ctx *ctx, const dmolp_gen_spec *spec, dmolp_graph **out);
+int dmolp_graph_info(const dmolp_graph *g, int64_t *n, int64_t *nnz);
+/* Copy the CSR back to the host (off: n + 1 int64, col: nnz). */
+int dmolp_graph_download(dmolp_ctx *ctx, const dmolp_graph *g, int64_t *off, int32_t *col);
+int dmolp_graph_free(dmolp_ctx *ctx, dmolp_graph *g);
+
+/* Resident labels (n int32) of a graph: phase 1 writes them, phase 2 reads
+ * and updates them in place on the device. */
+int dmolp_graph_set_labels(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels);
+int dmolp_graph_get_labels(dmolp_ctx *ctx, const dmolp_graph *g, int32_t *labels);
+/* Device-side checkpoint of the resident labels: op 0 saves, op 1 restores.
+ * Lets a caller rerun phase 2 from the same start without host copies. */
+int dmolp_graph_labels_checkpoint(dmolp_ctx *ctx, dmolp_graph *g, int op);
+
+/* ---------------- phase 1 ---------------- */
+
+/* Phase1Metrics (graph_types.h:276-309) with int64 counters. */
+typedef struct {
+    int64_t initial_edge_cut;
+    double initial_vertex_balance;
+    double initial_edge_balance;
+    int64_t total_vertices;
+    int64_t total_edges;
+    int32_t first_seed;
+    int32_t first_seed_degree;
+    int64_t n_hubs;
+    int32_t n_seeds_found;    /* seeds from the hub search */
+    int32_t n_seeds_fallback; /* seeds added by the documented fallback */
+    int64_t rounds;           /* expansion rounds */
+    int32_t fill_round;       /* round of the fill step, -1 if none */
+    int32_t reserved;
+    double device_ms;         /* device time of phase 1 (hipEvents) */
+    double host_ms;           /* host time of the hub sort */
+} dmolp_phase1_metrics;
+
+/* Replaces run_phase1's hub-seed path (phase1.cpp:18-161 with mode=bfs;
+ * find_hub_nodes init.cpp:139-177, find_first_seed init.cpp:179-187,
+ * find_seeds seed.cpp:192-242, partition_expansion partition.cpp:323-621).
+ * Labels become resident in g; labels_out (n, nullable) receives a copy;
+ * seeds_out (k, nullable); vc, ec (k, nullable). */
+int dmolp_phase1(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t theta, int32_t *labels_out,
+                 int32_t *seeds_out, int64_t *vc, int64_t *ec, dmolp_phase1_metrics *m);
+
+/* ---------------- phase 2 ---------------- */
+
+/* PartitioningMetrics (graph_types.h:312-346) with int64 counters. */
+typedef struct {
+    int64_t edge_cut;
+    double vertex_balance;
+    double edge_balance;
+    int64_t total_vertices;
+    int64_t total_edges;
+    int32_t rounds;      /* 1-based index of the last round started */
+    int32_t exit_reason; /* DMOLP_EXIT_* */
+    int64_t lp_edges;    /* sum over rounds of sum_{b in B_t} deg b */
+    int64_t initial_cut;
+    double device_ms;    /* device time from phase-2 start to final labels */
+} dmolp_phase2_metrics;
+
+/* Per-round counters, arrays of max_rounds entries (any pointer may be
+ * NULL).  Same meaning as the oracle's trace. */
+typedef struct {
+    int64_t *prev_count; /* |B_{t-1}| */
+    int64_t *prev_deg;   /* sum deg over B_{t-1} */
+    int64_t *cand_count; /* |B_{t-1} u N(B_{t-1})| */
+    int64_t *bd_count;   /* boundary vertices before selection */
+    int64_t *b_count;    /* |B_t| */
+    int64_t *b_deg;      /* sum deg over B_t */
+    int64_t *chg_count;  /* |Delta_t| */
+    int64_t *chg_deg;    /* sum deg over Delta_t */
+    int64_t *cut;        /* cut after round t, -1 if the round stopped before stats */
+    int64_t *b_light_count; /* entries of B_t processed by the light-row kernel */
+    int64_t *b_light_deg;   /* their degree sum */
+} dmolp_trace;
+
+/* Replaces run_phase2 (phase2.cpp:256-449) with max_iter and epsilon as
+ * parameters (phase2.cpp:263-264).  Runs on g's resident labels; the whole
+ * round loop is enqueued without host synchronisation.  vc, ic (k,
+ * nullable) receive the final counts. */
+int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, double epsilon,
+                 int64_t *vc, int64_t *ic, dmolp_phase2_metrics *m, dmolp_trace *trace);
+
+/* ---------------- pieces (test hooks and the reference's GPU seam) ------- */
+
+/* computePartitionStats + computeEdgeCut (phase2.cpp:55-117, 214-253) on
+ * host labels. */
+int dmolp_stats(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, const int32_t *labels, int64_t *vc,
+                int64_t *ic, int64_t *cut);
+/* calculatePenalties (phase2.cpp:158-209) evaluated on the device. */
+int dmolp_penalties(dmolp_ctx *ctx, int32_t k, const int64_t *vc, const int64_t *ic, int32_t t,
+                    double *pen, double *re);
+/* expandBoundaryNodes (gpu_lp_boundary.cu:185-305): B_t from B_{t-1}.
+ * out (capacity n) is written in ascending id order. */
+int dmolp_select_boundary(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels,
+                          const int32_t *prev, int64_t n_prev, const double *re, int32_t k,
+                          int32_t t, int32_t *out, int64_t *n_out);
+/* runBoundaryLPOnGPU_Streaming's decision (gpu_lp_boundary.cu:525-565 with
+ * boundaryLPKernel_unified :327-427): labels_out = labels with every b in B
+ * replaced by its decision (Jacobi). */
+int dmolp_lp_round(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels, const int32_t *b,
+                   int64_t nb, const double *pen, int32_t k, int32_t *labels_out);
+
+/* ---------------- batched and served modes ---------------- */
+
+/* One job of a batch: a graph, its parameters, and where results go. */
+typedef struct {
+    dmolp_graph *graph;
+    int32_t k;
+    int32_t theta;
+    int32_t max_rounds;
+    int32_t reserved;
+    double epsilon;
+    int32_t *labels_out;  /* n, nullable */
+} dmolp_job;
+
+typedef struct {
+    int32_t status;
+    int32_t rounds;
+    int32_t exit_reason;
+    int32_t n_seeds_fallback;
+    int64_t phase1_cut;
+    int64_t edge_cut;
+    int64_t lp_edges;
+    double vertex_balance;
+    double edge_balance;
+} dmolp_result;
+
+/* Partition many graphs: every job's phase 1 + phase 2 rounds are advanced
+ * together, one multi-graph launch per kernel step, captured once as a HIP
+ * graph per step shape and replayed.  Each result equals the job run alone. */
+int dmolp_batch_run(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, dmolp_result *results,
+                    double *device_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

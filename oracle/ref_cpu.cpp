@@ -1,0 +1,663 @@
+// ref_cpu.cpp -- CPU oracle for the DMOLP partitioner path at one rank.
+//
+// TEST INFRASTRUCTURE ONLY: the checker the HIP path is compared against.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// it.  It is built with -O2 -fno-fast-math -ffp-contract=off so every double
+// operation is a correctly rounded IEEE operation in source order.
+//
+// Each function names the reference lines it restates.  The restatement is
+// written for one rank: owner(v) = 0, no ghosts, every MPI collective is the
+// identity (partition.cpp:55-60, utils.cpp:159-166).
+//
+// Pinning (what checks each function beyond this file):
+//   ref_probe_degree_map_order : itself a probe of libstdc++; tests assert
+//                                it yields descending ids.
+//   ref_first_seed             : closed form (min id < D among max-degree
+//                                vertices, else max id) on crafted graphs.
+//   ref_hubs                   : constant-degree graph (all hubs), stars,
+//                                the |H| ~ N*avg/max bound, NaN total.
+//   ref_ball2 / ref_seeds      : networkx shortest-path distance <= 2,
+//                                brute-force seed rule in Python.
+//   ref_expand                 : hand traces (path, star, 3x3 grid) for
+//                                theta in {1,2,inf}; theta=inf equals
+//                                lowest-p multi-source BFS (networkx);
+//                                invariants.
+//   ref_stats / metrics        : numpy recount, cut + sum ic = nnz/2.
+//   ref_penalties              : special cases t=1, balanced, two-part
+//                                closed forms computed by hand.
+//   ref_select_boundary        : Python brute force with full sorts.
+//   ref_lp_round               : simulation of the 32-lane shuffle tree of
+//                                gpu_lp_boundary.cu:393-421 in Python.
+#include "ref_cpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+namespace {
+
+inline int64_t deg_of(const int64_t *off, int64_t v) { return off[v + 1] - off[v]; }
+
+// Labels invisible to the LP: gpu_lp_boundary.cu:360 caps at 32.
+constexpr int kLpLabelCap = 32;
+// Consecutive small-change rounds before stopping: phase2.cpp:265.
+constexpr int kConvergeRounds = 10;
+
+// 5-bit bit reversal: the lane priority the shuffle tree of
+// gpu_lp_boundary.cu:406-414 produces among equal scores.
+inline int bitrev5(int x) {
+    int r = 0;
+    for (int i = 0; i < 5; ++i) r |= ((x >> i) & 1) << (4 - i);
+    return r;
+}
+
+// Breadth-first ball of depth <= 2 (seed.cpp:18 MAX_LEVELS = 3 keeps levels
+// 0, 1, 2; seed.cpp:33-98).
+void ball2(int64_t n, const int64_t *off, const int32_t *col, int32_t s,
+           std::vector<uint8_t> &mark) {
+    (void)n;
+    mark[s] = 1;
+    std::vector<int32_t> lvl1;
+    for (int64_t e = off[s]; e < off[s + 1]; ++e) {
+        int32_t v = col[e];
+        if (!mark[v]) { mark[v] = 1; lvl1.push_back(v); }
+    }
+    for (int32_t u : lvl1)
+        for (int64_t e = off[u]; e < off[u + 1]; ++e) mark[col[e]] = 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+// init.cpp:43-44,68,90-93 (adj and local_degree built with reserve(2V) in
+// ascending source order) and init.cpp:123-135 (global_degree with
+// reserve(V) filled in local_degree's iteration order), single rank.
+int ref_probe_degree_map_order(int32_t n, int32_t *order_out) {
+    if (n < 0 || (n > 0 && !order_out)) return REF_E_ARG;
+    std::unordered_map<int, std::vector<int>> adj;
+    std::unordered_map<int, int> local_degree;
+    adj.reserve((size_t)n * 2);
+    local_degree.reserve((size_t)n * 2);
+    for (int s = 0; s < n; ++s) (void)adj[s];
+    for (auto &kv : adj) local_degree[kv.first] = (int)kv.second.size();
+    std::vector<int> flat;
+    for (auto &kv : local_degree) { flat.push_back(kv.first); flat.push_back(kv.second); }
+    std::unordered_map<int, int> global_degree;
+    global_degree.reserve(flat.size() / 2);
+    for (size_t i = 0; i < flat.size(); i += 2) global_degree[flat[i]] = flat[i + 1];
+    int32_t i = 0;
+    for (auto &kv : global_degree) order_out[i++] = kv.first;
+    return REF_OK;
+}
+
+// init.cpp:179-187.  The degree map iterates in descending id at one rank;
+// the tie test compares a vertex id with the running best *degree*.
+int ref_first_seed(int64_t n, const int64_t *off, int32_t *seed_out, int32_t *deg_out) {
+    if (n <= 0 || !off) return REF_E_ARG;
+    int64_t best_node = -1, best_deg = -1;
+    for (int64_t v = n - 1; v >= 0; --v) {
+        int64_t d = deg_of(off, v);
+        if (d > best_deg || (d == best_deg && v < best_deg)) { best_node = v; best_deg = d; }
+    }
+    if (seed_out) *seed_out = (int32_t)best_node;
+    if (deg_out) *deg_out = (int32_t)best_deg;
+    return REF_OK;
+}
+
+// init.cpp:139-177.  The (id, degree) pairs enter std::sort in the degree
+// map's iteration order (descending id); the sort is libstdc++ introsort,
+// not stable, so the same call on the same sequence is required.
+int ref_hubs(int64_t n, const int64_t *off, int32_t *hubs_out, int64_t *n_hubs_out) {
+    if (n <= 0 || !off || !n_hubs_out) return REF_E_ARG;
+    std::vector<std::pair<int, int>> sd;
+    sd.reserve((size_t)n);
+    for (int64_t v = n - 1; v >= 0; --v) sd.emplace_back((int)v, (int)deg_of(off, v));
+    std::sort(sd.begin(), sd.end(),
+              [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.second < b.second; });
+    const int N = (int)sd.size();
+    int64_t nh = 0;
+    if (N >= 2) {
+        std::vector<double> cum((size_t)N);
+        double total = 0.0, run = 0.0;
+        for (int i = 0; i < N; ++i) {
+            int d = sd[(size_t)i].second;
+            total += d;
+            run += d;
+            cum[(size_t)i] = run;
+        }
+        for (int i = 0; i < N; ++i) cum[(size_t)i] /= total;
+        double inv_N = 1.0 / N;
+        double x1 = N * inv_N;
+        double y1 = cum[(size_t)N - 1];
+        double x0 = (N - 1) * inv_N;
+        double y0 = cum[(size_t)N - 2];
+        double slope = (y1 - y0) / (x1 - x0);
+        double xi = x1 - y1 / slope;
+        for (int i = N - 1; i >= 0; --i) {
+            double x = (i + 1) * inv_N;
+            if (x > xi) {
+                if (hubs_out) hubs_out[nh] = sd[(size_t)i].first;
+                ++nh;
+            } else {
+                break;
+            }
+        }
+    }
+    *n_hubs_out = nh;
+    return REF_OK;
+}
+
+int ref_ball2(int64_t n, const int64_t *off, const int32_t *col, int32_t s, uint8_t *ball_out) {
+    if (n <= 0 || s < 0 || s >= n || !ball_out) return REF_E_ARG;
+    std::vector<uint8_t> mark((size_t)n, 0);
+    ball2(n, off, col, s, mark);
+    std::memcpy(ball_out, mark.data(), (size_t)n);
+    return REF_OK;
+}
+
+// seed.cpp:138-242 at one rank.  Next seed = first hub in hub order that is
+// unused, outside the union of the chosen seeds' depth-2 balls, with the
+// strictly largest degree (seed.cpp:158-170).  The search stops at the first
+// failure (seed.cpp:214-217).  Documented extension for the reference's
+// undefined behaviour when fewer than k seeds exist (partition.cpp:347 reads
+// seeds[p] past the end): the missing seeds are the remaining vertices in
+// (degree descending, id ascending) order, skipping chosen seeds.
+int ref_seeds(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
+              int32_t *seeds_out, int32_t *n_found_out, int32_t *n_fallback_out) {
+    if (n <= 0 || k <= 0 || !seeds_out) return REF_E_ARG;
+    if ((int64_t)k > n) return REF_E_SEEDS;
+    int32_t s0, d0;
+    ref_first_seed(n, off, &s0, &d0);
+    std::vector<int32_t> hubs((size_t)n);
+    int64_t nh = 0;
+    ref_hubs(n, off, hubs.data(), &nh);
+    hubs.resize((size_t)nh);
+
+    std::vector<int32_t> seeds{s0};
+    if (nh > 0) {
+        std::vector<uint8_t> used((size_t)nh, 0);
+        std::vector<uint8_t> covered((size_t)n, 0);
+        auto mark_used = [&](int32_t s) {
+            for (int64_t i = 0; i < nh; ++i)
+                if (hubs[(size_t)i] == s) { used[(size_t)i] = 1; break; }
+        };
+        mark_used(s0);
+        ball2(n, off, col, s0, covered);
+        for (int j = 1; j < k; ++j) {
+            int64_t best_deg = -1;
+            int32_t best = -1;
+            for (int64_t i = 0; i < nh; ++i) {
+                if (used[(size_t)i]) continue;
+                int32_t h = hubs[(size_t)i];
+                if (covered[(size_t)h]) continue;
+                int64_t d = deg_of(off, h);
+                if (d > best_deg) { best_deg = d; best = h; }
+            }
+            if (best == -1) break;
+            seeds.push_back(best);
+            mark_used(best);
+            ball2(n, off, col, best, covered);
+        }
+    }
+    int32_t found = (int32_t)seeds.size();
+    if (found < k) {
+        std::vector<uint8_t> chosen((size_t)n, 0);
+        for (int32_t s : seeds) chosen[(size_t)s] = 1;
+        std::vector<int32_t> order((size_t)n);
+        for (int64_t v = 0; v < n; ++v) order[(size_t)v] = (int32_t)v;
+        std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            int64_t da = deg_of(off, a), db = deg_of(off, b);
+            if (da != db) return da > db;
+            return a < b;
+        });
+        for (size_t i = 0; i < order.size() && (int32_t)seeds.size() < k; ++i)
+            if (!chosen[(size_t)order[i]]) seeds.push_back(order[i]);
+    }
+    for (int32_t p = 0; p < k; ++p) seeds_out[p] = seeds[(size_t)p];
+    if (n_found_out) *n_found_out = found < k ? found : k;
+    if (n_fallback_out) *n_fallback_out = found < k ? k - found : 0;
+    return REF_OK;
+}
+
+// partition.cpp:323-604 at one rank.  Per round: the fill step when every
+// frontier is empty (partition.cpp:376-394); per partition the min(theta,|F|)
+// frontier members smallest in (degree, id) are expanded
+// (partition.cpp:400-431); an unlabelled neighbour is claimed by the lowest
+// partition that reaches it, because partitions are scanned in ascending
+// order against one shared visited set (partition.cpp:451-476) and at one
+// rank each vertex then has a single claimant (partition.cpp:530-538);
+// frontiers become carry + newly labelled (partition.cpp:588-597); sizes are
+// recounted (partition.cpp:586).
+int ref_expand(int64_t n, const int64_t *off, const int32_t *col, int32_t k, int32_t theta,
+               const int32_t *seeds, int64_t *rounds_out, int32_t *fill_round_out,
+               int32_t *labels_out) {
+    if (n <= 0 || k <= 0 || theta <= 0 || !seeds || !labels_out) return REF_E_ARG;
+    int32_t *L = labels_out;
+    for (int64_t v = 0; v < n; ++v) L[v] = -1;
+    std::vector<std::vector<uint64_t>> F((size_t)k);  // keys (deg << 32) | id
+    auto key = [&](int32_t v) { return ((uint64_t)deg_of(off, v) << 32) | (uint32_t)v; };
+    for (int32_t p = 0; p < k; ++p) {
+        int32_t s = seeds[p];
+        if (s < 0 || s >= n) return REF_E_ARG;
+        L[s] = p;
+        F[(size_t)p].push_back(key(s));
+    }
+    std::vector<int64_t> size((size_t)k, 0);
+    int64_t labelled = 0;
+    for (int64_t v = 0; v < n; ++v)
+        if (L[v] >= 0) { ++size[(size_t)L[v]]; ++labelled; }
+    std::vector<uint8_t> visited((size_t)n, 0);
+    std::vector<std::vector<int32_t>> fresh((size_t)k);
+    int64_t round = 0;
+    int32_t fill_round = -1;
+    while (labelled < n) {
+        bool all_empty = true;
+        for (int32_t p = 0; p < k; ++p)
+            if (!F[(size_t)p].empty()) { all_empty = false; break; }
+        if (all_empty) {
+            int32_t b = 0;
+            for (int32_t p = 0; p < k; ++p)
+                if (size[(size_t)p] < size[(size_t)b] || (size[(size_t)p] == size[(size_t)b] && p < b)) b = p;
+            for (int64_t v = 0; v < n; ++v)
+                if (L[v] < 0) { L[v] = b; F[(size_t)b].push_back(key((int32_t)v)); }
+            if (fill_round < 0) fill_round = (int32_t)round;
+        }
+        // Selection: the theta smallest keys of each frontier.
+        std::vector<std::vector<uint64_t>> sel((size_t)k);
+        for (int32_t p = 0; p < k; ++p) {
+            auto &f = F[(size_t)p];
+            if ((int64_t)f.size() > theta) {
+                std::nth_element(f.begin(), f.begin() + theta, f.end());
+                sel[(size_t)p].assign(f.begin(), f.begin() + theta);
+                f.erase(f.begin(), f.begin() + theta);
+            } else {
+                sel[(size_t)p].swap(f);
+                f.clear();
+            }
+        }
+        // Claims: ascending partition order, shared visited set; the label
+        // test reads labels as they were before this round's claims.
+        std::vector<int32_t> touched;
+        for (int32_t p = 0; p < k; ++p) {
+            for (uint64_t kk : sel[(size_t)p]) {
+                int32_t u = (int32_t)(kk & 0xffffffffu);
+                for (int64_t e = off[u]; e < off[u + 1]; ++e) {
+                    int32_t v = col[e];
+                    if (L[v] == -1 && !visited[(size_t)v]) {
+                        visited[(size_t)v] = 1;
+                        touched.push_back(v);
+                        fresh[(size_t)p].push_back(v);
+                    }
+                }
+            }
+        }
+        for (int32_t v : touched) visited[(size_t)v] = 0;
+        for (int32_t p = 0; p < k; ++p) {
+            for (int32_t v : fresh[(size_t)p]) {
+                L[v] = p;
+                F[(size_t)p].push_back(key(v));
+            }
+            fresh[(size_t)p].clear();
+        }
+        std::fill(size.begin(), size.end(), 0);
+        labelled = 0;
+        for (int64_t v = 0; v < n; ++v)
+            if (L[v] >= 0) { ++size[(size_t)L[v]]; ++labelled; }
+        ++round;
+    }
+    if (rounds_out) *rounds_out = round;
+    if (fill_round_out) *fill_round_out = fill_round;
+    return REF_OK;
+}
+
+// phase1.cpp:268-339 at one rank.
+int ref_phase1_metrics(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
+                       const int32_t *labels, int64_t *vc, int64_t *ec, ref_p1_metrics *m) {
+    if (n <= 0 || k <= 0 || !labels || !m) return REF_E_ARG;
+    std::vector<int64_t> cv((size_t)k, 0), ce((size_t)k, 0);
+    int64_t cut2 = 0;
+    for (int64_t u = 0; u < n; ++u) {
+        int32_t lu = labels[u];
+        if (lu >= 0) ++cv[(size_t)lu];
+        for (int64_t e = off[u]; e < off[u + 1]; ++e) {
+            int32_t lv = labels[col[e]];
+            if (lu != -1 && lv != -1 && lu != lv) ++cut2;
+            if (lu >= 0 && lv == lu) ++ce[(size_t)lu];
+        }
+    }
+    for (int32_t p = 0; p < k; ++p) ce[(size_t)p] /= 2;
+    m->initial_edge_cut = cut2 / 2;
+    double expected_vertices = static_cast<double>(n) / k;
+    double max_vr = 0;
+    for (int32_t p = 0; p < k; ++p) max_vr = std::max(max_vr, (double)cv[(size_t)p] / expected_vertices);
+    m->initial_vertex_balance = max_vr;
+    double total_pe = 0;
+    for (int32_t p = 0; p < k; ++p) total_pe += (double)ce[(size_t)p];
+    double expected_edges = total_pe / k;
+    double max_er = 0;
+    for (int32_t p = 0; p < k; ++p) max_er = std::max(max_er, (double)ce[(size_t)p] / expected_edges);
+    m->initial_edge_balance = max_er;
+    m->total_vertices = n;
+    m->total_edges = off[n] / 2;
+    if (vc) std::copy(cv.begin(), cv.end(), vc);
+    if (ec) std::copy(ce.begin(), ce.end(), ec);
+    return REF_OK;
+}
+
+// phase1.cpp:18-161 (load excluded: the graph arrives as CSR).
+int ref_phase1(int64_t n, const int64_t *off, const int32_t *col, int32_t k, int32_t theta,
+               int32_t *labels_out, int32_t *seeds_out, int64_t *vc, int64_t *ec,
+               ref_p1_metrics *m) {
+    if (n <= 0 || k <= 0 || theta <= 0 || !labels_out || !m) return REF_E_ARG;
+    std::memset(m, 0, sizeof(*m));
+    std::vector<int32_t> seeds((size_t)k);
+    int32_t found = 0, fb = 0;
+    int rc = ref_seeds(n, off, col, k, seeds.data(), &found, &fb);
+    if (rc != REF_OK) return rc;
+    int64_t rounds = 0;
+    int32_t fill_round = -1;
+    rc = ref_expand(n, off, col, k, theta, seeds.data(), &rounds, &fill_round, labels_out);
+    if (rc != REF_OK) return rc;
+    rc = ref_phase1_metrics(n, off, col, k, labels_out, vc, ec, m);
+    if (rc != REF_OK) return rc;
+    int32_t s0, d0;
+    ref_first_seed(n, off, &s0, &d0);
+    int64_t nh = 0;
+    ref_hubs(n, off, nullptr, &nh);
+    m->first_seed = s0;
+    m->first_seed_degree = d0;
+    m->n_hubs = nh;
+    m->n_seeds_found = found;
+    m->n_seeds_fallback = fb;
+    m->rounds = rounds;
+    m->fill_round = fill_round;
+    if (seeds_out) std::copy(seeds.begin(), seeds.end(), seeds_out);
+    return REF_OK;
+}
+
+// phase2.cpp:55-117 (vertex and intra-partition counts, intra entries / 2
+// per partition) and phase2.cpp:214-253 (cut = differing entries / 2; abort
+// on a -1 label or an odd count).
+int ref_stats(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
+              const int32_t *labels, int64_t *vc, int64_t *ic, int64_t *cut_out) {
+    if (n <= 0 || k <= 0 || !labels) return REF_E_ARG;
+    std::vector<int64_t> cv((size_t)k, 0), ce((size_t)k, 0);
+    int64_t cut2 = 0, invalid = 0;
+    for (int64_t u = 0; u < n; ++u) {
+        int32_t lu = labels[u];
+        bool lu_ok = lu >= 0 && lu < k;
+        if (lu_ok) ++cv[(size_t)lu];
+        for (int64_t e = off[u]; e < off[u + 1]; ++e) {
+            int32_t lv = labels[col[e]];
+            if (lu_ok && lv == lu) ++ce[(size_t)lu];
+            if (lu == -1 || lv == -1) { ++invalid; continue; }
+            if (lu != lv) ++cut2;
+        }
+    }
+    if (invalid) return REF_E_INVALID_LABEL;
+    if (cut2 & 1) return REF_E_ODD_CUT;
+    for (int32_t p = 0; p < k; ++p) ce[(size_t)p] /= 2;
+    if (vc) std::copy(cv.begin(), cv.end(), vc);
+    if (ic) std::copy(ce.begin(), ce.end(), ic);
+    if (cut_out) *cut_out = cut2 >> 1;
+    return REF_OK;
+}
+
+// phase2.cpp:158-209 and phase2.cpp:112-115 (expected counts).  Source
+// order, no contraction; glibc log.
+void ref_penalties(int32_t k, const int64_t *vc, const int64_t *ic, int32_t t,
+                   double *pen, double *re) {
+    int64_t tv = 0, te = 0;
+    for (int32_t i = 0; i < k; ++i) { tv += vc[i]; te += ic[i]; }
+    double exp_v = static_cast<double>(tv) / k;
+    double exp_e = (te > 0) ? static_cast<double>(te) / k : 1.0;
+    std::vector<double> RV((size_t)k), RE((size_t)k);
+    for (int32_t i = 0; i < k; ++i) {
+        RV[(size_t)i] = (exp_v > 0) ? static_cast<double>(vc[i]) / exp_v : 1.0;
+        RE[(size_t)i] = (exp_e > 0) ? static_cast<double>(ic[i]) / exp_e : 1.0;
+    }
+    double rv_mean = 0.0, re_mean = 0.0;
+    for (int32_t i = 0; i < k; ++i) { rv_mean += RV[(size_t)i]; re_mean += RE[(size_t)i]; }
+    rv_mean /= k;
+    re_mean /= k;
+    double rv_var = 0.0, re_var = 0.0;
+    for (int32_t i = 0; i < k; ++i) {
+        rv_var += (RV[(size_t)i] - rv_mean) * (RV[(size_t)i] - rv_mean);
+        re_var += (RE[(size_t)i] - re_mean) * (RE[(size_t)i] - re_mean);
+    }
+    rv_var /= k;
+    re_var /= k;
+    double total_var = rv_var + re_var;
+    double imb_rv = (total_var > 0) ? rv_var / total_var : 0.0;
+    double imb_re = (total_var > 0) ? re_var / total_var : 0.0;
+    for (int32_t i = 0; i < k; ++i) {
+        double g_rv = (1.0 - RV[(size_t)i]) * (1.0 + log((double)t));
+        double g_re = (1.0 - RE[(size_t)i]) * (1.0 + log((double)t));
+        if (pen) pen[i] = imb_rv * g_rv + imb_re * g_re;
+        if (re) re[i] = RE[(size_t)i];
+    }
+}
+
+namespace {
+
+struct SelectCounts {
+    int64_t prev_deg = 0, cand = 0, scan = 0, bd = 0;
+};
+
+// gpu_lp_boundary.cu:185-305.
+void select_boundary(int64_t n, const int64_t *off, const int32_t *col, const int32_t *L,
+                     const std::vector<int32_t> &prev, const double *re, int32_t k, int32_t t,
+                     std::vector<int32_t> &out, std::vector<uint8_t> &mark, SelectCounts &sc) {
+    // Candidates: previous boundary and its neighbours, unique
+    // (gpu_lp_boundary.cu:207-240).
+    std::vector<int32_t> cand;
+    for (int32_t v : prev) {
+        if (v < 0 || v >= n) continue;
+        sc.prev_deg += deg_of(off, v);
+        if (!mark[(size_t)v]) { mark[(size_t)v] = 1; cand.push_back(v); }
+        for (int64_t e = off[v]; e < off[v + 1]; ++e) {
+            int32_t u = col[e];
+            if (!mark[(size_t)u]) { mark[(size_t)u] = 1; cand.push_back(u); }
+        }
+    }
+    for (int32_t v : cand) mark[(size_t)v] = 0;
+    std::sort(cand.begin(), cand.end());
+    sc.cand = (int64_t)cand.size();
+    // Boundary test with early exit (gpu_lp_boundary.cu:251-264), grouped by
+    // label (gpu_lp_boundary.cu:266-269).
+    std::vector<std::vector<std::pair<int32_t, int64_t>>> groups((size_t)k);
+    for (int32_t x : cand) {
+        int32_t lx = L[x];
+        bool bd = false;
+        int64_t scanned = 0;
+        for (int64_t e = off[x]; e < off[x + 1]; ++e) {
+            ++scanned;
+            if (L[col[e]] != lx) { bd = true; break; }
+        }
+        sc.scan += scanned;
+        if (bd) {
+            groups[(size_t)lx].emplace_back(x, deg_of(off, x));
+            ++sc.bd;
+        }
+    }
+    // Per label, the first ceil(ratio * size) by (degree desc, id) when
+    // RE > 1.0, else (degree asc, id) (gpu_lp_boundary.cu:244,277-302).
+    double ratio = exp((-t) / (5.0));
+    out.clear();
+    for (int32_t p = 0; p < k; ++p) {
+        auto &g = groups[(size_t)p];
+        if (g.empty()) continue;
+        if (re[p] > 1.0) {
+            std::sort(g.begin(), g.end(), [](const auto &a, const auto &b) {
+                if (a.second != b.second) return a.second > b.second;
+                return a.first < b.first;
+            });
+        } else {
+            std::sort(g.begin(), g.end(), [](const auto &a, const auto &b) {
+                if (a.second != b.second) return a.second < b.second;
+                return a.first < b.first;
+            });
+        }
+        int take = static_cast<int>(std::ceil(ratio * static_cast<double>(g.size())));
+        if (take > (int)g.size()) take = (int)g.size();
+        for (int i = 0; i < take; ++i) out.push_back(g[(size_t)i].first);
+    }
+    std::sort(out.begin(), out.end());
+}
+
+// gpu_lp_boundary.cu:354-426 for one boundary vertex: histogram over labels
+// below min(k,32); score = count * (1 + pen) for non-zero counts; the
+// shuffle-tree argmax prefers the smallest 5-bit bit-reversed label among
+// equal scores; the winner must beat the raw own-label count.
+inline int32_t lp_decide(const int64_t *off, const int32_t *col, const int32_t *L, int32_t b,
+                         const double *pen, int32_t k, int64_t *hist) {
+    const int E = std::min(k, kLpLabelCap);
+    for (int l = 0; l < E; ++l) hist[l] = 0;
+    for (int64_t e = off[b]; e < off[b + 1]; ++e) {
+        int32_t l = L[col[e]];
+        if (l >= 0 && l < E) ++hist[l];
+    }
+    int32_t own = L[b];
+    double base = (own >= 0 && own < E) ? (double)hist[own] : -1e300;
+    int best_l = -1;
+    double best_s = 0.0;
+    for (int l = 0; l < E; ++l) {
+        double s = (double)hist[l];
+        if (s > 0.0) s *= (1.0 + pen[l]);
+        if (best_l < 0 || s > best_s || (s == best_s && bitrev5(l) < bitrev5(best_l))) {
+            best_l = l;
+            best_s = s;
+        }
+    }
+    return (best_s > base) ? best_l : own;
+}
+
+}  // namespace
+
+int ref_select_boundary(int64_t n, const int64_t *off, const int32_t *col,
+                        const int32_t *labels, const int32_t *prev, int64_t n_prev,
+                        const double *re, int32_t k, int32_t t, int32_t *out, int64_t *n_out) {
+    if (n <= 0 || k <= 0 || !labels || !re || !out || !n_out || n_prev < 0) return REF_E_ARG;
+    std::vector<int32_t> pv(prev, prev + n_prev), o;
+    std::vector<uint8_t> mark((size_t)n, 0);
+    SelectCounts sc;
+    select_boundary(n, off, col, labels, pv, re, k, t, o, mark, sc);
+    std::copy(o.begin(), o.end(), out);
+    *n_out = (int64_t)o.size();
+    return REF_OK;
+}
+
+int ref_lp_round(int64_t n, const int64_t *off, const int32_t *col, const int32_t *labels,
+                 const int32_t *b, int64_t nb, const double *pen, int32_t k, int32_t *labels_out) {
+    if (n <= 0 || k <= 0 || !labels || !pen || !labels_out || nb < 0) return REF_E_ARG;
+    std::vector<int64_t> hist((size_t)kLpLabelCap);
+    for (int64_t i = 0; i < nb; ++i) {
+        int32_t v = b[i];
+        if (v < 0 || v >= n) return REF_E_ARG;
+        labels_out[v] = lp_decide(off, col, labels, v, pen, k, hist.data());
+    }
+    return REF_OK;
+}
+
+// phase2.cpp:256-449 at one rank.
+int ref_phase2(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
+               int32_t max_rounds, double epsilon, int32_t *labels,
+               int64_t *vc_out, int64_t *ic_out, ref_p2_metrics *m, ref_p2_trace *tr) {
+    if (n <= 0 || k <= 0 || max_rounds < 0 || !labels || !m) return REF_E_ARG;
+    std::memset(m, 0, sizeof(*m));
+    std::vector<int64_t> vc((size_t)k), ic((size_t)k);
+    int64_t prev_cut = 0;
+    int rc = ref_stats(n, off, col, k, labels, vc.data(), ic.data(), &prev_cut);
+    if (rc != REF_OK) return rc;
+    m->initial_cut = prev_cut;
+    int cc = 0;
+    std::vector<int32_t> B((size_t)n);
+    for (int64_t v = 0; v < n; ++v) B[(size_t)v] = (int32_t)v;
+    std::vector<int32_t> nextB;
+    std::vector<uint8_t> mark((size_t)n, 0);
+    std::vector<double> pen((size_t)k), re((size_t)k);
+    std::vector<int32_t> newL(labels, labels + n);
+    std::vector<int64_t> hist((size_t)kLpLabelCap);
+    std::vector<int32_t> changed;
+    int32_t rounds = 0, reason = 0;
+    int64_t lp_edges = 0;
+    for (int32_t t = 1; t <= max_rounds; ++t) {
+        rounds = t;
+        ref_penalties(k, vc.data(), ic.data(), t, pen.data(), re.data());
+        SelectCounts sc;
+        int64_t nprev = (int64_t)B.size();
+        select_boundary(n, off, col, labels, B, re.data(), k, t, nextB, mark, sc);
+        B.swap(nextB);
+        int64_t bdeg = 0;
+        for (int32_t v : B) bdeg += deg_of(off, v);
+        lp_edges += bdeg;
+        if (tr) {
+            int i = t - 1;
+            if (tr->prev_count) tr->prev_count[i] = nprev;
+            if (tr->prev_deg) tr->prev_deg[i] = sc.prev_deg;
+            if (tr->cand_count) tr->cand_count[i] = sc.cand;
+            if (tr->scan_len) tr->scan_len[i] = sc.scan;
+            if (tr->bd_count) tr->bd_count[i] = sc.bd;
+            if (tr->b_count) tr->b_count[i] = (int64_t)B.size();
+            if (tr->b_deg) tr->b_deg[i] = bdeg;
+            if (tr->chg_count) tr->chg_count[i] = 0;
+            if (tr->chg_deg) tr->chg_deg[i] = 0;
+            if (tr->cut) tr->cut[i] = -1;
+        }
+        if (B.empty()) { reason = 1; break; }
+        // Jacobi update: every decision reads the labels of round t-1.
+        changed.clear();
+        for (int32_t b : B) {
+            int32_t nl = lp_decide(off, col, labels, b, pen.data(), k, hist.data());
+            if (nl != labels[b]) { newL[(size_t)b] = nl; changed.push_back(b); }
+        }
+        int64_t cdeg = 0;
+        for (int32_t v : changed) { labels[v] = newL[(size_t)v]; cdeg += deg_of(off, v); }
+        if (tr) {
+            if (tr->chg_count) tr->chg_count[t - 1] = (int64_t)changed.size();
+            if (tr->chg_deg) tr->chg_deg[t - 1] = cdeg;
+        }
+        if (changed.empty()) { reason = 2; break; }
+        int64_t cur = 0;
+        rc = ref_stats(n, off, col, k, labels, vc.data(), ic.data(), &cur);
+        if (rc != REF_OK) return rc;
+        if (tr && tr->cut) tr->cut[t - 1] = cur;
+        double delta = (prev_cut > 0) ? std::abs((double)(cur - prev_cut) / (double)prev_cut) : 1.0;
+        if (delta < epsilon) ++cc; else cc = 0;
+        prev_cut = cur;
+        if (cc >= kConvergeRounds) { reason = 3; break; }
+    }
+    // Final statistics (phase2.cpp:424-441).
+    int64_t dummy_cut = 0;
+    rc = ref_stats(n, off, col, k, labels, vc.data(), ic.data(), &dummy_cut);
+    if (rc != REF_OK) return rc;
+    int64_t tv = 0, te = 0;
+    for (int32_t i = 0; i < k; ++i) { tv += vc[(size_t)i]; te += ic[(size_t)i]; }
+    double exp_v = static_cast<double>(tv) / k;
+    double exp_e = (te > 0) ? static_cast<double>(te) / k : 1.0;
+    double max_v = 0.0, max_e = 0.0;
+    for (int32_t i = 0; i < k; ++i) {
+        double rv = (exp_v > 0) ? static_cast<double>(vc[(size_t)i]) / exp_v : 1.0;
+        double r_e = (exp_e > 0) ? static_cast<double>(ic[(size_t)i]) / exp_e : 1.0;
+        max_v = std::max(max_v, rv);
+        max_e = std::max(max_e, r_e);
+    }
+    m->edge_cut = prev_cut;
+    m->vertex_balance = max_v;
+    m->edge_balance = max_e;
+    m->total_vertices = tv;
+    m->total_edges = te;
+    m->rounds = rounds;
+    m->exit_reason = reason;
+    m->lp_edges = lp_edges;
+    if (vc_out) std::copy(vc.begin(), vc.end(), vc_out);
+    if (ic_out) std::copy(ic.begin(), ic.end(), ic_out);
+    return REF_OK;
+}
+
+}  // extern "C"

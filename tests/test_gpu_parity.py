@@ -1,0 +1,176 @@
+"""Parity of the HIP path with the CPU oracle (bit-exact), through the C ABI."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import graphs
+import oracle_py as O
+from dmolp_amd import api
+
+pytestmark = pytest.mark.gpu
+
+TINY = {
+    "path9": graphs.path(9),
+    "star12": graphs.star(12),
+    "grid5x7": graphs.grid(5, 7),
+    "cliques6": graphs.two_cliques(6),
+    "rand300": graphs.random_graph(300, 900, 7),
+    "plaw500": graphs.powerlaw(500, 3000, 11),
+}
+
+
+def c1():
+    return api.gen_host(api.rmat_spec(16, 16, seed=1))
+
+
+@pytest.fixture(params=["default", "heavy"])
+def tuned(ctx, request):
+    """Run each test with the default binning and with tiny heavy thresholds,
+    so the chunked heavy-row kernels are exercised on small graphs."""
+    if request.param == "heavy":
+        ctx.set_tuning(heavy_degree=5, chunk_edges=64)
+    else:
+        ctx.set_tuning(heavy_degree=1024, chunk_edges=4096)
+    yield ctx
+    ctx.set_tuning(heavy_degree=1024, chunk_edges=4096)
+
+
+@pytest.mark.parametrize("spec", [
+    api.rmat_spec(12, 16, seed=5),
+    api.rmat_spec(10, 8, a=0.45, b=0.22, c=0.22, seed=9),
+    api.chung_lu_spec(5000, 40000, gamma=2.0, dmax=800.0, seed=3),
+    api.grid_spec(13, 17),
+], ids=["rmat12", "rmat10", "chunglu", "grid"])
+def test_device_generator_matches_host(ctx, spec):
+    off_h, col_h = api.gen_host(spec)
+    g = ctx.generate(spec)
+    off_d, col_d = g.download()
+    assert np.array_equal(off_h, off_d)
+    assert np.array_equal(col_h, col_d)
+
+
+@pytest.mark.parametrize("name", list(TINY))
+def test_stats_parity(tuned, name):
+    off, col = TINY[name]
+    g = tuned.upload(off, col)
+    rng = np.random.default_rng(1)
+    for k in (1, 3, 8, 40):
+        labels = rng.integers(0, k, len(off) - 1).astype(np.int32)
+        rc, vc, ic, cut = O.stats(off, col, k, labels)
+        assert rc == 0
+        dvc, dic, dcut = g.stats(k, labels)
+        assert np.array_equal(vc, dvc) and np.array_equal(ic, dic) and cut == dcut
+
+
+def test_penalties_bit_exact(ctx):
+    rng = np.random.default_rng(3)
+    for trial in range(200):
+        k = int(rng.integers(1, 130))
+        vc = rng.integers(0, 10 ** int(rng.integers(1, 9)), k)
+        ic = rng.integers(0, 10 ** int(rng.integers(1, 9)), k)
+        if trial % 7 == 0:
+            ic[:] = 0
+        t = int(rng.integers(1, 40))
+        pen, re = O.penalties(k, vc, ic, t)
+        dpen, dre = ctx.penalties(k, vc, ic, t)
+        assert np.array_equal(pen.view(np.uint64), dpen.view(np.uint64)), trial
+        assert np.array_equal(re.view(np.uint64), dre.view(np.uint64)), trial
+
+
+@pytest.mark.parametrize("name", list(TINY))
+def test_lp_round_parity(tuned, name):
+    off, col = TINY[name]
+    n = len(off) - 1
+    g = tuned.upload(off, col)
+    rng = np.random.default_rng(5)
+    for k in (2, 5, 32, 37):
+        for trial in range(3):
+            labels = rng.integers(0, k, n).astype(np.int32)
+            b = np.sort(rng.choice(n, size=max(1, n // 2), replace=False)).astype(np.int32)
+            pen = rng.normal(0, 0.5, k)
+            if trial == 2:
+                pen[:] = 0.0
+            want = O.lp_round(off, col, labels, b, pen, k)
+            got = g.lp_round(labels, b, pen, k)
+            assert np.array_equal(want, got), (k, trial)
+
+
+@pytest.mark.parametrize("name", list(TINY))
+def test_select_boundary_parity(tuned, name):
+    off, col = TINY[name]
+    n = len(off) - 1
+    g = tuned.upload(off, col)
+    rng = np.random.default_rng(9)
+    for k in (2, 4, 9):
+        for t in (1, 2, 5, 13):
+            labels = rng.integers(0, k, n).astype(np.int32)
+            prev = np.sort(rng.choice(n, size=max(1, n // 3), replace=False)).astype(np.int32)
+            re = rng.uniform(0.5, 1.5, k)
+            want = O.select_boundary(off, col, labels, prev, re, k, t)
+            got = g.select_boundary(labels, prev, re, k, t)
+            assert np.array_equal(want, got), (k, t)
+
+
+def _check_phase2(g, off, col, k, labels0, rounds, eps):
+    want = O.phase2(off, col, k, labels0, rounds, eps)
+    g.set_labels(labels0)
+    got = g.phase2(k, rounds, eps)
+    assert np.array_equal(want.labels, got.labels)
+    assert want.m.edge_cut == got.metrics.edge_cut
+    assert want.m.rounds == got.metrics.rounds
+    assert want.m.exit_reason == got.metrics.exit_reason
+    assert want.m.lp_edges == got.metrics.lp_edges
+    assert want.m.initial_cut == got.metrics.initial_cut
+    assert np.array_equal(want.vc, got.vc) and np.array_equal(want.ic, got.ic)
+    assert want.m.vertex_balance == got.metrics.vertex_balance
+    assert want.m.edge_balance == got.metrics.edge_balance
+    for f in ("prev_count", "prev_deg", "cand_count", "bd_count", "b_count", "b_deg",
+              "chg_count", "chg_deg", "cut"):
+        assert np.array_equal(want.trace[f], got.trace[f]), f
+    return want
+
+
+@pytest.mark.parametrize("name", list(TINY))
+def test_phase2_parity_tiny(tuned, name):
+    off, col = TINY[name]
+    g = tuned.upload(off, col)
+    rng = np.random.default_rng(13)
+    for k in (2, 3, 8):
+        labels0 = rng.integers(0, k, len(off) - 1).astype(np.int32)
+        _check_phase2(g, off, col, k, labels0, 12, 0.03)
+
+
+@pytest.mark.parametrize("name", list(TINY))
+def test_phase1_parity_tiny(tuned, name):
+    off, col = TINY[name]
+    g = tuned.upload(off, col)
+    n = len(off) - 1
+    for k in (1, 2, 4):
+        for theta in (1, 2, 10000):
+            if k > n:
+                continue
+            want = O.phase1(off, col, k, theta)
+            got = g.phase1(k, theta)
+            assert np.array_equal(want.seeds, got.seeds), (k, theta)
+            assert np.array_equal(want.labels, got.labels), (k, theta)
+            assert want.m.rounds == got.metrics.rounds
+            assert want.m.fill_round == got.metrics.fill_round
+            assert want.m.initial_edge_cut == got.metrics.initial_edge_cut
+            assert np.array_equal(want.vc, got.vc) and np.array_equal(want.ec, got.ec)
+            assert want.m.n_seeds_found == got.metrics.n_seeds_found
+            assert want.m.n_hubs == got.metrics.n_hubs
+
+
+def test_config1_end_to_end(tuned):
+    """R-MAT scale 16, edge factor 16, seed 1; k=8, theta=10000, 10 rounds."""
+    off, col = c1()
+    g = tuned.upload(off, col)
+    want1 = O.phase1(off, col, 8, 10000)
+    got1 = g.phase1(8, 10000)
+    assert np.array_equal(want1.labels, got1.labels)
+    assert np.array_equal(want1.seeds, got1.seeds)
+    assert want1.m.initial_edge_cut == got1.metrics.initial_edge_cut
+    assert want1.m.initial_vertex_balance == got1.metrics.initial_vertex_balance
+    assert want1.m.initial_edge_balance == got1.metrics.initial_edge_balance
+    _check_phase2(g, off, col, 8, want1.labels, 10, 0.03)

@@ -1,0 +1,247 @@
+"""Pinning the oracle's phase-1 functions against checks that do not reuse
+its formulas: libstdc++ container order, closed forms, networkx distances,
+hand-derived traces and invariants."""
+from __future__ import annotations
+
+from fractions import Fraction
+
+import networkx as nx
+import numpy as np
+import pytest
+
+import graphs
+import oracle_py as O
+
+
+@pytest.mark.parametrize("n", [5, 13, 1000, 65536, 100003])
+def test_degree_map_iterates_descending(n):
+    # init.cpp:43-44,68,90-93 and 102-136 at one rank: the degree map the
+    # reference builds iterates in descending id (the basis of A.1 and A.2).
+    assert np.array_equal(O.degree_map_order(n), np.arange(n - 1, -1, -1, dtype=np.int32))
+
+
+def closed_form_first_seed(off):
+    deg = np.diff(off)
+    D = int(deg.max())
+    tied = np.flatnonzero(deg == D)
+    below = tied[tied < D]
+    return (int(below.min()) if len(below) else int(tied.max())), D
+
+
+def test_first_seed_closed_form():
+    cases = [graphs.star(9), graphs.path(6), graphs.grid(4, 5), graphs.random_graph(200, 500, 3)]
+    # Max-degree vertices at ids above D: centres 50 and 60 with 3 leaves each.
+    e = [(50, x) for x in (1, 2, 3)] + [(60, x) for x in (4, 5, 6)]
+    cases.append(graphs.from_edges(70, e))
+    # Tie with one id below D and one above: degree 5 at ids 1 and 60.
+    e = [(1, x) for x in (10, 11, 12, 13, 14)] + [(60, x) for x in (20, 21, 22, 23, 24)]
+    cases.append(graphs.from_edges(70, e))
+    for off, col in cases:
+        assert O.first_seed(off) == closed_form_first_seed(off)
+    off, _ = cases[-2]
+    assert O.first_seed(off) == (60, 3)
+    off, _ = cases[-1]
+    assert O.first_seed(off) == (1, 5)
+
+
+def exact_hub_count(deg):
+    N = len(deg)
+    total = int(deg.sum())
+    dmax = int(deg.max())
+    s2 = int(np.sort(deg)[-2]) if N >= 2 else 0
+    # Lorenz tangent through the last two points: x_i = 1 - y1 / slope with
+    # y1 = 1 and slope = (dmax / total) / (1 / N).
+    xi = 1 - Fraction(total, N * dmax)
+    cnt = sum(1 for i in range(N) if Fraction(i + 1, N) > xi)
+    return cnt, xi, s2
+
+
+@pytest.mark.parametrize("name,g", [
+    ("cycle", graphs.from_edges(40, [(i, (i + 1) % 40) for i in range(40)])),
+    ("star", graphs.star(30)),
+    ("grid", graphs.grid(6, 7)),
+    ("rand", graphs.random_graph(500, 2000, 1)),
+    ("plaw", graphs.powerlaw(2000, 8000, 2)),
+])
+def test_hubs_count_and_order(name, g):
+    off, _ = g
+    deg = np.diff(off)
+    h = O.hubs(off)
+    cnt, xi, _ = exact_hub_count(deg)
+    # Floating evaluation may differ from the exact tangent only at a tie of
+    # (i+1)/N with x_i.
+    near = any(abs(Fraction(i + 1, len(deg)) - xi) < Fraction(1, 10 ** 9) for i in range(len(deg)))
+    assert len(h) == cnt or (near and abs(len(h) - cnt) <= 1)
+    # Hubs are the top-degree vertices in non-increasing degree order.
+    hd = deg[h]
+    assert np.all(hd[:-1] >= hd[1:])
+    assert np.array_equal(np.sort(hd)[::-1], np.sort(deg)[::-1][:len(h)])
+    assert len(set(h.tolist())) == len(h)
+
+
+def test_hubs_special_cases():
+    # Constant degree: the Lorenz curve is the diagonal, every vertex is a hub.
+    off, _ = graphs.from_edges(12, [(i, (i + 1) % 12) for i in range(12)])
+    assert len(O.hubs(off)) == 12
+    # Star: the centre first, then exactly one leaf.
+    off, _ = graphs.star(10)
+    h = O.hubs(off)
+    assert len(h) == 2 and h[0] == 0 and 1 <= h[1] <= 9
+    # No edges: total degree 0 gives NaN, no hubs; a single vertex has none.
+    off = np.zeros(6, np.int64)
+    assert len(O.hubs(off)) == 0
+    off = np.zeros(2, np.int64)
+    assert len(O.hubs(off)) == 0
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_ball2_matches_networkx(seed):
+    off, col = graphs.random_graph(300, 500, seed)
+    G = graphs.to_nx(off, col)
+    for s in (0, 17, 123, 299):
+        want = np.zeros(300, bool)
+        for v in nx.single_source_shortest_path_length(G, s, cutoff=2):
+            want[v] = True
+        assert np.array_equal(O.ball2(off, col, s), want)
+
+
+def brute_seeds(off, col, k):
+    """seed.cpp:192-242 by brute force with networkx balls."""
+    G = graphs.to_nx(off, col)
+    deg = np.diff(off)
+    hubs = O.hubs(off).tolist()
+    s0, _ = O.first_seed(off)
+    seeds = [s0]
+    used = {hubs.index(s0)} if s0 in hubs else set()
+    if hubs:
+        for _ in range(1, k):
+            cov = set()
+            for s in seeds:
+                cov |= set(nx.single_source_shortest_path_length(G, s, cutoff=2))
+            best, bi = -1, None
+            for i, h in enumerate(hubs):
+                if i in used or h in cov:
+                    continue
+                if deg[h] > best:
+                    best, bi = deg[h], i
+            if bi is None:
+                break
+            seeds.append(hubs[bi])
+            used.add(bi)
+    found = len(seeds)
+    rest = sorted((v for v in range(len(deg)) if v not in seeds), key=lambda v: (-deg[v], v))
+    seeds += rest[:k - found]
+    return seeds, min(found, k)
+
+
+@pytest.mark.parametrize("name,g,k", [
+    ("grid", graphs.grid(12, 12), 4),
+    ("rand", graphs.random_graph(400, 600, 5), 6),
+    ("plaw", graphs.powerlaw(600, 1500, 7), 5),
+    ("cliques", graphs.two_cliques(8), 3),
+    ("star", graphs.star(10), 3),
+])
+def test_seeds_brute_force(name, g, k):
+    off, col = g
+    seeds, found, fb = O.seeds(off, col, k)
+    want, wfound = brute_seeds(off, col, k)
+    assert seeds.tolist() == want
+    assert found == wfound and fb == k - wfound
+
+
+def test_seed_exhaustion_fallback():
+    # The centre's ball covers every vertex, so the hub search stops after
+    # one seed; the documented fallback takes (degree desc, id asc).
+    off, col = graphs.star(10)
+    seeds, found, fb = O.seeds(off, col, 3)
+    assert seeds.tolist() == [0, 1, 2] and found == 1 and fb == 2
+
+
+def test_expand_hand_traces():
+    off, col = graphs.path(5)
+    lab, rounds, fill = O.expand(off, col, 2, 1, [0, 4])
+    assert lab.tolist() == [0, 0, 0, 1, 1] and rounds == 2 and fill == -1
+    off, col = graphs.star(7)
+    lab, rounds, _ = O.expand(off, col, 2, 1, [0, 1])
+    assert lab.tolist() == [0, 1, 0, 0, 0, 0, 0] and rounds == 1
+    # 3x3 grid, seeds at opposite corners, theta = 1 (derived by hand: the
+    # lowest (degree, id) frontier member expands; partition 0 claims first).
+    off, col = graphs.grid(3, 3)
+    lab, rounds, _ = O.expand(off, col, 2, 1, [0, 8])
+    assert lab.tolist() == [0, 0, 0, 0, 0, 1, 1, 1, 1] and rounds == 3
+    # theta = 2 on the same grid: both frontier members expand each round.
+    lab, rounds, _ = O.expand(off, col, 2, 2, [0, 8])
+    assert lab.tolist() == [0, 0, 0, 0, 0, 1, 0, 1, 1] and rounds == 2
+
+
+def lowest_p_bfs(G, seeds, n):
+    """theta = infinity: v takes the lowest label among its neighbours one
+    BFS level closer to the seed set."""
+    dist = {}
+    for p, s in enumerate(seeds):
+        dist[s] = 0
+    lab = np.full(n, -1)
+    for p, s in enumerate(seeds):
+        lab[s] = p
+    frontier = list(seeds)
+    d = 0
+    while frontier:
+        nxt = {}
+        for u in frontier:
+            for v in G[u]:
+                if lab[v] == -1 and (v not in nxt or nxt[v] > lab[u]):
+                    nxt[v] = lab[u]
+        for v, p in nxt.items():
+            lab[v] = p
+        frontier = list(nxt)
+        d += 1
+    return lab
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_expand_theta_infinite_is_lowest_p_bfs(seed):
+    off, col = graphs.random_graph(500, 900, seed)
+    G = graphs.to_nx(off, col)
+    n = 500
+    rng = np.random.default_rng(seed)
+    seeds = rng.choice(n, 4, replace=False).tolist()
+    lab, _, fill = O.expand(off, col, 4, 10 ** 9, seeds)
+    want = lowest_p_bfs(G, seeds, n)
+    reach = want >= 0
+    assert np.array_equal(lab[reach], want[reach])
+    if (~reach).any():
+        # Unreachable vertices all go to the smallest partition (ties: lowest p).
+        sizes = np.bincount(want[reach], minlength=4)
+        b = int(np.argmin(sizes))
+        assert np.all(lab[~reach] == b) and fill >= 0
+
+
+@pytest.mark.parametrize("theta", [1, 3, 50])
+def test_expand_invariants(theta):
+    off, col = graphs.powerlaw(800, 2500, 9)
+    seeds, _, _ = O.seeds(off, col, 6)
+    lab, rounds, _ = O.expand(off, col, 6, theta, seeds)
+    assert lab.min() >= 0 and lab.max() < 6
+    assert all(lab[s] == p for p, s in enumerate(seeds))
+    # A claimed vertex was reached through an edge from its own partition; a
+    # filled vertex shares the fill label with its whole component.  Either
+    # way every non-seed vertex with an edge has a same-label neighbour.
+    deg = np.diff(off)
+    for v in range(800):
+        if v in seeds or deg[v] == 0:
+            continue
+        nb = col[off[v]:off[v + 1]]
+        assert (lab[nb] == lab[v]).any(), v
+
+
+def test_phase1_metrics_recount():
+    off, col = graphs.powerlaw(700, 3000, 4)
+    r = O.phase1(off, col, 5, 20)
+    lab = r.labels
+    src = np.repeat(np.arange(700), np.diff(off))
+    diff = lab[src] != lab[col]
+    assert r.m.initial_edge_cut == diff.sum() // 2
+    assert np.array_equal(r.vc, np.bincount(lab, minlength=5))
+    same = ~diff
+    assert np.array_equal(r.ec, np.bincount(lab[src][same], minlength=5) // 2)
+    assert r.m.initial_vertex_balance == pytest.approx((r.vc / (700 / 5)).max(), rel=1e-15)
