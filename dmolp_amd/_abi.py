@@ -17,7 +17,7 @@ P = C.POINTER
 
 class Tuning(C.Structure):
     _fields_ = [("heavy_degree", i32), ("chunk_edges", i32), ("block_threads", i32),
-                ("grid_blocks", i32)]
+                ("grid_blocks", i32), ("pull_per_mille", i32)]
 
 
 class GenSpec(C.Structure):
@@ -89,6 +89,7 @@ SIGNATURES = {
     "dmolp_graph_get_labels": (C.c_int, [VP, VP, P(i32)]),
     "dmolp_phase1": (C.c_int, [VP, VP, i32, i32, P(i32), P(i32), P(i64), P(i64),
                                P(Phase1Metrics)]),
+    "dmolp_phase1_random": (C.c_int, [VP, VP, i32, P(i32), P(i64), P(i64), P(Phase1Metrics)]),
     "dmolp_phase2": (C.c_int, [VP, VP, i32, i32, f64, P(i64), P(i64), P(Phase2Metrics),
                                P(Trace)]),
     "dmolp_stats": (C.c_int, [VP, VP, i32, P(i32), P(i64), P(i64), P(i64)]),

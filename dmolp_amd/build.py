@@ -29,7 +29,9 @@ SOURCES = [
     "batch.hip",
     "gen_host.cpp",
     "textio.cpp",
+    "driver/driver.cpp",
 ]
+CLI = PKG / "hpc_partitioning"
 
 ARCH = os.environ.get("DMOLP_OFFLOAD_ARCH", "gfx950")
 
@@ -56,13 +58,13 @@ def hipcc() -> str:
 
 
 def _obj(src: str) -> Path:
-    return BUILD / (src.replace(".", "_") + ".o")
+    return BUILD / (src.replace(".", "_").replace("/", "_") + ".o")
 
 
 def _deps_hash(src: Path) -> str:
     h = hashlib.sha256()
     h.update(src.read_bytes())
-    for hdr in sorted(CSRC.glob("*.hpp")) + sorted((ROOT / "include").glob("*.h")):
+    for hdr in sorted(CSRC.glob("*.hpp")) + sorted((ROOT / "include").rglob("*.h")):
         h.update(hdr.read_bytes())
     h.update(" ".join(COMMON + HIP_ONLY).encode())
     return h.hexdigest()
@@ -96,6 +98,13 @@ def build(jobs: int | None = None, force: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # The reference's command line over the drop-in run_phase1/run_phase2.
+    cmd = [hipcc()] + COMMON + ["-I", str(ROOT / "include"), str(CSRC / "driver" / "main.cpp"),
+                                "-o", str(CLI), "-L", str(PKG), "-ldmolp_hip",
+                                "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return LIB
 
 

@@ -240,17 +240,20 @@ int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out) {
     out->chunk_edges = ctx->tune.chunk_edges;
     out->block_threads = ctx->tune.block_threads;
     out->grid_blocks = ctx->grid_blocks();
+    out->pull_per_mille = (int32_t)(ctx->tune.pull_fraction * 1000.0 + 0.5);
     return DMOLP_OK;
 }
 
 int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
     if (!ctx || !t) return fail(ctx, DMOLP_E_ARG, "null argument");
     if (t->heavy_degree < 0 || t->heavy_degree > (1 << 20) || t->chunk_edges < 0 ||
-        t->grid_blocks < 0 || (t->block_threads != 0 && t->block_threads != 256))
+        t->grid_blocks < 0 || t->pull_per_mille < 0 || t->pull_per_mille > 1000000 ||
+        (t->block_threads != 0 && t->block_threads != 256))
         return fail(ctx, DMOLP_E_ARG, "tuning value out of range");
     if (t->heavy_degree) ctx->tune.heavy_degree = t->heavy_degree;
     if (t->chunk_edges) ctx->tune.chunk_edges = t->chunk_edges < 64 ? 64 : t->chunk_edges;
     if (t->grid_blocks) ctx->tune.grid_blocks = t->grid_blocks;
+    if (t->pull_per_mille > 0) ctx->tune.pull_fraction = t->pull_per_mille / 1000.0;
     return DMOLP_OK;
 }
 
@@ -419,6 +422,41 @@ int dmolp_phase1(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t theta, int32
     });
 }
 
+int dmolp_phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t *labels_out,
+                        int64_t *vc, int64_t *ec, dmolp_phase1_metrics *m) {
+    if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
+    return guarded(ctx, [&] {
+        DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+        phase1_random(ctx, g, k);
+        DMOLP_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+        std::vector<int64_t> cv, ce;
+        int64_t cut = 0;
+        const int err = stats_sweep(ctx, g, k, cv, ce, cut);
+        DMOLP_REQUIRE(err == 0, err, "labels failed the stats sweep");
+        float ms = 0.f;
+        DMOLP_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        dmolp_phase1_metrics mm{};
+        mm.initial_edge_cut = cut;
+        const double expected_vertices = static_cast<double>(g->n) / k;
+        double max_vr = 0, total_pe = 0, max_er = 0;
+        for (int p = 0; p < k; ++p) max_vr = std::max(max_vr, (double)cv[(size_t)p] / expected_vertices);
+        for (int p = 0; p < k; ++p) total_pe += (double)ce[(size_t)p];
+        const double expected_edges = total_pe / k;
+        for (int p = 0; p < k; ++p) max_er = std::max(max_er, (double)ce[(size_t)p] / expected_edges);
+        mm.initial_vertex_balance = max_vr;
+        mm.initial_edge_balance = max_er;
+        mm.total_vertices = g->n;
+        mm.total_edges = g->nnz / 2;
+        mm.fill_round = -1;
+        mm.device_ms = ms;
+        if (m) *m = mm;
+        if (vc) std::copy(cv.begin(), cv.end(), vc);
+        if (ec) std::copy(ce.begin(), ce.end(), ec);
+        if (labels_out) get_labels(ctx, g, labels_out);
+    });
+}
+
 int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, double epsilon,
                  int64_t *vc, int64_t *ic, dmolp_phase2_metrics *m, dmolp_trace *trace) {
     if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
@@ -435,12 +473,13 @@ int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, 
         ctx->ev_used = 0;
         ctx->prof_marks.clear();
         ctx->prof_ms.clear();
+        phase2_round_table(ctx, w, max_rounds);
         DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, s));
-        phase2_init_enqueue(ctx, g, w);
+        phase2_init_enqueue(ctx, g, w, max_rounds, epsilon);
         Ctl c{};
         const int batch = 8;
         for (int t = 1; t <= max_rounds; ++t) {
-            phase2_round_enqueue(ctx, g, w, t, max_rounds, epsilon);
+            phase2_round_launch(ctx, g, w);
             if (t % batch == 0 && t < max_rounds) {
                 DMOLP_HIP_CHECK(hipMemcpyAsync(&c, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
                 DMOLP_HIP_CHECK(hipStreamSynchronize(s));
@@ -567,11 +606,11 @@ int dmolp_select_boundary(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels,
         DMOLP_HIP_CHECK(hipMemcpyAsync(&c, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
         DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         DMOLP_REQUIRE(c.err == 0, c.err, "labels failed the stats sweep");
-        upload_list(ctx, g, prev, n_prev, w.B[0]);
+        upload_list(ctx, g, prev, n_prev, w.B);
         DMOLP_HIP_CHECK(hipMemcpyAsync(w.re.p, re, sizeof(double) * (size_t)k, hipMemcpyHostToDevice,
                                        ctx->stream));
         phase2_select_enqueue(ctx, g, w, t);
-        const std::vector<int32_t> b = download_list(ctx, w.B[1]);
+        const std::vector<int32_t> b = download_list(ctx, w.B);
         std::copy(b.begin(), b.end(), out);
         *n_out = (int64_t)b.size();
     });
@@ -590,7 +629,7 @@ int dmolp_lp_round(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels, const 
         DMOLP_REQUIRE(std::adjacent_find(ids.begin(), ids.end()) == ids.end(), DMOLP_E_ARG,
                       "boundary list has duplicates");
         DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(Ctl), ctx->stream));
-        upload_list(ctx, g, ids.data(), nb, w.B[1]);
+        upload_list(ctx, g, ids.data(), nb, w.B);
         DMOLP_HIP_CHECK(hipMemcpyAsync(w.pen.p, pen, sizeof(double) * (size_t)k,
                                        hipMemcpyHostToDevice, ctx->stream));
         phase2_lp_enqueue(ctx, g, w);

@@ -18,29 +18,60 @@ __global__ void k_widen_offsets(const int32_t *src, int64_t *dst, int64_t count)
         dst[i] = (int64_t)src[i];
 }
 
+constexpr int kValidateShort = 64;  // rows up to this length are checked by one thread
+
+__device__ __forceinline__ void check_entry(const GraphDev &g, int64_t v, int64_t e, int64_t s,
+                                            unsigned long long &bad_col,
+                                            unsigned long long &bad_ord,
+                                            unsigned long long &loops) {
+    const int32_t u = g.col[e];
+    if (u < 0 || u >= g.n) ++bad_col;
+    if (e > s && u <= g.col[e - 1]) ++bad_ord;
+    if (u == v) ++loops;
+}
+
 // Flags: [0] bad offsets, [1] column out of range, [2] unsorted or
-// duplicate, [3] self-loop; [4] max degree.
-__global__ void k_validate(GraphDev g, unsigned long long *flags) {
+// duplicate, [3] self-loop; [4] max degree; [5] long rows deferred.
+__global__ void k_validate(GraphDev g, unsigned long long *flags, int32_t *longrows) {
     unsigned long long bad_off = 0, bad_col = 0, bad_ord = 0, loops = 0, maxd = 0;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
-         v += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = g.off[v], t = g.off[v + 1];
-        if (s > t || s < 0 || t > g.nnz) { ++bad_off; continue; }
-        if ((unsigned long long)(t - s) > maxd) maxd = (unsigned long long)(t - s);
-        int32_t prev = -1;
-        for (int64_t e = s; e < t; ++e) {
-            const int32_t u = g.col[e];
-            if (u < 0 || u >= g.n) ++bad_col;
-            if (u <= prev) ++bad_ord;
-            if (u == v) ++loops;
-            prev = u;
+    for (int64_t b0 = global_wave() * kWave; b0 < g.n; b0 += wave_count() * kWave) {
+        const int64_t v = b0 + lane_id();
+        bool defer = false;
+        if (v < g.n) {
+            const int64_t s = g.off[v], t = g.off[v + 1];
+            if (s > t || s < 0 || t > g.nnz) {
+                ++bad_off;
+            } else {
+                if ((unsigned long long)(t - s) > maxd) maxd = (unsigned long long)(t - s);
+                if (t - s <= kValidateShort) {
+                    for (int64_t e = s; e < t; ++e) check_entry(g, v, e, s, bad_col, bad_ord, loops);
+                } else {
+                    defer = true;
+                }
+            }
         }
+        const int64_t pos = wave_append(defer, &flags[5]);
+        if (defer) longrows[pos] = (int32_t)v;
     }
     if (bad_off) atomicAdd(&flags[0], bad_off);
     if (bad_col) atomicAdd(&flags[1], bad_col);
     if (bad_ord) atomicAdd(&flags[2], bad_ord);
     if (loops) atomicAdd(&flags[3], loops);
     if (maxd) atomicMax(&flags[4], maxd);
+}
+
+// Long rows: one wave per row.
+__global__ void k_validate_long(GraphDev g, unsigned long long *flags, const int32_t *longrows) {
+    unsigned long long bad_col = 0, bad_ord = 0, loops = 0;
+    const int64_t nl = (int64_t)flags[5];
+    for (int64_t j = global_wave(); j < nl; j += wave_count()) {
+        const int64_t v = longrows[j];
+        const int64_t s = g.off[v], t = g.off[v + 1];
+        for (int64_t e = s + lane_id(); e < t; e += kWave) check_entry(g, v, e, s, bad_col, bad_ord, loops);
+    }
+    if (bad_col) atomicAdd(&flags[1], bad_col);
+    if (bad_ord) atomicAdd(&flags[2], bad_ord);
+    if (loops) atomicAdd(&flags[3], loops);
 }
 
 __global__ void k_bin_all(GraphDev g, ListDev out, int heavy_degree, int chunk_edges) {
@@ -67,11 +98,15 @@ __global__ void k_count_heavy(GraphDev g, int heavy_degree, int chunk_edges,
 
 void graph_validate(dmolp_ctx *ctx, dmolp_graph *g) {
     DevMem flags(sizeof(unsigned long long) * 8);
+    DevMem longrows(sizeof(int32_t) * (size_t)(g->n > 0 ? g->n : 1));
     DMOLP_HIP_CHECK(hipMemsetAsync(flags.p, 0, flags.bytes, ctx->stream));
     const int blocks = ctx->cu_count * 4;
-    if (g->n > 0)
+    if (g->n > 0) {
         hipLaunchKernelGGL(k_validate, dim3(blocks), dim3(256), 0, ctx->stream, g->dev(),
-                           flags.as<unsigned long long>());
+                           flags.as<unsigned long long>(), longrows.as<int32_t>());
+        hipLaunchKernelGGL(k_validate_long, dim3(blocks), dim3(256), 0, ctx->stream, g->dev(),
+                           flags.as<unsigned long long>(), longrows.as<int32_t>());
+    }
     DMOLP_HIP_CHECK(hipGetLastError());
     unsigned long long h[8] = {0};
     DMOLP_HIP_CHECK(hipMemcpyAsync(h, flags.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));

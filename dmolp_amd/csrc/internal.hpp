@@ -94,6 +94,7 @@ struct Tuning {
     int chunk_edges = 4096;
     int block_threads = 256;
     int grid_blocks = 0;  // 0: derived from the CU count
+    double pull_fraction = 0.05;  // pull candidates when B_{t-1} covers this share of entries
 };
 
 struct GraphDev {

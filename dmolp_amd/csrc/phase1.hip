@@ -301,6 +301,13 @@ __global__ void k_p1_round_end(P1Ctl *ctl) {
     ctl->rounds += 1;
 }
 
+// random.cpp:71-74: label = global id mod k.
+__global__ void k_mod_labels(int64_t n, int k, int32_t *L) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x)
+        L[v] = (int32_t)(v % k);
+}
+
 __global__ void k_degrees(GraphDev g, int32_t *deg) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
          v += (int64_t)gridDim.x * blockDim.x)
@@ -308,6 +315,15 @@ __global__ void k_degrees(GraphDev g, int32_t *deg) {
 }
 
 }  // namespace
+
+void phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int k) {
+    DMOLP_REQUIRE(k >= 1 && k <= DMOLP_MAX_K, DMOLP_E_ARG, "k outside [1, 1024]");
+    if (!g->labels.p) g->labels.alloc(sizeof(int32_t) * (size_t)g->n);
+    hipLaunchKernelGGL(k_mod_labels, dim3(ctx->cu_count * 4), dim3(256), 0, ctx->stream, g->n, k,
+                       g->labels.as<int32_t>());
+    DMOLP_HIP_CHECK(hipGetLastError());
+    g->labels_valid = true;
+}
 
 // ---------------------------------------------------------------- host side
 

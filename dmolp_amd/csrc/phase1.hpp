@@ -34,6 +34,7 @@ std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg);
 // Seeds + expansion; labels become resident in g.  Metrics fields about
 // seeds, rounds and timing are filled (cut and counts come from the stats
 // sweep).
+void phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int k);
 void phase1_run(dmolp_ctx *ctx, dmolp_graph *g, int k, int theta, std::vector<int32_t> &seeds_out,
                 dmolp_phase1_metrics *m);
 

@@ -26,6 +26,8 @@ constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr int kHistStride = kLpCap + 1;  // odd stride: no LDS bank conflicts across slots
 constexpr int kMaxKLds = DMOLP_MAX_K;
+constexpr int kLpUnroll = 4;     // light rows: gathers in flight per lane
+constexpr int kHeavyUnroll = 8;  // heavy rows: gathers in flight per lane
 
 __device__ __forceinline__ bool is_done(const Ctl *c) { return c->done != 0; }
 
@@ -121,9 +123,15 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(GraphDev g, ListDev all
     wave_add(&ctl->v[C_CUT2], cut2);
 }
 
-__global__ void k_init_finish(Ctl *ctl, unsigned long long *ic, int k) {
+__global__ void k_init_finish(Ctl *ctl, unsigned long long *ic, int k, int32_t epoch_base,
+                              int32_t max_rounds, double epsilon) {
     for (int i = threadIdx.x; i < k; i += blockDim.x) ic[i] = ic[i] / 2;  // phase2.cpp:110
     if (threadIdx.x == 0) {
+        ctl->t = 1;
+        ctl->epoch_base = epoch_base;
+        ctl->epoch = epoch_base;
+        ctl->max_rounds = max_rounds;
+        ctl->epsilon = epsilon;
         const unsigned long long cut2 = ctl->v[C_CUT2];
         if (ctl->v[C_KERNEL_ERR] != 0) { ctl->err = DMOLP_E_ARG; ctl->done = 1; }
         else if (ctl->v[C_INVALID] != 0) { ctl->err = DMOLP_E_INVALID_LABEL; ctl->done = 1; }
@@ -173,15 +181,29 @@ __global__ void k_penalties(const unsigned long long *vc, const unsigned long lo
     if (threadIdx.x == 0 && blockIdx.x == 0) penalties(vc, ic, k, lg, pen, re);
 }
 
+// Round start: penalties, counter resets, and the candidate direction.
+// Pushing from B_{t-1} reads every edge of B_{t-1} at random; pulling reads
+// the differing-neighbour count of every vertex in order and scans a
+// boundary vertex's row only until a B_{t-1} member is found.  Pull when
+// B_{t-1} covers at least pull_frac of the CSR entries.
+// B holds B_{t-1} here; its counts are kept for the trace before the
+// selection overwrites the list with B_t.
 __global__ void k_round_begin(Ctl *ctl, const unsigned long long *vc, const unsigned long long *ic,
-                              int k, double lg, double *pen, double *re, ListDev bcur, ListDev dl,
-                              RecDev bd) {
+                              int k, const double *rtab, double *pen, double *re, ListDev B,
+                              ListDev dl, RecDev bd, int64_t n, int64_t nnz, double pull_frac) {
     if (is_done(ctl)) return;
-    if (threadIdx.x < LC_N) { bcur.cnt[threadIdx.x] = 0; dl.cnt[threadIdx.x] = 0; }
+    if (threadIdx.x < LC_N) dl.cnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
+        const int t = ctl->t;
+        const unsigned long long prev_deg = t == 1 ? (unsigned long long)nnz : B.cnt[LC_DEG];
+        ctl->v[C_PREV_N] = t == 1 ? (unsigned long long)n : B.cnt[LC_LIGHT] + B.cnt[LC_SLOTS];
+        ctl->v[C_PREV_DEG] = prev_deg;
+        ctl->epoch = ctl->epoch_base + t;
         *bd.cnt = 0;
         ctl->v[C_CAND] = 0;
-        penalties(vc, ic, k, lg, pen, re);
+        ctl->v[C_DEFERRED] = 0;
+        ctl->cand_mode = t == 1 ? CAND_ALL : ((double)prev_deg >= pull_frac * (double)nnz ? CAND_PULL : CAND_PUSH);
+        penalties(vc, ic, k, rtab[t], pen, re);
     }
 }
 
@@ -190,11 +212,13 @@ __global__ void k_round_begin(Ctl *ctl, const unsigned long long *vc, const unsi
 // Round 1: B_0 is every vertex, so the candidates are every vertex.
 __global__ __launch_bounds__(kThreads) void k_cand_all(GraphDev g, const int32_t *L,
                                                        const int32_t *diff, RecDev bd, Ctl *ctl) {
-    if (is_done(ctl)) return;
-    for (int64_t b0 = global_wave() * kWave; b0 < g.n; b0 += wave_count() * kWave) {
-        const int64_t v = b0 + lane_id();
+    if (is_done(ctl) || ctl->cand_mode != CAND_ALL) return;
+    __shared__ BlockAppendLds s_app;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < g.n;
+         b0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = b0 + threadIdx.x;
         const bool is_bd = v < g.n && diff[v] > 0;
-        const int64_t pos = wave_append(is_bd, bd.cnt);
+        const int64_t pos = block_append(is_bd, bd.cnt, s_app);
         if (is_bd) {
             bd.x[pos] = (int32_t)v;
             bd.g[pos] = L[v];
@@ -219,10 +243,73 @@ __device__ __forceinline__ void touch(const GraphDev &g, bool pred, int32_t u, i
     }
 }
 
+// Pull: a boundary vertex (differing-neighbour count > 0) is a candidate
+// iff it or one of its neighbours was selected in round t-1 (bstamp ==
+// prev_epoch).  Rows longer than heavy_degree are scanned here up to
+// heavy_degree entries and finished by k_cand_pull_heavy.
+__global__ __launch_bounds__(kThreads) void k_cand_pull(GraphDev g, const int32_t *L,
+                                                        const int32_t *diff, const int32_t *bstamp,
+                                                        RecDev bd, int32_t *deferred,
+                                                        int heavy_degree, Ctl *ctl) {
+    if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
+    const int32_t prev_epoch = ctl->epoch - 1;
+    __shared__ BlockAppendLds s_app;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < g.n;
+         b0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = b0 + threadIdx.x;
+        bool rec = false, defer = false;
+        int64_t s = 0, e = 0;
+        if (v < g.n && diff[v] > 0) {
+            s = g.off[v];
+            e = g.off[v + 1];
+            if (bstamp[v] == prev_epoch) {
+                rec = true;
+            } else {
+                const int64_t lim = e - s > heavy_degree ? s + heavy_degree : e;
+                for (int64_t x = s; x < lim; ++x)
+                    if (bstamp[g.col[x]] == prev_epoch) { rec = true; break; }
+                defer = !rec && lim < e;
+            }
+        }
+        const int64_t pos = block_append(rec, bd.cnt, s_app);
+        if (rec) {
+            bd.x[pos] = (int32_t)v;
+            bd.g[pos] = L[v];
+            bd.d[pos] = (int32_t)(e - s);
+        }
+        const int64_t dpos = wave_append(defer, &ctl->v[C_DEFERRED]);
+        if (defer) deferred[dpos] = (int32_t)v;
+    }
+}
+
+__global__ void k_cand_pull_heavy(GraphDev g, const int32_t *L, const int32_t *bstamp,
+                                  RecDev bd, const int32_t *deferred, int heavy_degree, Ctl *ctl) {
+    if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
+    const int32_t prev_epoch = ctl->epoch - 1;
+    const int64_t nd = (int64_t)ctl->v[C_DEFERRED];
+    for (int64_t j = global_wave(); j < nd; j += wave_count()) {
+        const int32_t v = deferred[j];
+        const int64_t s = g.off[v], e = g.off[v + 1];
+        bool found = false;
+        for (int64_t b = s + heavy_degree; b < e && !found; b += kWave) {
+            const int64_t x = b + lane_id();
+            found = __ballot(x < e && bstamp[g.col[x]] == prev_epoch) != 0;
+        }
+        const bool rec = found && lane_id() == 0;
+        const int64_t pos = wave_append(rec, bd.cnt);
+        if (rec) {
+            bd.x[pos] = v;
+            bd.g[pos] = L[v];
+            bd.d[pos] = (int32_t)(e - s);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_cand_light(GraphDev g, ListDev prev, const int32_t *L,
                                                          const int32_t *diff, int32_t *stamp,
-                                                         int32_t epoch, RecDev bd, Ctl *ctl) {
-    if (is_done(ctl)) return;
+                                                         RecDev bd, Ctl *ctl) {
+    if (is_done(ctl) || ctl->cand_mode != CAND_PUSH) return;
+    const int32_t epoch = ctl->epoch;
     __shared__ BatchLds s_b[kWavesPerBlock];
     const int w = threadIdx.x / kWave, lane = lane_id();
     int64_t cand = 0;
@@ -246,9 +333,9 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(GraphDev g, ListDev pre
 
 __global__ __launch_bounds__(kThreads) void k_cand_heavy(GraphDev g, ListDev prev, const int32_t *L,
                                                          const int32_t *diff, int32_t *stamp,
-                                                         int32_t epoch, int chunk_edges, RecDev bd,
-                                                         Ctl *ctl) {
-    if (is_done(ctl)) return;
+                                                         int chunk_edges, RecDev bd, Ctl *ctl) {
+    if (is_done(ctl) || ctl->cand_mode != CAND_PUSH) return;
+    const int32_t epoch = ctl->epoch;
     const int lane = lane_id();
     int64_t cand = 0;
     const int64_t nch = (int64_t)prev.cnt[LC_CHUNKS];
@@ -302,19 +389,35 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(GraphDev g, const int32_t
     __syncthreads();
     const int w = threadIdx.x / kWave, lane = lane_id();
     int32_t *h = s_h[w];
+    __shared__ BlockAppendLds s_app;
     const int64_t count = (int64_t)B.cnt[LC_LIGHT];
-    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+    // Block-uniform loop over groups of kWavesPerBlock batches (one per
+    // wave), so the Delta append can be aggregated per block.
+    for (int64_t g0 = (int64_t)blockIdx.x * kWavesPerBlock * kWave; g0 < count;
+         g0 += (int64_t)gridDim.x * kWavesPerBlock * kWave) {
+        const int64_t b0 = g0 + (int64_t)w * kWave;
         int32_t v, deg;
         bool valid;
         for (int l = 0; l < E; ++l) h[lane * kHistStride + l] = 0;
         const int32_t total = batch_load(g, B.light, count, b0, s_b[w], v, deg, valid);
-        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
-            const int32_t q = q0 + lane;
-            if (q < total) {
-                const int slot = batch_slot(s_b[w], q);
-                const int32_t lab = L[g.col[s_b[w].base[slot] + q]];
-                if (lab >= 0 && lab < E) atomicAdd(&h[slot * kHistStride + lab], 1);
+        // kLpUnroll independent column loads, then kLpUnroll label gathers,
+        // per lane per step: several misses in flight per lane.
+        for (int32_t q0 = 0; q0 < total; q0 += kWave * kLpUnroll) {
+            int slot[kLpUnroll];
+            int32_t u[kLpUnroll];
+#pragma unroll
+            for (int j = 0; j < kLpUnroll; ++j) {
+                const int32_t q = q0 + j * kWave + lane;
+                slot[j] = q < total ? batch_slot(s_b[w], q) : -1;
+                u[j] = slot[j] >= 0 ? __builtin_nontemporal_load(&g.col[s_b[w].base[slot[j]] + q])
+                                    : -1;
             }
+            int32_t lab[kLpUnroll];
+#pragma unroll
+            for (int j = 0; j < kLpUnroll; ++j) lab[j] = u[j] >= 0 ? L[u[j]] : -1;
+#pragma unroll
+            for (int j = 0; j < kLpUnroll; ++j)
+                if (lab[j] >= 0 && lab[j] < E) atomicAdd(&h[slot[j] * kHistStride + lab[j]], 1);
         }
         wave_sync();
         bool changed = false;
@@ -325,66 +428,79 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(GraphDev g, const int32_t
             changed = nl != own;
             if (changed) chg[v] = nl;
         }
-        list_append(D, changed, v, deg, heavy_degree, chunk_edges);
-        wave_sync();
+        block_list_append(D, changed, v, deg, heavy_degree, chunk_edges, s_app);
     }
 }
 
-// Heavy rows: each wave counts one chunk with a peel loop over the distinct
-// labels present in each 64-edge step, adds the counts to the row's global
-// histogram, and the wave finishing the row's last chunk decides.
+// Heavy rows, pass 1: each wave counts one chunk with a peel loop over the
+// distinct labels of each 64-edge step and adds the counts to the row's
+// histogram in global memory.  Pass 2 (k_lp_heavy_decide) runs after the
+// kernel boundary, so no fence or completion counter is needed.
 __global__ __launch_bounds__(kThreads) void k_lp_heavy(GraphDev g, const int32_t *L, ListDev B,
-                                                       const double *pen, int k, int32_t *chg,
-                                                       ListDev D, int heavy_degree,
-                                                       int chunk_edges, int32_t *hub_hist,
-                                                       int32_t *hub_done, const Ctl *ctl) {
+                                                       int k, int chunk_edges, int32_t *hub_hist,
+                                                       const Ctl *ctl) {
     if (is_done(ctl)) return;
-    __shared__ int32_t s_fin[kWavesPerBlock][kLpCap];
-    __shared__ double s_pen[kLpCap];
     const int E = k < kLpCap ? k : kLpCap;
-    if ((int)threadIdx.x < E) s_pen[threadIdx.x] = pen[threadIdx.x];
-    __syncthreads();
-    const int w = threadIdx.x / kWave, lane = lane_id();
+    const int lane = lane_id();
     const int64_t nch = (int64_t)B.cnt[LC_CHUNKS];
     for (int64_t j = global_wave(); j < nch; j += wave_count()) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         chunk_bounds(g, B, B.hc[j], chunk_edges, slot, v, e0, e1, deg);
         int32_t mycnt = 0;  // lane l counts label l
-        for (int64_t eb = e0; eb < e1; eb += kWave) {
-            const int64_t e = eb + lane;
-            const int32_t lab = e < e1 ? L[g.col[e]] : -1;
-            uint64_t active = __ballot(lab >= 0 && lab < E);
-            while (active) {
-                const int leader = __ffsll((long long)active) - 1;
-                const int32_t l0 = __shfl(lab, leader, kWave);
-                const uint64_t m = __ballot(lab == l0) & active;
-                if (lane == l0) mycnt += __popcll(m);
-                active &= ~m;
+        for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
+            int32_t u[kHeavyUnroll];
+#pragma unroll
+            for (int j = 0; j < kHeavyUnroll; ++j) {
+                const int64_t e = eb + j * kWave + lane;
+                u[j] = e < e1 ? __builtin_nontemporal_load(&g.col[e]) : -1;
+            }
+            int32_t lab[kHeavyUnroll];
+#pragma unroll
+            for (int j = 0; j < kHeavyUnroll; ++j) lab[j] = u[j] >= 0 ? L[u[j]] : -1;
+#pragma unroll
+            for (int j = 0; j < kHeavyUnroll; ++j) {
+                uint64_t active = __ballot(lab[j] >= 0 && lab[j] < E);
+                while (active) {
+                    const int leader = __ffsll((long long)active) - 1;
+                    const int32_t l0 = __shfl(lab[j], leader, kWave);
+                    const uint64_t m = __ballot(lab[j] == l0) & active;
+                    if (lane == l0) mycnt += __popcll(m);
+                    active &= ~m;
+                }
             }
         }
-        if (lane < E && mycnt) atomicAdd(&hub_hist[slot * kLpCap + lane], mycnt);
-        __threadfence();
-        int prev = 0;
-        if (lane == 0) prev = atomicAdd(&hub_done[slot], 1);
-        prev = __shfl(prev, 0, kWave);
-        const bool last = prev == B.hn[slot] - 1;
+        if (lane < E && mycnt) atomicAdd(&hub_hist[(int64_t)slot * kLpCap + lane], mycnt);
+    }
+}
+
+// Heavy rows, pass 2: one thread per heavy row decides from its histogram
+// and clears it for the next round.
+__global__ void k_lp_heavy_decide(const int32_t *L, ListDev B, const double *pen, int k,
+                                  int32_t *chg, ListDev D, int heavy_degree, int chunk_edges,
+                                  int32_t *hub_hist, const int64_t *off, const Ctl *ctl) {
+    if (is_done(ctl)) return;
+    __shared__ double s_pen[kLpCap];
+    const int E = k < kLpCap ? k : kLpCap;
+    if ((int)threadIdx.x < E) s_pen[threadIdx.x] = pen[threadIdx.x];
+    __syncthreads();
+    const int64_t ns = (int64_t)B.cnt[LC_SLOTS];
+    for (int64_t b0 = global_wave() * kWave; b0 < ns; b0 += wave_count() * kWave) {
+        const int64_t slot = b0 + lane_id();
         bool changed = false;
-        int32_t nl = 0;
-        if (last) {
-            __threadfence();
-            if (lane < kLpCap) s_fin[w][lane] = lane < E ? atomicExch(&hub_hist[slot * kLpCap + lane], 0) : 0;
-            if (lane == 0) hub_done[slot] = 0;
-            wave_sync();
-            if (lane == 0) {
-                const int32_t own = L[v];
-                nl = lp_decide(s_fin[w], 1, E, s_pen, own);
-                changed = nl != own;
-                if (changed) chg[v] = nl;
-            }
+        int32_t v = 0;
+        int64_t deg = 0;
+        if (slot < ns) {
+            v = B.hv[slot];
+            deg = off[v + 1] - off[v];
+            int32_t *h = hub_hist + slot * kLpCap;
+            const int32_t own = L[v];
+            const int32_t nl = lp_decide(h, 1, E, s_pen, own);
+            for (int l = 0; l < E; ++l) h[l] = 0;
+            changed = nl != own;
+            if (changed) chg[v] = nl;
         }
         list_append(D, changed, v, deg, heavy_degree, chunk_edges);
-        wave_sync();
     }
 }
 
@@ -512,13 +628,15 @@ __global__ void k_apply(ListDev D, int32_t *L, int32_t *chg, const Ctl *ctl) {
 
 // ---------------------------------------------------------------- round end
 
-__global__ void k_round_end(Ctl *ctl, ListDev prev, ListDev cur, ListDev D, RecDev bd, int t,
-                            int max_rounds, double epsilon, int64_t *trace, int trace_cap,
-                            int64_t n, int64_t nnz) {
+__global__ void k_round_end(Ctl *ctl, ListDev cur, ListDev D, RecDev bd, int64_t *trace,
+                            int trace_cap, int64_t n) {
     if (is_done(ctl)) return;
-    const int64_t nprev = t == 1 ? n : (int64_t)(prev.cnt[LC_LIGHT] + prev.cnt[LC_SLOTS]);
-    const int64_t dprev = t == 1 ? nnz : (int64_t)prev.cnt[LC_DEG];
-    const int64_t ncand = t == 1 ? n : (int64_t)ctl->v[C_CAND];
+    const int t = ctl->t;
+    const int max_rounds = ctl->max_rounds;
+    const double epsilon = ctl->epsilon;
+    const int64_t nprev = (int64_t)ctl->v[C_PREV_N];
+    const int64_t dprev = (int64_t)ctl->v[C_PREV_DEG];
+    const int64_t ncand = ctl->cand_mode == CAND_ALL ? n : (ctl->cand_mode == CAND_PULL ? -1 : (int64_t)ctl->v[C_CAND]);
     const int64_t nb = (int64_t)(cur.cnt[LC_LIGHT] + cur.cnt[LC_SLOTS]);
     const int64_t bdeg = (int64_t)cur.cnt[LC_DEG];
     const int64_t nchg = (int64_t)(D.cnt[LC_LIGHT] + D.cnt[LC_SLOTS]);
@@ -557,9 +675,15 @@ __global__ void k_round_end(Ctl *ctl, ListDev prev, ListDev cur, ListDev D, RecD
         r[T_BL] = (int64_t)cur.cnt[LC_LIGHT];
         r[T_BLDEG] = (int64_t)(cur.cnt[LC_DEG] - cur.cnt[LC_HDEG]);
     }
+    if (!ctl->done) ctl->t = t + 1;
 }
 
 inline unsigned grid_for(dmolp_ctx *ctx) { return (unsigned)ctx->grid_blocks(); }
+// Grid for one thread per heavy row.
+inline unsigned heavy_grid(const dmolp_graph *g) {
+    const int64_t b = (g->n_heavy + kThreads - 1) / kThreads;
+    return (unsigned)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
 
 }  // namespace
 
@@ -575,20 +699,20 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.chg.alloc(sizeof(int32_t) * (size_t)n);
         w.diff.alloc(sizeof(int32_t) * (size_t)n);
         w.stamp.alloc(sizeof(int32_t) * (size_t)n);
+        w.bstamp.alloc(sizeof(int32_t) * (size_t)n);
         w.bd.alloc(n);
         DMOLP_HIP_CHECK(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, ctx->stream));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.bstamp.p, 0, w.bstamp.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.chg.p, 0xff, w.chg.bytes, ctx->stream));
         w.ctl.alloc(sizeof(Ctl));
     }
     if (w.cap_heavy != g->n_heavy || w.cap_chunks != g->n_heavy_chunks) {
-        for (auto &b : w.B) b.alloc(n, g->n_heavy, g->n_heavy_chunks);
+        w.B.alloc(n, g->n_heavy, g->n_heavy_chunks);
         w.D.alloc(n, g->n_heavy, g->n_heavy_chunks);
-        w.tmp.alloc(n, g->n_heavy, g->n_heavy_chunks);
         w.hub_hist.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1) * kLpCap);
-        w.hub_done.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1));
+        w.deferred.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.hub_hist.p, 0, w.hub_hist.bytes, ctx->stream));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.hub_done.p, 0, w.hub_done.bytes, ctx->stream));
-        for (auto &b : w.B) DMOLP_HIP_CHECK(hipMemsetAsync(b.cnt.p, 0, b.cnt.bytes, ctx->stream));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.B.cnt.p, 0, w.B.cnt.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.D.cnt.p, 0, w.D.cnt.bytes, ctx->stream));
         w.cap_heavy = g->n_heavy;
         w.cap_chunks = g->n_heavy_chunks;
@@ -604,7 +728,23 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
     return w;
 }
 
-void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
+void phase2_round_table(dmolp_ctx *ctx, Phase2Work &w, int R) {
+    if (w.rtab_rounds >= R) return;
+    std::vector<double> tab(2 * ((size_t)R + 1), 0.0);
+    for (int t = 1; t <= R; ++t) {
+        tab[(size_t)t] = 1.0 + log((double)t);                 // phase2.cpp:199-200
+        tab[(size_t)R + 1 + (size_t)t] = exp((-t) / (5.0));    // gpu_lp_boundary.cu:244
+    }
+    w.rtab.alloc(sizeof(double) * tab.size());
+    DMOLP_HIP_CHECK(hipMemcpyAsync(w.rtab.p, tab.data(), w.rtab.bytes, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    w.rtab_rounds = R;
+    w.round_key = 0;  // the captured round reads rtab: capture again
+}
+
+void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds,
+                         double epsilon) {
     hipStream_t s = ctx->stream;
     DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(Ctl), s));
     DMOLP_HIP_CHECK(hipMemsetAsync(w.vc.p, 0, w.vc.bytes, s));
@@ -619,23 +759,24 @@ void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
         hipLaunchKernelGGL(k_init_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, g->dev(),
                            g->all.dev, L, w.k, ctx->tune.chunk_edges, w.diff.as<int32_t>(), ctl,
                            w.vc.as<unsigned long long>(), w.ic.as<unsigned long long>());
+    // Stamp values only grow, so stamps left by earlier runs never match.
+    const int32_t epoch_base = w.epoch;
+    w.epoch += max_rounds + 2;
     hipLaunchKernelGGL(k_init_finish, dim3(1), dim3(256), 0, s, ctl, w.ic.as<unsigned long long>(),
-                       w.k);
+                       w.k, epoch_base, max_rounds, epsilon);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 
-void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t, int max_rounds,
-                          double epsilon) {
+void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     hipStream_t s = ctx->stream;
     Ctl *ctl = w.ctl.as<Ctl>();
     const int32_t *done = &ctl->done;
     const GraphDev gd = g->dev();
     int32_t *L = g->labels.as<int32_t>();
-    List &bprev = w.B[(t - 1) & 1];
-    List &bcur = w.B[t & 1];
+    List &bprev = w.B;  // holds B_{t-1} until the selection rewrites it as B_t
+    List &bcur = w.B;
     const int H = ctx->tune.heavy_degree, CH = ctx->tune.chunk_edges;
     const unsigned grid = grid_for(ctx);
-    const double lg = 1.0 + log((double)t);  // phase2.cpp:199-200
 
     auto mark = [&]() -> size_t {
         if (!ctx->profiling) return 0;
@@ -644,35 +785,43 @@ void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t, 
         return ctx->ev_used - 1;
     };
     const size_t m0 = mark();
+    const double *rtab = w.rtab.as<double>();
     hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, s, ctl, w.vc.as<unsigned long long>(),
-                       w.ic.as<unsigned long long>(), w.k, lg, w.pen.as<double>(),
-                       w.re.as<double>(), bcur.dev, w.D.dev, w.bd.dev);
-    if (t == 1) {
-        hipLaunchKernelGGL(k_cand_all, dim3(grid), dim3(kThreads), 0, s, gd, L, w.diff.as<int32_t>(),
-                           w.bd.dev, ctl);
-    } else {
-        const int32_t epoch = ++w.epoch;
-        hipLaunchKernelGGL(k_cand_light, dim3(grid), dim3(kThreads), 0, s, gd, bprev.dev, L,
-                           w.diff.as<int32_t>(), w.stamp.as<int32_t>(), epoch, w.bd.dev, ctl);
-        if (g->n_heavy > 0)
-            hipLaunchKernelGGL(k_cand_heavy, dim3(grid), dim3(kThreads), 0, s, gd, bprev.dev, L,
-                               w.diff.as<int32_t>(), w.stamp.as<int32_t>(), epoch, CH, w.bd.dev,
-                               ctl);
-    }
+                       w.ic.as<unsigned long long>(), w.k, rtab, w.pen.as<double>(),
+                       w.re.as<double>(), bprev.dev, w.D.dev, w.bd.dev, g->n, g->nnz,
+                       ctx->tune.pull_fraction);
+    // Round 1 takes every vertex; later rounds pull or push (ctl->cand_mode).
+    hipLaunchKernelGGL(k_cand_all, dim3(grid), dim3(kThreads), 0, s, gd, L, w.diff.as<int32_t>(),
+                       w.bd.dev, ctl);
+    hipLaunchKernelGGL(k_cand_pull, dim3(grid), dim3(kThreads), 0, s, gd, L, w.diff.as<int32_t>(),
+                       w.bstamp.as<int32_t>(), w.bd.dev, w.deferred.as<int32_t>(), H, ctl);
+    if (g->n_heavy > 0)
+        hipLaunchKernelGGL(k_cand_pull_heavy, dim3(grid), dim3(kThreads), 0, s, gd, L,
+                           w.bstamp.as<int32_t>(), w.bd.dev, w.deferred.as<int32_t>(), H, ctl);
+    hipLaunchKernelGGL(k_cand_light, dim3(grid), dim3(kThreads), 0, s, gd, bprev.dev, L,
+                       w.diff.as<int32_t>(), w.stamp.as<int32_t>(), w.bd.dev, ctl);
+    if (g->n_heavy > 0)
+        hipLaunchKernelGGL(k_cand_heavy, dim3(grid), dim3(kThreads), 0, s, gd, bprev.dev, L,
+                           w.diff.as<int32_t>(), w.stamp.as<int32_t>(), CH, w.bd.dev, ctl);
     const size_t m1 = mark();
     TakeRule rule{};
     rule.kind = 0;
-    rule.ratio = exp((-t) / (5.0));  // gpu_lp_boundary.cu:244
+    rule.ratio_tab = rtab + w.rtab_rounds + 1;
+    rule.t_ptr = &ctl->t;
     rule.re = w.re.as<double>();
-    select_enqueue(ctx, w.sel, gd, w.bd.dev, rule, done, bcur.dev, nullptr);
+    select_enqueue(ctx, w.sel, gd, w.bd.dev, rule, done, bcur.dev, nullptr, w.bstamp.as<int32_t>(),
+                   &ctl->epoch);
     const size_t m2 = mark();
     hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, gd, L, bcur.dev,
                        w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH, ctl);
     const size_t m3 = mark();
-    if (g->n_heavy > 0)
-        hipLaunchKernelGGL(k_lp_heavy, dim3(grid), dim3(kThreads), 0, s, gd, L, bcur.dev,
-                           w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH,
-                           w.hub_hist.as<int32_t>(), w.hub_done.as<int32_t>(), ctl);
+    if (g->n_heavy > 0) {
+        hipLaunchKernelGGL(k_lp_heavy, dim3(grid), dim3(kThreads), 0, s, gd, L, bcur.dev, w.k, CH,
+                           w.hub_hist.as<int32_t>(), ctl);
+        hipLaunchKernelGGL(k_lp_heavy_decide, dim3(heavy_grid(g)), dim3(kThreads), 0, s, L,
+                           bcur.dev, w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH,
+                           w.hub_hist.as<int32_t>(), gd.off, ctl);
+    }
     const size_t m4 = mark();
     hipLaunchKernelGGL(k_delta_light, dim3(grid), dim3(kThreads), 0, s, gd, w.D.dev, L,
                        w.chg.as<int32_t>(), w.k, w.diff.as<int32_t>(), ctl,
@@ -683,9 +832,8 @@ void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t, 
                            w.vc.as<unsigned long long>(), w.ic.as<unsigned long long>());
     hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, w.D.dev, L, w.chg.as<int32_t>(),
                        ctl);
-    hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ctl, bprev.dev, bcur.dev, w.D.dev,
-                       w.bd.dev, t, max_rounds, epsilon, w.trace.as<int64_t>(), w.trace_cap, g->n,
-                       g->nnz);
+    hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ctl, bcur.dev, w.D.dev, w.bd.dev,
+                       w.trace.as<int64_t>(), w.trace_cap, g->n);
     DMOLP_HIP_CHECK(hipGetLastError());
     if (ctx->profiling) {
         const size_t m5 = mark();
@@ -706,31 +854,88 @@ void phase2_penalties_device(dmolp_ctx *ctx, int k, const unsigned long long *d_
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 
-// Test hook pieces: run the candidate + select steps of round t against an
-// explicit previous boundary already placed in w.B[0] (as light/heavy
-// lists), with re already on the device.  Output in w.B[1].
+namespace {
+
+uint64_t mix_key(uint64_t h, uint64_t v) {
+    h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    return h;
+}
+
+__global__ void k_hook_state(Ctl *ctl, int32_t t, int32_t epoch, int32_t mode) {
+    ctl->done = 0;
+    ctl->t = t;
+    ctl->epoch = epoch;
+    ctl->cand_mode = mode;
+}
+
+}  // namespace
+
+void phase2_round_launch(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
+    if (ctx->profiling) {  // events between kernels: launch directly
+        phase2_round_enqueue(ctx, g, w);
+        return;
+    }
+    // Everything the captured launches bake in.
+    uint64_t key = 0x2545f4914f6cdd1dull;
+    const void *ptrs[] = {g->labels.p, g->off.p,    g->col.p,   w.trace.p,    w.rtab.p,
+                          w.B.light.p, w.D.light.p, w.vc.p,     w.sel.h1.p,   w.sel.h2.p,
+                          w.sel.st.p,  w.hub_hist.p, w.bd.x.p,  w.deferred.p, w.chg.p};
+    for (const void *p : ptrs) key = mix_key(key, (uint64_t)(uintptr_t)p);
+    const int64_t ints[] = {w.k, ctx->tune.heavy_degree, ctx->tune.chunk_edges, ctx->grid_blocks(),
+                            g->n_heavy, w.trace_cap, w.rtab_rounds, w.sel.passes,
+                            (int64_t)(ctx->tune.pull_fraction * 1e9)};
+    for (int64_t v : ints) key = mix_key(key, (uint64_t)v);
+    if (!w.round_exec || w.round_key != key) {
+        if (w.round_exec) {
+            DMOLP_HIP_CHECK(hipGraphExecDestroy(w.round_exec));
+            w.round_exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        DMOLP_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            phase2_round_enqueue(ctx, g, w);
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        DMOLP_HIP_CHECK(hipStreamEndCapture(ctx->stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&w.round_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        DMOLP_HIP_CHECK(e);
+        w.round_key = key;
+    }
+    DMOLP_HIP_CHECK(hipGraphLaunch(w.round_exec, ctx->stream));
+}
+
+// Test hook: the candidate + selection steps of round t from an explicit
+// previous boundary already uploaded into w.B, with re on the device (the
+// candidates are pushed from B's rows).  B_t is left in w.B.
 void phase2_select_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t) {
     hipStream_t s = ctx->stream;
     Ctl *ctl = w.ctl.as<Ctl>();
     const GraphDev gd = g->dev();
     const int32_t *L = g->labels.as<int32_t>();
+    phase2_round_table(ctx, w, t > w.rtab_rounds ? t : w.rtab_rounds);
+    w.epoch += 2;
+    hipLaunchKernelGGL(k_hook_state, dim3(1), dim3(1), 0, s, ctl, t, w.epoch, (int32_t)CAND_PUSH);
     DMOLP_HIP_CHECK(hipMemsetAsync(w.bd.cnt.p, 0, w.bd.cnt.bytes, s));
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.B[1].cnt.p, 0, w.B[1].cnt.bytes, s));
-    const int32_t epoch = ++w.epoch;
-    hipLaunchKernelGGL(k_cand_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, w.B[0].dev, L,
-                       w.diff.as<int32_t>(), w.stamp.as<int32_t>(), epoch, w.bd.dev, ctl);
+    hipLaunchKernelGGL(k_cand_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, w.B.dev, L,
+                       w.diff.as<int32_t>(), w.stamp.as<int32_t>(), w.bd.dev, ctl);
     if (g->n_heavy > 0)
-        hipLaunchKernelGGL(k_cand_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, w.B[0].dev,
-                           L, w.diff.as<int32_t>(), w.stamp.as<int32_t>(), epoch,
-                           ctx->tune.chunk_edges, w.bd.dev, ctl);
+        hipLaunchKernelGGL(k_cand_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, w.B.dev,
+                           L, w.diff.as<int32_t>(), w.stamp.as<int32_t>(), ctx->tune.chunk_edges,
+                           w.bd.dev, ctl);
     TakeRule rule{};
     rule.kind = 0;
-    rule.ratio = exp((-t) / (5.0));
+    rule.ratio_tab = w.rtab.as<double>() + w.rtab_rounds + 1;
+    rule.t_ptr = &ctl->t;
     rule.re = w.re.as<double>();
-    select_enqueue(ctx, w.sel, gd, w.bd.dev, rule, &ctl->done, w.B[1].dev, nullptr);
+    select_enqueue(ctx, w.sel, gd, w.bd.dev, rule, &ctl->done, w.B.dev, nullptr);
+    DMOLP_HIP_CHECK(hipGetLastError());
 }
 
-// Test hook: decisions for the list in w.B[1]; Delta applied to g's labels.
+// Test hook: decisions for the list in w.B; Delta applied to g's labels.
 void phase2_lp_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     hipStream_t s = ctx->stream;
     Ctl *ctl = w.ctl.as<Ctl>();
@@ -738,12 +943,15 @@ void phase2_lp_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     int32_t *L = g->labels.as<int32_t>();
     const int H = ctx->tune.heavy_degree, CH = ctx->tune.chunk_edges;
     DMOLP_HIP_CHECK(hipMemsetAsync(w.D.cnt.p, 0, w.D.cnt.bytes, s));
-    hipLaunchKernelGGL(k_lp_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L, w.B[1].dev,
+    hipLaunchKernelGGL(k_lp_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L, w.B.dev,
                        w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH, ctl);
-    if (g->n_heavy > 0)
-        hipLaunchKernelGGL(k_lp_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L,
-                           w.B[1].dev, w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H,
-                           CH, w.hub_hist.as<int32_t>(), w.hub_done.as<int32_t>(), ctl);
+    if (g->n_heavy > 0) {
+        hipLaunchKernelGGL(k_lp_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L, w.B.dev,
+                           w.k, CH, w.hub_hist.as<int32_t>(), ctl);
+        hipLaunchKernelGGL(k_lp_heavy_decide, dim3(heavy_grid(g)), dim3(kThreads), 0, s, L,
+                           w.B.dev, w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH,
+                           w.hub_hist.as<int32_t>(), gd.off, ctl);
+    }
     hipLaunchKernelGGL(k_apply, dim3(grid_for(ctx)), dim3(kThreads), 0, s, w.D.dev, L,
                        w.chg.as<int32_t>(), ctl);
     DMOLP_HIP_CHECK(hipGetLastError());

@@ -17,8 +17,13 @@ enum CtlIndex {
     C_PREV_N,        // |B_{t-1}|
     C_PREV_DEG,      // sum deg over B_{t-1}
     C_KERNEL_ERR,    // a kernel found an impossible state
+    C_DEFERRED,      // heavy rows left to the pull kernel's second pass
     C_NCTL = 16
 };
+
+// How a round finds its candidates: every vertex (round 1, B_0 = V),
+// pulled from the boundary side, or pushed from B_{t-1}'s rows.
+enum CandMode : int32_t { CAND_ALL = 0, CAND_PULL = 1, CAND_PUSH = 2 };
 
 struct Ctl {
     unsigned long long v[C_NCTL];
@@ -27,7 +32,12 @@ struct Ctl {
     int32_t rounds;
     int32_t cc;
     int32_t err;  // DMOLP_E_* raised on the device
-    int32_t pad[3];
+    int32_t cand_mode;  // CAND_ALL (round 1), CAND_PULL or CAND_PUSH
+    int32_t t;           // current round, 1-based
+    int32_t epoch;       // stamp value of the current round (epoch_base + t)
+    int32_t epoch_base;
+    int32_t max_rounds;
+    double epsilon;
 };
 
 // Trace columns written by the round-end kernel.
@@ -39,8 +49,15 @@ struct Phase2Work {
     int k = 0;
     int64_t cap_heavy = -1, cap_chunks = -1;  // heavy capacities the lists were sized for
     int32_t epoch = 0;  // stamp value of the current candidate pass
-    DevMem chg, diff, stamp;
-    List B[2], D, tmp;
+    DevMem chg, diff, stamp, bstamp, deferred;
+    List B, D;          // boundary list (B_{t-1} read, then B_t written), Delta
+    DevMem rtab;        // per-round constants: [0..R] 1 + log(t), [R+1..2R+1] exp(-t/5)
+    int rtab_rounds = -1;
+    hipGraphExec_t round_exec = nullptr;  // one captured round
+    uint64_t round_key = 0;               // launch shape the capture was made for
+    ~Phase2Work() {
+        if (round_exec) (void)hipGraphExecDestroy(round_exec);
+    }
     Recs bd;
     SelectWork sel;
     DevMem hub_hist, hub_done;
@@ -52,12 +69,17 @@ struct Phase2Work {
 };
 
 Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
+// Per-round constant table for rounds 1..R (host libm, as the reference).
+void phase2_round_table(dmolp_ctx *ctx, Phase2Work &w, int R);
 
 // Enqueue: reset state, full sweep of the labels (vc, ic, cut, per-vertex
 // differing-neighbour counts).  Labels are g's resident labels.
-void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
-// Enqueue one LP round t (1-based).
-void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t, int max_rounds,
-                          double epsilon);
+void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds = 0,
+                         double epsilon = 0.0);
+// Enqueue one LP round; the round index lives in the control block, so
+// the same enqueue (or its captured graph) serves every round.
+void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
+// Launch one round through a captured HIP graph (captured on first use).
+void phase2_round_launch(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
 
 }  // namespace dmolp

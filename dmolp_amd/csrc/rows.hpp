@@ -71,6 +71,71 @@ __device__ __forceinline__ void list_append(const ListDev &L, bool pred, int32_t
     wave_add(&L.cnt[LC_DEG], pred ? deg : 0);
 }
 
+// Block-level appends: one global atomic per block per call instead of one
+// per wave (a single counter hit by every wave serialises at the L2).
+// Every thread of the block must call, with block-uniform control flow.
+constexpr int kMaxWavesPerBlock = 16;
+struct BlockAppendLds {
+    int32_t wc[kMaxWavesPerBlock];
+    int64_t wd[kMaxWavesPerBlock];
+    unsigned long long base;
+};
+
+__device__ __forceinline__ int64_t block_append(bool pred, unsigned long long *counter,
+                                                BlockAppendLds &lds) {
+    const uint64_t m = __ballot(pred);
+    const int w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    if (lane_id() == 0) lds.wc[w] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t run = 0;
+        for (int i = 0; i < nw; ++i) { const int32_t c = lds.wc[i]; lds.wc[i] = run; run += c; }
+        lds.base = run ? atomicAdd(counter, (unsigned long long)run) : 0ull;
+    }
+    __syncthreads();
+    const int64_t pos = pred ? (int64_t)(lds.base + (unsigned long long)lds.wc[w] +
+                                         (unsigned long long)lanes_below(m))
+                             : -1;
+    __syncthreads();
+    return pos;
+}
+
+// Block-level list_append.
+__device__ __forceinline__ void block_list_append(const ListDev &L, bool pred, int32_t v,
+                                                  int64_t deg, int heavy_degree, int chunk_edges,
+                                                  BlockAppendLds &lds) {
+    const bool light = pred && deg <= heavy_degree;
+    const uint64_t m = __ballot(light);
+    const int64_t dsum = wave_sum(pred ? deg : 0);
+    const int w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    if (lane_id() == 0) { lds.wc[w] = __popcll(m); lds.wd[w] = dsum; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t run = 0;
+        int64_t dtot = 0;
+        for (int i = 0; i < nw; ++i) {
+            const int32_t c = lds.wc[i];
+            lds.wc[i] = run;
+            run += c;
+            dtot += lds.wd[i];
+        }
+        lds.base = run ? atomicAdd(&L.cnt[LC_LIGHT], (unsigned long long)run) : 0ull;
+        if (dtot) atomicAdd(&L.cnt[LC_DEG], (unsigned long long)dtot);
+    }
+    __syncthreads();
+    if (light) L.light[lds.base + (unsigned long long)lds.wc[w] + (unsigned long long)lanes_below(m)] = v;
+    if (pred && deg > heavy_degree) {
+        const int64_t slot = (int64_t)atomicAdd(&L.cnt[LC_SLOTS], 1ull);
+        const int64_t nch = (deg + chunk_edges - 1) / chunk_edges;
+        const int64_t base = (int64_t)atomicAdd(&L.cnt[LC_CHUNKS], (unsigned long long)nch);
+        L.hv[slot] = v;
+        L.hn[slot] = (int32_t)nch;
+        atomicAdd(&L.cnt[LC_HDEG], (unsigned long long)deg);
+        for (int64_t c = 0; c < nch; ++c) L.hc[base + c] = (slot << 32) | c;
+    }
+    __syncthreads();
+}
+
 // Bounds of one heavy chunk item.
 __device__ __forceinline__ void chunk_bounds(const GraphDev &g, const ListDev &L, int64_t item,
                                              int chunk_edges, int32_t &slot, int32_t &v,

@@ -72,8 +72,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(RecDev in, int32_t *h
 
 __global__ __launch_bounds__(kSelThreads) void k_sel_pick1(int32_t *h1, int b1_bins, SelState *st,
                                                            TakeRule rule, int id_bits, int deg_bits,
-                                                           int32_t *refining, const int32_t *done) {
+                                                           int32_t *refining, ListDev sel,
+                                                           const int32_t *done) {
     if (done && *done) return;
+    if (blockIdx.x == 0 && threadIdx.x < LC_N) sel.cnt[threadIdx.x] = 0;
     __shared__ int64_t s_tmp[kSelThreads];
     __shared__ int s_bin;
     __shared__ int64_t s_before;
@@ -95,7 +97,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_pick1(int32_t *h1, int b1_b
     if (rule.kind == 0) {
         // gpu_lp_boundary.cu:282,296-297.
         desc = rule.re[g] > 1.0 ? 1 : 0;
-        m = c > 0 ? (int64_t)(int32_t)ceil(rule.ratio * (double)c) : 0;
+        const double ratio = rule.ratio_tab[*rule.t_ptr];
+        m = c > 0 ? (int64_t)(int32_t)ceil(ratio * (double)c) : 0;
         if (m > c) m = c;
     } else {
         // partition.cpp:426-430.
@@ -193,11 +196,17 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(RecDev in, const Se
                                                              int deg_bits, ListDev sel,
                                                              RecDev carry, int has_carry,
                                                              int heavy_degree, int chunk_edges,
+                                                             int32_t *mark,
+                                                             const int32_t *mark_val_ptr,
                                                              const int32_t *done) {
     if (done && *done) return;
+    const int32_t mark_val = mark ? *mark_val_ptr : 0;
+    __shared__ BlockAppendLds s_app;
     const int64_t count = (int64_t)*in.cnt;
-    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
-        const int64_t i = b0 + lane_id();
+    // Block-uniform loop: the appends below synchronise the block.
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < count;
+         b0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = b0 + threadIdx.x;
         const bool valid = i < count;
         int32_t x = 0, gg = 0, d = 0;
         bool take = false;
@@ -222,10 +231,11 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(RecDev in, const Se
                 default: break;
             }
         }
-        list_append(sel, take, x, d, heavy_degree, chunk_edges);
+        block_list_append(sel, take, x, d, heavy_degree, chunk_edges, s_app);
+        if (mark && take) mark[x] = mark_val;
         if (has_carry) {
             const bool keep = valid && !take;
-            const int64_t pos = wave_append(keep, carry.cnt);
+            const int64_t pos = block_append(keep, carry.cnt, s_app);
             if (keep) { carry.x[pos] = x; carry.g[pos] = gg; carry.d[pos] = d; }
         }
     }
@@ -257,14 +267,14 @@ void SelectWork::prepare(dmolp_ctx *ctx, int ngroups, int64_t n, int32_t max_deg
 
 void select_enqueue(dmolp_ctx *ctx, SelectWork &w, const GraphDev &g, const RecDev &in,
                     const TakeRule &rule, const int32_t *done, const ListDev &sel,
-                    const RecDev *carry) {
+                    const RecDev *carry, int32_t *mark, const int32_t *mark_val) {
     (void)g;
     hipStream_t s = ctx->stream;
     hipLaunchKernelGGL(k_sel_hist1, dim3(w.hist1_blocks), dim3(kSelThreads), 0, s, in,
                        w.h1.as<int32_t>(), w.groups, w.b1_bins, done);
     hipLaunchKernelGGL(k_sel_pick1, dim3(w.groups), dim3(kSelThreads), 0, s, w.h1.as<int32_t>(),
                        w.b1_bins, w.st.as<SelState>(), rule, w.id_bits, w.deg_bits,
-                       w.refining.as<int32_t>(), done);
+                       w.refining.as<int32_t>(), sel, done);
     for (int p = 0; p < w.passes; ++p) {
         hipLaunchKernelGGL(k_sel_hist2, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, in,
                            w.h2.as<int32_t>(), w.st.as<SelState>(), w.b1_bins, w.id_bits,
@@ -275,7 +285,8 @@ void select_enqueue(dmolp_ctx *ctx, SelectWork &w, const GraphDev &g, const RecD
     RecDev c = carry ? *carry : RecDev{nullptr, nullptr, nullptr, nullptr};
     hipLaunchKernelGGL(k_sel_compact, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, in,
                        w.st.as<SelState>(), w.b1_bins, w.id_bits, w.deg_bits, sel, c,
-                       carry ? 1 : 0, ctx->tune.heavy_degree, ctx->tune.chunk_edges, done);
+                       carry ? 1 : 0, ctx->tune.heavy_degree, ctx->tune.chunk_edges, mark,
+                       mark_val, done);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 

@@ -57,7 +57,8 @@ struct SelState {
 struct TakeRule {
     int32_t kind;          // 0: ceil(ratio * c) capped at c, desc = re[g] > 1.0; 1: min(theta, c), asc
     int32_t theta;
-    double ratio;
+    const double *ratio_tab;  // device: ratio of round t at ratio_tab[*t_ptr] (kind 0)
+    const int32_t *t_ptr;
     const double *re;      // device, groups entries (kind 0)
 };
 
@@ -75,8 +76,11 @@ struct SelectWork {
 // Enqueue the whole selection: level-1 histogram, pick, digit passes,
 // compaction.  Selected records go to `sel` (a vertex list); when `carry`
 // is non-null the unselected records are appended to it.
+// `sel` is reset before it is written.  `mark` (nullable): selected
+// vertices get mark[x] = *mark_val.
 void select_enqueue(dmolp_ctx *ctx, SelectWork &w, const GraphDev &g, const RecDev &in,
                     const TakeRule &rule, const int32_t *done, const ListDev &sel,
-                    const RecDev *carry);
+                    const RecDev *carry, int32_t *mark = nullptr,
+                    const int32_t *mark_val = nullptr);
 
 }  // namespace dmolp

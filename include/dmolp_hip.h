@@ -98,6 +98,8 @@ typedef struct {
     int32_t chunk_edges;   /* edges per chunk of a heavy row */
     int32_t block_threads; /* threads per workgroup for the row kernels */
     int32_t grid_blocks;   /* workgroups of the persistent row kernels */
+    int32_t pull_per_mille; /* find candidates from the boundary side when B_{t-1}
+                               covers at least this many per mille of the CSR entries */
 } dmolp_tuning;
 int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out);
 int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t);
@@ -192,6 +194,11 @@ typedef struct {
  * seeds_out (k, nullable); vc, ec (k, nullable). */
 int dmolp_phase1(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t theta, int32_t *labels_out,
                  int32_t *seeds_out, int64_t *vc, int64_t *ec, dmolp_phase1_metrics *m);
+
+/* Replaces random_partition (random.cpp:24-127, mode = random): label =
+ * vertex id mod k.  Metrics as dmolp_phase1 (seed fields zero). */
+int dmolp_phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t *labels_out,
+                        int64_t *vc, int64_t *ec, dmolp_phase1_metrics *m);
 
 /* ---------------- phase 2 ---------------- */
 

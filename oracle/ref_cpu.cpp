@@ -56,18 +56,18 @@ inline int bitrev5(int x) {
 }
 
 // Breadth-first ball of depth <= 2 (seed.cpp:18 MAX_LEVELS = 3 keeps levels
-// 0, 1, 2; seed.cpp:33-98).
+// 0, 1, 2; seed.cpp:33-98), OR-ed into `mark`.  The ball of s is computed on
+// its own (compute_bfs starts from empty bitsets, seed.cpp:21-24), so level
+// 1 is every neighbour of s whatever `mark` already holds.
 void ball2(int64_t n, const int64_t *off, const int32_t *col, int32_t s,
            std::vector<uint8_t> &mark) {
     (void)n;
     mark[s] = 1;
-    std::vector<int32_t> lvl1;
     for (int64_t e = off[s]; e < off[s + 1]; ++e) {
-        int32_t v = col[e];
-        if (!mark[v]) { mark[v] = 1; lvl1.push_back(v); }
+        const int32_t u = col[e];
+        mark[u] = 1;
+        for (int64_t f = off[u]; f < off[u + 1]; ++f) mark[col[f]] = 1;
     }
-    for (int32_t u : lvl1)
-        for (int64_t e = off[u]; e < off[u + 1]; ++e) mark[col[e]] = 1;
 }
 
 }  // namespace

@@ -24,16 +24,22 @@ def c1():
     return api.gen_host(api.rmat_spec(16, 16, seed=1))
 
 
-@pytest.fixture(params=["default", "heavy"])
+DEFAULT_TUNING = dict(heavy_degree=1024, chunk_edges=4096, pull_per_mille=50)
+TUNINGS = {
+    "default": DEFAULT_TUNING,
+    # Tiny heavy threshold: the chunked heavy-row kernels run on small graphs.
+    "heavy": dict(heavy_degree=5, chunk_edges=64, pull_per_mille=50),
+    # Candidates always pushed from B_{t-1} / always pulled from the boundary.
+    "push": dict(heavy_degree=1024, chunk_edges=4096, pull_per_mille=1000000),
+    "pull": dict(heavy_degree=5, chunk_edges=64, pull_per_mille=1),
+}
+
+
+@pytest.fixture(params=list(TUNINGS))
 def tuned(ctx, request):
-    """Run each test with the default binning and with tiny heavy thresholds,
-    so the chunked heavy-row kernels are exercised on small graphs."""
-    if request.param == "heavy":
-        ctx.set_tuning(heavy_degree=5, chunk_edges=64)
-    else:
-        ctx.set_tuning(heavy_degree=1024, chunk_edges=4096)
+    ctx.set_tuning(**TUNINGS[request.param])
     yield ctx
-    ctx.set_tuning(heavy_degree=1024, chunk_edges=4096)
+    ctx.set_tuning(**DEFAULT_TUNING)
 
 
 @pytest.mark.parametrize("spec", [
@@ -125,9 +131,13 @@ def _check_phase2(g, off, col, k, labels0, rounds, eps):
     assert np.array_equal(want.vc, got.vc) and np.array_equal(want.ic, got.ic)
     assert want.m.vertex_balance == got.metrics.vertex_balance
     assert want.m.edge_balance == got.metrics.edge_balance
-    for f in ("prev_count", "prev_deg", "cand_count", "bd_count", "b_count", "b_deg",
-              "chg_count", "chg_deg", "cut"):
+    for f in ("prev_count", "prev_deg", "bd_count", "b_count", "b_deg", "chg_count", "chg_deg",
+              "cut"):
         assert np.array_equal(want.trace[f], got.trace[f]), f
+    # Candidates are counted only in rounds that push from B_{t-1} (-1 when
+    # they are pulled from the boundary side).
+    pushed = got.trace["cand_count"] >= 0
+    assert np.array_equal(want.trace["cand_count"][pushed], got.trace["cand_count"][pushed])
     return want
 
 
@@ -174,3 +184,25 @@ def test_config1_end_to_end(tuned):
     assert want1.m.initial_vertex_balance == got1.metrics.initial_vertex_balance
     assert want1.m.initial_edge_balance == got1.metrics.initial_edge_balance
     _check_phase2(g, off, col, 8, want1.labels, 10, 0.03)
+
+
+def test_graph_replays_match_eager(ctx):
+    """Phase 2 normally replays one captured round as a HIP graph; with
+    profiling on it launches the kernels directly.  Several graph runs, with
+    eager runs between them, must all equal the eager result and the oracle."""
+    off, col = api.gen_host(api.rmat_spec(14, 16, seed=4))
+    g = ctx.upload(off, col)
+    p1 = g.phase1(8, 100)
+    g.checkpoint_labels()
+    want = O.phase2(off, col, 8, p1.labels, 15, 0.01)
+    runs = []
+    for profiled in (True, False, False, True, False):
+        ctx.set_profiling(profiled)
+        g.checkpoint_labels(restore=True)
+        r = g.phase2(8, 15, 0.01)
+        runs.append(r)
+    ctx.set_profiling(False)
+    for r in runs:
+        assert np.array_equal(r.labels, want.labels)
+        assert r.metrics.edge_cut == want.m.edge_cut and r.metrics.rounds == want.m.rounds
+        assert np.array_equal(r.trace["b_count"], want.trace["b_count"])

@@ -136,6 +136,9 @@ def brute_seeds(off, col, k):
 
 @pytest.mark.parametrize("name,g,k", [
     ("grid", graphs.grid(12, 12), 4),
+    # Later balls overlap earlier ones at level 1 (the 4th seed is 13).
+    ("grid5x7", graphs.grid(5, 7), 4),
+    ("grid5x7k6", graphs.grid(5, 7), 6),
     ("rand", graphs.random_graph(400, 600, 5), 6),
     ("plaw", graphs.powerlaw(600, 1500, 7), 5),
     ("cliques", graphs.two_cliques(8), 3),
