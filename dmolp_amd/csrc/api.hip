@@ -246,7 +246,8 @@ int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out) {
 
 int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
     if (!ctx || !t) return fail(ctx, DMOLP_E_ARG, "null argument");
-    if (t->heavy_degree < 0 || t->heavy_degree > (1 << 20) || t->chunk_edges < 0 ||
+    // Light rows keep 16-bit per-row counters: heavy_degree stays below 2^16.
+    if (t->heavy_degree < 0 || t->heavy_degree > 65535 || t->chunk_edges < 0 ||
         t->grid_blocks < 0 || t->pull_per_mille < 0 || t->pull_per_mille > 1000000 ||
         (t->block_threads != 0 && t->block_threads != 256))
         return fail(ctx, DMOLP_E_ARG, "tuning value out of range");

@@ -28,6 +28,8 @@ constexpr int kHistStride = kLpCap + 1;  // odd stride: no LDS bank conflicts ac
 constexpr int kMaxKLds = DMOLP_MAX_K;
 constexpr int kLpUnroll = 4;     // light rows: gathers in flight per lane
 constexpr int kHeavyUnroll = 8;  // heavy rows: gathers in flight per lane
+constexpr int32_t kLabel8None = 255;  // byte-mirror value of labels the LP cannot see
+constexpr int kScanItems = 4;         // vertices per thread per block iteration in scans
 
 __device__ __forceinline__ bool is_done(const Ctl *c) { return c->done != 0; }
 
@@ -214,15 +216,24 @@ __global__ __launch_bounds__(kThreads) void k_cand_all(GraphDev g, const int32_t
                                                        const int32_t *diff, RecDev bd, Ctl *ctl) {
     if (is_done(ctl) || ctl->cand_mode != CAND_ALL) return;
     __shared__ BlockAppendLds s_app;
-    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < g.n;
-         b0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = b0 + threadIdx.x;
-        const bool is_bd = v < g.n && diff[v] > 0;
-        const int64_t pos = block_append(is_bd, bd.cnt, s_app);
-        if (is_bd) {
+    const int64_t span = (int64_t)blockDim.x * kScanItems;
+    for (int64_t b0 = (int64_t)blockIdx.x * span; b0 < g.n; b0 += (int64_t)gridDim.x * span) {
+        uint32_t m = 0;
+        int32_t nb = 0;
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            if (v < g.n && diff[v] > 0) { m |= 1u << j; ++nb; }
+        }
+        int64_t pos = block_reserve(nb, bd.cnt, s_app);
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            if (!(m & (1u << j))) continue;
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             bd.x[pos] = (int32_t)v;
             bd.g[pos] = L[v];
             bd.d[pos] = (int32_t)(g.off[v + 1] - g.off[v]);
+            ++pos;
         }
     }
 }
@@ -254,31 +265,39 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(GraphDev g, const int32_
     if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
     const int32_t prev_epoch = ctl->epoch - 1;
     __shared__ BlockAppendLds s_app;
-    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < g.n;
-         b0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = b0 + threadIdx.x;
-        bool rec = false, defer = false;
-        int64_t s = 0, e = 0;
-        if (v < g.n && diff[v] > 0) {
-            s = g.off[v];
-            e = g.off[v + 1];
-            if (bstamp[v] == prev_epoch) {
-                rec = true;
-            } else {
+    const int64_t span = (int64_t)blockDim.x * kScanItems;
+    for (int64_t b0 = (int64_t)blockIdx.x * span; b0 < g.n; b0 += (int64_t)gridDim.x * span) {
+        uint32_t rec = 0;
+        int32_t nr = 0, deg[kScanItems];
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            deg[j] = 0;
+            if (v >= g.n || diff[v] <= 0) continue;
+            const int64_t s = g.off[v], e = g.off[v + 1];
+            deg[j] = (int32_t)(e - s);
+            bool hit = bstamp[v] == prev_epoch;
+            if (!hit) {
                 const int64_t lim = e - s > heavy_degree ? s + heavy_degree : e;
                 for (int64_t x = s; x < lim; ++x)
-                    if (bstamp[g.col[x]] == prev_epoch) { rec = true; break; }
-                defer = !rec && lim < e;
+                    if (bstamp[g.col[x]] == prev_epoch) { hit = true; break; }
+                // A long row without a hit in its first heavy_degree entries
+                // is finished by k_cand_pull_heavy (rare).
+                if (!hit && lim < e)
+                    deferred[atomicAdd(&ctl->v[C_DEFERRED], 1ull)] = (int32_t)v;
             }
+            if (hit) { rec |= 1u << j; ++nr; }
         }
-        const int64_t pos = block_append(rec, bd.cnt, s_app);
-        if (rec) {
+        int64_t pos = block_reserve(nr, bd.cnt, s_app);
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            if (!(rec & (1u << j))) continue;
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             bd.x[pos] = (int32_t)v;
             bd.g[pos] = L[v];
-            bd.d[pos] = (int32_t)(e - s);
+            bd.d[pos] = deg[j];
+            ++pos;
         }
-        const int64_t dpos = wave_append(defer, &ctl->v[C_DEFERRED]);
-        if (defer) deferred[dpos] = (int32_t)v;
     }
 }
 
@@ -377,8 +396,8 @@ __device__ __forceinline__ int32_t lp_decide(const int32_t *h, int stride, int E
 }
 
 __global__ __launch_bounds__(kThreads) void k_lp_light(GraphDev g, const int32_t *L, ListDev B,
-                                                       const double *pen, int k, int32_t *chg,
-                                                       ListDev D, int heavy_degree,
+                                                       const uint8_t *L8, const double *pen, int k,
+                                                       int32_t *chg, ListDev D, int heavy_degree,
                                                        int chunk_edges, const Ctl *ctl) {
     if (is_done(ctl)) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
@@ -414,10 +433,10 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(GraphDev g, const int32_t
             }
             int32_t lab[kLpUnroll];
 #pragma unroll
-            for (int j = 0; j < kLpUnroll; ++j) lab[j] = u[j] >= 0 ? L[u[j]] : -1;
+            for (int j = 0; j < kLpUnroll; ++j) lab[j] = u[j] >= 0 ? (int32_t)L8[u[j]] : kLabel8None;
 #pragma unroll
             for (int j = 0; j < kLpUnroll; ++j)
-                if (lab[j] >= 0 && lab[j] < E) atomicAdd(&h[slot[j] * kHistStride + lab[j]], 1);
+                if (lab[j] < E) atomicAdd(&h[slot[j] * kHistStride + lab[j]], 1);
         }
         wave_sync();
         bool changed = false;
@@ -436,18 +455,22 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(GraphDev g, const int32_t
 // distinct labels of each 64-edge step and adds the counts to the row's
 // histogram in global memory.  Pass 2 (k_lp_heavy_decide) runs after the
 // kernel boundary, so no fence or completion counter is needed.
-__global__ __launch_bounds__(kThreads) void k_lp_heavy(GraphDev g, const int32_t *L, ListDev B,
+__global__ __launch_bounds__(kThreads) void k_lp_heavy(GraphDev g, const uint8_t *L8, ListDev B,
                                                        int k, int chunk_edges, int32_t *hub_hist,
                                                        const Ctl *ctl) {
     if (is_done(ctl)) return;
+    // One private histogram row per lane (odd stride: rows fall in different
+    // banks), summed across the wave once per chunk.
+    __shared__ int32_t s_h[kWavesPerBlock][kWave * kHistStride];
     const int E = k < kLpCap ? k : kLpCap;
-    const int lane = lane_id();
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    int32_t *row = &s_h[w][lane * kHistStride];
     const int64_t nch = (int64_t)B.cnt[LC_CHUNKS];
     for (int64_t j = global_wave(); j < nch; j += wave_count()) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         chunk_bounds(g, B, B.hc[j], chunk_edges, slot, v, e0, e1, deg);
-        int32_t mycnt = 0;  // lane l counts label l
+        for (int l = 0; l < E; ++l) row[l] = 0;
         for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
             int32_t u[kHeavyUnroll];
 #pragma unroll
@@ -457,20 +480,18 @@ __global__ __launch_bounds__(kThreads) void k_lp_heavy(GraphDev g, const int32_t
             }
             int32_t lab[kHeavyUnroll];
 #pragma unroll
-            for (int j = 0; j < kHeavyUnroll; ++j) lab[j] = u[j] >= 0 ? L[u[j]] : -1;
+            for (int j = 0; j < kHeavyUnroll; ++j) lab[j] = u[j] >= 0 ? (int32_t)L8[u[j]] : kLabel8None;
 #pragma unroll
-            for (int j = 0; j < kHeavyUnroll; ++j) {
-                uint64_t active = __ballot(lab[j] >= 0 && lab[j] < E);
-                while (active) {
-                    const int leader = __ffsll((long long)active) - 1;
-                    const int32_t l0 = __shfl(lab[j], leader, kWave);
-                    const uint64_t m = __ballot(lab[j] == l0) & active;
-                    if (lane == l0) mycnt += __popcll(m);
-                    active &= ~m;
-                }
-            }
+            for (int j = 0; j < kHeavyUnroll; ++j)
+                if (lab[j] < E) row[lab[j]] += 1;
         }
-        if (lane < E && mycnt) atomicAdd(&hub_hist[(int64_t)slot * kLpCap + lane], mycnt);
+        wave_sync();
+        if (lane < E) {
+            int32_t c = 0;
+            for (int r = 0; r < kWave; ++r) c += s_h[w][r * kHistStride + lane];
+            if (c) atomicAdd(&hub_hist[(int64_t)slot * kLpCap + lane], c);
+        }
+        wave_sync();
     }
 }
 
@@ -529,6 +550,24 @@ __device__ __forceinline__ int32_t pair_update(int32_t v, int32_t a, int32_t c, 
     return dd;
 }
 
+// The same pair update, returning its effect instead of applying the
+// intra-partition counts: bit 0 the old pair was intra (a == lu), bit 16 the
+// new one is (c == lu2); dd through `dd_out`.  Callers fold the counts once
+// per vertex, since a and c are the vertex's own old and new labels.
+__device__ __forceinline__ int32_t pair_effect(int32_t v, int32_t a, int32_t c, int32_t u,
+                                               const int32_t *L, const int32_t *chg,
+                                               int32_t *diff, int32_t &dd_out) {
+    dd_out = 0;
+    const int32_t cu = chg[u];
+    if (cu >= 0 && u < v) return 0;  // counted from u's side
+    const int32_t lu = L[u];
+    const int32_t lu2 = cu >= 0 ? cu : lu;
+    const int32_t dd = (c != lu2 ? 1 : 0) - (a != lu ? 1 : 0);
+    if (dd) atomicAdd(&diff[u], dd);
+    dd_out = dd;
+    return (a == lu ? 1 : 0) | (c == lu2 ? 1 << 16 : 0);
+}
+
 __global__ __launch_bounds__(kThreads) void k_delta_light(GraphDev g, ListDev D, const int32_t *L,
                                                           const int32_t *chg, int k, int32_t *diff,
                                                           Ctl *ctl, unsigned long long *vc,
@@ -536,6 +575,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(GraphDev g, ListDev D,
     if (is_done(ctl)) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ int32_t s_dv[kWavesPerBlock][kWave];
+    __shared__ int32_t s_io[kWavesPerBlock][kWave];
     __shared__ unsigned long long s_vc[kMaxKLds], s_ic[kMaxKLds];
     for (int i = threadIdx.x; i < k; i += blockDim.x) { s_vc[i] = 0; s_ic[i] = 0; }
     __syncthreads();
@@ -547,25 +587,41 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(GraphDev g, ListDev D,
         bool valid;
         const int32_t total = batch_load(g, D.light, count, b0, s_b[w], v, deg, valid);
         s_dv[w][lane] = 0;
+        s_io[w][lane] = 0;
+        s_b[w].aux[lane] = valid ? L[v] : 0;  // old label of the slot's vertex
         wave_sync();
-        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
-            const int32_t q = q0 + lane;
-            if (q < total) {
-                const int slot = batch_slot(s_b[w], q);
-                const int32_t vv = s_b[w].v[slot];
-                const int32_t a = L[vv], c = chg[vv];
-                const int32_t u = g.col[s_b[w].base[slot] + q];
-                bool counted;
-                const int32_t dd = pair_update(vv, a, c, u, L, chg, k, diff, s_ic, cut_delta, counted);
-                if (dd) atomicAdd(&s_dv[w][slot], dd);
+        for (int32_t q0 = 0; q0 < total; q0 += kWave * kLpUnroll) {
+            int slot[kLpUnroll];
+            int32_t u[kLpUnroll];
+#pragma unroll
+            for (int j = 0; j < kLpUnroll; ++j) {
+                const int32_t q = q0 + j * kWave + lane;
+                slot[j] = q < total ? batch_slot(s_b[w], q) : -1;
+                u[j] = slot[j] >= 0 ? g.col[s_b[w].base[slot[j]] + q] : -1;
+            }
+#pragma unroll
+            for (int j = 0; j < kLpUnroll; ++j) {
+                if (slot[j] < 0) continue;
+                const int32_t vv = s_b[w].v[slot[j]];
+                int32_t dd;
+                const int32_t io = pair_effect(vv, s_b[w].aux[slot[j]], chg[vv], u[j], L, chg, diff, dd);
+                if (dd) { atomicAdd(&s_dv[w][slot[j]], dd); cut_delta += dd; }
+                if (io) atomicAdd(&s_io[w][slot[j]], io);
             }
         }
         wave_sync();
         if (valid) {
             if (s_dv[w][lane]) atomicAdd(&diff[v], s_dv[w][lane]);
             const int32_t a = L[v], c = chg[v];
-            if (a >= 0 && a < k) atomicAdd(&s_vc[a], ~0ull);
-            if (c >= 0 && c < k) atomicAdd(&s_vc[c], 1ull);
+            const int32_t io = s_io[w][lane];
+            if (a >= 0 && a < k) {
+                atomicAdd(&s_vc[a], ~0ull);
+                if (io & 0xffff) atomicAdd(&s_ic[a], (unsigned long long)(-(int64_t)(io & 0xffff)));
+            }
+            if (c >= 0 && c < k) {
+                atomicAdd(&s_vc[c], 1ull);
+                if (io >> 16) atomicAdd(&s_ic[c], (unsigned long long)(io >> 16));
+            }
         }
         wave_sync();
     }
@@ -595,14 +651,22 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(GraphDev g, ListDev D,
         const int64_t item = D.hc[j];
         chunk_bounds(g, D, item, chunk_edges, slot, v, e0, e1, deg);
         const int32_t a = L[v], c = chg[v];
-        int64_t dv = 0;
+        int64_t dv = 0, oi = 0, ni = 0;
         for (int64_t e = e0 + lane; e < e1; e += kWave) {
-            bool counted;
-            dv += pair_update(v, a, c, g.col[e], L, chg, k, diff, s_ic, cut_delta, counted);
+            int32_t dd;
+            const int32_t io = pair_effect(v, a, c, g.col[e], L, chg, diff, dd);
+            dv += dd;
+            oi += io & 0xffff;
+            ni += io >> 16;
         }
         dv = wave_sum(dv);
+        oi = wave_sum(oi);
+        ni = wave_sum(ni);
+        cut_delta += lane == 0 ? dv : 0;
         if (lane == 0) {
             if (dv) atomicAdd(&diff[v], (int32_t)dv);
+            if (a >= 0 && a < k && oi) atomicAdd(&s_ic[a], (unsigned long long)(-oi));
+            if (c >= 0 && c < k && ni) atomicAdd(&s_ic[c], (unsigned long long)ni);
             if ((item & 0xffffffffll) == 0) {
                 if (a >= 0 && a < k) atomicAdd(&vc[a], ~0ull);
                 if (c >= 0 && c < k) atomicAdd(&vc[c], 1ull);
@@ -615,13 +679,25 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(GraphDev g, ListDev D,
         if (s_ic[i]) atomicAdd(&ic[i], s_ic[i]);
 }
 
-__global__ void k_apply(ListDev D, int32_t *L, int32_t *chg, const Ctl *ctl) {
+// Byte mirror of the labels for the LP gathers: label if below the LP cap
+// (gpu_lp_boundary.cu:360), else kLabel8None.
+__global__ void k_mirror8(int64_t n, const int32_t *L, uint8_t *L8) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t l = L[v];
+        L8[v] = (l >= 0 && l < kLpCap) ? (uint8_t)l : (uint8_t)kLabel8None;
+    }
+}
+
+__global__ void k_apply(ListDev D, int32_t *L, uint8_t *L8, int32_t *chg, const Ctl *ctl) {
     if (is_done(ctl)) return;
     const int64_t nl = (int64_t)D.cnt[LC_LIGHT], ns = (int64_t)D.cnt[LC_SLOTS];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl + ns;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t v = i < nl ? D.light[i] : D.hv[i - nl];
-        L[v] = chg[v];
+        const int32_t nl_v = chg[v];
+        L[v] = nl_v;
+        L8[v] = (nl_v >= 0 && nl_v < kLpCap) ? (uint8_t)nl_v : (uint8_t)kLabel8None;
         chg[v] = -1;
     }
 }
@@ -700,6 +776,7 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.diff.alloc(sizeof(int32_t) * (size_t)n);
         w.stamp.alloc(sizeof(int32_t) * (size_t)n);
         w.bstamp.alloc(sizeof(int32_t) * (size_t)n);
+        w.L8.alloc((size_t)n);
         w.bd.alloc(n);
         DMOLP_HIP_CHECK(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.bstamp.p, 0, w.bstamp.bytes, ctx->stream));
@@ -762,6 +839,8 @@ void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_
     // Stamp values only grow, so stamps left by earlier runs never match.
     const int32_t epoch_base = w.epoch;
     w.epoch += max_rounds + 2;
+    hipLaunchKernelGGL(k_mirror8, dim3(ctx->cu_count * 4), dim3(kThreads), 0, s, g->n, L,
+                       w.L8.as<uint8_t>());
     hipLaunchKernelGGL(k_init_finish, dim3(1), dim3(256), 0, s, ctl, w.ic.as<unsigned long long>(),
                        w.k, epoch_base, max_rounds, epsilon);
     DMOLP_HIP_CHECK(hipGetLastError());
@@ -812,11 +891,12 @@ void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     select_enqueue(ctx, w.sel, gd, w.bd.dev, rule, done, bcur.dev, nullptr, w.bstamp.as<int32_t>(),
                    &ctl->epoch);
     const size_t m2 = mark();
-    hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, gd, L, bcur.dev,
+    uint8_t *L8 = w.L8.as<uint8_t>();
+    hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, gd, L, bcur.dev, L8,
                        w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH, ctl);
     const size_t m3 = mark();
     if (g->n_heavy > 0) {
-        hipLaunchKernelGGL(k_lp_heavy, dim3(grid), dim3(kThreads), 0, s, gd, L, bcur.dev, w.k, CH,
+        hipLaunchKernelGGL(k_lp_heavy, dim3(grid), dim3(kThreads), 0, s, gd, L8, bcur.dev, w.k, CH,
                            w.hub_hist.as<int32_t>(), ctl);
         hipLaunchKernelGGL(k_lp_heavy_decide, dim3(heavy_grid(g)), dim3(kThreads), 0, s, L,
                            bcur.dev, w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH,
@@ -830,8 +910,8 @@ void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
         hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, gd, w.D.dev, L,
                            w.chg.as<int32_t>(), w.k, CH, w.diff.as<int32_t>(), ctl,
                            w.vc.as<unsigned long long>(), w.ic.as<unsigned long long>());
-    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, w.D.dev, L, w.chg.as<int32_t>(),
-                       ctl);
+    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, w.D.dev, L, L8,
+                       w.chg.as<int32_t>(), ctl);
     hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ctl, bcur.dev, w.D.dev, w.bd.dev,
                        w.trace.as<int64_t>(), w.trace_cap, g->n);
     DMOLP_HIP_CHECK(hipGetLastError());
@@ -879,7 +959,8 @@ void phase2_round_launch(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     uint64_t key = 0x2545f4914f6cdd1dull;
     const void *ptrs[] = {g->labels.p, g->off.p,    g->col.p,   w.trace.p,    w.rtab.p,
                           w.B.light.p, w.D.light.p, w.vc.p,     w.sel.h1.p,   w.sel.h2.p,
-                          w.sel.st.p,  w.hub_hist.p, w.bd.x.p,  w.deferred.p, w.chg.p};
+                          w.sel.st.p,  w.hub_hist.p, w.bd.x.p,  w.deferred.p, w.chg.p,
+                          w.sel.ref.x.p, w.L8.p};
     for (const void *p : ptrs) key = mix_key(key, (uint64_t)(uintptr_t)p);
     const int64_t ints[] = {w.k, ctx->tune.heavy_degree, ctx->tune.chunk_edges, ctx->grid_blocks(),
                             g->n_heavy, w.trace_cap, w.rtab_rounds, w.sel.passes,
@@ -943,16 +1024,18 @@ void phase2_lp_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     int32_t *L = g->labels.as<int32_t>();
     const int H = ctx->tune.heavy_degree, CH = ctx->tune.chunk_edges;
     DMOLP_HIP_CHECK(hipMemsetAsync(w.D.cnt.p, 0, w.D.cnt.bytes, s));
-    hipLaunchKernelGGL(k_lp_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L, w.B.dev,
+    uint8_t *L8 = w.L8.as<uint8_t>();
+    hipLaunchKernelGGL(k_mirror8, dim3(ctx->cu_count * 4), dim3(kThreads), 0, s, g->n, L, L8);
+    hipLaunchKernelGGL(k_lp_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L, w.B.dev, L8,
                        w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH, ctl);
     if (g->n_heavy > 0) {
-        hipLaunchKernelGGL(k_lp_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L, w.B.dev,
+        hipLaunchKernelGGL(k_lp_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L8, w.B.dev,
                            w.k, CH, w.hub_hist.as<int32_t>(), ctl);
         hipLaunchKernelGGL(k_lp_heavy_decide, dim3(heavy_grid(g)), dim3(kThreads), 0, s, L,
                            w.B.dev, w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH,
                            w.hub_hist.as<int32_t>(), gd.off, ctl);
     }
-    hipLaunchKernelGGL(k_apply, dim3(grid_for(ctx)), dim3(kThreads), 0, s, w.D.dev, L,
+    hipLaunchKernelGGL(k_apply, dim3(grid_for(ctx)), dim3(kThreads), 0, s, w.D.dev, L, L8,
                        w.chg.as<int32_t>(), ctl);
     DMOLP_HIP_CHECK(hipGetLastError());
 }

@@ -50,6 +50,7 @@ struct Phase2Work {
     int64_t cap_heavy = -1, cap_chunks = -1;  // heavy capacities the lists were sized for
     int32_t epoch = 0;  // stamp value of the current candidate pass
     DevMem chg, diff, stamp, bstamp, deferred;
+    DevMem L8;  // byte mirror of the labels for the LP gathers
     List B, D;          // boundary list (B_{t-1} read, then B_t written), Delta
     DevMem rtab;        // per-round constants: [0..R] 1 + log(t), [R+1..2R+1] exp(-t/5)
     int rtab_rounds = -1;

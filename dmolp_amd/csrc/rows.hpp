@@ -100,6 +100,40 @@ __device__ __forceinline__ int64_t block_append(bool pred, unsigned long long *c
     return pos;
 }
 
+// Reserve my_n consecutive slots of a global list: returns this thread's
+// first slot.  One atomic per block per call (plus one for `sum_counter`,
+// which receives the block's sum of my_sum).  Block-uniform calls only.
+__device__ __forceinline__ int64_t block_reserve(int32_t my_n, unsigned long long *counter,
+                                                 BlockAppendLds &lds, int64_t my_sum = 0,
+                                                 unsigned long long *sum_counter = nullptr) {
+    const int lane = lane_id(), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    int32_t incl = my_n;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int32_t y = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += y;
+    }
+    const int64_t wsum = sum_counter ? wave_sum(my_sum) : 0;
+    if (lane == kWave - 1) { lds.wc[w] = incl; lds.wd[w] = wsum; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t run = 0;
+        int64_t dt = 0;
+        for (int i = 0; i < nw; ++i) {
+            const int32_t c = lds.wc[i];
+            lds.wc[i] = run;
+            run += c;
+            dt += lds.wd[i];
+        }
+        lds.base = run ? atomicAdd(counter, (unsigned long long)run) : 0ull;
+        if (sum_counter && dt) atomicAdd(sum_counter, (unsigned long long)dt);
+    }
+    __syncthreads();
+    const int64_t first = (int64_t)(lds.base + (unsigned long long)lds.wc[w]) + (incl - my_n);
+    __syncthreads();
+    return first;
+}
+
 // Block-level list_append.
 __device__ __forceinline__ void block_list_append(const ListDev &L, bool pred, int32_t v,
                                                   int64_t deg, int heavy_degree, int chunk_edges,

@@ -9,6 +9,7 @@ namespace {
 
 constexpr int kSelThreads = 256;
 constexpr int kHist1LdsInts = 8192;  // 32 KB of LDS for the level-1 histogram
+constexpr int kCompactItems = 8;     // records per thread per block iteration
 
 __device__ __forceinline__ uint64_t sel_key(int32_t x, int32_t d, int desc, int id_bits,
                                             int deg_bits) {
@@ -73,9 +74,11 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(RecDev in, int32_t *h
 __global__ __launch_bounds__(kSelThreads) void k_sel_pick1(int32_t *h1, int b1_bins, SelState *st,
                                                            TakeRule rule, int id_bits, int deg_bits,
                                                            int32_t *refining, ListDev sel,
+                                                           unsigned long long *ref_cnt,
                                                            const int32_t *done) {
     if (done && *done) return;
     if (blockIdx.x == 0 && threadIdx.x < LC_N) sel.cnt[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ref_cnt = 0;
     __shared__ int64_t s_tmp[kSelThreads];
     __shared__ int s_bin;
     __shared__ int64_t s_before;
@@ -138,6 +141,48 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_pick1(int32_t *h1, int b1_b
     if (threadIdx.x == 0) st[g] = s;
     __syncthreads();
     for (int j = threadIdx.x; j < b1_bins; j += blockDim.x) bins[j] = 0;
+}
+
+// The records the digit passes still have to look at: those in the chosen
+// level-1 bin of a group whose cut falls inside that bin.  Usually a small
+// share of all records, so the passes below scan this list only.
+__global__ __launch_bounds__(kSelThreads) void k_sel_collect(RecDev in, const SelState *st,
+                                                             int b1_bins, RecDev ref,
+                                                             const int32_t *refining,
+                                                             const int32_t *done) {
+    if (done && *done) return;
+    if (*refining == 0) return;
+    __shared__ BlockAppendLds s_app;
+    const int64_t count = (int64_t)*in.cnt;
+    const int64_t span = (int64_t)blockDim.x * kCompactItems;
+    for (int64_t b0 = (int64_t)blockIdx.x * span; b0 < count; b0 += (int64_t)gridDim.x * span) {
+        int32_t x[kCompactItems], gg[kCompactItems], d[kCompactItems];
+        uint32_t keep_mask = 0;
+        int32_t nk = 0;
+#pragma unroll
+        for (int j = 0; j < kCompactItems; ++j) {
+            const int64_t i = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            x[j] = 0; gg[j] = 0; d[j] = 0;
+            if (i >= count) continue;
+            gg[j] = in.g[i];
+            d[j] = in.d[i];
+            const SelState s = st[gg[j]];
+            if (s.mode == SEL_REFINE && bin1_of(d[j], b1_bins) == s.b1) {
+                x[j] = in.x[i];
+                keep_mask |= 1u << j;
+                ++nk;
+            }
+        }
+        int64_t pos = block_reserve(nk, ref.cnt, s_app);
+#pragma unroll
+        for (int j = 0; j < kCompactItems; ++j) {
+            if (!(keep_mask & (1u << j))) continue;
+            ref.x[pos] = x[j];
+            ref.g[pos] = gg[j];
+            ref.d[pos] = d[j];
+            ++pos;
+        }
+    }
 }
 
 __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(RecDev in, int32_t *h2,
@@ -203,40 +248,76 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(RecDev in, const Se
     const int32_t mark_val = mark ? *mark_val_ptr : 0;
     __shared__ BlockAppendLds s_app;
     const int64_t count = (int64_t)*in.cnt;
-    // Block-uniform loop: the appends below synchronise the block.
-    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < count;
-         b0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = b0 + threadIdx.x;
-        const bool valid = i < count;
-        int32_t x = 0, gg = 0, d = 0;
-        bool take = false;
-        if (valid) {
-            x = in.x[i];
-            gg = in.g[i];
-            d = in.d[i];
-            const SelState s = st[gg];
-            const int b = bin1_of(d, b1_bins);
+    const int64_t span = (int64_t)blockDim.x * kCompactItems;
+    // Block-uniform loop; each thread takes kCompactItems records (strided
+    // by the block size, so loads stay coalesced) and reserves its output
+    // slots with one block-level atomic per iteration.
+    for (int64_t b0 = (int64_t)blockIdx.x * span; b0 < count; b0 += (int64_t)gridDim.x * span) {
+        int32_t x[kCompactItems], gg[kCompactItems], d[kCompactItems];
+        uint32_t take_mask = 0, keep_mask = 0;
+        int32_t n_light = 0, n_keep = 0;
+        int64_t deg_sum = 0;
+#pragma unroll
+        for (int j = 0; j < kCompactItems; ++j) {
+            const int64_t i = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            x[j] = 0; gg[j] = 0; d[j] = 0;
+            if (i >= count) continue;
+            x[j] = in.x[i];
+            gg[j] = in.g[i];
+            d[j] = in.d[i];
+            const SelState s = st[gg[j]];
+            const int b = bin1_of(d[j], b1_bins);
             const bool earlier = s.desc ? b > s.b1 : b < s.b1;
+            bool take = false;
             switch (s.mode) {
                 case SEL_ALL: take = true; break;
                 case SEL_THROUGH: take = earlier || b == s.b1; break;
                 case SEL_KEY: {
                     if (earlier) { take = true; break; }
                     if (b != s.b1) break;
-                    const uint64_t K = sel_key(x, d, s.desc, id_bits, deg_bits);
+                    const uint64_t K = sel_key(x[j], d[j], s.desc, id_bits, deg_bits);
                     const uint64_t kmax = s.prefix | ((s.r >= 64) ? ~0ull : ((1ull << s.r) - 1));
                     take = K <= kmax;
                     break;
                 }
                 default: break;
             }
+            if (take) {
+                take_mask |= 1u << j;
+                deg_sum += d[j];
+                if (d[j] <= heavy_degree) ++n_light;
+                if (mark) mark[x[j]] = mark_val;
+            } else {
+                keep_mask |= 1u << j;
+                ++n_keep;
+            }
         }
-        block_list_append(sel, take, x, d, heavy_degree, chunk_edges, s_app);
-        if (mark && take) mark[x] = mark_val;
+        int64_t pos = block_reserve(n_light, &sel.cnt[LC_LIGHT], s_app, deg_sum, &sel.cnt[LC_DEG]);
+#pragma unroll
+        for (int j = 0; j < kCompactItems; ++j) {
+            if (!(take_mask & (1u << j))) continue;
+            if (d[j] <= heavy_degree) {
+                sel.light[pos++] = x[j];
+            } else {  // heavy rows are rare: one slot each
+                const int64_t slot = (int64_t)atomicAdd(&sel.cnt[LC_SLOTS], 1ull);
+                const int64_t nch = ((int64_t)d[j] + chunk_edges - 1) / chunk_edges;
+                const int64_t base = (int64_t)atomicAdd(&sel.cnt[LC_CHUNKS], (unsigned long long)nch);
+                sel.hv[slot] = x[j];
+                sel.hn[slot] = (int32_t)nch;
+                atomicAdd(&sel.cnt[LC_HDEG], (unsigned long long)d[j]);
+                for (int64_t c = 0; c < nch; ++c) sel.hc[base + c] = (slot << 32) | c;
+            }
+        }
         if (has_carry) {
-            const bool keep = valid && !take;
-            const int64_t pos = block_append(keep, carry.cnt, s_app);
-            if (keep) { carry.x[pos] = x; carry.g[pos] = gg; carry.d[pos] = d; }
+            int64_t cp = block_reserve(n_keep, carry.cnt, s_app);
+#pragma unroll
+            for (int j = 0; j < kCompactItems; ++j) {
+                if (!(keep_mask & (1u << j))) continue;
+                carry.x[cp] = x[j];
+                carry.g[cp] = gg[j];
+                carry.d[cp] = d[j];
+                ++cp;
+            }
         }
     }
 }
@@ -246,6 +327,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(RecDev in, const Se
 void SelectWork::prepare(dmolp_ctx *ctx, int ngroups, int64_t n, int32_t max_degree) {
     const int ib = (int)(bits_for(n > 1 ? n - 1 : 1));
     const int db = (int)(bits_for(max_degree > 1 ? max_degree : 1));
+    if (ref_cap < n) {
+        ref.alloc(n);
+        ref_cap = n;
+    }
     if (groups == ngroups && id_bits == ib && deg_bits == db) return;
     groups = ngroups;
     id_bits = ib;
@@ -274,9 +359,11 @@ void select_enqueue(dmolp_ctx *ctx, SelectWork &w, const GraphDev &g, const RecD
                        w.h1.as<int32_t>(), w.groups, w.b1_bins, done);
     hipLaunchKernelGGL(k_sel_pick1, dim3(w.groups), dim3(kSelThreads), 0, s, w.h1.as<int32_t>(),
                        w.b1_bins, w.st.as<SelState>(), rule, w.id_bits, w.deg_bits,
-                       w.refining.as<int32_t>(), sel, done);
+                       w.refining.as<int32_t>(), sel, w.ref.dev.cnt, done);
+    hipLaunchKernelGGL(k_sel_collect, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, in,
+                       w.st.as<SelState>(), w.b1_bins, w.ref.dev, w.refining.as<int32_t>(), done);
     for (int p = 0; p < w.passes; ++p) {
-        hipLaunchKernelGGL(k_sel_hist2, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, in,
+        hipLaunchKernelGGL(k_sel_hist2, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, w.ref.dev,
                            w.h2.as<int32_t>(), w.st.as<SelState>(), w.b1_bins, w.id_bits,
                            w.deg_bits, w.refining.as<int32_t>(), done);
         hipLaunchKernelGGL(k_sel_pick2, dim3(w.groups), dim3(kSelThreads), 0, s, w.h2.as<int32_t>(),

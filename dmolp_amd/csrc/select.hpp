@@ -70,6 +70,8 @@ struct SelectWork {
     int passes = 0;
     int hist1_blocks = 0;
     DevMem h1, h2, st, refining;
+    Recs ref;  // records still being refined after the level-1 pick
+    int64_t ref_cap = 0;
     void prepare(dmolp_ctx *ctx, int groups, int64_t n, int32_t max_degree);
 };
 
