@@ -1,13 +1,16 @@
 """Phase-2 label-propagation throughput on one MI355X.
 
 A step is one full phase-2 run (device-resident boundary LP rounds until an
-exit condition) over the graph's phase-1 labels, restored on the device
-before each step.  value = LP edges/sec = sum over rounds of sum_{b in B_t}
-deg b divided by device time (hipEvents) of the timed steps.
+exit condition) over the graph's phase-1 labels, which are restored on the
+device before each step.  value = LP edges/sec = sum over rounds of
+sum_{b in B_t} deg b, divided by the device time (hipEvents on the library's
+stream) of the timed steps.
 
-Default workload: R-MAT scale 22, edge factor 16, (0.57, 0.19, 0.19, 0.05),
-seed 2, k=16, theta=10000, 20 rounds, epsilon 0.03.  --config c3 runs the
-41.6M-vertex Chung-Lu graph (k=32, 30 rounds, epsilon 0.05).
+Default workload: the power-law follower graph -- Chung-Lu, 41,652,230
+vertices, 1,468,365,182 sampled directed edges, degree exponent 2.0, expected
+maximum degree 3e6, seed 3, symmetrised (~2.43e9 CSR entries); k=32,
+theta=10000, 30 rounds, epsilon 0.05.  --config c2 runs R-MAT scale 22
+(k=16, 20 rounds), --config c1 R-MAT scale 16 (k=8, 10 rounds).
 """
 from __future__ import annotations
 
@@ -32,6 +35,10 @@ CONFIGS = {
                gen=("chunglu", 41652230, 1468365182, 3), k=32, rounds=30, eps=0.05),
 }
 THETA = 10000
+# Algorithmic bytes of the LP decision per boundary vertex b (int64 offsets,
+# int32 columns and labels): 2 offsets + own label read + label write, and a
+# column plus a label per neighbour.
+O_BYTES, C_BYTES, L_BYTES = 8, 4, 4
 
 
 def spec_for(cfg):
@@ -46,27 +53,35 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(off, col, k, labels0, eps, budget_s=20.0):
-    """The oracle's phase 2 on the same graph and start labels, run for as
-    many whole rounds as fit a ~20 s budget (single thread)."""
+def cpu_baseline(off, col, k, labels0, eps, rounds_cap):
+    """The CPU oracle's phase 2 (single thread) on the same graph and start
+    labels, for the first rounds_cap round(s)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_py as O
-    rounds, elapsed, res = 0, 0.0, None
-    for r in (1, 2, 3, 5, 8, 12, 20):
-        t0 = time.perf_counter()
-        res = O.phase2(off, col, k, labels0, r, eps)
-        elapsed = time.perf_counter() - t0
-        rounds = r
-        if elapsed * 1.6 > budget_s or res.m.rounds < r:
-            break
+    t0 = time.perf_counter()
+    res = O.phase2(off, col, k, labels0, rounds_cap, eps)
+    elapsed = time.perf_counter() - t0
     return {
         "value": res.m.lp_edges / elapsed,
         "unit": "LP edges/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"oracle phase 2, first {res.m.rounds} round(s) of the same graph and "
-                  f"phase-1 labels ({res.m.lp_edges} LP edges in {elapsed:.2f} s)",
+        "sample": f"oracle phase 2 (stats sweep + {res.m.rounds} round(s) + final stats) on the "
+                  f"same graph and phase-1 labels: {res.m.lp_edges} LP edges in {elapsed:.1f} s",
     }
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if one exists."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d[workload][kernel]["hbm_bytes_per_launch"]
+    except (KeyError, ValueError):
+        return None
 
 
 def main(argv=None):
@@ -74,13 +89,14 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default=os.environ.get("DMOLP_BENCH_CONFIG", "c2"),
+    ap.add_argument("--config", default=os.environ.get("DMOLP_BENCH_CONFIG", "c3"),
                     choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed "
-                                   "but not built (see DESIGN.md)", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; how the path would shard over "
+                                   "several is designed in DESIGN.md but not built",
+                          "n_gpus": a.gpus}))
         return 2
     from dmolp_amd import api
     cfg = CONFIGS[a.config]
@@ -98,50 +114,56 @@ def main(argv=None):
         f"cut {p1.metrics.initial_edge_cut}")
     g.checkpoint_labels()
 
-    def step():
-        g.checkpoint_labels(restore=True)
-        return g.phase2(k, R, eps, want_labels=False, want_trace=False)
-
     for _ in range(a.warmup):
-        step()
-    ctx_dev_ms, lp_edges, walls, res = 0.0, 0, [], None
+        g.checkpoint_labels(restore=True)
+        g.phase2(k, R, eps, want_labels=False, want_trace=False)
+    dev_ms, lp_edges, walls, res = 0.0, 0, [], None
     for _ in range(a.steps):
         g.checkpoint_labels(restore=True)
         t0 = time.perf_counter()
         res = g.phase2(k, R, eps, want_labels=False, want_trace=False)
         walls.append(time.perf_counter() - t0)
-        ctx_dev_ms += res.metrics.device_ms
+        dev_ms += res.metrics.device_ms
         lp_edges += res.metrics.lp_edges
-    value = lp_edges / (ctx_dev_ms / 1e3)
+    value = lp_edges / (dev_ms / 1e3)
     log(f"[bench] phase 2: {a.steps} steps, cut {res.metrics.initial_cut} -> "
         f"{res.metrics.edge_cut}, rounds {res.metrics.rounds}, "
-        f"device {ctx_dev_ms / a.steps:.2f} ms/step, wall {1e3 * np.mean(walls):.2f} ms/step")
+        f"device {dev_ms / a.steps:.2f} ms/step, wall {1e3 * np.mean(walls):.2f} ms/step")
 
-    # Roofline of the dominant kernel (light-row LP decision), HIP events on
-    # the context stream, one profiled step.
+    # One profiled step: HIP events on the library's stream around each
+    # kernel group of every round (kernels launched directly, not replayed).
     ctx.set_profiling(True)
     g.checkpoint_labels(restore=True)
     pr = g.phase2(k, R, eps, want_labels=False, want_trace=True)
     ctx.set_profiling(False)
-    lp_ms = ctx.round_times(0)
+    names = ["lp_light", "lp_heavy", "candidates", "select", "delta_apply_end", "round"]
+    per_round = {nm: ctx.round_times(i) for i, nm in enumerate(names)}
+    groups = {nm: float(v.sum()) for nm, v in per_round.items()}
     tr = pr.trace
-    o, c, lab = 8, 4, 4  # int64 offsets, int32 columns and labels
-    bytes_per_round = (2 * o + 2 * lab) * tr["b_light_count"] + (c + lab) * tr["b_light_deg"]
-    achieved = float(bytes_per_round.sum()) / (float(lp_ms.sum()) / 1e3) / 1e9
-    round_ms = ctx.round_times(5)
-    groups = {name: float(ctx.round_times(i).sum()) for i, name in enumerate(
-        ["lp_light", "lp_heavy", "candidates", "select", "delta_apply_end", "round"])}
-    log(f"[bench] profiled step: groups ms {json.dumps({k_: round(v, 3) for k_, v in groups.items()})}")
+    light_n, light_deg = tr["b_light_count"], tr["b_light_deg"]
+    heavy_n, heavy_deg = tr["b_count"] - light_n, tr["b_deg"] - light_deg
+    unit = 2 * O_BYTES + 2 * L_BYTES
+    kern = {
+        "k_lp_light": (per_round["lp_light"], unit * light_n + (C_BYTES + L_BYTES) * light_deg),
+        "k_lp_heavy": (per_round["lp_heavy"], unit * heavy_n + (C_BYTES + L_BYTES) * heavy_deg),
+    }
+    dom = max(kern, key=lambda kk: float(kern[kk][0].sum()))
+    ms_r, bytes_r = kern[dom]
+    live = bytes_r > 0
+    achieved = float(bytes_r.sum()) / (float(ms_r.sum()) / 1e3) / 1e9
+    log(f"[bench] profiled step: groups ms "
+        f"{json.dumps({kk: round(v, 3) for kk, v in groups.items()})}; dominant {dom}")
 
     cpu = None
     if not a.no_cpu_baseline:
         off, col = g.download()
         g.checkpoint_labels(restore=True)
         labels0 = g.get_labels()
-        cpu = cpu_baseline(off, col, k, labels0, eps)
+        cpu = cpu_baseline(off, col, k, labels0, eps, 1 if a.config == "c3" else 3)
         log(f"[bench] cpu baseline {cpu['value']:.3e} LP edges/s ({cpu['sample']})")
         del off, col
 
+    traffic = measured_traffic(cfg["workload"], dom)
     out = {
         "metric": "LP edges/sec (phase-2 boundary label propagation)",
         "value": value,
@@ -158,19 +180,20 @@ def main(argv=None):
         "config": {"workload": cfg["workload"], "n": g.n, "nnz": g.nnz, "k": k, "theta": THETA,
                    "max_rounds": R, "epsilon": eps, "rounds_run": int(res.metrics.rounds),
                    "exit_reason": api.EXIT_NAMES[res.metrics.exit_reason],
-                   "initial_cut": int(res.metrics.initial_cut), "final_cut": int(res.metrics.edge_cut),
+                   "initial_cut": int(res.metrics.initial_cut),
+                   "final_cut": int(res.metrics.edge_cut),
                    "lp_edges_per_step": int(res.metrics.lp_edges),
-                   "phase2_device_ms": ctx_dev_ms / a.steps,
+                   "phase2_device_ms": dev_ms / a.steps,
                    "phase1_device_ms": p1.metrics.device_ms, "phase1_wall_s": p1_wall,
                    "phase1_rounds": int(p1.metrics.rounds)},
-        "roofline": {"bound": "hbm", "kernel": "k_lp_light", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
-                     "avg_launch_ms": float(lp_ms.mean()) if len(lp_ms) else None,
-                     "launches": int(len(lp_ms))},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic,
+                     "algorithmic_bytes_per_launch": float(bytes_r[live].mean()),
+                     "avg_launch_ms": float(ms_r[live].mean()),
+                     "launches": int(live.sum())},
         "cpu_baseline": cpu,
         "kernel_groups_ms": groups,
-        "round_ms": [round(float(x), 4) for x in round_ms],
     }
     print(json.dumps(out))
     g.free()
