@@ -71,6 +71,108 @@ def cpu_baseline(off, col, k, labels0, eps, rounds_cap):
     }
 
 
+C4_WORKLOAD = "batch256-rmat-s10to18-grid64to1024-seed4"
+
+
+def c4_specs(seed=4, count=256):
+    """256 jobs: half R-MAT (scale uniform in 10..18, edge factor 16), half 2-D
+    5-point grids (sides log-uniform in 64..1024); k from {2,4,8,16,64}."""
+    from dmolp_amd import api
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(count):
+        k = int(rng.choice([2, 4, 8, 16, 64]))
+        if i % 2 == 0:
+            jobs.append((api.rmat_spec(int(rng.integers(10, 19)), 16, seed=1000 + i), k))
+        else:
+            r, c = (int(round(x)) for x in np.exp(rng.uniform(np.log(64), np.log(1024), 2)))
+            jobs.append((api.grid_spec(r, c), k))
+    return jobs
+
+
+def run_batched(a):
+    """Batched mode: every graph's phase 1 + 10 LP rounds in one job set,
+    against the same graphs partitioned one at a time (plain mode)."""
+    from dmolp_amd import api
+    theta, rounds, eps = THETA, 10, 0.03
+    ctx = api.Context(0)
+    specs = c4_specs()
+    t0 = time.perf_counter()
+    graphs = [ctx.generate(sp) for sp, _ in specs]
+    n_tot = sum(g.n for g in graphs)
+    nnz_tot = sum(g.nnz for g in graphs)
+    log(f"[bench] generated {len(graphs)} graphs: n={n_tot} nnz={nnz_tot} in "
+        f"{time.perf_counter() - t0:.2f} s")
+    jobs = [api.BatchJob(g, k, theta, rounds, eps) for g, (_, k) in zip(graphs, specs)]
+    # Plain: one graph at a time (phase 1 + phase 2 each), after one warm pass.
+    plain_walls, plain_res = [], None
+    for it in range(1 + a.warmup):
+        t0 = time.perf_counter()
+        res = []
+        for j in jobs:
+            j.graph.phase1(j.k, j.theta, want_labels=False)
+            res.append(j.graph.phase2(j.k, j.max_rounds, j.epsilon, want_labels=True))
+        plain_walls.append(time.perf_counter() - t0)
+        plain_res = res
+    plain_s = min(plain_walls[1:] if len(plain_walls) > 1 else plain_walls)
+    log(f"[bench] plain: {len(jobs)} graphs in {plain_s:.3f} s")
+    for _ in range(a.warmup):
+        api.batch_run(ctx, jobs, want_labels=False)
+    walls, dev = [], []
+    bres = None
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        bres, ms = api.batch_run(ctx, jobs, want_labels=True)
+        walls.append(time.perf_counter() - t0)
+        dev.append(ms)
+    batch_s = float(np.mean(walls))
+    same_cut = all(b.edge_cut == p.metrics.edge_cut for b, p in zip(bres, plain_res))
+    same_labels = all(np.array_equal(b.labels, p.labels) for b, p in zip(bres, plain_res))
+    log(f"[bench] batched: {len(jobs)} graphs in {batch_s:.3f} s (device {np.mean(dev):.1f} ms), "
+        f"speedup {plain_s / batch_s:.2f}x, identical cut {same_cut}, identical labels {same_labels}")
+    cpu = None
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_py as O
+        sample = list(range(0, 16))
+        t0 = time.perf_counter()
+        for i in sample:
+            off, col = api.gen_host(specs[i][0])
+            p1 = O.phase1(off, col, specs[i][1], theta)
+            O.phase2(off, col, specs[i][1], p1.labels, rounds, eps)
+        el = time.perf_counter() - t0
+        cpu = {"value": len(sample) / el, "unit": "graphs/s", "cores": 1, "kind": "port",
+               "sample": f"oracle phase 1 + phase 2 on the first {len(sample)} jobs of the same "
+                         f"batch (generation excluded): {el:.1f} s"}
+    out = {
+        "metric": "batched graphs/sec (phase 1 + 10 LP rounds per graph)",
+        "value": len(jobs) / batch_s,
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * batch_s,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (project generator, seeded)",
+        "config": {"workload": C4_WORKLOAD, "graphs": len(jobs), "n_total": n_tot,
+                   "nnz_total": nnz_tot, "theta": theta, "max_rounds": rounds, "epsilon": eps},
+        "plain_graphs_per_s": len(jobs) / plain_s,
+        "speedup_vs_plain": plain_s / batch_s,
+        "identical_cut": same_cut,
+        "identical_labels": same_labels,
+        "batch_device_ms": float(np.mean(dev)),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    for g in graphs:
+        g.free()
+    ctx.close()
+    return out
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if one exists."""
@@ -90,7 +192,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default=os.environ.get("DMOLP_BENCH_CONFIG", "c3"),
-                    choices=sorted(CONFIGS))
+                    choices=sorted(CONFIGS) + ["c4"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -98,6 +200,9 @@ def main(argv=None):
                                    "several is designed in DESIGN.md but not built",
                           "n_gpus": a.gpus}))
         return 2
+    if a.config == "c4":
+        print(json.dumps(run_batched(a)))
+        return 0
     from dmolp_amd import api
     cfg = CONFIGS[a.config]
     k, R, eps = cfg["k"], cfg["rounds"], cfg["eps"]

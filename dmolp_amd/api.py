@@ -123,9 +123,11 @@ class Context:
         _check(_abi.lib().dmolp_ctx_get_tuning(self._h, C.byref(t)), self._h)
         return t
 
-    def set_tuning(self, heavy_degree=0, chunk_edges=0, grid_blocks=0, pull_per_mille=0):
+    def set_tuning(self, heavy_degree=0, chunk_edges=0, grid_blocks=0, pull_per_mille=0,
+                   batch_work_per_block=0):
         t = _abi.Tuning(heavy_degree=heavy_degree, chunk_edges=chunk_edges, block_threads=0,
-                        grid_blocks=grid_blocks, pull_per_mille=pull_per_mille)
+                        grid_blocks=grid_blocks, pull_per_mille=pull_per_mille,
+                        batch_work_per_block=batch_work_per_block)
         _check(_abi.lib().dmolp_ctx_set_tuning(self._h, C.byref(t)), self._h)
 
     def set_profiling(self, on: bool):
@@ -165,6 +167,52 @@ class Context:
         _check(_abi.lib().dmolp_penalties(self._h, k, _p(vc, C.c_int64), _p(ic, C.c_int64), t,
                                           _p(pen, C.c_double), _p(re, C.c_double)), self._h)
         return pen, re
+
+
+@dataclass
+class BatchJob:
+    graph: "Graph"
+    k: int
+    theta: int
+    max_rounds: int
+    epsilon: float
+
+
+@dataclass
+class BatchResult:
+    status: int
+    rounds: int
+    exit_reason: int
+    n_seeds_fallback: int
+    phase1_cut: int
+    edge_cut: int
+    lp_edges: int
+    vertex_balance: float
+    edge_balance: float
+    labels: np.ndarray | None
+
+
+def batch_run(ctx: "Context", jobs, want_labels=False):
+    """Partition every job's graph (phase 1 + phase 2) in one job set.
+    Returns (results, device_ms)."""
+    n = len(jobs)
+    arr = (_abi.Job * max(n, 1))()
+    outs = []
+    for i, j in enumerate(jobs):
+        lab = np.zeros(j.graph.n, np.int32) if want_labels else None
+        outs.append(lab)
+        arr[i] = _abi.Job(graph=j.graph.handle, k=j.k, theta=j.theta, max_rounds=j.max_rounds,
+                          reserved=0, epsilon=j.epsilon,
+                          labels_out=_p(lab, C.c_int32) if lab is not None else None)
+    res = (_abi.Result * max(n, 1))()
+    ms = C.c_double()
+    _check(_abi.lib().dmolp_batch_run(ctx.handle, n, arr, res, C.byref(ms)), ctx.handle)
+    out = []
+    for i in range(n):
+        r = res[i]
+        out.append(BatchResult(r.status, r.rounds, r.exit_reason, r.n_seeds_fallback, r.phase1_cut,
+                               r.edge_cut, r.lp_edges, r.vertex_balance, r.edge_balance, outs[i]))
+    return out, ms.value
 
 
 class Graph:

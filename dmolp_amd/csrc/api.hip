@@ -15,6 +15,7 @@
 #include "internal.hpp"
 #include "phase1.hpp"
 #include "phase2.hpp"
+#include "batch.hpp"
 #include "rows.hpp"
 
 namespace dmolp {
@@ -119,7 +120,7 @@ void get_labels(dmolp_ctx *ctx, const dmolp_graph *g, int32_t *labels) {
 int stats_sweep(dmolp_ctx *ctx, dmolp_graph *g, int k, std::vector<int64_t> &vc,
                 std::vector<int64_t> &ic, int64_t &cut) {
     Phase2Work &w = phase2_work(ctx, g, k);
-    phase2_init_enqueue(ctx, g, w);
+    phase2_init_enqueue(ctx, phase2_single_set(ctx, g, w, 0, 0.0));
     Ctl c{};
     vc.assign((size_t)k, 0);
     ic.assign((size_t)k, 0);
@@ -241,6 +242,7 @@ int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out) {
     out->block_threads = ctx->tune.block_threads;
     out->grid_blocks = ctx->grid_blocks();
     out->pull_per_mille = (int32_t)(ctx->tune.pull_fraction * 1000.0 + 0.5);
+    out->batch_work_per_block = (int32_t)ctx->tune.batch_work_per_block;
     return DMOLP_OK;
 }
 
@@ -249,12 +251,14 @@ int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
     // Light rows keep 16-bit per-row counters: heavy_degree stays below 2^16.
     if (t->heavy_degree < 0 || t->heavy_degree > 65535 || t->chunk_edges < 0 ||
         t->grid_blocks < 0 || t->pull_per_mille < 0 || t->pull_per_mille > 1000000 ||
+        t->batch_work_per_block < 0 ||
         (t->block_threads != 0 && t->block_threads != 256))
         return fail(ctx, DMOLP_E_ARG, "tuning value out of range");
     if (t->heavy_degree) ctx->tune.heavy_degree = t->heavy_degree;
     if (t->chunk_edges) ctx->tune.chunk_edges = t->chunk_edges < 64 ? 64 : t->chunk_edges;
     if (t->grid_blocks) ctx->tune.grid_blocks = t->grid_blocks;
     if (t->pull_per_mille > 0) ctx->tune.pull_fraction = t->pull_per_mille / 1000.0;
+    if (t->batch_work_per_block > 0) ctx->tune.batch_work_per_block = t->batch_work_per_block;
     return DMOLP_OK;
 }
 
@@ -475,12 +479,14 @@ int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, 
         ctx->prof_marks.clear();
         ctx->prof_ms.clear();
         phase2_round_table(ctx, w, max_rounds);
+        const P2Set set = phase2_single_set(ctx, g, w, max_rounds, epsilon);
+        const uint64_t key = phase2_single_key(ctx, g, w);
         DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, s));
-        phase2_init_enqueue(ctx, g, w, max_rounds, epsilon);
+        phase2_init_enqueue(ctx, set);
         Ctl c{};
         const int batch = 8;
         for (int t = 1; t <= max_rounds; ++t) {
-            phase2_round_launch(ctx, g, w);
+            phase2_round_launch(ctx, set, &w.round_exec, &w.round_key, key);
             if (t % batch == 0 && t < max_rounds) {
                 DMOLP_HIP_CHECK(hipMemcpyAsync(&c, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
                 DMOLP_HIP_CHECK(hipStreamSynchronize(s));
@@ -602,7 +608,7 @@ int dmolp_select_boundary(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels,
     return guarded(ctx, [&] {
         set_labels(ctx, g, labels);
         Phase2Work &w = phase2_work(ctx, g, k);
-        phase2_init_enqueue(ctx, g, w);  // differing-neighbour counts
+        phase2_init_enqueue(ctx, phase2_single_set(ctx, g, w, 0, 0.0));  // differing-neighbour counts
         Ctl c{};
         DMOLP_HIP_CHECK(hipMemcpyAsync(&c, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
         DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -636,6 +642,13 @@ int dmolp_lp_round(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels, const 
         phase2_lp_enqueue(ctx, g, w);
         get_labels(ctx, g, labels_out);
     });
+}
+
+int dmolp_batch_run(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, dmolp_result *results,
+                    double *device_ms) {
+    if (!ctx || n_jobs < 0 || (n_jobs > 0 && (!jobs || !results)))
+        return fail(ctx, DMOLP_E_ARG, "null argument");
+    return guarded(ctx, [&] { batch_run(ctx, n_jobs, jobs, results, device_ms); });
 }
 
 }  // extern "C"

@@ -77,4 +77,40 @@ __device__ __forceinline__ int64_t wave_count() {
     return ((int64_t)gridDim.x * blockDim.x) / kWave;
 }
 
+// One launch serves a set of jobs (graphs): job j owns the blocks
+// [bstart[j], bstart[j+1]).  A kernel body sees its job's share of the
+// launch as a virtual grid, so the same body runs one graph or many.
+struct VGrid {
+    int64_t block;   // block index within the job
+    int64_t blocks;  // blocks of the job
+};
+
+// The job owning this block (largest j with bstart[j] <= blockIdx.x),
+// made wave-uniform so the job's fields load through the scalar unit.
+__device__ __forceinline__ int job_of_block(const int32_t *bstart, int jobs) {
+    int lo = 0, hi = jobs - 1;
+    const int b = (int)blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+__device__ __forceinline__ VGrid vgrid_of(const int32_t *bstart, int j) {
+    const int32_t b0 = bstart[j], b1 = bstart[j + 1];
+    return VGrid{(int64_t)blockIdx.x - b0, (int64_t)(b1 - b0)};
+}
+__device__ __forceinline__ int64_t vwave(const VGrid &v) {
+    return v.block * (int64_t)(blockDim.x / kWave) + (int64_t)(threadIdx.x / kWave);
+}
+__device__ __forceinline__ int64_t vwaves(const VGrid &v) {
+    return v.blocks * (int64_t)(blockDim.x / kWave);
+}
+__device__ __forceinline__ int64_t vthread(const VGrid &v) {
+    return v.block * (int64_t)blockDim.x + (int64_t)threadIdx.x;
+}
+__device__ __forceinline__ int64_t vthreads(const VGrid &v) {
+    return v.blocks * (int64_t)blockDim.x;
+}
+
 }  // namespace dmolp

@@ -56,6 +56,9 @@ struct DevMem {
         DMOLP_HIP_CHECK(hipMalloc(&p, b));
         bytes = b;
     }
+    void grow(size_t b) {
+        if (!p || bytes < b) alloc(b);
+    }
     void release() {
         if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
     }
@@ -95,6 +98,7 @@ struct Tuning {
     int block_threads = 256;
     int grid_blocks = 0;  // 0: derived from the CU count
     double pull_fraction = 0.05;  // pull candidates when B_{t-1} covers this share of entries
+    int64_t batch_work_per_block = 32768;  // batched launches: vertices + nnz/4 per block
 };
 
 struct GraphDev {
@@ -107,6 +111,38 @@ struct GraphDev {
 // Per-graph phase-2 state, allocated on first use for a given k.
 struct Phase2Work;
 struct Phase1Work;
+
+// How many blocks of a multi-job launch each job gets.
+enum LaunchPolicy {
+    P_GRID = 0,  // grid-stride kernels over vertices or edges
+    P_HIST1,     // level-1 selection histogram
+    P_GROUPS,    // one block per selection group
+    P_ONE,       // one block per job
+    P_HEAVY,     // one thread per heavy row
+    P_NPOL
+};
+
+// Device block-start arrays of a job set, one per launch policy.
+struct LaunchSet {
+    int jobs = 0;
+    int total[P_NPOL] = {};
+    DevMem bstart;  // int32 [P_NPOL][jobs + 1]
+    const int32_t *bs(int pol) const { return bstart.as<int32_t>() + (size_t)pol * (jobs + 1); }
+};
+
+// Device records and the captured round of the last phase-2 job set.
+struct BatchState {
+    DevMem jobs, sel, remaining;
+    LaunchSet ls;
+    hipGraphExec_t exec = nullptr;
+    uint64_t exec_key = 0;
+    BatchState() = default;
+    BatchState(const BatchState &) = delete;
+    BatchState &operator=(const BatchState &) = delete;
+    ~BatchState() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
 
 }  // namespace dmolp
 
@@ -127,6 +163,10 @@ struct dmolp_ctx {
     std::vector<std::vector<double>> prof_ms;                         // [group] -> ms per round
     hipEvent_t prof_event();
     int grid_blocks() const { return tune.grid_blocks > 0 ? tune.grid_blocks : cu_count * 8; }
+    // Device records of the last phase-1 job set.
+    dmolp::DevMem p1_remaining, p1_jobs, p1_sel;
+    dmolp::LaunchSet p1_ls;
+    dmolp::BatchState batch;  // phase-2 job set of dmolp_batch_run
 };
 
 struct dmolp_graph {

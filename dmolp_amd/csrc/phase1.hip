@@ -16,6 +16,8 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <array>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -33,6 +35,15 @@ constexpr int32_t kNoClaim = 0x7f7f7f7f;  // claims are reset with a 0x7f byte f
 
 __device__ __forceinline__ bool p1_done(const P1Ctl *c) { return c->done != 0; }
 
+// Every kernel takes the job set and runs on its job's share of the launch
+// (see VGrid): one graph, or many graphs in one launch.
+#define P1_JOB()                                   \
+    const int job_ = job_of_block(bstart, jobs);   \
+    const VGrid vg = vgrid_of(bstart, job_);       \
+    const P1Job &J = JS[job_];                     \
+    (void)vg
+
+
 __device__ __forceinline__ void mark_bit(uint32_t *bits, int32_t v) {
     const uint32_t m = 1u << (v & 31);
     if (!(bits[v >> 5] & m)) atomicOr(&bits[v >> 5], m);
@@ -41,14 +52,23 @@ __device__ __forceinline__ void mark_bit(uint32_t *bits, int32_t v) {
 // ---------------------------------------------------------------- seeds
 
 // Level 0 and 1 of the ball of seeds[j] (seed.cpp:26-29, 39-95).
-__global__ void k_ball1(GraphDev g, const int32_t *seeds, int j, uint32_t *covered, ListDev lvl1,
-                        int heavy_degree, int chunk_edges, const P1Ctl *ctl) {
+__global__ void k_ball1(const P1Job *JS, const int32_t *bstart, int jobs, int step) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    const int32_t *seeds = J.seeds;
+    uint32_t *covered = J.covered;
+    const ListDev lvl1 = J.lvl1;
+    const int heavy_degree = J.heavy_degree;
+    const int chunk_edges = J.chunk_edges;
+    const P1Ctl *ctl = J.ctl;
+    if (step >= J.seed_steps) return;
+    const int j = step - 1;
     if (ctl->seed_stop) return;
     const int32_t s = seeds[j];
     if (s < 0) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) mark_bit(covered, s);
+    if (vg.block == 0 && threadIdx.x == 0) mark_bit(covered, s);
     const int64_t e0 = g.off[s], e1 = g.off[s + 1];
-    for (int64_t b = e0 + (global_wave() * kWave); b < e1; b += wave_count() * kWave) {
+    for (int64_t b = e0 + (vwave(vg) * kWave); b < e1; b += vwaves(vg) * kWave) {
         const int64_t e = b + lane_id();
         const bool on = e < e1;
         int32_t u = 0;
@@ -63,13 +83,18 @@ __global__ void k_ball1(GraphDev g, const int32_t *seeds, int j, uint32_t *cover
 }
 
 // Level 2: every neighbour of a level-1 vertex.
-__global__ __launch_bounds__(kThreads) void k_ball2_light(GraphDev g, ListDev lvl1,
-                                                          uint32_t *covered, const P1Ctl *ctl) {
+__global__ __launch_bounds__(kThreads) void k_ball2_light(const P1Job *JS, const int32_t *bstart, int jobs, int step) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    const ListDev lvl1 = J.lvl1;
+    uint32_t *covered = J.covered;
+    const P1Ctl *ctl = J.ctl;
+    if (step >= J.seed_steps) return;
     if (ctl->seed_stop) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
     const int w = threadIdx.x / kWave, lane = lane_id();
     const int64_t count = (int64_t)lvl1.cnt[LC_LIGHT];
-    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+    for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, lvl1.light, count, b0, s_b[w], v, deg, valid);
@@ -81,11 +106,17 @@ __global__ __launch_bounds__(kThreads) void k_ball2_light(GraphDev g, ListDev lv
     }
 }
 
-__global__ void k_ball2_heavy(GraphDev g, ListDev lvl1, int chunk_edges, uint32_t *covered,
-                              const P1Ctl *ctl) {
+__global__ void k_ball2_heavy(const P1Job *JS, const int32_t *bstart, int jobs, int step) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    const ListDev lvl1 = J.lvl1;
+    const int chunk_edges = J.chunk_edges;
+    uint32_t *covered = J.covered;
+    const P1Ctl *ctl = J.ctl;
+    if (step >= J.seed_steps) return;
     if (ctl->seed_stop) return;
     const int64_t nch = (int64_t)lvl1.cnt[LC_CHUNKS];
-    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         chunk_bounds(g, lvl1, lvl1.hc[j], chunk_edges, slot, v, e0, e1, deg);
@@ -95,12 +126,20 @@ __global__ void k_ball2_heavy(GraphDev g, ListDev lvl1, int chunk_edges, uint32_
 
 // seed.cpp:158-170: over unused, uncovered hubs, the largest degree, first
 // in hub order; key = degree << 32 | ~index.
-__global__ void k_next_seed(GraphDev g, const int32_t *hubs, int64_t nh, const uint8_t *used,
-                            const uint32_t *covered, unsigned long long *best, const P1Ctl *ctl) {
+__global__ void k_next_seed(const P1Job *JS, const int32_t *bstart, int jobs, int step) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    const int32_t *hubs = J.hubs;
+    const int64_t nh = J.nh;
+    const uint8_t *used = J.used;
+    const uint32_t *covered = J.covered;
+    unsigned long long *best = J.best;
+    const P1Ctl *ctl = J.ctl;
+    if (step >= J.seed_steps) return;
     if (ctl->seed_stop) return;
     unsigned long long mine = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = vthread(vg); i < nh;
+         i += vthreads(vg)) {
         if (used[i]) continue;
         const int32_t h = hubs[i];
         if (covered[h >> 5] & (1u << (h & 31))) continue;
@@ -116,8 +155,16 @@ __global__ void k_next_seed(GraphDev g, const int32_t *hubs, int64_t nh, const u
     if (lane_id() == 0 && mine) atomicMax(best, mine);
 }
 
-__global__ void k_take_seed(unsigned long long *best, const int32_t *hubs, uint8_t *used,
-                            int32_t *seeds, int j, P1Ctl *ctl, ListDev lvl1) {
+__global__ void k_take_seed(const P1Job *JS, const int32_t *bstart, int jobs, int step) {
+    P1_JOB();
+    unsigned long long *best = J.best;
+    const int32_t *hubs = J.hubs;
+    uint8_t *used = J.used;
+    int32_t *seeds = J.seeds;
+    P1Ctl *ctl = J.ctl;
+    const ListDev lvl1 = J.lvl1;
+    if (step >= J.seed_steps) return;
+    const int j = step;
     if (ctl->seed_stop) return;
     for (int i = 0; i < LC_N; ++i) lvl1.cnt[i] = 0;
     const unsigned long long b = *best;
@@ -135,10 +182,16 @@ __global__ void k_take_seed(unsigned long long *best, const int32_t *hubs, uint8
 
 // ---------------------------------------------------------------- expansion
 
-__global__ void k_expand_init(GraphDev g, const int32_t *seeds, int k, int32_t *L, RecDev f,
-                              unsigned long long *sizes, P1Ctl *ctl) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < k) {
+__global__ void k_expand_init(const P1Job *JS, const int32_t *bstart, int jobs) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    const int32_t *seeds = J.seeds;
+    const int k = J.k;
+    int32_t *L = J.L;
+    const RecDev f = J.F[0];
+    unsigned long long *sizes = J.sizes;
+    P1Ctl *ctl = J.ctl;
+    for (int p = threadIdx.x; p < k; p += blockDim.x) {
         const int32_t s = seeds[p];
         L[s] = p;  // partition.cpp:346-352
         f.x[p] = s;
@@ -146,22 +199,34 @@ __global__ void k_expand_init(GraphDev g, const int32_t *seeds, int k, int32_t *
         f.d[p] = (int32_t)(g.off[s + 1] - g.off[s]);
         sizes[p] = 1;
     }
-    if (p == 0) {
+    if (threadIdx.x == 0) {
         *f.cnt = (unsigned long long)k;
         ctl->labelled = k;
         ctl->rounds = 0;
         ctl->fill_round = -1;
         ctl->done = (int64_t)k >= g.n ? 1 : 0;
+        if (ctl->done) atomicSub(J.remaining, 1);
     }
 }
 
 // Round start: stop when everything is labelled (partition.cpp:371); when
 // every frontier is empty pick the fill partition (partition.cpp:376-387).
-__global__ void k_p1_round_begin(P1Ctl *ctl, RecDev fcur, RecDev fnext, ListDev sel,
-                                 const unsigned long long *sizes, int k, int64_t n,
-                                 int64_t *touched_cnt) {
+__global__ void k_p1_round_begin(const P1Job *JS, const int32_t *bstart, int jobs, int parity) {
+    P1_JOB();
+    P1Ctl *ctl = J.ctl;
+    const RecDev fcur = J.F[parity];
+    const RecDev fnext = J.F[parity ^ 1];
+    const ListDev sel = J.sel_list;
+    const unsigned long long *sizes = J.sizes;
+    const int k = J.k;
+    const int64_t n = J.g.n;
+    int64_t *touched_cnt = J.touched_cnt;
     if (p1_done(ctl)) return;
-    if (ctl->labelled >= n) { ctl->done = 1; return; }
+    if (ctl->labelled >= n) {
+        ctl->done = 1;
+        atomicSub(J.remaining, 1);
+        return;
+    }
     *fnext.cnt = 0;
     for (int i = 0; i < LC_N; ++i) sel.cnt[i] = 0;
     *touched_cnt = 0;
@@ -177,13 +242,18 @@ __global__ void k_p1_round_begin(P1Ctl *ctl, RecDev fcur, RecDev fnext, ListDev 
 
 // partition.cpp:379-392: every unlabelled vertex joins the fill partition
 // and its frontier.
-__global__ void k_p1_fill(GraphDev g, int32_t *L, RecDev fcur, unsigned long long *sizes,
-                          P1Ctl *ctl) {
+__global__ void k_p1_fill(const P1Job *JS, const int32_t *bstart, int jobs, int parity) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    int32_t *L = J.L;
+    const RecDev fcur = J.F[parity];
+    unsigned long long *sizes = J.sizes;
+    P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
     const int b = ctl->fill_part;
     if (b < 0) return;
     int64_t added = 0;
-    for (int64_t b0 = global_wave() * kWave; b0 < g.n; b0 += wave_count() * kWave) {
+    for (int64_t b0 = vwave(vg) * kWave; b0 < g.n; b0 += vwaves(vg) * kWave) {
         const int64_t v = b0 + lane_id();
         const bool un = v < g.n && L[v] == -1;
         const int64_t pos = wave_append(un, fcur.cnt);
@@ -213,14 +283,20 @@ __device__ __forceinline__ void claim(int32_t v, int32_t p, int32_t *claims, int
 }
 
 // partition.cpp:451-476: claims by the selected frontier members.
-__global__ __launch_bounds__(kThreads) void k_expand_light(GraphDev g, ListDev sel, const int32_t *L,
-                                                           int32_t *claims, int32_t *touched,
-                                                           int64_t *touched_cnt, const P1Ctl *ctl) {
+__global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, const int32_t *bstart, int jobs) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    const ListDev sel = J.sel_list;
+    const int32_t *L = J.L;
+    int32_t *claims = J.claims;
+    int32_t *touched = J.touched;
+    int64_t *touched_cnt = J.touched_cnt;
+    const P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
     const int w = threadIdx.x / kWave, lane = lane_id();
     const int64_t count = (int64_t)sel.cnt[LC_LIGHT];
-    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+    for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, sel.light, count, b0, s_b[w], v, deg, valid);
@@ -241,12 +317,19 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(GraphDev g, ListDev s
     }
 }
 
-__global__ void k_expand_heavy(GraphDev g, ListDev sel, int chunk_edges, const int32_t *L,
-                               int32_t *claims, int32_t *touched, int64_t *touched_cnt,
-                               const P1Ctl *ctl) {
+__global__ void k_expand_heavy(const P1Job *JS, const int32_t *bstart, int jobs) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    const ListDev sel = J.sel_list;
+    const int chunk_edges = J.chunk_edges;
+    const int32_t *L = J.L;
+    int32_t *claims = J.claims;
+    int32_t *touched = J.touched;
+    int64_t *touched_cnt = J.touched_cnt;
+    const P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
     const int64_t nch = (int64_t)sel.cnt[LC_CHUNKS];
-    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         chunk_bounds(g, sel, sel.hc[j], chunk_edges, slot, v, e0, e1, deg);
@@ -261,17 +344,23 @@ __global__ void k_expand_heavy(GraphDev g, ListDev sel, int chunk_edges, const i
 
 // partition.cpp:539-597: label the claimed vertices, extend the frontiers,
 // recount sizes.
-__global__ __launch_bounds__(kThreads) void k_p1_apply(GraphDev g, int32_t *L, int32_t *claims,
-                                                       const int32_t *touched,
-                                                       const int64_t *touched_cnt, RecDev fnext,
-                                                       unsigned long long *sizes, int k,
-                                                       P1Ctl *ctl) {
+__global__ __launch_bounds__(kThreads) void k_p1_apply(const P1Job *JS, const int32_t *bstart, int jobs, int parity) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    int32_t *L = J.L;
+    int32_t *claims = J.claims;
+    const int32_t *touched = J.touched;
+    const int64_t *touched_cnt = J.touched_cnt;
+    const RecDev fnext = J.F[parity ^ 1];
+    unsigned long long *sizes = J.sizes;
+    const int k = J.k;
+    P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
     __shared__ unsigned long long s_sz[DMOLP_MAX_K];
     for (int i = threadIdx.x; i < k; i += blockDim.x) s_sz[i] = 0;
     __syncthreads();
     const int64_t count = *touched_cnt;
-    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+    for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
         const int64_t i = b0 + lane_id();
         const bool on = i < count;
         int32_t v = 0, p = 0;
@@ -292,11 +381,13 @@ __global__ __launch_bounds__(kThreads) void k_p1_apply(GraphDev g, int32_t *L, i
     __syncthreads();
     for (int i = threadIdx.x; i < k; i += blockDim.x)
         if (s_sz[i]) atomicAdd(&sizes[i], s_sz[i]);
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (vg.block == 0 && threadIdx.x == 0)
         atomicAdd((unsigned long long *)&ctl->labelled, (unsigned long long)count);
 }
 
-__global__ void k_p1_round_end(P1Ctl *ctl) {
+__global__ void k_p1_round_end(const P1Job *JS, const int32_t *bstart, int jobs) {
+    P1_JOB();
+    P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
     ctl->rounds += 1;
 }
@@ -308,9 +399,12 @@ __global__ void k_mod_labels(int64_t n, int k, int32_t *L) {
         L[v] = (int32_t)(v % k);
 }
 
-__global__ void k_degrees(GraphDev g, int32_t *deg) {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
-         v += (int64_t)gridDim.x * blockDim.x)
+__global__ void k_degrees(const P1Job *JS, const int32_t *bstart, int jobs) {
+    P1_JOB();
+    const GraphDev g = J.g;
+    int32_t *deg = J.deg;
+    for (int64_t v = vthread(vg); v < g.n;
+         v += vthreads(vg))
         deg[v] = (int32_t)(g.off[v + 1] - g.off[v]);
 }
 
@@ -407,156 +501,292 @@ Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
     return w;
 }
 
+namespace {
+
+// Per-job host state of a phase-1 set run.
+struct P1Host {
+    Phase1Work *w = nullptr;
+    std::vector<int32_t> deg, hubs, seeds;
+    std::vector<uint8_t> used;
+    std::pair<int32_t, int32_t> fs{-1, -1};
+    double host_ms = 0.0;
+    int32_t found = 0, fallback = 0;
+};
+
+P1Job p1_job(dmolp_ctx *ctx, dmolp_graph *g, Phase1Work &w, int32_t *remaining) {
+    P1Job j{};
+    j.g = g->dev();
+    j.lvl1 = w.lvl1.dev;
+    j.sel_list = w.sel_list.dev;
+    j.F[0] = w.F[0].dev;
+    j.F[1] = w.F[1].dev;
+    j.L = g->labels.as<int32_t>();
+    j.claims = w.claims.as<int32_t>();
+    j.touched = w.touched.as<int32_t>();
+    j.touched_cnt = w.touched_cnt.as<int64_t>();
+    j.covered = w.covered.as<uint32_t>();
+    j.deg = w.deg.as<int32_t>();
+    j.best = w.best.as<unsigned long long>();
+    j.sizes = w.sizes.as<unsigned long long>();
+    j.seeds = w.seeds.as<int32_t>();
+    j.hubs = w.hubs.as<int32_t>();
+    j.used = w.used.as<uint8_t>();
+    j.nh = 0;
+    j.ctl = w.ctl.as<P1Ctl>();
+    j.remaining = remaining;
+    j.k = w.k;
+    j.seed_steps = 1;
+    j.heavy_degree = ctx->tune.heavy_degree;
+    j.chunk_edges = ctx->tune.chunk_edges;
+    return j;
+}
+
+__global__ void k_p1_set_remaining(int32_t *remaining, int32_t v) { *remaining = v; }
+
+// Hubs and first seed of every job on the host; jobs in parallel threads.
+void host_hub_search(std::vector<P1Host> &H) {
+    auto one = [&](size_t j) {
+        const auto t0 = std::chrono::steady_clock::now();
+        H[j].fs = first_seed_host(H[j].deg);
+        H[j].hubs = find_hubs_host(H[j].deg);
+        const auto t1 = std::chrono::steady_clock::now();
+        H[j].host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    };
+    const size_t G = H.size();
+    size_t T = std::thread::hardware_concurrency();
+    if (T > 16) T = 16;
+    if (T > G) T = G;
+    if (T <= 1) {
+        for (size_t j = 0; j < G; ++j) one(j);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < T; ++t)
+        pool.emplace_back([&, t]() { for (size_t j = t; j < G; j += T) one(j); });
+    for (auto &th : pool) th.join();
+}
+
+}  // namespace
+
 void phase1_run(dmolp_ctx *ctx, dmolp_graph *g, int k, int theta, std::vector<int32_t> &seeds_out,
                 dmolp_phase1_metrics *m) {
-    DMOLP_REQUIRE(k >= 1 && k <= DMOLP_MAX_K, DMOLP_E_ARG, "k outside [1, 1024]");
-    DMOLP_REQUIRE(theta > 0, DMOLP_E_ARG, "theta must be positive (theta <= 0 never terminates)");
-    DMOLP_REQUIRE((int64_t)k <= g->n, DMOLP_E_SEEDS, "k larger than the vertex count");
-    Phase1Work &w = phase1_work(ctx, g, k);
+    std::vector<P1Item> items(1);
+    items[0].g = g;
+    items[0].k = k;
+    items[0].theta = theta;
+    phase1_run_set(ctx, items);
+    seeds_out = items[0].seeds;
+    if (m) *m = items[0].m;
+}
+
+void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items) {
+    const int G = (int)items.size();
+    DMOLP_REQUIRE(G >= 1, DMOLP_E_ARG, "empty job set");
     hipStream_t s = ctx->stream;
-    const GraphDev gd = g->dev();
-    const int H = ctx->tune.heavy_degree, CH = ctx->tune.chunk_edges;
-    const unsigned grid = (unsigned)ctx->grid_blocks();
-    if (!g->labels.p) g->labels.alloc(sizeof(int32_t) * (size_t)g->n);
-    int32_t *L = g->labels.as<int32_t>();
-    P1Ctl *ctl = w.ctl.as<P1Ctl>();
+    std::vector<P1Host> H((size_t)G);
+    int64_t max_n = 0;
+    bool any_heavy = false;
+    int passes = 0, kmax = 1;
+    for (int j = 0; j < G; ++j) {
+        P1Item &it = items[(size_t)j];
+        dmolp_graph *g = it.g;
+        DMOLP_REQUIRE(it.k >= 1 && it.k <= DMOLP_MAX_K, DMOLP_E_ARG, "k outside [1, 1024]");
+        DMOLP_REQUIRE(it.theta > 0, DMOLP_E_ARG, "theta must be positive (theta <= 0 never terminates)");
+        DMOLP_REQUIRE((int64_t)it.k <= g->n, DMOLP_E_SEEDS, "k larger than the vertex count");
+        H[(size_t)j].w = &phase1_work(ctx, g, it.k);
+        if (!g->labels.p) g->labels.alloc(sizeof(int32_t) * (size_t)g->n);
+        max_n = std::max(max_n, g->n);
+        any_heavy = any_heavy || g->n_heavy > 0;
+        passes = std::max(passes, H[(size_t)j].w->sel.passes);
+        kmax = std::max(kmax, it.k);
+    }
+    // Set-level device state: job records, selection args, launch shape.
+    if (!ctx->p1_remaining.p) ctx->p1_remaining.alloc(sizeof(int32_t) * 2);
+    int32_t *remaining = ctx->p1_remaining.as<int32_t>();
+    std::vector<P1Job> jobs((size_t)G);
+    std::vector<SelArgs> sel0((size_t)G), sel1((size_t)G);
+    std::vector<std::array<int32_t, P_NPOL>> blocks((size_t)G);
+    for (int j = 0; j < G; ++j) {
+        P1Item &it = items[(size_t)j];
+        Phase1Work &w = *H[(size_t)j].w;
+        jobs[(size_t)j] = p1_job(ctx, it.g, w, remaining);
+        P1Ctl *ctl = w.ctl.as<P1Ctl>();
+        TakeRule rule{};
+        rule.kind = 1;
+        rule.theta = it.theta;
+        sel0[(size_t)j] = select_args(ctx, w.sel, w.F[0].dev, rule, &ctl->done, w.sel_list.dev, &w.F[1].dev);
+        sel1[(size_t)j] = select_args(ctx, w.sel, w.F[1].dev, rule, &ctl->done, w.sel_list.dev, &w.F[0].dev);
+        job_blocks(ctx, it.g->n, it.g->nnz, it.k, it.g->n_heavy, G == 1, blocks[(size_t)j].data());
+    }
+    LaunchSet &ls = ctx->p1_ls;
+    launch_set_build(ctx, ls, blocks);
+    ctx->p1_jobs.grow(sizeof(P1Job) * (size_t)G);
+    ctx->p1_sel.grow(sizeof(SelArgs) * 2 * (size_t)G);
+    P1Job *d_jobs = ctx->p1_jobs.as<P1Job>();
+    SelArgs *d_sel0 = ctx->p1_sel.as<SelArgs>();
+    SelArgs *d_sel1 = d_sel0 + G;
+    auto upload_jobs = [&]() {
+        DMOLP_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(P1Job) * (size_t)G,
+                                       hipMemcpyHostToDevice, s));
+    };
+    upload_jobs();
+    DMOLP_HIP_CHECK(hipMemcpyAsync(d_sel0, sel0.data(), sizeof(SelArgs) * (size_t)G, hipMemcpyHostToDevice, s));
+    DMOLP_HIP_CHECK(hipMemcpyAsync(d_sel1, sel1.data(), sizeof(SelArgs) * (size_t)G, hipMemcpyHostToDevice, s));
 
     DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, s));
     // Degrees to the host for the hub sort.
-    hipLaunchKernelGGL(k_degrees, dim3(ctx->cu_count * 4), dim3(256), 0, s, gd, w.deg.as<int32_t>());
-    std::vector<int32_t> deg((size_t)g->n);
-    DMOLP_HIP_CHECK(hipMemcpyAsync(deg.data(), w.deg.p, sizeof(int32_t) * (size_t)g->n,
-                                   hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_degrees, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G);
+    for (int j = 0; j < G; ++j) {
+        const dmolp_graph *g = items[(size_t)j].g;
+        H[(size_t)j].deg.resize((size_t)g->n);
+        DMOLP_HIP_CHECK(hipMemcpyAsync(H[(size_t)j].deg.data(), H[(size_t)j].w->deg.p,
+                                       sizeof(int32_t) * (size_t)g->n, hipMemcpyDeviceToHost, s));
+    }
     DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-    const auto th0 = std::chrono::steady_clock::now();
-    const std::pair<int32_t, int32_t> fs = first_seed_host(deg);
-    const std::vector<int32_t> hubs = find_hubs_host(deg);
-    const auto th1 = std::chrono::steady_clock::now();
-    const int64_t nh = (int64_t)hubs.size();
+    host_hub_search(H);
 
     // Seeds (seed.cpp:192-242).
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(P1Ctl), s));
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.covered.p, 0, w.covered.bytes, s));
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.best.p, 0, w.best.bytes, s));
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.seeds.p, 0xff, w.seeds.bytes, s));
-    DMOLP_HIP_CHECK(hipMemcpyAsync(w.seeds.p, &fs.first, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    DevMem hubs_d(sizeof(int32_t) * (size_t)(nh > 0 ? nh : 1));
-    DevMem used_d((size_t)(nh > 0 ? nh : 1));
-    std::vector<uint8_t> used((size_t)(nh > 0 ? nh : 1), 0);
-    for (int64_t i = 0; i < nh; ++i)
-        if (hubs[(size_t)i] == fs.first) { used[(size_t)i] = 1; break; }
-    if (nh > 0) {
-        DMOLP_HIP_CHECK(hipMemcpyAsync(hubs_d.p, hubs.data(), sizeof(int32_t) * (size_t)nh,
-                                       hipMemcpyHostToDevice, s));
-        DMOLP_HIP_CHECK(hipMemcpyAsync(used_d.p, used.data(), (size_t)nh, hipMemcpyHostToDevice, s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.lvl1.cnt.p, 0, w.lvl1.cnt.bytes, s));
-        for (int j = 1; j < k; ++j) {
-            // Ball of seed j-1, then the next seed.
-            hipLaunchKernelGGL(k_ball1, dim3(grid), dim3(kThreads), 0, s, gd, w.seeds.as<int32_t>(),
-                               j - 1, w.covered.as<uint32_t>(), w.lvl1.dev, H, CH, ctl);
-            hipLaunchKernelGGL(k_ball2_light, dim3(grid), dim3(kThreads), 0, s, gd, w.lvl1.dev,
-                               w.covered.as<uint32_t>(), ctl);
-            if (g->n_heavy > 0)
-                hipLaunchKernelGGL(k_ball2_heavy, dim3(grid), dim3(kThreads), 0, s, gd, w.lvl1.dev,
-                                   CH, w.covered.as<uint32_t>(), ctl);
-            hipLaunchKernelGGL(k_next_seed, dim3(ctx->cu_count * 2), dim3(kThreads), 0, s, gd,
-                               hubs_d.as<int32_t>(), nh, used_d.as<uint8_t>(),
-                               w.covered.as<uint32_t>(), w.best.as<unsigned long long>(), ctl);
-            hipLaunchKernelGGL(k_take_seed, dim3(1), dim3(1), 0, s, w.best.as<unsigned long long>(),
-                               hubs_d.as<int32_t>(), used_d.as<uint8_t>(), w.seeds.as<int32_t>(), j,
-                               ctl, w.lvl1.dev);
+    int steps = 1;
+    for (int j = 0; j < G; ++j) {
+        P1Host &h = H[(size_t)j];
+        Phase1Work &w = *h.w;
+        const int k = items[(size_t)j].k;
+        const int64_t nh = (int64_t)h.hubs.size();
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(P1Ctl), s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.covered.p, 0, w.covered.bytes, s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.best.p, 0, w.best.bytes, s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.seeds.p, 0xff, w.seeds.bytes, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(w.seeds.p, &h.fs.first, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        h.used.assign((size_t)(nh > 0 ? nh : 1), 0);
+        for (int64_t i = 0; i < nh; ++i)
+            if (h.hubs[(size_t)i] == h.fs.first) { h.used[(size_t)i] = 1; break; }
+        if (w.hubs.bytes < sizeof(int32_t) * (size_t)(nh > 0 ? nh : 1)) {
+            w.hubs.alloc(sizeof(int32_t) * (size_t)(nh > 0 ? nh : 1));
+            w.used.alloc((size_t)(nh > 0 ? nh : 1));
         }
-        DMOLP_HIP_CHECK(hipGetLastError());
+        if (nh > 0) {
+            DMOLP_HIP_CHECK(hipMemcpyAsync(w.hubs.p, h.hubs.data(), sizeof(int32_t) * (size_t)nh,
+                                           hipMemcpyHostToDevice, s));
+            DMOLP_HIP_CHECK(hipMemcpyAsync(w.used.p, h.used.data(), (size_t)nh, hipMemcpyHostToDevice, s));
+            DMOLP_HIP_CHECK(hipMemsetAsync(w.lvl1.cnt.p, 0, w.lvl1.cnt.bytes, s));
+        }
+        P1Job &jb = jobs[(size_t)j];
+        jb.hubs = w.hubs.as<int32_t>();
+        jb.used = w.used.as<uint8_t>();
+        jb.nh = nh;
+        jb.seed_steps = nh > 0 ? k : 1;
+        steps = std::max(steps, jb.seed_steps);
     }
-    std::vector<int32_t> seeds((size_t)k, -1);
-    P1Ctl hc{};
-    DMOLP_HIP_CHECK(hipMemcpyAsync(seeds.data(), w.seeds.p, sizeof(int32_t) * (size_t)k,
-                                   hipMemcpyDeviceToHost, s));
-    DMOLP_HIP_CHECK(hipMemcpyAsync(&hc, w.ctl.p, sizeof(P1Ctl), hipMemcpyDeviceToHost, s));
+    upload_jobs();
+    for (int step = 1; step < steps; ++step) {
+        // Ball of seed step-1, then the next seed.
+        hipLaunchKernelGGL(k_ball1, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
+        hipLaunchKernelGGL(k_ball2_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
+        if (any_heavy)
+            hipLaunchKernelGGL(k_ball2_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
+        hipLaunchKernelGGL(k_next_seed, dim3(ls.total[P_HIST1]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_HIST1), G, step);
+        hipLaunchKernelGGL(k_take_seed, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G, step);
+    }
+    DMOLP_HIP_CHECK(hipGetLastError());
+    std::vector<P1Ctl> hc((size_t)G);
+    for (int j = 0; j < G; ++j) {
+        P1Host &h = H[(size_t)j];
+        h.seeds.assign((size_t)items[(size_t)j].k, -1);
+        DMOLP_HIP_CHECK(hipMemcpyAsync(h.seeds.data(), h.w->seeds.p, sizeof(int32_t) * h.seeds.size(),
+                                       hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&hc[(size_t)j], h.w->ctl.p, sizeof(P1Ctl), hipMemcpyDeviceToHost, s));
+    }
     DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-    int32_t found = (nh > 0) ? (hc.seed_stop ? hc.seeds_found : k) : 1;
-    if (k == 1) found = 1;
-    int32_t fallback = 0;
-    if (found < k) {
-        // Documented extension for seed.cpp:214-217 followed by the
-        // out-of-range seeds[p] of partition.cpp:347: the missing seeds are
-        // the remaining vertices by (degree descending, id ascending).
-        std::vector<uint8_t> chosen((size_t)g->n, 0);
-        for (int j = 0; j < found; ++j) chosen[(size_t)seeds[(size_t)j]] = 1;
-        std::vector<int32_t> order;
-        order.reserve((size_t)g->n);
-        for (int64_t v = 0; v < g->n; ++v)
-            if (!chosen[(size_t)v]) order.push_back((int32_t)v);
-        const size_t need = (size_t)(k - found);
-        auto cmp = [&](int32_t a, int32_t b) {
-            if (deg[(size_t)a] != deg[(size_t)b]) return deg[(size_t)a] > deg[(size_t)b];
-            return a < b;
-        };
-        std::partial_sort(order.begin(), order.begin() + (ptrdiff_t)need, order.end(), cmp);
-        for (size_t i = 0; i < need; ++i) seeds[(size_t)found + i] = order[i];
-        fallback = k - found;
-        DMOLP_HIP_CHECK(hipMemcpyAsync(w.seeds.p, seeds.data(), sizeof(int32_t) * (size_t)k,
-                                       hipMemcpyHostToDevice, s));
+    for (int j = 0; j < G; ++j) {
+        P1Host &h = H[(size_t)j];
+        const int k = items[(size_t)j].k;
+        const int64_t n = items[(size_t)j].g->n;
+        const int64_t nh = (int64_t)h.hubs.size();
+        int32_t found = (nh > 0) ? (hc[(size_t)j].seed_stop ? hc[(size_t)j].seeds_found : k) : 1;
+        if (k == 1) found = 1;
+        h.found = found;
+        h.fallback = 0;
+        if (found < k) {
+            // Documented extension for seed.cpp:214-217 followed by the
+            // out-of-range seeds[p] of partition.cpp:347: the missing seeds are
+            // the remaining vertices by (degree descending, id ascending).
+            std::vector<uint8_t> chosen((size_t)n, 0);
+            for (int i = 0; i < found; ++i) chosen[(size_t)h.seeds[(size_t)i]] = 1;
+            std::vector<int32_t> order;
+            order.reserve((size_t)n);
+            for (int64_t v = 0; v < n; ++v)
+                if (!chosen[(size_t)v]) order.push_back((int32_t)v);
+            const size_t need = (size_t)(k - found);
+            const std::vector<int32_t> &deg = h.deg;
+            auto cmp = [&](int32_t a, int32_t b) {
+                if (deg[(size_t)a] != deg[(size_t)b]) return deg[(size_t)a] > deg[(size_t)b];
+                return a < b;
+            };
+            std::partial_sort(order.begin(), order.begin() + (ptrdiff_t)need, order.end(), cmp);
+            for (size_t i = 0; i < need; ++i) h.seeds[(size_t)found + i] = order[i];
+            h.fallback = k - found;
+            DMOLP_HIP_CHECK(hipMemcpyAsync(h.w->seeds.p, h.seeds.data(), sizeof(int32_t) * (size_t)k,
+                                           hipMemcpyHostToDevice, s));
+        }
     }
 
     // Expansion (partition.cpp:323-604).
-    DMOLP_HIP_CHECK(hipMemsetAsync(L, 0xff, sizeof(int32_t) * (size_t)g->n, s));
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.sizes.p, 0, w.sizes.bytes, s));
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(P1Ctl), s));
-    hipLaunchKernelGGL(k_expand_init, dim3((k + 255) / 256), dim3(256), 0, s, gd,
-                       w.seeds.as<int32_t>(), k, L, w.F[0].dev, w.sizes.as<unsigned long long>(), ctl);
-    TakeRule rule{};
-    rule.kind = 1;
-    rule.theta = theta;
+    for (int j = 0; j < G; ++j) {
+        dmolp_graph *g = items[(size_t)j].g;
+        Phase1Work &w = *H[(size_t)j].w;
+        DMOLP_HIP_CHECK(hipMemsetAsync(g->labels.p, 0xff, sizeof(int32_t) * (size_t)g->n, s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.sizes.p, 0, w.sizes.bytes, s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(P1Ctl), s));
+    }
+    hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, remaining, (int32_t)G);
+    hipLaunchKernelGGL(k_expand_init, dim3(ls.total[P_ONE]), dim3(256), 0, s, d_jobs, ls.bs(P_ONE), G);
     int64_t enq = 0;
     const int batch = 16;
+    int32_t left = G;
     for (;;) {
         for (int b = 0; b < batch; ++b, ++enq) {
-            Recs &fcur = w.F[enq & 1];
-            Recs &fnext = w.F[(enq + 1) & 1];
-            hipLaunchKernelGGL(k_p1_round_begin, dim3(1), dim3(1), 0, s, ctl, fcur.dev, fnext.dev,
-                               w.sel_list.dev, w.sizes.as<unsigned long long>(), k, g->n,
-                               w.touched_cnt.as<int64_t>());
-            hipLaunchKernelGGL(k_p1_fill, dim3(grid), dim3(kThreads), 0, s, gd, L, fcur.dev,
-                               w.sizes.as<unsigned long long>(), ctl);
-            select_enqueue(ctx, w.sel, gd, fcur.dev, rule, &ctl->done, w.sel_list.dev, &fnext.dev);
-            hipLaunchKernelGGL(k_expand_light, dim3(grid), dim3(kThreads), 0, s, gd, w.sel_list.dev,
-                               L, w.claims.as<int32_t>(), w.touched.as<int32_t>(),
-                               w.touched_cnt.as<int64_t>(), ctl);
-            if (g->n_heavy > 0)
-                hipLaunchKernelGGL(k_expand_heavy, dim3(grid), dim3(kThreads), 0, s, gd,
-                                   w.sel_list.dev, CH, L, w.claims.as<int32_t>(),
-                                   w.touched.as<int32_t>(), w.touched_cnt.as<int64_t>(), ctl);
-            hipLaunchKernelGGL(k_p1_apply, dim3(grid), dim3(kThreads), 0, s, gd, L,
-                               w.claims.as<int32_t>(), w.touched.as<int32_t>(),
-                               w.touched_cnt.as<int64_t>(), fnext.dev,
-                               w.sizes.as<unsigned long long>(), k, ctl);
-            hipLaunchKernelGGL(k_p1_round_end, dim3(1), dim3(1), 0, s, ctl);
+            const int par = (int)(enq & 1);
+            const SelArgs *sa = par ? d_sel1 : d_sel0;
+            hipLaunchKernelGGL(k_p1_round_begin, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G, par);
+            hipLaunchKernelGGL(k_p1_fill, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, par);
+            select_enqueue(ctx, sa, ls, passes);
+            hipLaunchKernelGGL(k_expand_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G);
+            if (any_heavy)
+                hipLaunchKernelGGL(k_expand_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G);
+            hipLaunchKernelGGL(k_p1_apply, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, par);
+            hipLaunchKernelGGL(k_p1_round_end, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G);
         }
         DMOLP_HIP_CHECK(hipGetLastError());
-        DMOLP_HIP_CHECK(hipMemcpyAsync(&hc, w.ctl.p, sizeof(P1Ctl), hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&left, remaining, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-        if (hc.done) break;
-        DMOLP_REQUIRE(enq < 64ll * (g->n + 16), DMOLP_E_HIP, "phase-1 expansion did not finish");
+        if (left <= 0) break;
+        DMOLP_REQUIRE(enq < 64ll * (max_n + 16), DMOLP_E_HIP, "phase-1 expansion did not finish");
     }
     DMOLP_HIP_CHECK(hipEventRecord(ctx->ev1, s));
+    for (int j = 0; j < G; ++j)
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&hc[(size_t)j], H[(size_t)j].w->ctl.p, sizeof(P1Ctl),
+                                       hipMemcpyDeviceToHost, s));
     DMOLP_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;
     DMOLP_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    seeds_out = seeds;
-    if (m) {
-        std::memset(m, 0, sizeof(*m));
-        m->first_seed = fs.first;
-        m->first_seed_degree = fs.second;
-        m->n_hubs = nh;
-        m->n_seeds_found = found;
-        m->n_seeds_fallback = fallback;
-        m->rounds = hc.rounds;
-        m->fill_round = hc.fill_round;
-        m->device_ms = ms;
-        m->host_ms = std::chrono::duration<double, std::milli>(th1 - th0).count();
+    for (int j = 0; j < G; ++j) {
+        P1Item &it = items[(size_t)j];
+        P1Host &h = H[(size_t)j];
+        it.seeds = h.seeds;
+        dmolp_phase1_metrics &m = it.m;
+        std::memset(&m, 0, sizeof(m));
+        m.first_seed = h.fs.first;
+        m.first_seed_degree = h.fs.second;
+        m.n_hubs = (int64_t)h.hubs.size();
+        m.n_seeds_found = h.found;
+        m.n_seeds_fallback = h.fallback;
+        m.rounds = hc[(size_t)j].rounds;
+        m.fill_round = hc[(size_t)j].fill_round;
+        m.device_ms = ms;
+        m.host_ms = h.host_ms;
+        it.g->labels_valid = true;
     }
-    g->labels_valid = true;
 }
 
 }  // namespace dmolp

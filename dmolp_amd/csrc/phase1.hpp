@@ -23,9 +23,40 @@ struct Phase1Work {
     int k = 0;
     int64_t cap_heavy = -1, cap_chunks = -1;
     DevMem claims, touched, touched_cnt, covered, deg, ctl, best, sizes, seeds;
+    DevMem hubs, used;  // hub list of the current run and its used flags
     Recs F[2];
     List sel_list, lvl1;
     SelectWork sel;
+};
+
+// Everything one graph's phase-1 kernels read and write (device pointers).
+struct P1Job {
+    GraphDev g;
+    ListDev lvl1, sel_list;
+    RecDev F[2];
+    int32_t *L, *claims, *touched;
+    int64_t *touched_cnt;
+    uint32_t *covered;
+    int32_t *deg;
+    unsigned long long *best, *sizes;
+    int32_t *seeds;
+    const int32_t *hubs;
+    uint8_t *used;
+    int64_t nh;
+    P1Ctl *ctl;
+    int32_t *remaining;  // unfinished jobs of the set (shared by the set)
+    int32_t k;
+    int32_t seed_steps;  // steps of the hub search (k, or 1 when there are no hubs)
+    int32_t heavy_degree, chunk_edges;
+};
+
+// One job of a phase-1 set: inputs and results.
+struct P1Item {
+    dmolp_graph *g = nullptr;
+    int k = 0;
+    int theta = 0;
+    std::vector<int32_t> seeds;
+    dmolp_phase1_metrics m{};
 };
 
 Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
@@ -37,5 +68,8 @@ std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg);
 void phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int k);
 void phase1_run(dmolp_ctx *ctx, dmolp_graph *g, int k, int theta, std::vector<int32_t> &seeds_out,
                 dmolp_phase1_metrics *m);
+// The same for a set of graphs advancing together: one launch per kernel
+// step serves every job.  Each job's result equals its plain run.
+void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items);
 
 }  // namespace dmolp

@@ -14,7 +14,9 @@
 // Every kernel returns at once when the control block says "done", so a
 // finished run costs empty launches only.
 #include <cmath>
+#include <array>
 #include <cstring>
+#include <vector>
 
 #include "phase2.hpp"
 #include "rows.hpp"
@@ -33,14 +35,29 @@ constexpr int kScanItems = 4;         // vertices per thread per block iteration
 
 __device__ __forceinline__ bool is_done(const Ctl *c) { return c->done != 0; }
 
+// Every round kernel takes the job set and runs on its job's share of the
+// launch (see VGrid): one graph, or many graphs in one launch.
+#define P2_JOB()                                   \
+    const int job_ = job_of_block(bstart, jobs);   \
+    const VGrid vg = vgrid_of(bstart, job_);       \
+    const P2Job &J = JS[job_];                     \
+    (void)vg
+
+
 // ---------------------------------------------------------------- init sweep
 
 // phase2.cpp:55-117 and 214-253 over all vertices, plus the per-vertex count
 // of differing neighbours used as the boundary test.
-__global__ __launch_bounds__(kThreads) void k_init_light(GraphDev g, ListDev all, const int32_t *L,
-                                                         int k, int32_t *diff, Ctl *ctl,
-                                                         unsigned long long *vc,
-                                                         unsigned long long *ic) {
+__global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev all = J.all;
+    const int32_t *L = J.L;
+    const int k = J.k;
+    int32_t *diff = J.diff;
+    Ctl *ctl = J.ctl;
+    unsigned long long *vc = J.vc;
+    unsigned long long *ic = J.ic;
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ int32_t s_cnt[kWavesPerBlock][kWave];
     __shared__ unsigned long long s_vc[kMaxKLds], s_ic[kMaxKLds];
@@ -49,7 +66,7 @@ __global__ __launch_bounds__(kThreads) void k_init_light(GraphDev g, ListDev all
     const int w = threadIdx.x / kWave, lane = lane_id();
     int64_t invalid = 0, cut2 = 0;
     const int64_t count = (int64_t)all.cnt[LC_LIGHT];
-    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+    for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, all.light, count, b0, s_b[w], v, deg, valid);
@@ -90,14 +107,21 @@ __global__ __launch_bounds__(kThreads) void k_init_light(GraphDev g, ListDev all
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_init_heavy(GraphDev g, ListDev all, const int32_t *L,
-                                                         int k, int chunk_edges, int32_t *diff,
-                                                         Ctl *ctl, unsigned long long *vc,
-                                                         unsigned long long *ic) {
+__global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev all = J.all;
+    const int32_t *L = J.L;
+    const int k = J.k;
+    const int chunk_edges = J.chunk_edges;
+    int32_t *diff = J.diff;
+    Ctl *ctl = J.ctl;
+    unsigned long long *vc = J.vc;
+    unsigned long long *ic = J.ic;
     const int lane = lane_id();
     int64_t invalid = 0, cut2 = 0;
     const int64_t nch = (int64_t)all.cnt[LC_CHUNKS];
-    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         const int64_t item = all.hc[j];
@@ -125,8 +149,14 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(GraphDev g, ListDev all
     wave_add(&ctl->v[C_CUT2], cut2);
 }
 
-__global__ void k_init_finish(Ctl *ctl, unsigned long long *ic, int k, int32_t epoch_base,
-                              int32_t max_rounds, double epsilon) {
+__global__ void k_init_finish(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    Ctl *ctl = J.ctl;
+    unsigned long long *ic = J.ic;
+    const int k = J.k;
+    const int32_t epoch_base = J.epoch_base;
+    const int32_t max_rounds = J.max_rounds;
+    const double epsilon = J.epsilon;
     for (int i = threadIdx.x; i < k; i += blockDim.x) ic[i] = ic[i] / 2;  // phase2.cpp:110
     if (threadIdx.x == 0) {
         ctl->t = 1;
@@ -141,6 +171,7 @@ __global__ void k_init_finish(Ctl *ctl, unsigned long long *ic, int k, int32_t e
         ctl->v[C_CUT] = cut2 >> 1;
         ctl->v[C_PREV_CUT] = cut2 >> 1;
         ctl->v[C_INIT_CUT] = cut2 >> 1;
+        if (ctl->done) atomicSub(J.remaining, 1);
     }
 }
 
@@ -190,9 +221,21 @@ __global__ void k_penalties(const unsigned long long *vc, const unsigned long lo
 // B_{t-1} covers at least pull_frac of the CSR entries.
 // B holds B_{t-1} here; its counts are kept for the trace before the
 // selection overwrites the list with B_t.
-__global__ void k_round_begin(Ctl *ctl, const unsigned long long *vc, const unsigned long long *ic,
-                              int k, const double *rtab, double *pen, double *re, ListDev B,
-                              ListDev dl, RecDev bd, int64_t n, int64_t nnz, double pull_frac) {
+__global__ void k_round_begin(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    Ctl *ctl = J.ctl;
+    const unsigned long long *vc = J.vc;
+    const unsigned long long *ic = J.ic;
+    const int k = J.k;
+    const double *rtab = J.rtab;
+    double *pen = J.pen;
+    double *re = J.re;
+    const ListDev B = J.B;
+    const ListDev dl = J.D;
+    const RecDev bd = J.bd;
+    const int64_t n = J.g.n;
+    const int64_t nnz = J.g.nnz;
+    const double pull_frac = J.pull_frac;
     if (is_done(ctl)) return;
     if (threadIdx.x < LC_N) dl.cnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
@@ -212,12 +255,17 @@ __global__ void k_round_begin(Ctl *ctl, const unsigned long long *vc, const unsi
 // ---------------------------------------------------------------- candidates
 
 // Round 1: B_0 is every vertex, so the candidates are every vertex.
-__global__ __launch_bounds__(kThreads) void k_cand_all(GraphDev g, const int32_t *L,
-                                                       const int32_t *diff, RecDev bd, Ctl *ctl) {
+__global__ __launch_bounds__(kThreads) void k_cand_all(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const int32_t *L = J.L;
+    const int32_t *diff = J.diff;
+    const RecDev bd = J.bd;
+    Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_ALL) return;
     __shared__ BlockAppendLds s_app;
     const int64_t span = (int64_t)blockDim.x * kScanItems;
-    for (int64_t b0 = (int64_t)blockIdx.x * span; b0 < g.n; b0 += (int64_t)gridDim.x * span) {
+    for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
         uint32_t m = 0;
         int32_t nb = 0;
 #pragma unroll
@@ -258,15 +306,21 @@ __device__ __forceinline__ void touch(const GraphDev &g, bool pred, int32_t u, i
 // iff it or one of its neighbours was selected in round t-1 (bstamp ==
 // prev_epoch).  Rows longer than heavy_degree are scanned here up to
 // heavy_degree entries and finished by k_cand_pull_heavy.
-__global__ __launch_bounds__(kThreads) void k_cand_pull(GraphDev g, const int32_t *L,
-                                                        const int32_t *diff, const int32_t *bstamp,
-                                                        RecDev bd, int32_t *deferred,
-                                                        int heavy_degree, Ctl *ctl) {
+__global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const int32_t *L = J.L;
+    const int32_t *diff = J.diff;
+    const int32_t *bstamp = J.bstamp;
+    const RecDev bd = J.bd;
+    int32_t *deferred = J.deferred;
+    const int heavy_degree = J.heavy_degree;
+    Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
     const int32_t prev_epoch = ctl->epoch - 1;
     __shared__ BlockAppendLds s_app;
     const int64_t span = (int64_t)blockDim.x * kScanItems;
-    for (int64_t b0 = (int64_t)blockIdx.x * span; b0 < g.n; b0 += (int64_t)gridDim.x * span) {
+    for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
         uint32_t rec = 0;
         int32_t nr = 0, deg[kScanItems];
 #pragma unroll
@@ -301,12 +355,19 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(GraphDev g, const int32_
     }
 }
 
-__global__ void k_cand_pull_heavy(GraphDev g, const int32_t *L, const int32_t *bstamp,
-                                  RecDev bd, const int32_t *deferred, int heavy_degree, Ctl *ctl) {
+__global__ void k_cand_pull_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const int32_t *L = J.L;
+    const int32_t *bstamp = J.bstamp;
+    const RecDev bd = J.bd;
+    const int32_t *deferred = J.deferred;
+    const int heavy_degree = J.heavy_degree;
+    Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
     const int32_t prev_epoch = ctl->epoch - 1;
     const int64_t nd = (int64_t)ctl->v[C_DEFERRED];
-    for (int64_t j = global_wave(); j < nd; j += wave_count()) {
+    for (int64_t j = vwave(vg); j < nd; j += vwaves(vg)) {
         const int32_t v = deferred[j];
         const int64_t s = g.off[v], e = g.off[v + 1];
         bool found = false;
@@ -324,16 +385,22 @@ __global__ void k_cand_pull_heavy(GraphDev g, const int32_t *L, const int32_t *b
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_cand_light(GraphDev g, ListDev prev, const int32_t *L,
-                                                         const int32_t *diff, int32_t *stamp,
-                                                         RecDev bd, Ctl *ctl) {
+__global__ __launch_bounds__(kThreads) void k_cand_light(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev prev = J.B;
+    const int32_t *L = J.L;
+    const int32_t *diff = J.diff;
+    int32_t *stamp = J.stamp;
+    const RecDev bd = J.bd;
+    Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PUSH) return;
     const int32_t epoch = ctl->epoch;
     __shared__ BatchLds s_b[kWavesPerBlock];
     const int w = threadIdx.x / kWave, lane = lane_id();
     int64_t cand = 0;
     const int64_t count = (int64_t)prev.cnt[LC_LIGHT];
-    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+    for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, prev.light, count, b0, s_b[w], v, deg, valid);
@@ -350,15 +417,22 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(GraphDev g, ListDev pre
     wave_add(&ctl->v[C_CAND], cand);
 }
 
-__global__ __launch_bounds__(kThreads) void k_cand_heavy(GraphDev g, ListDev prev, const int32_t *L,
-                                                         const int32_t *diff, int32_t *stamp,
-                                                         int chunk_edges, RecDev bd, Ctl *ctl) {
+__global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev prev = J.B;
+    const int32_t *L = J.L;
+    const int32_t *diff = J.diff;
+    int32_t *stamp = J.stamp;
+    const int chunk_edges = J.chunk_edges;
+    const RecDev bd = J.bd;
+    Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PUSH) return;
     const int32_t epoch = ctl->epoch;
     const int lane = lane_id();
     int64_t cand = 0;
     const int64_t nch = (int64_t)prev.cnt[LC_CHUNKS];
-    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         const int64_t item = prev.hc[j];
@@ -395,10 +469,19 @@ __device__ __forceinline__ int32_t lp_decide(const int32_t *h, int stride, int E
     return (best_s > base) ? best_l : own;
 }
 
-__global__ __launch_bounds__(kThreads) void k_lp_light(GraphDev g, const int32_t *L, ListDev B,
-                                                       const uint8_t *L8, const double *pen, int k,
-                                                       int32_t *chg, ListDev D, int heavy_degree,
-                                                       int chunk_edges, const Ctl *ctl) {
+__global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const int32_t *L = J.L;
+    const ListDev B = J.B;
+    const uint8_t *L8 = J.L8;
+    const double *pen = J.pen;
+    const int k = J.k;
+    int32_t *chg = J.chg;
+    const ListDev D = J.D;
+    const int heavy_degree = J.heavy_degree;
+    const int chunk_edges = J.chunk_edges;
+    const Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ int32_t s_h[kWavesPerBlock][kWave * kHistStride];
@@ -412,8 +495,8 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(GraphDev g, const int32_t
     const int64_t count = (int64_t)B.cnt[LC_LIGHT];
     // Block-uniform loop over groups of kWavesPerBlock batches (one per
     // wave), so the Delta append can be aggregated per block.
-    for (int64_t g0 = (int64_t)blockIdx.x * kWavesPerBlock * kWave; g0 < count;
-         g0 += (int64_t)gridDim.x * kWavesPerBlock * kWave) {
+    for (int64_t g0 = vg.block * kWavesPerBlock * kWave; g0 < count;
+         g0 += vg.blocks * kWavesPerBlock * kWave) {
         const int64_t b0 = g0 + (int64_t)w * kWave;
         int32_t v, deg;
         bool valid;
@@ -455,9 +538,15 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(GraphDev g, const int32_t
 // distinct labels of each 64-edge step and adds the counts to the row's
 // histogram in global memory.  Pass 2 (k_lp_heavy_decide) runs after the
 // kernel boundary, so no fence or completion counter is needed.
-__global__ __launch_bounds__(kThreads) void k_lp_heavy(GraphDev g, const uint8_t *L8, ListDev B,
-                                                       int k, int chunk_edges, int32_t *hub_hist,
-                                                       const Ctl *ctl) {
+__global__ __launch_bounds__(kThreads) void k_lp_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const uint8_t *L8 = J.L8;
+    const ListDev B = J.B;
+    const int k = J.k;
+    const int chunk_edges = J.chunk_edges;
+    int32_t *hub_hist = J.hub_hist;
+    const Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
     // One private histogram row per lane (odd stride: rows fall in different
     // banks), summed across the wave once per chunk.
@@ -466,7 +555,7 @@ __global__ __launch_bounds__(kThreads) void k_lp_heavy(GraphDev g, const uint8_t
     const int w = threadIdx.x / kWave, lane = lane_id();
     int32_t *row = &s_h[w][lane * kHistStride];
     const int64_t nch = (int64_t)B.cnt[LC_CHUNKS];
-    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         chunk_bounds(g, B, B.hc[j], chunk_edges, slot, v, e0, e1, deg);
@@ -497,16 +586,26 @@ __global__ __launch_bounds__(kThreads) void k_lp_heavy(GraphDev g, const uint8_t
 
 // Heavy rows, pass 2: one thread per heavy row decides from its histogram
 // and clears it for the next round.
-__global__ void k_lp_heavy_decide(const int32_t *L, ListDev B, const double *pen, int k,
-                                  int32_t *chg, ListDev D, int heavy_degree, int chunk_edges,
-                                  int32_t *hub_hist, const int64_t *off, const Ctl *ctl) {
+__global__ void k_lp_heavy_decide(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const int32_t *L = J.L;
+    const ListDev B = J.B;
+    const double *pen = J.pen;
+    const int k = J.k;
+    int32_t *chg = J.chg;
+    const ListDev D = J.D;
+    const int heavy_degree = J.heavy_degree;
+    const int chunk_edges = J.chunk_edges;
+    int32_t *hub_hist = J.hub_hist;
+    const int64_t *off = J.g.off;
+    const Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
     __shared__ double s_pen[kLpCap];
     const int E = k < kLpCap ? k : kLpCap;
     if ((int)threadIdx.x < E) s_pen[threadIdx.x] = pen[threadIdx.x];
     __syncthreads();
     const int64_t ns = (int64_t)B.cnt[LC_SLOTS];
-    for (int64_t b0 = global_wave() * kWave; b0 < ns; b0 += wave_count() * kWave) {
+    for (int64_t b0 = vwave(vg) * kWave; b0 < ns; b0 += vwaves(vg) * kWave) {
         const int64_t slot = b0 + lane_id();
         bool changed = false;
         int32_t v = 0;
@@ -568,10 +667,17 @@ __device__ __forceinline__ int32_t pair_effect(int32_t v, int32_t a, int32_t c, 
     return (a == lu ? 1 : 0) | (c == lu2 ? 1 << 16 : 0);
 }
 
-__global__ __launch_bounds__(kThreads) void k_delta_light(GraphDev g, ListDev D, const int32_t *L,
-                                                          const int32_t *chg, int k, int32_t *diff,
-                                                          Ctl *ctl, unsigned long long *vc,
-                                                          unsigned long long *ic) {
+__global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev D = J.D;
+    const int32_t *L = J.L;
+    const int32_t *chg = J.chg;
+    const int k = J.k;
+    int32_t *diff = J.diff;
+    Ctl *ctl = J.ctl;
+    unsigned long long *vc = J.vc;
+    unsigned long long *ic = J.ic;
     if (is_done(ctl)) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ int32_t s_dv[kWavesPerBlock][kWave];
@@ -582,7 +688,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(GraphDev g, ListDev D,
     const int w = threadIdx.x / kWave, lane = lane_id();
     int64_t cut_delta = 0;
     const int64_t count = (int64_t)D.cnt[LC_LIGHT];
-    for (int64_t b0 = global_wave() * kWave; b0 < count; b0 += wave_count() * kWave) {
+    for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, D.light, count, b0, s_b[w], v, deg, valid);
@@ -633,11 +739,18 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(GraphDev g, ListDev D,
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_delta_heavy(GraphDev g, ListDev D, const int32_t *L,
-                                                          const int32_t *chg, int k,
-                                                          int chunk_edges, int32_t *diff, Ctl *ctl,
-                                                          unsigned long long *vc,
-                                                          unsigned long long *ic) {
+__global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev D = J.D;
+    const int32_t *L = J.L;
+    const int32_t *chg = J.chg;
+    const int k = J.k;
+    const int chunk_edges = J.chunk_edges;
+    int32_t *diff = J.diff;
+    Ctl *ctl = J.ctl;
+    unsigned long long *vc = J.vc;
+    unsigned long long *ic = J.ic;
     if (is_done(ctl)) return;
     __shared__ unsigned long long s_ic[kMaxKLds];
     for (int i = threadIdx.x; i < k; i += blockDim.x) s_ic[i] = 0;
@@ -645,7 +758,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(GraphDev g, ListDev D,
     const int lane = lane_id();
     int64_t cut_delta = 0;
     const int64_t nch = (int64_t)D.cnt[LC_CHUNKS];
-    for (int64_t j = global_wave(); j < nch; j += wave_count()) {
+    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         const int64_t item = D.hc[j];
@@ -681,19 +794,29 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(GraphDev g, ListDev D,
 
 // Byte mirror of the labels for the LP gathers: label if below the LP cap
 // (gpu_lp_boundary.cu:360), else kLabel8None.
-__global__ void k_mirror8(int64_t n, const int32_t *L, uint8_t *L8) {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-         v += (int64_t)gridDim.x * blockDim.x) {
+__global__ void k_mirror8(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const int64_t n = J.g.n;
+    const int32_t *L = J.L;
+    uint8_t *L8 = J.L8;
+    for (int64_t v = vthread(vg); v < n;
+         v += vthreads(vg)) {
         const int32_t l = L[v];
         L8[v] = (l >= 0 && l < kLpCap) ? (uint8_t)l : (uint8_t)kLabel8None;
     }
 }
 
-__global__ void k_apply(ListDev D, int32_t *L, uint8_t *L8, int32_t *chg, const Ctl *ctl) {
+__global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const ListDev D = J.D;
+    int32_t *L = J.L;
+    uint8_t *L8 = J.L8;
+    int32_t *chg = J.chg;
+    const Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
     const int64_t nl = (int64_t)D.cnt[LC_LIGHT], ns = (int64_t)D.cnt[LC_SLOTS];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl + ns;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = vthread(vg); i < nl + ns;
+         i += vthreads(vg)) {
         const int32_t v = i < nl ? D.light[i] : D.hv[i - nl];
         const int32_t nl_v = chg[v];
         L[v] = nl_v;
@@ -704,8 +827,15 @@ __global__ void k_apply(ListDev D, int32_t *L, uint8_t *L8, int32_t *chg, const 
 
 // ---------------------------------------------------------------- round end
 
-__global__ void k_round_end(Ctl *ctl, ListDev cur, ListDev D, RecDev bd, int64_t *trace,
-                            int trace_cap, int64_t n) {
+__global__ void k_round_end(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    Ctl *ctl = J.ctl;
+    const ListDev cur = J.B;
+    const ListDev D = J.D;
+    const RecDev bd = J.bd;
+    int64_t *trace = J.trace;
+    const int trace_cap = J.trace_cap;
+    const int64_t n = J.g.n;
     if (is_done(ctl)) return;
     const int t = ctl->t;
     const int max_rounds = ctl->max_rounds;
@@ -752,14 +882,22 @@ __global__ void k_round_end(Ctl *ctl, ListDev cur, ListDev D, RecDev bd, int64_t
         r[T_BLDEG] = (int64_t)(cur.cnt[LC_DEG] - cur.cnt[LC_HDEG]);
     }
     if (!ctl->done) ctl->t = t + 1;
+    else atomicSub(J.remaining, 1);
 }
 
-inline unsigned grid_for(dmolp_ctx *ctx) { return (unsigned)ctx->grid_blocks(); }
-// Grid for one thread per heavy row.
-inline unsigned heavy_grid(const dmolp_graph *g) {
-    const int64_t b = (g->n_heavy + kThreads - 1) / kThreads;
-    return (unsigned)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+// Zero a job's per-run state before the initial sweep.
+__global__ void k_init_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const int64_t n = J.g.n;
+    for (int64_t v = vthread(vg); v < n; v += vthreads(vg)) J.diff[v] = 0;
+    if (vg.block == 0) {
+        uint32_t *c = reinterpret_cast<uint32_t *>(J.ctl);
+        for (int i = threadIdx.x; i < (int)(sizeof(Ctl) / 4); i += blockDim.x) c[i] = 0;
+        for (int i = threadIdx.x; i < J.k; i += blockDim.x) { J.vc[i] = 0; J.ic[i] = 0; }
+    }
 }
+
+__global__ void k_set_remaining(int32_t *remaining, int32_t v) { *remaining = v; }
 
 }  // namespace
 
@@ -820,42 +958,107 @@ void phase2_round_table(dmolp_ctx *ctx, Phase2Work &w, int R) {
     w.round_key = 0;  // the captured round reads rtab: capture again
 }
 
-void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds,
-                         double epsilon) {
-    hipStream_t s = ctx->stream;
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(Ctl), s));
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.vc.p, 0, w.vc.bytes, s));
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.ic.p, 0, w.ic.bytes, s));
-    DMOLP_HIP_CHECK(hipMemsetAsync(w.diff.p, 0, w.diff.bytes, s));
-    Ctl *ctl = w.ctl.as<Ctl>();
-    const int32_t *L = g->labels.as<int32_t>();
-    hipLaunchKernelGGL(k_init_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, g->dev(), g->all.dev,
-                       L, w.k, w.diff.as<int32_t>(), ctl, w.vc.as<unsigned long long>(),
-                       w.ic.as<unsigned long long>());
-    if (g->n_heavy > 0)
-        hipLaunchKernelGGL(k_init_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, g->dev(),
-                           g->all.dev, L, w.k, ctx->tune.chunk_edges, w.diff.as<int32_t>(), ctl,
-                           w.vc.as<unsigned long long>(), w.ic.as<unsigned long long>());
+P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, double epsilon,
+                 int32_t *remaining) {
+    P2Job j{};
+    j.g = g->dev();
+    j.all = g->all.dev;
+    j.B = w.B.dev;
+    j.D = w.D.dev;
+    j.L = g->labels.as<int32_t>();
+    j.L8 = w.L8.as<uint8_t>();
+    j.chg = w.chg.as<int32_t>();
+    j.diff = w.diff.as<int32_t>();
+    j.stamp = w.stamp.as<int32_t>();
+    j.bstamp = w.bstamp.as<int32_t>();
+    j.deferred = w.deferred.as<int32_t>();
+    j.hub_hist = w.hub_hist.as<int32_t>();
+    j.bd = w.bd.dev;
+    j.ctl = w.ctl.as<Ctl>();
+    j.vc = w.vc.as<unsigned long long>();
+    j.ic = w.ic.as<unsigned long long>();
+    j.pen = w.pen.as<double>();
+    j.re = w.re.as<double>();
+    j.rtab = w.rtab.as<double>();
+    j.trace = w.trace.as<int64_t>();
+    j.trace_cap = w.trace_cap;
+    j.remaining = remaining;
+    j.k = w.k;
+    j.heavy_degree = ctx->tune.heavy_degree;
+    j.chunk_edges = ctx->tune.chunk_edges;
     // Stamp values only grow, so stamps left by earlier runs never match.
-    const int32_t epoch_base = w.epoch;
+    j.epoch_base = w.epoch;
     w.epoch += max_rounds + 2;
-    hipLaunchKernelGGL(k_mirror8, dim3(ctx->cu_count * 4), dim3(kThreads), 0, s, g->n, L,
-                       w.L8.as<uint8_t>());
-    hipLaunchKernelGGL(k_init_finish, dim3(1), dim3(256), 0, s, ctl, w.ic.as<unsigned long long>(),
-                       w.k, epoch_base, max_rounds, epsilon);
+    j.max_rounds = max_rounds;
+    j.epsilon = epsilon;
+    j.pull_frac = ctx->tune.pull_fraction;
+    return j;
+}
+
+SelArgs phase2_sel_args(dmolp_ctx *ctx, Phase2Work &w) {
+    Ctl *ctl = w.ctl.as<Ctl>();
+    TakeRule rule{};
+    rule.kind = 0;
+    rule.ratio_tab = w.rtab.p ? w.rtab.as<double>() + w.rtab_rounds + 1 : nullptr;
+    rule.t_ptr = &ctl->t;
+    rule.re = w.re.as<double>();
+    return select_args(ctx, w.sel, w.bd.dev, rule, &ctl->done, w.B.dev, nullptr,
+                       w.bstamp.as<int32_t>(), &ctl->epoch);
+}
+
+P2Set phase2_single_set(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds,
+                        double epsilon) {
+    hipStream_t s = ctx->stream;
+    if (!w.remaining.p) {
+        w.remaining.alloc(sizeof(int32_t) * 2);
+        w.job_dev.alloc(sizeof(P2Job));
+        w.sel_dev.alloc(sizeof(SelArgs));
+    }
+    w.job_host = phase2_job(ctx, g, w, max_rounds, epsilon, w.remaining.as<int32_t>());
+    w.sel_host = phase2_sel_args(ctx, w);
+    DMOLP_HIP_CHECK(hipMemcpyAsync(w.job_dev.p, &w.job_host, sizeof(P2Job), hipMemcpyHostToDevice, s));
+    DMOLP_HIP_CHECK(hipMemcpyAsync(w.sel_dev.p, &w.sel_host, sizeof(SelArgs), hipMemcpyHostToDevice, s));
+    std::array<int32_t, P_NPOL> blk{};
+    job_blocks(ctx, g->n, g->nnz, w.k, g->n_heavy, true, blk.data());
+    bool same = w.ls.jobs == 1;
+    for (int p = 0; same && p < P_NPOL; ++p) same = w.ls.total[p] == blk[(size_t)p];
+    if (!same) launch_set_build(ctx, w.ls, {blk});
+    P2Set set;
+    set.jobs = w.job_dev.as<P2Job>();
+    set.sel = w.sel_dev.as<SelArgs>();
+    set.ls = &w.ls;
+    set.passes = w.sel.passes;
+    set.any_heavy = g->n_heavy > 0;
+    set.remaining = w.remaining.as<int32_t>();
+    return set;
+}
+
+void phase2_init_enqueue(dmolp_ctx *ctx, const P2Set &set) {
+    hipStream_t s = ctx->stream;
+    const LaunchSet &ls = *set.ls;
+    const int J = ls.jobs;
+    hipLaunchKernelGGL(k_set_remaining, dim3(1), dim3(1), 0, s, set.remaining, (int32_t)J);
+    hipLaunchKernelGGL(k_init_zero, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), J);
+    hipLaunchKernelGGL(k_init_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), J);
+    if (set.any_heavy)
+        hipLaunchKernelGGL(k_init_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                           ls.bs(P_GRID), J);
+    hipLaunchKernelGGL(k_mirror8, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), J);
+    hipLaunchKernelGGL(k_init_finish, dim3(ls.total[P_ONE]), dim3(256), 0, s, set.jobs, ls.bs(P_ONE),
+                       J);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 
-void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
+void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     hipStream_t s = ctx->stream;
-    Ctl *ctl = w.ctl.as<Ctl>();
-    const int32_t *done = &ctl->done;
-    const GraphDev gd = g->dev();
-    int32_t *L = g->labels.as<int32_t>();
-    List &bprev = w.B;  // holds B_{t-1} until the selection rewrites it as B_t
-    List &bcur = w.B;
-    const int H = ctx->tune.heavy_degree, CH = ctx->tune.chunk_edges;
-    const unsigned grid = grid_for(ctx);
+    const LaunchSet &ls = *set.ls;
+    const int J = ls.jobs;
+    const P2Job *jobs = set.jobs;
+    const unsigned grid = (unsigned)ls.total[P_GRID];
+    const int32_t *bg = ls.bs(P_GRID);
 
     auto mark = [&]() -> size_t {
         if (!ctx->profiling) return 0;
@@ -864,56 +1067,31 @@ void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
         return ctx->ev_used - 1;
     };
     const size_t m0 = mark();
-    const double *rtab = w.rtab.as<double>();
-    hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, s, ctl, w.vc.as<unsigned long long>(),
-                       w.ic.as<unsigned long long>(), w.k, rtab, w.pen.as<double>(),
-                       w.re.as<double>(), bprev.dev, w.D.dev, w.bd.dev, g->n, g->nnz,
-                       ctx->tune.pull_fraction);
+    hipLaunchKernelGGL(k_round_begin, dim3(ls.total[P_ONE]), dim3(64), 0, s, jobs, ls.bs(P_ONE), J);
     // Round 1 takes every vertex; later rounds pull or push (ctl->cand_mode).
-    hipLaunchKernelGGL(k_cand_all, dim3(grid), dim3(kThreads), 0, s, gd, L, w.diff.as<int32_t>(),
-                       w.bd.dev, ctl);
-    hipLaunchKernelGGL(k_cand_pull, dim3(grid), dim3(kThreads), 0, s, gd, L, w.diff.as<int32_t>(),
-                       w.bstamp.as<int32_t>(), w.bd.dev, w.deferred.as<int32_t>(), H, ctl);
-    if (g->n_heavy > 0)
-        hipLaunchKernelGGL(k_cand_pull_heavy, dim3(grid), dim3(kThreads), 0, s, gd, L,
-                           w.bstamp.as<int32_t>(), w.bd.dev, w.deferred.as<int32_t>(), H, ctl);
-    hipLaunchKernelGGL(k_cand_light, dim3(grid), dim3(kThreads), 0, s, gd, bprev.dev, L,
-                       w.diff.as<int32_t>(), w.stamp.as<int32_t>(), w.bd.dev, ctl);
-    if (g->n_heavy > 0)
-        hipLaunchKernelGGL(k_cand_heavy, dim3(grid), dim3(kThreads), 0, s, gd, bprev.dev, L,
-                           w.diff.as<int32_t>(), w.stamp.as<int32_t>(), CH, w.bd.dev, ctl);
+    hipLaunchKernelGGL(k_cand_all, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_cand_pull, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    if (set.any_heavy)
+        hipLaunchKernelGGL(k_cand_pull_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_cand_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    if (set.any_heavy)
+        hipLaunchKernelGGL(k_cand_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m1 = mark();
-    TakeRule rule{};
-    rule.kind = 0;
-    rule.ratio_tab = rtab + w.rtab_rounds + 1;
-    rule.t_ptr = &ctl->t;
-    rule.re = w.re.as<double>();
-    select_enqueue(ctx, w.sel, gd, w.bd.dev, rule, done, bcur.dev, nullptr, w.bstamp.as<int32_t>(),
-                   &ctl->epoch);
+    select_enqueue(ctx, set.sel, ls, set.passes);
     const size_t m2 = mark();
-    uint8_t *L8 = w.L8.as<uint8_t>();
-    hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, gd, L, bcur.dev, L8,
-                       w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH, ctl);
+    hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m3 = mark();
-    if (g->n_heavy > 0) {
-        hipLaunchKernelGGL(k_lp_heavy, dim3(grid), dim3(kThreads), 0, s, gd, L8, bcur.dev, w.k, CH,
-                           w.hub_hist.as<int32_t>(), ctl);
-        hipLaunchKernelGGL(k_lp_heavy_decide, dim3(heavy_grid(g)), dim3(kThreads), 0, s, L,
-                           bcur.dev, w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH,
-                           w.hub_hist.as<int32_t>(), gd.off, ctl);
+    if (set.any_heavy) {
+        hipLaunchKernelGGL(k_lp_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+        hipLaunchKernelGGL(k_lp_heavy_decide, dim3(ls.total[P_HEAVY]), dim3(kThreads), 0, s, jobs,
+                           ls.bs(P_HEAVY), J);
     }
     const size_t m4 = mark();
-    hipLaunchKernelGGL(k_delta_light, dim3(grid), dim3(kThreads), 0, s, gd, w.D.dev, L,
-                       w.chg.as<int32_t>(), w.k, w.diff.as<int32_t>(), ctl,
-                       w.vc.as<unsigned long long>(), w.ic.as<unsigned long long>());
-    if (g->n_heavy > 0)
-        hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, gd, w.D.dev, L,
-                           w.chg.as<int32_t>(), w.k, CH, w.diff.as<int32_t>(), ctl,
-                           w.vc.as<unsigned long long>(), w.ic.as<unsigned long long>());
-    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, w.D.dev, L, L8,
-                       w.chg.as<int32_t>(), ctl);
-    hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ctl, bcur.dev, w.D.dev, w.bd.dev,
-                       w.trace.as<int64_t>(), w.trace_cap, g->n);
+    hipLaunchKernelGGL(k_delta_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    if (set.any_heavy)
+        hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_round_end, dim3(ls.total[P_ONE]), dim3(1), 0, s, jobs, ls.bs(P_ONE), J);
     DMOLP_HIP_CHECK(hipGetLastError());
     if (ctx->profiling) {
         const size_t m5 = mark();
@@ -950,43 +1128,44 @@ __global__ void k_hook_state(Ctl *ctl, int32_t t, int32_t epoch, int32_t mode) {
 
 }  // namespace
 
-void phase2_round_launch(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
-    if (ctx->profiling) {  // events between kernels: launch directly
-        phase2_round_enqueue(ctx, g, w);
-        return;
-    }
+uint64_t phase2_single_key(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     // Everything the captured launches bake in.
     uint64_t key = 0x2545f4914f6cdd1dull;
-    const void *ptrs[] = {g->labels.p, g->off.p,    g->col.p,   w.trace.p,    w.rtab.p,
-                          w.B.light.p, w.D.light.p, w.vc.p,     w.sel.h1.p,   w.sel.h2.p,
-                          w.sel.st.p,  w.hub_hist.p, w.bd.x.p,  w.deferred.p, w.chg.p,
-                          w.sel.ref.x.p, w.L8.p};
+    const void *ptrs[] = {w.job_dev.p, w.sel_dev.p, w.ls.bstart.p, w.remaining.p, g->labels.p};
     for (const void *p : ptrs) key = mix_key(key, (uint64_t)(uintptr_t)p);
-    const int64_t ints[] = {w.k, ctx->tune.heavy_degree, ctx->tune.chunk_edges, ctx->grid_blocks(),
-                            g->n_heavy, w.trace_cap, w.rtab_rounds, w.sel.passes,
-                            (int64_t)(ctx->tune.pull_fraction * 1e9)};
+    int64_t ints[3 + P_NPOL] = {w.sel.passes, g->n_heavy > 0 ? 1 : 0, w.ls.jobs};
+    for (int p = 0; p < P_NPOL; ++p) ints[3 + p] = w.ls.total[p];
     for (int64_t v : ints) key = mix_key(key, (uint64_t)v);
-    if (!w.round_exec || w.round_key != key) {
-        if (w.round_exec) {
-            DMOLP_HIP_CHECK(hipGraphExecDestroy(w.round_exec));
-            w.round_exec = nullptr;
+    return key;
+}
+
+void phase2_round_launch(dmolp_ctx *ctx, const P2Set &set, hipGraphExec_t *exec, uint64_t *exec_key,
+                         uint64_t key) {
+    if (ctx->profiling) {  // events between kernels: launch directly
+        phase2_round_enqueue(ctx, set);
+        return;
+    }
+    if (!*exec || *exec_key != key) {
+        if (*exec) {
+            DMOLP_HIP_CHECK(hipGraphExecDestroy(*exec));
+            *exec = nullptr;
         }
         hipGraph_t graph = nullptr;
         DMOLP_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         try {
-            phase2_round_enqueue(ctx, g, w);
+            phase2_round_enqueue(ctx, set);
         } catch (...) {
             (void)hipStreamEndCapture(ctx->stream, &graph);
             if (graph) (void)hipGraphDestroy(graph);
             throw;
         }
         DMOLP_HIP_CHECK(hipStreamEndCapture(ctx->stream, &graph));
-        const hipError_t e = hipGraphInstantiate(&w.round_exec, graph, nullptr, nullptr, 0);
+        const hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         DMOLP_HIP_CHECK(e);
-        w.round_key = key;
+        *exec_key = key;
     }
-    DMOLP_HIP_CHECK(hipGraphLaunch(w.round_exec, ctx->stream));
+    DMOLP_HIP_CHECK(hipGraphLaunch(*exec, ctx->stream));
 }
 
 // Test hook: the candidate + selection steps of round t from an explicit
@@ -995,48 +1174,36 @@ void phase2_round_launch(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
 void phase2_select_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t) {
     hipStream_t s = ctx->stream;
     Ctl *ctl = w.ctl.as<Ctl>();
-    const GraphDev gd = g->dev();
-    const int32_t *L = g->labels.as<int32_t>();
     phase2_round_table(ctx, w, t > w.rtab_rounds ? t : w.rtab_rounds);
-    w.epoch += 2;
+    const P2Set set = phase2_single_set(ctx, g, w, t, 0.0);
+    const LaunchSet &ls = *set.ls;
     hipLaunchKernelGGL(k_hook_state, dim3(1), dim3(1), 0, s, ctl, t, w.epoch, (int32_t)CAND_PUSH);
     DMOLP_HIP_CHECK(hipMemsetAsync(w.bd.cnt.p, 0, w.bd.cnt.bytes, s));
-    hipLaunchKernelGGL(k_cand_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, w.B.dev, L,
-                       w.diff.as<int32_t>(), w.stamp.as<int32_t>(), w.bd.dev, ctl);
-    if (g->n_heavy > 0)
-        hipLaunchKernelGGL(k_cand_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, w.B.dev,
-                           L, w.diff.as<int32_t>(), w.stamp.as<int32_t>(), ctx->tune.chunk_edges,
-                           w.bd.dev, ctl);
-    TakeRule rule{};
-    rule.kind = 0;
-    rule.ratio_tab = w.rtab.as<double>() + w.rtab_rounds + 1;
-    rule.t_ptr = &ctl->t;
-    rule.re = w.re.as<double>();
-    select_enqueue(ctx, w.sel, gd, w.bd.dev, rule, &ctl->done, w.B.dev, nullptr);
+    hipLaunchKernelGGL(k_cand_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), ls.jobs);
+    if (set.any_heavy)
+        hipLaunchKernelGGL(k_cand_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                           ls.bs(P_GRID), ls.jobs);
+    select_enqueue(ctx, set.sel, ls, set.passes);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 
 // Test hook: decisions for the list in w.B; Delta applied to g's labels.
 void phase2_lp_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     hipStream_t s = ctx->stream;
-    Ctl *ctl = w.ctl.as<Ctl>();
-    const GraphDev gd = g->dev();
-    int32_t *L = g->labels.as<int32_t>();
-    const int H = ctx->tune.heavy_degree, CH = ctx->tune.chunk_edges;
+    const P2Set set = phase2_single_set(ctx, g, w, 1, 0.0);
+    const LaunchSet &ls = *set.ls;
+    const unsigned grid = (unsigned)ls.total[P_GRID];
+    const int32_t *bg = ls.bs(P_GRID);
     DMOLP_HIP_CHECK(hipMemsetAsync(w.D.cnt.p, 0, w.D.cnt.bytes, s));
-    uint8_t *L8 = w.L8.as<uint8_t>();
-    hipLaunchKernelGGL(k_mirror8, dim3(ctx->cu_count * 4), dim3(kThreads), 0, s, g->n, L, L8);
-    hipLaunchKernelGGL(k_lp_light, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L, w.B.dev, L8,
-                       w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH, ctl);
-    if (g->n_heavy > 0) {
-        hipLaunchKernelGGL(k_lp_heavy, dim3(grid_for(ctx)), dim3(kThreads), 0, s, gd, L8, w.B.dev,
-                           w.k, CH, w.hub_hist.as<int32_t>(), ctl);
-        hipLaunchKernelGGL(k_lp_heavy_decide, dim3(heavy_grid(g)), dim3(kThreads), 0, s, L,
-                           w.B.dev, w.pen.as<double>(), w.k, w.chg.as<int32_t>(), w.D.dev, H, CH,
-                           w.hub_hist.as<int32_t>(), gd.off, ctl);
+    hipLaunchKernelGGL(k_mirror8, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs);
+    hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs);
+    if (set.any_heavy) {
+        hipLaunchKernelGGL(k_lp_heavy, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs);
+        hipLaunchKernelGGL(k_lp_heavy_decide, dim3(ls.total[P_HEAVY]), dim3(kThreads), 0, s,
+                           set.jobs, ls.bs(P_HEAVY), ls.jobs);
     }
-    hipLaunchKernelGGL(k_apply, dim3(grid_for(ctx)), dim3(kThreads), 0, s, w.D.dev, L, L8,
-                       w.chg.as<int32_t>(), ctl);
+    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 

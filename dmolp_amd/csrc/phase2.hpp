@@ -45,6 +45,36 @@ enum TraceCol {
     T_PREV_N = 0, T_PREV_DEG, T_CAND, T_BD, T_B, T_BDEG, T_CHG, T_CHGDEG, T_CUT, T_BL, T_BLDEG, T_NCOL
 };
 
+// Everything one graph's phase-2 kernels read and write (device pointers).
+struct P2Job {
+    GraphDev g;
+    ListDev all, B, D;
+    int32_t *L;
+    uint8_t *L8;
+    int32_t *chg, *diff, *stamp, *bstamp, *deferred, *hub_hist;
+    RecDev bd;
+    Ctl *ctl;
+    unsigned long long *vc, *ic;
+    double *pen, *re;
+    const double *rtab;
+    int64_t *trace;
+    int32_t *remaining;  // unfinished jobs of the set (shared by the set)
+    int32_t trace_cap, k, heavy_degree, chunk_edges;
+    int32_t epoch_base, max_rounds;
+    double epsilon;
+    double pull_frac;
+};
+
+// A set of jobs that advance together, one launch per kernel step.
+struct P2Set {
+    const P2Job *jobs = nullptr;    // device
+    const SelArgs *sel = nullptr;   // device, one per job
+    const LaunchSet *ls = nullptr;
+    int passes = 0;                 // largest selection digit-pass count
+    bool any_heavy = false;
+    int32_t *remaining = nullptr;   // device counter of unfinished jobs
+};
+
 struct Phase2Work {
     int k = 0;
     int64_t cap_heavy = -1, cap_chunks = -1;  // heavy capacities the lists were sized for
@@ -56,6 +86,11 @@ struct Phase2Work {
     int rtab_rounds = -1;
     hipGraphExec_t round_exec = nullptr;  // one captured round
     uint64_t round_key = 0;               // launch shape the capture was made for
+    // This graph as a job set of one (the plain path).
+    DevMem job_dev, sel_dev, remaining;
+    LaunchSet ls;
+    P2Job job_host{};
+    SelArgs sel_host{};
     ~Phase2Work() {
         if (round_exec) (void)hipGraphExecDestroy(round_exec);
     }
@@ -73,14 +108,27 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
 // Per-round constant table for rounds 1..R (host libm, as the reference).
 void phase2_round_table(dmolp_ctx *ctx, Phase2Work &w, int R);
 
+// The job record of g for a run of at most max_rounds rounds (advances the
+// stamp epoch).  `remaining` is the set's counter of unfinished jobs.
+P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, double epsilon,
+                 int32_t *remaining);
+SelArgs phase2_sel_args(dmolp_ctx *ctx, Phase2Work &w);
+// g as a job set of one, uploaded; returns the set view.
+P2Set phase2_single_set(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds,
+                        double epsilon);
+
 // Enqueue: reset state, full sweep of the labels (vc, ic, cut, per-vertex
-// differing-neighbour counts).  Labels are g's resident labels.
-void phase2_init_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds = 0,
-                         double epsilon = 0.0);
-// Enqueue one LP round; the round index lives in the control block, so
-// the same enqueue (or its captured graph) serves every round.
-void phase2_round_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
-// Launch one round through a captured HIP graph (captured on first use).
-void phase2_round_launch(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
+// differing-neighbour counts).  Labels are each graph's resident labels.
+void phase2_init_enqueue(dmolp_ctx *ctx, const P2Set &set);
+// Enqueue one LP round of every job; the round index lives in each job's
+// control block, so the same enqueue (or its captured graph) serves every
+// round.
+void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set);
+// Launch one round through a captured HIP graph (captured on first use,
+// cached in *exec under `key`).
+void phase2_round_launch(dmolp_ctx *ctx, const P2Set &set, hipGraphExec_t *exec, uint64_t *exec_key,
+                         uint64_t key);
+// Shape key of the single-graph set for phase2_round_launch.
+uint64_t phase2_single_key(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
 
 }  // namespace dmolp

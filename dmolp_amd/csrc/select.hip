@@ -53,16 +53,27 @@ __device__ void block_crossing(const int32_t *bins, int nb, bool reverse, int64_
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kSelThreads) void k_sel_hist1(RecDev in, int32_t *h1, int groups,
-                                                           int b1_bins, const int32_t *done) {
-    if (done && *done) return;
+// Kernels take the job set (one SelArgs per job) and run on the job's
+// share of the launch.
+#define SEL_JOB()                                      \
+    const int job_ = job_of_block(bstart, jobs);       \
+    const VGrid vg = vgrid_of(bstart, job_);           \
+    const SelArgs &a = A[job_];                        \
+    (void)vg
+
+__global__ __launch_bounds__(kSelThreads) void k_sel_hist1(const SelArgs *A, const int32_t *bstart,
+                                                           int jobs) {
+    SEL_JOB();
+    if (a.done && *a.done) return;
     __shared__ int32_t s_h[kHist1LdsInts];
-    const int nb = groups * b1_bins;
+    const RecDev in = a.in;
+    int32_t *h1 = a.h1;
+    const int b1_bins = a.b1_bins;
+    const int nb = a.groups * b1_bins;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) s_h[i] = 0;
     __syncthreads();
     const int64_t count = (int64_t)*in.cnt;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = vthread(vg); i < count; i += vthreads(vg)) {
         const int32_t gg = in.g[i];
         atomicAdd(&s_h[gg * b1_bins + bin1_of(in.d[i], b1_bins)], 1);
     }
@@ -71,19 +82,20 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(RecDev in, int32_t *h
         if (s_h[i]) atomicAdd(&h1[i], s_h[i]);
 }
 
-__global__ __launch_bounds__(kSelThreads) void k_sel_pick1(int32_t *h1, int b1_bins, SelState *st,
-                                                           TakeRule rule, int id_bits, int deg_bits,
-                                                           int32_t *refining, ListDev sel,
-                                                           unsigned long long *ref_cnt,
-                                                           const int32_t *done) {
-    if (done && *done) return;
-    if (blockIdx.x == 0 && threadIdx.x < LC_N) sel.cnt[threadIdx.x] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *ref_cnt = 0;
+__global__ __launch_bounds__(kSelThreads) void k_sel_pick1(const SelArgs *A, const int32_t *bstart,
+                                                           int jobs) {
+    SEL_JOB();
+    if (a.done && *a.done) return;
+    const ListDev sel = a.sel;
+    if (vg.block == 0 && threadIdx.x < LC_N) sel.cnt[threadIdx.x] = 0;
+    if (vg.block == 0 && threadIdx.x == 0) *a.ref.cnt = 0;
     __shared__ int64_t s_tmp[kSelThreads];
     __shared__ int s_bin;
     __shared__ int64_t s_before;
-    const int g = blockIdx.x;
-    int32_t *bins = h1 + (int64_t)g * b1_bins;
+    const int g = (int)vg.block;
+    const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
+    const TakeRule rule = a.rule;
+    int32_t *bins = a.h1 + (int64_t)g * b1_bins;
     // Group size.
     int64_t part = 0;
     for (int j = threadIdx.x; j < b1_bins; j += blockDim.x) part += bins[j];
@@ -135,10 +147,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_pick1(int32_t *h1, int b1_b
                 s.prefix = 0;
                 s.r = id_bits + deg_bits;
             }
-            if (threadIdx.x == 0) atomicAdd(refining, 1);
+            if (threadIdx.x == 0) atomicAdd(a.refining, 1);
         }
     }
-    if (threadIdx.x == 0) st[g] = s;
+    if (threadIdx.x == 0) a.st[g] = s;
     __syncthreads();
     for (int j = threadIdx.x; j < b1_bins; j += blockDim.x) bins[j] = 0;
 }
@@ -146,16 +158,18 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_pick1(int32_t *h1, int b1_b
 // The records the digit passes still have to look at: those in the chosen
 // level-1 bin of a group whose cut falls inside that bin.  Usually a small
 // share of all records, so the passes below scan this list only.
-__global__ __launch_bounds__(kSelThreads) void k_sel_collect(RecDev in, const SelState *st,
-                                                             int b1_bins, RecDev ref,
-                                                             const int32_t *refining,
-                                                             const int32_t *done) {
-    if (done && *done) return;
-    if (*refining == 0) return;
+__global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, const int32_t *bstart,
+                                                             int jobs) {
+    SEL_JOB();
+    if (a.done && *a.done) return;
+    if (*a.refining == 0) return;
     __shared__ BlockAppendLds s_app;
+    const RecDev in = a.in, ref = a.ref;
+    const SelState *st = a.st;
+    const int b1_bins = a.b1_bins;
     const int64_t count = (int64_t)*in.cnt;
     const int64_t span = (int64_t)blockDim.x * kCompactItems;
-    for (int64_t b0 = (int64_t)blockIdx.x * span; b0 < count; b0 += (int64_t)gridDim.x * span) {
+    for (int64_t b0 = vg.block * span; b0 < count; b0 += vg.blocks * span) {
         int32_t x[kCompactItems], gg[kCompactItems], d[kCompactItems];
         uint32_t keep_mask = 0;
         int32_t nk = 0;
@@ -185,16 +199,17 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect(RecDev in, const Se
     }
 }
 
-__global__ __launch_bounds__(kSelThreads) void k_sel_hist2(RecDev in, int32_t *h2,
-                                                           const SelState *st, int b1_bins,
-                                                           int id_bits, int deg_bits,
-                                                           const int32_t *refining,
-                                                           const int32_t *done) {
-    if (done && *done) return;
-    if (*refining == 0) return;
+__global__ __launch_bounds__(kSelThreads) void k_sel_hist2(const SelArgs *A, const int32_t *bstart,
+                                                           int jobs) {
+    SEL_JOB();
+    if (a.done && *a.done) return;
+    if (*a.refining == 0) return;
+    const RecDev in = a.ref;
+    int32_t *h2 = a.h2;
+    const SelState *st = a.st;
+    const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
     const int64_t count = (int64_t)*in.cnt;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = vthread(vg); i < count; i += vthreads(vg)) {
         const int32_t gg = in.g[i];
         const SelState s = st[gg];
         if (s.mode != SEL_REFINE) continue;
@@ -208,19 +223,19 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(RecDev in, int32_t *h
     }
 }
 
-__global__ __launch_bounds__(kSelThreads) void k_sel_pick2(int32_t *h2, SelState *st,
-                                                           int32_t *refining,
-                                                           const int32_t *done) {
-    if (done && *done) return;
+__global__ __launch_bounds__(kSelThreads) void k_sel_pick2(const SelArgs *A, const int32_t *bstart,
+                                                           int jobs) {
+    SEL_JOB();
+    if (a.done && *a.done) return;
     __shared__ int64_t s_tmp[kSelThreads];
     __shared__ int s_bin;
     __shared__ int64_t s_before;
-    const int g = blockIdx.x;
-    SelState s = st[g];
+    const int g = (int)vg.block;
+    SelState s = a.st[g];
     if (s.mode != SEL_REFINE) return;  // uniform per block
     const int w = s.r < kDigitBits ? s.r : kDigitBits;
     const int nb = 1 << w;
-    int32_t *bins = h2 + (int64_t)g * kDigitBins;
+    int32_t *bins = a.h2 + (int64_t)g * kDigitBins;
     block_crossing(bins, nb, false, s.need, &s_bin, &s_before, s_tmp);
     const int digit = s_bin;
     const int64_t cnt = bins[digit];
@@ -229,30 +244,32 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_pick2(int32_t *h2, SelState
     s.r -= w;
     if (s.need == cnt) {
         s.mode = SEL_KEY;
-        if (threadIdx.x == 0) atomicSub(refining, 1);
+        if (threadIdx.x == 0) atomicSub(a.refining, 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) st[g] = s;
+    if (threadIdx.x == 0) a.st[g] = s;
     for (int j = threadIdx.x; j < nb; j += blockDim.x) bins[j] = 0;
 }
 
-__global__ __launch_bounds__(kSelThreads) void k_sel_compact(RecDev in, const SelState *st,
-                                                             int b1_bins, int id_bits,
-                                                             int deg_bits, ListDev sel,
-                                                             RecDev carry, int has_carry,
-                                                             int heavy_degree, int chunk_edges,
-                                                             int32_t *mark,
-                                                             const int32_t *mark_val_ptr,
-                                                             const int32_t *done) {
-    if (done && *done) return;
-    const int32_t mark_val = mark ? *mark_val_ptr : 0;
+__global__ __launch_bounds__(kSelThreads) void k_sel_compact(const SelArgs *A, const int32_t *bstart,
+                                                             int jobs) {
+    SEL_JOB();
+    if (a.done && *a.done) return;
+    int32_t *mark = a.mark;
+    const int32_t mark_val = mark ? *a.mark_val : 0;
+    const RecDev in = a.in, carry = a.carry;
+    const ListDev sel = a.sel;
+    const SelState *st = a.st;
+    const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
+    const int heavy_degree = a.heavy_degree, chunk_edges = a.chunk_edges;
+    const int has_carry = a.has_carry;
     __shared__ BlockAppendLds s_app;
     const int64_t count = (int64_t)*in.cnt;
     const int64_t span = (int64_t)blockDim.x * kCompactItems;
     // Block-uniform loop; each thread takes kCompactItems records (strided
     // by the block size, so loads stay coalesced) and reserves its output
     // slots with one block-level atomic per iteration.
-    for (int64_t b0 = (int64_t)blockIdx.x * span; b0 < count; b0 += (int64_t)gridDim.x * span) {
+    for (int64_t b0 = vg.block * span; b0 < count; b0 += vg.blocks * span) {
         int32_t x[kCompactItems], gg[kCompactItems], d[kCompactItems];
         uint32_t take_mask = 0, keep_mask = 0;
         int32_t n_light = 0, n_keep = 0;
@@ -322,6 +339,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(RecDev in, const Se
     }
 }
 
+#undef SEL_JOB
+
 }  // namespace
 
 void SelectWork::prepare(dmolp_ctx *ctx, int ngroups, int64_t n, int32_t max_degree) {
@@ -340,7 +359,6 @@ void SelectWork::prepare(dmolp_ctx *ctx, int ngroups, int64_t n, int32_t max_deg
     DMOLP_REQUIRE(b1 >= 2, DMOLP_E_UNSUPPORTED, "too many groups for the selection histogram");
     b1_bins = b1;
     passes = (id_bits + deg_bits + kDigitBits - 1) / kDigitBits;
-    hist1_blocks = ctx->cu_count * 2;
     h1.alloc(sizeof(int32_t) * (size_t)groups * b1_bins);
     h2.alloc(sizeof(int32_t) * (size_t)groups * kDigitBins);
     st.alloc(sizeof(SelState) * (size_t)groups);
@@ -350,31 +368,91 @@ void SelectWork::prepare(dmolp_ctx *ctx, int ngroups, int64_t n, int32_t max_deg
     DMOLP_HIP_CHECK(hipMemsetAsync(refining.p, 0, refining.bytes, ctx->stream));
 }
 
-void select_enqueue(dmolp_ctx *ctx, SelectWork &w, const GraphDev &g, const RecDev &in,
-                    const TakeRule &rule, const int32_t *done, const ListDev &sel,
-                    const RecDev *carry, int32_t *mark, const int32_t *mark_val) {
-    (void)g;
+SelArgs select_args(dmolp_ctx *ctx, SelectWork &w, const RecDev &in, const TakeRule &rule,
+                    const int32_t *done, const ListDev &sel, const RecDev *carry, int32_t *mark,
+                    const int32_t *mark_val) {
+    SelArgs a{};
+    a.in = in;
+    a.sel = sel;
+    a.carry = carry ? *carry : RecDev{nullptr, nullptr, nullptr, nullptr};
+    a.has_carry = carry ? 1 : 0;
+    a.ref = w.ref.dev;
+    a.h1 = w.h1.as<int32_t>();
+    a.h2 = w.h2.as<int32_t>();
+    a.st = w.st.as<SelState>();
+    a.refining = w.refining.as<int32_t>();
+    a.done = done;
+    a.mark = mark;
+    a.mark_val = mark_val;
+    a.rule = rule;
+    a.groups = w.groups;
+    a.b1_bins = w.b1_bins;
+    a.id_bits = w.id_bits;
+    a.deg_bits = w.deg_bits;
+    a.heavy_degree = ctx->tune.heavy_degree;
+    a.chunk_edges = ctx->tune.chunk_edges;
+    return a;
+}
+
+void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, int passes) {
     hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(k_sel_hist1, dim3(w.hist1_blocks), dim3(kSelThreads), 0, s, in,
-                       w.h1.as<int32_t>(), w.groups, w.b1_bins, done);
-    hipLaunchKernelGGL(k_sel_pick1, dim3(w.groups), dim3(kSelThreads), 0, s, w.h1.as<int32_t>(),
-                       w.b1_bins, w.st.as<SelState>(), rule, w.id_bits, w.deg_bits,
-                       w.refining.as<int32_t>(), sel, w.ref.dev.cnt, done);
-    hipLaunchKernelGGL(k_sel_collect, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, in,
-                       w.st.as<SelState>(), w.b1_bins, w.ref.dev, w.refining.as<int32_t>(), done);
-    for (int p = 0; p < w.passes; ++p) {
-        hipLaunchKernelGGL(k_sel_hist2, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, w.ref.dev,
-                           w.h2.as<int32_t>(), w.st.as<SelState>(), w.b1_bins, w.id_bits,
-                           w.deg_bits, w.refining.as<int32_t>(), done);
-        hipLaunchKernelGGL(k_sel_pick2, dim3(w.groups), dim3(kSelThreads), 0, s, w.h2.as<int32_t>(),
-                           w.st.as<SelState>(), w.refining.as<int32_t>(), done);
+    const int J = ls.jobs;
+    hipLaunchKernelGGL(k_sel_hist1, dim3(ls.total[P_HIST1]), dim3(kSelThreads), 0, s, args,
+                       ls.bs(P_HIST1), J);
+    hipLaunchKernelGGL(k_sel_pick1, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
+                       ls.bs(P_GROUPS), J);
+    hipLaunchKernelGGL(k_sel_collect, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                       ls.bs(P_GRID), J);
+    for (int p = 0; p < passes; ++p) {
+        hipLaunchKernelGGL(k_sel_hist2, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                           ls.bs(P_GRID), J);
+        hipLaunchKernelGGL(k_sel_pick2, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
+                           ls.bs(P_GROUPS), J);
     }
-    RecDev c = carry ? *carry : RecDev{nullptr, nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL(k_sel_compact, dim3(ctx->grid_blocks()), dim3(kSelThreads), 0, s, in,
-                       w.st.as<SelState>(), w.b1_bins, w.id_bits, w.deg_bits, sel, c,
-                       carry ? 1 : 0, ctx->tune.heavy_degree, ctx->tune.chunk_edges, mark,
-                       mark_val, done);
+    hipLaunchKernelGGL(k_sel_compact, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                       ls.bs(P_GRID), J);
     DMOLP_HIP_CHECK(hipGetLastError());
+}
+
+void job_blocks(const dmolp_ctx *ctx, int64_t n, int64_t nnz, int groups, int64_t heavy, bool single,
+                int32_t out[P_NPOL]) {
+    const int64_t full = ctx->grid_blocks();
+    if (single) {
+        out[P_GRID] = (int32_t)full;
+        out[P_HIST1] = ctx->cu_count * 2;
+    } else {
+        // Blocks in proportion to the job's work, so small graphs of a batch
+        // do not hold idle blocks.
+        const int64_t per = ctx->tune.batch_work_per_block > 0 ? ctx->tune.batch_work_per_block : 32768;
+        int64_t b = (n + nnz / 4 + per - 1) / per;
+        out[P_GRID] = (int32_t)(b < 1 ? 1 : (b > full ? full : b));
+        int64_t h = (n + 4 * per - 1) / (4 * per);
+        out[P_HIST1] = (int32_t)(h < 1 ? 1 : (h > ctx->cu_count * 2 ? ctx->cu_count * 2 : h));
+    }
+    out[P_GROUPS] = groups > 0 ? groups : 1;
+    out[P_ONE] = 1;
+    int64_t hb = (heavy + 255) / 256;
+    out[P_HEAVY] = (int32_t)(hb < 1 ? 1 : (hb > 1024 ? 1024 : hb));
+}
+
+void launch_set_build(dmolp_ctx *ctx, LaunchSet &ls, const std::vector<std::array<int32_t, P_NPOL>> &blocks) {
+    const int J = (int)blocks.size();
+    std::vector<int32_t> h((size_t)P_NPOL * (J + 1));
+    for (int p = 0; p < P_NPOL; ++p) {
+        int64_t run = 0;
+        for (int j = 0; j < J; ++j) {
+            h[(size_t)p * (J + 1) + j] = (int32_t)run;
+            run += blocks[(size_t)j][(size_t)p];
+        }
+        h[(size_t)p * (J + 1) + J] = (int32_t)run;
+        DMOLP_REQUIRE(run < (1ll << 31), DMOLP_E_UNSUPPORTED, "launch too large");
+        ls.total[p] = (int)run;
+    }
+    if (ls.jobs != J || !ls.bstart.p) ls.bstart.alloc(sizeof(int32_t) * h.size());
+    ls.jobs = J;
+    DMOLP_HIP_CHECK(hipMemcpyAsync(ls.bstart.p, h.data(), sizeof(int32_t) * h.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace dmolp

@@ -11,6 +11,9 @@
 // descending order) find the exact threshold key.  Every pass is a
 // histogram, so the selected set does not depend on record order.
 #pragma once
+#include <array>
+#include <vector>
+
 #include "internal.hpp"
 
 namespace dmolp {
@@ -68,21 +71,43 @@ struct SelectWork {
     int id_bits = 0;
     int deg_bits = 0;
     int passes = 0;
-    int hist1_blocks = 0;
     DevMem h1, h2, st, refining;
     Recs ref;  // records still being refined after the level-1 pick
     int64_t ref_cap = 0;
     void prepare(dmolp_ctx *ctx, int groups, int64_t n, int32_t max_degree);
 };
 
-// Enqueue the whole selection: level-1 histogram, pick, digit passes,
-// compaction.  Selected records go to `sel` (a vertex list); when `carry`
-// is non-null the unselected records are appended to it.
-// `sel` is reset before it is written.  `mark` (nullable): selected
-// vertices get mark[x] = *mark_val.
-void select_enqueue(dmolp_ctx *ctx, SelectWork &w, const GraphDev &g, const RecDev &in,
-                    const TakeRule &rule, const int32_t *done, const ListDev &sel,
-                    const RecDev *carry, int32_t *mark = nullptr,
-                    const int32_t *mark_val = nullptr);
+// Everything one job's selection reads and writes (device pointers).
+struct SelArgs {
+    RecDev in;         // records to select from
+    ListDev sel;       // selected vertices (reset before writing)
+    RecDev carry;      // unselected records are appended here when has_carry
+    RecDev ref;
+    int32_t *h1, *h2;
+    SelState *st;
+    int32_t *refining;
+    const int32_t *done;      // the job's stop flag (nullable)
+    int32_t *mark;            // selected vertices get mark[x] = *mark_val (nullable)
+    const int32_t *mark_val;
+    TakeRule rule;
+    int32_t groups, b1_bins, id_bits, deg_bits;
+    int32_t has_carry, heavy_degree, chunk_edges, reserved;
+};
+
+SelArgs select_args(dmolp_ctx *ctx, SelectWork &w, const RecDev &in, const TakeRule &rule,
+                    const int32_t *done, const ListDev &sel, const RecDev *carry,
+                    int32_t *mark = nullptr, const int32_t *mark_val = nullptr);
+
+// Enqueue the whole selection for a set of jobs (args: device array, one
+// SelArgs per job): level-1 histogram, pick, digit passes, compaction.
+// `passes` is the largest digit-pass count over the jobs.
+void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, int passes);
+
+// Block counts of one job for the launch policies (jobs with n vertices
+// and nnz entries, `groups` selection groups, `heavy` heavy rows).
+void job_blocks(const dmolp_ctx *ctx, int64_t n, int64_t nnz, int groups, int64_t heavy,
+                bool single, int32_t out[P_NPOL]);
+// Upload the block-start arrays of a job set.
+void launch_set_build(dmolp_ctx *ctx, LaunchSet &ls, const std::vector<std::array<int32_t, P_NPOL>> &blocks);
 
 }  // namespace dmolp

@@ -100,6 +100,7 @@ typedef struct {
     int32_t grid_blocks;   /* workgroups of the persistent row kernels */
     int32_t pull_per_mille; /* find candidates from the boundary side when B_{t-1}
                                covers at least this many per mille of the CSR entries */
+    int32_t batch_work_per_block; /* batched launches: (vertices + entries/4) per workgroup */
 } dmolp_tuning;
 int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out);
 int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t);
