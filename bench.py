@@ -173,6 +173,113 @@ def run_batched(a):
     return out
 
 
+C5_WORKLOAD = "served-rmat-n1e3to1e7-ef8to32-seed5"
+
+
+def c5_jobs(count, seed=5):
+    """Requests: R-MAT, vertices log-uniform in 1e3..1e7 (scale = nearest
+    power of two), edge factor uniform in 8..32, k from {4..128}."""
+    from dmolp_amd import api
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = float(np.exp(rng.uniform(np.log(1e3), np.log(1e7))))
+        scale = int(min(23, max(10, round(np.log2(n)))))
+        out.append((api.rmat_spec(scale, int(rng.integers(8, 33)), seed=5000 + i),
+                    int(rng.choice([4, 8, 16, 32, 64, 128]))))
+    return out
+
+
+def run_served(a):
+    """Served mode: Poisson arrivals at several rates; continuous batching
+    against a one-job-at-a-time plain server (FIFO over measured plain
+    service times of the same jobs)."""
+    from dmolp_amd import api
+    theta, rounds, eps = THETA, 20, 0.03
+    rates = [10, 30, 100, 300, 1000]
+    jobs = c5_jobs(a.jobs)
+    ctx = api.Context(0)
+    # Plain service time of every job (generate + phase 1 + phase 2 + free).
+    service = []
+    for it in range(2):
+        service = []
+        for sp, k in jobs:
+            t0 = time.perf_counter()
+            g = ctx.generate(sp)
+            g.phase1(k, theta, want_labels=False)
+            g.phase2(k, rounds, eps, want_labels=False)
+            g.free()
+            service.append(time.perf_counter() - t0)
+    log(f"[bench] plain service: total {sum(service):.2f} s over {len(jobs)} jobs, "
+        f"max {max(service) * 1e3:.1f} ms")
+    srv = api.Server(ctx, 200 << 30, a.max_active)
+    per_rate = {}
+    rng = np.random.default_rng(5)
+    for lam in rates:
+        gaps = rng.exponential(1.0 / lam, len(jobs))
+        arr_rel = np.cumsum(gaps) - gaps[0]
+        base = srv.clock() + 0.01
+        arr = base + arr_rel
+        tickets, done, nxt = [], {}, 0
+        while len(done) < len(jobs):
+            now = srv.clock()
+            while nxt < len(jobs) and arr[nxt] <= now:
+                sp, k = jobs[nxt]
+                tickets.append(srv.submit(k, theta, rounds, eps, spec=sp, arrival_s=float(arr[nxt])))
+                nxt += 1
+            active, queued = srv.step()
+            for t in tickets:
+                if t not in done:
+                    r = srv.take(t)
+                    if r is not None:
+                        done[t] = r[0]
+            if active == 0 and queued == 0 and nxt < len(jobs) and len(done) == nxt:
+                time.sleep(max(0.0, min(0.05, arr[nxt] - srv.clock())))
+        lat = np.array([done[t].done_s - done[t].arrival_s for t in tickets])
+        span = max(d.done_s for d in done.values()) - float(arr[0])
+        bad = [t for t in tickets if done[t].result.status != 0]
+        # Plain server: FIFO, one job at a time.
+        fin, plat = 0.0, []
+        for x, sv in zip(arr_rel, service):
+            fin = max(fin, x) + sv
+            plat.append(fin - x)
+        plat = np.array(plat)
+        per_rate[str(lam)] = {
+            "jobs_per_s": len(jobs) / span, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "plain_jobs_per_s": len(jobs) / fin, "plain_p50_ms": 1e3 * float(np.percentile(plat, 50)),
+            "plain_p99_ms": 1e3 * float(np.percentile(plat, 99)), "failed": len(bad)}
+        log(f"[bench] rate {lam}/s: served {per_rate[str(lam)]['jobs_per_s']:.1f} jobs/s "
+            f"p50 {per_rate[str(lam)]['p50_ms']:.1f} ms p99 {per_rate[str(lam)]['p99_ms']:.1f} ms | "
+            f"plain {per_rate[str(lam)]['plain_jobs_per_s']:.1f} jobs/s "
+            f"p50 {per_rate[str(lam)]['plain_p50_ms']:.1f} ms p99 {per_rate[str(lam)]['plain_p99_ms']:.1f} ms")
+    srv.close()
+    top = per_rate[str(rates[-1])]
+    out = {
+        "metric": "served jobs/sec at saturation (Poisson arrivals), with p50/p99 latency",
+        "value": top["jobs_per_s"],
+        "unit": "jobs/s",
+        "n_gpus": 1,
+        "steps": len(rates),
+        "warmup": 0,
+        "ms_per_step": 1e3 / top["jobs_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (project generator, seeded)",
+        "config": {"workload": C5_WORKLOAD, "jobs_per_rate": len(jobs), "rates_per_s": rates,
+                   "theta": theta, "max_rounds": rounds, "epsilon": eps,
+                   "max_active": a.max_active, "budget_gib": 200},
+        "per_rate": per_rate,
+        "speedup_vs_plain_at_saturation": top["jobs_per_s"] / top["plain_jobs_per_s"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    ctx.close()
+    return out
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if one exists."""
@@ -192,7 +299,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default=os.environ.get("DMOLP_BENCH_CONFIG", "c3"),
-                    choices=sorted(CONFIGS) + ["c4"])
+                    choices=sorted(CONFIGS) + ["c4", "c5"])
+    ap.add_argument("--jobs", type=int, default=96, help="served mode: jobs per arrival rate")
+    ap.add_argument("--max-active", type=int, default=32, help="served mode: admitted jobs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -202,6 +311,9 @@ def main(argv=None):
         return 2
     if a.config == "c4":
         print(json.dumps(run_batched(a)))
+        return 0
+    if a.config == "c5":
+        print(json.dumps(run_served(a)))
         return 0
     from dmolp_amd import api
     cfg = CONFIGS[a.config]

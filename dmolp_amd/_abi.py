@@ -63,6 +63,16 @@ class Result(C.Structure):
                 ("lp_edges", i64), ("vertex_balance", f64), ("edge_balance", f64)]
 
 
+class Request(C.Structure):
+    _fields_ = [("graph", C.c_void_p), ("spec", GenSpec), ("k", i32), ("theta", i32),
+                ("max_rounds", i32), ("want_labels", i32), ("epsilon", f64), ("arrival_s", f64)]
+
+
+class Served(C.Structure):
+    _fields_ = [("result", Result), ("n", i64), ("nnz", i64), ("arrival_s", f64),
+                ("admit_s", f64), ("done_s", f64)]
+
+
 # Every function include/dmolp_hip.h declares: name -> (restype, argtypes).
 VP = C.c_void_p
 SIGNATURES = {
@@ -99,6 +109,13 @@ SIGNATURES = {
                                         P(i32), P(i64)]),
     "dmolp_lp_round": (C.c_int, [VP, VP, P(i32), P(i32), i64, P(f64), i32, P(i32)]),
     "dmolp_batch_run": (C.c_int, [VP, i32, P(Job), P(Result), P(f64)]),
+    "dmolp_batch_autotune": (C.c_int, [VP, i32, P(Job), i32, P(Tuning)]),
+    "dmolp_server_create": (C.c_int, [VP, i64, i32, P(VP)]),
+    "dmolp_server_destroy": (C.c_int, [VP]),
+    "dmolp_server_clock": (f64, [VP]),
+    "dmolp_server_submit": (C.c_int, [VP, P(Request), P(i64)]),
+    "dmolp_server_step": (C.c_int, [VP, P(i32), P(i32)]),
+    "dmolp_server_take": (C.c_int, [VP, i64, P(Served), P(i32)]),
 }
 
 _lib = None

@@ -215,6 +215,87 @@ def batch_run(ctx: "Context", jobs, want_labels=False):
     return out, ms.value
 
 
+def _job_array(jobs):
+    arr = (_abi.Job * max(len(jobs), 1))()
+    for i, j in enumerate(jobs):
+        arr[i] = _abi.Job(graph=j.graph.handle, k=j.k, theta=j.theta, max_rounds=j.max_rounds,
+                          reserved=0, epsilon=j.epsilon, labels_out=None)
+    return arr
+
+
+def batch_autotune(ctx: "Context", jobs, max_trials=0) -> _abi.Tuning:
+    """Pick launch parameters for this batch (kept in ctx); returns them."""
+    t = _abi.Tuning()
+    _check(_abi.lib().dmolp_batch_autotune(ctx.handle, len(jobs), _job_array(jobs), max_trials,
+                                           C.byref(t)), ctx.handle)
+    return t
+
+
+class Server:
+    """Served mode: continuous batching of partition jobs on ctx's device.
+    step() must be called from the thread that owns ctx."""
+
+    def __init__(self, ctx: "Context", budget_bytes: int, max_active: int):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        _check(_abi.lib().dmolp_server_create(ctx.handle, int(budget_bytes), int(max_active),
+                                              C.byref(self._h)), ctx.handle)
+        self._n = {}
+
+    def close(self):
+        if self._h:
+            _abi.lib().dmolp_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self) -> float:
+        return _abi.lib().dmolp_server_clock(self._h)
+
+    def submit(self, k, theta, max_rounds, epsilon, spec=None, graph=None, want_labels=False,
+               arrival_s=-1.0) -> int:
+        req = _abi.Request(graph=graph.handle if graph is not None else None,
+                           spec=spec if spec is not None else _abi.GenSpec(), k=k, theta=theta,
+                           max_rounds=max_rounds, want_labels=1 if want_labels else 0,
+                           epsilon=epsilon, arrival_s=arrival_s)
+        t = C.c_int64()
+        _check(_abi.lib().dmolp_server_submit(self._h, C.byref(req), C.byref(t)))
+        if want_labels:
+            self._n[t.value] = graph.n if graph is not None else _spec_n(spec)
+        return t.value
+
+    def step(self):
+        a, q = C.c_int32(), C.c_int32()
+        _check(_abi.lib().dmolp_server_step(self._h, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def take(self, ticket):
+        """(Served, labels or None) when the job has finished, else None."""
+        out = _abi.Served()
+        n = self._n.get(ticket)
+        lab = np.zeros(n, np.int32) if n is not None else None
+        rc = _abi.lib().dmolp_server_take(self._h, ticket, C.byref(out),
+                                          _p(lab, C.c_int32) if lab is not None else None)
+        if rc < 0:
+            _check(rc)
+        if rc == 0:
+            return None
+        self._n.pop(ticket, None)
+        return out, lab
+
+
+def _spec_n(spec) -> int:
+    if spec.kind == GEN_RMAT:
+        return 1 << spec.scale
+    if spec.kind == GEN_GRID:
+        return spec.rows * spec.cols
+    return spec.n
+
+
 class Graph:
     def __init__(self, ctx: Context, h):
         self.ctx = ctx

@@ -27,6 +27,7 @@ SOURCES = [
     "gen_dev.hip",
     "api.hip",
     "batch.hip",
+    "server.hip",
     "gen_host.cpp",
     "textio.cpp",
     "driver/driver.cpp",

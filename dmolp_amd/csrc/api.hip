@@ -651,4 +651,40 @@ int dmolp_batch_run(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, dmolp
     return guarded(ctx, [&] { batch_run(ctx, n_jobs, jobs, results, device_ms); });
 }
 
+int dmolp_batch_autotune(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, int32_t max_trials,
+                         dmolp_tuning *best) {
+    if (!ctx || n_jobs <= 0 || !jobs) return fail(ctx, DMOLP_E_ARG, "null argument");
+    return guarded(ctx, [&] { batch_autotune(ctx, n_jobs, jobs, max_trials, best); });
+}
+
+int dmolp_server_create(dmolp_ctx *ctx, int64_t budget, int32_t max_active, dmolp_server **out) {
+    if (!ctx || !out) return fail(ctx, DMOLP_E_ARG, "null argument");
+    *out = nullptr;
+    return guarded(ctx, [&] { *out = server_create(ctx, budget, max_active); });
+}
+
+int dmolp_server_destroy(dmolp_server *srv) {
+    if (srv) server_destroy(srv);
+    return DMOLP_OK;
+}
+
+double dmolp_server_clock(const dmolp_server *srv) { return srv ? server_clock(srv) : 0.0; }
+
+int dmolp_server_submit(dmolp_server *srv, const dmolp_request *req, int64_t *ticket) {
+    if (!srv || !req || !ticket) return fail(nullptr, DMOLP_E_ARG, "null argument");
+    return guarded(nullptr, [&] { *ticket = server_submit(srv, *req); });
+}
+
+int dmolp_server_step(dmolp_server *srv, int32_t *active, int32_t *queued) {
+    if (!srv) return fail(nullptr, DMOLP_E_ARG, "null argument");
+    return guarded(nullptr, [&] { server_step(srv, active, queued); });
+}
+
+int dmolp_server_take(dmolp_server *srv, int64_t ticket, dmolp_served *out, int32_t *labels_out) {
+    if (!srv) return fail(nullptr, DMOLP_E_ARG, "null argument");
+    int ready = 0;
+    const int rc = guarded(nullptr, [&] { ready = server_take(srv, ticket, out, labels_out); });
+    return rc != DMOLP_OK ? rc : ready;
+}
+
 }  // extern "C"

@@ -50,37 +50,41 @@ void batch_check_job(const dmolp_job &j) {
     DMOLP_REQUIRE(j.max_rounds >= 0, DMOLP_E_ARG, "max_rounds must be >= 0");
 }
 
-void phase2_run_set(dmolp_ctx *ctx, std::vector<P2Item> &items, BatchState &bs) {
+P2Set p2_set_build(dmolp_ctx *ctx, std::vector<P2Item *> &items, BatchState &bs, uint64_t *key_out) {
     const int G = (int)items.size();
     hipStream_t s = ctx->stream;
     std::vector<P2Job> jobs((size_t)G);
     std::vector<SelArgs> sels((size_t)G);
     std::vector<std::array<int32_t, P_NPOL>> blocks((size_t)G);
-    std::vector<Phase2Work *> W((size_t)G);
-    int Rmax = 0, passes = 0;
+    int passes = 0;
     bool any_heavy = false;
     bs.remaining.grow(sizeof(int32_t) * 2);
     for (int j = 0; j < G; ++j) {
-        P2Item &it = items[(size_t)j];
+        P2Item &it = *items[(size_t)j];
         Phase2Work &w = phase2_work(ctx, it.g, it.k);
-        W[(size_t)j] = &w;
-        phase2_round_table(ctx, w, it.max_rounds);
-        jobs[(size_t)j] = phase2_job(ctx, it.g, w, it.max_rounds, it.epsilon, bs.remaining.as<int32_t>());
-        jobs[(size_t)j].trace = nullptr;
-        jobs[(size_t)j].trace_cap = 0;
+        if (!it.have_job) {
+            phase2_round_table(ctx, w, it.max_rounds);
+            it.job = phase2_job(ctx, it.g, w, it.max_rounds, it.epsilon, nullptr);
+            it.job.trace = nullptr;
+            it.job.trace_cap = 0;
+            it.have_job = true;
+        }
+        it.job.remaining = bs.remaining.as<int32_t>();
+        jobs[(size_t)j] = it.job;
         sels[(size_t)j] = phase2_sel_args(ctx, w);
         job_blocks(ctx, it.g->n, it.g->nnz, it.k, it.g->n_heavy, G == 1, blocks[(size_t)j].data());
-        Rmax = std::max(Rmax, it.max_rounds);
         passes = std::max(passes, w.sel.passes);
         any_heavy = any_heavy || it.g->n_heavy > 0;
     }
     launch_set_build(ctx, bs.ls, blocks);
     bs.jobs.grow(sizeof(P2Job) * (size_t)G);
     bs.sel.grow(sizeof(SelArgs) * (size_t)G);
+    bs.done.grow(sizeof(int32_t) * (size_t)G);
     DMOLP_HIP_CHECK(hipMemcpyAsync(bs.jobs.p, jobs.data(), sizeof(P2Job) * (size_t)G,
                                    hipMemcpyHostToDevice, s));
     DMOLP_HIP_CHECK(hipMemcpyAsync(bs.sel.p, sels.data(), sizeof(SelArgs) * (size_t)G,
                                    hipMemcpyHostToDevice, s));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors go out of scope
     P2Set set;
     set.jobs = bs.jobs.as<P2Job>();
     set.sel = bs.sel.as<SelArgs>();
@@ -88,43 +92,39 @@ void phase2_run_set(dmolp_ctx *ctx, std::vector<P2Item> &items, BatchState &bs) 
     set.passes = passes;
     set.any_heavy = any_heavy;
     set.remaining = bs.remaining.as<int32_t>();
-    uint64_t key = 0x51ed270b27c1f1f3ull;
-    for (const void *p : {bs.jobs.p, bs.sel.p, bs.ls.bstart.p, bs.remaining.p})
-        key = mix(key, (uint64_t)(uintptr_t)p);
-    key = mix(key, (uint64_t)G);
-    key = mix(key, (uint64_t)passes);
-    key = mix(key, any_heavy ? 1 : 0);
-    for (int p = 0; p < P_NPOL; ++p) key = mix(key, (uint64_t)bs.ls.total[p]);
-
-    phase2_init_enqueue(ctx, set);
-    int32_t left = G;
-    const int check = 8;
-    for (int t = 1; t <= Rmax; ++t) {
-        phase2_round_launch(ctx, set, &bs.exec, &bs.exec_key, key);
-        if (t % check == 0 && t < Rmax) {
-            DMOLP_HIP_CHECK(hipMemcpyAsync(&left, set.remaining, sizeof(int32_t),
-                                           hipMemcpyDeviceToHost, s));
-            DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-            if (left <= 0) break;
-        }
+    if (key_out) {
+        uint64_t key = 0x51ed270b27c1f1f3ull;
+        for (const void *p : {bs.jobs.p, bs.sel.p, bs.ls.bstart.p, bs.remaining.p})
+            key = mix(key, (uint64_t)(uintptr_t)p);
+        key = mix(key, (uint64_t)G);
+        key = mix(key, (uint64_t)passes);
+        key = mix(key, any_heavy ? 1 : 0);
+        for (int p = 0; p < P_NPOL; ++p) key = mix(key, (uint64_t)bs.ls.total[p]);
+        *key_out = key;
     }
-    // Results.
+    return set;
+}
+
+void p2_collect(dmolp_ctx *ctx, std::vector<P2Item *> &items) {
+    hipStream_t s = ctx->stream;
+    const int G = (int)items.size();
     std::vector<Ctl> cs((size_t)G);
     std::vector<std::vector<int64_t>> cv((size_t)G), ci((size_t)G);
     for (int j = 0; j < G; ++j) {
-        const int k = items[(size_t)j].k;
+        P2Item &it = *items[(size_t)j];
+        Phase2Work &w = *it.g->p2;
+        const int k = it.k;
         cv[(size_t)j].assign((size_t)k, 0);
         ci[(size_t)j].assign((size_t)k, 0);
-        DMOLP_HIP_CHECK(hipMemcpyAsync(&cs[(size_t)j], W[(size_t)j]->ctl.p, sizeof(Ctl),
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&cs[(size_t)j], w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(cv[(size_t)j].data(), w.vc.p, sizeof(int64_t) * (size_t)k,
                                        hipMemcpyDeviceToHost, s));
-        DMOLP_HIP_CHECK(hipMemcpyAsync(cv[(size_t)j].data(), W[(size_t)j]->vc.p,
-                                       sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
-        DMOLP_HIP_CHECK(hipMemcpyAsync(ci[(size_t)j].data(), W[(size_t)j]->ic.p,
-                                       sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(ci[(size_t)j].data(), w.ic.p, sizeof(int64_t) * (size_t)k,
+                                       hipMemcpyDeviceToHost, s));
     }
     DMOLP_HIP_CHECK(hipStreamSynchronize(s));
     for (int j = 0; j < G; ++j) {
-        P2Item &it = items[(size_t)j];
+        P2Item &it = *items[(size_t)j];
         const Ctl &c = cs[(size_t)j];
         it.err = c.err;
         dmolp_phase2_metrics &m = it.m;
@@ -140,6 +140,33 @@ void phase2_run_set(dmolp_ctx *ctx, std::vector<P2Item> &items, BatchState &bs) 
         m.lp_edges = (int64_t)c.v[C_LP_EDGES];
         m.initial_cut = (int64_t)c.v[C_INIT_CUT];
     }
+}
+
+void phase2_run_set(dmolp_ctx *ctx, std::vector<P2Item> &items, BatchState &bs) {
+    const int G = (int)items.size();
+    hipStream_t s = ctx->stream;
+    std::vector<P2Item *> ip((size_t)G);
+    int Rmax = 0;
+    for (int j = 0; j < G; ++j) {
+        ip[(size_t)j] = &items[(size_t)j];
+        items[(size_t)j].have_job = false;
+        Rmax = std::max(Rmax, items[(size_t)j].max_rounds);
+    }
+    uint64_t key = 0;
+    const P2Set set = p2_set_build(ctx, ip, bs, &key);
+    phase2_init_enqueue(ctx, set);
+    int32_t left = G;
+    const int check = 8;
+    for (int t = 1; t <= Rmax; ++t) {
+        phase2_round_launch(ctx, set, &bs.exec, &bs.exec_key, key);
+        if (t % check == 0 && t < Rmax) {
+            DMOLP_HIP_CHECK(hipMemcpyAsync(&left, set.remaining, sizeof(int32_t),
+                                           hipMemcpyDeviceToHost, s));
+            DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+            if (left <= 0) break;
+        }
+    }
+    p2_collect(ctx, ip);
 }
 
 void batch_run(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, dmolp_result *results,
@@ -200,6 +227,66 @@ void batch_run(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, dmolp_resu
         r.edge_balance = b.m.edge_balance;
     }
     if (device_ms) *device_ms = ms;
+}
+
+void batch_autotune(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, int32_t max_trials,
+                    dmolp_tuning *best_out) {
+    DMOLP_REQUIRE(n_jobs > 0, DMOLP_E_ARG, "autotune needs at least one job");
+    // Candidates: degree-bin threshold x chunk length x blocks per unit of
+    // work.  Every candidate must give the same results; the fastest
+    // (device time of the whole batch) is kept in the context.
+    const int heavy[] = {256, 1024, 4096};
+    const int chunk[] = {2048, 4096};
+    const int64_t per[] = {8192, 32768, 131072};
+    const Tuning orig = ctx->tune;
+    std::vector<dmolp_result> ref((size_t)n_jobs), cur((size_t)n_jobs);
+    std::vector<dmolp_job> js(jobs, jobs + n_jobs);
+    for (auto &j : js) j.labels_out = nullptr;
+    double best_ms = -1.0;
+    Tuning best = orig;
+    int trials = 0;
+    bool have_ref = false;
+    try {
+        for (int h : heavy)
+            for (int c : chunk)
+                for (int64_t p : per) {
+                    if (max_trials > 0 && trials >= max_trials) break;
+                    ctx->tune.heavy_degree = h;
+                    ctx->tune.chunk_edges = c;
+                    ctx->tune.batch_work_per_block = p;
+                    double ms = 0.0;
+                    batch_run(ctx, n_jobs, js.data(), cur.data(), &ms);  // warm (allocations, capture)
+                    batch_run(ctx, n_jobs, js.data(), cur.data(), &ms);
+                    ++trials;
+                    if (!have_ref) {
+                        ref = cur;
+                        have_ref = true;
+                    } else {
+                        for (int32_t i = 0; i < n_jobs; ++i)
+                            DMOLP_REQUIRE(cur[(size_t)i].edge_cut == ref[(size_t)i].edge_cut &&
+                                              cur[(size_t)i].rounds == ref[(size_t)i].rounds &&
+                                              cur[(size_t)i].lp_edges == ref[(size_t)i].lp_edges,
+                                          DMOLP_E_HIP, "a tuning candidate changed a result");
+                    }
+                    if (best_ms < 0 || ms < best_ms) {
+                        best_ms = ms;
+                        best = ctx->tune;
+                    }
+                }
+    } catch (...) {
+        ctx->tune = orig;
+        throw;
+    }
+    ctx->tune = best;
+    if (best_out) {
+        std::memset(best_out, 0, sizeof(*best_out));
+        best_out->heavy_degree = best.heavy_degree;
+        best_out->chunk_edges = best.chunk_edges;
+        best_out->block_threads = best.block_threads;
+        best_out->grid_blocks = ctx->grid_blocks();
+        best_out->pull_per_mille = (int32_t)(best.pull_fraction * 1000.0 + 0.5);
+        best_out->batch_work_per_block = (int32_t)best.batch_work_per_block;
+    }
 }
 
 }  // namespace dmolp

@@ -130,9 +130,19 @@ struct LaunchSet {
     const int32_t *bs(int pol) const { return bstart.as<int32_t>() + (size_t)pol * (jobs + 1); }
 };
 
+// Device records of a phase-1 job set.
+struct P1SetDev {
+    int G = 0;
+    int passes = 0;
+    bool any_heavy = false;
+    int64_t max_n = 0;
+    LaunchSet ls;
+    DevMem jobs, sel, done, remaining;
+};
+
 // Device records and the captured round of the last phase-2 job set.
 struct BatchState {
-    DevMem jobs, sel, remaining;
+    DevMem jobs, sel, remaining, done;
     LaunchSet ls;
     hipGraphExec_t exec = nullptr;
     uint64_t exec_key = 0;
@@ -163,9 +173,7 @@ struct dmolp_ctx {
     std::vector<std::vector<double>> prof_ms;                         // [group] -> ms per round
     hipEvent_t prof_event();
     int grid_blocks() const { return tune.grid_blocks > 0 ? tune.grid_blocks : cu_count * 8; }
-    // Device records of the last phase-1 job set.
-    dmolp::DevMem p1_remaining, p1_jobs, p1_sel;
-    dmolp::LaunchSet p1_ls;
+    dmolp::P1SetDev p1_set;   // phase-1 job set of the plain and batched paths
     dmolp::BatchState batch;  // phase-2 job set of dmolp_batch_run
 };
 

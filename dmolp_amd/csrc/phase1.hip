@@ -503,15 +503,15 @@ Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
 
 namespace {
 
-// Per-job host state of a phase-1 set run.
-struct P1Host {
-    Phase1Work *w = nullptr;
-    std::vector<int32_t> deg, hubs, seeds;
-    std::vector<uint8_t> used;
-    std::pair<int32_t, int32_t> fs{-1, -1};
-    double host_ms = 0.0;
-    int32_t found = 0, fallback = 0;
-};
+__global__ void k_p1_set_remaining(int32_t *remaining, int32_t v) { *remaining = v; }
+
+// done flag of every job of a set, gathered for one host copy.
+__global__ void k_p1_gather_done(const P1Job *JS, int jobs, int32_t *out) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < jobs; j += gridDim.x * blockDim.x)
+        out[j] = JS[j].ctl->done;
+}
+
+}  // namespace
 
 P1Job p1_job(dmolp_ctx *ctx, dmolp_graph *g, Phase1Work &w, int32_t *remaining) {
     P1Job j{};
@@ -541,32 +541,277 @@ P1Job p1_job(dmolp_ctx *ctx, dmolp_graph *g, Phase1Work &w, int32_t *remaining) 
     return j;
 }
 
-__global__ void k_p1_set_remaining(int32_t *remaining, int32_t v) { *remaining = v; }
+void p1_hub_search(P1Stage &h) {
+    const auto t0 = std::chrono::steady_clock::now();
+    h.fs = first_seed_host(h.deg);
+    h.hubs = find_hubs_host(h.deg);
+    const auto t1 = std::chrono::steady_clock::now();
+    h.host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    // Candidates of the seed fallback: the first 2k vertices by (degree
+    // descending, id ascending).  At most k of them can already be seeds,
+    // so the first k - found unchosen vertices of the full order are here.
+    const int64_t n = (int64_t)h.deg.size();
+    const size_t m = (size_t)std::min<int64_t>(n, 2 * (int64_t)h.k);
+    const std::vector<int32_t> &deg = h.deg;
+    auto better = [&](int32_t a, int32_t b) {
+        if (deg[(size_t)a] != deg[(size_t)b]) return deg[(size_t)a] > deg[(size_t)b];
+        return a < b;
+    };
+    // Bounded heap with the worst kept candidate on top: one pass over n.
+    std::vector<int32_t> heap;
+    heap.reserve(m);
+    for (int64_t v = 0; v < n; ++v) {
+        if (heap.size() < m) {
+            heap.push_back((int32_t)v);
+            std::push_heap(heap.begin(), heap.end(), better);
+        } else if (m > 0 && better((int32_t)v, heap.front())) {
+            std::pop_heap(heap.begin(), heap.end(), better);
+            heap.back() = (int32_t)v;
+            std::push_heap(heap.begin(), heap.end(), better);
+        }
+    }
+    std::sort_heap(heap.begin(), heap.end(), better);
+    h.top = std::move(heap);
+}
+
+namespace {
 
 // Hubs and first seed of every job on the host; jobs in parallel threads.
-void host_hub_search(std::vector<P1Host> &H) {
-    auto one = [&](size_t j) {
-        const auto t0 = std::chrono::steady_clock::now();
-        H[j].fs = first_seed_host(H[j].deg);
-        H[j].hubs = find_hubs_host(H[j].deg);
-        const auto t1 = std::chrono::steady_clock::now();
-        H[j].host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    };
+void host_hub_search(std::vector<P1Stage> &H) {
     const size_t G = H.size();
     size_t T = std::thread::hardware_concurrency();
     if (T > 16) T = 16;
     if (T > G) T = G;
     if (T <= 1) {
-        for (size_t j = 0; j < G; ++j) one(j);
+        for (size_t j = 0; j < G; ++j) p1_hub_search(H[j]);
         return;
     }
     std::vector<std::thread> pool;
     for (size_t t = 0; t < T; ++t)
-        pool.emplace_back([&, t]() { for (size_t j = t; j < G; j += T) one(j); });
+        pool.emplace_back([&, t]() { for (size_t j = t; j < G; j += T) p1_hub_search(H[j]); });
     for (auto &th : pool) th.join();
 }
 
 }  // namespace
+
+void p1_stage_init(dmolp_ctx *ctx, P1Item &it, P1Stage &h) {
+    dmolp_graph *g = it.g;
+    DMOLP_REQUIRE(it.k >= 1 && it.k <= DMOLP_MAX_K, DMOLP_E_ARG, "k outside [1, 1024]");
+    DMOLP_REQUIRE(it.theta > 0, DMOLP_E_ARG, "theta must be positive (theta <= 0 never terminates)");
+    DMOLP_REQUIRE((int64_t)it.k <= g->n, DMOLP_E_SEEDS, "k larger than the vertex count");
+    h.w = &phase1_work(ctx, g, it.k);
+    if (!g->labels.p) g->labels.alloc(sizeof(int32_t) * (size_t)g->n);
+    h.g = g;
+    h.k = it.k;
+    h.theta = it.theta;
+}
+
+void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
+    const int G = (int)H.size();
+    hipStream_t s = ctx->stream;
+    sd.G = G;
+    sd.any_heavy = false;
+    sd.passes = 0;
+    sd.max_n = 0;
+    sd.remaining.grow(sizeof(int32_t) * 2);
+    std::vector<P1Job> jobs((size_t)G);
+    std::vector<SelArgs> sel((size_t)2 * G);
+    std::vector<std::array<int32_t, P_NPOL>> blocks((size_t)G);
+    for (int j = 0; j < G; ++j) {
+        P1Stage &h = *H[(size_t)j];
+        Phase1Work &w = *h.w;
+        if (!h.have_job) {
+            h.job = p1_job(ctx, h.g, w, nullptr);
+            h.have_job = true;
+        }
+        h.job.remaining = sd.remaining.as<int32_t>();
+        jobs[(size_t)j] = h.job;
+        P1Ctl *ctl = w.ctl.as<P1Ctl>();
+        TakeRule rule{};
+        rule.kind = 1;
+        rule.theta = h.theta;
+        sel[(size_t)j] = select_args(ctx, w.sel, w.F[0].dev, rule, &ctl->done, w.sel_list.dev, &w.F[1].dev);
+        sel[(size_t)G + j] = select_args(ctx, w.sel, w.F[1].dev, rule, &ctl->done, w.sel_list.dev, &w.F[0].dev);
+        job_blocks(ctx, h.g->n, h.g->nnz, h.k, h.g->n_heavy, G == 1, blocks[(size_t)j].data());
+        sd.any_heavy = sd.any_heavy || h.g->n_heavy > 0;
+        sd.passes = std::max(sd.passes, w.sel.passes);
+        sd.max_n = std::max(sd.max_n, h.g->n);
+    }
+    launch_set_build(ctx, sd.ls, blocks);
+    sd.jobs.grow(sizeof(P1Job) * (size_t)G);
+    sd.sel.grow(sizeof(SelArgs) * 2 * (size_t)G);
+    sd.done.grow(sizeof(int32_t) * (size_t)G);
+    DMOLP_HIP_CHECK(hipMemcpyAsync(sd.jobs.p, jobs.data(), sizeof(P1Job) * (size_t)G,
+                                   hipMemcpyHostToDevice, s));
+    DMOLP_HIP_CHECK(hipMemcpyAsync(sd.sel.p, sel.data(), sizeof(SelArgs) * 2 * (size_t)G,
+                                   hipMemcpyHostToDevice, s));
+    // The host vectors are released below: finish the copies first.
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void p1_degrees(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
+    hipStream_t s = ctx->stream;
+    const LaunchSet &ls = sd.ls;
+    hipLaunchKernelGGL(k_degrees, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, sd.jobs.as<P1Job>(),
+                       ls.bs(P_GRID), sd.G);
+    for (P1Stage *h : H) {
+        h->deg.resize((size_t)h->g->n);
+        DMOLP_HIP_CHECK(hipMemcpyAsync(h->deg.data(), h->w->deg.p, sizeof(int32_t) * (size_t)h->g->n,
+                                       hipMemcpyDeviceToHost, s));
+    }
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
+    hipStream_t s = ctx->stream;
+    const int G = (int)H.size();
+    int steps = 1;
+    for (P1Stage *hp : H) {
+        P1Stage &h = *hp;
+        Phase1Work &w = *h.w;
+        const int k = h.k;
+        const int64_t nh = (int64_t)h.hubs.size();
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(P1Ctl), s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.covered.p, 0, w.covered.bytes, s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.best.p, 0, w.best.bytes, s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.seeds.p, 0xff, w.seeds.bytes, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(w.seeds.p, &h.fs.first, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        h.used.assign((size_t)(nh > 0 ? nh : 1), 0);
+        for (int64_t i = 0; i < nh; ++i)
+            if (h.hubs[(size_t)i] == h.fs.first) { h.used[(size_t)i] = 1; break; }
+        w.hubs.grow(sizeof(int32_t) * (size_t)(nh > 0 ? nh : 1));
+        w.used.grow((size_t)(nh > 0 ? nh : 1));
+        if (nh > 0) {
+            DMOLP_HIP_CHECK(hipMemcpyAsync(w.hubs.p, h.hubs.data(), sizeof(int32_t) * (size_t)nh,
+                                           hipMemcpyHostToDevice, s));
+            DMOLP_HIP_CHECK(hipMemcpyAsync(w.used.p, h.used.data(), (size_t)nh, hipMemcpyHostToDevice, s));
+            DMOLP_HIP_CHECK(hipMemsetAsync(w.lvl1.cnt.p, 0, w.lvl1.cnt.bytes, s));
+        }
+        h.job.hubs = w.hubs.as<int32_t>();
+        h.job.used = w.used.as<uint8_t>();
+        h.job.nh = nh;
+        h.job.seed_steps = nh > 0 ? k : 1;
+        steps = std::max(steps, h.job.seed_steps);
+    }
+    p1_set_build(ctx, H, sd);  // uploads the hub fields
+    const LaunchSet &ls = sd.ls;
+    P1Job *d_jobs = sd.jobs.as<P1Job>();
+    for (int step = 1; step < steps; ++step) {
+        // Ball of seed step-1, then the next seed.
+        hipLaunchKernelGGL(k_ball1, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
+        hipLaunchKernelGGL(k_ball2_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
+        if (sd.any_heavy)
+            hipLaunchKernelGGL(k_ball2_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
+        hipLaunchKernelGGL(k_next_seed, dim3(ls.total[P_HIST1]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_HIST1), G, step);
+        hipLaunchKernelGGL(k_take_seed, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G, step);
+    }
+    DMOLP_HIP_CHECK(hipGetLastError());
+    std::vector<P1Ctl> hc((size_t)G);
+    for (int j = 0; j < G; ++j) {
+        P1Stage &h = *H[(size_t)j];
+        h.seeds.assign((size_t)h.k, -1);
+        DMOLP_HIP_CHECK(hipMemcpyAsync(h.seeds.data(), h.w->seeds.p, sizeof(int32_t) * h.seeds.size(),
+                                       hipMemcpyDeviceToHost, s));
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&hc[(size_t)j], h.w->ctl.p, sizeof(P1Ctl), hipMemcpyDeviceToHost, s));
+    }
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    for (int j = 0; j < G; ++j) {
+        P1Stage &h = *H[(size_t)j];
+        const int k = h.k;
+        const int64_t n = h.g->n;
+        const int64_t nh = (int64_t)h.hubs.size();
+        int32_t found = (nh > 0) ? (hc[(size_t)j].seed_stop ? hc[(size_t)j].seeds_found : k) : 1;
+        if (k == 1) found = 1;
+        h.found = found;
+        h.fallback = 0;
+        if (found < k) {
+            // Documented extension for seed.cpp:214-217 followed by the
+            // out-of-range seeds[p] of partition.cpp:347: the missing seeds are
+            // the remaining vertices by (degree descending, id ascending).
+            (void)n;
+            std::vector<int32_t> chosen(h.seeds.begin(), h.seeds.begin() + found);
+            std::sort(chosen.begin(), chosen.end());
+            int32_t at = found;
+            for (int32_t v : h.top) {
+                if (at == k) break;
+                if (!std::binary_search(chosen.begin(), chosen.end(), v)) h.seeds[(size_t)at++] = v;
+            }
+            DMOLP_REQUIRE(at == k, DMOLP_E_HIP, "seed fallback ran out of candidates");
+            h.fallback = k - found;
+            DMOLP_HIP_CHECK(hipMemcpyAsync(h.w->seeds.p, h.seeds.data(), sizeof(int32_t) * (size_t)k,
+                                           hipMemcpyHostToDevice, s));
+        }
+    }
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void p1_expand_begin(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
+    hipStream_t s = ctx->stream;
+    for (P1Stage *h : H) {
+        DMOLP_HIP_CHECK(hipMemsetAsync(h->g->labels.p, 0xff, sizeof(int32_t) * (size_t)h->g->n, s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(h->w->sizes.p, 0, h->w->sizes.bytes, s));
+        DMOLP_HIP_CHECK(hipMemsetAsync(h->w->ctl.p, 0, sizeof(P1Ctl), s));
+    }
+    hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, sd.remaining.as<int32_t>(), (int32_t)sd.G);
+    hipLaunchKernelGGL(k_expand_init, dim3(sd.ls.total[P_ONE]), dim3(256), 0, s, sd.jobs.as<P1Job>(),
+                       sd.ls.bs(P_ONE), sd.G);
+    DMOLP_HIP_CHECK(hipGetLastError());
+}
+
+void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds) {
+    hipStream_t s = ctx->stream;
+    const LaunchSet &ls = sd.ls;
+    const int G = sd.G;
+    P1Job *d_jobs = sd.jobs.as<P1Job>();
+    const SelArgs *d_sel0 = sd.sel.as<SelArgs>();
+    const SelArgs *d_sel1 = d_sel0 + G;
+    for (int r = 0; r < rounds; ++r) {
+        const int par = r & 1;
+        const SelArgs *sa = par ? d_sel1 : d_sel0;
+        hipLaunchKernelGGL(k_p1_round_begin, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G, par);
+        hipLaunchKernelGGL(k_p1_fill, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, par);
+        select_enqueue(ctx, sa, ls, sd.passes);
+        hipLaunchKernelGGL(k_expand_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G);
+        if (sd.any_heavy)
+            hipLaunchKernelGGL(k_expand_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G);
+        hipLaunchKernelGGL(k_p1_apply, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, par);
+        hipLaunchKernelGGL(k_p1_round_end, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G);
+    }
+    DMOLP_HIP_CHECK(hipGetLastError());
+}
+
+int32_t p1_remaining(dmolp_ctx *ctx, P1SetDev &sd) {
+    int32_t left = 0;
+    DMOLP_HIP_CHECK(hipMemcpyAsync(&left, sd.remaining.p, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return left;
+}
+
+void p1_done_flags(dmolp_ctx *ctx, P1SetDev &sd, std::vector<int32_t> &out) {
+    out.assign((size_t)sd.G, 0);
+    hipLaunchKernelGGL(k_p1_gather_done, dim3((sd.G + 255) / 256), dim3(256), 0, ctx->stream,
+                       sd.jobs.as<P1Job>(), sd.G, sd.done.as<int32_t>());
+    DMOLP_HIP_CHECK(hipMemcpyAsync(out.data(), sd.done.p, sizeof(int32_t) * (size_t)sd.G,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double device_ms) {
+    P1Ctl hc{};
+    DMOLP_HIP_CHECK(hipMemcpyAsync(&hc, h.w->ctl.p, sizeof(P1Ctl), hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memset(&m, 0, sizeof(m));
+    m.first_seed = h.fs.first;
+    m.first_seed_degree = h.fs.second;
+    m.n_hubs = (int64_t)h.hubs.size();
+    m.n_seeds_found = h.found;
+    m.n_seeds_fallback = h.fallback;
+    m.rounds = hc.rounds;
+    m.fill_round = hc.fill_round;
+    m.device_ms = device_ms;
+    m.host_ms = h.host_ms;
+    h.g->labels_valid = true;
+}
 
 void phase1_run(dmolp_ctx *ctx, dmolp_graph *g, int k, int theta, std::vector<int32_t> &seeds_out,
                 dmolp_phase1_metrics *m) {
@@ -583,209 +828,34 @@ void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items) {
     const int G = (int)items.size();
     DMOLP_REQUIRE(G >= 1, DMOLP_E_ARG, "empty job set");
     hipStream_t s = ctx->stream;
-    std::vector<P1Host> H((size_t)G);
-    int64_t max_n = 0;
-    bool any_heavy = false;
-    int passes = 0, kmax = 1;
+    std::vector<P1Stage> H((size_t)G);
+    std::vector<P1Stage *> HP((size_t)G);
     for (int j = 0; j < G; ++j) {
-        P1Item &it = items[(size_t)j];
-        dmolp_graph *g = it.g;
-        DMOLP_REQUIRE(it.k >= 1 && it.k <= DMOLP_MAX_K, DMOLP_E_ARG, "k outside [1, 1024]");
-        DMOLP_REQUIRE(it.theta > 0, DMOLP_E_ARG, "theta must be positive (theta <= 0 never terminates)");
-        DMOLP_REQUIRE((int64_t)it.k <= g->n, DMOLP_E_SEEDS, "k larger than the vertex count");
-        H[(size_t)j].w = &phase1_work(ctx, g, it.k);
-        if (!g->labels.p) g->labels.alloc(sizeof(int32_t) * (size_t)g->n);
-        max_n = std::max(max_n, g->n);
-        any_heavy = any_heavy || g->n_heavy > 0;
-        passes = std::max(passes, H[(size_t)j].w->sel.passes);
-        kmax = std::max(kmax, it.k);
+        p1_stage_init(ctx, items[(size_t)j], H[(size_t)j]);
+        HP[(size_t)j] = &H[(size_t)j];
     }
-    // Set-level device state: job records, selection args, launch shape.
-    if (!ctx->p1_remaining.p) ctx->p1_remaining.alloc(sizeof(int32_t) * 2);
-    int32_t *remaining = ctx->p1_remaining.as<int32_t>();
-    std::vector<P1Job> jobs((size_t)G);
-    std::vector<SelArgs> sel0((size_t)G), sel1((size_t)G);
-    std::vector<std::array<int32_t, P_NPOL>> blocks((size_t)G);
-    for (int j = 0; j < G; ++j) {
-        P1Item &it = items[(size_t)j];
-        Phase1Work &w = *H[(size_t)j].w;
-        jobs[(size_t)j] = p1_job(ctx, it.g, w, remaining);
-        P1Ctl *ctl = w.ctl.as<P1Ctl>();
-        TakeRule rule{};
-        rule.kind = 1;
-        rule.theta = it.theta;
-        sel0[(size_t)j] = select_args(ctx, w.sel, w.F[0].dev, rule, &ctl->done, w.sel_list.dev, &w.F[1].dev);
-        sel1[(size_t)j] = select_args(ctx, w.sel, w.F[1].dev, rule, &ctl->done, w.sel_list.dev, &w.F[0].dev);
-        job_blocks(ctx, it.g->n, it.g->nnz, it.k, it.g->n_heavy, G == 1, blocks[(size_t)j].data());
-    }
-    LaunchSet &ls = ctx->p1_ls;
-    launch_set_build(ctx, ls, blocks);
-    ctx->p1_jobs.grow(sizeof(P1Job) * (size_t)G);
-    ctx->p1_sel.grow(sizeof(SelArgs) * 2 * (size_t)G);
-    P1Job *d_jobs = ctx->p1_jobs.as<P1Job>();
-    SelArgs *d_sel0 = ctx->p1_sel.as<SelArgs>();
-    SelArgs *d_sel1 = d_sel0 + G;
-    auto upload_jobs = [&]() {
-        DMOLP_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(P1Job) * (size_t)G,
-                                       hipMemcpyHostToDevice, s));
-    };
-    upload_jobs();
-    DMOLP_HIP_CHECK(hipMemcpyAsync(d_sel0, sel0.data(), sizeof(SelArgs) * (size_t)G, hipMemcpyHostToDevice, s));
-    DMOLP_HIP_CHECK(hipMemcpyAsync(d_sel1, sel1.data(), sizeof(SelArgs) * (size_t)G, hipMemcpyHostToDevice, s));
-
+    P1SetDev &sd = ctx->p1_set;
+    p1_set_build(ctx, HP, sd);
     DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, s));
-    // Degrees to the host for the hub sort.
-    hipLaunchKernelGGL(k_degrees, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G);
-    for (int j = 0; j < G; ++j) {
-        const dmolp_graph *g = items[(size_t)j].g;
-        H[(size_t)j].deg.resize((size_t)g->n);
-        DMOLP_HIP_CHECK(hipMemcpyAsync(H[(size_t)j].deg.data(), H[(size_t)j].w->deg.p,
-                                       sizeof(int32_t) * (size_t)g->n, hipMemcpyDeviceToHost, s));
-    }
-    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    p1_degrees(ctx, HP, sd);
     host_hub_search(H);
-
-    // Seeds (seed.cpp:192-242).
-    int steps = 1;
-    for (int j = 0; j < G; ++j) {
-        P1Host &h = H[(size_t)j];
-        Phase1Work &w = *h.w;
-        const int k = items[(size_t)j].k;
-        const int64_t nh = (int64_t)h.hubs.size();
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(P1Ctl), s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.covered.p, 0, w.covered.bytes, s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.best.p, 0, w.best.bytes, s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.seeds.p, 0xff, w.seeds.bytes, s));
-        DMOLP_HIP_CHECK(hipMemcpyAsync(w.seeds.p, &h.fs.first, sizeof(int32_t), hipMemcpyHostToDevice, s));
-        h.used.assign((size_t)(nh > 0 ? nh : 1), 0);
-        for (int64_t i = 0; i < nh; ++i)
-            if (h.hubs[(size_t)i] == h.fs.first) { h.used[(size_t)i] = 1; break; }
-        if (w.hubs.bytes < sizeof(int32_t) * (size_t)(nh > 0 ? nh : 1)) {
-            w.hubs.alloc(sizeof(int32_t) * (size_t)(nh > 0 ? nh : 1));
-            w.used.alloc((size_t)(nh > 0 ? nh : 1));
-        }
-        if (nh > 0) {
-            DMOLP_HIP_CHECK(hipMemcpyAsync(w.hubs.p, h.hubs.data(), sizeof(int32_t) * (size_t)nh,
-                                           hipMemcpyHostToDevice, s));
-            DMOLP_HIP_CHECK(hipMemcpyAsync(w.used.p, h.used.data(), (size_t)nh, hipMemcpyHostToDevice, s));
-            DMOLP_HIP_CHECK(hipMemsetAsync(w.lvl1.cnt.p, 0, w.lvl1.cnt.bytes, s));
-        }
-        P1Job &jb = jobs[(size_t)j];
-        jb.hubs = w.hubs.as<int32_t>();
-        jb.used = w.used.as<uint8_t>();
-        jb.nh = nh;
-        jb.seed_steps = nh > 0 ? k : 1;
-        steps = std::max(steps, jb.seed_steps);
-    }
-    upload_jobs();
-    for (int step = 1; step < steps; ++step) {
-        // Ball of seed step-1, then the next seed.
-        hipLaunchKernelGGL(k_ball1, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
-        hipLaunchKernelGGL(k_ball2_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
-        if (any_heavy)
-            hipLaunchKernelGGL(k_ball2_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
-        hipLaunchKernelGGL(k_next_seed, dim3(ls.total[P_HIST1]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_HIST1), G, step);
-        hipLaunchKernelGGL(k_take_seed, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G, step);
-    }
-    DMOLP_HIP_CHECK(hipGetLastError());
-    std::vector<P1Ctl> hc((size_t)G);
-    for (int j = 0; j < G; ++j) {
-        P1Host &h = H[(size_t)j];
-        h.seeds.assign((size_t)items[(size_t)j].k, -1);
-        DMOLP_HIP_CHECK(hipMemcpyAsync(h.seeds.data(), h.w->seeds.p, sizeof(int32_t) * h.seeds.size(),
-                                       hipMemcpyDeviceToHost, s));
-        DMOLP_HIP_CHECK(hipMemcpyAsync(&hc[(size_t)j], h.w->ctl.p, sizeof(P1Ctl), hipMemcpyDeviceToHost, s));
-    }
-    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-    for (int j = 0; j < G; ++j) {
-        P1Host &h = H[(size_t)j];
-        const int k = items[(size_t)j].k;
-        const int64_t n = items[(size_t)j].g->n;
-        const int64_t nh = (int64_t)h.hubs.size();
-        int32_t found = (nh > 0) ? (hc[(size_t)j].seed_stop ? hc[(size_t)j].seeds_found : k) : 1;
-        if (k == 1) found = 1;
-        h.found = found;
-        h.fallback = 0;
-        if (found < k) {
-            // Documented extension for seed.cpp:214-217 followed by the
-            // out-of-range seeds[p] of partition.cpp:347: the missing seeds are
-            // the remaining vertices by (degree descending, id ascending).
-            std::vector<uint8_t> chosen((size_t)n, 0);
-            for (int i = 0; i < found; ++i) chosen[(size_t)h.seeds[(size_t)i]] = 1;
-            std::vector<int32_t> order;
-            order.reserve((size_t)n);
-            for (int64_t v = 0; v < n; ++v)
-                if (!chosen[(size_t)v]) order.push_back((int32_t)v);
-            const size_t need = (size_t)(k - found);
-            const std::vector<int32_t> &deg = h.deg;
-            auto cmp = [&](int32_t a, int32_t b) {
-                if (deg[(size_t)a] != deg[(size_t)b]) return deg[(size_t)a] > deg[(size_t)b];
-                return a < b;
-            };
-            std::partial_sort(order.begin(), order.begin() + (ptrdiff_t)need, order.end(), cmp);
-            for (size_t i = 0; i < need; ++i) h.seeds[(size_t)found + i] = order[i];
-            h.fallback = k - found;
-            DMOLP_HIP_CHECK(hipMemcpyAsync(h.w->seeds.p, h.seeds.data(), sizeof(int32_t) * (size_t)k,
-                                           hipMemcpyHostToDevice, s));
-        }
-    }
-
-    // Expansion (partition.cpp:323-604).
-    for (int j = 0; j < G; ++j) {
-        dmolp_graph *g = items[(size_t)j].g;
-        Phase1Work &w = *H[(size_t)j].w;
-        DMOLP_HIP_CHECK(hipMemsetAsync(g->labels.p, 0xff, sizeof(int32_t) * (size_t)g->n, s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.sizes.p, 0, w.sizes.bytes, s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(P1Ctl), s));
-    }
-    hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, remaining, (int32_t)G);
-    hipLaunchKernelGGL(k_expand_init, dim3(ls.total[P_ONE]), dim3(256), 0, s, d_jobs, ls.bs(P_ONE), G);
+    p1_seeds(ctx, HP, sd);
+    p1_expand_begin(ctx, HP, sd);
     int64_t enq = 0;
     const int batch = 16;
-    int32_t left = G;
     for (;;) {
-        for (int b = 0; b < batch; ++b, ++enq) {
-            const int par = (int)(enq & 1);
-            const SelArgs *sa = par ? d_sel1 : d_sel0;
-            hipLaunchKernelGGL(k_p1_round_begin, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G, par);
-            hipLaunchKernelGGL(k_p1_fill, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, par);
-            select_enqueue(ctx, sa, ls, passes);
-            hipLaunchKernelGGL(k_expand_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G);
-            if (any_heavy)
-                hipLaunchKernelGGL(k_expand_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G);
-            hipLaunchKernelGGL(k_p1_apply, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, par);
-            hipLaunchKernelGGL(k_p1_round_end, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G);
-        }
-        DMOLP_HIP_CHECK(hipGetLastError());
-        DMOLP_HIP_CHECK(hipMemcpyAsync(&left, remaining, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-        if (left <= 0) break;
-        DMOLP_REQUIRE(enq < 64ll * (max_n + 16), DMOLP_E_HIP, "phase-1 expansion did not finish");
+        p1_expand_rounds(ctx, sd, batch);
+        enq += batch;
+        if (p1_remaining(ctx, sd) <= 0) break;
+        DMOLP_REQUIRE(enq < 64ll * (sd.max_n + 16), DMOLP_E_HIP, "phase-1 expansion did not finish");
     }
     DMOLP_HIP_CHECK(hipEventRecord(ctx->ev1, s));
-    for (int j = 0; j < G; ++j)
-        DMOLP_HIP_CHECK(hipMemcpyAsync(&hc[(size_t)j], H[(size_t)j].w->ctl.p, sizeof(P1Ctl),
-                                       hipMemcpyDeviceToHost, s));
     DMOLP_HIP_CHECK(hipEventSynchronize(ctx->ev1));
-    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;
     DMOLP_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     for (int j = 0; j < G; ++j) {
-        P1Item &it = items[(size_t)j];
-        P1Host &h = H[(size_t)j];
-        it.seeds = h.seeds;
-        dmolp_phase1_metrics &m = it.m;
-        std::memset(&m, 0, sizeof(m));
-        m.first_seed = h.fs.first;
-        m.first_seed_degree = h.fs.second;
-        m.n_hubs = (int64_t)h.hubs.size();
-        m.n_seeds_found = h.found;
-        m.n_seeds_fallback = h.fallback;
-        m.rounds = hc[(size_t)j].rounds;
-        m.fill_round = hc[(size_t)j].fill_round;
-        m.device_ms = ms;
-        m.host_ms = h.host_ms;
-        it.g->labels_valid = true;
+        p1_finish(ctx, H[(size_t)j], items[(size_t)j].m, ms);
+        items[(size_t)j].seeds = H[(size_t)j].seeds;
     }
 }
 

@@ -59,6 +59,34 @@ struct P1Item {
     dmolp_phase1_metrics m{};
 };
 
+// Host state of one job while it is in phase 1.
+struct P1Stage {
+    dmolp_graph *g = nullptr;
+    Phase1Work *w = nullptr;
+    int k = 0, theta = 0;
+    std::vector<int32_t> deg, hubs, seeds;
+    std::vector<int32_t> top;  // first 2k vertices by (degree desc, id asc)
+    std::vector<uint8_t> used;
+    std::pair<int32_t, int32_t> fs{-1, -1};
+    double host_ms = 0.0;
+    int32_t found = 0, fallback = 0;
+    P1Job job{};
+    bool have_job = false;
+};
+
+// Stages of phase 1 for a set of jobs (phase1_run_set runs them in order;
+// the serving loop interleaves them with other jobs' work).
+void p1_stage_init(dmolp_ctx *ctx, P1Item &it, P1Stage &h);
+void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);
+void p1_degrees(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);  // degrees to the host
+void p1_hub_search(P1Stage &h);                                           // host: hubs, first seed
+void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);    // hub-ball seeds + fallback
+void p1_expand_begin(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);
+void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds);          // rounds: even
+int32_t p1_remaining(dmolp_ctx *ctx, P1SetDev &sd);
+void p1_done_flags(dmolp_ctx *ctx, P1SetDev &sd, std::vector<int32_t> &out);
+void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double device_ms);
+
 Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
 std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg);
 std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg);

@@ -899,6 +899,12 @@ __global__ void k_init_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
 
 __global__ void k_set_remaining(int32_t *remaining, int32_t v) { *remaining = v; }
 
+// done flag of every job of a set, gathered for one host copy.
+__global__ void k_p2_gather_done(const P2Job *JS, int jobs, int32_t *out) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < jobs; j += gridDim.x * blockDim.x)
+        out[j] = JS[j].ctl->done;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -1031,6 +1037,16 @@ P2Set phase2_single_set(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_r
     set.any_heavy = g->n_heavy > 0;
     set.remaining = w.remaining.as<int32_t>();
     return set;
+}
+
+void phase2_done_flags(dmolp_ctx *ctx, const P2Set &set, int32_t *d_out, std::vector<int32_t> &out) {
+    const int G = set.ls->jobs;
+    out.assign((size_t)G, 0);
+    hipLaunchKernelGGL(k_p2_gather_done, dim3((G + 255) / 256), dim3(256), 0, ctx->stream, set.jobs, G,
+                       d_out);
+    DMOLP_HIP_CHECK(hipMemcpyAsync(out.data(), d_out, sizeof(int32_t) * (size_t)G,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 void phase2_init_enqueue(dmolp_ctx *ctx, const P2Set &set) {

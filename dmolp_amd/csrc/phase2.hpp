@@ -128,6 +128,8 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set);
 // cached in *exec under `key`).
 void phase2_round_launch(dmolp_ctx *ctx, const P2Set &set, hipGraphExec_t *exec, uint64_t *exec_key,
                          uint64_t key);
+// Every job's done flag (d_out: device scratch of one int per job).
+void phase2_done_flags(dmolp_ctx *ctx, const P2Set &set, int32_t *d_out, std::vector<int32_t> &out);
 // Shape key of the single-graph set for phase2_round_launch.
 uint64_t phase2_single_key(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
 

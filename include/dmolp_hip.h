@@ -291,6 +291,56 @@ typedef struct {
 int dmolp_batch_run(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, dmolp_result *results,
                     double *device_ms);
 
+/* Autotuner (no counterpart in the reference): runs the batch under
+ * candidate launch parameters (degree-bin threshold, heavy-row chunk
+ * length, blocks per unit of work; at most max_trials, 0 = all), checks that
+ * every candidate gives the same results, and keeps the fastest in ctx.
+ * best (nullable) receives it. */
+int dmolp_batch_autotune(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, int32_t max_trials,
+                         dmolp_tuning *best);
+
+/* Served mode (no counterpart in the reference, whose driver partitions one
+ * file per process, main.cpp:123-193): a request loop with continuous
+ * batching.  Requests wait until a slot and their device-memory footprint
+ * are free; every dmolp_server_step advances all admitted jobs together,
+ * whatever stage each is in, and finished jobs free their slot and memory at
+ * once.  A job's result is the same as its plain run, alone or in any mix. */
+typedef struct dmolp_server dmolp_server;
+
+typedef struct {
+    dmolp_graph *graph;    /* resident graph (kept by the caller until the result is
+                              taken), or NULL: the graph is generated from spec on
+                              admission and freed when the job finishes */
+    dmolp_gen_spec spec;
+    int32_t k;
+    int32_t theta;
+    int32_t max_rounds;
+    int32_t want_labels;   /* keep the final labels for dmolp_server_take */
+    double epsilon;
+    double arrival_s;      /* arrival time on the server clock; < 0: now */
+} dmolp_request;
+
+typedef struct {
+    dmolp_result result;
+    int64_t n, nnz;
+    double arrival_s, admit_s, done_s; /* server clock, seconds */
+} dmolp_served;
+
+/* budget: device bytes the admitted jobs may hold together; max_active:
+ * jobs admitted at once.  The server drives ctx's stream: call
+ * dmolp_server_step from ctx's thread; submit and take may come from any
+ * thread. */
+int dmolp_server_create(dmolp_ctx *ctx, int64_t budget, int32_t max_active, dmolp_server **out);
+int dmolp_server_destroy(dmolp_server *srv);
+/* Seconds since the server was created. */
+double dmolp_server_clock(const dmolp_server *srv);
+int dmolp_server_submit(dmolp_server *srv, const dmolp_request *req, int64_t *ticket);
+/* One scheduling step; active/queued (nullable) receive the job counts after it. */
+int dmolp_server_step(dmolp_server *srv, int32_t *active, int32_t *queued);
+/* 1 when the job has finished (result copied out and released), 0 while it
+ * is pending; labels_out (n, nullable) receives the labels if requested. */
+int dmolp_server_take(dmolp_server *srv, int64_t ticket, dmolp_served *out, int32_t *labels_out);
+
 #ifdef __cplusplus
 }
 #endif

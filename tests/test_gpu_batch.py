@@ -116,3 +116,24 @@ def test_batch_rejects_bad_jobs(ctx):
             api.batch_run(ctx, [api.BatchJob(g, 4, 10, 10, 0.03), api.BatchJob(g, 4, 10, 10, 0.03)])
     finally:
         g.free()
+
+
+def test_autotune_keeps_results(ctx):
+    specs = mixed_specs(seed=9, count=8)
+    base, _ = run_batch(ctx, specs, 1000, 10, 0.03)
+    old = ctx.tuning()
+    graphs = [ctx.generate(sp) for sp, _ in specs]
+    try:
+        jobs = [api.BatchJob(g, k, 1000, 10, 0.03) for g, (_, k) in zip(graphs, specs)]
+        t = api.batch_autotune(ctx, jobs, max_trials=4)
+        assert t.heavy_degree in (256, 1024, 4096)
+        cur = ctx.tuning()
+        assert cur.heavy_degree == t.heavy_degree and cur.batch_work_per_block == t.batch_work_per_block
+        res, _ = api.batch_run(ctx, jobs, want_labels=True)
+        for x, y in zip(base, res):
+            assert np.array_equal(x.labels, y.labels) and x.edge_cut == y.edge_cut
+    finally:
+        for g in graphs:
+            g.free()
+        ctx.set_tuning(heavy_degree=old.heavy_degree, chunk_edges=old.chunk_edges,
+                       batch_work_per_block=old.batch_work_per_block)
