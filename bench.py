@@ -353,16 +353,21 @@ def main(argv=None):
     g.checkpoint_labels(restore=True)
     pr = g.phase2(k, R, eps, want_labels=False, want_trace=True)
     ctx.set_profiling(False)
-    names = ["lp_light", "lp_heavy", "candidates", "select", "delta_apply_end", "round"]
+    names = ["lp_light", "lp_heavy", "candidates", "select", "delta_apply", "round", "hist_update"]
     per_round = {nm: ctx.round_times(i) for i, nm in enumerate(names)}
     groups = {nm: float(v.sum()) for nm, v in per_round.items()}
     tr = pr.trace
     light_n, light_deg = tr["b_light_count"], tr["b_light_deg"]
     heavy_n, heavy_deg = tr["b_count"] - light_n, tr["b_deg"] - light_deg
     unit = 2 * O_BYTES + 2 * L_BYTES
+    # The heavy rows' LP histograms are kept across rounds: after each round
+    # k_hist_update streams every heavy row's columns (4 B per entry) and
+    # tests one bit per entry in the n-bit changed-vertex bitmap.
+    h_rows, h_entries = g.bins()
+    hist_bytes = np.full(len(per_round["hist_update"]), C_BYTES * h_entries + (g.n + 7) // 8, np.float64)
     kern = {
         "k_lp_light": (per_round["lp_light"], unit * light_n + (C_BYTES + L_BYTES) * light_deg),
-        "k_lp_heavy": (per_round["lp_heavy"], unit * heavy_n + (C_BYTES + L_BYTES) * heavy_deg),
+        "k_hist_update": (per_round["hist_update"], hist_bytes),
     }
     dom = max(kern, key=lambda kk: float(kern[kk][0].sum()))
     ms_r, bytes_r = kern[dom]

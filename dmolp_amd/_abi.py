@@ -109,6 +109,7 @@ SIGNATURES = {
                                         P(i32), P(i64)]),
     "dmolp_lp_round": (C.c_int, [VP, VP, P(i32), P(i32), i64, P(f64), i32, P(i32)]),
     "dmolp_batch_run": (C.c_int, [VP, i32, P(Job), P(Result), P(f64)]),
+    "dmolp_graph_bins": (C.c_int, [VP, VP, P(i64), P(i64)]),
     "dmolp_batch_autotune": (C.c_int, [VP, i32, P(Job), i32, P(Tuning)]),
     "dmolp_server_create": (C.c_int, [VP, i64, i32, P(VP)]),
     "dmolp_server_destroy": (C.c_int, [VP]),

@@ -337,6 +337,13 @@ class Graph:
         _check(_abi.lib().dmolp_graph_labels_checkpoint(self.ctx.handle, self._h,
                                                         1 if restore else 0), self.ctx.handle)
 
+    def bins(self):
+        """(heavy rows, CSR entries they hold) under the context's tuning."""
+        h, e = C.c_int64(), C.c_int64()
+        _check(_abi.lib().dmolp_graph_bins(self.ctx.handle, self._h, C.byref(h), C.byref(e)),
+               self.ctx.handle)
+        return h.value, e.value
+
     def get_labels(self) -> np.ndarray:
         out = np.zeros(self.n, np.int32)
         _check(_abi.lib().dmolp_graph_get_labels(self.ctx.handle, self._h, _p(out, C.c_int32)),

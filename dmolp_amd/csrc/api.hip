@@ -224,7 +224,7 @@ int dmolp_ctx_set_profiling(dmolp_ctx *ctx, int on) {
 }
 
 int dmolp_ctx_round_times(dmolp_ctx *ctx, int group, double *ms_out, int cap, int *n_out) {
-    if (!ctx || group < 0 || group > 5 || cap < 0 || (cap > 0 && !ms_out))
+    if (!ctx || group < 0 || group > DMOLP_PROF_HIST || cap < 0 || (cap > 0 && !ms_out))
         return fail(ctx, DMOLP_E_ARG, "bad arguments");
     const std::vector<double> empty;
     const std::vector<double> &v =
@@ -338,6 +338,18 @@ int dmolp_graph_info(const dmolp_graph *g, int64_t *n, int64_t *nnz) {
     if (n) *n = g->n;
     if (nnz) *nnz = g->nnz;
     return DMOLP_OK;
+}
+
+int dmolp_graph_bins(dmolp_ctx *ctx, dmolp_graph *g, int64_t *heavy_rows, int64_t *heavy_entries) {
+    if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    return guarded(ctx, [&] {
+        graph_prepare_bins(ctx, g);
+        unsigned long long c[LC_N] = {};
+        DMOLP_HIP_CHECK(hipMemcpyAsync(c, g->all.cnt.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+        DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (heavy_rows) *heavy_rows = (int64_t)c[LC_SLOTS];
+        if (heavy_entries) *heavy_entries = (int64_t)c[LC_HDEG];
+    });
 }
 
 int dmolp_graph_download(dmolp_ctx *ctx, const dmolp_graph *g, int64_t *off, int32_t *col) {

@@ -77,6 +77,18 @@ __device__ __forceinline__ int64_t wave_count() {
     return ((int64_t)gridDim.x * blockDim.x) / kWave;
 }
 
+// Job records live in device memory, so the pointers in them are loaded at
+// run time and the compiler cannot tell which address space they point to:
+// it would emit flat memory instructions, which serialise against LDS
+// traffic.  gload reads a stored pointer through a global-address-space
+// view, so every access derived from it compiles to global instructions.
+template <class T>
+using as1_ptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ T *gload(T *const &field) {
+    return (T *)(*reinterpret_cast<const as1_ptr<T> *>(&field));
+}
+
 // One launch serves a set of jobs (graphs): job j owns the blocks
 // [bstart[j], bstart[j+1]).  A kernel body sees its job's share of the
 // launch as a virtual grid, so the same body runs one graph or many.

@@ -83,6 +83,13 @@ __global__ void k_bin_all(GraphDev g, ListDev out, int heavy_degree, int chunk_e
     }
 }
 
+__global__ void k_heavy_ids(ListDev all, int32_t *hid) {
+    const int64_t ns = (int64_t)all.cnt[LC_SLOTS];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns;
+         i += (int64_t)gridDim.x * blockDim.x)
+        hid[all.hv[i]] = (int32_t)i;
+}
+
 __global__ void k_count_heavy(GraphDev g, int heavy_degree, int chunk_edges,
                               unsigned long long *acc) {
     unsigned long long nh = 0, nc = 0;
@@ -135,6 +142,10 @@ void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g) {
     DMOLP_HIP_CHECK(hipMemsetAsync(g->all.cnt.p, 0, g->all.cnt.bytes, ctx->stream));
     hipLaunchKernelGGL(k_bin_all, dim3(ctx->grid_blocks()), dim3(256), 0, ctx->stream, g->dev(),
                        g->all.dev, H, CH);
+    g->hid.grow(sizeof(int32_t) * (size_t)(g->n > 0 ? g->n : 1));
+    DMOLP_HIP_CHECK(hipMemsetAsync(g->hid.p, 0xff, g->hid.bytes, ctx->stream));
+    hipLaunchKernelGGL(k_heavy_ids, dim3(ctx->cu_count), dim3(256), 0, ctx->stream, g->all.dev,
+                       g->hid.as<int32_t>());
     DMOLP_HIP_CHECK(hipGetLastError());
     g->all_heavy_degree = H;
     g->all_chunk_edges = CH;

@@ -191,6 +191,7 @@ struct dmolp_graph {
     dmolp::List all;
     int all_heavy_degree = -1, all_chunk_edges = -1;
     int64_t n_heavy = 0, n_heavy_chunks = 0;
+    dmolp::DevMem hid;  // int32[n]: heavy slot of the all list, -1 for light rows
     std::unique_ptr<dmolp::Phase2Work> p2;
     std::unique_ptr<dmolp::Phase1Work> p1;
     dmolp::GraphDev dev() const {

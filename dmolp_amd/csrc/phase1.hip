@@ -37,10 +37,34 @@ __device__ __forceinline__ bool p1_done(const P1Ctl *c) { return c->done != 0; }
 
 // Every kernel takes the job set and runs on its job's share of the launch
 // (see VGrid): one graph, or many graphs in one launch.
+using ::dmolp::gload;
+__device__ __forceinline__ P1Job gload(const P1Job &v) {
+    P1Job r = v;
+    r.g = gload(v.g);
+    r.lvl1 = gload(v.lvl1);
+    r.sel_list = gload(v.sel_list);
+    r.F[0] = gload(v.F[0]);
+    r.F[1] = gload(v.F[1]);
+    r.L = gload(v.L);
+    r.claims = gload(v.claims);
+    r.touched = gload(v.touched);
+    r.touched_cnt = gload(v.touched_cnt);
+    r.covered = gload(v.covered);
+    r.deg = gload(v.deg);
+    r.best = gload(v.best);
+    r.sizes = gload(v.sizes);
+    r.seeds = gload(v.seeds);
+    r.hubs = gload(v.hubs);
+    r.used = gload(v.used);
+    r.ctl = gload(v.ctl);
+    r.remaining = gload(v.remaining);
+    return r;
+}
+
 #define P1_JOB()                                   \
     const int job_ = job_of_block(bstart, jobs);   \
     const VGrid vg = vgrid_of(bstart, job_);       \
-    const P1Job &J = JS[job_];                     \
+    const P1Job J = gload(JS[job_]);               \
     (void)vg
 
 
@@ -214,8 +238,8 @@ __global__ void k_expand_init(const P1Job *JS, const int32_t *bstart, int jobs) 
 __global__ void k_p1_round_begin(const P1Job *JS, const int32_t *bstart, int jobs, int parity) {
     P1_JOB();
     P1Ctl *ctl = J.ctl;
-    const RecDev fcur = J.F[parity];
-    const RecDev fnext = J.F[parity ^ 1];
+    const RecDev fcur = parity ? J.F[1] : J.F[0];
+    const RecDev fnext = parity ? J.F[0] : J.F[1];
     const ListDev sel = J.sel_list;
     const unsigned long long *sizes = J.sizes;
     const int k = J.k;
@@ -246,7 +270,7 @@ __global__ void k_p1_fill(const P1Job *JS, const int32_t *bstart, int jobs, int 
     P1_JOB();
     const GraphDev g = J.g;
     int32_t *L = J.L;
-    const RecDev fcur = J.F[parity];
+    const RecDev fcur = parity ? J.F[1] : J.F[0];
     unsigned long long *sizes = J.sizes;
     P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
@@ -351,7 +375,7 @@ __global__ __launch_bounds__(kThreads) void k_p1_apply(const P1Job *JS, const in
     int32_t *claims = J.claims;
     const int32_t *touched = J.touched;
     const int64_t *touched_cnt = J.touched_cnt;
-    const RecDev fnext = J.F[parity ^ 1];
+    const RecDev fnext = parity ? J.F[0] : J.F[1];
     unsigned long long *sizes = J.sizes;
     const int k = J.k;
     P1Ctl *ctl = J.ctl;
@@ -508,7 +532,7 @@ __global__ void k_p1_set_remaining(int32_t *remaining, int32_t v) { *remaining =
 // done flag of every job of a set, gathered for one host copy.
 __global__ void k_p1_gather_done(const P1Job *JS, int jobs, int32_t *out) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < jobs; j += gridDim.x * blockDim.x)
-        out[j] = JS[j].ctl->done;
+        out[j] = gload(JS[j].ctl)->done;
 }
 
 }  // namespace

@@ -4,11 +4,12 @@
 //
 // Per round t the host enqueues, without waiting:
 //   round_begin   penalties (phase2.cpp:158-209) from the device counts
-//   candidates    B_{t-1} u N(B_{t-1}), first touch by stamp; a candidate is
-//                 a boundary vertex iff its differing-neighbour count > 0
+//   candidates    B_{t-1} u N(B_{t-1}); a candidate is a record iff it has a
+//                 neighbour with another label (round 1: the initial sweep's
+//                 differing-neighbour counts; later rounds: a row scan)
 //   select        per label top ceil(ratio*c) by (degree, id)  -> B_t
 //   lp            decision for every b in B_t (Jacobi)        -> Delta
-//   delta         incremental vc, ic, cut and differing-neighbour counts
+//   delta         incremental vc, ic and cut
 //   apply         commit Delta to the labels
 //   round_end     exit tests (phase2.cpp:309-312, 384-415), trace
 // Every kernel returns at once when the control block says "done", so a
@@ -32,15 +33,68 @@ constexpr int kLpUnroll = 4;     // light rows: gathers in flight per lane
 constexpr int kHeavyUnroll = 8;  // heavy rows: gathers in flight per lane
 constexpr int32_t kLabel8None = 255;  // byte-mirror value of labels the LP cannot see
 constexpr int kScanItems = 4;         // vertices per thread per block iteration in scans
+// Up to this k every valid label fits the byte mirror exactly (255 stands
+// for -1 and out-of-range labels), so the sweeps gather bytes, not words.
+constexpr int kExact8 = 254;
+// Recount the heavy-row histograms when the changed vertices cover at least
+// this share of the CSR entries; below it, update them from the changes.
+constexpr double kRecountShare = 0.2;
 
 __device__ __forceinline__ bool is_done(const Ctl *c) { return c->done != 0; }
+__device__ __forceinline__ bool bit_of(const uint32_t *bits, int32_t v) {
+    return (bits[v >> 5] >> (v & 31)) & 1u;
+}
+
+// Neighbour label for the sweeps: from the byte mirror when k <= kExact8
+// (255 maps back to -1), else the int32 label.
+__device__ __forceinline__ int32_t nb_label(bool exact8, const uint8_t *L8, const int32_t *L,
+                                            int32_t u) {
+    if (exact8) {
+        const int32_t b = L8[u];
+        return b == kLabel8None ? -1 : b;
+    }
+    return L[u];
+}
 
 // Every round kernel takes the job set and runs on its job's share of the
 // launch (see VGrid): one graph, or many graphs in one launch.
+using ::dmolp::gload;
+__device__ __forceinline__ P2Job gload(const P2Job &v) {
+    P2Job r = v;
+    r.g = gload(v.g);
+    r.all = gload(v.all);
+    r.B = gload(v.B);
+    r.D = gload(v.D);
+    r.L = gload(v.L);
+    r.L8 = gload(v.L8);
+    r.chg = gload(v.chg);
+    r.diff = gload(v.diff);
+    r.stamp = gload(v.stamp);
+    r.deferred = gload(v.deferred);
+    r.hub_hist = gload(v.hub_hist);
+    r.bstamp = gload(v.bstamp);
+    r.bbits = gload(v.bbits);
+    r.hh = gload(v.hh);
+    r.hid = gload(v.hid);
+    r.cbits = gload(v.cbits);
+    r.O8 = gload(v.O8);
+    r.N8 = gload(v.N8);
+    r.bd = gload(v.bd);
+    r.ctl = gload(v.ctl);
+    r.vc = gload(v.vc);
+    r.ic = gload(v.ic);
+    r.pen = gload(v.pen);
+    r.re = gload(v.re);
+    r.rtab = gload(v.rtab);
+    r.trace = gload(v.trace);
+    r.remaining = gload(v.remaining);
+    return r;
+}
+
 #define P2_JOB()                                   \
     const int job_ = job_of_block(bstart, jobs);   \
     const VGrid vg = vgrid_of(bstart, job_);       \
-    const P2Job &J = JS[job_];                     \
+    const P2Job J = gload(JS[job_]);               \
     (void)vg
 
 
@@ -53,7 +107,9 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
     const GraphDev g = J.g;
     const ListDev all = J.all;
     const int32_t *L = J.L;
+    const uint8_t *L8 = J.L8;
     const int k = J.k;
+    const bool exact8 = k <= kExact8;
     int32_t *diff = J.diff;
     Ctl *ctl = J.ctl;
     unsigned long long *vc = J.vc;
@@ -78,7 +134,7 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
                 const int slot = batch_slot(s_b[w], q);
                 const int64_t e = s_b[w].base[slot] + q;
                 const int32_t vv = s_b[w].v[slot];
-                const int32_t lv = L[vv], lu = L[g.col[e]];
+                const int32_t lv = L[vv], lu = nb_label(exact8, L8, L, g.col[e]);
                 if (lv == -1 || lu == -1) {
                     ++invalid;
                 } else if (lv != lu) {
@@ -118,7 +174,15 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
     Ctl *ctl = J.ctl;
     unsigned long long *vc = J.vc;
     unsigned long long *ic = J.ic;
-    const int lane = lane_id();
+    const uint8_t *L8 = J.L8;
+    int32_t *hh = J.hh;
+    const bool exact8 = k <= kExact8;
+    const int E = k < kLpCap ? k : kLpCap;
+    // Per-lane histogram rows of the row's neighbour labels (below E), the
+    // heavy row's starting histogram for the LP rounds.
+    __shared__ int32_t s_h[kWavesPerBlock][kWave * kHistStride];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    int32_t *row = &s_h[w][lane * kHistStride];
     int64_t invalid = 0, cut2 = 0;
     const int64_t nch = (int64_t)all.cnt[LC_CHUNKS];
     for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
@@ -128,14 +192,25 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
         chunk_bounds(g, all, item, chunk_edges, slot, v, e0, e1, deg);
         const int32_t lv = L[v];
         int64_t dcount = 0, intra = 0;
+        for (int l = 0; l < E; ++l) row[l] = 0;
         for (int64_t e = e0 + lane; e < e1; e += kWave) {
-            const int32_t lu = L[g.col[e]];
+            const int32_t u = g.col[e];
+            const int32_t b = L8[u];
+            if (b < E) row[b] += 1;
+            const int32_t lu = exact8 ? (b == kLabel8None ? -1 : b) : L[u];
             if (lv == -1 || lu == -1) ++invalid;
             else if (lv != lu) { ++cut2; ++dcount; }
             else if (lv >= 0 && lv < k) ++intra;
         }
         dcount = wave_sum(dcount);
         intra = wave_sum(intra);
+        wave_sync();
+        if (lane < E) {
+            int32_t c = 0;
+            for (int r = 0; r < kWave; ++r) c += s_h[w][r * kHistStride + lane];
+            if (c) atomicAdd(&hh[(int64_t)slot * kLpCap + lane], c);
+        }
+        wave_sync();
         if (lane == 0) {
             if (dcount) atomicAdd(&diff[v], (int32_t)dcount);
             if (intra) atomicAdd(&ic[lv], (unsigned long long)intra);
@@ -216,9 +291,9 @@ __global__ void k_penalties(const unsigned long long *vc, const unsigned long lo
 
 // Round start: penalties, counter resets, and the candidate direction.
 // Pushing from B_{t-1} reads every edge of B_{t-1} at random; pulling reads
-// the differing-neighbour count of every vertex in order and scans a
-// boundary vertex's row only until a B_{t-1} member is found.  Pull when
-// B_{t-1} covers at least pull_frac of the CSR entries.
+// every vertex in order and scans its row only until it has found both a
+// B_{t-1} member and a differing label.  Pull when B_{t-1} covers at least
+// pull_frac of the CSR entries.
 // B holds B_{t-1} here; its counts are kept for the trace before the
 // selection overwrites the list with B_t.
 __global__ void k_round_begin(const P2Job *JS, const int32_t *bstart, int jobs) {
@@ -287,60 +362,85 @@ __global__ __launch_bounds__(kThreads) void k_cand_all(const P2Job *JS, const in
 }
 
 // First touch of u in this pass; a touched boundary vertex becomes a record.
+// Boundary test (gpu_lp_boundary.cu:252-263): does v (label lv) have a
+// neighbour with another label among entries [s, e)?  Stops at the first.
+__device__ __forceinline__ bool differs_in(const GraphDev &g, int32_t lv, int64_t s, int64_t e,
+                                           bool exact8, const uint8_t *L8, const int32_t *L) {
+    for (int64_t x = s; x < e; ++x)
+        if (nb_label(exact8, L8, L, g.col[x]) != lv) return true;
+    return false;
+}
+
+// First touch of u in this pass; a touched boundary vertex becomes a record.
 __device__ __forceinline__ void touch(const GraphDev &g, bool pred, int32_t u, int32_t *stamp,
-                                      int32_t epoch, const int32_t *L, const int32_t *diff,
-                                      const RecDev &bd, int64_t &cand) {
+                                      int32_t epoch, const int32_t *L, const uint8_t *L8,
+                                      bool exact8, const RecDev &bd, int64_t &cand) {
     bool first = false;
     if (pred && stamp[u] != epoch) first = atomicExch(&stamp[u], epoch) != epoch;
     if (first) ++cand;
-    const bool rec = first && diff[u] > 0;
+    int32_t lu = 0;
+    int64_t s = 0, e = 0;
+    if (first) {
+        lu = L[u];
+        s = g.off[u];
+        e = g.off[u + 1];
+    }
+    const bool rec = first && differs_in(g, lu, s, e, exact8, L8, L);
     const int64_t pos = wave_append(rec, bd.cnt);
     if (rec) {
         bd.x[pos] = u;
-        bd.g[pos] = L[u];
-        bd.d[pos] = (int32_t)(g.off[u + 1] - g.off[u]);
+        bd.g[pos] = lu;
+        bd.d[pos] = (int32_t)(e - s);
     }
 }
 
-// Pull: a boundary vertex (differing-neighbour count > 0) is a candidate
-// iff it or one of its neighbours was selected in round t-1 (bstamp ==
-// prev_epoch).  Rows longer than heavy_degree are scanned here up to
-// heavy_degree entries and finished by k_cand_pull_heavy.
+// Pull: a vertex is a candidate iff it or one of its neighbours was selected
+// in round t-1 (bit set in the B_{t-1} bitmap), and a record iff it is on the
+// boundary.  One pass over the row answers both and stops when both are
+// known.  Rows longer than heavy_degree are scanned here up to heavy_degree
+// entries and finished by k_cand_pull_heavy.
 __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
     const int32_t *L = J.L;
-    const int32_t *diff = J.diff;
-    const int32_t *bstamp = J.bstamp;
+    const uint8_t *L8 = J.L8;
     const RecDev bd = J.bd;
-    int32_t *deferred = J.deferred;
+    int64_t *deferred = J.deferred;
     const int heavy_degree = J.heavy_degree;
+    const bool exact8 = J.k <= kExact8;
     Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
-    const int32_t prev_epoch = ctl->epoch - 1;
+    const uint32_t *inb = J.bbits + (int64_t)((ctl->t - 1) & 1) * J.bwords;  // B_{t-1}
     __shared__ BlockAppendLds s_app;
     const int64_t span = (int64_t)blockDim.x * kScanItems;
     for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
         uint32_t rec = 0;
-        int32_t nr = 0, deg[kScanItems];
+        int32_t nr = 0, deg[kScanItems], lab[kScanItems];
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             deg[j] = 0;
-            if (v >= g.n || diff[v] <= 0) continue;
+            lab[j] = 0;
+            if (v >= g.n) continue;
             const int64_t s = g.off[v], e = g.off[v + 1];
+            const int32_t lv = L[v];
             deg[j] = (int32_t)(e - s);
-            bool hit = bstamp[v] == prev_epoch;
-            if (!hit) {
-                const int64_t lim = e - s > heavy_degree ? s + heavy_degree : e;
-                for (int64_t x = s; x < lim; ++x)
-                    if (bstamp[g.col[x]] == prev_epoch) { hit = true; break; }
-                // A long row without a hit in its first heavy_degree entries
-                // is finished by k_cand_pull_heavy (rare).
-                if (!hit && lim < e)
-                    deferred[atomicAdd(&ctl->v[C_DEFERRED], 1ull)] = (int32_t)v;
+            lab[j] = lv;
+            bool hit = bit_of(inb, (int32_t)v), dif = false;
+            const int64_t lim = e - s > heavy_degree ? s + heavy_degree : e;
+            for (int64_t x = s; x < lim && !(hit && dif); ++x) {
+                const int32_t u = g.col[x];
+                if (!dif) dif = nb_label(exact8, L8, L, u) != lv;
+                if (!hit) hit = bit_of(inb, u);
             }
-            if (hit) { rec |= 1u << j; ++nr; }
+            // A long row still undecided after heavy_degree entries is
+            // finished by k_cand_pull_heavy (rare).
+            if (!(hit && dif) && lim < e) {
+                const unsigned long long i = atomicAdd(&ctl->v[C_DEFERRED], 1ull);
+                deferred[i] = ((int64_t)((dif ? 1 : 0) | (hit ? 2 : 0)) << 32) | (int64_t)v;
+                continue;
+            }
+            if (hit && dif) { rec |= 1u << j; ++nr; }
         }
         int64_t pos = block_reserve(nr, bd.cnt, s_app);
 #pragma unroll
@@ -348,7 +448,7 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
             if (!(rec & (1u << j))) continue;
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             bd.x[pos] = (int32_t)v;
-            bd.g[pos] = L[v];
+            bd.g[pos] = lab[j];
             bd.d[pos] = deg[j];
             ++pos;
         }
@@ -359,27 +459,34 @@ __global__ void k_cand_pull_heavy(const P2Job *JS, const int32_t *bstart, int jo
     P2_JOB();
     const GraphDev g = J.g;
     const int32_t *L = J.L;
-    const int32_t *bstamp = J.bstamp;
+    const uint8_t *L8 = J.L8;
     const RecDev bd = J.bd;
-    const int32_t *deferred = J.deferred;
+    const int64_t *deferred = J.deferred;
     const int heavy_degree = J.heavy_degree;
+    const bool exact8 = J.k <= kExact8;
     Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
-    const int32_t prev_epoch = ctl->epoch - 1;
+    const uint32_t *inb = J.bbits + (int64_t)((ctl->t - 1) & 1) * J.bwords;  // B_{t-1}
     const int64_t nd = (int64_t)ctl->v[C_DEFERRED];
     for (int64_t j = vwave(vg); j < nd; j += vwaves(vg)) {
-        const int32_t v = deferred[j];
+        const int64_t item = deferred[j];
+        const int32_t v = (int32_t)(item & 0xffffffffll);
+        bool dif = (item >> 32) & 1, hit = (item >> 33) & 1;
+        const int32_t lv = L[v];
         const int64_t s = g.off[v], e = g.off[v + 1];
-        bool found = false;
-        for (int64_t b = s + heavy_degree; b < e && !found; b += kWave) {
+        for (int64_t b = s + heavy_degree; b < e && !(dif && hit); b += kWave) {
             const int64_t x = b + lane_id();
-            found = __ballot(x < e && bstamp[g.col[x]] == prev_epoch) != 0;
+            const int32_t u = x < e ? g.col[x] : -1;
+            const bool d = u >= 0 && !dif && nb_label(exact8, L8, L, u) != lv;
+            const bool h = u >= 0 && !hit && bit_of(inb, u);
+            dif = dif || __ballot(d) != 0;
+            hit = hit || __ballot(h) != 0;
         }
-        const bool rec = found && lane_id() == 0;
+        const bool rec = dif && hit && lane_id() == 0;
         const int64_t pos = wave_append(rec, bd.cnt);
         if (rec) {
             bd.x[pos] = v;
-            bd.g[pos] = L[v];
+            bd.g[pos] = lv;
             bd.d[pos] = (int32_t)(e - s);
         }
     }
@@ -390,7 +497,8 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(const P2Job *JS, const 
     const GraphDev g = J.g;
     const ListDev prev = J.B;
     const int32_t *L = J.L;
-    const int32_t *diff = J.diff;
+    const uint8_t *L8 = J.L8;
+    const bool exact8 = J.k <= kExact8;
     int32_t *stamp = J.stamp;
     const RecDev bd = J.bd;
     Ctl *ctl = J.ctl;
@@ -404,13 +512,13 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(const P2Job *JS, const 
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, prev.light, count, b0, s_b[w], v, deg, valid);
-        touch(g, valid, v, stamp, epoch, L, diff, bd, cand);
+        touch(g, valid, v, stamp, epoch, L, L8, exact8, bd, cand);
         for (int32_t q0 = 0; q0 < total; q0 += kWave) {
             const int32_t q = q0 + lane;
             const bool on = q < total;
             int32_t u = 0;
             if (on) u = g.col[s_b[w].base[batch_slot(s_b[w], q)] + q];
-            touch(g, on, u, stamp, epoch, L, diff, bd, cand);
+            touch(g, on, u, stamp, epoch, L, L8, exact8, bd, cand);
         }
         wave_sync();
     }
@@ -422,7 +530,8 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
     const GraphDev g = J.g;
     const ListDev prev = J.B;
     const int32_t *L = J.L;
-    const int32_t *diff = J.diff;
+    const uint8_t *L8 = J.L8;
+    const bool exact8 = J.k <= kExact8;
     int32_t *stamp = J.stamp;
     const int chunk_edges = J.chunk_edges;
     const RecDev bd = J.bd;
@@ -437,11 +546,11 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
         int64_t e0, e1, deg;
         const int64_t item = prev.hc[j];
         chunk_bounds(g, prev, item, chunk_edges, slot, v, e0, e1, deg);
-        touch(g, lane == 0 && (item & 0xffffffffll) == 0, v, stamp, epoch, L, diff, bd, cand);
+        touch(g, lane == 0 && (item & 0xffffffffll) == 0, v, stamp, epoch, L, L8, exact8, bd, cand);
         for (int64_t eb = e0; eb < e1; eb += kWave) {
             const int64_t e = eb + lane;
             const bool on = e < e1;
-            touch(g, on, on ? g.col[e] : 0, stamp, epoch, L, diff, bd, cand);
+            touch(g, on, on ? g.col[e] : 0, stamp, epoch, L, L8, exact8, bd, cand);
         }
     }
     wave_add(&ctl->v[C_CAND], cand);
@@ -478,6 +587,7 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
     const double *pen = J.pen;
     const int k = J.k;
     int32_t *chg = J.chg;
+    uint8_t *N8 = J.N8;
     const ListDev D = J.D;
     const int heavy_degree = J.heavy_degree;
     const int chunk_edges = J.chunk_edges;
@@ -528,7 +638,10 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
             const int32_t own = L[v];
             nl = lp_decide(&h[lane * kHistStride], 1, E, s_pen, own);
             changed = nl != own;
-            if (changed) chg[v] = nl;
+            if (changed) {
+                chg[v] = nl;
+                N8[v] = (uint8_t)(nl < kLabel8None ? nl : kLabel8None);
+            }
         }
         block_list_append(D, changed, v, deg, heavy_degree, chunk_edges, s_app);
     }
@@ -593,6 +706,7 @@ __global__ void k_lp_heavy_decide(const P2Job *JS, const int32_t *bstart, int jo
     const double *pen = J.pen;
     const int k = J.k;
     int32_t *chg = J.chg;
+    uint8_t *N8 = J.N8;
     const ListDev D = J.D;
     const int heavy_degree = J.heavy_degree;
     const int chunk_edges = J.chunk_edges;
@@ -618,7 +732,10 @@ __global__ void k_lp_heavy_decide(const P2Job *JS, const int32_t *bstart, int jo
             const int32_t nl = lp_decide(h, 1, E, s_pen, own);
             for (int l = 0; l < E; ++l) h[l] = 0;
             changed = nl != own;
-            if (changed) chg[v] = nl;
+            if (changed) {
+                chg[v] = nl;
+                N8[v] = (uint8_t)(nl < kLabel8None ? nl : kLabel8None);
+            }
         }
         list_append(D, changed, v, deg, heavy_degree, chunk_edges);
     }
@@ -626,28 +743,6 @@ __global__ void k_lp_heavy_decide(const P2Job *JS, const int32_t *bstart, int jo
 
 // ---------------------------------------------------------------- delta stats
 
-// For a changed vertex v (old a, new c) and neighbour u (old lu, new lu2):
-// the pair is counted once, from the smaller id when both changed.
-struct PairDelta {
-    int32_t dd;     // change of "labels differ" (-1, 0, +1)
-};
-
-__device__ __forceinline__ int32_t pair_update(int32_t v, int32_t a, int32_t c, int32_t u,
-                                               const int32_t *L, const int32_t *chg, int k,
-                                               int32_t *diff, unsigned long long *s_ic,
-                                               int64_t &cut_delta, bool &counted) {
-    const int32_t cu = chg[u];
-    counted = !(cu >= 0 && u < v);
-    if (!counted) return 0;
-    const int32_t lu = L[u];
-    const int32_t lu2 = cu >= 0 ? cu : lu;
-    const int32_t od = a != lu ? 1 : 0, nd = c != lu2 ? 1 : 0;
-    const int32_t dd = nd - od;
-    if (dd) { atomicAdd(&diff[u], dd); cut_delta += dd; }
-    if (a == lu && a >= 0 && a < k) atomicAdd(&s_ic[a], ~0ull);          // -1
-    if (c == lu2 && c >= 0 && c < k) atomicAdd(&s_ic[c], 1ull);
-    return dd;
-}
 
 // The same pair update, returning its effect instead of applying the
 // intra-partition counts: bit 0 the old pair was intra (a == lu), bit 16 the
@@ -655,14 +750,22 @@ __device__ __forceinline__ int32_t pair_update(int32_t v, int32_t a, int32_t c, 
 // per vertex, since a and c are the vertex's own old and new labels.
 __device__ __forceinline__ int32_t pair_effect(int32_t v, int32_t a, int32_t c, int32_t u,
                                                const int32_t *L, const int32_t *chg,
-                                               int32_t *diff, int32_t &dd_out) {
+                                               const uint8_t *L8, const uint8_t *N8, bool exact8,
+                                               int32_t &dd_out) {
     dd_out = 0;
-    const int32_t cu = chg[u];
-    if (cu >= 0 && u < v) return 0;  // counted from u's side
-    const int32_t lu = L[u];
-    const int32_t lu2 = cu >= 0 ? cu : lu;
+    int32_t lu, lu2;
+    if (exact8) {  // byte gathers: old label and this round's new label (255: unchanged)
+        const int32_t nu = N8[u];
+        if (nu != kLabel8None && u < v) return 0;  // counted from u's side
+        lu = L8[u];
+        lu2 = nu != kLabel8None ? nu : lu;
+    } else {
+        const int32_t cu = chg[u];
+        if (cu >= 0 && u < v) return 0;  // counted from u's side
+        lu = L[u];
+        lu2 = cu >= 0 ? cu : lu;
+    }
     const int32_t dd = (c != lu2 ? 1 : 0) - (a != lu ? 1 : 0);
-    if (dd) atomicAdd(&diff[u], dd);
     dd_out = dd;
     return (a == lu ? 1 : 0) | (c == lu2 ? 1 << 16 : 0);
 }
@@ -673,14 +776,14 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
     const ListDev D = J.D;
     const int32_t *L = J.L;
     const int32_t *chg = J.chg;
+    const uint8_t *L8 = J.L8, *N8 = J.N8;
     const int k = J.k;
-    int32_t *diff = J.diff;
+    const bool exact8 = k <= kExact8;
     Ctl *ctl = J.ctl;
     unsigned long long *vc = J.vc;
     unsigned long long *ic = J.ic;
     if (is_done(ctl)) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
-    __shared__ int32_t s_dv[kWavesPerBlock][kWave];
     __shared__ int32_t s_io[kWavesPerBlock][kWave];
     __shared__ unsigned long long s_vc[kMaxKLds], s_ic[kMaxKLds];
     for (int i = threadIdx.x; i < k; i += blockDim.x) { s_vc[i] = 0; s_ic[i] = 0; }
@@ -692,7 +795,6 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, D.light, count, b0, s_b[w], v, deg, valid);
-        s_dv[w][lane] = 0;
         s_io[w][lane] = 0;
         s_b[w].aux[lane] = valid ? L[v] : 0;  // old label of the slot's vertex
         wave_sync();
@@ -710,14 +812,14 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
                 if (slot[j] < 0) continue;
                 const int32_t vv = s_b[w].v[slot[j]];
                 int32_t dd;
-                const int32_t io = pair_effect(vv, s_b[w].aux[slot[j]], chg[vv], u[j], L, chg, diff, dd);
-                if (dd) { atomicAdd(&s_dv[w][slot[j]], dd); cut_delta += dd; }
+                const int32_t io = pair_effect(vv, s_b[w].aux[slot[j]], chg[vv], u[j], L, chg, L8, N8,
+                                               exact8, dd);
+                cut_delta += dd;
                 if (io) atomicAdd(&s_io[w][slot[j]], io);
             }
         }
         wave_sync();
         if (valid) {
-            if (s_dv[w][lane]) atomicAdd(&diff[v], s_dv[w][lane]);
             const int32_t a = L[v], c = chg[v];
             const int32_t io = s_io[w][lane];
             if (a >= 0 && a < k) {
@@ -745,9 +847,10 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
     const ListDev D = J.D;
     const int32_t *L = J.L;
     const int32_t *chg = J.chg;
+    const uint8_t *L8 = J.L8, *N8 = J.N8;
     const int k = J.k;
+    const bool exact8 = k <= kExact8;
     const int chunk_edges = J.chunk_edges;
-    int32_t *diff = J.diff;
     Ctl *ctl = J.ctl;
     unsigned long long *vc = J.vc;
     unsigned long long *ic = J.ic;
@@ -767,7 +870,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
         int64_t dv = 0, oi = 0, ni = 0;
         for (int64_t e = e0 + lane; e < e1; e += kWave) {
             int32_t dd;
-            const int32_t io = pair_effect(v, a, c, g.col[e], L, chg, diff, dd);
+            const int32_t io = pair_effect(v, a, c, g.col[e], L, chg, L8, N8, exact8, dd);
             dv += dd;
             oi += io & 0xffff;
             ni += io >> 16;
@@ -777,7 +880,6 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
         ni = wave_sum(ni);
         cut_delta += lane == 0 ? dv : 0;
         if (lane == 0) {
-            if (dv) atomicAdd(&diff[v], (int32_t)dv);
             if (a >= 0 && a < k && oi) atomicAdd(&s_ic[a], (unsigned long long)(-oi));
             if (c >= 0 && c < k && ni) atomicAdd(&s_ic[c], (unsigned long long)ni);
             if ((item & 0xffffffffll) == 0) {
@@ -792,8 +894,8 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
         if (s_ic[i]) atomicAdd(&ic[i], s_ic[i]);
 }
 
-// Byte mirror of the labels for the LP gathers: label if below the LP cap
-// (gpu_lp_boundary.cu:360), else kLabel8None.
+// Byte mirror of the labels: the label if it fits a byte, else kLabel8None.
+// The LP ignores entries at or above its label cap (gpu_lp_boundary.cu:360).
 __global__ void k_mirror8(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const int64_t n = J.g.n;
@@ -802,7 +904,7 @@ __global__ void k_mirror8(const P2Job *JS, const int32_t *bstart, int jobs) {
     for (int64_t v = vthread(vg); v < n;
          v += vthreads(vg)) {
         const int32_t l = L[v];
-        L8[v] = (l >= 0 && l < kLpCap) ? (uint8_t)l : (uint8_t)kLabel8None;
+        L8[v] = (l >= 0 && l < kLabel8None) ? (uint8_t)l : (uint8_t)kLabel8None;
     }
 }
 
@@ -810,7 +912,8 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const ListDev D = J.D;
     int32_t *L = J.L;
-    uint8_t *L8 = J.L8;
+    uint8_t *L8 = J.L8, *O8 = J.O8, *N8 = J.N8;
+    uint32_t *cbits = J.cbits;
     int32_t *chg = J.chg;
     const Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
@@ -820,8 +923,158 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
         const int32_t v = i < nl ? D.light[i] : D.hv[i - nl];
         const int32_t nl_v = chg[v];
         L[v] = nl_v;
-        L8[v] = (nl_v >= 0 && nl_v < kLpCap) ? (uint8_t)nl_v : (uint8_t)kLabel8None;
+        O8[v] = L8[v];
+        L8[v] = (nl_v >= 0 && nl_v < kLabel8None) ? (uint8_t)nl_v : (uint8_t)kLabel8None;
+        N8[v] = (uint8_t)kLabel8None;
         chg[v] = -1;
+        atomicOr(&cbits[v >> 5], 1u << (v & 31));
+    }
+}
+
+// ---------------------------------------------------------------- heavy-row histograms
+
+// Heavy rows of B_t decide from their kept histograms (one thread per row).
+__global__ void k_lp_heavy_hist(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const int32_t *L = J.L;
+    const ListDev B = J.B;
+    const int k = J.k;
+    int32_t *chg = J.chg;
+    uint8_t *N8 = J.N8;
+    const ListDev D = J.D;
+    const int64_t *off = J.g.off;
+    if (is_done(J.ctl)) return;
+    __shared__ double s_pen[kLpCap];
+    const int E = k < kLpCap ? k : kLpCap;
+    if ((int)threadIdx.x < E) s_pen[threadIdx.x] = J.pen[threadIdx.x];
+    __syncthreads();
+    const int64_t ns = (int64_t)B.cnt[LC_SLOTS];
+    for (int64_t b0 = vwave(vg) * kWave; b0 < ns; b0 += vwaves(vg) * kWave) {
+        const int64_t slot = b0 + lane_id();
+        bool changed = false;
+        int32_t v = 0;
+        int64_t deg = 0;
+        if (slot < ns) {
+            v = B.hv[slot];
+            deg = off[v + 1] - off[v];
+            const int32_t *h = J.hh + (int64_t)J.hid[v] * kLpCap;
+            const int32_t own = L[v];
+            const int32_t nl = lp_decide(h, 1, E, s_pen, own);
+            changed = nl != own;
+            if (changed) {
+                chg[v] = nl;
+                N8[v] = (uint8_t)(nl < kLabel8None ? nl : kLabel8None);
+            }
+        }
+        list_append(D, changed, v, deg, J.heavy_degree, J.chunk_edges);
+    }
+}
+
+// After the apply: recount the heavy histograms when many vertices changed,
+// else update them from the changed neighbours only.
+__global__ void k_hist_mode(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    Ctl *ctl = J.ctl;
+    if (is_done(ctl)) return;
+    const unsigned long long chg_deg = J.D.cnt[LC_DEG];
+    ctl->hist_mode = (double)chg_deg >= kRecountShare * (double)J.g.nnz ? 1 : 0;
+}
+
+__global__ void k_hist_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    if (is_done(J.ctl) || J.ctl->hist_mode == 0) return;
+    const int64_t cnt = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
+    for (int64_t i = vthread(vg); i < cnt; i += vthreads(vg)) J.hh[i] = 0;
+}
+
+// One chunk of a heavy row per wave.  Update mode tests each neighbour's bit
+// in the changed-vertex bitmap (n bits: a small, well-cached array) and
+// moves one count from its old to its new label in a per-wave LDS row (few
+// neighbours change, so LDS atomics rarely collide); recount mode counts
+// every neighbour's label in per-lane rows.
+__global__ __launch_bounds__(kThreads) void k_hist_update(const P2Job *JS, const int32_t *bstart,
+                                                          int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev all = J.all;
+    const uint8_t *L8 = J.L8, *O8 = J.O8;
+    const uint32_t *cbits = J.cbits;
+    int32_t *hh = J.hh;
+    if (is_done(J.ctl)) return;
+    const bool recount = J.ctl->hist_mode != 0;
+    const int E = J.k < kLpCap ? J.k : kLpCap;
+    const int chunk_edges = J.chunk_edges;
+    __shared__ int32_t s_h[kWavesPerBlock][kWave * kHistStride];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    int32_t *row = &s_h[w][lane * kHistStride];  // recount: this lane's row
+    int32_t *wrow = &s_h[w][0];                  // update: the wave's row
+    // Rows stay zero between chunks (the flush clears what it read).
+    for (int l = 0; l < E; ++l) row[l] = 0;
+    wave_sync();
+    const int64_t nch = (int64_t)all.cnt[LC_CHUNKS];
+    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
+        int32_t slot, v;
+        int64_t e0, e1, deg;
+        chunk_bounds(g, all, all.hc[j], chunk_edges, slot, v, e0, e1, deg);
+        bool touched = recount;
+        for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
+            int32_t u[kHeavyUnroll];
+#pragma unroll
+            for (int q = 0; q < kHeavyUnroll; ++q) {
+                const int64_t e = eb + q * kWave + lane;
+                u[q] = e < e1 ? __builtin_nontemporal_load(&g.col[e]) : -1;
+            }
+            if (recount) {
+                int32_t lab[kHeavyUnroll];
+#pragma unroll
+                for (int q = 0; q < kHeavyUnroll; ++q) lab[q] = u[q] >= 0 ? (int32_t)L8[u[q]] : kLabel8None;
+#pragma unroll
+                for (int q = 0; q < kHeavyUnroll; ++q)
+                    if (lab[q] < E) row[lab[q]] += 1;
+            } else {
+                uint32_t word[kHeavyUnroll];
+#pragma unroll
+                for (int q = 0; q < kHeavyUnroll; ++q) word[q] = u[q] >= 0 ? cbits[u[q] >> 5] : 0u;
+#pragma unroll
+                for (int q = 0; q < kHeavyUnroll; ++q) {
+                    if (!((word[q] >> (u[q] & 31)) & 1u)) continue;
+                    const int32_t o = O8[u[q]], nw = L8[u[q]];
+                    if (o < E) atomicAdd(&wrow[o], -1);
+                    if (nw < E) atomicAdd(&wrow[nw], 1);
+                    touched = true;
+                }
+            }
+        }
+        if (__ballot(touched) == 0) continue;  // wave-uniform
+        wave_sync();
+        if (lane < E) {
+            int32_t c = 0;
+            if (recount) {
+                for (int r = 0; r < kWave; ++r) c += s_h[w][r * kHistStride + lane];
+            } else {
+                c = wrow[lane];
+            }
+            if (c) atomicAdd(&hh[(int64_t)slot * kLpCap + lane], c);
+        }
+        wave_sync();
+        if (recount) {
+            for (int l = 0; l < E; ++l) row[l] = 0;
+        } else if (lane < E) {
+            wrow[lane] = 0;
+        }
+        wave_sync();
+    }
+}
+
+// Clear the changed-vertex bits of this round (every writer stores zero).
+__global__ void k_cbits_clear(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    const ListDev D = J.D;
+    if (is_done(J.ctl)) return;
+    const int64_t nl = (int64_t)D.cnt[LC_LIGHT], ns = (int64_t)D.cnt[LC_SLOTS];
+    for (int64_t i = vthread(vg); i < nl + ns; i += vthreads(vg)) {
+        const int32_t v = i < nl ? D.light[i] : D.hv[i - nl];
+        J.cbits[v >> 5] = 0u;
     }
 }
 
@@ -890,6 +1143,11 @@ __global__ void k_init_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const int64_t n = J.g.n;
     for (int64_t v = vthread(vg); v < n; v += vthreads(vg)) J.diff[v] = 0;
+    for (int64_t i = vthread(vg); i < 2 * J.bwords; i += vthreads(vg)) J.bbits[i] = 0u;
+    for (int64_t i = vthread(vg); i < J.bwords; i += vthreads(vg)) J.cbits[i] = 0u;
+    for (int64_t v = vthread(vg); v < n; v += vthreads(vg)) J.N8[v] = (uint8_t)kLabel8None;
+    const int64_t nhh = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
+    for (int64_t i = vthread(vg); i < nhh; i += vthreads(vg)) J.hh[i] = 0;
     if (vg.block == 0) {
         uint32_t *c = reinterpret_cast<uint32_t *>(J.ctl);
         for (int i = threadIdx.x; i < (int)(sizeof(Ctl) / 4); i += blockDim.x) c[i] = 0;
@@ -899,10 +1157,28 @@ __global__ void k_init_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
 
 __global__ void k_set_remaining(int32_t *remaining, int32_t v) { *remaining = v; }
 
+// After the selection: pack B_t (vertices stamped with this round's epoch)
+// into the bitmap of half (t & 1), which the next round's pull candidates
+// test.  One ballot per 64 vertices: coalesced stamp reads, two word stores.
+__global__ void k_pack_bbits(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    if (is_done(J.ctl)) return;
+    const int32_t epoch = J.ctl->epoch;
+    const int64_t n = J.g.n;
+    uint32_t *half = J.bbits + (int64_t)(J.ctl->t & 1) * J.bwords;
+    for (int64_t base = vwave(vg) * kWave; base < n; base += vwaves(vg) * kWave) {
+        const int64_t v = base + lane_id();
+        const uint64_t m = __ballot(v < n && J.bstamp[v] == epoch);
+        const int64_t wi = base >> 5;
+        if (lane_id() == 0) half[wi] = (uint32_t)m;
+        if (lane_id() == 1 && wi + 1 < J.bwords) half[wi + 1] = (uint32_t)(m >> 32);
+    }
+}
+
 // done flag of every job of a set, gathered for one host copy.
 __global__ void k_p2_gather_done(const P2Job *JS, int jobs, int32_t *out) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < jobs; j += gridDim.x * blockDim.x)
-        out[j] = JS[j].ctl->done;
+        out[j] = gload(JS[j].ctl)->done;
 }
 
 }  // namespace
@@ -920,10 +1196,17 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.diff.alloc(sizeof(int32_t) * (size_t)n);
         w.stamp.alloc(sizeof(int32_t) * (size_t)n);
         w.bstamp.alloc(sizeof(int32_t) * (size_t)n);
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.bstamp.p, 0, w.bstamp.bytes, ctx->stream));
+        w.bbits.alloc(sizeof(uint32_t) * 2 * (size_t)((n + 31) / 32));
         w.L8.alloc((size_t)n);
+        w.O8.alloc((size_t)n);
+        w.N8.alloc((size_t)n);
+        w.cbits.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.N8.p, 0xff, w.N8.bytes, ctx->stream));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.cbits.p, 0, w.cbits.bytes, ctx->stream));
         w.bd.alloc(n);
         DMOLP_HIP_CHECK(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, ctx->stream));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.bstamp.p, 0, w.bstamp.bytes, ctx->stream));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.bbits.p, 0, w.bbits.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.chg.p, 0xff, w.chg.bytes, ctx->stream));
         w.ctl.alloc(sizeof(Ctl));
     }
@@ -931,7 +1214,8 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.B.alloc(n, g->n_heavy, g->n_heavy_chunks);
         w.D.alloc(n, g->n_heavy, g->n_heavy_chunks);
         w.hub_hist.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1) * kLpCap);
-        w.deferred.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1));
+        w.hh.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1) * kLpCap);
+        w.deferred.alloc(sizeof(int64_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.hub_hist.p, 0, w.hub_hist.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.B.cnt.p, 0, w.B.cnt.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.D.cnt.p, 0, w.D.cnt.bytes, ctx->stream));
@@ -977,7 +1261,14 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.diff = w.diff.as<int32_t>();
     j.stamp = w.stamp.as<int32_t>();
     j.bstamp = w.bstamp.as<int32_t>();
-    j.deferred = w.deferred.as<int32_t>();
+    j.bbits = w.bbits.as<uint32_t>();
+    j.bwords = (g->n + 31) / 32;
+    j.hh = w.hh.as<int32_t>();
+    j.hid = g->hid.as<int32_t>();
+    j.cbits = w.cbits.as<uint32_t>();
+    j.O8 = w.O8.as<uint8_t>();
+    j.N8 = w.N8.as<uint8_t>();
+    j.deferred = w.deferred.as<int64_t>();
     j.hub_hist = w.hub_hist.as<int32_t>();
     j.bd = w.bd.dev;
     j.ctl = w.ctl.as<Ctl>();
@@ -1056,13 +1347,13 @@ void phase2_init_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     hipLaunchKernelGGL(k_set_remaining, dim3(1), dim3(1), 0, s, set.remaining, (int32_t)J);
     hipLaunchKernelGGL(k_init_zero, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), J);
+    hipLaunchKernelGGL(k_mirror8, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), J);
     hipLaunchKernelGGL(k_init_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), J);
     if (set.any_heavy)
         hipLaunchKernelGGL(k_init_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                            ls.bs(P_GRID), J);
-    hipLaunchKernelGGL(k_mirror8, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
-                       ls.bs(P_GRID), J);
     hipLaunchKernelGGL(k_init_finish, dim3(ls.total[P_ONE]), dim3(256), 0, s, set.jobs, ls.bs(P_ONE),
                        J);
     DMOLP_HIP_CHECK(hipGetLastError());
@@ -1094,28 +1385,36 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
         hipLaunchKernelGGL(k_cand_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m1 = mark();
     select_enqueue(ctx, set.sel, ls, set.passes);
+    hipLaunchKernelGGL(k_pack_bbits, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m2 = mark();
     hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m3 = mark();
-    if (set.any_heavy) {
-        hipLaunchKernelGGL(k_lp_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
-        hipLaunchKernelGGL(k_lp_heavy_decide, dim3(ls.total[P_HEAVY]), dim3(kThreads), 0, s, jobs,
+    if (set.any_heavy)  // heavy rows decide from their kept histograms
+        hipLaunchKernelGGL(k_lp_heavy_hist, dim3(ls.total[P_HEAVY]), dim3(kThreads), 0, s, jobs,
                            ls.bs(P_HEAVY), J);
-    }
     const size_t m4 = mark();
     hipLaunchKernelGGL(k_delta_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     if (set.any_heavy)
         hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    const size_t m5 = mark();
+    if (set.any_heavy) {  // bring the heavy-row histograms to this round's labels
+        hipLaunchKernelGGL(k_hist_mode, dim3(ls.total[P_ONE]), dim3(1), 0, s, jobs, ls.bs(P_ONE), J);
+        hipLaunchKernelGGL(k_hist_zero, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+        hipLaunchKernelGGL(k_hist_update, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    }
+    hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    const size_t m6 = mark();
     hipLaunchKernelGGL(k_round_end, dim3(ls.total[P_ONE]), dim3(1), 0, s, jobs, ls.bs(P_ONE), J);
     DMOLP_HIP_CHECK(hipGetLastError());
     if (ctx->profiling) {
-        const size_t m5 = mark();
-        // Groups: 0 lp_light, 1 lp_heavy, 2 candidates, 3 select, 4 delta+apply+end, 5 round.
-        const std::pair<size_t, size_t> spans[6] = {{m2, m3}, {m3, m4}, {m0, m1}, {m1, m2},
-                                                    {m4, m5}, {m0, m5}};
-        if (ctx->prof_marks.size() < 6) ctx->prof_marks.resize(6);
-        for (int i = 0; i < 6; ++i) ctx->prof_marks[(size_t)i].push_back(spans[i]);
+        const size_t m7 = mark();
+        // Groups: 0 lp_light, 1 lp_heavy, 2 candidates, 3 select, 4 delta+apply,
+        // 5 round, 6 heavy-row histogram update.
+        const std::pair<size_t, size_t> spans[7] = {{m2, m3}, {m3, m4}, {m0, m1}, {m1, m2},
+                                                    {m4, m5}, {m0, m7}, {m5, m6}};
+        if (ctx->prof_marks.size() < 7) ctx->prof_marks.resize(7);
+        for (int i = 0; i < 7; ++i) ctx->prof_marks[(size_t)i].push_back(spans[i]);
     }
 }
 
@@ -1220,6 +1519,7 @@ void phase2_lp_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
                            set.jobs, ls.bs(P_HEAVY), ls.jobs);
     }
     hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs);
+    hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 

@@ -37,6 +37,8 @@ struct Ctl {
     int32_t epoch;       // stamp value of the current round (epoch_base + t)
     int32_t epoch_base;
     int32_t max_rounds;
+    int32_t hist_mode;   // heavy-row histograms after this round: 0 update, 1 recount
+    int32_t pad;
     double epsilon;
 };
 
@@ -51,7 +53,19 @@ struct P2Job {
     ListDev all, B, D;
     int32_t *L;
     uint8_t *L8;
-    int32_t *chg, *diff, *stamp, *bstamp, *deferred, *hub_hist;
+    int32_t *chg, *diff, *stamp, *hub_hist;
+    int64_t *deferred;  // long rows the pull pass left undecided: (flags << 32) | v
+    int32_t *bstamp;  // round stamp of the vertices selected into B_t
+    uint32_t *bbits;  // two bitmaps of n bits: B_t packed into half (t & 1)
+    int64_t bwords;   // words per bitmap
+    // Heavy rows keep their label histograms across rounds: hh[hid(v)][l]
+    // counts v's neighbours with label l < min(k, 32), updated after each
+    // round from the vertices that changed (bit set in cbits; old label in
+    // O8, new in L8).
+    int32_t *hh;
+    const int32_t *hid;  // vertex -> heavy row index of the graph, -1 for light rows
+    uint32_t *cbits;
+    uint8_t *O8, *N8;    // old label / new label this round (255: unchanged)
     RecDev bd;
     Ctl *ctl;
     unsigned long long *vc, *ic;
@@ -79,8 +93,9 @@ struct Phase2Work {
     int k = 0;
     int64_t cap_heavy = -1, cap_chunks = -1;  // heavy capacities the lists were sized for
     int32_t epoch = 0;  // stamp value of the current candidate pass
-    DevMem chg, diff, stamp, bstamp, deferred;
+    DevMem chg, diff, stamp, bstamp, bbits, deferred;
     DevMem L8;  // byte mirror of the labels for the LP gathers
+    DevMem hh, cbits, O8, N8;
     List B, D;          // boundary list (B_{t-1} read, then B_t written), Delta
     DevMem rtab;        // per-round constants: [0..R] 1 + log(t), [R+1..2R+1] exp(-t/5)
     int rtab_rounds = -1;

@@ -11,6 +11,22 @@
 
 namespace dmolp {
 
+__device__ __forceinline__ GraphDev gload(const GraphDev &v) {
+    GraphDev r = v;
+    r.off = gload(v.off);
+    r.col = gload(v.col);
+    return r;
+}
+__device__ __forceinline__ ListDev gload(const ListDev &v) {
+    ListDev r = v;
+    r.light = gload(v.light);
+    r.hv = gload(v.hv);
+    r.hn = gload(v.hn);
+    r.hc = gload(v.hc);
+    r.cnt = gload(v.cnt);
+    return r;
+}
+
 // LDS block of one wave's batch: inclusive degree prefix and the value
 // e - q for each slot, so edge index = s_base[slot] + q.
 struct BatchLds {

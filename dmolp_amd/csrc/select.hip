@@ -53,12 +53,32 @@ __device__ void block_crossing(const int32_t *bins, int nb, bool reverse, int64_
     __syncthreads();
 }
 
+using ::dmolp::gload;
+__device__ __forceinline__ SelArgs gload(const SelArgs &v) {
+    SelArgs r = v;
+    r.in = gload(v.in);
+    r.sel = gload(v.sel);
+    r.carry = gload(v.carry);
+    r.ref = gload(v.ref);
+    r.h1 = gload(v.h1);
+    r.h2 = gload(v.h2);
+    r.st = gload(v.st);
+    r.refining = gload(v.refining);
+    r.done = gload(v.done);
+    r.mark = gload(v.mark);
+    r.mark_val = gload(v.mark_val);
+    r.rule.ratio_tab = gload(v.rule.ratio_tab);
+    r.rule.t_ptr = gload(v.rule.t_ptr);
+    r.rule.re = gload(v.rule.re);
+    return r;
+}
+
 // Kernels take the job set (one SelArgs per job) and run on the job's
 // share of the launch.
 #define SEL_JOB()                                      \
     const int job_ = job_of_block(bstart, jobs);       \
     const VGrid vg = vgrid_of(bstart, job_);           \
-    const SelArgs &a = A[job_];                        \
+    const SelArgs a = gload(A[job_]);                  \
     (void)vg
 
 __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(const SelArgs *A, const int32_t *bstart,

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "rows.hpp"
 
 namespace dmolp {
 
@@ -24,6 +25,15 @@ struct RecDev {
     int32_t *d;  // degree
     unsigned long long *cnt;
 };
+
+__device__ __forceinline__ RecDev gload(const RecDev &v) {
+    RecDev r = v;
+    r.x = gload(v.x);
+    r.g = gload(v.g);
+    r.d = gload(v.d);
+    r.cnt = gload(v.cnt);
+    return r;
+}
 
 struct Recs {
     DevMem x, g, d, cnt;

@@ -82,13 +82,14 @@ void *dmolp_ctx_stream(dmolp_ctx *ctx);
  * in the reference, which times whole GPU calls with std::chrono,
  * phase2.cpp:324-339).  group: 0 light-row LP decision, 1 heavy-row LP
  * decision, 2 round start + candidates, 3 selection, 4 delta statistics +
- * apply + round end, 5 whole round. */
+ * apply, 5 whole round, 6 heavy-row histogram update. */
 #define DMOLP_PROF_LP_LIGHT 0
 #define DMOLP_PROF_LP_HEAVY 1
 #define DMOLP_PROF_CANDIDATES 2
 #define DMOLP_PROF_SELECT 3
 #define DMOLP_PROF_DELTA 4
 #define DMOLP_PROF_ROUND 5
+#define DMOLP_PROF_HIST 6   /* heavy-row histogram update after the apply */
 int dmolp_ctx_set_profiling(dmolp_ctx *ctx, int on);
 int dmolp_ctx_round_times(dmolp_ctx *ctx, int group, double *ms_out, int cap, int *n_out);
 
@@ -155,6 +156,9 @@ int dmolp_graph_upload(dmolp_ctx *ctx, int64_t n, int64_t nnz, const void *offse
 /* Generate directly on the device; same graph as dmolp_gen_host. */
 int dmolp_graph_generate(dmolp_ctx *ctx, const dmolp_gen_spec *spec, dmolp_graph **out);
 int dmolp_graph_info(const dmolp_graph *g, int64_t *n, int64_t *nnz);
+/* Degree binning of g under ctx's tuning (no counterpart in the reference):
+ * rows longer than heavy_degree, and the CSR entries they hold. */
+int dmolp_graph_bins(dmolp_ctx *ctx, dmolp_graph *g, int64_t *heavy_rows, int64_t *heavy_entries);
 /* Copy the CSR back to the host (off: n + 1 int64, col: nnz). */
 int dmolp_graph_download(dmolp_ctx *ctx, const dmolp_graph *g, int64_t *off, int32_t *col);
 int dmolp_graph_free(dmolp_ctx *ctx, dmolp_graph *g);
