@@ -11,7 +11,7 @@ namespace dmolp {
 constexpr int kWave = 64;
 constexpr int kLpCap = 32;          // gpu_lp_boundary.cu:360
 constexpr int kSelBins = 4096;      // level-1 selection bins: min(deg, 4095)
-constexpr int kDigitBits = 12;      // radix-select digit width
+constexpr int kDigitBits = 8;       // radix-select digit width (histograms fit LDS)
 constexpr int kDigitBins = 1 << kDigitBits;
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }

@@ -33,6 +33,7 @@ constexpr int kLpUnroll = 4;     // light rows: gathers in flight per lane
 constexpr int kHeavyUnroll = 8;  // heavy rows: gathers in flight per lane
 constexpr int32_t kLabel8None = 255;  // byte-mirror value of labels the LP cannot see
 constexpr int kScanItems = 4;         // vertices per thread per block iteration in scans
+constexpr int kCandFirst = 2;         // neighbours per row examined in the joint first pass
 // Up to this k every valid label fits the byte mirror exactly (255 stands
 // for -1 and out-of-range labels), so the sweeps gather bytes, not words.
 constexpr int kExact8 = 254;
@@ -77,7 +78,7 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.hh = gload(v.hh);
     r.hid = gload(v.hid);
     r.cbits = gload(v.cbits);
-    r.O8 = gload(v.O8);
+    r.ON = gload(v.ON);
     r.N8 = gload(v.N8);
     r.bd = gload(v.bd);
     r.ctl = gload(v.ctl);
@@ -416,31 +417,61 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
     for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
         uint32_t rec = 0;
         int32_t nr = 0, deg[kScanItems], lab[kScanItems];
+        int64_t xs[kScanItems], lim[kScanItems], end[kScanItems];
+        bool hit[kScanItems], dif[kScanItems];
+        // The items' first neighbours are examined together (independent
+        // loads in flight), then undecided rows continue one by one.
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             deg[j] = 0;
             lab[j] = 0;
+            xs[j] = 0;
+            lim[j] = 0;
+            end[j] = 0;
+            hit[j] = true;
+            dif[j] = true;
             if (v >= g.n) continue;
             const int64_t s = g.off[v], e = g.off[v + 1];
-            const int32_t lv = L[v];
             deg[j] = (int32_t)(e - s);
-            lab[j] = lv;
-            bool hit = bit_of(inb, (int32_t)v), dif = false;
-            const int64_t lim = e - s > heavy_degree ? s + heavy_degree : e;
-            for (int64_t x = s; x < lim && !(hit && dif); ++x) {
-                const int32_t u = g.col[x];
-                if (!dif) dif = nb_label(exact8, L8, L, u) != lv;
-                if (!hit) hit = bit_of(inb, u);
+            lab[j] = L[v];
+            hit[j] = bit_of(inb, (int32_t)v);
+            dif[j] = false;
+            xs[j] = s;
+            end[j] = e;
+            lim[j] = e - s > heavy_degree ? s + heavy_degree : e;
+        }
+#pragma unroll
+        for (int r = 0; r < kCandFirst; ++r) {
+            int32_t u[kScanItems];
+#pragma unroll
+            for (int j = 0; j < kScanItems; ++j)
+                u[j] = (xs[j] < lim[j] && !(hit[j] && dif[j])) ? g.col[xs[j]] : -1;
+#pragma unroll
+            for (int j = 0; j < kScanItems; ++j) {
+                if (u[j] < 0) continue;
+                if (!dif[j]) dif[j] = nb_label(exact8, L8, L, u[j]) != lab[j];
+                if (!hit[j]) hit[j] = bit_of(inb, u[j]);
+                ++xs[j];
             }
+        }
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            for (int64_t x = xs[j]; x < lim[j] && !(hit[j] && dif[j]); ++x) {
+                const int32_t u = g.col[x];
+                if (!dif[j]) dif[j] = nb_label(exact8, L8, L, u) != lab[j];
+                if (!hit[j]) hit[j] = bit_of(inb, u);
+            }
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            if (v >= g.n) continue;
             // A long row still undecided after heavy_degree entries is
             // finished by k_cand_pull_heavy (rare).
-            if (!(hit && dif) && lim < e) {
+            if (!(hit[j] && dif[j]) && lim[j] < end[j]) {
                 const unsigned long long i = atomicAdd(&ctl->v[C_DEFERRED], 1ull);
-                deferred[i] = ((int64_t)((dif ? 1 : 0) | (hit ? 2 : 0)) << 32) | (int64_t)v;
+                deferred[i] = ((int64_t)((dif[j] ? 1 : 0) | (hit[j] ? 2 : 0)) << 32) | v;
                 continue;
             }
-            if (hit && dif) { rec |= 1u << j; ++nr; }
+            if (hit[j] && dif[j]) { rec |= 1u << j; ++nr; }
         }
         int64_t pos = block_reserve(nr, bd.cnt, s_app);
 #pragma unroll
@@ -562,13 +593,13 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
 // below E; among equal best scores the smallest 5-bit bit-reversed label
 // wins (the order the 32-lane shuffle-down tree yields); the winner must
 // beat the raw own-label count.
-__device__ __forceinline__ int32_t lp_decide(const int32_t *h, int stride, int E, const double *pen,
-                                             int32_t own) {
-    const double base = (own >= 0 && own < E) ? (double)h[own * stride] : -1e300;
+template <class Count>
+__device__ __forceinline__ int32_t lp_decide_by(Count count, int E, const double *pen, int32_t own) {
+    const double base = (own >= 0 && own < E) ? (double)count(own) : -1e300;
     int best_l = 0;
     double best_s = 0.0;
     for (int l = 0; l < E; ++l) {
-        double s = (double)h[l * stride];
+        double s = (double)count(l);
         if (s > 0.0) s *= (1.0 + pen[l]);
         if (l == 0 || s > best_s || (s == best_s && bitrev5(l) < bitrev5(best_l))) {
             best_l = l;
@@ -577,6 +608,16 @@ __device__ __forceinline__ int32_t lp_decide(const int32_t *h, int stride, int E
     }
     return (best_s > base) ? best_l : own;
 }
+
+__device__ __forceinline__ int32_t lp_decide(const int32_t *h, int stride, int E, const double *pen,
+                                             int32_t own) {
+    return lp_decide_by([&](int l) { return h[l * stride]; }, E, pen, own);
+}
+
+// Light rows count in 16-bit halves of 32-bit LDS words (a light row has at
+// most 65535 entries): label l in word l / 2, half l % 2.
+constexpr int kPackedWords = kLpCap / 2;
+constexpr int kPackedStride = kPackedWords + 1;  // odd: rows of a wave fall in different banks
 
 __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
@@ -594,13 +635,14 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
     const Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
-    __shared__ int32_t s_h[kWavesPerBlock][kWave * kHistStride];
+    __shared__ uint32_t s_h[kWavesPerBlock][kWave * kPackedStride];
     __shared__ double s_pen[kLpCap];
     const int E = k < kLpCap ? k : kLpCap;
+    const int EW = (E + 1) / 2;
     if ((int)threadIdx.x < E) s_pen[threadIdx.x] = pen[threadIdx.x];
     __syncthreads();
     const int w = threadIdx.x / kWave, lane = lane_id();
-    int32_t *h = s_h[w];
+    uint32_t *h = s_h[w];
     __shared__ BlockAppendLds s_app;
     const int64_t count = (int64_t)B.cnt[LC_LIGHT];
     // Block-uniform loop over groups of kWavesPerBlock batches (one per
@@ -610,7 +652,7 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
         const int64_t b0 = g0 + (int64_t)w * kWave;
         int32_t v, deg;
         bool valid;
-        for (int l = 0; l < E; ++l) h[lane * kHistStride + l] = 0;
+        for (int l = 0; l < EW; ++l) h[lane * kPackedStride + l] = 0u;
         const int32_t total = batch_load(g, B.light, count, b0, s_b[w], v, deg, valid);
         // kLpUnroll independent column loads, then kLpUnroll label gathers,
         // per lane per step: several misses in flight per lane.
@@ -629,14 +671,17 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
             for (int j = 0; j < kLpUnroll; ++j) lab[j] = u[j] >= 0 ? (int32_t)L8[u[j]] : kLabel8None;
 #pragma unroll
             for (int j = 0; j < kLpUnroll; ++j)
-                if (lab[j] < E) atomicAdd(&h[slot[j] * kHistStride + lab[j]], 1);
+                if (lab[j] < E)
+                    atomicAdd(&h[slot[j] * kPackedStride + (lab[j] >> 1)], 1u << ((lab[j] & 1) * 16));
         }
         wave_sync();
         bool changed = false;
         int32_t nl = 0;
         if (valid) {
             const int32_t own = L[v];
-            nl = lp_decide(&h[lane * kHistStride], 1, E, s_pen, own);
+            const uint32_t *row = &h[lane * kPackedStride];
+            nl = lp_decide_by([&](int l) { return (int32_t)((row[l >> 1] >> ((l & 1) * 16)) & 0xffffu); },
+                              E, s_pen, own);
             changed = nl != own;
             if (changed) {
                 chg[v] = nl;
@@ -912,7 +957,8 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const ListDev D = J.D;
     int32_t *L = J.L;
-    uint8_t *L8 = J.L8, *O8 = J.O8, *N8 = J.N8;
+    uint8_t *L8 = J.L8, *N8 = J.N8;
+    uint16_t *ON = J.ON;
     uint32_t *cbits = J.cbits;
     int32_t *chg = J.chg;
     const Ctl *ctl = J.ctl;
@@ -923,8 +969,10 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
         const int32_t v = i < nl ? D.light[i] : D.hv[i - nl];
         const int32_t nl_v = chg[v];
         L[v] = nl_v;
-        O8[v] = L8[v];
-        L8[v] = (nl_v >= 0 && nl_v < kLabel8None) ? (uint8_t)nl_v : (uint8_t)kLabel8None;
+        const uint32_t ob = L8[v];
+        const uint32_t nb = (nl_v >= 0 && nl_v < kLabel8None) ? (uint32_t)nl_v : (uint32_t)kLabel8None;
+        ON[v] = (uint16_t)(ob | (nb << 8));
+        L8[v] = (uint8_t)nb;
         N8[v] = (uint8_t)kLabel8None;
         chg[v] = -1;
         atomicOr(&cbits[v >> 5], 1u << (v & 31));
@@ -987,36 +1035,27 @@ __global__ void k_hist_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
     for (int64_t i = vthread(vg); i < cnt; i += vthreads(vg)) J.hh[i] = 0;
 }
 
-// One chunk of a heavy row per wave.  Update mode tests each neighbour's bit
-// in the changed-vertex bitmap (n bits: a small, well-cached array) and
-// moves one count from its old to its new label in a per-wave LDS row (few
-// neighbours change, so LDS atomics rarely collide); recount mode counts
-// every neighbour's label in per-lane rows.
-__global__ __launch_bounds__(kThreads) void k_hist_update(const P2Job *JS, const int32_t *bstart,
-                                                          int jobs) {
+// Recount mode (after rounds where many vertices changed): every heavy
+// row's label histogram from scratch, one chunk per wave, per-lane LDS rows.
+__global__ __launch_bounds__(kThreads) void k_hist_recount(const P2Job *JS, const int32_t *bstart,
+                                                           int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
     const ListDev all = J.all;
-    const uint8_t *L8 = J.L8, *O8 = J.O8;
-    const uint32_t *cbits = J.cbits;
+    const uint8_t *L8 = J.L8;
     int32_t *hh = J.hh;
-    if (is_done(J.ctl)) return;
-    const bool recount = J.ctl->hist_mode != 0;
+    if (is_done(J.ctl) || J.ctl->hist_mode == 0) return;
     const int E = J.k < kLpCap ? J.k : kLpCap;
     const int chunk_edges = J.chunk_edges;
     __shared__ int32_t s_h[kWavesPerBlock][kWave * kHistStride];
     const int w = threadIdx.x / kWave, lane = lane_id();
-    int32_t *row = &s_h[w][lane * kHistStride];  // recount: this lane's row
-    int32_t *wrow = &s_h[w][0];                  // update: the wave's row
-    // Rows stay zero between chunks (the flush clears what it read).
-    for (int l = 0; l < E; ++l) row[l] = 0;
-    wave_sync();
+    int32_t *row = &s_h[w][lane * kHistStride];
     const int64_t nch = (int64_t)all.cnt[LC_CHUNKS];
     for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         chunk_bounds(g, all, all.hc[j], chunk_edges, slot, v, e0, e1, deg);
-        bool touched = recount;
+        for (int l = 0; l < E; ++l) row[l] = 0;
         for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
             int32_t u[kHeavyUnroll];
 #pragma unroll
@@ -1024,42 +1063,76 @@ __global__ __launch_bounds__(kThreads) void k_hist_update(const P2Job *JS, const
                 const int64_t e = eb + q * kWave + lane;
                 u[q] = e < e1 ? __builtin_nontemporal_load(&g.col[e]) : -1;
             }
-            if (recount) {
-                int32_t lab[kHeavyUnroll];
+            int32_t lab[kHeavyUnroll];
 #pragma unroll
-                for (int q = 0; q < kHeavyUnroll; ++q) lab[q] = u[q] >= 0 ? (int32_t)L8[u[q]] : kLabel8None;
+            for (int q = 0; q < kHeavyUnroll; ++q) lab[q] = u[q] >= 0 ? (int32_t)L8[u[q]] : kLabel8None;
 #pragma unroll
-                for (int q = 0; q < kHeavyUnroll; ++q)
-                    if (lab[q] < E) row[lab[q]] += 1;
-            } else {
-                uint32_t word[kHeavyUnroll];
+            for (int q = 0; q < kHeavyUnroll; ++q)
+                if (lab[q] < E) row[lab[q]] += 1;
+        }
+        wave_sync();
+        if (lane < E) {
+            int32_t c = 0;
+            for (int r = 0; r < kWave; ++r) c += s_h[w][r * kHistStride + lane];
+            if (c) atomicAdd(&hh[(int64_t)slot * kLpCap + lane], c);
+        }
+        wave_sync();
+    }
+}
+
+// Update mode: one chunk of a heavy row per wave.  Each neighbour's bit in
+// the changed-vertex bitmap (n bits: a small, well-cached array) is tested;
+// a changed neighbour moves one count from its old to its new label (one
+// 16-bit gather: old | new << 8) in a per-wave LDS row.  Few neighbours
+// change, so the LDS atomics rarely collide, and the kernel needs almost no
+// LDS, which keeps occupancy high for the column stream.
+__global__ __launch_bounds__(kThreads) void k_hist_update(const P2Job *JS, const int32_t *bstart,
+                                                          int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev all = J.all;
+    const uint16_t *ON = J.ON;
+    const uint32_t *cbits = J.cbits;
+    int32_t *hh = J.hh;
+    if (is_done(J.ctl) || J.ctl->hist_mode != 0) return;
+    const int E = J.k < kLpCap ? J.k : kLpCap;
+    const int chunk_edges = J.chunk_edges;
+    __shared__ int32_t s_w[kWavesPerBlock][kLpCap];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    int32_t *wrow = s_w[w];
+    if (lane < kLpCap) wrow[lane] = 0;
+    wave_sync();
+    const int64_t nch = (int64_t)all.cnt[LC_CHUNKS];
+    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
+        int32_t slot, v;
+        int64_t e0, e1, deg;
+        chunk_bounds(g, all, all.hc[j], chunk_edges, slot, v, e0, e1, deg);
+        bool touched = false;
+        for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
+            int32_t u[kHeavyUnroll];
 #pragma unroll
-                for (int q = 0; q < kHeavyUnroll; ++q) word[q] = u[q] >= 0 ? cbits[u[q] >> 5] : 0u;
+            for (int q = 0; q < kHeavyUnroll; ++q) {
+                const int64_t e = eb + q * kWave + lane;
+                u[q] = e < e1 ? __builtin_nontemporal_load(&g.col[e]) : -1;
+            }
+            uint32_t word[kHeavyUnroll];
 #pragma unroll
-                for (int q = 0; q < kHeavyUnroll; ++q) {
-                    if (!((word[q] >> (u[q] & 31)) & 1u)) continue;
-                    const int32_t o = O8[u[q]], nw = L8[u[q]];
-                    if (o < E) atomicAdd(&wrow[o], -1);
-                    if (nw < E) atomicAdd(&wrow[nw], 1);
-                    touched = true;
-                }
+            for (int q = 0; q < kHeavyUnroll; ++q) word[q] = u[q] >= 0 ? cbits[u[q] >> 5] : 0u;
+#pragma unroll
+            for (int q = 0; q < kHeavyUnroll; ++q) {
+                if (!((word[q] >> (u[q] & 31)) & 1u)) continue;
+                const uint32_t on = ON[u[q]];
+                const int32_t o = (int32_t)(on & 0xffu), nw = (int32_t)(on >> 8);
+                if (o < E) atomicAdd(&wrow[o], -1);
+                if (nw < E) atomicAdd(&wrow[nw], 1);
+                touched = true;
             }
         }
         if (__ballot(touched) == 0) continue;  // wave-uniform
         wave_sync();
         if (lane < E) {
-            int32_t c = 0;
-            if (recount) {
-                for (int r = 0; r < kWave; ++r) c += s_h[w][r * kHistStride + lane];
-            } else {
-                c = wrow[lane];
-            }
+            const int32_t c = wrow[lane];
             if (c) atomicAdd(&hh[(int64_t)slot * kLpCap + lane], c);
-        }
-        wave_sync();
-        if (recount) {
-            for (int l = 0; l < E; ++l) row[l] = 0;
-        } else if (lane < E) {
             wrow[lane] = 0;
         }
         wave_sync();
@@ -1199,7 +1272,7 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         DMOLP_HIP_CHECK(hipMemsetAsync(w.bstamp.p, 0, w.bstamp.bytes, ctx->stream));
         w.bbits.alloc(sizeof(uint32_t) * 2 * (size_t)((n + 31) / 32));
         w.L8.alloc((size_t)n);
-        w.O8.alloc((size_t)n);
+        w.ON.alloc(sizeof(uint16_t) * (size_t)n);
         w.N8.alloc((size_t)n);
         w.cbits.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.N8.p, 0xff, w.N8.bytes, ctx->stream));
@@ -1266,7 +1339,7 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.hh = w.hh.as<int32_t>();
     j.hid = g->hid.as<int32_t>();
     j.cbits = w.cbits.as<uint32_t>();
-    j.O8 = w.O8.as<uint8_t>();
+    j.ON = w.ON.as<uint16_t>();
     j.N8 = w.N8.as<uint8_t>();
     j.deferred = w.deferred.as<int64_t>();
     j.hub_hist = w.hub_hist.as<int32_t>();
@@ -1401,6 +1474,7 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     if (set.any_heavy) {  // bring the heavy-row histograms to this round's labels
         hipLaunchKernelGGL(k_hist_mode, dim3(ls.total[P_ONE]), dim3(1), 0, s, jobs, ls.bs(P_ONE), J);
         hipLaunchKernelGGL(k_hist_zero, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+        hipLaunchKernelGGL(k_hist_recount, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_update, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     }
     hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);

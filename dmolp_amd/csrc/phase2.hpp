@@ -60,12 +60,13 @@ struct P2Job {
     int64_t bwords;   // words per bitmap
     // Heavy rows keep their label histograms across rounds: hh[hid(v)][l]
     // counts v's neighbours with label l < min(k, 32), updated after each
-    // round from the vertices that changed (bit set in cbits; old label in
-    // O8, new in L8).
+    // round from the vertices that changed (bit set in cbits; old and new
+    // byte labels in ON as old | new << 8).
     int32_t *hh;
     const int32_t *hid;  // vertex -> heavy row index of the graph, -1 for light rows
     uint32_t *cbits;
-    uint8_t *O8, *N8;    // old label / new label this round (255: unchanged)
+    uint16_t *ON;
+    uint8_t *N8;         // new label this round (255: unchanged)
     RecDev bd;
     Ctl *ctl;
     unsigned long long *vc, *ic;
@@ -95,7 +96,7 @@ struct Phase2Work {
     int32_t epoch = 0;  // stamp value of the current candidate pass
     DevMem chg, diff, stamp, bstamp, bbits, deferred;
     DevMem L8;  // byte mirror of the labels for the LP gathers
-    DevMem hh, cbits, O8, N8;
+    DevMem hh, cbits, ON, N8;
     List B, D;          // boundary list (B_{t-1} read, then B_t written), Delta
     DevMem rtab;        // per-round constants: [0..R] 1 + log(t), [R+1..2R+1] exp(-t/5)
     int rtab_rounds = -1;

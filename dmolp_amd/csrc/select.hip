@@ -219,6 +219,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, c
     }
 }
 
+// Digit histogram of the records still being refined.  With few enough
+// groups the block accumulates in LDS and adds its counts once; otherwise
+// every record adds to the global histogram.
+constexpr int kHist2LdsInts = 16384;
 __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(const SelArgs *A, const int32_t *bstart,
                                                            int jobs) {
     SEL_JOB();
@@ -229,6 +233,13 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(const SelArgs *A, con
     const SelState *st = a.st;
     const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
     const int64_t count = (int64_t)*in.cnt;
+    const int nb = a.groups * kDigitBins;
+    const bool lds = nb <= kHist2LdsInts;
+    __shared__ int32_t s_h[kHist2LdsInts];
+    if (lds) {
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) s_h[i] = 0;
+        __syncthreads();
+    }
     for (int64_t i = vthread(vg); i < count; i += vthreads(vg)) {
         const int32_t gg = in.g[i];
         const SelState s = st[gg];
@@ -239,7 +250,13 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(const SelArgs *A, con
         if ((K >> s.r) != (s.prefix >> s.r)) continue;
         const int w = s.r < kDigitBits ? s.r : kDigitBits;
         const int digit = (int)((K >> (s.r - w)) & ((1ull << w) - 1));
-        atomicAdd(&h2[(int64_t)gg * kDigitBins + digit], 1);
+        if (lds) atomicAdd(&s_h[gg * kDigitBins + digit], 1);
+        else atomicAdd(&h2[(int64_t)gg * kDigitBins + digit], 1);
+    }
+    if (lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb; i += blockDim.x)
+            if (s_h[i]) atomicAdd(&h2[i], s_h[i]);
     }
 }
 
