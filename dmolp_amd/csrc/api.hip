@@ -243,6 +243,8 @@ int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out) {
     out->grid_blocks = ctx->grid_blocks();
     out->pull_per_mille = (int32_t)(ctx->tune.pull_fraction * 1000.0 + 0.5);
     out->batch_work_per_block = (int32_t)ctx->tune.batch_work_per_block;
+    out->hist_recount_per_mille = (int32_t)(ctx->tune.hist_recount_share * 1000.0 + 0.5);
+    out->hist_push_per_mille = (int32_t)(ctx->tune.hist_push_share * 1000.0 + 0.5);
     return DMOLP_OK;
 }
 
@@ -251,7 +253,9 @@ int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
     // Light rows keep 16-bit per-row counters: heavy_degree stays below 2^16.
     if (t->heavy_degree < 0 || t->heavy_degree > 65535 || t->chunk_edges < 0 ||
         t->grid_blocks < 0 || t->pull_per_mille < 0 || t->pull_per_mille > 1000000 ||
-        t->batch_work_per_block < 0 ||
+        t->batch_work_per_block < 0 || t->hist_recount_per_mille < 0 ||
+        t->hist_recount_per_mille > 1001 || t->hist_push_per_mille < 0 ||
+        t->hist_push_per_mille > 1001 ||
         (t->block_threads != 0 && t->block_threads != 256))
         return fail(ctx, DMOLP_E_ARG, "tuning value out of range");
     if (t->heavy_degree) ctx->tune.heavy_degree = t->heavy_degree;
@@ -259,6 +263,10 @@ int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
     if (t->grid_blocks) ctx->tune.grid_blocks = t->grid_blocks;
     if (t->pull_per_mille > 0) ctx->tune.pull_fraction = t->pull_per_mille / 1000.0;
     if (t->batch_work_per_block > 0) ctx->tune.batch_work_per_block = t->batch_work_per_block;
+    // 0 keeps the current share; 1..1000 sets it; 1001 means "never" (above 1).
+    if (t->hist_recount_per_mille > 0)
+        ctx->tune.hist_recount_share = t->hist_recount_per_mille / 1000.0;
+    if (t->hist_push_per_mille > 0) ctx->tune.hist_push_share = t->hist_push_per_mille / 1000.0;
     return DMOLP_OK;
 }
 

@@ -99,6 +99,8 @@ struct Tuning {
     int grid_blocks = 0;  // 0: derived from the CU count
     double pull_fraction = 0.05;  // pull candidates when B_{t-1} covers this share of entries
     int64_t batch_work_per_block = 32768;  // batched launches: vertices + nnz/4 per block
+    double hist_recount_share = 0.2;  // heavy histograms: recount when changes hold >= this share
+    double hist_push_share = 0.02;    // ... push the changes when they hold <= this share
 };
 
 struct GraphDev {

@@ -37,9 +37,7 @@ constexpr int kCandFirst = 2;         // neighbours per row examined in the join
 // Up to this k every valid label fits the byte mirror exactly (255 stands
 // for -1 and out-of-range labels), so the sweeps gather bytes, not words.
 constexpr int kExact8 = 254;
-// Recount the heavy-row histograms when the changed vertices cover at least
-// this share of the CSR entries; below it, update them from the changes.
-constexpr double kRecountShare = 0.2;
+
 
 __device__ __forceinline__ bool is_done(const Ctl *c) { return c->done != 0; }
 __device__ __forceinline__ bool bit_of(const uint32_t *bits, int32_t v) {
@@ -1018,19 +1016,79 @@ __global__ void k_lp_heavy_hist(const P2Job *JS, const int32_t *bstart, int jobs
     }
 }
 
-// After the apply: recount the heavy histograms when many vertices changed,
-// else update them from the changed neighbours only.
+// After the apply: how the heavy histograms follow this round's changes.
+// Many changes: recount them.  Few: push each changed vertex's move to its
+// heavy neighbours.  Otherwise: scan the heavy rows against the
+// changed-vertex bitmap.
+enum HistMode : int32_t { HIST_SCAN = 0, HIST_RECOUNT = 1, HIST_PUSH = 2 };
+
 __global__ void k_hist_mode(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
-    const unsigned long long chg_deg = J.D.cnt[LC_DEG];
-    ctl->hist_mode = (double)chg_deg >= kRecountShare * (double)J.g.nnz ? 1 : 0;
+    const double chg_deg = (double)J.D.cnt[LC_DEG], nnz = (double)J.g.nnz;
+    ctl->hist_mode = chg_deg >= J.recount_share * nnz ? HIST_RECOUNT
+                     : (chg_deg <= J.push_share * nnz ? HIST_PUSH : HIST_SCAN);
+}
+
+// Push: every entry (v, u) of a changed vertex v whose neighbour u is heavy
+// moves one count of u's histogram from v's old label to its new one.
+__device__ __forceinline__ void push_move(const int32_t *hid, int32_t *hh, int32_t u, uint32_t on,
+                                          int E) {
+    const int32_t h = hid[u];
+    if (h < 0) return;
+    const int32_t o = (int32_t)(on & 0xffu), nw = (int32_t)(on >> 8);
+    if (o < E) atomicAdd(&hh[(int64_t)h * kLpCap + o], -1);
+    if (nw < E) atomicAdd(&hh[(int64_t)h * kLpCap + nw], 1);
+}
+
+__global__ __launch_bounds__(kThreads) void k_hist_push_light(const P2Job *JS, const int32_t *bstart,
+                                                              int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev D = J.D;
+    const uint16_t *ON = J.ON;
+    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_PUSH) return;
+    const int E = J.k < kLpCap ? J.k : kLpCap;
+    __shared__ BatchLds s_b[kWavesPerBlock];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    const int64_t count = (int64_t)D.cnt[LC_LIGHT];
+    for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
+        int32_t v, deg;
+        bool valid;
+        const int32_t total = batch_load(g, D.light, count, b0, s_b[w], v, deg, valid);
+        s_b[w].aux[lane] = valid ? (int32_t)ON[v] : 0;
+        wave_sync();
+        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
+            const int32_t q = q0 + lane;
+            if (q >= total) continue;
+            const int slot = batch_slot(s_b[w], q);
+            push_move(J.hid, J.hh, g.col[s_b[w].base[slot] + q], (uint32_t)s_b[w].aux[slot], E);
+        }
+        wave_sync();
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_hist_push_heavy(const P2Job *JS, const int32_t *bstart,
+                                                              int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const ListDev D = J.D;
+    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_PUSH) return;
+    const int E = J.k < kLpCap ? J.k : kLpCap;
+    const int64_t nch = (int64_t)D.cnt[LC_CHUNKS];
+    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
+        int32_t slot, v;
+        int64_t e0, e1, deg;
+        chunk_bounds(g, D, D.hc[j], J.chunk_edges, slot, v, e0, e1, deg);
+        const uint32_t on = J.ON[v];
+        for (int64_t e = e0 + lane_id(); e < e1; e += kWave) push_move(J.hid, J.hh, g.col[e], on, E);
+    }
 }
 
 __global__ void k_hist_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
-    if (is_done(J.ctl) || J.ctl->hist_mode == 0) return;
+    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_RECOUNT) return;
     const int64_t cnt = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
     for (int64_t i = vthread(vg); i < cnt; i += vthreads(vg)) J.hh[i] = 0;
 }
@@ -1044,7 +1102,7 @@ __global__ __launch_bounds__(kThreads) void k_hist_recount(const P2Job *JS, cons
     const ListDev all = J.all;
     const uint8_t *L8 = J.L8;
     int32_t *hh = J.hh;
-    if (is_done(J.ctl) || J.ctl->hist_mode == 0) return;
+    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_RECOUNT) return;
     const int E = J.k < kLpCap ? J.k : kLpCap;
     const int chunk_edges = J.chunk_edges;
     __shared__ int32_t s_h[kWavesPerBlock][kWave * kHistStride];
@@ -1094,7 +1152,7 @@ __global__ __launch_bounds__(kThreads) void k_hist_update(const P2Job *JS, const
     const uint16_t *ON = J.ON;
     const uint32_t *cbits = J.cbits;
     int32_t *hh = J.hh;
-    if (is_done(J.ctl) || J.ctl->hist_mode != 0) return;
+    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_SCAN) return;
     const int E = J.k < kLpCap ? J.k : kLpCap;
     const int chunk_edges = J.chunk_edges;
     __shared__ int32_t s_w[kWavesPerBlock][kLpCap];
@@ -1362,6 +1420,8 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.max_rounds = max_rounds;
     j.epsilon = epsilon;
     j.pull_frac = ctx->tune.pull_fraction;
+    j.recount_share = ctx->tune.hist_recount_share;
+    j.push_share = ctx->tune.hist_push_share;
     return j;
 }
 
@@ -1476,6 +1536,8 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
         hipLaunchKernelGGL(k_hist_zero, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_recount, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_update, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+        hipLaunchKernelGGL(k_hist_push_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+        hipLaunchKernelGGL(k_hist_push_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     }
     hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m6 = mark();

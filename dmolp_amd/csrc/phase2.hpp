@@ -78,6 +78,7 @@ struct P2Job {
     int32_t epoch_base, max_rounds;
     double epsilon;
     double pull_frac;
+    double recount_share, push_share;  // heavy-histogram modes (see k_hist_mode)
 };
 
 // A set of jobs that advance together, one launch per kernel step.

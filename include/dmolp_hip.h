@@ -102,6 +102,10 @@ typedef struct {
     int32_t pull_per_mille; /* find candidates from the boundary side when B_{t-1}
                                covers at least this many per mille of the CSR entries */
     int32_t batch_work_per_block; /* batched launches: (vertices + entries/4) per workgroup */
+    int32_t hist_recount_per_mille; /* heavy-row histograms: recount after a round whose
+                                       changed vertices hold at least this share of entries */
+    int32_t hist_push_per_mille;    /* ... push the changes when they hold at most this
+                                       share (in 1/1000); between the two, scan a bitmap */
 } dmolp_tuning;
 int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out);
 int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t);

@@ -24,14 +24,23 @@ def c1():
     return api.gen_host(api.rmat_spec(16, 16, seed=1))
 
 
-DEFAULT_TUNING = dict(heavy_degree=1024, chunk_edges=4096, pull_per_mille=50)
+DEFAULT_TUNING = dict(heavy_degree=1024, chunk_edges=4096, pull_per_mille=50,
+                      hist_recount_per_mille=200, hist_push_per_mille=20)
 TUNINGS = {
     "default": DEFAULT_TUNING,
     # Tiny heavy threshold: the chunked heavy-row kernels run on small graphs.
-    "heavy": dict(heavy_degree=5, chunk_edges=64, pull_per_mille=50),
+    "heavy": dict(DEFAULT_TUNING, heavy_degree=5, chunk_edges=64),
     # Candidates always pushed from B_{t-1} / always pulled from the boundary.
-    "push": dict(heavy_degree=1024, chunk_edges=4096, pull_per_mille=1000000),
-    "pull": dict(heavy_degree=5, chunk_edges=64, pull_per_mille=1),
+    "push": dict(DEFAULT_TUNING, pull_per_mille=1000000),
+    "pull": dict(DEFAULT_TUNING, heavy_degree=5, chunk_edges=64, pull_per_mille=1),
+    # Heavy-row histograms always followed by one mode: bitmap scan, push of
+    # the changes, or recount (1001 per mille: never).
+    "hist_scan": dict(DEFAULT_TUNING, heavy_degree=5, chunk_edges=64,
+                      hist_recount_per_mille=1001, hist_push_per_mille=1),
+    "hist_push": dict(DEFAULT_TUNING, heavy_degree=5, chunk_edges=64,
+                      hist_recount_per_mille=1001, hist_push_per_mille=1000),
+    "hist_recount": dict(DEFAULT_TUNING, heavy_degree=5, chunk_edges=64,
+                         hist_recount_per_mille=1, hist_push_per_mille=1),
 }
 
 
