@@ -215,3 +215,24 @@ def test_graph_replays_match_eager(ctx):
         assert np.array_equal(r.labels, want.labels)
         assert r.metrics.edge_cut == want.m.edge_cut and r.metrics.rounds == want.m.rounds
         assert np.array_equal(r.trace["b_count"], want.trace["b_count"])
+
+
+@pytest.mark.parametrize("k", [33, 40, 254, 255, 300, 1024])
+def test_phase2_parity_large_k(tuned, k):
+    """k above the LP's 32-label cap (labels >= 32 always move) and above
+    the byte mirror's exact range (k > 254: sweeps use the int32 labels)."""
+    off, col = api.gen_host(api.rmat_spec(12, 8, seed=21))
+    g = tuned.upload(off, col)
+    rng = np.random.default_rng(k)
+    labels0 = rng.integers(0, k, len(off) - 1).astype(np.int32)
+    _check_phase2(g, off, col, k, labels0, 8, 0.01)
+
+
+@pytest.mark.parametrize("k", [40, 300])
+def test_phase1_then_phase2_large_k(tuned, k):
+    off, col = api.gen_host(api.rmat_spec(11, 16, seed=22))
+    g = tuned.upload(off, col)
+    want1 = O.phase1(off, col, k, 500)
+    got1 = g.phase1(k, 500)
+    assert np.array_equal(want1.labels, got1.labels)
+    _check_phase2(g, off, col, k, want1.labels, 10, 0.03)
