@@ -359,6 +359,9 @@ def main(argv=None):
     tr = pr.trace
     light_n, light_deg = tr["b_light_count"], tr["b_light_deg"]
     heavy_n, heavy_deg = tr["b_count"] - light_n, tr["b_deg"] - light_deg
+    log(f"[bench] per round: |B light| {light_n.tolist()} light entries {light_deg.tolist()} "
+        f"|B heavy| {heavy_n.tolist()} changed {tr['chg_count'].tolist()} "
+        f"changed entries {tr['chg_deg'].tolist()}")
     unit = 2 * O_BYTES + 2 * L_BYTES
     # The heavy rows' LP histograms are kept across rounds: after each round
     # k_hist_update streams every heavy row's columns (4 B per entry) and
