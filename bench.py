@@ -362,6 +362,8 @@ def main(argv=None):
     log(f"[bench] per round: |B light| {light_n.tolist()} light entries {light_deg.tolist()} "
         f"|B heavy| {heavy_n.tolist()} changed {tr['chg_count'].tolist()} "
         f"changed entries {tr['chg_deg'].tolist()}")
+    log("[bench] per round ms: " + json.dumps(
+        {nm: [round(float(x), 3) for x in v] for nm, v in per_round.items()}))
     unit = 2 * O_BYTES + 2 * L_BYTES
     # The heavy rows' LP histograms are kept across rounds: after each round
     # k_hist_update streams every heavy row's columns (4 B per entry) and

@@ -19,7 +19,7 @@ class Tuning(C.Structure):
     _fields_ = [("heavy_degree", i32), ("chunk_edges", i32), ("block_threads", i32),
                 ("grid_blocks", i32), ("pull_per_mille", i32),
                 ("batch_work_per_block", i32), ("hist_recount_per_mille", i32),
-                ("hist_push_per_mille", i32)]
+                ("hist_push_per_mille", i32), ("light_hist", i32)]
 
 
 class GenSpec(C.Structure):

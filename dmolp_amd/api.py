@@ -124,12 +124,13 @@ class Context:
         return t
 
     def set_tuning(self, heavy_degree=0, chunk_edges=0, grid_blocks=0, pull_per_mille=0,
-                   batch_work_per_block=0, hist_recount_per_mille=0, hist_push_per_mille=0):
+                   batch_work_per_block=0, hist_recount_per_mille=0, hist_push_per_mille=0,
+                   light_hist=0):
         t = _abi.Tuning(heavy_degree=heavy_degree, chunk_edges=chunk_edges, block_threads=0,
                         grid_blocks=grid_blocks, pull_per_mille=pull_per_mille,
                         batch_work_per_block=batch_work_per_block,
                         hist_recount_per_mille=hist_recount_per_mille,
-                        hist_push_per_mille=hist_push_per_mille)
+                        hist_push_per_mille=hist_push_per_mille, light_hist=light_hist)
         _check(_abi.lib().dmolp_ctx_set_tuning(self._h, C.byref(t)), self._h)
 
     def set_profiling(self, on: bool):

@@ -245,6 +245,7 @@ int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out) {
     out->batch_work_per_block = (int32_t)ctx->tune.batch_work_per_block;
     out->hist_recount_per_mille = (int32_t)(ctx->tune.hist_recount_share * 1000.0 + 0.5);
     out->hist_push_per_mille = (int32_t)(ctx->tune.hist_push_share * 1000.0 + 0.5);
+    out->light_hist = ctx->tune.light_hist ? 1 : 2;
     return DMOLP_OK;
 }
 
@@ -255,7 +256,7 @@ int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
         t->grid_blocks < 0 || t->pull_per_mille < 0 || t->pull_per_mille > 1000000 ||
         t->batch_work_per_block < 0 || t->hist_recount_per_mille < 0 ||
         t->hist_recount_per_mille > 1001 || t->hist_push_per_mille < 0 ||
-        t->hist_push_per_mille > 1001 ||
+        t->hist_push_per_mille > 1001 || t->light_hist < 0 || t->light_hist > 2 ||
         (t->block_threads != 0 && t->block_threads != 256))
         return fail(ctx, DMOLP_E_ARG, "tuning value out of range");
     if (t->heavy_degree) ctx->tune.heavy_degree = t->heavy_degree;
@@ -267,6 +268,7 @@ int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
     if (t->hist_recount_per_mille > 0)
         ctx->tune.hist_recount_share = t->hist_recount_per_mille / 1000.0;
     if (t->hist_push_per_mille > 0) ctx->tune.hist_push_share = t->hist_push_per_mille / 1000.0;
+    if (t->light_hist) ctx->tune.light_hist = t->light_hist == 1;
     return DMOLP_OK;
 }
 

@@ -70,6 +70,8 @@ P2Set p2_set_build(dmolp_ctx *ctx, std::vector<P2Item *> &items, BatchState &bs,
             it.have_job = true;
         }
         it.job.remaining = bs.remaining.as<int32_t>();
+        DMOLP_REQUIRE(it.job.lh_on == (ctx->tune.light_hist ? 1 : 0), DMOLP_E_ARG,
+                      "light_hist tuning changed while a job was in flight");
         jobs[(size_t)j] = it.job;
         sels[(size_t)j] = phase2_sel_args(ctx, w);
         job_blocks(ctx, it.g->n, it.g->nnz, it.k, it.g->n_heavy, G == 1, blocks[(size_t)j].data());
@@ -91,6 +93,7 @@ P2Set p2_set_build(dmolp_ctx *ctx, std::vector<P2Item *> &items, BatchState &bs,
     set.ls = &bs.ls;
     set.passes = passes;
     set.any_heavy = any_heavy;
+    set.lh_on = ctx->tune.light_hist;
     set.remaining = bs.remaining.as<int32_t>();
     if (key_out) {
         uint64_t key = 0x51ed270b27c1f1f3ull;
@@ -99,6 +102,7 @@ P2Set p2_set_build(dmolp_ctx *ctx, std::vector<P2Item *> &items, BatchState &bs,
         key = mix(key, (uint64_t)G);
         key = mix(key, (uint64_t)passes);
         key = mix(key, any_heavy ? 1 : 0);
+        key = mix(key, set.lh_on ? 1 : 0);
         for (int p = 0; p < P_NPOL; ++p) key = mix(key, (uint64_t)bs.ls.total[p]);
         *key_out = key;
     }

@@ -83,11 +83,14 @@ __global__ void k_bin_all(GraphDev g, ListDev out, int heavy_degree, int chunk_e
     }
 }
 
-__global__ void k_heavy_ids(ListDev all, int32_t *hid) {
+__global__ void k_heavy_ids(ListDev all, int32_t *hid, uint32_t *hbits) {
     const int64_t ns = (int64_t)all.cnt[LC_SLOTS];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns;
-         i += (int64_t)gridDim.x * blockDim.x)
-        hid[all.hv[i]] = (int32_t)i;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = all.hv[i];
+        hid[v] = (int32_t)i;
+        atomicOr(&hbits[v >> 5], 1u << (v & 31));
+    }
 }
 
 __global__ void k_count_heavy(GraphDev g, int heavy_degree, int chunk_edges,
@@ -144,8 +147,10 @@ void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g) {
                        g->all.dev, H, CH);
     g->hid.grow(sizeof(int32_t) * (size_t)(g->n > 0 ? g->n : 1));
     DMOLP_HIP_CHECK(hipMemsetAsync(g->hid.p, 0xff, g->hid.bytes, ctx->stream));
+    g->hbits.grow(sizeof(uint32_t) * (size_t)((g->n + 31) / 32 > 0 ? (g->n + 31) / 32 : 1));
+    DMOLP_HIP_CHECK(hipMemsetAsync(g->hbits.p, 0, g->hbits.bytes, ctx->stream));
     hipLaunchKernelGGL(k_heavy_ids, dim3(ctx->cu_count), dim3(256), 0, ctx->stream, g->all.dev,
-                       g->hid.as<int32_t>());
+                       g->hid.as<int32_t>(), g->hbits.as<uint32_t>());
     DMOLP_HIP_CHECK(hipGetLastError());
     g->all_heavy_degree = H;
     g->all_chunk_edges = CH;

@@ -101,6 +101,7 @@ struct Tuning {
     int64_t batch_work_per_block = 32768;  // batched launches: vertices + nnz/4 per block
     double hist_recount_share = 0.2;  // heavy histograms: recount when changes hold >= this share
     double hist_push_share = 0.02;    // ... push the changes when they hold <= this share
+    bool light_hist = true;  // light rows keep their label counts across rounds
 };
 
 struct GraphDev {
@@ -193,7 +194,8 @@ struct dmolp_graph {
     dmolp::List all;
     int all_heavy_degree = -1, all_chunk_edges = -1;
     int64_t n_heavy = 0, n_heavy_chunks = 0;
-    dmolp::DevMem hid;  // int32[n]: heavy slot of the all list, -1 for light rows
+    dmolp::DevMem hid;    // int32[n]: heavy slot of the all list, -1 for light rows
+    dmolp::DevMem hbits;  // uint32[(n + 31) / 32]: bit v set for the heavy rows
     std::unique_ptr<dmolp::Phase2Work> p2;
     std::unique_ptr<dmolp::Phase1Work> p1;
     dmolp::GraphDev dev() const {

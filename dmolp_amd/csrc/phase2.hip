@@ -28,7 +28,9 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr int kHistStride = kLpCap + 1;  // odd stride: no LDS bank conflicts across slots
-constexpr int kMaxKLds = DMOLP_MAX_K;
+// Per-block label counters (vc, ic) live in LDS for labels below kLdsK;
+// larger labels (k above kLdsK) add to the global counters directly.
+constexpr int kLdsK = 256;
 constexpr int kLpUnroll = 4;     // light rows: gathers in flight per lane
 constexpr int kHeavyUnroll = 8;  // heavy rows: gathers in flight per lane
 constexpr int32_t kLabel8None = 255;  // byte-mirror value of labels the LP cannot see
@@ -37,9 +39,24 @@ constexpr int kCandFirst = 2;         // neighbours per row examined in the join
 // Up to this k every valid label fits the byte mirror exactly (255 stands
 // for -1 and out-of-range labels), so the sweeps gather bytes, not words.
 constexpr int kExact8 = 254;
+// Light rows count in 16-bit halves of 32-bit words (a light row has at most
+// 65535 entries): label l in word l / 2, half l % 2.  LDS rows of a wave are
+// kPackedStride words apart; kept rows in HBM are kPackedWords (64 bytes).
+constexpr int kPackedWords = kLpCap / 2;
+constexpr int kPackedStride = kPackedWords + 1;  // odd: rows of a wave fall in different banks
 
 
 __device__ __forceinline__ bool is_done(const Ctl *c) { return c->done != 0; }
+__device__ __forceinline__ void label_add(unsigned long long *lds, unsigned long long *glob, int32_t l,
+                                          unsigned long long x) {
+    atomicAdd(l < kLdsK ? &lds[l] : &glob[l], x);
+}
+__device__ __forceinline__ void label_flush(const unsigned long long *lds, unsigned long long *glob,
+                                            int k) {
+    const int kk = k < kLdsK ? k : kLdsK;
+    for (int i = threadIdx.x; i < kk; i += blockDim.x)
+        if (lds[i]) atomicAdd(&glob[i], lds[i]);
+}
 __device__ __forceinline__ bool bit_of(const uint32_t *bits, int32_t v) {
     return (bits[v >> 5] >> (v & 31)) & 1u;
 }
@@ -78,6 +95,8 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.cbits = gload(v.cbits);
     r.ON = gload(v.ON);
     r.N8 = gload(v.N8);
+    r.lh = gload(v.lh);
+    r.hbits = gload(v.hbits);
     r.bd = gload(v.bd);
     r.ctl = gload(v.ctl);
     r.vc = gload(v.vc);
@@ -100,7 +119,8 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
 // ---------------------------------------------------------------- init sweep
 
 // phase2.cpp:55-117 and 214-253 over all vertices, plus the per-vertex count
-// of differing neighbours used as the boundary test.
+// of differing neighbours used as the boundary test, and (lh_on) each light
+// row's label counts for the kept histograms.
 __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
@@ -113,12 +133,18 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
     Ctl *ctl = J.ctl;
     unsigned long long *vc = J.vc;
     unsigned long long *ic = J.ic;
+    const bool lh_on = J.lh_on != 0;
+    uint32_t *lh = J.lh;
+    const int E = k < kLpCap ? k : kLpCap;
+    const int nvec = (E + 7) / 8;  // 16-byte vectors of a row that hold labels below E
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ int32_t s_cnt[kWavesPerBlock][kWave];
-    __shared__ unsigned long long s_vc[kMaxKLds], s_ic[kMaxKLds];
-    for (int i = threadIdx.x; i < k; i += blockDim.x) { s_vc[i] = 0; s_ic[i] = 0; }
+    __shared__ uint32_t s_h[kWavesPerBlock][kWave * kPackedStride];
+    __shared__ unsigned long long s_vc[kLdsK], s_ic[kLdsK];
+    for (int i = threadIdx.x; i < kLdsK; i += blockDim.x) { s_vc[i] = 0; s_ic[i] = 0; }
     __syncthreads();
     const int w = threadIdx.x / kWave, lane = lane_id();
+    uint32_t *h = s_h[w];
     int64_t invalid = 0, cut2 = 0;
     const int64_t count = (int64_t)all.cnt[LC_LIGHT];
     for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
@@ -126,6 +152,8 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
         bool valid;
         const int32_t total = batch_load(g, all.light, count, b0, s_b[w], v, deg, valid);
         s_cnt[w][lane] = 0;
+        if (lh_on)
+            for (int l = 0; l < 4 * nvec; ++l) h[lane * kPackedStride + l] = 0u;
         wave_sync();
         for (int32_t q0 = 0; q0 < total; q0 += kWave) {
             const int32_t q = q0 + lane;
@@ -133,14 +161,17 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
                 const int slot = batch_slot(s_b[w], q);
                 const int64_t e = s_b[w].base[slot] + q;
                 const int32_t vv = s_b[w].v[slot];
-                const int32_t lv = L[vv], lu = nb_label(exact8, L8, L, g.col[e]);
+                const int32_t u = g.col[e];
+                const int32_t b = L8[u];
+                const int32_t lv = L[vv], lu = exact8 ? (b == kLabel8None ? -1 : b) : L[u];
+                if (lh_on && b < E) atomicAdd(&h[slot * kPackedStride + (b >> 1)], 1u << ((b & 1) * 16));
                 if (lv == -1 || lu == -1) {
                     ++invalid;
                 } else if (lv != lu) {
                     ++cut2;
                     atomicAdd(&s_cnt[w][slot], 1);
                 } else if (lv >= 0 && lv < k) {
-                    atomicAdd(&s_ic[lv], 1ull);
+                    label_add(s_ic, ic, lv, 1ull);
                 }
             }
         }
@@ -148,18 +179,22 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
         if (valid) {
             diff[v] = s_cnt[w][lane];
             const int32_t lv = L[v];
-            if (lv >= 0 && lv < k) atomicAdd(&s_vc[lv], 1ull);
+            if (lv >= 0 && lv < k) label_add(s_vc, vc, lv, 1ull);
             else if (lv != -1) atomicAdd(&ctl->v[C_KERNEL_ERR], 1ull);
+            if (lh_on) {
+                const uint32_t *r = &h[lane * kPackedStride];
+                uint4 *dst = reinterpret_cast<uint4 *>(lh + (int64_t)v * kPackedWords);
+                for (int q = 0; q < nvec; ++q)
+                    dst[q] = make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+            }
         }
         wave_sync();
     }
     wave_add(&ctl->v[C_INVALID], invalid);
     wave_add(&ctl->v[C_CUT2], cut2);
     __syncthreads();
-    for (int i = threadIdx.x; i < k; i += blockDim.x) {
-        if (s_vc[i]) atomicAdd(&vc[i], s_vc[i]);
-        if (s_ic[i]) atomicAdd(&ic[i], s_ic[i]);
-    }
+    label_flush(s_vc, vc, k);
+    label_flush(s_ic, ic, k);
 }
 
 __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
@@ -245,6 +280,9 @@ __global__ void k_init_finish(const P2Job *JS, const int32_t *bstart, int jobs) 
         ctl->v[C_CUT] = cut2 >> 1;
         ctl->v[C_PREV_CUT] = cut2 >> 1;
         ctl->v[C_INIT_CUT] = cut2 >> 1;
+        // Every label is valid from here on, so the sweep's differing counts
+        // are exact; Delta keeps them exact after each round.
+        ctl->diff_ok = 1;
         if (ctl->done) atomicSub(J.remaining, 1);
     }
 }
@@ -370,9 +408,24 @@ __device__ __forceinline__ bool differs_in(const GraphDev &g, int32_t lv, int64_
     return false;
 }
 
+// Boundary test of v (label lv, row [s, e)) in rounds after the first, once
+// the initial sweep has run (ctl->diff_ok): a light row's differing count is
+// kept exact by Delta; a heavy row counts its own label in its kept
+// histogram when every label is counted (k <= 32); otherwise the row is
+// scanned.  Heavy rows take no differing-count updates: thousands of
+// changed neighbours would all add to the same word.
+__device__ __forceinline__ bool on_boundary(const P2Job &J, bool diff_ok, int32_t v, int32_t lv,
+                                            int64_t s, int64_t e, bool exact8) {
+    if (diff_ok) {
+        if (!bit_of(J.hbits, v)) return J.diff[v] > 0;
+        if (J.k <= kLpCap) return (int64_t)J.hh[(int64_t)J.hid[v] * kLpCap + lv] < e - s;
+    }
+    return differs_in(J.g, lv, s, e, exact8, J.L8, J.L);
+}
+
 // First touch of u in this pass; a touched boundary vertex becomes a record.
-__device__ __forceinline__ void touch(const GraphDev &g, bool pred, int32_t u, int32_t *stamp,
-                                      int32_t epoch, const int32_t *L, const uint8_t *L8,
+__device__ __forceinline__ void touch(const P2Job &J, bool diff_ok, bool pred, int32_t u,
+                                      int32_t *stamp, int32_t epoch, const int32_t *L,
                                       bool exact8, const RecDev &bd, int64_t &cand) {
     bool first = false;
     if (pred && stamp[u] != epoch) first = atomicExch(&stamp[u], epoch) != epoch;
@@ -381,10 +434,10 @@ __device__ __forceinline__ void touch(const GraphDev &g, bool pred, int32_t u, i
     int64_t s = 0, e = 0;
     if (first) {
         lu = L[u];
-        s = g.off[u];
-        e = g.off[u + 1];
+        s = J.g.off[u];
+        e = J.g.off[u + 1];
     }
-    const bool rec = first && differs_in(g, lu, s, e, exact8, L8, L);
+    const bool rec = first && on_boundary(J, diff_ok, u, lu, s, e, exact8);
     const int64_t pos = wave_append(rec, bd.cnt);
     if (rec) {
         bd.x[pos] = u;
@@ -410,10 +463,14 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
     Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
     const uint32_t *inb = J.bbits + (int64_t)((ctl->t - 1) & 1) * J.bwords;  // B_{t-1}
+    // Once the differing counts are kept (see on_boundary), interior vertices
+    // are skipped outright and boundary rows are scanned for B_{t-1} only.
+    const bool use_diff = ctl->diff_ok != 0;
+    const int32_t *diff = J.diff;
     __shared__ BlockAppendLds s_app;
     const int64_t span = (int64_t)blockDim.x * kScanItems;
     for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
-        uint32_t rec = 0;
+        uint32_t rec = 0, live = 0;
         int32_t nr = 0, deg[kScanItems], lab[kScanItems];
         int64_t xs[kScanItems], lim[kScanItems], end[kScanItems];
         bool hit[kScanItems], dif[kScanItems];
@@ -430,11 +487,20 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
             hit[j] = true;
             dif[j] = true;
             if (v >= g.n) continue;
+            const bool heavy_v = use_diff && bit_of(J.hbits, (int32_t)v);
+            if (use_diff && !heavy_v && diff[v] == 0) continue;  // light interior: never a record
             const int64_t s = g.off[v], e = g.off[v + 1];
+            const int32_t lv = L[v];
+            bool known = use_diff && !heavy_v;
+            if (heavy_v && J.k <= kLpCap) {  // every label counted in the kept histogram
+                if ((int64_t)J.hh[(int64_t)J.hid[v] * kLpCap + lv] == e - s) continue;
+                known = true;
+            }
+            live |= 1u << j;
             deg[j] = (int32_t)(e - s);
-            lab[j] = L[v];
+            lab[j] = lv;
             hit[j] = bit_of(inb, (int32_t)v);
-            dif[j] = false;
+            dif[j] = known;
             xs[j] = s;
             end[j] = e;
             lim[j] = e - s > heavy_degree ? s + heavy_degree : e;
@@ -461,7 +527,7 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
                 if (!hit[j]) hit[j] = bit_of(inb, u);
             }
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
-            if (v >= g.n) continue;
+            if (!((live >> j) & 1u)) continue;
             // A long row still undecided after heavy_degree entries is
             // finished by k_cand_pull_heavy (rare).
             if (!(hit[j] && dif[j]) && lim[j] < end[j]) {
@@ -532,6 +598,7 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(const P2Job *JS, const 
     const RecDev bd = J.bd;
     Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PUSH) return;
+    const bool diff_ok = ctl->diff_ok != 0;
     const int32_t epoch = ctl->epoch;
     __shared__ BatchLds s_b[kWavesPerBlock];
     const int w = threadIdx.x / kWave, lane = lane_id();
@@ -541,13 +608,13 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(const P2Job *JS, const 
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, prev.light, count, b0, s_b[w], v, deg, valid);
-        touch(g, valid, v, stamp, epoch, L, L8, exact8, bd, cand);
+        touch(J, diff_ok, valid, v, stamp, epoch, L, exact8, bd, cand);
         for (int32_t q0 = 0; q0 < total; q0 += kWave) {
             const int32_t q = q0 + lane;
             const bool on = q < total;
             int32_t u = 0;
             if (on) u = g.col[s_b[w].base[batch_slot(s_b[w], q)] + q];
-            touch(g, on, u, stamp, epoch, L, L8, exact8, bd, cand);
+            touch(J, diff_ok, on, u, stamp, epoch, L, exact8, bd, cand);
         }
         wave_sync();
     }
@@ -566,6 +633,7 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
     const RecDev bd = J.bd;
     Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PUSH) return;
+    const bool diff_ok = ctl->diff_ok != 0;
     const int32_t epoch = ctl->epoch;
     const int lane = lane_id();
     int64_t cand = 0;
@@ -575,11 +643,12 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
         int64_t e0, e1, deg;
         const int64_t item = prev.hc[j];
         chunk_bounds(g, prev, item, chunk_edges, slot, v, e0, e1, deg);
-        touch(g, lane == 0 && (item & 0xffffffffll) == 0, v, stamp, epoch, L, L8, exact8, bd, cand);
+        touch(J, diff_ok, lane == 0 && (item & 0xffffffffll) == 0, v, stamp, epoch, L, exact8, bd,
+              cand);
         for (int64_t eb = e0; eb < e1; eb += kWave) {
             const int64_t e = eb + lane;
             const bool on = e < e1;
-            touch(g, on, on ? g.col[e] : 0, stamp, epoch, L, L8, exact8, bd, cand);
+            touch(J, diff_ok, on, on ? g.col[e] : 0, stamp, epoch, L, exact8, bd, cand);
         }
     }
     wave_add(&ctl->v[C_CAND], cand);
@@ -612,10 +681,6 @@ __device__ __forceinline__ int32_t lp_decide(const int32_t *h, int stride, int E
     return lp_decide_by([&](int l) { return h[l * stride]; }, E, pen, own);
 }
 
-// Light rows count in 16-bit halves of 32-bit LDS words (a light row has at
-// most 65535 entries): label l in word l / 2, half l % 2.
-constexpr int kPackedWords = kLpCap / 2;
-constexpr int kPackedStride = kPackedWords + 1;  // odd: rows of a wave fall in different banks
 
 __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
@@ -687,6 +752,109 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
             }
         }
         block_list_append(D, changed, v, deg, heavy_degree, chunk_edges, s_app);
+    }
+}
+
+// lp_decide_by over a fixed trip count (labels at or above E are skipped),
+// so a row held in registers is indexed with constants only.
+template <class Count>
+__device__ __forceinline__ int32_t lp_decide_unrolled(Count count, int E, const double *pen,
+                                                      int32_t own) {
+    double base = -1e300;
+    int best_l = 0;
+    double best_s = 0.0;
+#pragma unroll
+    for (int l = 0; l < kLpCap; ++l) {
+        if (l < E) {
+            double s = (double)count(l);
+            if (l == own) base = s;
+            if (s > 0.0) s *= (1.0 + pen[l]);
+            if (l == 0 || s > best_s || (s == best_s && bitrev5(l) < bitrev5(best_l))) {
+                best_l = l;
+                best_s = s;
+            }
+        }
+    }
+    return (best_s > base) ? best_l : own;
+}
+
+// Light rows of B_t decide from their kept histograms: one lane per row and
+// one 64-byte row read (32 bytes or less for k <= 16) in place of a column
+// read and a label gather per entry.  B_t is taken from its bitmap in vertex
+// order, so the rows (and labels) are read as ascending addresses, which
+// keeps the DRAM pages open: the rows table is far larger than the caches.
+// Each wave takes 64 consecutive vertices per item; groups without a light
+// B_t member cost two bitmap words.
+__global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, const int32_t *bstart,
+                                                            int jobs) {
+    P2_JOB();
+    const int64_t n = J.g.n;
+    const int64_t *off = J.g.off;
+    const int32_t *L = J.L;
+    const uint32_t *lh = J.lh;
+    const uint32_t *hbits = J.hbits;
+    const double *pen = J.pen;
+    const int k = J.k;
+    int32_t *chg = J.chg;
+    uint8_t *N8 = J.N8;
+    const ListDev D = J.D;
+    const int64_t bwords = J.bwords;
+    const Ctl *ctl = J.ctl;
+    if (is_done(ctl)) return;
+    const uint32_t *bt = J.bbits + (int64_t)(ctl->t & 1) * bwords;  // B_t
+    __shared__ double s_pen[kLpCap];
+    __shared__ BlockAppendLds s_app;
+    const int E = k < kLpCap ? k : kLpCap;
+    const int nvec = (E + 7) / 8;
+    if ((int)threadIdx.x < E) s_pen[threadIdx.x] = pen[threadIdx.x];
+    __syncthreads();
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    const int64_t span = (int64_t)blockDim.x * kScanItems;
+    // Block-uniform loop, so the Delta append can be aggregated per block.
+    for (int64_t b0 = vg.block * span; b0 < n; b0 += vg.blocks * span) {
+        uint32_t cm = 0;
+        int64_t dsum = 0;
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int64_t gbase = b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave;
+            uint64_t m = 0;
+            if (gbase < n) {
+                const int64_t wi = gbase >> 5;  // gbase is a multiple of 64
+                uint64_t bw = bt[wi], hw = hbits[wi];
+                if (wi + 1 < bwords) {
+                    bw |= (uint64_t)bt[wi + 1] << 32;
+                    hw |= (uint64_t)hbits[wi + 1] << 32;
+                }
+                m = bw & ~hw;
+            }
+            if (!((m >> lane) & 1ull)) continue;
+            const int32_t v = (int32_t)(gbase + lane);
+            const uint4 *row = reinterpret_cast<const uint4 *>(lh + (int64_t)v * kPackedWords);
+            uint32_t c[kPackedWords];
+#pragma unroll
+            for (int q = 0; q < kPackedWords / 4; ++q) {
+                const uint4 x = q < nvec ? row[q] : make_uint4(0u, 0u, 0u, 0u);
+                c[4 * q] = x.x;
+                c[4 * q + 1] = x.y;
+                c[4 * q + 2] = x.z;
+                c[4 * q + 3] = x.w;
+            }
+            const int32_t own = L[v];
+            const int32_t nl = lp_decide_unrolled(
+                [&](int l) { return (int32_t)((c[l >> 1] >> ((l & 1) * 16)) & 0xffffu); }, E, s_pen, own);
+            if (nl != own) {
+                cm |= 1u << j;
+                chg[v] = nl;
+                N8[v] = (uint8_t)(nl < kLabel8None ? nl : kLabel8None);
+                dsum += off[v + 1] - off[v];
+            }
+        }
+        int64_t pos = block_reserve(__popc(cm), &D.cnt[LC_LIGHT], s_app, dsum, &D.cnt[LC_DEG]);
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            if (!(cm & (1u << j))) continue;
+            D.light[pos++] = (int32_t)(b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave + lane);
+        }
     }
 }
 
@@ -787,30 +955,82 @@ __global__ void k_lp_heavy_decide(const P2Job *JS, const int32_t *bstart, int jo
 // ---------------------------------------------------------------- delta stats
 
 
-// The same pair update, returning its effect instead of applying the
-// intra-partition counts: bit 0 the old pair was intra (a == lu), bit 16 the
-// new one is (c == lu2); dd through `dd_out`.  Callers fold the counts once
-// per vertex, since a and c are the vertex's own old and new labels.
+// One entry (v, u) of a changed vertex v (label a -> c).  Returns the
+// entry's intra-partition effect: bit 0 the old pair was intra (a == lu),
+// bit 16 the new one is (c == lu2); the cut change through `dd_out`.  An
+// entry whose ends both changed is counted from the smaller id only.
+// Callers fold the counts once per vertex, since a and c are the vertex's
+// own old and new labels.  `same_out` tells whether u's new label is c (every
+// entry: v's new differing count is its degree minus these).  When u keeps
+// its label, its differing count moves here (keep_diff).
 __device__ __forceinline__ int32_t pair_effect(int32_t v, int32_t a, int32_t c, int32_t u,
                                                const int32_t *L, const int32_t *chg,
                                                const uint8_t *L8, const uint8_t *N8, bool exact8,
-                                               int32_t &dd_out) {
+                                               bool keep_diff, int32_t *diff, int32_t &dd_out,
+                                               bool &same_out) {
     dd_out = 0;
     int32_t lu, lu2;
+    bool uchg;
     if (exact8) {  // byte gathers: old label and this round's new label (255: unchanged)
         const int32_t nu = N8[u];
-        if (nu != kLabel8None && u < v) return 0;  // counted from u's side
+        uchg = nu != kLabel8None;
+        if (uchg) {
+            same_out = nu == c;
+            if (u < v) return 0;  // counted from u's side
+        }
         lu = L8[u];
-        lu2 = nu != kLabel8None ? nu : lu;
+        lu2 = uchg ? nu : lu;
     } else {
         const int32_t cu = chg[u];
-        if (cu >= 0 && u < v) return 0;  // counted from u's side
+        uchg = cu >= 0;
+        if (uchg) {
+            same_out = cu == c;
+            if (u < v) return 0;  // counted from u's side
+        }
         lu = L[u];
-        lu2 = cu >= 0 ? cu : lu;
+        lu2 = uchg ? cu : lu;
+    }
+    if (!uchg) {
+        same_out = lu == c;
+        const int32_t du = (c != lu ? 1 : 0) - (a != lu ? 1 : 0);
+        if (keep_diff && du) atomicAdd(&diff[u], du);
     }
     const int32_t dd = (c != lu2 ? 1 : 0) - (a != lu ? 1 : 0);
     dd_out = dd;
     return (a == lu ? 1 : 0) | (c == lu2 ? 1 << 16 : 0);
+}
+
+// Row upkeep of the kept histograms, done while Delta's rows are read (the
+// round's decisions are final, and the histograms are next read by the next
+// round).  A changed vertex v (label a -> c) appears once in the row of each
+// neighbour u (the CSR is symmetric), so it moves one count of u's row from
+// a to c: u's kept light row (lh_on) or, in push mode, u's heavy histogram.
+// Labels outside [0, E) are not counted.  The 16-bit halves of a light row
+// stay within [0, 65535], so one 32-bit add moves both halves of a word.
+__device__ __forceinline__ void lh_move(uint32_t *lh, int32_t u, int32_t a, int32_t c, int E) {
+    uint32_t *row = lh + (int64_t)u * kPackedWords;
+    const bool da = a >= 0 && a < E, dc = c >= 0 && c < E;
+    const uint32_t ma = da ? 1u << ((a & 1) * 16) : 0u, mc = dc ? 1u << ((c & 1) * 16) : 0u;
+    if (da && dc && (a >> 1) == (c >> 1)) {
+        atomicAdd(&row[a >> 1], mc - ma);
+        return;
+    }
+    if (da) atomicAdd(&row[a >> 1], 0u - ma);
+    if (dc) atomicAdd(&row[c >> 1], mc);
+}
+__device__ __forceinline__ void hh_move(const int32_t *hid, int32_t *hh, int32_t u, int32_t a, int32_t c,
+                                        int E) {
+    int32_t *row = hh + (int64_t)hid[u] * kLpCap;
+    if (a >= 0 && a < E) atomicAdd(&row[a], -1);
+    if (c >= 0 && c < E) atomicAdd(&row[c], 1);
+}
+__device__ __forceinline__ void row_upkeep(const P2Job &J, bool lh_on, bool hh_push, bool u_heavy,
+                                           int32_t u, int32_t a, int32_t c, int E) {
+    if (u_heavy) {
+        if (hh_push) hh_move(J.hid, J.hh, u, a, c, E);
+    } else if (lh_on) {
+        lh_move(J.lh, u, a, c, E);
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const int32_t *bstart, int jobs) {
@@ -826,10 +1046,17 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
     unsigned long long *vc = J.vc;
     unsigned long long *ic = J.ic;
     if (is_done(ctl)) return;
+    const bool lh_on = J.lh_on != 0, hh_push = ctl->hist_mode == HIST_PUSH;
+    const bool upkeep = lh_on || hh_push;
+    const bool keep_diff = ctl->diff_ok != 0;
+    int32_t *diff = J.diff;
+    const int E = k < kLpCap ? k : kLpCap;
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ int32_t s_io[kWavesPerBlock][kWave];
-    __shared__ unsigned long long s_vc[kMaxKLds], s_ic[kMaxKLds];
-    for (int i = threadIdx.x; i < k; i += blockDim.x) { s_vc[i] = 0; s_ic[i] = 0; }
+    __shared__ int32_t s_new[kWavesPerBlock][kWave];
+    __shared__ int32_t s_same[kWavesPerBlock][kWave];
+    __shared__ unsigned long long s_vc[kLdsK], s_ic[kLdsK];
+    for (int i = threadIdx.x; i < kLdsK; i += blockDim.x) { s_vc[i] = 0; s_ic[i] = 0; }
     __syncthreads();
     const int w = threadIdx.x / kWave, lane = lane_id();
     int64_t cut_delta = 0;
@@ -839,7 +1066,9 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
         bool valid;
         const int32_t total = batch_load(g, D.light, count, b0, s_b[w], v, deg, valid);
         s_io[w][lane] = 0;
+        s_same[w][lane] = 0;
         s_b[w].aux[lane] = valid ? L[v] : 0;  // old label of the slot's vertex
+        s_new[w][lane] = valid ? chg[v] : 0;  // its new label
         wave_sync();
         for (int32_t q0 = 0; q0 < total; q0 += kWave * kLpUnroll) {
             int slot[kLpUnroll];
@@ -854,34 +1083,38 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
             for (int j = 0; j < kLpUnroll; ++j) {
                 if (slot[j] < 0) continue;
                 const int32_t vv = s_b[w].v[slot[j]];
+                const int32_t a = s_b[w].aux[slot[j]], c = s_new[w][slot[j]];
                 int32_t dd;
-                const int32_t io = pair_effect(vv, s_b[w].aux[slot[j]], chg[vv], u[j], L, chg, L8, N8,
-                                               exact8, dd);
+                bool same;
+                const bool uh = bit_of(J.hbits, u[j]);
+                const int32_t io = pair_effect(vv, a, c, u[j], L, chg, L8, N8, exact8, keep_diff && !uh,
+                                               diff, dd, same);
                 cut_delta += dd;
                 if (io) atomicAdd(&s_io[w][slot[j]], io);
+                if (same) atomicAdd(&s_same[w][slot[j]], 1);
+                if (upkeep) row_upkeep(J, lh_on, hh_push, uh, u[j], a, c, E);
             }
         }
         wave_sync();
         if (valid) {
             const int32_t a = L[v], c = chg[v];
             const int32_t io = s_io[w][lane];
+            if (keep_diff) diff[v] = deg - s_same[w][lane];
             if (a >= 0 && a < k) {
-                atomicAdd(&s_vc[a], ~0ull);
-                if (io & 0xffff) atomicAdd(&s_ic[a], (unsigned long long)(-(int64_t)(io & 0xffff)));
+                label_add(s_vc, vc, a, ~0ull);
+                if (io & 0xffff) label_add(s_ic, ic, a, (unsigned long long)(-(int64_t)(io & 0xffff)));
             }
             if (c >= 0 && c < k) {
-                atomicAdd(&s_vc[c], 1ull);
-                if (io >> 16) atomicAdd(&s_ic[c], (unsigned long long)(io >> 16));
+                label_add(s_vc, vc, c, 1ull);
+                if (io >> 16) label_add(s_ic, ic, c, (unsigned long long)(io >> 16));
             }
         }
         wave_sync();
     }
     wave_add(&ctl->v[C_CUT], cut_delta);
     __syncthreads();
-    for (int i = threadIdx.x; i < k; i += blockDim.x) {
-        if (s_vc[i]) atomicAdd(&vc[i], s_vc[i]);
-        if (s_ic[i]) atomicAdd(&ic[i], s_ic[i]);
-    }
+    label_flush(s_vc, vc, k);
+    label_flush(s_ic, ic, k);
 }
 
 __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
@@ -898,8 +1131,13 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
     unsigned long long *vc = J.vc;
     unsigned long long *ic = J.ic;
     if (is_done(ctl)) return;
-    __shared__ unsigned long long s_ic[kMaxKLds];
-    for (int i = threadIdx.x; i < k; i += blockDim.x) s_ic[i] = 0;
+    const bool lh_on = J.lh_on != 0, hh_push = ctl->hist_mode == HIST_PUSH;
+    const bool upkeep = lh_on || hh_push;
+    const bool keep_diff = ctl->diff_ok != 0;
+    int32_t *diff = J.diff;
+    const int E = k < kLpCap ? k : kLpCap;
+    __shared__ unsigned long long s_ic[kLdsK];
+    for (int i = threadIdx.x; i < kLdsK; i += blockDim.x) s_ic[i] = 0;
     __syncthreads();
     const int lane = lane_id();
     int64_t cut_delta = 0;
@@ -913,7 +1151,12 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
         int64_t dv = 0, oi = 0, ni = 0;
         for (int64_t e = e0 + lane; e < e1; e += kWave) {
             int32_t dd;
-            const int32_t io = pair_effect(v, a, c, g.col[e], L, chg, L8, N8, exact8, dd);
+            bool same;
+            const int32_t u = g.col[e];
+            const bool uh = bit_of(J.hbits, u);
+            const int32_t io = pair_effect(v, a, c, u, L, chg, L8, N8, exact8, keep_diff && !uh, diff,
+                                           dd, same);
+            if (upkeep) row_upkeep(J, lh_on, hh_push, uh, u, a, c, E);
             dv += dd;
             oi += io & 0xffff;
             ni += io >> 16;
@@ -923,8 +1166,8 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
         ni = wave_sum(ni);
         cut_delta += lane == 0 ? dv : 0;
         if (lane == 0) {
-            if (a >= 0 && a < k && oi) atomicAdd(&s_ic[a], (unsigned long long)(-oi));
-            if (c >= 0 && c < k && ni) atomicAdd(&s_ic[c], (unsigned long long)ni);
+            if (a >= 0 && a < k && oi) label_add(s_ic, ic, a, (unsigned long long)(-oi));
+            if (c >= 0 && c < k && ni) label_add(s_ic, ic, c, (unsigned long long)ni);
             if ((item & 0xffffffffll) == 0) {
                 if (a >= 0 && a < k) atomicAdd(&vc[a], ~0ull);
                 if (c >= 0 && c < k) atomicAdd(&vc[c], 1ull);
@@ -933,8 +1176,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
     }
     wave_add(&ctl->v[C_CUT], cut_delta);
     __syncthreads();
-    for (int i = threadIdx.x; i < k; i += blockDim.x)
-        if (s_ic[i]) atomicAdd(&ic[i], s_ic[i]);
+    label_flush(s_ic, ic, k);
 }
 
 // Byte mirror of the labels: the label if it fits a byte, else kLabel8None.
@@ -1020,7 +1262,6 @@ __global__ void k_lp_heavy_hist(const P2Job *JS, const int32_t *bstart, int jobs
 // Many changes: recount them.  Few: push each changed vertex's move to its
 // heavy neighbours.  Otherwise: scan the heavy rows against the
 // changed-vertex bitmap.
-enum HistMode : int32_t { HIST_SCAN = 0, HIST_RECOUNT = 1, HIST_PUSH = 2 };
 
 __global__ void k_hist_mode(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
@@ -1029,61 +1270,6 @@ __global__ void k_hist_mode(const P2Job *JS, const int32_t *bstart, int jobs) {
     const double chg_deg = (double)J.D.cnt[LC_DEG], nnz = (double)J.g.nnz;
     ctl->hist_mode = chg_deg >= J.recount_share * nnz ? HIST_RECOUNT
                      : (chg_deg <= J.push_share * nnz ? HIST_PUSH : HIST_SCAN);
-}
-
-// Push: every entry (v, u) of a changed vertex v whose neighbour u is heavy
-// moves one count of u's histogram from v's old label to its new one.
-__device__ __forceinline__ void push_move(const int32_t *hid, int32_t *hh, int32_t u, uint32_t on,
-                                          int E) {
-    const int32_t h = hid[u];
-    if (h < 0) return;
-    const int32_t o = (int32_t)(on & 0xffu), nw = (int32_t)(on >> 8);
-    if (o < E) atomicAdd(&hh[(int64_t)h * kLpCap + o], -1);
-    if (nw < E) atomicAdd(&hh[(int64_t)h * kLpCap + nw], 1);
-}
-
-__global__ __launch_bounds__(kThreads) void k_hist_push_light(const P2Job *JS, const int32_t *bstart,
-                                                              int jobs) {
-    P2_JOB();
-    const GraphDev g = J.g;
-    const ListDev D = J.D;
-    const uint16_t *ON = J.ON;
-    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_PUSH) return;
-    const int E = J.k < kLpCap ? J.k : kLpCap;
-    __shared__ BatchLds s_b[kWavesPerBlock];
-    const int w = threadIdx.x / kWave, lane = lane_id();
-    const int64_t count = (int64_t)D.cnt[LC_LIGHT];
-    for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
-        int32_t v, deg;
-        bool valid;
-        const int32_t total = batch_load(g, D.light, count, b0, s_b[w], v, deg, valid);
-        s_b[w].aux[lane] = valid ? (int32_t)ON[v] : 0;
-        wave_sync();
-        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
-            const int32_t q = q0 + lane;
-            if (q >= total) continue;
-            const int slot = batch_slot(s_b[w], q);
-            push_move(J.hid, J.hh, g.col[s_b[w].base[slot] + q], (uint32_t)s_b[w].aux[slot], E);
-        }
-        wave_sync();
-    }
-}
-
-__global__ __launch_bounds__(kThreads) void k_hist_push_heavy(const P2Job *JS, const int32_t *bstart,
-                                                              int jobs) {
-    P2_JOB();
-    const GraphDev g = J.g;
-    const ListDev D = J.D;
-    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_PUSH) return;
-    const int E = J.k < kLpCap ? J.k : kLpCap;
-    const int64_t nch = (int64_t)D.cnt[LC_CHUNKS];
-    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
-        int32_t slot, v;
-        int64_t e0, e1, deg;
-        chunk_bounds(g, D, D.hc[j], J.chunk_edges, slot, v, e0, e1, deg);
-        const uint32_t on = J.ON[v];
-        for (int64_t e = e0 + lane_id(); e < e1; e += kWave) push_move(J.hid, J.hh, g.col[e], on, E);
-    }
 }
 
 __global__ void k_hist_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
@@ -1360,6 +1546,7 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.pen.alloc(sizeof(double) * (size_t)k);
         w.re.alloc(sizeof(double) * (size_t)k);
     }
+    if (ctx->tune.light_hist) w.lh.grow(sizeof(uint32_t) * kPackedWords * (size_t)(n > 0 ? n : 1));
     w.sel.prepare(ctx, k, n, g->max_degree);
     return w;
 }
@@ -1399,6 +1586,9 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.cbits = w.cbits.as<uint32_t>();
     j.ON = w.ON.as<uint16_t>();
     j.N8 = w.N8.as<uint8_t>();
+    j.lh_on = ctx->tune.light_hist ? 1 : 0;
+    j.lh = j.lh_on ? w.lh.as<uint32_t>() : nullptr;
+    j.hbits = g->hbits.as<uint32_t>();
     j.deferred = w.deferred.as<int64_t>();
     j.hub_hist = w.hub_hist.as<int32_t>();
     j.bd = w.bd.dev;
@@ -1459,6 +1649,7 @@ P2Set phase2_single_set(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_r
     set.ls = &w.ls;
     set.passes = w.sel.passes;
     set.any_heavy = g->n_heavy > 0;
+    set.lh_on = ctx->tune.light_hist;
     set.remaining = w.remaining.as<int32_t>();
     return set;
 }
@@ -1520,24 +1711,28 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     select_enqueue(ctx, set.sel, ls, set.passes);
     hipLaunchKernelGGL(k_pack_bbits, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m2 = mark();
-    hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    if (set.lh_on)
+        hipLaunchKernelGGL(k_lp_light_kept, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    else
+        hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m3 = mark();
-    if (set.any_heavy)  // heavy rows decide from their kept histograms
+    if (set.any_heavy) {  // heavy rows decide from their kept histograms
         hipLaunchKernelGGL(k_lp_heavy_hist, dim3(ls.total[P_HEAVY]), dim3(kThreads), 0, s, jobs,
                            ls.bs(P_HEAVY), J);
+        // How the heavy histograms follow this round's changes (push mode
+        // moves their counts in the Delta pass).
+        hipLaunchKernelGGL(k_hist_mode, dim3(ls.total[P_ONE]), dim3(1), 0, s, jobs, ls.bs(P_ONE), J);
+    }
     const size_t m4 = mark();
     hipLaunchKernelGGL(k_delta_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     if (set.any_heavy)
         hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m5 = mark();
-    if (set.any_heavy) {  // bring the heavy-row histograms to this round's labels
-        hipLaunchKernelGGL(k_hist_mode, dim3(ls.total[P_ONE]), dim3(1), 0, s, jobs, ls.bs(P_ONE), J);
+    if (set.any_heavy) {  // recount or scan modes bring the heavy histograms up to date
         hipLaunchKernelGGL(k_hist_zero, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_recount, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_update, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
-        hipLaunchKernelGGL(k_hist_push_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
-        hipLaunchKernelGGL(k_hist_push_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     }
     hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m6 = mark();
@@ -1572,6 +1767,7 @@ uint64_t mix_key(uint64_t h, uint64_t v) {
 
 __global__ void k_hook_state(Ctl *ctl, int32_t t, int32_t epoch, int32_t mode) {
     ctl->done = 0;
+    ctl->diff_ok = 0;
     ctl->t = t;
     ctl->epoch = epoch;
     ctl->cand_mode = mode;
@@ -1584,8 +1780,9 @@ uint64_t phase2_single_key(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     uint64_t key = 0x2545f4914f6cdd1dull;
     const void *ptrs[] = {w.job_dev.p, w.sel_dev.p, w.ls.bstart.p, w.remaining.p, g->labels.p};
     for (const void *p : ptrs) key = mix_key(key, (uint64_t)(uintptr_t)p);
-    int64_t ints[3 + P_NPOL] = {w.sel.passes, g->n_heavy > 0 ? 1 : 0, w.ls.jobs};
-    for (int p = 0; p < P_NPOL; ++p) ints[3 + p] = w.ls.total[p];
+    int64_t ints[4 + P_NPOL] = {w.sel.passes, g->n_heavy > 0 ? 1 : 0, w.ls.jobs,
+                                ctx->tune.light_hist ? 1 : 0};
+    for (int p = 0; p < P_NPOL; ++p) ints[4 + p] = w.ls.total[p];
     for (int64_t v : ints) key = mix_key(key, (uint64_t)v);
     return key;
 }

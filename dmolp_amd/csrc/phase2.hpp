@@ -24,6 +24,9 @@ enum CtlIndex {
 // How a round finds its candidates: every vertex (round 1, B_0 = V),
 // pulled from the boundary side, or pushed from B_{t-1}'s rows.
 enum CandMode : int32_t { CAND_ALL = 0, CAND_PULL = 1, CAND_PUSH = 2 };
+// How the heavy-row histograms follow a round's changes: scan the rows
+// against the changed-vertex bitmap, recount them, or push the changes.
+enum HistMode : int32_t { HIST_SCAN = 0, HIST_RECOUNT = 1, HIST_PUSH = 2 };
 
 struct Ctl {
     unsigned long long v[C_NCTL];
@@ -37,8 +40,8 @@ struct Ctl {
     int32_t epoch;       // stamp value of the current round (epoch_base + t)
     int32_t epoch_base;
     int32_t max_rounds;
-    int32_t hist_mode;   // heavy-row histograms after this round: 0 update, 1 recount
-    int32_t pad;
+    int32_t hist_mode;   // HistMode of this round
+    int32_t diff_ok;     // diff[v] holds v's count of differing neighbours
     double epsilon;
 };
 
@@ -67,6 +70,13 @@ struct P2Job {
     uint32_t *cbits;
     uint16_t *ON;
     uint8_t *N8;         // new label this round (255: unchanged)
+    // Light rows keep their label counts too (when lh_on): row v of lh holds
+    // kLpCap / 2 words, the count of label l < min(k, 32) in the 16-bit half
+    // l % 2 of word l / 2.  Built by the initial sweep; after each round's
+    // decisions every changed vertex moves one count of each light
+    // neighbour's row from its old label to its new one.
+    uint32_t *lh;
+    const uint32_t *hbits;  // heavy-row bitmap of the graph
     RecDev bd;
     Ctl *ctl;
     unsigned long long *vc, *ic;
@@ -76,6 +86,7 @@ struct P2Job {
     int32_t *remaining;  // unfinished jobs of the set (shared by the set)
     int32_t trace_cap, k, heavy_degree, chunk_edges;
     int32_t epoch_base, max_rounds;
+    int32_t lh_on, pad0;
     double epsilon;
     double pull_frac;
     double recount_share, push_share;  // heavy-histogram modes (see k_hist_mode)
@@ -88,6 +99,7 @@ struct P2Set {
     const LaunchSet *ls = nullptr;
     int passes = 0;                 // largest selection digit-pass count
     bool any_heavy = false;
+    bool lh_on = false;             // light rows decide from kept histograms
     int32_t *remaining = nullptr;   // device counter of unfinished jobs
 };
 
@@ -98,6 +110,7 @@ struct Phase2Work {
     DevMem chg, diff, stamp, bstamp, bbits, deferred;
     DevMem L8;  // byte mirror of the labels for the LP gathers
     DevMem hh, cbits, ON, N8;
+    DevMem lh;  // uint32 [n][kLpCap / 2]: kept light-row label counts
     List B, D;          // boundary list (B_{t-1} read, then B_t written), Delta
     DevMem rtab;        // per-round constants: [0..R] 1 + log(t), [R+1..2R+1] exp(-t/5)
     int rtab_rounds = -1;

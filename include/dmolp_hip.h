@@ -106,6 +106,8 @@ typedef struct {
                                        changed vertices hold at least this share of entries */
     int32_t hist_push_per_mille;    /* ... push the changes when they hold at most this
                                        share (in 1/1000); between the two, scan a bitmap */
+    int32_t light_hist;  /* light rows' label counts: 1 kept in HBM across rounds and
+                            moved by the changed vertices, 2 gathered afresh each round */
 } dmolp_tuning;
 int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out);
 int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t);

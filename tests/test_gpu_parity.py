@@ -25,7 +25,7 @@ def c1():
 
 
 DEFAULT_TUNING = dict(heavy_degree=1024, chunk_edges=4096, pull_per_mille=50,
-                      hist_recount_per_mille=200, hist_push_per_mille=20)
+                      hist_recount_per_mille=200, hist_push_per_mille=20, light_hist=1)
 TUNINGS = {
     "default": DEFAULT_TUNING,
     # Tiny heavy threshold: the chunked heavy-row kernels run on small graphs.
@@ -41,6 +41,11 @@ TUNINGS = {
                       hist_recount_per_mille=1001, hist_push_per_mille=1000),
     "hist_recount": dict(DEFAULT_TUNING, heavy_degree=5, chunk_edges=64,
                          hist_recount_per_mille=1, hist_push_per_mille=1),
+    # Light rows gather their neighbours' labels every round instead of
+    # keeping histograms (alone, and with heavy histograms pushed).
+    "gather": dict(DEFAULT_TUNING, light_hist=2),
+    "gather_push": dict(DEFAULT_TUNING, heavy_degree=5, chunk_edges=64, light_hist=2,
+                        hist_recount_per_mille=1001, hist_push_per_mille=1000),
 }
 
 
