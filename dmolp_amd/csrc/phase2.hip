@@ -97,6 +97,7 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.N8 = gload(v.N8);
     r.lh = gload(v.lh);
     r.hbits = gload(v.hbits);
+    r.hubnb = gload(v.hubnb);
     r.bd = gload(v.bd);
     r.ctl = gload(v.ctl);
     r.vc = gload(v.vc);
@@ -499,7 +500,10 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
             live |= 1u << j;
             deg[j] = (int32_t)(e - s);
             lab[j] = lv;
-            hit[j] = bit_of(inb, (int32_t)v);
+            // v itself, then its neighbour of largest degree (usually in
+            // B_{t-1}), before any scan of the row.
+            const int32_t hn = J.hubnb[v];
+            hit[j] = bit_of(inb, (int32_t)v) || (hn >= 0 && bit_of(inb, hn));
             dif[j] = known;
             xs[j] = s;
             end[j] = e;
@@ -778,6 +782,48 @@ __device__ __forceinline__ int32_t lp_decide_unrolled(Count count, int E, const 
     return (best_s > base) ? best_l : own;
 }
 
+// lp_decide_by for a row whose own label is counted (0 <= own < E), with
+// fac[l] = 1 + pen[l] (the same double the loop multiplies by) and fmax the
+// largest fac.  The loop's winner is the lexicographic maximum of
+// (score, -bitrev5(l)), and it replaces own only when its score exceeds
+// base = count(own); so only labels scoring above base matter, and the
+// winner is the lexicographic maximum among them.  count * fac <= count *
+// fmax, and rounding is monotone, so a label whose count is at most c_skip
+// (the largest count c with c * fmax rounding to at most base) cannot score
+// above base: only the few larger counts are scored.  Counts are read from
+// the lane's LDS row (16-bit halves, label l in word l / 2).
+__device__ __forceinline__ int32_t lp_decide_kept(const uint32_t *row, int E, const double *fac,
+                                                  double fmax, int32_t own) {
+    const int32_t own_cnt = (int32_t)((row[own >> 1] >> ((own & 1) * 16)) & 0xffffu);
+    const double base = (double)own_cnt;
+    if (!(fmax > 0.0)) return own;  // every score is at most 0 <= base
+    const double q0 = base / fmax;  // a first guess, made exact below
+    int32_t cs = q0 >= 65535.0 ? 65535 : (int32_t)q0;
+    while (cs < 65535 && (double)(cs + 1) * fmax <= base) ++cs;
+    while (cs >= 0 && (double)cs * fmax > base) --cs;
+    uint32_t surv = 0;
+#pragma unroll
+    for (int q = 0; q < kPackedWords; ++q) {
+        const uint32_t wq = row[q];
+        surv |= ((int32_t)(wq & 0xffffu) > cs ? 1u : 0u) << (2 * q);
+        surv |= ((int32_t)(wq >> 16) > cs ? 1u : 0u) << (2 * q + 1);
+    }
+    if (E < 32) surv &= (1u << E) - 1u;
+    int32_t best_l = -1;
+    double best_s = 0.0;
+    while (surv) {
+        const int l = __builtin_ctz(surv);
+        surv &= surv - 1u;
+        const int32_t cnt = (int32_t)((row[l >> 1] >> ((l & 1) * 16)) & 0xffffu);
+        const double sc = (double)cnt * fac[l];
+        if (sc > base && (best_l < 0 || sc > best_s || (sc == best_s && bitrev5(l) < bitrev5(best_l)))) {
+            best_l = l;
+            best_s = sc;
+        }
+    }
+    return best_l >= 0 ? best_l : own;
+}
+
 // Light rows of B_t decide from their kept histograms: one lane per row and
 // one 64-byte row read (32 bytes or less for k <= 16) in place of a column
 // read and a label gather per entry.  B_t is taken from its bitmap in vertex
@@ -802,12 +848,25 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
     const Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
     const uint32_t *bt = J.bbits + (int64_t)(ctl->t & 1) * bwords;  // B_t
-    __shared__ double s_pen[kLpCap];
+    __shared__ double s_pen[kLpCap], s_fac[kLpCap];
+    __shared__ double s_fmax;
     __shared__ BlockAppendLds s_app;
+    __shared__ uint32_t s_row[kThreads * kPackedStride];
     const int E = k < kLpCap ? k : kLpCap;
     const int nvec = (E + 7) / 8;
-    if ((int)threadIdx.x < E) s_pen[threadIdx.x] = pen[threadIdx.x];
+    if ((int)threadIdx.x < E) {
+        s_pen[threadIdx.x] = pen[threadIdx.x];
+        s_fac[threadIdx.x] = 1.0 + pen[threadIdx.x];
+    }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        double f = s_fac[0];
+        for (int l = 1; l < E; ++l) f = s_fac[l] > f ? s_fac[l] : f;
+        s_fmax = f;
+    }
+    __syncthreads();
+    const double fmax = s_fmax;
+    uint32_t *lrow = &s_row[threadIdx.x * kPackedStride];
     const int w = threadIdx.x / kWave, lane = lane_id();
     const int64_t span = (int64_t)blockDim.x * kScanItems;
     // Block-uniform loop, so the Delta append can be aggregated per block.
@@ -830,18 +889,20 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
             if (!((m >> lane) & 1ull)) continue;
             const int32_t v = (int32_t)(gbase + lane);
             const uint4 *row = reinterpret_cast<const uint4 *>(lh + (int64_t)v * kPackedWords);
-            uint32_t c[kPackedWords];
 #pragma unroll
             for (int q = 0; q < kPackedWords / 4; ++q) {
                 const uint4 x = q < nvec ? row[q] : make_uint4(0u, 0u, 0u, 0u);
-                c[4 * q] = x.x;
-                c[4 * q + 1] = x.y;
-                c[4 * q + 2] = x.z;
-                c[4 * q + 3] = x.w;
+                lrow[4 * q] = x.x;
+                lrow[4 * q + 1] = x.y;
+                lrow[4 * q + 2] = x.z;
+                lrow[4 * q + 3] = x.w;
             }
             const int32_t own = L[v];
-            const int32_t nl = lp_decide_unrolled(
-                [&](int l) { return (int32_t)((c[l >> 1] >> ((l & 1) * 16)) & 0xffffu); }, E, s_pen, own);
+            const int32_t nl =
+                (own >= 0 && own < E)
+                    ? lp_decide_kept(lrow, E, s_fac, fmax, own)
+                    : lp_decide_by([&](int l) { return (int32_t)((lrow[l >> 1] >> ((l & 1) * 16)) & 0xffffu); },
+                                   E, s_pen, own);
             if (nl != own) {
                 cm |= 1u << j;
                 chg[v] = nl;
@@ -1505,6 +1566,7 @@ __global__ void k_p2_gather_done(const P2Job *JS, int jobs, int32_t *out) {
 Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
     DMOLP_REQUIRE(k >= 1 && k <= DMOLP_MAX_K, DMOLP_E_ARG, "k outside [1, 1024]");
     graph_prepare_bins(ctx, g);
+    graph_prepare_hubnb(ctx, g);
     if (!g->p2) g->p2.reset(new Phase2Work());
     Phase2Work &w = *g->p2;
     const int64_t n = g->n;
@@ -1589,6 +1651,7 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.lh_on = ctx->tune.light_hist ? 1 : 0;
     j.lh = j.lh_on ? w.lh.as<uint32_t>() : nullptr;
     j.hbits = g->hbits.as<uint32_t>();
+    j.hubnb = g->hubnb.as<int32_t>();
     j.deferred = w.deferred.as<int64_t>();
     j.hub_hist = w.hub_hist.as<int32_t>();
     j.bd = w.bd.dev;

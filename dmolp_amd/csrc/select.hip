@@ -22,6 +22,31 @@ __device__ __forceinline__ int bin1_of(int32_t d, int b1_bins) {
     return d < b1_bins - 1 ? d : b1_bins - 1;
 }
 
+// What the per-record passes need of a group's state, cached in LDS for the
+// first kSelLdsGroups groups (a global gather per record otherwise).
+constexpr int kSelLdsGroups = 64;
+struct SelLite {
+    int32_t mode, b1, desc, pad;
+    uint64_t kmax;  // SEL_KEY: largest key taken inside bin b1
+};
+__device__ __forceinline__ SelLite sel_lite(const SelState &s) {
+    SelLite l;
+    l.mode = s.mode;
+    l.b1 = s.b1;
+    l.desc = s.desc;
+    l.pad = 0;
+    l.kmax = s.prefix | ((s.r >= 64) ? ~0ull : ((1ull << s.r) - 1));
+    return l;
+}
+__device__ __forceinline__ void sel_lite_load(SelLite *lds, const SelState *st, int groups) {
+    const int ng = groups < kSelLdsGroups ? groups : kSelLdsGroups;
+    for (int i = threadIdx.x; i < ng; i += blockDim.x) lds[i] = sel_lite(st[i]);
+    __syncthreads();
+}
+__device__ __forceinline__ SelLite sel_get(const SelLite *lds, const SelState *st, int g) {
+    return g < kSelLdsGroups ? lds[g] : sel_lite(st[g]);
+}
+
 // Running count over bins in scan order (reverse: from the last bin down);
 // finds the first bin where it reaches target (1 <= target <= total).
 // Returns through LDS variables; all threads must call.
@@ -93,9 +118,19 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(const SelArgs *A, con
     for (int i = threadIdx.x; i < nb; i += blockDim.x) s_h[i] = 0;
     __syncthreads();
     const int64_t count = (int64_t)*in.cnt;
-    for (int64_t i = vthread(vg); i < count; i += vthreads(vg)) {
-        const int32_t gg = in.g[i];
-        atomicAdd(&s_h[gg * b1_bins + bin1_of(in.d[i], b1_bins)], 1);
+    constexpr int kH1Items = 4;  // independent record loads in flight per thread
+    const int64_t step = vthreads(vg);
+    for (int64_t i0 = vthread(vg); i0 < count; i0 += kH1Items * step) {
+        int32_t gg[kH1Items], dd[kH1Items];
+#pragma unroll
+        for (int j = 0; j < kH1Items; ++j) {
+            const int64_t i = i0 + j * step;
+            gg[j] = i < count ? in.g[i] : -1;
+            dd[j] = i < count ? in.d[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kH1Items; ++j)
+            if (gg[j] >= 0) atomicAdd(&s_h[gg[j] * b1_bins + bin1_of(dd[j], b1_bins)], 1);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += blockDim.x)
@@ -184,8 +219,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, c
     if (a.done && *a.done) return;
     if (*a.refining == 0) return;
     __shared__ BlockAppendLds s_app;
+    __shared__ SelLite s_st[kSelLdsGroups];
     const RecDev in = a.in, ref = a.ref;
     const SelState *st = a.st;
+    sel_lite_load(s_st, st, a.groups);
     const int b1_bins = a.b1_bins;
     const int64_t count = (int64_t)*in.cnt;
     const int64_t span = (int64_t)blockDim.x * kCompactItems;
@@ -200,7 +237,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, c
             if (i >= count) continue;
             gg[j] = in.g[i];
             d[j] = in.d[i];
-            const SelState s = st[gg[j]];
+            const SelLite s = sel_get(s_st, st, gg[j]);
             if (s.mode == SEL_REFINE && bin1_of(d[j], b1_bins) == s.b1) {
                 x[j] = in.x[i];
                 keep_mask |= 1u << j;
@@ -301,6 +338,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(const SelArgs *A, c
     const int heavy_degree = a.heavy_degree, chunk_edges = a.chunk_edges;
     const int has_carry = a.has_carry;
     __shared__ BlockAppendLds s_app;
+    __shared__ SelLite s_st[kSelLdsGroups];
+    sel_lite_load(s_st, st, a.groups);
     const int64_t count = (int64_t)*in.cnt;
     const int64_t span = (int64_t)blockDim.x * kCompactItems;
     // Block-uniform loop; each thread takes kCompactItems records (strided
@@ -319,7 +358,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(const SelArgs *A, c
             x[j] = in.x[i];
             gg[j] = in.g[i];
             d[j] = in.d[i];
-            const SelState s = st[gg[j]];
+            const SelLite s = sel_get(s_st, st, gg[j]);
             const int b = bin1_of(d[j], b1_bins);
             const bool earlier = s.desc ? b > s.b1 : b < s.b1;
             bool take = false;
@@ -329,9 +368,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(const SelArgs *A, c
                 case SEL_KEY: {
                     if (earlier) { take = true; break; }
                     if (b != s.b1) break;
-                    const uint64_t K = sel_key(x[j], d[j], s.desc, id_bits, deg_bits);
-                    const uint64_t kmax = s.prefix | ((s.r >= 64) ? ~0ull : ((1ull << s.r) - 1));
-                    take = K <= kmax;
+                    take = sel_key(x[j], d[j], s.desc, id_bits, deg_bits) <= s.kmax;
                     break;
                 }
                 default: break;
