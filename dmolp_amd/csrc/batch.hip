@@ -73,7 +73,7 @@ P2Set p2_set_build(dmolp_ctx *ctx, std::vector<P2Item *> &items, BatchState &bs,
         DMOLP_REQUIRE(it.job.lh_on == (ctx->tune.light_hist ? 1 : 0), DMOLP_E_ARG,
                       "light_hist tuning changed while a job was in flight");
         jobs[(size_t)j] = it.job;
-        sels[(size_t)j] = phase2_sel_args(ctx, w);
+        sels[(size_t)j] = phase2_sel_args(ctx, it.g, w);
         job_blocks(ctx, it.g->n, it.g->nnz, it.k, it.g->n_heavy, G == 1, blocks[(size_t)j].data());
         passes = std::max(passes, w.sel.passes);
         any_heavy = any_heavy || it.g->n_heavy > 0;

@@ -72,6 +72,26 @@ __device__ __forceinline__ int32_t nb_label(bool exact8, const uint8_t *L8, cons
     return L[u];
 }
 
+// Add each thread's x to *dst with one atomic per block (block-uniform
+// call, after the kernel's loops).  Many blocks adding to one counter once
+// per loop iteration would serialise on that counter's cache line.
+__device__ __forceinline__ void block_count_add(unsigned long long *dst, int64_t x) {
+    __shared__ unsigned long long s_acc;
+    if (threadIdx.x == 0) s_acc = 0;
+    __syncthreads();
+    x = wave_sum(x);
+    if (lane_id() == 0 && x) atomicAdd(&s_acc, (unsigned long long)x);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_acc) atomicAdd(dst, s_acc);
+}
+
+// LN entry of a vertex deciding to move from own to nl (byte mirrors).
+__device__ __forceinline__ uint16_t ln_pair(int32_t own, int32_t nl) {
+    const uint32_t o = (own >= 0 && own < kLabel8None) ? (uint32_t)own : (uint32_t)kLabel8None;
+    const uint32_t nw = (nl >= 0 && nl < kLabel8None) ? (uint32_t)nl : (uint32_t)kLabel8None;
+    return (uint16_t)(o | (nw << 8));
+}
+
 // Every round kernel takes the job set and runs on its job's share of the
 // launch (see VGrid): one graph, or many graphs in one launch.
 using ::dmolp::gload;
@@ -88,13 +108,13 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.stamp = gload(v.stamp);
     r.deferred = gload(v.deferred);
     r.hub_hist = gload(v.hub_hist);
-    r.bstamp = gload(v.bstamp);
+    r.rdeg = gload(v.rdeg);
     r.bbits = gload(v.bbits);
     r.hh = gload(v.hh);
     r.hid = gload(v.hid);
     r.cbits = gload(v.cbits);
     r.ON = gload(v.ON);
-    r.N8 = gload(v.N8);
+    r.LN = gload(v.LN);
     r.lh = gload(v.lh);
     r.hbits = gload(v.hbits);
     r.hubnb = gload(v.hubnb);
@@ -358,6 +378,7 @@ __global__ void k_round_begin(const P2Job *JS, const int32_t *bstart, int jobs) 
         ctl->v[C_PREV_DEG] = prev_deg;
         ctl->epoch = ctl->epoch_base + t;
         *bd.cnt = 0;
+        B.cnt[LC_FILL] = 0;
         ctl->v[C_CAND] = 0;
         ctl->v[C_DEFERRED] = 0;
         ctl->cand_mode = t == 1 ? CAND_ALL : ((double)prev_deg >= pull_frac * (double)nnz ? CAND_PULL : CAND_PUSH);
@@ -371,32 +392,23 @@ __global__ void k_round_begin(const P2Job *JS, const int32_t *bstart, int jobs) 
 __global__ __launch_bounds__(kThreads) void k_cand_all(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
-    const int32_t *L = J.L;
     const int32_t *diff = J.diff;
-    const RecDev bd = J.bd;
+    int32_t *rdeg = J.rdeg;
     Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_ALL) return;
-    __shared__ BlockAppendLds s_app;
+    int64_t nrec = 0;
     const int64_t span = (int64_t)blockDim.x * kScanItems;
     for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
-        uint32_t m = 0;
-        int32_t nb = 0;
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
-            if (v < g.n && diff[v] > 0) { m |= 1u << j; ++nb; }
-        }
-        int64_t pos = block_reserve(nb, bd.cnt, s_app);
-#pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            if (!(m & (1u << j))) continue;
-            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
-            bd.x[pos] = (int32_t)v;
-            bd.g[pos] = L[v];
-            bd.d[pos] = (int32_t)(g.off[v + 1] - g.off[v]);
-            ++pos;
+            if (v >= g.n) continue;
+            const bool rec = diff[v] > 0;
+            rdeg[v] = rec ? (int32_t)(g.off[v + 1] - g.off[v]) : -1;
+            nrec += rec ? 1 : 0;
         }
     }
+    block_count_add(J.bd.cnt, nrec);
 }
 
 // First touch of u in this pass; a touched boundary vertex becomes a record.
@@ -427,7 +439,7 @@ __device__ __forceinline__ bool on_boundary(const P2Job &J, bool diff_ok, int32_
 // First touch of u in this pass; a touched boundary vertex becomes a record.
 __device__ __forceinline__ void touch(const P2Job &J, bool diff_ok, bool pred, int32_t u,
                                       int32_t *stamp, int32_t epoch, const int32_t *L,
-                                      bool exact8, const RecDev &bd, int64_t &cand) {
+                                      bool exact8, int64_t &cand, int64_t &nrec) {
     bool first = false;
     if (pred && stamp[u] != epoch) first = atomicExch(&stamp[u], epoch) != epoch;
     if (first) ++cand;
@@ -439,11 +451,9 @@ __device__ __forceinline__ void touch(const P2Job &J, bool diff_ok, bool pred, i
         e = J.g.off[u + 1];
     }
     const bool rec = first && on_boundary(J, diff_ok, u, lu, s, e, exact8);
-    const int64_t pos = wave_append(rec, bd.cnt);
     if (rec) {
-        bd.x[pos] = u;
-        bd.g[pos] = lu;
-        bd.d[pos] = (int32_t)(e - s);
+        J.rdeg[u] = (int32_t)(e - s);
+        ++nrec;
     }
 }
 
@@ -468,7 +478,8 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
     // are skipped outright and boundary rows are scanned for B_{t-1} only.
     const bool use_diff = ctl->diff_ok != 0;
     const int32_t *diff = J.diff;
-    __shared__ BlockAppendLds s_app;
+    int32_t *rdeg = J.rdeg;
+    int64_t nrec_total = 0;
     const int64_t span = (int64_t)blockDim.x * kScanItems;
     for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
         uint32_t rec = 0, live = 0;
@@ -541,17 +552,16 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
             }
             if (hit[j] && dif[j]) { rec |= 1u << j; ++nr; }
         }
-        int64_t pos = block_reserve(nr, bd.cnt, s_app);
+        nrec_total += nr;
+        // Every vertex gets its record entry: its degree, or -1 (deferred
+        // rows: k_cand_pull_heavy writes the degree if they turn out records).
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
-            if (!(rec & (1u << j))) continue;
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
-            bd.x[pos] = (int32_t)v;
-            bd.g[pos] = lab[j];
-            bd.d[pos] = deg[j];
-            ++pos;
+            if (v < g.n) rdeg[v] = (rec & (1u << j)) ? deg[j] : -1;
         }
     }
+    block_count_add(bd.cnt, nrec_total);
 }
 
 __global__ void k_cand_pull_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
@@ -582,11 +592,9 @@ __global__ void k_cand_pull_heavy(const P2Job *JS, const int32_t *bstart, int jo
             hit = hit || __ballot(h) != 0;
         }
         const bool rec = dif && hit && lane_id() == 0;
-        const int64_t pos = wave_append(rec, bd.cnt);
         if (rec) {
-            bd.x[pos] = v;
-            bd.g[pos] = lv;
-            bd.d[pos] = (int32_t)(e - s);
+            J.rdeg[v] = (int32_t)(e - s);
+            atomicAdd(bd.cnt, 1ull);
         }
     }
 }
@@ -596,7 +604,6 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(const P2Job *JS, const 
     const GraphDev g = J.g;
     const ListDev prev = J.B;
     const int32_t *L = J.L;
-    const uint8_t *L8 = J.L8;
     const bool exact8 = J.k <= kExact8;
     int32_t *stamp = J.stamp;
     const RecDev bd = J.bd;
@@ -606,23 +613,24 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(const P2Job *JS, const 
     const int32_t epoch = ctl->epoch;
     __shared__ BatchLds s_b[kWavesPerBlock];
     const int w = threadIdx.x / kWave, lane = lane_id();
-    int64_t cand = 0;
+    int64_t cand = 0, nrec = 0;
     const int64_t count = (int64_t)prev.cnt[LC_LIGHT];
     for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, prev.light, count, b0, s_b[w], v, deg, valid);
-        touch(J, diff_ok, valid, v, stamp, epoch, L, exact8, bd, cand);
+        touch(J, diff_ok, valid, v, stamp, epoch, L, exact8, cand, nrec);
         for (int32_t q0 = 0; q0 < total; q0 += kWave) {
             const int32_t q = q0 + lane;
             const bool on = q < total;
             int32_t u = 0;
             if (on) u = g.col[s_b[w].base[batch_slot(s_b[w], q)] + q];
-            touch(J, diff_ok, on, u, stamp, epoch, L, exact8, bd, cand);
+            touch(J, diff_ok, on, u, stamp, epoch, L, exact8, cand, nrec);
         }
         wave_sync();
     }
     wave_add(&ctl->v[C_CAND], cand);
+    block_count_add(bd.cnt, nrec);
 }
 
 __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
@@ -630,7 +638,6 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
     const GraphDev g = J.g;
     const ListDev prev = J.B;
     const int32_t *L = J.L;
-    const uint8_t *L8 = J.L8;
     const bool exact8 = J.k <= kExact8;
     int32_t *stamp = J.stamp;
     const int chunk_edges = J.chunk_edges;
@@ -640,22 +647,23 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
     const bool diff_ok = ctl->diff_ok != 0;
     const int32_t epoch = ctl->epoch;
     const int lane = lane_id();
-    int64_t cand = 0;
+    int64_t cand = 0, nrec = 0;
     const int64_t nch = (int64_t)prev.cnt[LC_CHUNKS];
     for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
         const int64_t item = prev.hc[j];
         chunk_bounds(g, prev, item, chunk_edges, slot, v, e0, e1, deg);
-        touch(J, diff_ok, lane == 0 && (item & 0xffffffffll) == 0, v, stamp, epoch, L, exact8, bd,
-              cand);
+        touch(J, diff_ok, lane == 0 && (item & 0xffffffffll) == 0, v, stamp, epoch, L, exact8, cand,
+              nrec);
         for (int64_t eb = e0; eb < e1; eb += kWave) {
             const int64_t e = eb + lane;
             const bool on = e < e1;
-            touch(J, diff_ok, on, on ? g.col[e] : 0, stamp, epoch, L, exact8, bd, cand);
+            touch(J, diff_ok, on, on ? g.col[e] : 0, stamp, epoch, L, exact8, cand, nrec);
         }
     }
     wave_add(&ctl->v[C_CAND], cand);
+    block_count_add(bd.cnt, nrec);
 }
 
 // ---------------------------------------------------------------- LP decision
@@ -695,7 +703,7 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
     const double *pen = J.pen;
     const int k = J.k;
     int32_t *chg = J.chg;
-    uint8_t *N8 = J.N8;
+    uint16_t *LN = J.LN;
     const ListDev D = J.D;
     const int heavy_degree = J.heavy_degree;
     const int chunk_edges = J.chunk_edges;
@@ -752,7 +760,7 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
             changed = nl != own;
             if (changed) {
                 chg[v] = nl;
-                N8[v] = (uint8_t)(nl < kLabel8None ? nl : kLabel8None);
+                LN[v] = ln_pair(own, nl);
             }
         }
         block_list_append(D, changed, v, deg, heavy_degree, chunk_edges, s_app);
@@ -824,6 +832,20 @@ __device__ __forceinline__ int32_t lp_decide_kept(const uint32_t *row, int E, co
     return best_l >= 0 ? best_l : own;
 }
 
+// Delta entries staged in LDS by a block: one reservation, a coalesced copy.
+// Block-uniform call.
+constexpr int kDStage = 4096;
+__device__ __forceinline__ void d_flush(const ListDev &D, const int32_t *s_dq, int32_t &s_dn,
+                                        unsigned long long &s_base) {
+    const int32_t cnt = s_dn;
+    if (threadIdx.x == 0) s_base = atomicAdd(&D.cnt[LC_LIGHT], (unsigned long long)cnt);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) D.light[s_base + (unsigned long long)i] = s_dq[i];
+    __syncthreads();
+    if (threadIdx.x == 0) s_dn = 0;
+    __syncthreads();
+}
+
 // Light rows of B_t decide from their kept histograms: one lane per row and
 // one 64-byte row read (32 bytes or less for k <= 16) in place of a column
 // read and a label gather per entry.  B_t is taken from its bitmap in vertex
@@ -842,7 +864,7 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
     const double *pen = J.pen;
     const int k = J.k;
     int32_t *chg = J.chg;
-    uint8_t *N8 = J.N8;
+    uint16_t *LN = J.LN;
     const ListDev D = J.D;
     const int64_t bwords = J.bwords;
     const Ctl *ctl = J.ctl;
@@ -850,7 +872,6 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
     const uint32_t *bt = J.bbits + (int64_t)(ctl->t & 1) * bwords;  // B_t
     __shared__ double s_pen[kLpCap], s_fac[kLpCap];
     __shared__ double s_fmax;
-    __shared__ BlockAppendLds s_app;
     __shared__ uint32_t s_row[kThreads * kPackedStride];
     const int E = k < kLpCap ? k : kLpCap;
     const int nvec = (E + 7) / 8;
@@ -869,10 +890,15 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
     uint32_t *lrow = &s_row[threadIdx.x * kPackedStride];
     const int w = threadIdx.x / kWave, lane = lane_id();
     const int64_t span = (int64_t)blockDim.x * kScanItems;
+    __shared__ int32_t s_dq[kDStage];
+    __shared__ int32_t s_dn;
+    __shared__ unsigned long long s_base;
+    if (threadIdx.x == 0) s_dn = 0;
+    __syncthreads();
+    int64_t dsum = 0;
     // Block-uniform loop, so the Delta append can be aggregated per block.
     for (int64_t b0 = vg.block * span; b0 < n; b0 += vg.blocks * span) {
         uint32_t cm = 0;
-        int64_t dsum = 0;
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t gbase = b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave;
@@ -906,17 +932,23 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
             if (nl != own) {
                 cm |= 1u << j;
                 chg[v] = nl;
-                N8[v] = (uint8_t)(nl < kLabel8None ? nl : kLabel8None);
+                LN[v] = ln_pair(own, nl);
                 dsum += off[v + 1] - off[v];
             }
         }
-        int64_t pos = block_reserve(__popc(cm), &D.cnt[LC_LIGHT], s_app, dsum, &D.cnt[LC_DEG]);
+        // Changed vertices are staged in LDS and flushed to Delta with one
+        // global reservation per kDStage / 2 or more entries.
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             if (!(cm & (1u << j))) continue;
-            D.light[pos++] = (int32_t)(b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave + lane);
+            s_dq[atomicAdd(&s_dn, 1)] = (int32_t)(b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave + lane);
         }
+        __syncthreads();
+        if (s_dn > kDStage - (int)span) d_flush(D, s_dq, s_dn, s_base);
     }
+    __syncthreads();
+    if (s_dn > 0) d_flush(D, s_dq, s_dn, s_base);
+    block_count_add(&D.cnt[LC_DEG], dsum);
 }
 
 // Heavy rows, pass 1: each wave counts one chunk with a peel loop over the
@@ -978,7 +1010,7 @@ __global__ void k_lp_heavy_decide(const P2Job *JS, const int32_t *bstart, int jo
     const double *pen = J.pen;
     const int k = J.k;
     int32_t *chg = J.chg;
-    uint8_t *N8 = J.N8;
+    uint16_t *LN = J.LN;
     const ListDev D = J.D;
     const int heavy_degree = J.heavy_degree;
     const int chunk_edges = J.chunk_edges;
@@ -1006,7 +1038,7 @@ __global__ void k_lp_heavy_decide(const P2Job *JS, const int32_t *bstart, int jo
             changed = nl != own;
             if (changed) {
                 chg[v] = nl;
-                N8[v] = (uint8_t)(nl < kLabel8None ? nl : kLabel8None);
+                LN[v] = ln_pair(own, nl);
             }
         }
         list_append(D, changed, v, deg, heavy_degree, chunk_edges);
@@ -1026,20 +1058,21 @@ __global__ void k_lp_heavy_decide(const P2Job *JS, const int32_t *bstart, int jo
 // its label, its differing count moves here (keep_diff).
 __device__ __forceinline__ int32_t pair_effect(int32_t v, int32_t a, int32_t c, int32_t u,
                                                const int32_t *L, const int32_t *chg,
-                                               const uint8_t *L8, const uint8_t *N8, bool exact8,
+                                               const uint16_t *LN, bool exact8,
                                                bool keep_diff, int32_t *diff, int32_t &dd_out,
                                                bool &same_out) {
     dd_out = 0;
     int32_t lu, lu2;
     bool uchg;
-    if (exact8) {  // byte gathers: old label and this round's new label (255: unchanged)
-        const int32_t nu = N8[u];
+    if (exact8) {  // one 16-bit gather: old label byte | this round's new one << 8 (255: unchanged)
+        const uint32_t ln = LN[u];
+        const int32_t nu = (int32_t)(ln >> 8);
         uchg = nu != kLabel8None;
         if (uchg) {
             same_out = nu == c;
             if (u < v) return 0;  // counted from u's side
         }
-        lu = L8[u];
+        lu = (int32_t)(ln & 0xffu);
         lu2 = uchg ? nu : lu;
     } else {
         const int32_t cu = chg[u];
@@ -1100,7 +1133,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
     const ListDev D = J.D;
     const int32_t *L = J.L;
     const int32_t *chg = J.chg;
-    const uint8_t *L8 = J.L8, *N8 = J.N8;
+    const uint16_t *LN = J.LN;
     const int k = J.k;
     const bool exact8 = k <= kExact8;
     Ctl *ctl = J.ctl;
@@ -1148,7 +1181,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
                 int32_t dd;
                 bool same;
                 const bool uh = bit_of(J.hbits, u[j]);
-                const int32_t io = pair_effect(vv, a, c, u[j], L, chg, L8, N8, exact8, keep_diff && !uh,
+                const int32_t io = pair_effect(vv, a, c, u[j], L, chg, LN, exact8, keep_diff && !uh,
                                                diff, dd, same);
                 cut_delta += dd;
                 if (io) atomicAdd(&s_io[w][slot[j]], io);
@@ -1184,7 +1217,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
     const ListDev D = J.D;
     const int32_t *L = J.L;
     const int32_t *chg = J.chg;
-    const uint8_t *L8 = J.L8, *N8 = J.N8;
+    const uint16_t *LN = J.LN;
     const int k = J.k;
     const bool exact8 = k <= kExact8;
     const int chunk_edges = J.chunk_edges;
@@ -1215,7 +1248,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
             bool same;
             const int32_t u = g.col[e];
             const bool uh = bit_of(J.hbits, u);
-            const int32_t io = pair_effect(v, a, c, u, L, chg, L8, N8, exact8, keep_diff && !uh, diff,
+            const int32_t io = pair_effect(v, a, c, u, L, chg, LN, exact8, keep_diff && !uh, diff,
                                            dd, same);
             if (upkeep) row_upkeep(J, lh_on, hh_push, uh, u, a, c, E);
             dv += dd;
@@ -1247,10 +1280,13 @@ __global__ void k_mirror8(const P2Job *JS, const int32_t *bstart, int jobs) {
     const int64_t n = J.g.n;
     const int32_t *L = J.L;
     uint8_t *L8 = J.L8;
+    uint16_t *LN = J.LN;
     for (int64_t v = vthread(vg); v < n;
          v += vthreads(vg)) {
         const int32_t l = L[v];
-        L8[v] = (l >= 0 && l < kLabel8None) ? (uint8_t)l : (uint8_t)kLabel8None;
+        const uint32_t b = (l >= 0 && l < kLabel8None) ? (uint32_t)l : (uint32_t)kLabel8None;
+        L8[v] = (uint8_t)b;
+        LN[v] = (uint16_t)(b | (kLabel8None << 8));
     }
 }
 
@@ -1258,7 +1294,8 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const ListDev D = J.D;
     int32_t *L = J.L;
-    uint8_t *L8 = J.L8, *N8 = J.N8;
+    uint8_t *L8 = J.L8;
+    uint16_t *LN = J.LN;
     uint16_t *ON = J.ON;
     uint32_t *cbits = J.cbits;
     int32_t *chg = J.chg;
@@ -1274,7 +1311,7 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
         const uint32_t nb = (nl_v >= 0 && nl_v < kLabel8None) ? (uint32_t)nl_v : (uint32_t)kLabel8None;
         ON[v] = (uint16_t)(ob | (nb << 8));
         L8[v] = (uint8_t)nb;
-        N8[v] = (uint8_t)kLabel8None;
+        LN[v] = (uint16_t)(nb | (kLabel8None << 8));
         chg[v] = -1;
         atomicOr(&cbits[v >> 5], 1u << (v & 31));
     }
@@ -1289,7 +1326,7 @@ __global__ void k_lp_heavy_hist(const P2Job *JS, const int32_t *bstart, int jobs
     const ListDev B = J.B;
     const int k = J.k;
     int32_t *chg = J.chg;
-    uint8_t *N8 = J.N8;
+    uint16_t *LN = J.LN;
     const ListDev D = J.D;
     const int64_t *off = J.g.off;
     if (is_done(J.ctl)) return;
@@ -1312,7 +1349,7 @@ __global__ void k_lp_heavy_hist(const P2Job *JS, const int32_t *bstart, int jobs
             changed = nl != own;
             if (changed) {
                 chg[v] = nl;
-                N8[v] = (uint8_t)(nl < kLabel8None ? nl : kLabel8None);
+                LN[v] = ln_pair(own, nl);
             }
         }
         list_append(D, changed, v, deg, J.heavy_degree, J.chunk_edges);
@@ -1523,7 +1560,6 @@ __global__ void k_init_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
     for (int64_t v = vthread(vg); v < n; v += vthreads(vg)) J.diff[v] = 0;
     for (int64_t i = vthread(vg); i < 2 * J.bwords; i += vthreads(vg)) J.bbits[i] = 0u;
     for (int64_t i = vthread(vg); i < J.bwords; i += vthreads(vg)) J.cbits[i] = 0u;
-    for (int64_t v = vthread(vg); v < n; v += vthreads(vg)) J.N8[v] = (uint8_t)kLabel8None;
     const int64_t nhh = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
     for (int64_t i = vthread(vg); i < nhh; i += vthreads(vg)) J.hh[i] = 0;
     if (vg.block == 0) {
@@ -1538,22 +1574,40 @@ __global__ void k_set_remaining(int32_t *remaining, int32_t v) { *remaining = v;
 // After the selection: pack B_t (vertices stamped with this round's epoch)
 // into the bitmap of half (t & 1), which the next round's pull candidates
 // test.  One ballot per 64 vertices: coalesced stamp reads, two word stores.
-__global__ void k_pack_bbits(const P2Job *JS, const int32_t *bstart, int jobs) {
+
+// done flag of every job of a set, gathered for one host copy.
+// Push rounds write record entries only for the vertices they touch: the
+// rest of the dense record array is cleared first.
+__global__ void k_rdeg_clear(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
-    if (is_done(J.ctl)) return;
-    const int32_t epoch = J.ctl->epoch;
-    const int64_t n = J.g.n;
-    uint32_t *half = J.bbits + (int64_t)(J.ctl->t & 1) * J.bwords;
+    if (is_done(J.ctl) || J.ctl->cand_mode != CAND_PUSH) return;
+    for (int64_t v = vthread(vg); v < J.g.n; v += vthreads(vg)) J.rdeg[v] = -1;
+}
+
+// The light-vertex list of a boundary set from its bitmap (the selection
+// writes bitmaps), for the kernels that walk the set as a list: the
+// gathering LP (after the selection, shift 0: B_t) and push candidates
+// (at round start, shift 1: B_{t-1}; only_push: skip unless this round
+// pushes).  Heavy rows keep their slots from the selection.
+__global__ void k_b_list(const P2Job *JS, const int32_t *bstart, int jobs, int shift, int only_push) {
+    P2_JOB();
+    const Ctl *ctl = J.ctl;
+    if (is_done(ctl) || (only_push && ctl->cand_mode != CAND_PUSH)) return;
+    const ListDev B = J.B;
+    const int64_t bwords = J.bwords, n = J.g.n;
+    const uint32_t *half = J.bbits + (int64_t)((ctl->t - shift) & 1) * bwords;
+    const int lane = lane_id();
     for (int64_t base = vwave(vg) * kWave; base < n; base += vwaves(vg) * kWave) {
-        const int64_t v = base + lane_id();
-        const uint64_t m = __ballot(v < n && J.bstamp[v] == epoch);
         const int64_t wi = base >> 5;
-        if (lane_id() == 0) half[wi] = (uint32_t)m;
-        if (lane_id() == 1 && wi + 1 < J.bwords) half[wi + 1] = (uint32_t)(m >> 32);
+        uint64_t m = half[wi] & ~J.hbits[wi];
+        if (wi + 1 < bwords) m |= (uint64_t)(half[wi + 1] & ~J.hbits[wi + 1]) << 32;
+        if (!m) continue;  // wave-uniform
+        const bool on = (m >> lane) & 1ull;
+        const int64_t pos = wave_append(on, &B.cnt[LC_FILL]);
+        if (on) B.light[pos] = (int32_t)(base + lane);
     }
 }
 
-// done flag of every job of a set, gathered for one host copy.
 __global__ void k_p2_gather_done(const P2Job *JS, int jobs, int32_t *out) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < jobs; j += gridDim.x * blockDim.x)
         out[j] = gload(JS[j].ctl)->done;
@@ -1574,14 +1628,13 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.chg.alloc(sizeof(int32_t) * (size_t)n);
         w.diff.alloc(sizeof(int32_t) * (size_t)n);
         w.stamp.alloc(sizeof(int32_t) * (size_t)n);
-        w.bstamp.alloc(sizeof(int32_t) * (size_t)n);
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.bstamp.p, 0, w.bstamp.bytes, ctx->stream));
+        w.rdeg.alloc(sizeof(int32_t) * (size_t)n);
         w.bbits.alloc(sizeof(uint32_t) * 2 * (size_t)((n + 31) / 32));
         w.L8.alloc((size_t)n);
         w.ON.alloc(sizeof(uint16_t) * (size_t)n);
-        w.N8.alloc((size_t)n);
+        w.LN.alloc(sizeof(uint16_t) * (size_t)n);
         w.cbits.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.N8.p, 0xff, w.N8.bytes, ctx->stream));
+        DMOLP_HIP_CHECK(hipMemsetAsync(w.LN.p, 0xff, w.LN.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.cbits.p, 0, w.cbits.bytes, ctx->stream));
         w.bd.alloc(n);
         DMOLP_HIP_CHECK(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, ctx->stream));
@@ -1640,14 +1693,14 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.chg = w.chg.as<int32_t>();
     j.diff = w.diff.as<int32_t>();
     j.stamp = w.stamp.as<int32_t>();
-    j.bstamp = w.bstamp.as<int32_t>();
+    j.rdeg = w.rdeg.as<int32_t>();
     j.bbits = w.bbits.as<uint32_t>();
     j.bwords = (g->n + 31) / 32;
     j.hh = w.hh.as<int32_t>();
     j.hid = g->hid.as<int32_t>();
     j.cbits = w.cbits.as<uint32_t>();
     j.ON = w.ON.as<uint16_t>();
-    j.N8 = w.N8.as<uint8_t>();
+    j.LN = w.LN.as<uint16_t>();
     j.lh_on = ctx->tune.light_hist ? 1 : 0;
     j.lh = j.lh_on ? w.lh.as<uint32_t>() : nullptr;
     j.hbits = g->hbits.as<uint32_t>();
@@ -1678,15 +1731,19 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     return j;
 }
 
-SelArgs phase2_sel_args(dmolp_ctx *ctx, Phase2Work &w) {
+SelArgs phase2_sel_args(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     Ctl *ctl = w.ctl.as<Ctl>();
     TakeRule rule{};
     rule.kind = 0;
     rule.ratio_tab = w.rtab.p ? w.rtab.as<double>() + w.rtab_rounds + 1 : nullptr;
     rule.t_ptr = &ctl->t;
     rule.re = w.re.as<double>();
-    return select_args(ctx, w.sel, w.bd.dev, rule, &ctl->done, w.B.dev, nullptr,
-                       w.bstamp.as<int32_t>(), &ctl->epoch);
+    // Records are dense (one entry per vertex, labels read in place) and the
+    // selected set goes straight to the B_t bitmap.
+    SelArgs a = select_args(ctx, w.sel, w.bd.dev, rule, &ctl->done, w.B.dev, nullptr);
+    select_args_dense(a, w.rdeg.as<int32_t>(), g->labels.as<int32_t>(), g->n, w.bbits.as<uint32_t>(),
+                      (g->n + 31) / 32);
+    return a;
 }
 
 P2Set phase2_single_set(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds,
@@ -1698,7 +1755,7 @@ P2Set phase2_single_set(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_r
         w.sel_dev.alloc(sizeof(SelArgs));
     }
     w.job_host = phase2_job(ctx, g, w, max_rounds, epsilon, w.remaining.as<int32_t>());
-    w.sel_host = phase2_sel_args(ctx, w);
+    w.sel_host = phase2_sel_args(ctx, g, w);
     DMOLP_HIP_CHECK(hipMemcpyAsync(w.job_dev.p, &w.job_host, sizeof(P2Job), hipMemcpyHostToDevice, s));
     DMOLP_HIP_CHECK(hipMemcpyAsync(w.sel_dev.p, &w.sel_host, sizeof(SelArgs), hipMemcpyHostToDevice, s));
     std::array<int32_t, P_NPOL> blk{};
@@ -1763,16 +1820,20 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     const size_t m0 = mark();
     hipLaunchKernelGGL(k_round_begin, dim3(ls.total[P_ONE]), dim3(64), 0, s, jobs, ls.bs(P_ONE), J);
     // Round 1 takes every vertex; later rounds pull or push (ctl->cand_mode).
+    // Each writes one dense record entry per vertex (push: after a clear).
     hipLaunchKernelGGL(k_cand_all, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     hipLaunchKernelGGL(k_cand_pull, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     if (set.any_heavy)
         hipLaunchKernelGGL(k_cand_pull_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_rdeg_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_b_list, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 1, 1);  // B_{t-1} list
     hipLaunchKernelGGL(k_cand_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     if (set.any_heavy)
         hipLaunchKernelGGL(k_cand_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m1 = mark();
-    select_enqueue(ctx, set.sel, ls, set.passes);
-    hipLaunchKernelGGL(k_pack_bbits, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    select_enqueue(ctx, set.sel, ls, set.passes);  // B_t: bitmap, heavy slots, counts
+    if (!set.lh_on)  // the gathering LP walks B_t as a list
+        hipLaunchKernelGGL(k_b_list, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 0, 0);
     const size_t m2 = mark();
     if (set.lh_on)
         hipLaunchKernelGGL(k_lp_light_kept, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
@@ -1890,12 +1951,16 @@ void phase2_select_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t)
     const LaunchSet &ls = *set.ls;
     hipLaunchKernelGGL(k_hook_state, dim3(1), dim3(1), 0, s, ctl, t, w.epoch, (int32_t)CAND_PUSH);
     DMOLP_HIP_CHECK(hipMemsetAsync(w.bd.cnt.p, 0, w.bd.cnt.bytes, s));
+    hipLaunchKernelGGL(k_rdeg_clear, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), ls.jobs);
     hipLaunchKernelGGL(k_cand_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), ls.jobs);
     if (set.any_heavy)
         hipLaunchKernelGGL(k_cand_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                            ls.bs(P_GRID), ls.jobs);
     select_enqueue(ctx, set.sel, ls, set.passes);
+    hipLaunchKernelGGL(k_b_list, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), ls.jobs, 0, 0);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 

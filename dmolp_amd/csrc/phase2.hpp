@@ -58,7 +58,7 @@ struct P2Job {
     uint8_t *L8;
     int32_t *chg, *diff, *stamp, *hub_hist;
     int64_t *deferred;  // long rows the pull pass left undecided: (flags << 32) | v
-    int32_t *bstamp;  // round stamp of the vertices selected into B_t
+    int32_t *rdeg;    // dense selection records: degree of each record vertex, else -1
     uint32_t *bbits;  // two bitmaps of n bits: B_t packed into half (t & 1)
     int64_t bwords;   // words per bitmap
     // Heavy rows keep their label histograms across rounds: hh[hid(v)][l]
@@ -69,7 +69,7 @@ struct P2Job {
     const int32_t *hid;  // vertex -> heavy row index of the graph, -1 for light rows
     uint32_t *cbits;
     uint16_t *ON;
-    uint8_t *N8;         // new label this round (255: unchanged)
+    uint16_t *LN;        // label byte | this round's new label byte << 8 (255: unchanged)
     // Light rows keep their label counts too (when lh_on): row v of lh holds
     // kLpCap / 2 words, the count of label l < min(k, 32) in the 16-bit half
     // l % 2 of word l / 2.  Built by the initial sweep; after each round's
@@ -108,9 +108,9 @@ struct Phase2Work {
     int k = 0;
     int64_t cap_heavy = -1, cap_chunks = -1;  // heavy capacities the lists were sized for
     int32_t epoch = 0;  // stamp value of the current candidate pass
-    DevMem chg, diff, stamp, bstamp, bbits, deferred;
+    DevMem chg, diff, stamp, rdeg, bbits, deferred;
     DevMem L8;  // byte mirror of the labels for the LP gathers
-    DevMem hh, cbits, ON, N8;
+    DevMem hh, cbits, ON, LN;
     DevMem lh;  // uint32 [n][kLpCap / 2]: kept light-row label counts
     List B, D;          // boundary list (B_{t-1} read, then B_t written), Delta
     DevMem rtab;        // per-round constants: [0..R] 1 + log(t), [R+1..2R+1] exp(-t/5)
@@ -143,7 +143,7 @@ void phase2_round_table(dmolp_ctx *ctx, Phase2Work &w, int R);
 // stamp epoch).  `remaining` is the set's counter of unfinished jobs.
 P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, double epsilon,
                  int32_t *remaining);
-SelArgs phase2_sel_args(dmolp_ctx *ctx, Phase2Work &w);
+SelArgs phase2_sel_args(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w);
 // g as a job set of one, uploaded; returns the set view.
 P2Set phase2_single_set(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds,
                         double epsilon);

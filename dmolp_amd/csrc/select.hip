@@ -95,6 +95,9 @@ __device__ __forceinline__ SelArgs gload(const SelArgs &v) {
     r.rule.ratio_tab = gload(v.rule.ratio_tab);
     r.rule.t_ptr = gload(v.rule.t_ptr);
     r.rule.re = gload(v.rule.re);
+    r.rdeg = gload(v.rdeg);
+    r.rlab = gload(v.rlab);
+    r.bits = gload(v.bits);
     return r;
 }
 
@@ -117,7 +120,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(const SelArgs *A, con
     const int nb = a.groups * b1_bins;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) s_h[i] = 0;
     __syncthreads();
-    const int64_t count = (int64_t)*in.cnt;
+    const bool dense = a.dense != 0;
+    const int64_t count = dense ? a.rn : (int64_t)*in.cnt;
     constexpr int kH1Items = 4;  // independent record loads in flight per thread
     const int64_t step = vthreads(vg);
     for (int64_t i0 = vthread(vg); i0 < count; i0 += kH1Items * step) {
@@ -125,8 +129,13 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(const SelArgs *A, con
 #pragma unroll
         for (int j = 0; j < kH1Items; ++j) {
             const int64_t i = i0 + j * step;
-            gg[j] = i < count ? in.g[i] : -1;
-            dd[j] = i < count ? in.d[i] : 0;
+            if (dense) {
+                dd[j] = i < count ? a.rdeg[i] : -1;
+                gg[j] = dd[j] >= 0 ? a.rlab[i] : -1;
+            } else {
+                gg[j] = i < count ? in.g[i] : -1;
+                dd[j] = i < count ? in.d[i] : 0;
+            }
         }
 #pragma unroll
         for (int j = 0; j < kH1Items; ++j)
@@ -224,7 +233,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, c
     const SelState *st = a.st;
     sel_lite_load(s_st, st, a.groups);
     const int b1_bins = a.b1_bins;
-    const int64_t count = (int64_t)*in.cnt;
+    const bool dense = a.dense != 0;
+    const int64_t count = dense ? a.rn : (int64_t)*in.cnt;
     const int64_t span = (int64_t)blockDim.x * kCompactItems;
     for (int64_t b0 = vg.block * span; b0 < count; b0 += vg.blocks * span) {
         int32_t x[kCompactItems], gg[kCompactItems], d[kCompactItems];
@@ -235,11 +245,17 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, c
             const int64_t i = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             x[j] = 0; gg[j] = 0; d[j] = 0;
             if (i >= count) continue;
-            gg[j] = in.g[i];
-            d[j] = in.d[i];
+            if (dense) {
+                d[j] = a.rdeg[i];
+                if (d[j] < 0) continue;
+                gg[j] = a.rlab[i];
+            } else {
+                gg[j] = in.g[i];
+                d[j] = in.d[i];
+            }
             const SelLite s = sel_get(s_st, st, gg[j]);
             if (s.mode == SEL_REFINE && bin1_of(d[j], b1_bins) == s.b1) {
-                x[j] = in.x[i];
+                x[j] = dense ? (int32_t)i : in.x[i];
                 keep_mask |= 1u << j;
                 ++nk;
             }
@@ -328,7 +344,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_pick2(const SelArgs *A, con
 __global__ __launch_bounds__(kSelThreads) void k_sel_compact(const SelArgs *A, const int32_t *bstart,
                                                              int jobs) {
     SEL_JOB();
-    if (a.done && *a.done) return;
+    if ((a.done && *a.done) || a.dense) return;
     int32_t *mark = a.mark;
     const int32_t mark_val = mark ? *a.mark_val : 0;
     const RecDev in = a.in, carry = a.carry;
@@ -413,6 +429,90 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(const SelArgs *A, c
     }
 }
 
+// Dense records: one wave per 64 consecutive ids and item; the wave's take
+// decisions (light and heavy) become two words of the B bitmap (plain stores: every word is
+// written by exactly one wave), heavy rows get their slots, and the counts
+// are summed per block and added once per block.
+constexpr int kDenseItems = 4;  // 64-id groups per wave per iteration
+__global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs *A,
+                                                                   const int32_t *bstart, int jobs) {
+    SEL_JOB();
+    if ((a.done && *a.done) || !a.dense) return;
+    const ListDev sel = a.sel;
+    const SelState *st = a.st;
+    const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
+    const int heavy_degree = a.heavy_degree, chunk_edges = a.chunk_edges;
+    const int64_t rn = a.rn, bwords = a.bwords;
+    const int32_t *rdeg = a.rdeg, *rlab = a.rlab;
+    uint32_t *half = a.bits + (int64_t)(*a.rule.t_ptr & 1) * bwords;
+    __shared__ SelLite s_st[kSelLdsGroups];
+    __shared__ unsigned long long s_sum[2];
+    if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
+    sel_lite_load(s_st, st, a.groups);
+    const int lane = lane_id();
+    const int64_t ng = (rn + kWave - 1) / kWave;
+    int64_t n_light = 0, deg_sum = 0;
+    for (int64_t g0 = vwave(vg) * kDenseItems; g0 < ng; g0 += vwaves(vg) * kDenseItems) {
+        int32_t d[kDenseItems], gg[kDenseItems];
+#pragma unroll
+        for (int j = 0; j < kDenseItems; ++j) {
+            const int64_t v = (g0 + j) * kWave + lane;
+            d[j] = v < rn ? rdeg[v] : -1;
+            gg[j] = d[j] >= 0 ? rlab[v] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kDenseItems; ++j) {
+            const int64_t gi = g0 + j;
+            if (gi >= ng) break;  // wave-uniform
+            const int64_t v = gi * kWave + lane;
+            bool take = false;
+            if (d[j] >= 0) {
+                const SelLite s = sel_get(s_st, st, gg[j]);
+                const int b = bin1_of(d[j], b1_bins);
+                const bool earlier = s.desc ? b > s.b1 : b < s.b1;
+                switch (s.mode) {
+                    case SEL_ALL: take = true; break;
+                    case SEL_THROUGH: take = earlier || b == s.b1; break;
+                    case SEL_KEY:
+                        take = earlier ||
+                               (b == s.b1 && sel_key((int32_t)v, d[j], s.desc, id_bits, deg_bits) <= s.kmax);
+                        break;
+                    default: break;
+                }
+            }
+            const bool light = take && d[j] <= heavy_degree;
+            const uint64_t m = __ballot(take);  // the bitmap holds heavy members too
+            if (lane == 0) half[2 * gi] = (uint32_t)m;
+            if (lane == 1 && 2 * gi + 1 < bwords) half[2 * gi + 1] = (uint32_t)(m >> 32);
+            if (take) {
+                deg_sum += d[j];
+                if (light) {
+                    ++n_light;
+                } else {  // heavy rows are rare: one slot each
+                    const int64_t slot = (int64_t)atomicAdd(&sel.cnt[LC_SLOTS], 1ull);
+                    const int64_t nch = ((int64_t)d[j] + chunk_edges - 1) / chunk_edges;
+                    const int64_t base = (int64_t)atomicAdd(&sel.cnt[LC_CHUNKS], (unsigned long long)nch);
+                    sel.hv[slot] = (int32_t)v;
+                    sel.hn[slot] = (int32_t)nch;
+                    atomicAdd(&sel.cnt[LC_HDEG], (unsigned long long)d[j]);
+                    for (int64_t c = 0; c < nch; ++c) sel.hc[base + c] = (slot << 32) | c;
+                }
+            }
+        }
+    }
+    n_light = wave_sum(n_light);
+    deg_sum = wave_sum(deg_sum);
+    if (lane == 0) {
+        if (n_light) atomicAdd(&s_sum[0], (unsigned long long)n_light);
+        if (deg_sum) atomicAdd(&s_sum[1], (unsigned long long)deg_sum);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_sum[0]) atomicAdd(&sel.cnt[LC_LIGHT], s_sum[0]);
+        if (s_sum[1]) atomicAdd(&sel.cnt[LC_DEG], s_sum[1]);
+    }
+}
+
 #undef SEL_JOB
 
 }  // namespace
@@ -465,7 +565,18 @@ SelArgs select_args(dmolp_ctx *ctx, SelectWork &w, const RecDev &in, const TakeR
     a.deg_bits = w.deg_bits;
     a.heavy_degree = ctx->tune.heavy_degree;
     a.chunk_edges = ctx->tune.chunk_edges;
+    a.dense = 0;
     return a;
+}
+
+void select_args_dense(SelArgs &a, const int32_t *rdeg, const int32_t *rlab, int64_t rn,
+                       uint32_t *bits, int64_t bwords) {
+    a.dense = 1;
+    a.rdeg = rdeg;
+    a.rlab = rlab;
+    a.rn = rn;
+    a.bits = bits;
+    a.bwords = bwords;
 }
 
 void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, int passes) {
@@ -484,6 +595,8 @@ void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, in
                            ls.bs(P_GROUPS), J);
     }
     hipLaunchKernelGGL(k_sel_compact, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                       ls.bs(P_GRID), J);
+    hipLaunchKernelGGL(k_sel_compact_dense, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_GRID), J);
     DMOLP_HIP_CHECK(hipGetLastError());
 }

@@ -101,12 +101,25 @@ struct SelArgs {
     const int32_t *mark_val;
     TakeRule rule;
     int32_t groups, b1_bins, id_bits, deg_bits;
-    int32_t has_carry, heavy_degree, chunk_edges, reserved;
+    int32_t has_carry, heavy_degree, chunk_edges, dense;
+    // Dense records (dense != 0): record v exists for 0 <= v < rn iff
+    // rdeg[v] >= 0, with id v, group rlab[v] and degree rdeg[v]; `in` is
+    // unused.  The selected vertices then go to half (*rule.t_ptr & 1) of
+    // the double bitmap `bits` (bwords words per half), every word of it
+    // written; sel gets the heavy slots and the counts, not the light list.
+    const int32_t *rdeg;
+    const int32_t *rlab;
+    int64_t rn;
+    uint32_t *bits;
+    int64_t bwords;
 };
 
 SelArgs select_args(dmolp_ctx *ctx, SelectWork &w, const RecDev &in, const TakeRule &rule,
                     const int32_t *done, const ListDev &sel, const RecDev *carry,
                     int32_t *mark = nullptr, const int32_t *mark_val = nullptr);
+// Switch a job's selection to dense records (see SelArgs::dense).
+void select_args_dense(SelArgs &a, const int32_t *rdeg, const int32_t *rlab, int64_t rn,
+                       uint32_t *bits, int64_t bwords);
 
 // Enqueue the whole selection for a set of jobs (args: device array, one
 // SelArgs per job): level-1 histogram, pick, digit passes, compaction.
