@@ -73,8 +73,9 @@ __device__ __forceinline__ int32_t nb_label(bool exact8, const uint8_t *L8, cons
 }
 
 // Add each thread's x to *dst with one atomic per block (block-uniform
-// call, after the kernel's loops).  Many blocks adding to one counter once
-// per loop iteration would serialise on that counter's cache line.
+// call, after the kernel's loops).  Atomics on one cache line serialise
+// (about 12 ns each on MI355X): one per block, not one per wave or per loop
+// iteration.
 __device__ __forceinline__ void block_count_add(unsigned long long *dst, int64_t x) {
     __shared__ unsigned long long s_acc;
     if (threadIdx.x == 0) s_acc = 0;
@@ -83,6 +84,11 @@ __device__ __forceinline__ void block_count_add(unsigned long long *dst, int64_t
     if (lane_id() == 0 && x) atomicAdd(&s_acc, (unsigned long long)x);
     __syncthreads();
     if (threadIdx.x == 0 && s_acc) atomicAdd(dst, s_acc);
+}
+
+// Word q of lane `lane`'s packed LDS counter row.
+__device__ __forceinline__ uint32_t r_word(const uint32_t *h, int lane, int q) {
+    return h[lane * kPackedStride + q];
 }
 
 // LN entry of a vertex deciding to move from own to nl (byte mirrors).
@@ -158,6 +164,9 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
     uint32_t *lh = J.lh;
     const int E = k < kLpCap ? k : kLpCap;
     const int nvec = (E + 7) / 8;  // 16-byte vectors of a row that hold labels below E
+    // With kept histograms and every label counted (k <= 32), the row's
+    // differing, intra and invalid counts follow from its histogram.
+    const bool from_hist = lh_on && k <= kLpCap;
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ int32_t s_cnt[kWavesPerBlock][kWave];
     __shared__ uint32_t s_h[kWavesPerBlock][kWave * kPackedStride];
@@ -184,6 +193,11 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
                 const int32_t vv = s_b[w].v[slot];
                 const int32_t u = g.col[e];
                 const int32_t b = L8[u];
+                if (from_hist) {  // one LDS count per entry; the rest per row below
+                    if (b < E) atomicAdd(&h[slot * kPackedStride + (b >> 1)], 1u << ((b & 1) * 16));
+                    else atomicAdd(&s_cnt[w][slot], b == kLabel8None ? 1 : 0x10000);
+                    continue;
+                }
                 const int32_t lv = L[vv], lu = exact8 ? (b == kLabel8None ? -1 : b) : L[u];
                 if (lh_on && b < E) atomicAdd(&h[slot * kPackedStride + (b >> 1)], 1u << ((b & 1) * 16));
                 if (lv == -1 || lu == -1) {
@@ -198,7 +212,6 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
         }
         wave_sync();
         if (valid) {
-            diff[v] = s_cnt[w][lane];
             const int32_t lv = L[v];
             if (lv >= 0 && lv < k) label_add(s_vc, vc, lv, 1ull);
             else if (lv != -1) atomicAdd(&ctl->v[C_KERNEL_ERR], 1ull);
@@ -208,11 +221,33 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
                 for (int q = 0; q < nvec; ++q)
                     dst[q] = make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
             }
+            if (from_hist) {
+                // The row's counts from its histogram: `same` entries share
+                // v's label, `inv` have label -1 (byte 255), the rest differ.
+                // Labels in [k, 255) are counted as differing, as the entry
+                // loop would; their vertices raise the kernel error anyway.
+                const int32_t oc = s_cnt[w][lane];
+                const int32_t inv = oc & 0xffff;
+                if (lv == -1) {
+                    invalid += deg;
+                    diff[v] = 0;
+                } else {
+                    const int32_t same =
+                        (lv >= 0 && lv < E) ? (int32_t)((r_word(h, lane, lv >> 1) >> ((lv & 1) * 16)) & 0xffffu) : 0;
+                    const int32_t dif = deg - inv - same;
+                    invalid += inv;
+                    cut2 += dif;
+                    diff[v] = dif;
+                    if (same && lv >= 0 && lv < k) label_add(s_ic, ic, lv, (unsigned long long)same);
+                }
+            } else {
+                diff[v] = s_cnt[w][lane];
+            }
         }
         wave_sync();
     }
-    wave_add(&ctl->v[C_INVALID], invalid);
-    wave_add(&ctl->v[C_CUT2], cut2);
+    block_count_add(&ctl->v[C_INVALID], invalid);
+    block_count_add(&ctl->v[C_CUT2], cut2);
     __syncthreads();
     label_flush(s_vc, vc, k);
     label_flush(s_ic, ic, k);
@@ -275,8 +310,8 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
             }
         }
     }
-    wave_add(&ctl->v[C_INVALID], invalid);
-    wave_add(&ctl->v[C_CUT2], cut2);
+    block_count_add(&ctl->v[C_INVALID], invalid);
+    block_count_add(&ctl->v[C_CUT2], cut2);
 }
 
 __global__ void k_init_finish(const P2Job *JS, const int32_t *bstart, int jobs) {
@@ -629,7 +664,7 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(const P2Job *JS, const 
         }
         wave_sync();
     }
-    wave_add(&ctl->v[C_CAND], cand);
+    block_count_add(&ctl->v[C_CAND], cand);
     block_count_add(bd.cnt, nrec);
 }
 
@@ -662,7 +697,7 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
             touch(J, diff_ok, on, on ? g.col[e] : 0, stamp, epoch, L, exact8, cand, nrec);
         }
     }
-    wave_add(&ctl->v[C_CAND], cand);
+    block_count_add(&ctl->v[C_CAND], cand);
     block_count_add(bd.cnt, nrec);
 }
 
@@ -1205,7 +1240,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
         }
         wave_sync();
     }
-    wave_add(&ctl->v[C_CUT], cut_delta);
+    block_count_add(&ctl->v[C_CUT], cut_delta);
     __syncthreads();
     label_flush(s_vc, vc, k);
     label_flush(s_ic, ic, k);
@@ -1268,7 +1303,7 @@ __global__ __launch_bounds__(kThreads) void k_delta_heavy(const P2Job *JS, const
             }
         }
     }
-    wave_add(&ctl->v[C_CUT], cut_delta);
+    block_count_add(&ctl->v[C_CUT], cut_delta);
     __syncthreads();
     label_flush(s_ic, ic, k);
 }
