@@ -36,6 +36,7 @@ constexpr int kHeavyUnroll = 8;  // heavy rows: gathers in flight per lane
 constexpr int32_t kLabel8None = 255;  // byte-mirror value of labels the LP cannot see
 constexpr int kScanItems = 4;         // vertices per thread per block iteration in scans
 constexpr int kCandFirst = 2;         // neighbours per row examined in the joint first pass
+constexpr int kLaneScan = 8;          // further neighbours per row scanned lane by lane
 // Up to this k every valid label fits the byte mirror exactly (255 stands
 // for -1 and out-of-range labels), so the sweeps gather bytes, not words.
 constexpr int kExact8 = 254;
@@ -523,37 +524,48 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
         bool hit[kScanItems], dif[kScanItems];
         // The items' first neighbours are examined together (independent
         // loads in flight), then undecided rows continue one by one.
+        // Loads in stages, each stage's loads for all items issued together
+        // (a branch on one item's load before the next item's loads would
+        // serialise them).
+        bool heavy_v[kScanItems], cand[kScanItems];
+        int32_t hn[kScanItems];
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
-            deg[j] = 0;
-            lab[j] = 0;
-            xs[j] = 0;
-            lim[j] = 0;
-            end[j] = 0;
-            hit[j] = true;
-            dif[j] = true;
-            if (v >= g.n) continue;
-            const bool heavy_v = use_diff && bit_of(J.hbits, (int32_t)v);
-            if (use_diff && !heavy_v && diff[v] == 0) continue;  // light interior: never a record
-            const int64_t s = g.off[v], e = g.off[v + 1];
-            const int32_t lv = L[v];
-            bool known = use_diff && !heavy_v;
-            if (heavy_v && J.k <= kLpCap) {  // every label counted in the kept histogram
-                if ((int64_t)J.hh[(int64_t)J.hid[v] * kLpCap + lv] == e - s) continue;
+            const bool in = v < g.n;
+            const int32_t dv = in && use_diff ? diff[v] : 1;
+            heavy_v[j] = in && use_diff && bit_of(J.hbits, (int32_t)v);
+            cand[j] = in && (heavy_v[j] || dv != 0);  // light interior: never a record
+        }
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            xs[j] = cand[j] ? g.off[v] : 0;
+            end[j] = cand[j] ? g.off[v + 1] : 0;
+            lab[j] = cand[j] ? L[v] : 0;
+            hn[j] = cand[j] ? J.hubnb[v] : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            deg[j] = (int32_t)(end[j] - xs[j]);
+            bool known = use_diff && !heavy_v[j];
+            if (cand[j] && heavy_v[j] && J.k <= kLpCap) {  // every label counted in the kept histogram
+                if ((int64_t)J.hh[(int64_t)J.hid[v] * kLpCap + lab[j]] == end[j] - xs[j]) cand[j] = false;
                 known = true;
             }
+            if (!cand[j]) {
+                deg[j] = 0;
+                xs[j] = lim[j] = end[j] = 0;
+                hit[j] = dif[j] = true;
+                continue;
+            }
             live |= 1u << j;
-            deg[j] = (int32_t)(e - s);
-            lab[j] = lv;
             // v itself, then its neighbour of largest degree (usually in
             // B_{t-1}), before any scan of the row.
-            const int32_t hn = J.hubnb[v];
-            hit[j] = bit_of(inb, (int32_t)v) || (hn >= 0 && bit_of(inb, hn));
+            hit[j] = bit_of(inb, (int32_t)v) || (hn[j] >= 0 && bit_of(inb, hn[j]));
             dif[j] = known;
-            xs[j] = s;
-            end[j] = e;
-            lim[j] = e - s > heavy_degree ? s + heavy_degree : e;
+            lim[j] = end[j] - xs[j] > heavy_degree ? xs[j] + heavy_degree : end[j];
         }
 #pragma unroll
         for (int r = 0; r < kCandFirst; ++r) {
@@ -569,13 +581,48 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
                 ++xs[j];
             }
         }
+        // A few more entries lane by lane; rows still undecided are then
+        // scanned by the whole wave, 64 entries per step, so one long row
+        // does not hold its wave for hundreds of dependent loads.
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
-            for (int64_t x = xs[j]; x < lim[j] && !(hit[j] && dif[j]); ++x) {
+            const int64_t stop = xs[j] + kLaneScan < lim[j] ? xs[j] + kLaneScan : lim[j];
+            int64_t x = xs[j];
+            for (; x < stop && !(hit[j] && dif[j]); ++x) {
                 const int32_t u = g.col[x];
                 if (!dif[j]) dif[j] = nb_label(exact8, L8, L, u) != lab[j];
                 if (!hit[j]) hit[j] = bit_of(inb, u);
             }
+            xs[j] = x;
+        }
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            uint64_t pend = __ballot(((live >> j) & 1u) && !(hit[j] && dif[j]) && xs[j] < lim[j]);
+            while (pend) {
+                const int src = __ffsll((long long)pend) - 1;
+                pend &= pend - 1;
+                const int64_t x0 = __shfl(xs[j], src, kWave), x1 = __shfl(lim[j], src, kWave);
+                const int32_t lv = __shfl(lab[j], src, kWave);
+                bool h = __shfl(hit[j] ? 1 : 0, src, kWave) != 0;
+                bool d = __shfl(dif[j] ? 1 : 0, src, kWave) != 0;
+                int64_t b = x0;
+                for (; b < x1 && !(h && d); b += kWave) {
+                    const int64_t x = b + lane_id();
+                    const int32_t u = x < x1 ? g.col[x] : -1;
+                    const bool dd = u >= 0 && !d && nb_label(exact8, L8, L, u) != lv;
+                    const bool hh = u >= 0 && !h && bit_of(inb, u);
+                    d = d || __ballot(dd) != 0;
+                    h = h || __ballot(hh) != 0;
+                }
+                if (lane_id() == src) {
+                    hit[j] = h;
+                    dif[j] = d;
+                    xs[j] = b < x1 ? b : x1;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             if (!((live >> j) & 1u)) continue;
             // A long row still undecided after heavy_degree entries is
@@ -933,7 +980,10 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
     int64_t dsum = 0;
     // Block-uniform loop, so the Delta append can be aggregated per block.
     for (int64_t b0 = vg.block * span; b0 < n; b0 += vg.blocks * span) {
-        uint32_t cm = 0;
+        uint32_t cm = 0, act = 0;
+        int32_t ownj[kScanItems];
+        // Stage 1 for every item (loads issued together): the group's bitmap
+        // words, then the own labels of the lanes with work.
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t gbase = b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave;
@@ -947,7 +997,17 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
                 }
                 m = bw & ~hw;
             }
-            if (!((m >> lane) & 1ull)) continue;
+            if ((m >> lane) & 1ull) act |= 1u << j;
+        }
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int64_t v = b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave + lane;
+            ownj[j] = (act >> j) & 1u ? L[v] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            if (!((act >> j) & 1u)) continue;
+            const int64_t gbase = b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave;
             const int32_t v = (int32_t)(gbase + lane);
             const uint4 *row = reinterpret_cast<const uint4 *>(lh + (int64_t)v * kPackedWords);
 #pragma unroll
@@ -958,7 +1018,7 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
                 lrow[4 * q + 2] = x.z;
                 lrow[4 * q + 3] = x.w;
             }
-            const int32_t own = L[v];
+            const int32_t own = ownj[j];
             const int32_t nl =
                 (own >= 0 && own < E)
                     ? lp_decide_kept(lrow, E, s_fac, fmax, own)

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_pick1(const SelArgs *A, con
 __global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, const int32_t *bstart,
                                                              int jobs) {
     SEL_JOB();
-    if (a.done && *a.done) return;
+    if ((a.done && *a.done) || a.dense) return;
     if (*a.refining == 0) return;
     __shared__ BlockAppendLds s_app;
     __shared__ SelLite s_st[kSelLdsGroups];
@@ -233,8 +233,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, c
     const SelState *st = a.st;
     sel_lite_load(s_st, st, a.groups);
     const int b1_bins = a.b1_bins;
-    const bool dense = a.dense != 0;
-    const int64_t count = dense ? a.rn : (int64_t)*in.cnt;
+    const int64_t count = (int64_t)*in.cnt;
     const int64_t span = (int64_t)blockDim.x * kCompactItems;
     for (int64_t b0 = vg.block * span; b0 < count; b0 += vg.blocks * span) {
         int32_t x[kCompactItems], gg[kCompactItems], d[kCompactItems];
@@ -245,17 +244,11 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, c
             const int64_t i = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             x[j] = 0; gg[j] = 0; d[j] = 0;
             if (i >= count) continue;
-            if (dense) {
-                d[j] = a.rdeg[i];
-                if (d[j] < 0) continue;
-                gg[j] = a.rlab[i];
-            } else {
-                gg[j] = in.g[i];
-                d[j] = in.d[i];
-            }
+            gg[j] = in.g[i];
+            d[j] = in.d[i];
             const SelLite s = sel_get(s_st, st, gg[j]);
             if (s.mode == SEL_REFINE && bin1_of(d[j], b1_bins) == s.b1) {
-                x[j] = dense ? (int32_t)i : in.x[i];
+                x[j] = in.x[i];
                 keep_mask |= 1u << j;
                 ++nk;
             }
@@ -270,6 +263,63 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect(const SelArgs *A, c
             ++pos;
         }
     }
+}
+
+// Dense records: the refine list is staged in LDS (ids only) and flushed
+// with one reservation per kRefStage - span or more entries; the flush
+// re-reads the staged records' group and degree (recently read lines).
+constexpr int kRefStage = 8192;
+__device__ __forceinline__ void ref_flush(const SelArgs &a, const int32_t *s_x, int32_t &s_n,
+                                          unsigned long long &s_base) {
+    const int32_t cnt = s_n;
+    if (threadIdx.x == 0) s_base = atomicAdd(a.ref.cnt, (unsigned long long)cnt);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const int32_t x = s_x[i];
+        const unsigned long long p = s_base + (unsigned long long)i;
+        a.ref.x[p] = x;
+        a.ref.g[p] = a.rlab[x];
+        a.ref.d[p] = a.rdeg[x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+}
+__global__ __launch_bounds__(kSelThreads) void k_sel_collect_dense(const SelArgs *A,
+                                                                   const int32_t *bstart, int jobs) {
+    SEL_JOB();
+    if ((a.done && *a.done) || !a.dense) return;
+    if (*a.refining == 0) return;
+    __shared__ SelLite s_st[kSelLdsGroups];
+    __shared__ int32_t s_x[kRefStage];
+    __shared__ int32_t s_n;
+    __shared__ unsigned long long s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    const SelState *st = a.st;
+    sel_lite_load(s_st, st, a.groups);
+    const int b1_bins = a.b1_bins;
+    const int64_t count = a.rn;
+    const int64_t span = (int64_t)blockDim.x * kCompactItems;
+    for (int64_t b0 = vg.block * span; b0 < count; b0 += vg.blocks * span) {
+        int32_t d[kCompactItems], gg[kCompactItems];
+#pragma unroll
+        for (int j = 0; j < kCompactItems; ++j) {
+            const int64_t i = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            d[j] = i < count ? a.rdeg[i] : -1;
+            gg[j] = d[j] >= 0 ? a.rlab[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kCompactItems; ++j) {
+            if (d[j] < 0) continue;
+            const SelLite s = sel_get(s_st, st, gg[j]);
+            if (s.mode == SEL_REFINE && bin1_of(d[j], b1_bins) == s.b1)
+                s_x[atomicAdd(&s_n, 1)] = (int32_t)(b0 + (int64_t)j * blockDim.x + threadIdx.x);
+        }
+        __syncthreads();
+        if (s_n > kRefStage - (int)span) ref_flush(a, s_x, s_n, s_base);
+    }
+    __syncthreads();
+    if (s_n > 0) ref_flush(a, s_x, s_n, s_base);
 }
 
 // Digit histogram of the records still being refined.  With few enough
@@ -587,6 +637,8 @@ void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, in
     hipLaunchKernelGGL(k_sel_pick1, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_GROUPS), J);
     hipLaunchKernelGGL(k_sel_collect, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                       ls.bs(P_GRID), J);
+    hipLaunchKernelGGL(k_sel_collect_dense, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_GRID), J);
     for (int p = 0; p < passes; ++p) {
         hipLaunchKernelGGL(k_sel_hist2, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
