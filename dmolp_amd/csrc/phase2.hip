@@ -118,6 +118,7 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.rdeg = gload(v.rdeg);
     r.bbits = gload(v.bbits);
     r.hh = gload(v.hh);
+    r.hhc = gload(v.hhc);
     r.hid = gload(v.hid);
     r.cbits = gload(v.cbits);
     r.ON = gload(v.ON);
@@ -1207,16 +1208,16 @@ __device__ __forceinline__ void lh_move(uint32_t *lh, int32_t u, int32_t a, int3
     if (da) atomicAdd(&row[a >> 1], 0u - ma);
     if (dc) atomicAdd(&row[c >> 1], mc);
 }
-__device__ __forceinline__ void hh_move(const int32_t *hid, int32_t *hh, int32_t u, int32_t a, int32_t c,
-                                        int E) {
-    int32_t *row = hh + (int64_t)hid[u] * kLpCap;
+__device__ __forceinline__ void hh_move(const int32_t *hid, int32_t *hhc, int64_t nheavy, int32_t u,
+                                        int32_t a, int32_t c, int E) {
+    int32_t *row = hhc + ((int64_t)(blockIdx.x % kHhCopies) * nheavy + hid[u]) * kLpCap;
     if (a >= 0 && a < E) atomicAdd(&row[a], -1);
     if (c >= 0 && c < E) atomicAdd(&row[c], 1);
 }
 __device__ __forceinline__ void row_upkeep(const P2Job &J, bool lh_on, bool hh_push, bool u_heavy,
                                            int32_t u, int32_t a, int32_t c, int E) {
     if (u_heavy) {
-        if (hh_push) hh_move(J.hid, J.hh, u, a, c, E);
+        if (hh_push) hh_move(J.hid, J.hhc, (int64_t)J.all.cnt[LC_SLOTS], u, a, c, E);
     } else if (lh_on) {
         lh_move(J.lh, u, a, c, E);
     }
@@ -1465,6 +1466,23 @@ __global__ void k_hist_mode(const P2Job *JS, const int32_t *bstart, int jobs) {
                      : (chg_deg <= J.push_share * nnz ? HIST_PUSH : HIST_SCAN);
 }
 
+// Push mode: add the copies of this round's moves into hh and clear them.
+__global__ void k_hist_fold(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_PUSH) return;
+    const int64_t nhh = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
+    for (int64_t i = vthread(vg); i < nhh; i += vthreads(vg)) {
+        int32_t sum = 0;
+#pragma unroll
+        for (int c = 0; c < kHhCopies; ++c) {
+            int32_t *p = &J.hhc[(int64_t)c * nhh + i];
+            sum += *p;
+            *p = 0;
+        }
+        if (sum) J.hh[i] += sum;
+    }
+}
+
 __global__ void k_hist_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     if (is_done(J.ctl) || J.ctl->hist_mode != HIST_RECOUNT) return;
@@ -1657,6 +1675,7 @@ __global__ void k_init_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
     for (int64_t i = vthread(vg); i < J.bwords; i += vthreads(vg)) J.cbits[i] = 0u;
     const int64_t nhh = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
     for (int64_t i = vthread(vg); i < nhh; i += vthreads(vg)) J.hh[i] = 0;
+    for (int64_t i = vthread(vg); i < kHhCopies * nhh; i += vthreads(vg)) J.hhc[i] = 0;
     if (vg.block == 0) {
         uint32_t *c = reinterpret_cast<uint32_t *>(J.ctl);
         for (int i = threadIdx.x; i < (int)(sizeof(Ctl) / 4); i += blockDim.x) c[i] = 0;
@@ -1742,6 +1761,7 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.D.alloc(n, g->n_heavy, g->n_heavy_chunks);
         w.hub_hist.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1) * kLpCap);
         w.hh.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1) * kLpCap);
+        w.hhc.alloc(sizeof(int32_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1) * kLpCap * kHhCopies);
         w.deferred.alloc(sizeof(int64_t) * (size_t)(g->n_heavy > 0 ? g->n_heavy : 1));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.hub_hist.p, 0, w.hub_hist.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.B.cnt.p, 0, w.B.cnt.bytes, ctx->stream));
@@ -1792,6 +1812,7 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.bbits = w.bbits.as<uint32_t>();
     j.bwords = (g->n + 31) / 32;
     j.hh = w.hh.as<int32_t>();
+    j.hhc = w.hhc.as<int32_t>();
     j.hid = g->hid.as<int32_t>();
     j.cbits = w.cbits.as<uint32_t>();
     j.ON = w.ON.as<uint16_t>();
@@ -1948,7 +1969,8 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
         hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m5 = mark();
-    if (set.any_heavy) {  // recount or scan modes bring the heavy histograms up to date
+    if (set.any_heavy) {  // fold the pushed moves, or recount, or scan
+        hipLaunchKernelGGL(k_hist_fold, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_zero, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_recount, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_update, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);

@@ -28,6 +28,8 @@ enum CandMode : int32_t { CAND_ALL = 0, CAND_PULL = 1, CAND_PUSH = 2 };
 // against the changed-vertex bitmap, recount them, or push the changes.
 enum HistMode : int32_t { HIST_SCAN = 0, HIST_RECOUNT = 1, HIST_PUSH = 2 };
 
+constexpr int kHhCopies = 8;
+
 struct Ctl {
     unsigned long long v[C_NCTL];
     int32_t done;
@@ -66,6 +68,10 @@ struct P2Job {
     // round from the vertices that changed (bit set in cbits; old and new
     // byte labels in ON as old | new << 8).
     int32_t *hh;
+    // Push-mode moves go to kHhCopies copies of hh (copy blockIdx % copies):
+    // the highest-degree rows take thousands of moves a round, and atomics
+    // on one cache line serialise.  k_hist_fold adds the copies into hh.
+    int32_t *hhc;
     const int32_t *hid;  // vertex -> heavy row index of the graph, -1 for light rows
     uint32_t *cbits;
     uint16_t *ON;
@@ -110,7 +116,7 @@ struct Phase2Work {
     int32_t epoch = 0;  // stamp value of the current candidate pass
     DevMem chg, diff, stamp, rdeg, bbits, deferred;
     DevMem L8;  // byte mirror of the labels for the LP gathers
-    DevMem hh, cbits, ON, LN;
+    DevMem hh, hhc, cbits, ON, LN;
     DevMem lh;  // uint32 [n][kLpCap / 2]: kept light-row label counts
     List B, D;          // boundary list (B_{t-1} read, then B_t written), Delta
     DevMem rtab;        // per-round constants: [0..R] 1 + log(t), [R+1..2R+1] exp(-t/5)
