@@ -280,6 +280,59 @@ def run_served(a):
     return out
 
 
+def kernel_models(g, k, tr, per_round):
+    """Algorithmic bytes of the hot kernels, per launch (one launch per round,
+    the initial sweep once per step), from the step's trace, next to their
+    measured per-launch times.  Offsets o = 8, columns c = 4, labels l = 4;
+    a kept light-row histogram is 16 bytes per 8 labels below min(k, 32).
+    Also returns the step priced with the reference algorithm's bytes."""
+    o, c, l = O_BYTES, C_BYTES, L_BYTES
+    E = min(k, 32)
+    row_b = 16 * ((E + 7) // 8)
+    h_rows, h_entries = g.bins()
+    n, nnz = g.n, g.nnz
+    light_rows, light_entries = n - h_rows, nnz - h_entries
+    dl_n = tr["chg_light_count"].astype(np.float64)
+    dl_deg = tr["chg_light_deg"].astype(np.float64)
+    bl_n = tr["b_light_count"].astype(np.float64)
+    bd = tr["bd_count"].astype(np.float64)
+    pulled = tr["cand_count"] < 0
+    kern = {
+        # Per changed light row: offsets, old label, new label, its differing
+        # count; per entry: the column, the neighbour's old|new label pair
+        # (2 bytes) and one word of the neighbour's kept row read and written.
+        "k_delta_light": (per_round["delta_light"], (2 * o + 3 * l) * dl_n + (c + 2 + 8) * dl_deg),
+        # Per light row of B_t: its kept row and own label; the B_t and heavy
+        # bitmaps (n/4 bytes); per changed row: new label, label pair,
+        # offsets (its degree) and its Delta entry.
+        "k_lp_light_kept": (per_round["lp_light"],
+                            (row_b + l) * bl_n + n / 4 + (l + 2 + 2 * o + 4) * dl_n),
+        # Pull rounds: per vertex its differing count and record entry; per
+        # boundary vertex offsets, label and largest-degree neighbour
+        # (scanned entries not counted).  The group holds the round start and
+        # the other candidate kernels, which do no work in pull rounds.
+        "k_cand_pull": (per_round["candidates"],
+                        np.where(pulled, (2 * l) * n + (2 * o + 2 * l) * bd, 0.0)),
+        # Per light row: offset, label, differing count and kept row written;
+        # per light entry: the column and a label byte.
+        "k_init_light": (per_round["init_light"],
+                         np.array([(o + 2 * l + row_b) * light_rows + (c + 1) * light_entries],
+                                  np.float64)),
+    }
+    # The reference algorithm's bytes for the whole step: candidates pushed
+    # from B_{t-1}, the boundary test reading at least one entry per
+    # candidate, selection keys, recounted LP histograms, incremental
+    # statistics and the initial sweep.
+    s0 = (n + 1) * o + (c + l) * nnz + l * n
+    cand = np.maximum(tr["cand_count"], tr["bd_count"]).astype(np.float64)
+    rounds = (2 * o * tr["prev_count"] + c * tr["prev_deg"]
+              + (2 * o + l + (c + l)) * cand
+              + 8 * bd
+              + (2 * o + 2 * l) * tr["b_count"] + (c + l) * tr["b_deg"]
+              + 2 * o * tr["chg_count"] + (c + l) * tr["chg_deg"])
+    return kern, float(s0 + rounds.astype(np.float64).sum())
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if one exists."""
@@ -353,7 +406,8 @@ def main(argv=None):
     g.checkpoint_labels(restore=True)
     pr = g.phase2(k, R, eps, want_labels=False, want_trace=True)
     ctx.set_profiling(False)
-    names = ["lp_light", "lp_heavy", "candidates", "select", "delta_apply", "round", "hist_update"]
+    names = ["lp_light", "lp_heavy", "candidates", "select", "delta_apply", "round", "hist_update",
+             "delta_light", "init_light"]
     per_round = {nm: ctx.round_times(i) for i, nm in enumerate(names)}
     groups = {nm: float(v.sum()) for nm, v in per_round.items()}
     tr = pr.trace
@@ -364,22 +418,15 @@ def main(argv=None):
         f"changed entries {tr['chg_deg'].tolist()}")
     log("[bench] per round ms: " + json.dumps(
         {nm: [round(float(x), 3) for x in v] for nm, v in per_round.items()}))
-    unit = 2 * O_BYTES + 2 * L_BYTES
-    # The heavy rows' LP histograms are kept across rounds: after each round
-    # k_hist_update streams every heavy row's columns (4 B per entry) and
-    # tests one bit per entry in the n-bit changed-vertex bitmap.
-    h_rows, h_entries = g.bins()
-    hist_bytes = np.full(len(per_round["hist_update"]), C_BYTES * h_entries + (g.n + 7) // 8, np.float64)
-    kern = {
-        "k_lp_light": (per_round["lp_light"], unit * light_n + (C_BYTES + L_BYTES) * light_deg),
-        "k_hist_update": (per_round["hist_update"], hist_bytes),
-    }
+    kern, p2eq = kernel_models(g, k, tr, per_round)
     dom = max(kern, key=lambda kk: float(kern[kk][0].sum()))
     ms_r, bytes_r = kern[dom]
     live = bytes_r > 0
-    achieved = float(bytes_r.sum()) / (float(ms_r.sum()) / 1e3) / 1e9
+    achieved = float(bytes_r[live].sum()) / (float(ms_r[live].sum()) / 1e3) / 1e9
+    p2eq_gbs = p2eq / (float(pr.metrics.device_ms) / 1e3) / 1e9
     log(f"[bench] profiled step: groups ms "
-        f"{json.dumps({kk: round(v, 3) for kk, v in groups.items()})}; dominant {dom}")
+        f"{json.dumps({kk: round(v, 3) for kk, v in groups.items()})}; dominant {dom}; "
+        f"kernel ms {json.dumps({kk: round(float(v[0].sum()), 3) for kk, v in kern.items()})}")
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -419,6 +466,12 @@ def main(argv=None):
                      "algorithmic_bytes_per_launch": float(bytes_r[live].mean()),
                      "avg_launch_ms": float(ms_r[live].mean()),
                      "launches": int(live.sum())},
+        # The whole phase-2 step priced with the reference algorithm's bytes
+        # (recounted LP histograms, candidates pushed from B_{t-1}): what a
+        # streaming implementation of that algorithm would have to move.
+        "phase2_equivalent": {"bytes_per_step": p2eq, "achieved": p2eq_gbs, "unit": "GB/s",
+                              "frac": p2eq_gbs / HBM_PEAK_GBS,
+                              "device_ms": float(pr.metrics.device_ms)},
         "cpu_baseline": cpu,
         "kernel_groups_ms": groups,
     }

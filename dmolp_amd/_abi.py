@@ -50,7 +50,8 @@ class Phase2Metrics(C.Structure):
 class Trace(C.Structure):
     _fields_ = [(name, P(i64)) for name in (
         "prev_count", "prev_deg", "cand_count", "bd_count", "b_count", "b_deg",
-        "chg_count", "chg_deg", "cut", "b_light_count", "b_light_deg")]
+        "chg_count", "chg_deg", "cut", "b_light_count", "b_light_deg", "chg_light_count",
+        "chg_light_deg")]
 
 
 class Job(C.Structure):

@@ -224,7 +224,7 @@ int dmolp_ctx_set_profiling(dmolp_ctx *ctx, int on) {
 }
 
 int dmolp_ctx_round_times(dmolp_ctx *ctx, int group, double *ms_out, int cap, int *n_out) {
-    if (!ctx || group < 0 || group > DMOLP_PROF_HIST || cap < 0 || (cap > 0 && !ms_out))
+    if (!ctx || group < 0 || group > DMOLP_PROF_INIT_LIGHT || cap < 0 || (cap > 0 && !ms_out))
         return fail(ctx, DMOLP_E_ARG, "bad arguments");
     const std::vector<double> empty;
     const std::vector<double> &v =
@@ -576,10 +576,11 @@ int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, 
             int64_t *cols[] = {trace->prev_count, trace->prev_deg, trace->cand_count,
                                trace->bd_count,   trace->b_count,  trace->b_deg,
                                trace->chg_count,  trace->chg_deg,  trace->cut,
-                               trace->b_light_count, trace->b_light_deg};
+                               trace->b_light_count, trace->b_light_deg,
+                               trace->chg_light_count, trace->chg_light_deg};
             const int map[] = {T_PREV_N, T_PREV_DEG, T_CAND, T_BD, T_B, T_BDEG, T_CHG, T_CHGDEG, T_CUT,
-                               T_BL, T_BLDEG};
-            for (int j = 0; j < 11; ++j) {
+                               T_BL, T_BLDEG, T_DL, T_DLDEG};
+            for (int j = 0; j < 13; ++j) {
                 if (!cols[j]) continue;
                 for (int t = 0; t < R; ++t) cols[j][t] = tr[(size_t)t * T_NCOL + map[j]];
             }

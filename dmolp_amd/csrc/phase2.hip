@@ -1661,6 +1661,8 @@ __global__ void k_round_end(const P2Job *JS, const int32_t *bstart, int jobs) {
         r[T_CUT] = cut_rec;
         r[T_BL] = (int64_t)cur.cnt[LC_LIGHT];
         r[T_BLDEG] = (int64_t)(cur.cnt[LC_DEG] - cur.cnt[LC_HDEG]);
+        r[T_DL] = nb == 0 ? 0 : (int64_t)D.cnt[LC_LIGHT];
+        r[T_DLDEG] = nb == 0 ? 0 : (int64_t)(D.cnt[LC_DEG] - D.cnt[LC_HDEG]);
     }
     if (!ctl->done) ctl->t = t + 1;
     else atomicSub(J.remaining, 1);
@@ -1909,8 +1911,18 @@ void phase2_init_enqueue(dmolp_ctx *ctx, const P2Set &set) {
                        ls.bs(P_GRID), J);
     hipLaunchKernelGGL(k_mirror8, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), J);
+    size_t i0 = 0;
+    if (ctx->profiling) {
+        DMOLP_HIP_CHECK(hipEventRecord(ctx->prof_event(), s));
+        i0 = ctx->ev_used - 1;
+    }
     hipLaunchKernelGGL(k_init_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), J);
+    if (ctx->profiling) {  // group 8: the initial light-row sweep, once per run
+        DMOLP_HIP_CHECK(hipEventRecord(ctx->prof_event(), s));
+        if (ctx->prof_marks.size() < 9) ctx->prof_marks.resize(9);
+        ctx->prof_marks[8].push_back({i0, ctx->ev_used - 1});
+    }
     if (set.any_heavy)
         hipLaunchKernelGGL(k_init_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                            ls.bs(P_GRID), J);
@@ -1965,6 +1977,7 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     }
     const size_t m4 = mark();
     hipLaunchKernelGGL(k_delta_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    const size_t m4b = mark();
     if (set.any_heavy)
         hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
@@ -1982,11 +1995,11 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     if (ctx->profiling) {
         const size_t m7 = mark();
         // Groups: 0 lp_light, 1 lp_heavy, 2 candidates, 3 select, 4 delta+apply,
-        // 5 round, 6 heavy-row histogram update.
-        const std::pair<size_t, size_t> spans[7] = {{m2, m3}, {m3, m4}, {m0, m1}, {m1, m2},
-                                                    {m4, m5}, {m0, m7}, {m5, m6}};
-        if (ctx->prof_marks.size() < 7) ctx->prof_marks.resize(7);
-        for (int i = 0; i < 7; ++i) ctx->prof_marks[(size_t)i].push_back(spans[i]);
+        // 5 round, 6 heavy-row histogram update, 7 k_delta_light alone.
+        const std::pair<size_t, size_t> spans[8] = {{m2, m3}, {m3, m4}, {m0, m1}, {m1, m2},
+                                                    {m4, m5}, {m0, m7}, {m5, m6}, {m4, m4b}};
+        if (ctx->prof_marks.size() < 9) ctx->prof_marks.resize(9);
+        for (int i = 0; i < 8; ++i) ctx->prof_marks[(size_t)i].push_back(spans[i]);
     }
 }
 

@@ -49,7 +49,8 @@ struct Ctl {
 
 // Trace columns written by the round-end kernel.
 enum TraceCol {
-    T_PREV_N = 0, T_PREV_DEG, T_CAND, T_BD, T_B, T_BDEG, T_CHG, T_CHGDEG, T_CUT, T_BL, T_BLDEG, T_NCOL
+    T_PREV_N = 0, T_PREV_DEG, T_CAND, T_BD, T_B, T_BDEG, T_CHG, T_CHGDEG, T_CUT, T_BL, T_BLDEG,
+    T_DL, T_DLDEG, T_NCOL
 };
 
 // Everything one graph's phase-2 kernels read and write (device pointers).

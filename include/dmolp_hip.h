@@ -82,7 +82,8 @@ void *dmolp_ctx_stream(dmolp_ctx *ctx);
  * in the reference, which times whole GPU calls with std::chrono,
  * phase2.cpp:324-339).  group: 0 light-row LP decision, 1 heavy-row LP
  * decision, 2 round start + candidates, 3 selection, 4 delta statistics +
- * apply, 5 whole round, 6 heavy-row histogram update. */
+ * apply, 5 whole round, 6 heavy-row histogram update, 7 light-row delta
+ * kernel, 8 initial light-row sweep (once per run). */
 #define DMOLP_PROF_LP_LIGHT 0
 #define DMOLP_PROF_LP_HEAVY 1
 #define DMOLP_PROF_CANDIDATES 2
@@ -90,6 +91,8 @@ void *dmolp_ctx_stream(dmolp_ctx *ctx);
 #define DMOLP_PROF_DELTA 4
 #define DMOLP_PROF_ROUND 5
 #define DMOLP_PROF_HIST 6   /* heavy-row histogram update after the apply */
+#define DMOLP_PROF_DELTA_LIGHT 7  /* the light-row Delta kernel alone */
+#define DMOLP_PROF_INIT_LIGHT 8   /* the initial light-row sweep (one entry per run) */
 int dmolp_ctx_set_profiling(dmolp_ctx *ctx, int on);
 int dmolp_ctx_round_times(dmolp_ctx *ctx, int group, double *ms_out, int cap, int *n_out);
 
@@ -241,6 +244,8 @@ typedef struct {
     int64_t *cut;        /* cut after round t, -1 if the round stopped before stats */
     int64_t *b_light_count; /* entries of B_t processed by the light-row kernel */
     int64_t *b_light_deg;   /* their degree sum */
+    int64_t *chg_light_count; /* entries of Delta_t processed by the light-row kernels */
+    int64_t *chg_light_deg;   /* their degree sum */
 } dmolp_trace;
 
 /* Replaces run_phase2 (phase2.cpp:256-449) with max_iter and epsilon as
