@@ -93,49 +93,6 @@ __global__ void k_heavy_ids(ListDev all, int32_t *hid, uint32_t *hbits) {
     }
 }
 
-// Each row's neighbour of largest degree (smallest id among equals): the
-// neighbour most likely to sit in a boundary set, tested first by the pull
-// candidate pass.  Light rows one per thread, heavy rows one per wave.
-__device__ __forceinline__ bool hub_better(int64_t da, int32_t a, int64_t db, int32_t b) {
-    return da > db || (da == db && a < b);
-}
-__global__ void k_hubnb_light(GraphDev g, int heavy_degree, int32_t *hn) {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
-         v += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = g.off[v], e = g.off[v + 1];
-        if (e - s > heavy_degree) continue;
-        int32_t best = -1;
-        int64_t bd = -1;
-        for (int64_t x = s; x < e; ++x) {
-            const int32_t u = g.col[x];
-            const int64_t du = g.off[u + 1] - g.off[u];
-            if (hub_better(du, u, bd, best)) { best = u; bd = du; }
-        }
-        hn[v] = best;
-    }
-}
-__global__ void k_hubnb_heavy(GraphDev g, ListDev all, int32_t *hn) {
-    const int64_t ns = (int64_t)all.cnt[LC_SLOTS];
-    for (int64_t j = global_wave(); j < ns; j += wave_count()) {
-        const int32_t v = all.hv[j];
-        const int64_t s = g.off[v], e = g.off[v + 1];
-        int32_t best = -1;
-        int64_t bd = -1;
-        for (int64_t x = s + lane_id(); x < e; x += kWave) {
-            const int32_t u = g.col[x];
-            const int64_t du = g.off[u + 1] - g.off[u];
-            if (hub_better(du, u, bd, best)) { best = u; bd = du; }
-        }
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            const int32_t ob = __shfl_xor(best, o, kWave);
-            const int64_t od = __shfl_xor(bd, o, kWave);
-            if (hub_better(od, ob, bd, best)) { best = ob; bd = od; }
-        }
-        if (lane_id() == 0) hn[v] = best;
-    }
-}
-
 __global__ void k_count_heavy(GraphDev g, int heavy_degree, int chunk_edges,
                               unsigned long long *acc) {
     unsigned long long nh = 0, nc = 0;
@@ -197,19 +154,6 @@ void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g) {
     DMOLP_HIP_CHECK(hipGetLastError());
     g->all_heavy_degree = H;
     g->all_chunk_edges = CH;
-}
-
-void graph_prepare_hubnb(dmolp_ctx *ctx, dmolp_graph *g) {
-    if (g->hubnb_valid) return;
-    g->hubnb.grow(sizeof(int32_t) * (size_t)(g->n > 0 ? g->n : 1));
-    if (g->n > 0) {
-        hipLaunchKernelGGL(k_hubnb_light, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, g->dev(),
-                           g->all_heavy_degree, g->hubnb.as<int32_t>());
-        hipLaunchKernelGGL(k_hubnb_heavy, dim3(ctx->cu_count * 4), dim3(256), 0, ctx->stream, g->dev(),
-                           g->all.dev, g->hubnb.as<int32_t>());
-        DMOLP_HIP_CHECK(hipGetLastError());
-    }
-    g->hubnb_valid = true;
 }
 
 void graph_widen_offsets(dmolp_ctx *ctx, const int32_t *d_src, int64_t *d_dst, int64_t count) {

@@ -96,7 +96,7 @@ struct List {
 };
 
 struct Tuning {
-    int heavy_degree = 1024;
+    int heavy_degree = 2048;  // rows longer than this are heavy (kept int32 histograms, chunked)
     int chunk_edges = 4096;
     int block_threads = 256;
     int grid_blocks = 0;  // 0: derived from the CU count
@@ -199,8 +199,6 @@ struct dmolp_graph {
     int64_t n_heavy = 0, n_heavy_chunks = 0;
     dmolp::DevMem hid;    // int32[n]: heavy slot of the all list, -1 for light rows
     dmolp::DevMem hbits;  // uint32[(n + 31) / 32]: bit v set for the heavy rows
-    dmolp::DevMem hubnb;  // int32[n]: v's neighbour of largest degree (-1: none)
-    bool hubnb_valid = false;
     std::unique_ptr<dmolp::Phase2Work> p2;
     std::unique_ptr<dmolp::Phase1Work> p1;
     dmolp::GraphDev dev() const {
@@ -212,7 +210,6 @@ struct dmolp_graph {
 namespace dmolp {
 // Implemented in the .hip files.
 void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g);  // builds g->all for ctx->tune
-void graph_prepare_hubnb(dmolp_ctx *ctx, dmolp_graph *g);  // g->hubnb (after the bins)
 void graph_validate(dmolp_ctx *ctx, dmolp_graph *g);       // offsets, sorted rows, no self-loops
 int64_t bits_for(int64_t v);                                // bits needed to hold 0..v
 }  // namespace dmolp

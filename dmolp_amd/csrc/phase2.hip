@@ -125,7 +125,6 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.LN = gload(v.LN);
     r.lh = gload(v.lh);
     r.hbits = gload(v.hbits);
-    r.hubnb = gload(v.hubnb);
     r.bd = gload(v.bd);
     r.ctl = gload(v.ctl);
     r.vc = gload(v.vc);
@@ -529,7 +528,6 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
         // (a branch on one item's load before the next item's loads would
         // serialise them).
         bool heavy_v[kScanItems], cand[kScanItems];
-        int32_t hn[kScanItems];
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
@@ -544,7 +542,6 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
             xs[j] = cand[j] ? g.off[v] : 0;
             end[j] = cand[j] ? g.off[v + 1] : 0;
             lab[j] = cand[j] ? L[v] : 0;
-            hn[j] = cand[j] ? J.hubnb[v] : -1;
         }
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
@@ -562,9 +559,7 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
                 continue;
             }
             live |= 1u << j;
-            // v itself, then its neighbour of largest degree (usually in
-            // B_{t-1}), before any scan of the row.
-            hit[j] = bit_of(inb, (int32_t)v) || (hn[j] >= 0 && bit_of(inb, hn[j]));
+            hit[j] = bit_of(inb, (int32_t)v);  // v itself, before any scan of the row
             dif[j] = known;
             lim[j] = end[j] - xs[j] > heavy_degree ? xs[j] + heavy_degree : end[j];
         }
@@ -1736,7 +1731,6 @@ __global__ void k_p2_gather_done(const P2Job *JS, int jobs, int32_t *out) {
 Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
     DMOLP_REQUIRE(k >= 1 && k <= DMOLP_MAX_K, DMOLP_E_ARG, "k outside [1, 1024]");
     graph_prepare_bins(ctx, g);
-    graph_prepare_hubnb(ctx, g);
     if (!g->p2) g->p2.reset(new Phase2Work());
     Phase2Work &w = *g->p2;
     const int64_t n = g->n;
@@ -1822,7 +1816,6 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.lh_on = ctx->tune.light_hist ? 1 : 0;
     j.lh = j.lh_on ? w.lh.as<uint32_t>() : nullptr;
     j.hbits = g->hbits.as<uint32_t>();
-    j.hubnb = g->hubnb.as<int32_t>();
     j.deferred = w.deferred.as<int64_t>();
     j.hub_hist = w.hub_hist.as<int32_t>();
     j.bd = w.bd.dev;

@@ -84,7 +84,6 @@ struct P2Job {
     // neighbour's row from its old label to its new one.
     uint32_t *lh;
     const uint32_t *hbits;  // heavy-row bitmap of the graph
-    const int32_t *hubnb;   // each vertex's neighbour of largest degree (-1: none)
     RecDev bd;
     Ctl *ctl;
     unsigned long long *vc, *ic;
