@@ -69,9 +69,11 @@ struct DevMem {
 // are "light" and processed 64 list entries per wave; longer rows are
 // "heavy": they get a slot and are split into chunks of `chunk_edges`
 // edges, one chunk per wave.
-// LC_FILL: fill cursor of a light list rebuilt from a bitmap (k_b_list).
+// LC_FILL / LC_HFILL: fill cursors of light ids / heavy slots rebuilt from
+// a bitmap (k_b_list).
 enum ListCount {
-    LC_LIGHT = 0, LC_SLOTS = 1, LC_CHUNKS = 2, LC_DEG = 3, LC_HDEG = 4, LC_FILL = 5, LC_N = 6
+    LC_LIGHT = 0, LC_SLOTS = 1, LC_CHUNKS = 2, LC_DEG = 3, LC_HDEG = 4, LC_FILL = 5, LC_HFILL = 6,
+    LC_N = 7
 };
 struct ListDev {
     int32_t *light;            // light vertex ids

@@ -714,7 +714,6 @@ __global__ __launch_bounds__(kThreads) void k_cand_light(const P2Job *JS, const 
 __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
-    const ListDev prev = J.B;
     const int32_t *L = J.L;
     const bool exact8 = J.k <= kExact8;
     int32_t *stamp = J.stamp;
@@ -726,12 +725,16 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
     const int32_t epoch = ctl->epoch;
     const int lane = lane_id();
     int64_t cand = 0, nrec = 0;
-    const int64_t nch = (int64_t)prev.cnt[LC_CHUNKS];
+    // Every chunk of the graph's heavy rows; rows in B_{t-1} (bitmap) push.
+    const uint32_t *prevb = J.bbits + (int64_t)((ctl->t - 1) & 1) * J.bwords;
+    const ListDev all = J.all;
+    const int64_t nch = (int64_t)all.cnt[LC_CHUNKS];
     for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
-        const int64_t item = prev.hc[j];
-        chunk_bounds(g, prev, item, chunk_edges, slot, v, e0, e1, deg);
+        const int64_t item = all.hc[j];
+        chunk_bounds(g, all, item, chunk_edges, slot, v, e0, e1, deg);
+        if (!bit_of(prevb, v)) continue;  // wave-uniform
         touch(J, diff_ok, lane == 0 && (item & 0xffffffffll) == 0, v, stamp, epoch, L, exact8, cand,
               nrec);
         for (int64_t eb = e0; eb < e1; eb += kWave) {
@@ -1414,7 +1417,6 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
 __global__ void k_lp_heavy_hist(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const int32_t *L = J.L;
-    const ListDev B = J.B;
     const int k = J.k;
     int32_t *chg = J.chg;
     uint16_t *LN = J.LN;
@@ -1425,14 +1427,17 @@ __global__ void k_lp_heavy_hist(const P2Job *JS, const int32_t *bstart, int jobs
     const int E = k < kLpCap ? k : kLpCap;
     if ((int)threadIdx.x < E) s_pen[threadIdx.x] = J.pen[threadIdx.x];
     __syncthreads();
-    const int64_t ns = (int64_t)B.cnt[LC_SLOTS];
+    // Every heavy row of the graph; those in B_t (bitmap) decide.
+    const uint32_t *bt = J.bbits + (int64_t)(J.ctl->t & 1) * J.bwords;
+    const ListDev all = J.all;
+    const int64_t ns = (int64_t)all.cnt[LC_SLOTS];
     for (int64_t b0 = vwave(vg) * kWave; b0 < ns; b0 += vwaves(vg) * kWave) {
         const int64_t slot = b0 + lane_id();
         bool changed = false;
         int32_t v = 0;
         int64_t deg = 0;
-        if (slot < ns) {
-            v = B.hv[slot];
+        if (slot < ns && bit_of(bt, all.hv[slot])) {
+            v = all.hv[slot];
             deg = off[v + 1] - off[v];
             const int32_t *h = J.hh + (int64_t)J.hid[v] * kLpCap;
             const int32_t own = L[v];
@@ -1700,7 +1705,8 @@ __global__ void k_rdeg_clear(const P2Job *JS, const int32_t *bstart, int jobs) {
 // gathering LP (after the selection, shift 0: B_t) and push candidates
 // (at round start, shift 1: B_{t-1}; only_push: skip unless this round
 // pushes).  Heavy rows keep their slots from the selection.
-__global__ void k_b_list(const P2Job *JS, const int32_t *bstart, int jobs, int shift, int only_push) {
+__global__ void k_b_list(const P2Job *JS, const int32_t *bstart, int jobs, int shift, int only_push,
+                         int heavy) {
     P2_JOB();
     const Ctl *ctl = J.ctl;
     if (is_done(ctl) || (only_push && ctl->cand_mode != CAND_PUSH)) return;
@@ -1710,12 +1716,39 @@ __global__ void k_b_list(const P2Job *JS, const int32_t *bstart, int jobs, int s
     const int lane = lane_id();
     for (int64_t base = vwave(vg) * kWave; base < n; base += vwaves(vg) * kWave) {
         const int64_t wi = base >> 5;
-        uint64_t m = half[wi] & ~J.hbits[wi];
-        if (wi + 1 < bwords) m |= (uint64_t)(half[wi + 1] & ~J.hbits[wi + 1]) << 32;
-        if (!m) continue;  // wave-uniform
-        const bool on = (m >> lane) & 1ull;
-        const int64_t pos = wave_append(on, &B.cnt[LC_FILL]);
-        if (on) B.light[pos] = (int32_t)(base + lane);
+        uint64_t m = half[wi] & ~J.hbits[wi], mh = half[wi] & J.hbits[wi];
+        if (wi + 1 < bwords) {
+            m |= (uint64_t)(half[wi + 1] & ~J.hbits[wi + 1]) << 32;
+            mh |= (uint64_t)(half[wi + 1] & J.hbits[wi + 1]) << 32;
+        }
+        if (m) {  // wave-uniform
+            const bool on = (m >> lane) & 1ull;
+            const int64_t pos = wave_append(on, &B.cnt[LC_FILL]);
+            if (on) B.light[pos] = (int32_t)(base + lane);
+        }
+        if (heavy && mh) {  // heavy members too (slots only, for a caller reading the list)
+            const bool on = (mh >> lane) & 1ull;
+            const int64_t pos = wave_append(on, &B.cnt[LC_HFILL]);
+            if (on) B.hv[pos] = (int32_t)(base + lane);
+        }
+    }
+}
+
+// A boundary list given as a list (light ids and heavy slots) into half
+// ((t - shift) & 1) of the bitmap, for the test hooks.
+__global__ void k_list_bits(const P2Job *JS, const int32_t *bstart, int jobs, int shift) {
+    P2_JOB();
+    uint32_t *half = J.bbits + (int64_t)((J.ctl->t - shift) & 1) * J.bwords;
+    for (int64_t i = vthread(vg); i < J.bwords; i += vthreads(vg)) half[i] = 0u;
+}
+__global__ void k_list_bits_set(const P2Job *JS, const int32_t *bstart, int jobs, int shift) {
+    P2_JOB();
+    const ListDev B = J.B;
+    uint32_t *half = J.bbits + (int64_t)((J.ctl->t - shift) & 1) * J.bwords;
+    const int64_t nl = (int64_t)B.cnt[LC_LIGHT], ns = (int64_t)B.cnt[LC_SLOTS];
+    for (int64_t i = vthread(vg); i < nl + ns; i += vthreads(vg)) {
+        const int32_t v = i < nl ? B.light[i] : B.hv[i - nl];
+        atomicOr(&half[v >> 5], 1u << (v & 31));
     }
 }
 
@@ -1947,14 +1980,14 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     if (set.any_heavy)
         hipLaunchKernelGGL(k_cand_pull_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     hipLaunchKernelGGL(k_rdeg_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
-    hipLaunchKernelGGL(k_b_list, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 1, 1);  // B_{t-1} list
+    hipLaunchKernelGGL(k_b_list, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 1, 1, 0);  // B_{t-1} list
     hipLaunchKernelGGL(k_cand_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     if (set.any_heavy)
         hipLaunchKernelGGL(k_cand_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m1 = mark();
     select_enqueue(ctx, set.sel, ls, set.passes);  // B_t: bitmap, heavy slots, counts
     if (!set.lh_on)  // the gathering LP walks B_t as a list
-        hipLaunchKernelGGL(k_b_list, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 0, 0);
+        hipLaunchKernelGGL(k_b_list, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 0, 0, 0);
     const size_t m2 = mark();
     if (set.lh_on)
         hipLaunchKernelGGL(k_lp_light_kept, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
@@ -2074,6 +2107,11 @@ void phase2_select_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t)
     const LaunchSet &ls = *set.ls;
     hipLaunchKernelGGL(k_hook_state, dim3(1), dim3(1), 0, s, ctl, t, w.epoch, (int32_t)CAND_PUSH);
     DMOLP_HIP_CHECK(hipMemsetAsync(w.bd.cnt.p, 0, w.bd.cnt.bytes, s));
+    // B_{t-1} arrives as a list: its bitmap for the heavy-row push.
+    hipLaunchKernelGGL(k_list_bits, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), ls.jobs, 1);
+    hipLaunchKernelGGL(k_list_bits_set, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), ls.jobs, 1);
     hipLaunchKernelGGL(k_rdeg_clear, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), ls.jobs);
     hipLaunchKernelGGL(k_cand_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
@@ -2083,7 +2121,7 @@ void phase2_select_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t)
                            ls.bs(P_GRID), ls.jobs);
     select_enqueue(ctx, set.sel, ls, set.passes);
     hipLaunchKernelGGL(k_b_list, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
-                       ls.bs(P_GRID), ls.jobs, 0, 0);
+                       ls.bs(P_GRID), ls.jobs, 0, 0, 1);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 

@@ -480,9 +480,11 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(const SelArgs *A, c
 }
 
 // Dense records: one wave per 64 consecutive ids and item; the wave's take
-// decisions (light and heavy) become two words of the B bitmap (plain stores: every word is
-// written by exactly one wave), heavy rows get their slots, and the counts
-// are summed per block and added once per block.
+// decisions (light and heavy) become two words of the B bitmap (plain stores:
+// every word is written by exactly one wave), and the counts (light and
+// heavy members, their degree sums) are summed per block and added once per
+// block.  Heavy members get no list slots: their kernels walk the graph's
+// heavy rows and test the bitmap.
 constexpr int kDenseItems = 4;  // 64-id groups per wave per iteration
 __global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs *A,
                                                                    const int32_t *bstart, int jobs) {
@@ -491,17 +493,17 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs
     const ListDev sel = a.sel;
     const SelState *st = a.st;
     const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
-    const int heavy_degree = a.heavy_degree, chunk_edges = a.chunk_edges;
+    const int heavy_degree = a.heavy_degree;
     const int64_t rn = a.rn, bwords = a.bwords;
     const int32_t *rdeg = a.rdeg, *rlab = a.rlab;
     uint32_t *half = a.bits + (int64_t)(*a.rule.t_ptr & 1) * bwords;
     __shared__ SelLite s_st[kSelLdsGroups];
-    __shared__ unsigned long long s_sum[2];
-    if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
+    __shared__ unsigned long long s_sum[4];
+    if (threadIdx.x < 4) s_sum[threadIdx.x] = 0;
     sel_lite_load(s_st, st, a.groups);
     const int lane = lane_id();
     const int64_t ng = (rn + kWave - 1) / kWave;
-    int64_t n_light = 0, deg_sum = 0;
+    int64_t n_light = 0, deg_sum = 0, n_heavy = 0, heavy_deg = 0;
     for (int64_t g0 = vwave(vg) * kDenseItems; g0 < ng; g0 += vwaves(vg) * kDenseItems) {
         int32_t d[kDenseItems], gg[kDenseItems];
 #pragma unroll
@@ -538,28 +540,29 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs
                 deg_sum += d[j];
                 if (light) {
                     ++n_light;
-                } else {  // heavy rows are rare: one slot each
-                    const int64_t slot = (int64_t)atomicAdd(&sel.cnt[LC_SLOTS], 1ull);
-                    const int64_t nch = ((int64_t)d[j] + chunk_edges - 1) / chunk_edges;
-                    const int64_t base = (int64_t)atomicAdd(&sel.cnt[LC_CHUNKS], (unsigned long long)nch);
-                    sel.hv[slot] = (int32_t)v;
-                    sel.hn[slot] = (int32_t)nch;
-                    atomicAdd(&sel.cnt[LC_HDEG], (unsigned long long)d[j]);
-                    for (int64_t c = 0; c < nch; ++c) sel.hc[base + c] = (slot << 32) | c;
+                } else {
+                    ++n_heavy;
+                    heavy_deg += d[j];
                 }
             }
         }
     }
     n_light = wave_sum(n_light);
     deg_sum = wave_sum(deg_sum);
+    n_heavy = wave_sum(n_heavy);
+    heavy_deg = wave_sum(heavy_deg);
     if (lane == 0) {
         if (n_light) atomicAdd(&s_sum[0], (unsigned long long)n_light);
         if (deg_sum) atomicAdd(&s_sum[1], (unsigned long long)deg_sum);
+        if (n_heavy) atomicAdd(&s_sum[2], (unsigned long long)n_heavy);
+        if (heavy_deg) atomicAdd(&s_sum[3], (unsigned long long)heavy_deg);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         if (s_sum[0]) atomicAdd(&sel.cnt[LC_LIGHT], s_sum[0]);
         if (s_sum[1]) atomicAdd(&sel.cnt[LC_DEG], s_sum[1]);
+        if (s_sum[2]) atomicAdd(&sel.cnt[LC_SLOTS], s_sum[2]);
+        if (s_sum[3]) atomicAdd(&sel.cnt[LC_HDEG], s_sum[3]);
     }
 }
 

@@ -106,7 +106,8 @@ struct SelArgs {
     // rdeg[v] >= 0, with id v, group rlab[v] and degree rdeg[v]; `in` is
     // unused.  The selected vertices then go to half (*rule.t_ptr & 1) of
     // the double bitmap `bits` (bwords words per half), every word of it
-    // written; sel gets the heavy slots and the counts, not the light list.
+    // written; sel gets the counts only (LC_LIGHT, LC_DEG, LC_SLOTS as the
+    // heavy member count, LC_HDEG), no lists.
     const int32_t *rdeg;
     const int32_t *rlab;
     int64_t rn;
