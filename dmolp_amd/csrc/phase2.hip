@@ -1779,7 +1779,7 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.cbits.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.LN.p, 0xff, w.LN.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.cbits.p, 0, w.cbits.bytes, ctx->stream));
-        w.bd.alloc(n);
+        w.bd.alloc(1);  // records are dense (rdeg): only bd's counter is used
         DMOLP_HIP_CHECK(hipMemsetAsync(w.stamp.p, 0, w.stamp.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.bbits.p, 0, w.bbits.bytes, ctx->stream));
         DMOLP_HIP_CHECK(hipMemsetAsync(w.chg.p, 0xff, w.chg.bytes, ctx->stream));

@@ -131,7 +131,7 @@ struct Phase2Work {
     ~Phase2Work() {
         if (round_exec) (void)hipGraphExecDestroy(round_exec);
     }
-    Recs bd;
+    Recs bd;  // record counter of the round (the records themselves are dense, in rdeg)
     SelectWork sel;
     DevMem hub_hist, hub_done;
     DevMem ctl;
