@@ -150,6 +150,19 @@ hipEvent_t dmolp_ctx::prof_event() {
     return ev_pool[ev_used++];
 }
 
+namespace dmolp {
+bool round_poll(dmolp_ctx *ctx, const int32_t *d_flag, int t, bool stop_at_zero) {
+    const int slot = t & 1;
+    DMOLP_HIP_CHECK(hipMemcpyAsync(&ctx->poll_host[slot], d_flag, sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipEventRecord(ctx->poll_ev[slot], ctx->stream));
+    if (t < 2) return false;
+    DMOLP_HIP_CHECK(hipEventSynchronize(ctx->poll_ev[slot ^ 1]));
+    const int32_t v = ctx->poll_host[slot ^ 1];
+    return stop_at_zero ? v <= 0 : v != 0;
+}
+}  // namespace dmolp
+
 extern "C" {
 
 int dmolp_abi_version(void) { return DMOLP_ABI_VERSION; }
@@ -192,6 +205,8 @@ int dmolp_ctx_create(int device, dmolp_ctx **out) {
         DMOLP_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         DMOLP_HIP_CHECK(hipEventCreate(&ctx->ev0));
         DMOLP_HIP_CHECK(hipEventCreate(&ctx->ev1));
+        DMOLP_HIP_CHECK(hipHostMalloc((void **)&ctx->poll_host, 64, hipHostMallocDefault));
+        for (hipEvent_t &e : ctx->poll_ev) DMOLP_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     });
     if (rc != DMOLP_OK) {
         g_tls_error = ctx->last_error;
@@ -210,12 +225,16 @@ int dmolp_ctx_destroy(dmolp_ctx *ctx) {
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->poll_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->poll_host) (void)hipHostFree(ctx->poll_host);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DMOLP_OK;
 }
 
 void *dmolp_ctx_stream(dmolp_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
 
 int dmolp_ctx_set_profiling(dmolp_ctx *ctx, int on) {
     if (!ctx) return fail(nullptr, DMOLP_E_ARG, "null context");
@@ -506,14 +525,10 @@ int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, 
         DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, s));
         phase2_init_enqueue(ctx, set);
         Ctl c{};
-        const int batch = 8;
+        const int32_t *d_done = &w.ctl.as<Ctl>()->done;
         for (int t = 1; t <= max_rounds; ++t) {
             phase2_round_launch(ctx, set, &w.round_exec, &w.round_key, key);
-            if (t % batch == 0 && t < max_rounds) {
-                DMOLP_HIP_CHECK(hipMemcpyAsync(&c, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
-                DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-                if (c.done) break;
-            }
+            if (t < max_rounds && round_poll(ctx, d_done, t, false)) break;
         }
         DMOLP_HIP_CHECK(hipEventRecord(ctx->ev1, s));
         std::vector<int64_t> cv((size_t)k), ci((size_t)k);

@@ -159,16 +159,9 @@ void phase2_run_set(dmolp_ctx *ctx, std::vector<P2Item> &items, BatchState &bs) 
     uint64_t key = 0;
     const P2Set set = p2_set_build(ctx, ip, bs, &key);
     phase2_init_enqueue(ctx, set);
-    int32_t left = G;
-    const int check = 8;
     for (int t = 1; t <= Rmax; ++t) {
         phase2_round_launch(ctx, set, &bs.exec, &bs.exec_key, key);
-        if (t % check == 0 && t < Rmax) {
-            DMOLP_HIP_CHECK(hipMemcpyAsync(&left, set.remaining, sizeof(int32_t),
-                                           hipMemcpyDeviceToHost, s));
-            DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-            if (left <= 0) break;
-        }
+        if (t < Rmax && round_poll(ctx, set.remaining, t, true)) break;
     }
     p2_collect(ctx, ip);
 }

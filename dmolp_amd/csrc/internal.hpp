@@ -183,6 +183,10 @@ struct dmolp_ctx {
     int grid_blocks() const { return tune.grid_blocks > 0 ? tune.grid_blocks : cu_count * 8; }
     dmolp::P1SetDev p1_set;   // phase-1 job set of the plain and batched paths
     dmolp::BatchState batch;  // phase-2 job set of dmolp_batch_run
+    // Stop polling of round loops: a device flag copied to pinned memory
+    // behind each round, read one round later (see round_poll).
+    int32_t *poll_host = nullptr;
+    hipEvent_t poll_ev[2] = {nullptr, nullptr};
 };
 
 struct dmolp_graph {
@@ -214,4 +218,9 @@ namespace dmolp {
 void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g);  // builds g->all for ctx->tune
 void graph_validate(dmolp_ctx *ctx, dmolp_graph *g);       // offsets, sorted rows, no self-loops
 int64_t bits_for(int64_t v);                                // bits needed to hold 0..v
+// After enqueueing round t of a loop: queue a copy of the device flag behind
+// it and read the copy queued after round t-1 (the host stays one round
+// ahead, so at most one round runs past the stop).  True when that earlier
+// flag says stop: nonzero (stop_at_zero false) or <= 0 (stop_at_zero true).
+bool round_poll(dmolp_ctx *ctx, const int32_t *d_flag, int t, bool stop_at_zero);
 }  // namespace dmolp
