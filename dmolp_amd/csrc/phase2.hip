@@ -848,29 +848,6 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
     }
 }
 
-// lp_decide_by over a fixed trip count (labels at or above E are skipped),
-// so a row held in registers is indexed with constants only.
-template <class Count>
-__device__ __forceinline__ int32_t lp_decide_unrolled(Count count, int E, const double *pen,
-                                                      int32_t own) {
-    double base = -1e300;
-    int best_l = 0;
-    double best_s = 0.0;
-#pragma unroll
-    for (int l = 0; l < kLpCap; ++l) {
-        if (l < E) {
-            double s = (double)count(l);
-            if (l == own) base = s;
-            if (s > 0.0) s *= (1.0 + pen[l]);
-            if (l == 0 || s > best_s || (s == best_s && bitrev5(l) < bitrev5(best_l))) {
-                best_l = l;
-                best_s = s;
-            }
-        }
-    }
-    return (best_s > base) ? best_l : own;
-}
-
 // lp_decide_by for a row whose own label is counted (0 <= own < E), with
 // fac[l] = 1 + pen[l] (the same double the loop multiplies by) and fmax the
 // largest fac.  The loop's winner is the lexicographic maximum of

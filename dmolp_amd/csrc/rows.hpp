@@ -97,25 +97,6 @@ struct BlockAppendLds {
     unsigned long long base;
 };
 
-__device__ __forceinline__ int64_t block_append(bool pred, unsigned long long *counter,
-                                                BlockAppendLds &lds) {
-    const uint64_t m = __ballot(pred);
-    const int w = threadIdx.x / kWave, nw = blockDim.x / kWave;
-    if (lane_id() == 0) lds.wc[w] = __popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int32_t run = 0;
-        for (int i = 0; i < nw; ++i) { const int32_t c = lds.wc[i]; lds.wc[i] = run; run += c; }
-        lds.base = run ? atomicAdd(counter, (unsigned long long)run) : 0ull;
-    }
-    __syncthreads();
-    const int64_t pos = pred ? (int64_t)(lds.base + (unsigned long long)lds.wc[w] +
-                                         (unsigned long long)lanes_below(m))
-                             : -1;
-    __syncthreads();
-    return pos;
-}
-
 // Reserve my_n consecutive slots of a global list: returns this thread's
 // first slot.  One atomic per block per call (plus one for `sum_counter`,
 // which receives the block's sum of my_sum).  Block-uniform calls only.
