@@ -93,6 +93,14 @@ __global__ void k_heavy_ids(ListDev all, int32_t *hid, uint32_t *hbits) {
     }
 }
 
+__global__ void k_first_nb(GraphDev g, int32_t *first) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = g.off[v];
+        first[v] = s < g.off[v + 1] ? g.col[s] : -1;
+    }
+}
+
 __global__ void k_count_heavy(GraphDev g, int heavy_degree, int chunk_edges,
                               unsigned long long *acc) {
     unsigned long long nh = 0, nc = 0;
@@ -154,6 +162,17 @@ void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g) {
     DMOLP_HIP_CHECK(hipGetLastError());
     g->all_heavy_degree = H;
     g->all_chunk_edges = CH;
+}
+
+void graph_prepare_first(dmolp_ctx *ctx, dmolp_graph *g) {
+    if (g->first_valid) return;
+    g->first.grow(sizeof(int32_t) * (size_t)(g->n > 0 ? g->n : 1));
+    if (g->n > 0) {
+        hipLaunchKernelGGL(k_first_nb, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, g->dev(),
+                           g->first.as<int32_t>());
+        DMOLP_HIP_CHECK(hipGetLastError());
+    }
+    g->first_valid = true;
 }
 
 void graph_widen_offsets(dmolp_ctx *ctx, const int32_t *d_src, int64_t *d_dst, int64_t count) {

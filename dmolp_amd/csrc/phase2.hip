@@ -125,6 +125,7 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.LN = gload(v.LN);
     r.lh = gload(v.lh);
     r.hbits = gload(v.hbits);
+    r.first = gload(v.first);
     r.bd = gload(v.bd);
     r.ctl = gload(v.ctl);
     r.vc = gload(v.vc);
@@ -536,12 +537,14 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
             heavy_v[j] = in && use_diff && bit_of(J.hbits, (int32_t)v);
             cand[j] = in && (heavy_v[j] || dv != 0);  // light interior: never a record
         }
+        int32_t fnb[kScanItems];
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             xs[j] = cand[j] ? g.off[v] : 0;
             end[j] = cand[j] ? g.off[v + 1] : 0;
             lab[j] = cand[j] ? L[v] : 0;
+            fnb[j] = cand[j] ? J.first[v] : -1;
         }
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
@@ -559,7 +562,10 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
                 continue;
             }
             live |= 1u << j;
-            hit[j] = bit_of(inb, (int32_t)v);  // v itself, before any scan of the row
+            // v itself, then its first neighbour (read from a per-vertex
+            // array, not the row: sorted rows start with the smallest id,
+            // in these graphs usually a hub), before any scan of the row.
+            hit[j] = bit_of(inb, (int32_t)v) || (fnb[j] >= 0 && bit_of(inb, fnb[j]));
             dif[j] = known;
             lim[j] = end[j] - xs[j] > heavy_degree ? xs[j] + heavy_degree : end[j];
         }
@@ -1741,6 +1747,7 @@ __global__ void k_p2_gather_done(const P2Job *JS, int jobs, int32_t *out) {
 Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
     DMOLP_REQUIRE(k >= 1 && k <= DMOLP_MAX_K, DMOLP_E_ARG, "k outside [1, 1024]");
     graph_prepare_bins(ctx, g);
+    graph_prepare_first(ctx, g);
     if (!g->p2) g->p2.reset(new Phase2Work());
     Phase2Work &w = *g->p2;
     const int64_t n = g->n;
@@ -1826,6 +1833,7 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.lh_on = ctx->tune.light_hist ? 1 : 0;
     j.lh = j.lh_on ? w.lh.as<uint32_t>() : nullptr;
     j.hbits = g->hbits.as<uint32_t>();
+    j.first = g->first.as<int32_t>();
     j.deferred = w.deferred.as<int64_t>();
     j.hub_hist = w.hub_hist.as<int32_t>();
     j.bd = w.bd.dev;

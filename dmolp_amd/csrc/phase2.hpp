@@ -84,6 +84,7 @@ struct P2Job {
     // neighbour's row from its old label to its new one.
     uint32_t *lh;
     const uint32_t *hbits;  // heavy-row bitmap of the graph
+    const int32_t *first;   // each row's first (smallest) neighbour, -1 if empty
     RecDev bd;
     Ctl *ctl;
     unsigned long long *vc, *ic;
