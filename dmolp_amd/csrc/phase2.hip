@@ -898,7 +898,7 @@ __device__ __forceinline__ int32_t lp_decide_kept(const uint32_t *row, int E, co
 
 // Delta entries staged in LDS by a block: one reservation, a coalesced copy.
 // Block-uniform call.
-constexpr int kDStage = 4096;
+constexpr int kDStage = 2048;
 __device__ __forceinline__ void d_flush(const ListDev &D, const int32_t *s_dq, int32_t &s_dn,
                                         unsigned long long &s_base) {
     const int32_t cnt = s_dn;

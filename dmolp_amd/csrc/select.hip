@@ -661,7 +661,7 @@ void job_blocks(const dmolp_ctx *ctx, int64_t n, int64_t nnz, int groups, int64_
     const int64_t full = ctx->grid_blocks();
     if (single) {
         out[P_GRID] = (int32_t)full;
-        out[P_HIST1] = ctx->cu_count * 2;
+        out[P_HIST1] = ctx->cu_count * 4;
     } else {
         // Blocks in proportion to the job's work, so small graphs of a batch
         // do not hold idle blocks.

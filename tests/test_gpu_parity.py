@@ -241,3 +241,38 @@ def test_phase1_then_phase2_large_k(tuned, k):
     got1 = g.phase1(k, 500)
     assert np.array_equal(want1.labels, got1.labels)
     _check_phase2(g, off, col, k, want1.labels, 10, 0.03)
+
+
+def test_trace_light_parts_and_profile_groups(ctx):
+    """The light parts of Delta_t and B_t in the trace, checked against the
+    labels themselves, and the shape of the per-round kernel-group times."""
+    ctx.set_tuning(**dict(DEFAULT_TUNING, heavy_degree=40))
+    try:
+        off, col = api.gen_host(api.rmat_spec(13, 16, seed=7))
+        deg = np.diff(off)
+        g = ctx.upload(off, col)
+        labels0 = g.phase1(8, 1000).labels.copy()
+        g.checkpoint_labels()
+        r = g.phase2(8, 1, 0.01)
+        changed = np.flatnonzero(r.labels != labels0)
+        light = changed[deg[changed] <= 40]
+        assert r.trace["chg_count"][0] == len(changed)
+        assert r.trace["chg_light_count"][0] == len(light)
+        assert r.trace["chg_light_deg"][0] == int(deg[light].sum())
+        ctx.set_profiling(True)
+        g.checkpoint_labels(restore=True)
+        r = g.phase2(8, 6, 0.01)
+        ctx.set_profiling(False)
+        rounds = r.metrics.rounds
+        for group in range(8):
+            t = ctx.round_times(group)
+            assert len(t) == rounds and np.all(t >= 0), group
+        assert len(ctx.round_times(8)) == 1  # the initial light sweep, once per run
+        # The light Delta kernel is part of the Delta + apply group.
+        assert np.all(ctx.round_times(7) <= ctx.round_times(4) + 1e-3)
+        assert np.all(r.trace["b_light_count"] <= r.trace["b_count"])
+        assert np.all(r.trace["chg_light_count"] <= r.trace["chg_count"])
+        assert np.all(r.trace["chg_light_deg"] <= r.trace["chg_deg"])
+    finally:
+        ctx.set_profiling(False)
+        ctx.set_tuning(**DEFAULT_TUNING)
