@@ -854,6 +854,18 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
     }
 }
 
+// Word i (0..15) of a row held in registers: a select tree, so a
+// lane-dependent index needs no scratch memory.
+__device__ __forceinline__ uint32_t pick16(const uint32_t (&c)[kPackedWords], int i) {
+    const bool b0 = i & 1, b1 = i & 2, b2 = i & 4, b3 = i & 8;
+    const uint32_t a0 = b0 ? c[1] : c[0], a1 = b0 ? c[3] : c[2], a2 = b0 ? c[5] : c[4],
+                   a3 = b0 ? c[7] : c[6], a4 = b0 ? c[9] : c[8], a5 = b0 ? c[11] : c[10],
+                   a6 = b0 ? c[13] : c[12], a7 = b0 ? c[15] : c[14];
+    const uint32_t d0 = b1 ? a1 : a0, d1 = b1 ? a3 : a2, d2 = b1 ? a5 : a4, d3 = b1 ? a7 : a6;
+    const uint32_t e0 = b2 ? d1 : d0, e1 = b2 ? d3 : d2;
+    return b3 ? e1 : e0;
+}
+
 // lp_decide_by for a row whose own label is counted (0 <= own < E), with
 // fac[l] = 1 + pen[l] (the same double the loop multiplies by) and fmax the
 // largest fac.  The loop's winner is the lexicographic maximum of
@@ -862,11 +874,12 @@ __global__ __launch_bounds__(kThreads) void k_lp_light(const P2Job *JS, const in
 // winner is the lexicographic maximum among them.  count * fac <= count *
 // fmax, and rounding is monotone, so a label whose count is at most c_skip
 // (the largest count c with c * fmax rounding to at most base) cannot score
-// above base: only the few larger counts are scored.  Counts are read from
-// the lane's LDS row (16-bit halves, label l in word l / 2).
-__device__ __forceinline__ int32_t lp_decide_kept(const uint32_t *row, int E, const double *fac,
-                                                  double fmax, int32_t own) {
-    const int32_t own_cnt = (int32_t)((row[own >> 1] >> ((own & 1) * 16)) & 0xffffu);
+// above base: only the few larger counts are scored.  row(q) returns word q
+// of the row (16-bit halves, label l in word l / 2).
+template <class Word>
+__device__ __forceinline__ int32_t lp_decide_kept(Word row, int E, const double *fac, double fmax,
+                                                  int32_t own) {
+    const int32_t own_cnt = (int32_t)((row(own >> 1) >> ((own & 1) * 16)) & 0xffffu);
     const double base = (double)own_cnt;
     if (!(fmax > 0.0)) return own;  // every score is at most 0 <= base
     const double q0 = base / fmax;  // a first guess, made exact below
@@ -876,7 +889,7 @@ __device__ __forceinline__ int32_t lp_decide_kept(const uint32_t *row, int E, co
     uint32_t surv = 0;
 #pragma unroll
     for (int q = 0; q < kPackedWords; ++q) {
-        const uint32_t wq = row[q];
+        const uint32_t wq = row(q);
         surv |= ((int32_t)(wq & 0xffffu) > cs ? 1u : 0u) << (2 * q);
         surv |= ((int32_t)(wq >> 16) > cs ? 1u : 0u) << (2 * q + 1);
     }
@@ -886,7 +899,7 @@ __device__ __forceinline__ int32_t lp_decide_kept(const uint32_t *row, int E, co
     while (surv) {
         const int l = __builtin_ctz(surv);
         surv &= surv - 1u;
-        const int32_t cnt = (int32_t)((row[l >> 1] >> ((l & 1) * 16)) & 0xffffu);
+        const int32_t cnt = (int32_t)((row(l >> 1) >> ((l & 1) * 16)) & 0xffffu);
         const double sc = (double)cnt * fac[l];
         if (sc > base && (best_l < 0 || sc > best_s || (sc == best_s && bitrev5(l) < bitrev5(best_l)))) {
             best_l = l;
@@ -936,7 +949,6 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
     const uint32_t *bt = J.bbits + (int64_t)(ctl->t & 1) * bwords;  // B_t
     __shared__ double s_pen[kLpCap], s_fac[kLpCap];
     __shared__ double s_fmax;
-    __shared__ uint32_t s_row[kThreads * kPackedStride];
     const int E = k < kLpCap ? k : kLpCap;
     const int nvec = (E + 7) / 8;
     if ((int)threadIdx.x < E) {
@@ -951,7 +963,6 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
     }
     __syncthreads();
     const double fmax = s_fmax;
-    uint32_t *lrow = &s_row[threadIdx.x * kPackedStride];
     const int w = threadIdx.x / kWave, lane = lane_id();
     const int64_t span = (int64_t)blockDim.x * kScanItems;
     __shared__ int32_t s_dq[kDStage];
@@ -992,19 +1003,21 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
             const int64_t gbase = b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave;
             const int32_t v = (int32_t)(gbase + lane);
             const uint4 *row = reinterpret_cast<const uint4 *>(lh + (int64_t)v * kPackedWords);
+            uint32_t c[kPackedWords];
 #pragma unroll
             for (int q = 0; q < kPackedWords / 4; ++q) {
                 const uint4 x = q < nvec ? row[q] : make_uint4(0u, 0u, 0u, 0u);
-                lrow[4 * q] = x.x;
-                lrow[4 * q + 1] = x.y;
-                lrow[4 * q + 2] = x.z;
-                lrow[4 * q + 3] = x.w;
+                c[4 * q] = x.x;
+                c[4 * q + 1] = x.y;
+                c[4 * q + 2] = x.z;
+                c[4 * q + 3] = x.w;
             }
             const int32_t own = ownj[j];
+            const auto word = [&](int q) { return pick16(c, q); };
             const int32_t nl =
                 (own >= 0 && own < E)
-                    ? lp_decide_kept(lrow, E, s_fac, fmax, own)
-                    : lp_decide_by([&](int l) { return (int32_t)((lrow[l >> 1] >> ((l & 1) * 16)) & 0xffffu); },
+                    ? lp_decide_kept(word, E, s_fac, fmax, own)
+                    : lp_decide_by([&](int l) { return (int32_t)((word(l >> 1) >> ((l & 1) * 16)) & 0xffffu); },
                                    E, s_pen, own);
             if (nl != own) {
                 cm |= 1u << j;
