@@ -33,10 +33,9 @@ constexpr int kHistStride = kLpCap + 1;  // odd stride: no LDS bank conflicts ac
 constexpr int kLdsK = 256;
 constexpr int kLpUnroll = 4;     // light rows: gathers in flight per lane
 constexpr int kHeavyUnroll = 8;  // heavy rows: gathers in flight per lane
+constexpr int kLightUnroll = 4;  // light rows in the initial sweep: gathers in flight per lane
 constexpr int32_t kLabel8None = 255;  // byte-mirror value of labels the LP cannot see
 constexpr int kScanItems = 4;         // vertices per thread per block iteration in scans
-constexpr int kCandFirst = 2;         // neighbours per row examined in the joint first pass
-constexpr int kLaneScan = 8;          // further neighbours per row scanned lane by lane
 // Up to this k every valid label fits the byte mirror exactly (255 stands
 // for -1 and out-of-range labels), so the sweeps gather bytes, not words.
 constexpr int kExact8 = 254;
@@ -126,6 +125,7 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.lh = gload(v.lh);
     r.hbits = gload(v.hbits);
     r.first = gload(v.first);
+    r.wit = gload(v.wit);
     r.bd = gload(v.bd);
     r.ctl = gload(v.ctl);
     r.vc = gload(v.vc);
@@ -187,7 +187,34 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
         if (lh_on)
             for (int l = 0; l < 4 * nvec; ++l) h[lane * kPackedStride + l] = 0u;
         wave_sync();
-        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
+        if (from_hist) {
+            // Counting only: kLightUnroll entries per lane per step, their
+            // slot searches, column loads and label gathers issued together.
+            for (int32_t q0 = 0; q0 < total; q0 += kWave * kLightUnroll) {
+                int slot[kLightUnroll];
+                int32_t u[kLightUnroll];
+#pragma unroll
+                for (int j = 0; j < kLightUnroll; ++j) {
+                    const int32_t q = q0 + j * kWave + lane;
+                    slot[j] = q < total ? batch_slot(s_b[w], q) : -1;
+                }
+#pragma unroll
+                for (int j = 0; j < kLightUnroll; ++j) {
+                    const int32_t q = q0 + j * kWave + lane;
+                    u[j] = slot[j] >= 0 ? g.col[s_b[w].base[slot[j]] + q] : -1;
+                }
+                int32_t b[kLightUnroll];
+#pragma unroll
+                for (int j = 0; j < kLightUnroll; ++j) b[j] = u[j] >= 0 ? (int32_t)L8[u[j]] : -1;
+#pragma unroll
+                for (int j = 0; j < kLightUnroll; ++j) {
+                    if (b[j] < 0) continue;
+                    if (b[j] < E) atomicAdd(&h[slot[j] * kPackedStride + (b[j] >> 1)], 1u << ((b[j] & 1) * 16));
+                    else atomicAdd(&s_cnt[w][slot[j]], b[j] == kLabel8None ? 1 : 0x10000);
+                }
+            }
+        }
+        for (int32_t q0 = 0; q0 < (from_hist ? 0 : total); q0 += kWave) {
             const int32_t q = q0 + lane;
             if (q < total) {
                 const int slot = batch_slot(s_b[w], q);
@@ -195,11 +222,6 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
                 const int32_t vv = s_b[w].v[slot];
                 const int32_t u = g.col[e];
                 const int32_t b = L8[u];
-                if (from_hist) {  // one LDS count per entry; the rest per row below
-                    if (b < E) atomicAdd(&h[slot * kPackedStride + (b >> 1)], 1u << ((b & 1) * 16));
-                    else atomicAdd(&s_cnt[w][slot], b == kLabel8None ? 1 : 0x10000);
-                    continue;
-                }
                 const int32_t lv = L[vv], lu = exact8 ? (b == kLabel8None ? -1 : b) : L[u];
                 if (lh_on && b < E) atomicAdd(&h[slot * kPackedStride + (b >> 1)], 1u << ((b & 1) * 16));
                 if (lv == -1 || lu == -1) {
@@ -285,6 +307,57 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
         const int32_t lv = L[v];
         int64_t dcount = 0, intra = 0;
         for (int l = 0; l < E; ++l) row[l] = 0;
+        if (k <= kLpCap) {
+            // Every label is counted (E = k): the chunk's same, invalid and
+            // differing counts follow from its histogram, so the entry loop
+            // only counts, with kHeavyUnroll gathers in flight per lane.
+            // Labels in [k, 255) count as differing, as the loop below
+            // would; their vertices raise the kernel error anyway.
+            int32_t inv = 0;
+            for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
+                int32_t u[kHeavyUnroll];
+#pragma unroll
+                for (int q = 0; q < kHeavyUnroll; ++q) {
+                    const int64_t e = eb + q * kWave + lane;
+                    u[q] = e < e1 ? __builtin_nontemporal_load(&g.col[e]) : -1;
+                }
+                int32_t lab[kHeavyUnroll];
+#pragma unroll
+                for (int q = 0; q < kHeavyUnroll; ++q) lab[q] = u[q] >= 0 ? (int32_t)L8[u[q]] : -1;
+#pragma unroll
+                for (int q = 0; q < kHeavyUnroll; ++q) {
+                    if (lab[q] >= 0 && lab[q] < E) row[lab[q]] += 1;
+                    inv += lab[q] == kLabel8None ? 1 : 0;
+                }
+            }
+            inv = (int32_t)wave_sum((int64_t)inv);
+            wave_sync();
+            int32_t c = 0;
+            if (lane < E) {
+                for (int r = 0; r < kWave; ++r) c += s_h[w][r * kHistStride + lane];
+                if (c) atomicAdd(&hh[(int64_t)slot * kLpCap + lane], c);
+            }
+            const int32_t same = __shfl(c, lv >= 0 && lv < E ? lv : 0, kWave);
+            wave_sync();
+            if (lane == 0) {
+                const int64_t len = e1 - e0;
+                if (lv == -1) {
+                    invalid += len;
+                } else {
+                    const int32_t s = lv >= 0 && lv < E ? same : 0;
+                    const int64_t d = len - inv - s;
+                    invalid += inv;
+                    cut2 += d;
+                    if (d) atomicAdd(&diff[v], (int32_t)d);
+                    if (s) atomicAdd(&ic[lv], (unsigned long long)s);
+                }
+                if ((item & 0xffffffffll) == 0) {
+                    if (lv >= 0 && lv < k) atomicAdd(&vc[lv], 1ull);
+                    else if (lv != -1) atomicAdd(&ctl->v[C_KERNEL_ERR], 1ull);
+                }
+            }
+            continue;
+        }
         for (int64_t e = e0 + lane; e < e1; e += kWave) {
             const int32_t u = g.col[e];
             const int32_t b = L8[u];
@@ -418,6 +491,7 @@ __global__ void k_round_begin(const P2Job *JS, const int32_t *bstart, int jobs) 
         B.cnt[LC_FILL] = 0;
         ctl->v[C_CAND] = 0;
         ctl->v[C_DEFERRED] = 0;
+        ctl->v[C_PENDING] = 0;
         ctl->cand_mode = t == 1 ? CAND_ALL : ((double)prev_deg >= pull_frac * (double)nnz ? CAND_PULL : CAND_PUSH);
         penalties(vc, ic, k, rtab[t], pen, re);
     }
@@ -496,18 +570,29 @@ __device__ __forceinline__ void touch(const P2Job &J, bool diff_ok, bool pred, i
 
 // Pull: a vertex is a candidate iff it or one of its neighbours was selected
 // in round t-1 (bit set in the B_{t-1} bitmap), and a record iff it is on the
-// boundary.  One pass over the row answers both and stops when both are
-// known.  Rows longer than heavy_degree are scanned here up to heavy_degree
-// entries and finished by k_cand_pull_heavy.
+// boundary.  This pass decides what the vertex itself and its first
+// neighbour answer; rows still undecided are staged in LDS and appended to a
+// pending list that k_cand_pull_rest scans.  Scanning them here would hold
+// nearly every wave: a few undecided rows are spread over all of them.
+constexpr int kPendStage = 2048;
+__device__ __forceinline__ void pend_flush(int32_t *out, unsigned long long *cnt, const int32_t *s_q,
+                                           int32_t &s_n, unsigned long long &s_base) {
+    const int32_t c = s_n;
+    if (threadIdx.x == 0) s_base = atomicAdd(cnt, (unsigned long long)c);
+    __syncthreads();
+    for (int i = threadIdx.x; i < c; i += blockDim.x) out[s_base + (unsigned long long)i] = s_q[i];
+    __syncthreads();
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
     const int32_t *L = J.L;
-    const uint8_t *L8 = J.L8;
     const RecDev bd = J.bd;
     int64_t *deferred = J.deferred;
     const int heavy_degree = J.heavy_degree;
-    const bool exact8 = J.k <= kExact8;
     Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
     const uint32_t *inb = J.bbits + (int64_t)((ctl->t - 1) & 1) * J.bwords;  // B_{t-1}
@@ -516,138 +601,180 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
     const bool use_diff = ctl->diff_ok != 0;
     const int32_t *diff = J.diff;
     int32_t *rdeg = J.rdeg;
+    // Pending rows go to the Delta list, unused until the LP fills it.
+    int32_t *pend = J.D.light;
+    __shared__ int32_t s_q[kPendStage];
+    __shared__ int64_t s_dq[kThreads * kScanItems];  // one block iteration's long rows
+    __shared__ int32_t s_n, s_dn;
+    __shared__ unsigned long long s_base;
+    if (threadIdx.x == 0) { s_n = 0; s_dn = 0; }
+    __syncthreads();
     int64_t nrec_total = 0;
     const int64_t span = (int64_t)blockDim.x * kScanItems;
     for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
-        uint32_t rec = 0, live = 0;
-        int32_t nr = 0, deg[kScanItems], lab[kScanItems];
-        int64_t xs[kScanItems], lim[kScanItems], end[kScanItems];
-        bool hit[kScanItems], dif[kScanItems];
-        // The items' first neighbours are examined together (independent
-        // loads in flight), then undecided rows continue one by one.
-        // Loads in stages, each stage's loads for all items issued together
-        // (a branch on one item's load before the next item's loads would
-        // serialise them).
-        bool heavy_v[kScanItems], cand[kScanItems];
+        uint32_t rec = 0, pen = 0;
+        int32_t deg[kScanItems], fnb[kScanItems];
+        int64_t xs[kScanItems], end[kScanItems];
+        bool heavy_v[kScanItems], cand[kScanItems], known[kScanItems], hit[kScanItems];
+        // Two stages of loads, each stage's loads for all items issued
+        // together (a branch on one item's load before the next item's loads
+        // would serialise them): first every per-vertex word, the row bounds
+        // and witness included even for interior vertices, then the
+        // witness's bit.
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             const bool in = v < g.n;
             const int32_t dv = in && use_diff ? diff[v] : 1;
             heavy_v[j] = in && use_diff && bit_of(J.hbits, (int32_t)v);
+            xs[j] = in ? g.off[v] : 0;
+            end[j] = in ? g.off[v + 1] : 0;
+            fnb[j] = in ? J.wit[v] : -1;
+            hit[j] = in && bit_of(inb, (int32_t)v);
             cand[j] = in && (heavy_v[j] || dv != 0);  // light interior: never a record
-        }
-        int32_t fnb[kScanItems];
-#pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
-            xs[j] = cand[j] ? g.off[v] : 0;
-            end[j] = cand[j] ? g.off[v + 1] : 0;
-            lab[j] = cand[j] ? L[v] : 0;
-            fnb[j] = cand[j] ? J.first[v] : -1;
+            known[j] = use_diff && !heavy_v[j];
         }
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             deg[j] = (int32_t)(end[j] - xs[j]);
-            bool known = use_diff && !heavy_v[j];
             if (cand[j] && heavy_v[j] && J.k <= kLpCap) {  // every label counted in the kept histogram
-                if ((int64_t)J.hh[(int64_t)J.hid[v] * kLpCap + lab[j]] == end[j] - xs[j]) cand[j] = false;
-                known = true;
-            }
-            if (!cand[j]) {
-                deg[j] = 0;
-                xs[j] = lim[j] = end[j] = 0;
-                hit[j] = dif[j] = true;
-                continue;
-            }
-            live |= 1u << j;
-            // v itself, then its first neighbour (read from a per-vertex
-            // array, not the row: sorted rows start with the smallest id,
-            // in these graphs usually a hub), before any scan of the row.
-            hit[j] = bit_of(inb, (int32_t)v) || (fnb[j] >= 0 && bit_of(inb, fnb[j]));
-            dif[j] = known;
-            lim[j] = end[j] - xs[j] > heavy_degree ? xs[j] + heavy_degree : end[j];
-        }
-#pragma unroll
-        for (int r = 0; r < kCandFirst; ++r) {
-            int32_t u[kScanItems];
-#pragma unroll
-            for (int j = 0; j < kScanItems; ++j)
-                u[j] = (xs[j] < lim[j] && !(hit[j] && dif[j])) ? g.col[xs[j]] : -1;
-#pragma unroll
-            for (int j = 0; j < kScanItems; ++j) {
-                if (u[j] < 0) continue;
-                if (!dif[j]) dif[j] = nb_label(exact8, L8, L, u[j]) != lab[j];
-                if (!hit[j]) hit[j] = bit_of(inb, u[j]);
-                ++xs[j];
-            }
-        }
-        // A few more entries lane by lane; rows still undecided are then
-        // scanned by the whole wave, 64 entries per step, so one long row
-        // does not hold its wave for hundreds of dependent loads.
-#pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            const int64_t stop = xs[j] + kLaneScan < lim[j] ? xs[j] + kLaneScan : lim[j];
-            int64_t x = xs[j];
-            for (; x < stop && !(hit[j] && dif[j]); ++x) {
-                const int32_t u = g.col[x];
-                if (!dif[j]) dif[j] = nb_label(exact8, L8, L, u) != lab[j];
-                if (!hit[j]) hit[j] = bit_of(inb, u);
-            }
-            xs[j] = x;
-        }
-#pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            uint64_t pend = __ballot(((live >> j) & 1u) && !(hit[j] && dif[j]) && xs[j] < lim[j]);
-            while (pend) {
-                const int src = __ffsll((long long)pend) - 1;
-                pend &= pend - 1;
-                const int64_t x0 = __shfl(xs[j], src, kWave), x1 = __shfl(lim[j], src, kWave);
-                const int32_t lv = __shfl(lab[j], src, kWave);
-                bool h = __shfl(hit[j] ? 1 : 0, src, kWave) != 0;
-                bool d = __shfl(dif[j] ? 1 : 0, src, kWave) != 0;
-                int64_t b = x0;
-                for (; b < x1 && !(h && d); b += kWave) {
-                    const int64_t x = b + lane_id();
-                    const int32_t u = x < x1 ? g.col[x] : -1;
-                    const bool dd = u >= 0 && !d && nb_label(exact8, L8, L, u) != lv;
-                    const bool hh = u >= 0 && !h && bit_of(inb, u);
-                    d = d || __ballot(dd) != 0;
-                    h = h || __ballot(hh) != 0;
-                }
-                if (lane_id() == src) {
-                    hit[j] = h;
-                    dif[j] = d;
-                    xs[j] = b < x1 ? b : x1;
-                }
+                if ((int64_t)J.hh[(int64_t)J.hid[v] * kLpCap + L[v]] == end[j] - xs[j]) cand[j] = false;
+                known[j] = true;
             }
         }
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
-            if (!((live >> j) & 1u)) continue;
-            // A long row still undecided after heavy_degree entries is
-            // finished by k_cand_pull_heavy (rare).
-            if (!(hit[j] && dif[j]) && lim[j] < end[j]) {
-                const unsigned long long i = atomicAdd(&ctl->v[C_DEFERRED], 1ull);
-                deferred[i] = ((int64_t)((dif[j] ? 1 : 0) | (hit[j] ? 2 : 0)) << 32) | v;
-                continue;
+            if (!cand[j]) continue;
+            // v itself, then its witness: the neighbour that last made v a
+            // candidate (at first the row's smallest id, in these graphs
+            // usually a hub).  The row's differing label is left to the
+            // scan unless the kept counts already tell.
+            if (!hit[j] && fnb[j] >= 0) hit[j] = bit_of(inb, fnb[j]);
+            if (hit[j] && known[j]) {
+                rec |= 1u << j;
+            } else if (deg[j] > heavy_degree) {
+                // Long rows are scanned one per wave by k_cand_pull_heavy.
+                s_dq[atomicAdd(&s_dn, 1)] = ((int64_t)((known[j] ? 1 : 0) | (hit[j] ? 2 : 0)) << 32) | v;
+            } else {
+                pen |= 1u << j;
             }
-            if (hit[j] && dif[j]) { rec |= 1u << j; ++nr; }
         }
-        nrec_total += nr;
-        // Every vertex gets its record entry: its degree, or -1 (deferred
-        // rows: k_cand_pull_heavy writes the degree if they turn out records).
+        nrec_total += __popc(rec);
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             if (v < g.n) rdeg[v] = (rec & (1u << j)) ? deg[j] : -1;
+            if (pen & (1u << j)) s_q[atomicAdd(&s_n, 1)] = (int32_t)v;
+        }
+        __syncthreads();
+        if (s_n > kPendStage - (int)span) pend_flush(pend, &ctl->v[C_PENDING], s_q, s_n, s_base);
+        if (s_dn > 0) {  // block-uniform: s_dn was read after the barrier
+            const int32_t c = s_dn;
+            if (threadIdx.x == 0) s_base = atomicAdd(&ctl->v[C_DEFERRED], (unsigned long long)c);
+            __syncthreads();
+            for (int i = threadIdx.x; i < c; i += blockDim.x) deferred[s_base + (unsigned long long)i] = s_dq[i];
+            __syncthreads();
+            if (threadIdx.x == 0) s_dn = 0;
+            __syncthreads();
         }
     }
+    __syncthreads();
+    if (s_n > 0) pend_flush(pend, &ctl->v[C_PENDING], s_q, s_n, s_base);
     block_count_add(bd.cnt, nrec_total);
 }
 
+// The pull pass's pending rows (at most heavy_degree entries), one lane per
+// row: kRestUnroll entries of the row per step, their column loads and
+// tests issued together (the outcome is an OR over entries, so testing a
+// few past the deciding one changes nothing).  Rows still undecided after
+// kRestLane entries are scanned by the whole wave, 64 entries per step.
+constexpr int kRestUnroll = 4;
+constexpr int kRestLane = 16;
+__global__ __launch_bounds__(kThreads) void k_cand_pull_rest(const P2Job *JS, const int32_t *bstart,
+                                                             int jobs) {
+    P2_JOB();
+    const GraphDev g = J.g;
+    const int32_t *L = J.L;
+    const uint8_t *L8 = J.L8;
+    const RecDev bd = J.bd;
+    const bool exact8 = J.k <= kExact8;
+    Ctl *ctl = J.ctl;
+    if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
+    const uint32_t *inb = J.bbits + (int64_t)((ctl->t - 1) & 1) * J.bwords;  // B_{t-1}
+    const bool use_diff = ctl->diff_ok != 0;
+    const int32_t *pend = J.D.light;
+    const int64_t np = (int64_t)ctl->v[C_PENDING];
+    int64_t nrec = 0;
+    for (int64_t b0 = vwave(vg) * kWave; b0 < np; b0 += vwaves(vg) * kWave) {
+        const int64_t i = b0 + lane_id();
+        const bool live = i < np;
+        const int32_t v = live ? pend[i] : 0;
+        const int64_t s0 = live ? g.off[v] : 0;
+        const int64_t lim = live ? g.off[v + 1] : 0;
+        const int32_t lab = live ? L[v] : 0;
+        int64_t xs = s0;
+        const bool heavy_v = live && use_diff && bit_of(J.hbits, v);
+        // As in k_cand_pull: a pending row passed its boundary test, which
+        // the kept counts answer for light rows and (k <= 32) heavy ones.
+        bool dif = !live || (use_diff && (!heavy_v || J.k <= kLpCap));
+        bool hit = !live || bit_of(inb, v);
+        int32_t found = -1;  // the neighbour that answers `hit`, kept as v's witness
+        const int64_t stop = xs + kRestLane < lim ? xs + kRestLane : lim;
+        while (xs < stop && !(hit && dif)) {
+            int32_t u[kRestUnroll];
+#pragma unroll
+            for (int q = 0; q < kRestUnroll; ++q) u[q] = xs + q < stop ? g.col[xs + q] : -1;
+#pragma unroll
+            for (int q = 0; q < kRestUnroll; ++q) {
+                if (u[q] < 0) continue;
+                if (!dif) dif = nb_label(exact8, L8, L, u[q]) != lab;
+                if (!hit && bit_of(inb, u[q])) {
+                    hit = true;
+                    found = u[q];
+                }
+            }
+            xs = xs + kRestUnroll < stop ? xs + kRestUnroll : stop;
+        }
+        uint64_t todo = __ballot(!(hit && dif) && xs < lim);
+        while (todo) {
+            const int src = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const int64_t x0 = __shfl(xs, src, kWave), x1 = __shfl(lim, src, kWave);
+            const int32_t lv = __shfl(lab, src, kWave);
+            bool h = __shfl(hit ? 1 : 0, src, kWave) != 0;
+            bool d = __shfl(dif ? 1 : 0, src, kWave) != 0;
+            int32_t fw = -1;
+            int64_t b = x0;
+            for (; b < x1 && !(h && d); b += kWave) {
+                const int64_t x = b + lane_id();
+                const int32_t u = x < x1 ? g.col[x] : -1;
+                const bool dd = u >= 0 && !d && nb_label(exact8, L8, L, u) != lv;
+                const uint64_t hm = __ballot(u >= 0 && !h && bit_of(inb, u));
+                d = d || __ballot(dd) != 0;
+                if (hm) {
+                    fw = __shfl(u, __ffsll((long long)hm) - 1, kWave);
+                    h = true;
+                }
+            }
+            if (lane_id() == src) {
+                hit = h;
+                dif = d;
+                xs = b < x1 ? b : x1;
+                if (fw >= 0) found = fw;
+            }
+        }
+        if (found >= 0) J.wit[v] = found;
+        if (!live || !(hit && dif)) continue;
+        J.rdeg[v] = (int32_t)(lim - s0);
+        ++nrec;
+    }
+    block_count_add(bd.cnt, nrec);
+}
+
+// Rows longer than heavy_degree that the pull pass left undecided, one per
+// wave, 64 entries per step.
 __global__ void k_cand_pull_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
@@ -655,7 +782,6 @@ __global__ void k_cand_pull_heavy(const P2Job *JS, const int32_t *bstart, int jo
     const uint8_t *L8 = J.L8;
     const RecDev bd = J.bd;
     const int64_t *deferred = J.deferred;
-    const int heavy_degree = J.heavy_degree;
     const bool exact8 = J.k <= kExact8;
     Ctl *ctl = J.ctl;
     if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
@@ -667,13 +793,17 @@ __global__ void k_cand_pull_heavy(const P2Job *JS, const int32_t *bstart, int jo
         bool dif = (item >> 32) & 1, hit = (item >> 33) & 1;
         const int32_t lv = L[v];
         const int64_t s = g.off[v], e = g.off[v + 1];
-        for (int64_t b = s + heavy_degree; b < e && !(dif && hit); b += kWave) {
+        for (int64_t b = s; b < e && !(dif && hit); b += kWave) {
             const int64_t x = b + lane_id();
             const int32_t u = x < e ? g.col[x] : -1;
             const bool d = u >= 0 && !dif && nb_label(exact8, L8, L, u) != lv;
-            const bool h = u >= 0 && !hit && bit_of(inb, u);
+            const uint64_t hm = __ballot(u >= 0 && !hit && bit_of(inb, u));
             dif = dif || __ballot(d) != 0;
-            hit = hit || __ballot(h) != 0;
+            if (hm) {  // wave-uniform; the first such neighbour becomes v's witness
+                const int32_t fw = __shfl(u, __ffsll((long long)hm) - 1, kWave);
+                if (lane_id() == 0) J.wit[v] = fw;
+                hit = true;
+            }
         }
         const bool rec = dif && hit && lane_id() == 0;
         if (rec) {
@@ -1377,6 +1507,7 @@ __global__ void k_mirror8(const P2Job *JS, const int32_t *bstart, int jobs) {
         const uint32_t b = (l >= 0 && l < kLabel8None) ? (uint32_t)l : (uint32_t)kLabel8None;
         L8[v] = (uint8_t)b;
         LN[v] = (uint16_t)(b | (kLabel8None << 8));
+        J.wit[v] = J.first[v];
     }
 }
 
@@ -1769,6 +1900,7 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.diff.alloc(sizeof(int32_t) * (size_t)n);
         w.stamp.alloc(sizeof(int32_t) * (size_t)n);
         w.rdeg.alloc(sizeof(int32_t) * (size_t)n);
+        w.wit.alloc(sizeof(int32_t) * (size_t)n);
         w.bbits.alloc(sizeof(uint32_t) * 2 * (size_t)((n + 31) / 32));
         w.L8.alloc((size_t)n);
         w.ON.alloc(sizeof(uint16_t) * (size_t)n);
@@ -1847,6 +1979,7 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.lh = j.lh_on ? w.lh.as<uint32_t>() : nullptr;
     j.hbits = g->hbits.as<uint32_t>();
     j.first = g->first.as<int32_t>();
+    j.wit = w.wit.as<int32_t>();
     j.deferred = w.deferred.as<int64_t>();
     j.hub_hist = w.hub_hist.as<int32_t>();
     j.bd = w.bd.dev;
@@ -1975,6 +2108,7 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     // Each writes one dense record entry per vertex (push: after a clear).
     hipLaunchKernelGGL(k_cand_all, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     hipLaunchKernelGGL(k_cand_pull, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_cand_pull_rest, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     if (set.any_heavy)
         hipLaunchKernelGGL(k_cand_pull_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     hipLaunchKernelGGL(k_rdeg_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);

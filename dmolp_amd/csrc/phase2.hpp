@@ -18,6 +18,7 @@ enum CtlIndex {
     C_PREV_DEG,      // sum deg over B_{t-1}
     C_KERNEL_ERR,    // a kernel found an impossible state
     C_DEFERRED,      // heavy rows left to the pull kernel's second pass
+    C_PENDING,       // rows the pull pass left to k_cand_pull_rest
     C_NCTL = 16
 };
 
@@ -85,6 +86,7 @@ struct P2Job {
     uint32_t *lh;
     const uint32_t *hbits;  // heavy-row bitmap of the graph
     const int32_t *first;   // each row's first (smallest) neighbour, -1 if empty
+    int32_t *wit;           // per run: a neighbour that last put the row in the candidates
     RecDev bd;
     Ctl *ctl;
     unsigned long long *vc, *ic;
@@ -115,7 +117,7 @@ struct Phase2Work {
     int k = 0;
     int64_t cap_heavy = -1, cap_chunks = -1;  // heavy capacities the lists were sized for
     int32_t epoch = 0;  // stamp value of the current candidate pass
-    DevMem chg, diff, stamp, rdeg, bbits, deferred;
+    DevMem chg, diff, stamp, rdeg, bbits, deferred, wit;
     DevMem L8;  // byte mirror of the labels for the LP gathers
     DevMem hh, hhc, cbits, ON, LN;
     DevMem lh;  // uint32 [n][kLpCap / 2]: kept light-row label counts
