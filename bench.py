@@ -302,17 +302,18 @@ def kernel_models(g, k, tr, per_round):
         # count; per entry: the column, the neighbour's old|new label pair
         # (2 bytes) and one word of the neighbour's kept row read and written.
         "k_delta_light": (per_round["delta_light"], (2 * o + 3 * l) * dl_n + (c + 2 + 8) * dl_deg),
-        # Per light row of B_t: its kept row and own label; the B_t and heavy
-        # bitmaps (n/4 bytes); per changed row: new label, label pair,
-        # offsets (its degree) and its Delta entry.
-        "k_lp_light_kept": (per_round["lp_light"],
-                            (row_b + l) * bl_n + n / 4 + (l + 2 + 2 * o + 4) * dl_n),
-        # Pull rounds: per vertex its differing count and record entry; per
-        # boundary vertex offsets, label and largest-degree neighbour
-        # (scanned entries not counted).  The group holds the round start and
-        # the other candidate kernels, which do no work in pull rounds.
-        "k_cand_pull": (per_round["candidates"],
-                        np.where(pulled, (2 * l) * n + (2 * o + 2 * l) * bd, 0.0)),
+        # The selection's take pass fused with the light LP: per vertex its
+        # record entry, per record its label byte, the B_t bitmap written
+        # (n/8 bytes); per light member of B_t its kept row; per changed row
+        # its new label, label pair and Delta entry.
+        "k_lp_light_take": (per_round["lp_light"],
+                            4 * n + 1 * bd + n / 8 + row_b * bl_n + (l + 2 + 4) * dl_n),
+        # Pull rounds: per vertex its differing count, offset, witness (the
+        # neighbour that last made it a candidate) and record entry (rows
+        # scanned for pending vertices not counted).  The group holds the
+        # round start and the other candidate kernels, which do no work in
+        # pull rounds.
+        "k_cand_pull": (per_round["candidates"], np.where(pulled, (3 * l + o) * n, 0.0)),
         # Per light row: offset, label, differing count and kept row written;
         # per light entry: the column and a label byte.
         "k_init_light": (per_round["init_light"],

@@ -421,40 +421,54 @@ __global__ void k_init_finish(const P2Job *JS, const int32_t *bstart, int jobs) 
 // ---------------------------------------------------------------- round begin
 
 // phase2.cpp:158-209 in source order; lg = 1 + log(t) from the host's libm.
+// Called by every lane of a one-wave block.  Element-wise steps run one
+// label per lane; the means and variances are summed by lane 0 in label
+// order, so every double matches the sequential formula bit for bit.
 __device__ void penalties(const unsigned long long *vc, const unsigned long long *ic, int k,
                           double lg, double *pen, double *re) {
-    int64_t tv = 0, te = 0;
-    for (int i = 0; i < k; ++i) { tv += (int64_t)vc[i]; te += (int64_t)ic[i]; }
+    __shared__ double s_rv[DMOLP_MAX_K], s_re[DMOLP_MAX_K];
+    __shared__ double s_imb[2];
+    const int lane = lane_id();
+    int64_t tv = 0, te = 0;  // integer sums: any order
+    for (int i = lane; i < k; i += kWave) { tv += (int64_t)vc[i]; te += (int64_t)ic[i]; }
+    tv = wave_sum(tv);
+    te = wave_sum(te);
     const double exp_v = static_cast<double>(tv) / k;
     const double exp_e = (te > 0) ? static_cast<double>(te) / k : 1.0;
-    for (int i = 0; i < k; ++i) {
-        pen[i] = (exp_v > 0) ? static_cast<double>((int64_t)vc[i]) / exp_v : 1.0;  // RV
-        re[i] = (exp_e > 0) ? static_cast<double>((int64_t)ic[i]) / exp_e : 1.0;
+    for (int i = lane; i < k; i += kWave) {
+        s_rv[i] = (exp_v > 0) ? static_cast<double>((int64_t)vc[i]) / exp_v : 1.0;  // RV
+        s_re[i] = (exp_e > 0) ? static_cast<double>((int64_t)ic[i]) / exp_e : 1.0;
     }
-    double rv_mean = 0.0, re_mean = 0.0;
-    for (int i = 0; i < k; ++i) { rv_mean += pen[i]; re_mean += re[i]; }
-    rv_mean /= k;
-    re_mean /= k;
-    double rv_var = 0.0, re_var = 0.0;
-    for (int i = 0; i < k; ++i) {
-        rv_var += (pen[i] - rv_mean) * (pen[i] - rv_mean);
-        re_var += (re[i] - re_mean) * (re[i] - re_mean);
+    wave_sync();
+    if (lane == 0) {
+        double rv_mean = 0.0, re_mean = 0.0;
+        for (int i = 0; i < k; ++i) { rv_mean += s_rv[i]; re_mean += s_re[i]; }
+        rv_mean /= k;
+        re_mean /= k;
+        double rv_var = 0.0, re_var = 0.0;
+        for (int i = 0; i < k; ++i) {
+            rv_var += (s_rv[i] - rv_mean) * (s_rv[i] - rv_mean);
+            re_var += (s_re[i] - re_mean) * (s_re[i] - re_mean);
+        }
+        rv_var /= k;
+        re_var /= k;
+        const double total_var = rv_var + re_var;
+        s_imb[0] = (total_var > 0) ? rv_var / total_var : 0.0;
+        s_imb[1] = (total_var > 0) ? re_var / total_var : 0.0;
     }
-    rv_var /= k;
-    re_var /= k;
-    const double total_var = rv_var + re_var;
-    const double imb_rv = (total_var > 0) ? rv_var / total_var : 0.0;
-    const double imb_re = (total_var > 0) ? re_var / total_var : 0.0;
-    for (int i = 0; i < k; ++i) {
-        const double g_rv = (1.0 - pen[i]) * lg;
-        const double g_re = (1.0 - re[i]) * lg;
+    wave_sync();
+    const double imb_rv = s_imb[0], imb_re = s_imb[1];
+    for (int i = lane; i < k; i += kWave) {
+        const double g_rv = (1.0 - s_rv[i]) * lg;
+        const double g_re = (1.0 - s_re[i]) * lg;
         pen[i] = imb_rv * g_rv + imb_re * g_re;
+        re[i] = s_re[i];
     }
 }
 
 __global__ void k_penalties(const unsigned long long *vc, const unsigned long long *ic, int k,
                             double lg, double *pen, double *re) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) penalties(vc, ic, k, lg, pen, re);
+    if (threadIdx.x < kWave) penalties(vc, ic, k, lg, pen, re);
 }
 
 // Round start: penalties, counter resets, and the candidate direction.
@@ -493,8 +507,8 @@ __global__ void k_round_begin(const P2Job *JS, const int32_t *bstart, int jobs) 
         ctl->v[C_DEFERRED] = 0;
         ctl->v[C_PENDING] = 0;
         ctl->cand_mode = t == 1 ? CAND_ALL : ((double)prev_deg >= pull_frac * (double)nnz ? CAND_PULL : CAND_PUSH);
-        penalties(vc, ic, k, rtab[t], pen, re);
     }
+    if (threadIdx.x < kWave) penalties(vc, ic, k, rtab[ctl->t], pen, re);
 }
 
 // ---------------------------------------------------------------- candidates
@@ -1053,39 +1067,51 @@ __device__ __forceinline__ void d_flush(const ListDev &D, const int32_t *s_dq, i
     __syncthreads();
 }
 
-// Light rows of B_t decide from their kept histograms: one lane per row and
-// one 64-byte row read (32 bytes or less for k <= 16) in place of a column
-// read and a label gather per entry.  B_t is taken from its bitmap in vertex
-// order, so the rows (and labels) are read as ascending addresses, which
+// The selection's last pass and the light LP decisions in one pass over the
+// vertices in id order.  Each wave takes 64 consecutive vertices per item:
+// their records (degree, label byte) give the take decisions (B_t, written
+// as two bitmap words per item with the member counts, as the selection's
+// own dense pass would), and each light member decides at once from its
+// kept histogram: one 64-byte row read (32 bytes or less for k <= 16) in
+// place of a column read and a label gather per entry.  The decision needs
+// only the vertex's own row and the round's penalties, so it can follow its
+// take decision directly.  The rows are read as ascending addresses, which
 // keeps the DRAM pages open: the rows table is far larger than the caches.
-// Each wave takes 64 consecutive vertices per item; groups without a light
-// B_t member cost two bitmap words.
-__global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, const int32_t *bstart,
-                                                            int jobs) {
+__global__ __launch_bounds__(kThreads) void k_lp_light_take(const P2Job *JS, const SelArgs *SA,
+                                                            const int32_t *bstart, int jobs) {
     P2_JOB();
-    const int64_t n = J.g.n;
-    const int64_t *off = J.g.off;
-    const int32_t *L = J.L;
+    const SelArgs a = gload(SA[job_]);
     const uint32_t *lh = J.lh;
-    const uint32_t *hbits = J.hbits;
     const double *pen = J.pen;
     const int k = J.k;
     int32_t *chg = J.chg;
     uint16_t *LN = J.LN;
     const ListDev D = J.D;
-    const int64_t bwords = J.bwords;
+    const ListDev sel = a.sel;
     const Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
-    const uint32_t *bt = J.bbits + (int64_t)(ctl->t & 1) * bwords;  // B_t
+    const SelState *st = a.st;
+    const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
+    const int heavy_degree = a.heavy_degree;
+    const int64_t rn = a.rn, bwords = a.bwords;
+    const int32_t *rdeg = a.rdeg;
+    uint32_t *half = a.bits + (int64_t)(ctl->t & 1) * bwords;  // B_t
     __shared__ double s_pen[kLpCap], s_fac[kLpCap];
     __shared__ double s_fmax;
+    __shared__ SelLite s_st[kSelLdsGroups];
+    __shared__ unsigned long long s_sum[4];
+    __shared__ int32_t s_dq[kDStage];
+    __shared__ int32_t s_dn;
+    __shared__ unsigned long long s_base;
     const int E = k < kLpCap ? k : kLpCap;
     const int nvec = (E + 7) / 8;
     if ((int)threadIdx.x < E) {
         s_pen[threadIdx.x] = pen[threadIdx.x];
         s_fac[threadIdx.x] = 1.0 + pen[threadIdx.x];
     }
-    __syncthreads();
+    if (threadIdx.x < 4) s_sum[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_dn = 0;
+    sel_lite_load(s_st, st, a.groups);  // ends with a barrier
     if (threadIdx.x == 0) {
         double f = s_fac[0];
         for (int l = 1; l < E; ++l) f = s_fac[l] > f ? s_fac[l] : f;
@@ -1094,44 +1120,46 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
     __syncthreads();
     const double fmax = s_fmax;
     const int w = threadIdx.x / kWave, lane = lane_id();
-    const int64_t span = (int64_t)blockDim.x * kScanItems;
-    __shared__ int32_t s_dq[kDStage];
-    __shared__ int32_t s_dn;
-    __shared__ unsigned long long s_base;
-    if (threadIdx.x == 0) s_dn = 0;
-    __syncthreads();
-    int64_t dsum = 0;
+    const int64_t ng = (rn + kWave - 1) / kWave;
+    const int64_t per_block = (int64_t)(blockDim.x / kWave) * kDenseItems;  // 64-id groups
+    int64_t n_light = 0, deg_sum = 0, n_heavy = 0, heavy_deg = 0, dsum = 0;
     // Block-uniform loop, so the Delta append can be aggregated per block.
-    for (int64_t b0 = vg.block * span; b0 < n; b0 += vg.blocks * span) {
-        uint32_t cm = 0, act = 0;
-        int32_t ownj[kScanItems];
-        // Stage 1 for every item (loads issued together): the group's bitmap
-        // words, then the own labels of the lanes with work.
+    for (int64_t base = vg.block * per_block; base < ng; base += vg.blocks * per_block) {
+        const int64_t g0 = base + (int64_t)w * kDenseItems;
+        int32_t d[kDenseItems], gg[kDenseItems];
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            const int64_t gbase = b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave;
-            uint64_t m = 0;
-            if (gbase < n) {
-                const int64_t wi = gbase >> 5;  // gbase is a multiple of 64
-                uint64_t bw = bt[wi], hw = hbits[wi];
-                if (wi + 1 < bwords) {
-                    bw |= (uint64_t)bt[wi + 1] << 32;
-                    hw |= (uint64_t)hbits[wi + 1] << 32;
+        for (int j = 0; j < kDenseItems; ++j) {
+            const int64_t v = (g0 + j) * kWave + lane;
+            d[j] = v < rn ? rdeg[v] : -1;
+            gg[j] = d[j] >= 0 ? dense_group(a, v) : 0;
+        }
+        uint32_t act = 0, cm = 0;
+#pragma unroll
+        for (int j = 0; j < kDenseItems; ++j) {
+            const int64_t gi = g0 + j;
+            if (gi >= ng) break;  // wave-uniform
+            const int64_t v = gi * kWave + lane;
+            const bool take =
+                d[j] >= 0 && dense_take(sel_get(s_st, st, gg[j]), (int32_t)v, d[j], b1_bins, id_bits, deg_bits);
+            const bool light = take && d[j] <= heavy_degree;
+            const uint64_t m = __ballot(take);  // the bitmap holds heavy members too
+            if (lane == 0) half[2 * gi] = (uint32_t)m;
+            if (lane == 1 && 2 * gi + 1 < bwords) half[2 * gi + 1] = (uint32_t)(m >> 32);
+            if (take) {
+                deg_sum += d[j];
+                if (light) {
+                    ++n_light;
+                    act |= 1u << j;
+                } else {
+                    ++n_heavy;
+                    heavy_deg += d[j];
                 }
-                m = bw & ~hw;
             }
-            if ((m >> lane) & 1ull) act |= 1u << j;
         }
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            const int64_t v = b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave + lane;
-            ownj[j] = (act >> j) & 1u ? L[v] : 0;
-        }
-#pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
+        for (int j = 0; j < kDenseItems; ++j) {
             if (!((act >> j) & 1u)) continue;
-            const int64_t gbase = b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave;
-            const int32_t v = (int32_t)(gbase + lane);
+            const int32_t v = (int32_t)((g0 + j) * kWave + lane);
             const uint4 *row = reinterpret_cast<const uint4 *>(lh + (int64_t)v * kPackedWords);
             uint32_t c[kPackedWords];
 #pragma unroll
@@ -1142,7 +1170,7 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
                 c[4 * q + 2] = x.z;
                 c[4 * q + 3] = x.w;
             }
-            const int32_t own = ownj[j];
+            const int32_t own = gg[j];  // records carry valid labels
             const auto word = [&](int q) { return pick16(c, q); };
             const int32_t nl =
                 (own >= 0 && own < E)
@@ -1153,22 +1181,39 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_kept(const P2Job *JS, con
                 cm |= 1u << j;
                 chg[v] = nl;
                 LN[v] = ln_pair(own, nl);
-                dsum += off[v + 1] - off[v];
+                dsum += d[j];
             }
         }
         // Changed vertices are staged in LDS and flushed to Delta with one
         // global reservation per kDStage / 2 or more entries.
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
+        for (int j = 0; j < kDenseItems; ++j) {
             if (!(cm & (1u << j))) continue;
-            s_dq[atomicAdd(&s_dn, 1)] = (int32_t)(b0 + ((int64_t)j * (blockDim.x / kWave) + w) * kWave + lane);
+            s_dq[atomicAdd(&s_dn, 1)] = (int32_t)((g0 + j) * kWave + lane);
         }
         __syncthreads();
-        if (s_dn > kDStage - (int)span) d_flush(D, s_dq, s_dn, s_base);
+        if (s_dn > kDStage - (int)(per_block * kWave)) d_flush(D, s_dq, s_dn, s_base);
     }
     __syncthreads();
     if (s_dn > 0) d_flush(D, s_dq, s_dn, s_base);
     block_count_add(&D.cnt[LC_DEG], dsum);
+    n_light = wave_sum(n_light);
+    deg_sum = wave_sum(deg_sum);
+    n_heavy = wave_sum(n_heavy);
+    heavy_deg = wave_sum(heavy_deg);
+    if (lane == 0) {
+        if (n_light) atomicAdd(&s_sum[0], (unsigned long long)n_light);
+        if (deg_sum) atomicAdd(&s_sum[1], (unsigned long long)deg_sum);
+        if (n_heavy) atomicAdd(&s_sum[2], (unsigned long long)n_heavy);
+        if (heavy_deg) atomicAdd(&s_sum[3], (unsigned long long)heavy_deg);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_sum[0]) atomicAdd(&sel.cnt[LC_LIGHT], s_sum[0]);
+        if (s_sum[1]) atomicAdd(&sel.cnt[LC_DEG], s_sum[1]);
+        if (s_sum[2]) atomicAdd(&sel.cnt[LC_SLOTS], s_sum[2]);
+        if (s_sum[3]) atomicAdd(&sel.cnt[LC_HDEG], s_sum[3]);
+    }
 }
 
 // Heavy rows, pass 1: each wave counts one chunk with a peel loop over the
@@ -2013,10 +2058,13 @@ SelArgs phase2_sel_args(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     rule.ratio_tab = w.rtab.p ? w.rtab.as<double>() + w.rtab_rounds + 1 : nullptr;
     rule.t_ptr = &ctl->t;
     rule.re = w.re.as<double>();
-    // Records are dense (one entry per vertex, labels read in place) and the
-    // selected set goes straight to the B_t bitmap.
+    // Records are dense (one entry per vertex, labels read in place: the
+    // byte mirror when it holds every label) and the selected set goes
+    // straight to the B_t bitmap.  Records have valid labels: a run with
+    // any label outside [0, k) stops after the initial sweep.
     SelArgs a = select_args(ctx, w.sel, w.bd.dev, rule, &ctl->done, w.B.dev, nullptr);
-    select_args_dense(a, w.rdeg.as<int32_t>(), g->labels.as<int32_t>(), g->n, w.bbits.as<uint32_t>(),
+    select_args_dense(a, w.rdeg.as<int32_t>(), g->labels.as<int32_t>(),
+                      w.k <= kExact8 ? w.L8.as<uint8_t>() : nullptr, g->n, w.bbits.as<uint32_t>(),
                       (g->n + 31) / 32);
     return a;
 }
@@ -2117,12 +2165,14 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     if (set.any_heavy)
         hipLaunchKernelGGL(k_cand_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m1 = mark();
-    select_enqueue(ctx, set.sel, ls, set.passes);  // B_t: bitmap, heavy slots, counts
+    // B_t: the bitmap and counts.  With kept histograms the selection's last
+    // pass is fused with the light LP below.
+    select_enqueue(ctx, set.sel, ls, set.passes, !set.lh_on);
     if (!set.lh_on)  // the gathering LP walks B_t as a list
         hipLaunchKernelGGL(k_b_list, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 0, 0, 0);
     const size_t m2 = mark();
     if (set.lh_on)
-        hipLaunchKernelGGL(k_lp_light_kept, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+        hipLaunchKernelGGL(k_lp_light_take, dim3(grid), dim3(kThreads), 0, s, jobs, set.sel, bg, J);
     else
         hipLaunchKernelGGL(k_lp_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     const size_t m3 = mark();

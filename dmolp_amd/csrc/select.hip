@@ -11,42 +11,6 @@ constexpr int kSelThreads = 256;
 constexpr int kHist1LdsInts = 8192;  // 32 KB of LDS for the level-1 histogram
 constexpr int kCompactItems = 8;     // records per thread per block iteration
 
-__device__ __forceinline__ uint64_t sel_key(int32_t x, int32_t d, int desc, int id_bits,
-                                            int deg_bits) {
-    const uint64_t dmax = (deg_bits >= 63) ? ~0ull : ((1ull << deg_bits) - 1);
-    const uint64_t dk = desc ? dmax - (uint64_t)d : (uint64_t)d;
-    return (dk << id_bits) | (uint64_t)(uint32_t)x;
-}
-
-__device__ __forceinline__ int bin1_of(int32_t d, int b1_bins) {
-    return d < b1_bins - 1 ? d : b1_bins - 1;
-}
-
-// What the per-record passes need of a group's state, cached in LDS for the
-// first kSelLdsGroups groups (a global gather per record otherwise).
-constexpr int kSelLdsGroups = 64;
-struct SelLite {
-    int32_t mode, b1, desc, pad;
-    uint64_t kmax;  // SEL_KEY: largest key taken inside bin b1
-};
-__device__ __forceinline__ SelLite sel_lite(const SelState &s) {
-    SelLite l;
-    l.mode = s.mode;
-    l.b1 = s.b1;
-    l.desc = s.desc;
-    l.pad = 0;
-    l.kmax = s.prefix | ((s.r >= 64) ? ~0ull : ((1ull << s.r) - 1));
-    return l;
-}
-__device__ __forceinline__ void sel_lite_load(SelLite *lds, const SelState *st, int groups) {
-    const int ng = groups < kSelLdsGroups ? groups : kSelLdsGroups;
-    for (int i = threadIdx.x; i < ng; i += blockDim.x) lds[i] = sel_lite(st[i]);
-    __syncthreads();
-}
-__device__ __forceinline__ SelLite sel_get(const SelLite *lds, const SelState *st, int g) {
-    return g < kSelLdsGroups ? lds[g] : sel_lite(st[g]);
-}
-
 // Running count over bins in scan order (reverse: from the last bin down);
 // finds the first bin where it reaches target (1 <= target <= total).
 // Returns through LDS variables; all threads must call.
@@ -76,29 +40,6 @@ __device__ void block_crossing(const int32_t *bins, int nb, bool reverse, int64_
         }
     }
     __syncthreads();
-}
-
-using ::dmolp::gload;
-__device__ __forceinline__ SelArgs gload(const SelArgs &v) {
-    SelArgs r = v;
-    r.in = gload(v.in);
-    r.sel = gload(v.sel);
-    r.carry = gload(v.carry);
-    r.ref = gload(v.ref);
-    r.h1 = gload(v.h1);
-    r.h2 = gload(v.h2);
-    r.st = gload(v.st);
-    r.refining = gload(v.refining);
-    r.done = gload(v.done);
-    r.mark = gload(v.mark);
-    r.mark_val = gload(v.mark_val);
-    r.rule.ratio_tab = gload(v.rule.ratio_tab);
-    r.rule.t_ptr = gload(v.rule.t_ptr);
-    r.rule.re = gload(v.rule.re);
-    r.rdeg = gload(v.rdeg);
-    r.rlab = gload(v.rlab);
-    r.bits = gload(v.bits);
-    return r;
 }
 
 // Kernels take the job set (one SelArgs per job) and run on the job's
@@ -131,7 +72,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(const SelArgs *A, con
             const int64_t i = i0 + j * step;
             if (dense) {
                 dd[j] = i < count ? a.rdeg[i] : -1;
-                gg[j] = dd[j] >= 0 ? a.rlab[i] : -1;
+                gg[j] = dd[j] >= 0 ? dense_group(a, i) : -1;
             } else {
                 gg[j] = i < count ? in.g[i] : -1;
                 dd[j] = i < count ? in.d[i] : 0;
@@ -278,7 +219,7 @@ __device__ __forceinline__ void ref_flush(const SelArgs &a, const int32_t *s_x, 
         const int32_t x = s_x[i];
         const unsigned long long p = s_base + (unsigned long long)i;
         a.ref.x[p] = x;
-        a.ref.g[p] = a.rlab[x];
+        a.ref.g[p] = dense_group(a, x);
         a.ref.d[p] = a.rdeg[x];
     }
     __syncthreads();
@@ -306,7 +247,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect_dense(const SelArgs
         for (int j = 0; j < kCompactItems; ++j) {
             const int64_t i = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             d[j] = i < count ? a.rdeg[i] : -1;
-            gg[j] = d[j] >= 0 ? a.rlab[i] : 0;
+            gg[j] = d[j] >= 0 ? dense_group(a, i) : 0;
         }
 #pragma unroll
         for (int j = 0; j < kCompactItems; ++j) {
@@ -485,7 +426,6 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact(const SelArgs *A, c
 // heavy members, their degree sums) are summed per block and added once per
 // block.  Heavy members get no list slots: their kernels walk the graph's
 // heavy rows and test the bitmap.
-constexpr int kDenseItems = 4;  // 64-id groups per wave per iteration
 __global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs *A,
                                                                    const int32_t *bstart, int jobs) {
     SEL_JOB();
@@ -495,7 +435,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs
     const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
     const int heavy_degree = a.heavy_degree;
     const int64_t rn = a.rn, bwords = a.bwords;
-    const int32_t *rdeg = a.rdeg, *rlab = a.rlab;
+    const int32_t *rdeg = a.rdeg;
     uint32_t *half = a.bits + (int64_t)(*a.rule.t_ptr & 1) * bwords;
     __shared__ SelLite s_st[kSelLdsGroups];
     __shared__ unsigned long long s_sum[4];
@@ -510,28 +450,15 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs
         for (int j = 0; j < kDenseItems; ++j) {
             const int64_t v = (g0 + j) * kWave + lane;
             d[j] = v < rn ? rdeg[v] : -1;
-            gg[j] = d[j] >= 0 ? rlab[v] : 0;
+            gg[j] = d[j] >= 0 ? dense_group(a, v) : 0;
         }
 #pragma unroll
         for (int j = 0; j < kDenseItems; ++j) {
             const int64_t gi = g0 + j;
             if (gi >= ng) break;  // wave-uniform
             const int64_t v = gi * kWave + lane;
-            bool take = false;
-            if (d[j] >= 0) {
-                const SelLite s = sel_get(s_st, st, gg[j]);
-                const int b = bin1_of(d[j], b1_bins);
-                const bool earlier = s.desc ? b > s.b1 : b < s.b1;
-                switch (s.mode) {
-                    case SEL_ALL: take = true; break;
-                    case SEL_THROUGH: take = earlier || b == s.b1; break;
-                    case SEL_KEY:
-                        take = earlier ||
-                               (b == s.b1 && sel_key((int32_t)v, d[j], s.desc, id_bits, deg_bits) <= s.kmax);
-                        break;
-                    default: break;
-                }
-            }
+            const bool take =
+                d[j] >= 0 && dense_take(sel_get(s_st, st, gg[j]), (int32_t)v, d[j], b1_bins, id_bits, deg_bits);
             const bool light = take && d[j] <= heavy_degree;
             const uint64_t m = __ballot(take);  // the bitmap holds heavy members too
             if (lane == 0) half[2 * gi] = (uint32_t)m;
@@ -622,17 +549,19 @@ SelArgs select_args(dmolp_ctx *ctx, SelectWork &w, const RecDev &in, const TakeR
     return a;
 }
 
-void select_args_dense(SelArgs &a, const int32_t *rdeg, const int32_t *rlab, int64_t rn,
-                       uint32_t *bits, int64_t bwords) {
+void select_args_dense(SelArgs &a, const int32_t *rdeg, const int32_t *rlab, const uint8_t *rlab8,
+                       int64_t rn, uint32_t *bits, int64_t bwords) {
     a.dense = 1;
     a.rdeg = rdeg;
     a.rlab = rlab;
+    a.rlab8 = rlab8;
     a.rn = rn;
     a.bits = bits;
     a.bwords = bwords;
 }
 
-void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, int passes) {
+void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, int passes,
+                    bool compact) {
     hipStream_t s = ctx->stream;
     const int J = ls.jobs;
     hipLaunchKernelGGL(k_sel_hist1, dim3(ls.total[P_HIST1]), dim3(kSelThreads), 0, s, args,
@@ -651,8 +580,9 @@ void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, in
     }
     hipLaunchKernelGGL(k_sel_compact, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_GRID), J);
-    hipLaunchKernelGGL(k_sel_compact_dense, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
-                       ls.bs(P_GRID), J);
+    if (compact)
+        hipLaunchKernelGGL(k_sel_compact_dense, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                           ls.bs(P_GRID), J);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 

@@ -6,7 +6,7 @@
 // partition.cpp:400-431 (per partition, the theta smallest (degree, id)).
 //
 // No sort: a level-1 histogram over min(degree, B1-1) finds the degree bin
-// where the cut falls; if that bin must be split, 12-bit MSD digit passes
+// where the cut falls; if that bin must be split, 8-bit MSD digit passes
 // over the key K = (dk << IB) | id (dk = degree, or maxdeg - degree for the
 // descending order) find the exact threshold key.  Every pass is a
 // histogram, so the selected set does not depend on record order.
@@ -103,29 +103,115 @@ struct SelArgs {
     int32_t groups, b1_bins, id_bits, deg_bits;
     int32_t has_carry, heavy_degree, chunk_edges, dense;
     // Dense records (dense != 0): record v exists for 0 <= v < rn iff
-    // rdeg[v] >= 0, with id v, group rlab[v] and degree rdeg[v]; `in` is
+    // rdeg[v] >= 0, with id v, group rlab[v] (rlab8[v] when rlab8 is set:
+    // every group id below 255) and degree rdeg[v]; `in` is
     // unused.  The selected vertices then go to half (*rule.t_ptr & 1) of
     // the double bitmap `bits` (bwords words per half), every word of it
     // written; sel gets the counts only (LC_LIGHT, LC_DEG, LC_SLOTS as the
     // heavy member count, LC_HDEG), no lists.
     const int32_t *rdeg;
     const int32_t *rlab;
+    const uint8_t *rlab8;
     int64_t rn;
     uint32_t *bits;
     int64_t bwords;
 };
 
+// Device helpers shared by the selection kernels and phase 2's fused
+// take-and-decide pass.
+__device__ __forceinline__ uint64_t sel_key(int32_t x, int32_t d, int desc, int id_bits,
+                                            int deg_bits) {
+    const uint64_t dmax = (deg_bits >= 63) ? ~0ull : ((1ull << deg_bits) - 1);
+    const uint64_t dk = desc ? dmax - (uint64_t)d : (uint64_t)d;
+    return (dk << id_bits) | (uint64_t)(uint32_t)x;
+}
+
+__device__ __forceinline__ int bin1_of(int32_t d, int b1_bins) {
+    return d < b1_bins - 1 ? d : b1_bins - 1;
+}
+
+// What the per-record passes need of a group's state, cached in LDS for the
+// first kSelLdsGroups groups (a global gather per record otherwise).
+constexpr int kSelLdsGroups = 64;
+struct SelLite {
+    int32_t mode, b1, desc, pad;
+    uint64_t kmax;  // SEL_KEY: largest key taken inside bin b1
+};
+__device__ __forceinline__ SelLite sel_lite(const SelState &s) {
+    SelLite l;
+    l.mode = s.mode;
+    l.b1 = s.b1;
+    l.desc = s.desc;
+    l.pad = 0;
+    l.kmax = s.prefix | ((s.r >= 64) ? ~0ull : ((1ull << s.r) - 1));
+    return l;
+}
+__device__ __forceinline__ void sel_lite_load(SelLite *lds, const SelState *st, int groups) {
+    const int ng = groups < kSelLdsGroups ? groups : kSelLdsGroups;
+    for (int i = threadIdx.x; i < ng; i += blockDim.x) lds[i] = sel_lite(st[i]);
+    __syncthreads();
+}
+__device__ __forceinline__ SelLite sel_get(const SelLite *lds, const SelState *st, int g) {
+    return g < kSelLdsGroups ? lds[g] : sel_lite(st[g]);
+}
+
+__device__ __forceinline__ SelArgs gload(const SelArgs &v) {
+    SelArgs r = v;
+    r.in = gload(v.in);
+    r.sel = gload(v.sel);
+    r.carry = gload(v.carry);
+    r.ref = gload(v.ref);
+    r.h1 = gload(v.h1);
+    r.h2 = gload(v.h2);
+    r.st = gload(v.st);
+    r.refining = gload(v.refining);
+    r.done = gload(v.done);
+    r.mark = gload(v.mark);
+    r.mark_val = gload(v.mark_val);
+    r.rule.ratio_tab = gload(v.rule.ratio_tab);
+    r.rule.t_ptr = gload(v.rule.t_ptr);
+    r.rule.re = gload(v.rule.re);
+    r.rdeg = gload(v.rdeg);
+    r.rlab = gload(v.rlab);
+    r.rlab8 = gload(v.rlab8);
+    r.bits = gload(v.bits);
+    return r;
+}
+
+// Group of dense record x.
+__device__ __forceinline__ int32_t dense_group(const SelArgs &a, int64_t x) {
+    return a.rlab8 ? (int32_t)a.rlab8[x] : a.rlab[x];
+}
+
+constexpr int kDenseItems = 4;  // dense passes: 64-id groups per wave per iteration
+
+// Whether dense record v (degree d, its group's state s) is taken.
+__device__ __forceinline__ bool dense_take(const SelLite &s, int32_t v, int32_t d, int b1_bins,
+                                           int id_bits, int deg_bits) {
+    const int b = bin1_of(d, b1_bins);
+    const bool earlier = s.desc ? b > s.b1 : b < s.b1;
+    switch (s.mode) {
+        case SEL_ALL: return true;
+        case SEL_THROUGH: return earlier || b == s.b1;
+        case SEL_KEY: return earlier || (b == s.b1 && sel_key(v, d, s.desc, id_bits, deg_bits) <= s.kmax);
+        default: return false;
+    }
+}
+
 SelArgs select_args(dmolp_ctx *ctx, SelectWork &w, const RecDev &in, const TakeRule &rule,
                     const int32_t *done, const ListDev &sel, const RecDev *carry,
                     int32_t *mark = nullptr, const int32_t *mark_val = nullptr);
 // Switch a job's selection to dense records (see SelArgs::dense).
-void select_args_dense(SelArgs &a, const int32_t *rdeg, const int32_t *rlab, int64_t rn,
-                       uint32_t *bits, int64_t bwords);
+void select_args_dense(SelArgs &a, const int32_t *rdeg, const int32_t *rlab, const uint8_t *rlab8,
+                       int64_t rn, uint32_t *bits, int64_t bwords);
 
 // Enqueue the whole selection for a set of jobs (args: device array, one
 // SelArgs per job): level-1 histogram, pick, digit passes, compaction.
 // `passes` is the largest digit-pass count over the jobs.
-void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, int passes);
+// `compact` false leaves the dense jobs' final take pass (bitmap and counts)
+// to the caller, which must then run dense_take over every record itself.
+void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, int passes,
+                    bool compact = true);
 
 // Block counts of one job for the launch policies (jobs with n vertices
 // and nnz entries, `groups` selection groups, `heavy` heavy rows).
