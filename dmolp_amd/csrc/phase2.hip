@@ -198,14 +198,21 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
                     const int32_t q = q0 + j * kWave + lane;
                     slot[j] = q < total ? batch_slot(s_b[w], q) : -1;
                 }
+                // Loads from in-bounds indices, issued unconditionally (a
+                // lane-conditional load is a branch and a wait of its own).
 #pragma unroll
                 for (int j = 0; j < kLightUnroll; ++j) {
                     const int32_t q = q0 + j * kWave + lane;
-                    u[j] = slot[j] >= 0 ? g.col[s_b[w].base[slot[j]] + q] : -1;
+                    const int64_t e = s_b[w].base[slot[j] >= 0 ? slot[j] : 0] + q;
+                    const int32_t cu = g.col[slot[j] >= 0 ? e : 0];
+                    u[j] = slot[j] >= 0 ? cu : -1;
                 }
                 int32_t b[kLightUnroll];
 #pragma unroll
-                for (int j = 0; j < kLightUnroll; ++j) b[j] = u[j] >= 0 ? (int32_t)L8[u[j]] : -1;
+                for (int j = 0; j < kLightUnroll; ++j) {
+                    const int32_t lb = L8[u[j] >= 0 ? u[j] : 0];
+                    b[j] = u[j] >= 0 ? lb : -1;
+                }
 #pragma unroll
                 for (int j = 0; j < kLightUnroll; ++j) {
                     if (b[j] < 0) continue;
@@ -315,15 +322,20 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
             // would; their vertices raise the kernel error anyway.
             int32_t inv = 0;
             for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
+                // In-bounds indices: the loads issue unconditionally.
                 int32_t u[kHeavyUnroll];
 #pragma unroll
                 for (int q = 0; q < kHeavyUnroll; ++q) {
                     const int64_t e = eb + q * kWave + lane;
-                    u[q] = e < e1 ? __builtin_nontemporal_load(&g.col[e]) : -1;
+                    const int32_t cu = __builtin_nontemporal_load(&g.col[e < e1 ? e : e1 - 1]);
+                    u[q] = e < e1 ? cu : -1;
                 }
                 int32_t lab[kHeavyUnroll];
 #pragma unroll
-                for (int q = 0; q < kHeavyUnroll; ++q) lab[q] = u[q] >= 0 ? (int32_t)L8[u[q]] : -1;
+                for (int q = 0; q < kHeavyUnroll; ++q) {
+                    const int32_t lb = L8[u[q] >= 0 ? u[q] : 0];
+                    lab[q] = u[q] >= 0 ? lb : -1;
+                }
 #pragma unroll
                 for (int q = 0; q < kHeavyUnroll; ++q) {
                     if (lab[q] >= 0 && lab[q] < E) row[lab[q]] += 1;
@@ -635,17 +647,21 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
         // would serialise them): first every per-vertex word, the row bounds
         // and witness included even for interior vertices, then the
         // witness's bit.
+        // Loads from a clamped index (in bounds for every lane), so they are
+        // issued unconditionally, back to back, and waited for once.
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             const bool in = v < g.n;
-            const int32_t dv = in && use_diff ? diff[v] : 1;
-            heavy_v[j] = in && use_diff && bit_of(J.hbits, (int32_t)v);
-            xs[j] = in ? g.off[v] : 0;
-            end[j] = in ? g.off[v + 1] : 0;
-            fnb[j] = in ? J.wit[v] : -1;
-            hit[j] = in && bit_of(inb, (int32_t)v);
-            cand[j] = in && (heavy_v[j] || dv != 0);  // light interior: never a record
+            const int64_t vi = in ? v : g.n - 1;
+            const int32_t dv = diff[vi];
+            const uint32_t hw = J.hbits[vi >> 5], bw = inb[vi >> 5];
+            xs[j] = g.off[vi];
+            end[j] = g.off[vi + 1];
+            fnb[j] = J.wit[vi];
+            heavy_v[j] = in && use_diff && ((hw >> (vi & 31)) & 1u);
+            hit[j] = in && ((bw >> (vi & 31)) & 1u);
+            cand[j] = in && (heavy_v[j] || !use_diff || dv != 0);  // light interior: never a record
             known[j] = use_diff && !heavy_v[j];
         }
 #pragma unroll
@@ -657,15 +673,19 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
                 known[j] = true;
             }
         }
+        // v itself, then its witness: the neighbour that last made v a
+        // candidate (at first the row's smallest id, in these graphs usually
+        // a hub).  The witnesses' bitmap words are gathered together.
+        uint32_t wword[kScanItems];
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) wword[j] = inb[(fnb[j] >= 0 ? fnb[j] : 0) >> 5];
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             if (!cand[j]) continue;
-            // v itself, then its witness: the neighbour that last made v a
-            // candidate (at first the row's smallest id, in these graphs
-            // usually a hub).  The row's differing label is left to the
-            // scan unless the kept counts already tell.
-            if (!hit[j] && fnb[j] >= 0) hit[j] = bit_of(inb, fnb[j]);
+            // The row's differing label is left to the scan unless the kept
+            // counts already tell.
+            if (!hit[j] && fnb[j] >= 0) hit[j] = (wword[j] >> (fnb[j] & 31)) & 1u;
             if (hit[j] && known[j]) {
                 rec |= 1u << j;
             } else if (deg[j] > heavy_degree) {
@@ -724,27 +744,37 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull_rest(const P2Job *JS, co
     for (int64_t b0 = vwave(vg) * kWave; b0 < np; b0 += vwaves(vg) * kWave) {
         const int64_t i = b0 + lane_id();
         const bool live = i < np;
-        const int32_t v = live ? pend[i] : 0;
-        const int64_t s0 = live ? g.off[v] : 0;
-        const int64_t lim = live ? g.off[v + 1] : 0;
-        const int32_t lab = live ? L[v] : 0;
+        // Loads from in-bounds indices, issued unconditionally (a
+        // lane-conditional load is a branch and a wait of its own).
+        const int32_t v = pend[live ? i : np - 1];
+        const int64_t s0 = g.off[v];
+        const int64_t e0 = g.off[v + 1];
+        const int32_t lab = L[v];
+        const uint32_t hw = J.hbits[v >> 5], bw = inb[v >> 5];
+        const int64_t lim = live ? e0 : s0;
         int64_t xs = s0;
-        const bool heavy_v = live && use_diff && bit_of(J.hbits, v);
+        const bool heavy_v = live && use_diff && ((hw >> (v & 31)) & 1u);
         // As in k_cand_pull: a pending row passed its boundary test, which
         // the kept counts answer for light rows and (k <= 32) heavy ones.
         bool dif = !live || (use_diff && (!heavy_v || J.k <= kLpCap));
-        bool hit = !live || bit_of(inb, v);
+        bool hit = !live || ((bw >> (v & 31)) & 1u);
         int32_t found = -1;  // the neighbour that answers `hit`, kept as v's witness
         const int64_t stop = xs + kRestLane < lim ? xs + kRestLane : lim;
         while (xs < stop && !(hit && dif)) {
             int32_t u[kRestUnroll];
 #pragma unroll
-            for (int q = 0; q < kRestUnroll; ++q) u[q] = xs + q < stop ? g.col[xs + q] : -1;
+            for (int q = 0; q < kRestUnroll; ++q) {
+                const int32_t cu = g.col[xs + q < stop ? xs + q : xs];  // xs < stop: in the row
+                u[q] = xs + q < stop ? cu : -1;
+            }
+            uint32_t uw[kRestUnroll];
+#pragma unroll
+            for (int q = 0; q < kRestUnroll; ++q) uw[q] = inb[(u[q] >= 0 ? u[q] : 0) >> 5];
 #pragma unroll
             for (int q = 0; q < kRestUnroll; ++q) {
                 if (u[q] < 0) continue;
                 if (!dif) dif = nb_label(exact8, L8, L, u[q]) != lab;
-                if (!hit && bit_of(inb, u[q])) {
+                if (!hit && ((uw[q] >> (u[q] & 31)) & 1u)) {
                     hit = true;
                     found = u[q];
                 }
@@ -1103,6 +1133,9 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_take(const P2Job *JS, con
     __shared__ int32_t s_dq[kDStage];
     __shared__ int32_t s_dn;
     __shared__ unsigned long long s_base;
+    __shared__ int32_t s_av[kWavesPerBlock][kWave * kDenseItems];  // light members: id,
+    __shared__ int32_t s_ao[kWavesPerBlock][kWave * kDenseItems];  // own label,
+    __shared__ int32_t s_ad[kWavesPerBlock][kWave * kDenseItems];  // degree
     const int E = k < kLpCap ? k : kLpCap;
     const int nvec = (E + 7) / 8;
     if ((int)threadIdx.x < E) {
@@ -1127,13 +1160,9 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_take(const P2Job *JS, con
     for (int64_t base = vg.block * per_block; base < ng; base += vg.blocks * per_block) {
         const int64_t g0 = base + (int64_t)w * kDenseItems;
         int32_t d[kDenseItems], gg[kDenseItems];
-#pragma unroll
-        for (int j = 0; j < kDenseItems; ++j) {
-            const int64_t v = (g0 + j) * kWave + lane;
-            d[j] = v < rn ? rdeg[v] : -1;
-            gg[j] = d[j] >= 0 ? dense_group(a, v) : 0;
-        }
-        uint32_t act = 0, cm = 0;
+        if (a.rlab8) dense_load<true>(a, g0 * kWave + lane, kWave, d, gg);
+        else dense_load<false>(a, g0 * kWave + lane, kWave, d, gg);
+        uint32_t act = 0;
 #pragma unroll
         for (int j = 0; j < kDenseItems; ++j) {
             const int64_t gi = g0 + j;
@@ -1156,10 +1185,28 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_take(const P2Job *JS, con
                 }
             }
         }
+        // The wave's light members are packed into LDS, then decided one per
+        // lane: the rows of a full batch are loaded together instead of one
+        // item's (sparse) lanes at a time.
+        int32_t na = 0;
 #pragma unroll
         for (int j = 0; j < kDenseItems; ++j) {
-            if (!((act >> j) & 1u)) continue;
-            const int32_t v = (int32_t)((g0 + j) * kWave + lane);
+            const bool on = (act >> j) & 1u;
+            const uint64_t m = __ballot(on);
+            if (on) {
+                const int32_t pos = na + __popcll(m & ((1ull << lane) - 1ull));
+                s_av[w][pos] = (int32_t)((g0 + j) * kWave + lane);
+                s_ao[w][pos] = gg[j];
+                s_ad[w][pos] = d[j];
+            }
+            na += __popcll(m);
+        }
+        wave_sync();
+        for (int32_t b = 0; b < na; b += kWave) {  // wave-uniform
+            const int32_t i = b + lane;
+            if (i >= na) continue;
+            const int32_t v = s_av[w][i];
+            const int32_t own = s_ao[w][i];  // records carry valid labels
             const uint4 *row = reinterpret_cast<const uint4 *>(lh + (int64_t)v * kPackedWords);
             uint32_t c[kPackedWords];
 #pragma unroll
@@ -1170,7 +1217,6 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_take(const P2Job *JS, con
                 c[4 * q + 2] = x.z;
                 c[4 * q + 3] = x.w;
             }
-            const int32_t own = gg[j];  // records carry valid labels
             const auto word = [&](int q) { return pick16(c, q); };
             const int32_t nl =
                 (own >= 0 && own < E)
@@ -1178,18 +1224,13 @@ __global__ __launch_bounds__(kThreads) void k_lp_light_take(const P2Job *JS, con
                     : lp_decide_by([&](int l) { return (int32_t)((word(l >> 1) >> ((l & 1) * 16)) & 0xffffu); },
                                    E, s_pen, own);
             if (nl != own) {
-                cm |= 1u << j;
                 chg[v] = nl;
                 LN[v] = ln_pair(own, nl);
-                dsum += d[j];
+                dsum += s_ad[w][i];
+                // Changed vertices are staged in LDS and flushed to Delta
+                // with one global reservation per kDStage / 2 or more entries.
+                s_dq[atomicAdd(&s_dn, 1)] = v;
             }
-        }
-        // Changed vertices are staged in LDS and flushed to Delta with one
-        // global reservation per kDStage / 2 or more entries.
-#pragma unroll
-        for (int j = 0; j < kDenseItems; ++j) {
-            if (!(cm & (1u << j))) continue;
-            s_dq[atomicAdd(&s_dn, 1)] = (int32_t)((g0 + j) * kWave + lane);
         }
         __syncthreads();
         if (s_dn > kDStage - (int)(per_block * kWave)) d_flush(D, s_dq, s_dn, s_base);
@@ -1321,33 +1362,16 @@ __global__ void k_lp_heavy_decide(const P2Job *JS, const int32_t *bstart, int jo
 // own old and new labels.  `same_out` tells whether u's new label is c (every
 // entry: v's new differing count is its degree minus these).  When u keeps
 // its label, its differing count moves here (keep_diff).
-__device__ __forceinline__ int32_t pair_effect(int32_t v, int32_t a, int32_t c, int32_t u,
-                                               const int32_t *L, const int32_t *chg,
-                                               const uint16_t *LN, bool exact8,
-                                               bool keep_diff, int32_t *diff, int32_t &dd_out,
-                                               bool &same_out) {
+// u's old label lu, its label after this round lu2, and whether it changed
+// (uchg) come from the caller: pair_effect below, or a caller that gathered
+// the LN words of several entries together.
+__device__ __forceinline__ int32_t pair_effect_of(int32_t v, int32_t a, int32_t c, int32_t u, int32_t lu,
+                                                  int32_t lu2, bool uchg, bool keep_diff, int32_t *diff,
+                                                  int32_t &dd_out, bool &same_out) {
     dd_out = 0;
-    int32_t lu, lu2;
-    bool uchg;
-    if (exact8) {  // one 16-bit gather: old label byte | this round's new one << 8 (255: unchanged)
-        const uint32_t ln = LN[u];
-        const int32_t nu = (int32_t)(ln >> 8);
-        uchg = nu != kLabel8None;
-        if (uchg) {
-            same_out = nu == c;
-            if (u < v) return 0;  // counted from u's side
-        }
-        lu = (int32_t)(ln & 0xffu);
-        lu2 = uchg ? nu : lu;
-    } else {
-        const int32_t cu = chg[u];
-        uchg = cu >= 0;
-        if (uchg) {
-            same_out = cu == c;
-            if (u < v) return 0;  // counted from u's side
-        }
-        lu = L[u];
-        lu2 = uchg ? cu : lu;
+    if (uchg) {
+        same_out = lu2 == c;
+        if (u < v) return 0;  // counted from u's side
     }
     if (!uchg) {
         same_out = lu == c;
@@ -1357,6 +1381,31 @@ __device__ __forceinline__ int32_t pair_effect(int32_t v, int32_t a, int32_t c, 
     const int32_t dd = (c != lu2 ? 1 : 0) - (a != lu ? 1 : 0);
     dd_out = dd;
     return (a == lu ? 1 : 0) | (c == lu2 ? 1 << 16 : 0);
+}
+// LN word of a neighbour: old label byte | this round's new one << 8 (255:
+// unchanged).
+__device__ __forceinline__ void ln_labels(uint32_t ln, int32_t &lu, int32_t &lu2, bool &uchg) {
+    const int32_t nu = (int32_t)(ln >> 8);
+    uchg = nu != kLabel8None;
+    lu = (int32_t)(ln & 0xffu);
+    lu2 = uchg ? nu : lu;
+}
+__device__ __forceinline__ int32_t pair_effect(int32_t v, int32_t a, int32_t c, int32_t u,
+                                               const int32_t *L, const int32_t *chg,
+                                               const uint16_t *LN, bool exact8,
+                                               bool keep_diff, int32_t *diff, int32_t &dd_out,
+                                               bool &same_out) {
+    int32_t lu, lu2;
+    bool uchg;
+    if (exact8) {  // one 16-bit gather
+        ln_labels(LN[u], lu, lu2, uchg);
+    } else {
+        const int32_t cu = chg[u];
+        uchg = cu >= 0;
+        lu = L[u];
+        lu2 = uchg ? cu : lu;
+    }
+    return pair_effect_of(v, a, c, u, lu, lu2, uchg, keep_diff, diff, dd_out, same_out);
 }
 
 // Row upkeep of the kept histograms, done while Delta's rows are read (the
@@ -1406,8 +1455,8 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
     unsigned long long *ic = J.ic;
     if (is_done(ctl)) return;
     const bool lh_on = J.lh_on != 0, hh_push = ctl->hist_mode == HIST_PUSH;
-    const bool upkeep = lh_on || hh_push;
     const bool keep_diff = ctl->diff_ok != 0;
+    const int64_t nheavy = (int64_t)J.all.cnt[LC_SLOTS];
     int32_t *diff = J.diff;
     const int E = k < kLpCap ? k : kLpCap;
     __shared__ BatchLds s_b[kWavesPerBlock];
@@ -1429,31 +1478,68 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
         s_b[w].aux[lane] = valid ? L[v] : 0;  // old label of the slot's vertex
         s_new[w][lane] = valid ? chg[v] : 0;  // its new label
         wave_sync();
-        for (int32_t q0 = 0; q0 < total; q0 += kWave * kLpUnroll) {
-            int slot[kLpUnroll];
-            int32_t u[kLpUnroll];
-#pragma unroll
-            for (int j = 0; j < kLpUnroll; ++j) {
-                const int32_t q = q0 + j * kWave + lane;
-                slot[j] = q < total ? batch_slot(s_b[w], q) : -1;
-                u[j] = slot[j] >= 0 ? g.col[s_b[w].base[slot[j]] + q] : -1;
+        // Two versions of the entry loop, with and without the heavy-slot
+        // gather, so that each issues its loads unconditionally.
+        auto entries = [&](auto push) {
+            constexpr bool Push = decltype(push)::value;
+            for (int32_t q0 = 0; q0 < total; q0 += kWave * kLpUnroll) {
+                // Three stages of loads, each issued for all kLpUnroll entries
+                // together from in-bounds indices (a lane-conditional load would
+                // become a branch and a wait of its own): the columns, then each
+                // neighbour's heavy bit word and LN word (and heavy slot when
+                // heavy rows take pushed counts), then the updates.
+                int slot[kLpUnroll];
+                int32_t u[kLpUnroll];
+    #pragma unroll
+                for (int j = 0; j < kLpUnroll; ++j) {
+                    const int32_t q = q0 + j * kWave + lane;
+                    slot[j] = q < total ? batch_slot(s_b[w], q) : -1;
+                    const int64_t e = s_b[w].base[slot[j] >= 0 ? slot[j] : 0] + q;
+                    const int32_t cu = g.col[slot[j] >= 0 ? e : 0];
+                    u[j] = slot[j] >= 0 ? cu : -1;
+                }
+                uint32_t hw[kLpUnroll], ln[kLpUnroll], hs[kLpUnroll];  // hs: unsigned, no sign extension to wait for
+    #pragma unroll
+                for (int j = 0; j < kLpUnroll; ++j) {
+                    const int32_t uc = u[j] >= 0 ? u[j] : 0;
+                    hw[j] = J.hbits[uc >> 5];
+                    ln[j] = exact8 ? (uint32_t)LN[uc] : 0u;
+                    hs[j] = Push ? (uint32_t)J.hid[uc] : 0u;  // unconditional within the version
+                }
+    #pragma unroll
+                for (int j = 0; j < kLpUnroll; ++j) {
+                    if (slot[j] < 0) continue;
+                    const int32_t vv = s_b[w].v[slot[j]];
+                    const int32_t a = s_b[w].aux[slot[j]], c = s_new[w][slot[j]];
+                    int32_t dd, lu, lu2;
+                    bool same, uchg;
+                    const bool uh = (hw[j] >> (u[j] & 31)) & 1u;
+                    if (exact8) {
+                        ln_labels(ln[j], lu, lu2, uchg);
+                    } else {
+                        const int32_t cu = chg[u[j]];
+                        uchg = cu >= 0;
+                        lu = L[u[j]];
+                        lu2 = uchg ? cu : lu;
+                    }
+                    const int32_t io = pair_effect_of(vv, a, c, u[j], lu, lu2, uchg, keep_diff && !uh, diff, dd, same);
+                    cut_delta += dd;
+                    if (io) atomicAdd(&s_io[w][slot[j]], io);
+                    if (same) atomicAdd(&s_same[w][slot[j]], 1);
+                    if (uh) {
+                        if (Push) {
+                            int32_t *row = J.hhc + ((int64_t)(blockIdx.x % kHhCopies) * nheavy + (int64_t)hs[j]) * kLpCap;
+                            if (a >= 0 && a < E) atomicAdd(&row[a], -1);
+                            if (c >= 0 && c < E) atomicAdd(&row[c], 1);
+                        }
+                    } else if (lh_on) {
+                        lh_move(J.lh, u[j], a, c, E);
+                    }
+                }
             }
-#pragma unroll
-            for (int j = 0; j < kLpUnroll; ++j) {
-                if (slot[j] < 0) continue;
-                const int32_t vv = s_b[w].v[slot[j]];
-                const int32_t a = s_b[w].aux[slot[j]], c = s_new[w][slot[j]];
-                int32_t dd;
-                bool same;
-                const bool uh = bit_of(J.hbits, u[j]);
-                const int32_t io = pair_effect(vv, a, c, u[j], L, chg, LN, exact8, keep_diff && !uh,
-                                               diff, dd, same);
-                cut_delta += dd;
-                if (io) atomicAdd(&s_io[w][slot[j]], io);
-                if (same) atomicAdd(&s_same[w][slot[j]], 1);
-                if (upkeep) row_upkeep(J, lh_on, hh_push, uh, u[j], a, c, E);
-            }
-        }
+        };
+        if (hh_push) entries(std::true_type{});
+        else entries(std::false_type{});
         wave_sync();
         if (valid) {
             const int32_t a = L[v], c = chg[v];

@@ -63,25 +63,36 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(const SelArgs *A, con
     __syncthreads();
     const bool dense = a.dense != 0;
     const int64_t count = dense ? a.rn : (int64_t)*in.cnt;
-    constexpr int kH1Items = 4;  // independent record loads in flight per thread
+    constexpr int kH1Items = 8;  // independent record loads in flight per thread
     const int64_t step = vthreads(vg);
-    for (int64_t i0 = vthread(vg); i0 < count; i0 += kH1Items * step) {
-        int32_t gg[kH1Items], dd[kH1Items];
+    // M: 0 listed records, 1 dense with int32 groups, 2 dense with group bytes.
+    auto run = [&](auto mode) {
+        constexpr int M = decltype(mode)::value;
+        for (int64_t i0 = vthread(vg); i0 < count; i0 += kH1Items * step) {
+            int32_t gg[kH1Items], dd[kH1Items];
+            if (M == 0) {
 #pragma unroll
-        for (int j = 0; j < kH1Items; ++j) {
-            const int64_t i = i0 + j * step;
-            if (dense) {
-                dd[j] = i < count ? a.rdeg[i] : -1;
-                gg[j] = dd[j] >= 0 ? dense_group(a, i) : -1;
+                for (int j = 0; j < kH1Items; ++j) {
+                    const int64_t i = i0 + j * step;
+                    const int64_t ic = i < count ? i : count - 1;  // loads issued together
+                    const int32_t g = in.g[ic];
+                    dd[j] = in.d[ic];
+                    gg[j] = i < count ? g : -1;
+                }
             } else {
-                gg[j] = i < count ? in.g[i] : -1;
-                dd[j] = i < count ? in.d[i] : 0;
-            }
-        }
+                dense_load<M == 2>(a, i0, step, dd, gg);
 #pragma unroll
-        for (int j = 0; j < kH1Items; ++j)
-            if (gg[j] >= 0) atomicAdd(&s_h[gg[j] * b1_bins + bin1_of(dd[j], b1_bins)], 1);
-    }
+                for (int j = 0; j < kH1Items; ++j)
+                    if (dd[j] < 0) gg[j] = -1;
+            }
+#pragma unroll
+            for (int j = 0; j < kH1Items; ++j)
+                if (gg[j] >= 0) atomicAdd(&s_h[gg[j] * b1_bins + bin1_of(dd[j], b1_bins)], 1);
+        }
+    };
+    if (!dense) run(std::integral_constant<int, 0>{});
+    else if (a.rlab8) run(std::integral_constant<int, 2>{});
+    else run(std::integral_constant<int, 1>{});
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += blockDim.x)
         if (s_h[i]) atomicAdd(&h1[i], s_h[i]);
@@ -241,24 +252,23 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect_dense(const SelArgs
     const int b1_bins = a.b1_bins;
     const int64_t count = a.rn;
     const int64_t span = (int64_t)blockDim.x * kCompactItems;
-    for (int64_t b0 = vg.block * span; b0 < count; b0 += vg.blocks * span) {
-        int32_t d[kCompactItems], gg[kCompactItems];
+    auto run = [&](auto byte) {
+        for (int64_t b0 = vg.block * span; b0 < count; b0 += vg.blocks * span) {
+            int32_t d[kCompactItems], gg[kCompactItems];
+            dense_load<decltype(byte)::value>(a, b0 + threadIdx.x, blockDim.x, d, gg);
 #pragma unroll
-        for (int j = 0; j < kCompactItems; ++j) {
-            const int64_t i = b0 + (int64_t)j * blockDim.x + threadIdx.x;
-            d[j] = i < count ? a.rdeg[i] : -1;
-            gg[j] = d[j] >= 0 ? dense_group(a, i) : 0;
+            for (int j = 0; j < kCompactItems; ++j) {
+                if (d[j] < 0) continue;
+                const SelLite s = sel_get(s_st, st, gg[j]);
+                if (s.mode == SEL_REFINE && bin1_of(d[j], b1_bins) == s.b1)
+                    s_x[atomicAdd(&s_n, 1)] = (int32_t)(b0 + (int64_t)j * blockDim.x + threadIdx.x);
+            }
+            __syncthreads();
+            if (s_n > kRefStage - (int)span) ref_flush(a, s_x, s_n, s_base);
         }
-#pragma unroll
-        for (int j = 0; j < kCompactItems; ++j) {
-            if (d[j] < 0) continue;
-            const SelLite s = sel_get(s_st, st, gg[j]);
-            if (s.mode == SEL_REFINE && bin1_of(d[j], b1_bins) == s.b1)
-                s_x[atomicAdd(&s_n, 1)] = (int32_t)(b0 + (int64_t)j * blockDim.x + threadIdx.x);
-        }
-        __syncthreads();
-        if (s_n > kRefStage - (int)span) ref_flush(a, s_x, s_n, s_base);
-    }
+    };
+    if (a.rlab8) run(std::true_type{});
+    else run(std::false_type{});
     __syncthreads();
     if (s_n > 0) ref_flush(a, s_x, s_n, s_base);
 }
@@ -446,12 +456,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs
     int64_t n_light = 0, deg_sum = 0, n_heavy = 0, heavy_deg = 0;
     for (int64_t g0 = vwave(vg) * kDenseItems; g0 < ng; g0 += vwaves(vg) * kDenseItems) {
         int32_t d[kDenseItems], gg[kDenseItems];
-#pragma unroll
-        for (int j = 0; j < kDenseItems; ++j) {
-            const int64_t v = (g0 + j) * kWave + lane;
-            d[j] = v < rn ? rdeg[v] : -1;
-            gg[j] = d[j] >= 0 ? dense_group(a, v) : 0;
-        }
+        if (a.rlab8) dense_load<true>(a, g0 * kWave + lane, kWave, d, gg);
+        else dense_load<false>(a, g0 * kWave + lane, kWave, d, gg);
 #pragma unroll
         for (int j = 0; j < kDenseItems; ++j) {
             const int64_t gi = g0 + j;

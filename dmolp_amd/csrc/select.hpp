@@ -12,6 +12,7 @@
 // histogram, so the selected set does not depend on record order.
 #pragma once
 #include <array>
+#include <type_traits>
 #include <vector>
 
 #include "internal.hpp"
@@ -184,6 +185,24 @@ __device__ __forceinline__ int32_t dense_group(const SelArgs &a, int64_t x) {
 }
 
 constexpr int kDenseItems = 4;  // dense passes: 64-id groups per wave per iteration
+
+// Degree and group of dense records i0 + j * stride (j < N), d = -1 past rn
+// or for a non-record.  The indices are clamped into [0, rn), so all 2N
+// loads are issued together and waited for once (a load under a lane
+// condition becomes a branch and a wait of its own).  The group of a
+// non-record is read but meaningless.  Byte: groups from rlab8.
+template <bool Byte, int N>
+__device__ __forceinline__ void dense_load(const SelArgs &a, int64_t i0, int64_t stride, int32_t (&d)[N],
+                                           int32_t (&g)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int64_t i = i0 + (int64_t)j * stride;
+        const int64_t ic = i < a.rn ? i : a.rn - 1;
+        const int32_t dv = a.rdeg[ic];
+        g[j] = Byte ? (int32_t)a.rlab8[ic] : a.rlab[ic];
+        d[j] = i < a.rn ? dv : -1;
+    }
+}
 
 // Whether dense record v (degree d, its group's state s) is taken.
 __device__ __forceinline__ bool dense_take(const SelLite &s, int32_t v, int32_t d, int b1_bins,
