@@ -271,7 +271,10 @@ int dmolp_ctx_get_tuning(const dmolp_ctx *ctx, dmolp_tuning *out) {
 int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
     if (!ctx || !t) return fail(ctx, DMOLP_E_ARG, "null argument");
     // Light rows keep 16-bit per-row counters: heavy_degree stays below 2^16.
+    // A lane of the heavy initial sweep counts up to chunk_edges / 64 entries
+    // in 16-bit counters: chunk_edges stays at most 2^21.
     if (t->heavy_degree < 0 || t->heavy_degree > 65535 || t->chunk_edges < 0 ||
+        t->chunk_edges > (1 << 21) ||
         t->grid_blocks < 0 || t->pull_per_mille < 0 || t->pull_per_mille > 1000000 ||
         t->batch_work_per_block < 0 || t->hist_recount_per_mille < 0 ||
         t->hist_recount_per_mille > 1001 || t->hist_push_per_mille < 0 ||

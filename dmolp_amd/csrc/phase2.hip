@@ -30,7 +30,7 @@ constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr int kHistStride = kLpCap + 1;  // odd stride: no LDS bank conflicts across slots
 // Per-block label counters (vc, ic) live in LDS for labels below kLdsK;
 // larger labels (k above kLdsK) add to the global counters directly.
-constexpr int kLdsK = 256;
+constexpr int kLdsK = 64;
 constexpr int kLpUnroll = 4;     // light rows: gathers in flight per lane
 constexpr int kHeavyUnroll = 8;  // heavy rows: gathers in flight per lane
 constexpr int kLightUnroll = 4;  // light rows in the initial sweep: gathers in flight per lane
@@ -150,7 +150,7 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
 // phase2.cpp:55-117 and 214-253 over all vertices, plus the per-vertex count
 // of differing neighbours used as the boundary test, and (lh_on) each light
 // row's label counts for the kept histograms.
-__global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const int32_t *bstart, int jobs) {
+__global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
     const ListDev all = J.all;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kThreads) void k_init_light(const P2Job *JS, const 
     label_flush(s_ic, ic, k);
 }
 
-__global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
+__global__ __launch_bounds__(kThreads, 6) void k_init_heavy(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
     const ListDev all = J.all;
@@ -300,10 +300,19 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
     const bool exact8 = k <= kExact8;
     const int E = k < kLpCap ? k : kLpCap;
     // Per-lane histogram rows of the row's neighbour labels (below E), the
-    // heavy row's starting histogram for the LP rounds.
-    __shared__ int32_t s_h[kWavesPerBlock][kWave * kHistStride];
+    // heavy row's starting histogram for the LP rounds: 16-bit counts, two
+    // labels a word (a lane counts at most chunk_edges / 64 < 2^16 entries
+    // of a chunk), which halves the LDS and so raises the occupancy.
+    __shared__ uint32_t s_h[kWavesPerBlock][kWave * kPackedStride];
     const int w = threadIdx.x / kWave, lane = lane_id();
-    int32_t *row = &s_h[w][lane * kHistStride];
+    uint32_t *row = &s_h[w][lane * kPackedStride];
+    const int nw = (E + 1) / 2;  // words of a row in use
+    // Column lane of the chunk's label-lane count (lane < E).
+    const auto col_count = [&](int l) {
+        int32_t c = 0;
+        for (int r = 0; r < kWave; ++r) c += (int32_t)((s_h[w][r * kPackedStride + (l >> 1)] >> ((l & 1) * 16)) & 0xffffu);
+        return c;
+    };
     int64_t invalid = 0, cut2 = 0;
     const int64_t nch = (int64_t)all.cnt[LC_CHUNKS];
     for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
@@ -313,7 +322,7 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
         chunk_bounds(g, all, item, chunk_edges, slot, v, e0, e1, deg);
         const int32_t lv = L[v];
         int64_t dcount = 0, intra = 0;
-        for (int l = 0; l < E; ++l) row[l] = 0;
+        for (int l = 0; l < nw; ++l) row[l] = 0u;
         if (k <= kLpCap) {
             // Every label is counted (E = k): the chunk's same, invalid and
             // differing counts follow from its histogram, so the entry loop
@@ -338,7 +347,7 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
                 }
 #pragma unroll
                 for (int q = 0; q < kHeavyUnroll; ++q) {
-                    if (lab[q] >= 0 && lab[q] < E) row[lab[q]] += 1;
+                    if (lab[q] >= 0 && lab[q] < E) row[lab[q] >> 1] += 1u << ((lab[q] & 1) * 16);
                     inv += lab[q] == kLabel8None ? 1 : 0;
                 }
             }
@@ -346,7 +355,7 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
             wave_sync();
             int32_t c = 0;
             if (lane < E) {
-                for (int r = 0; r < kWave; ++r) c += s_h[w][r * kHistStride + lane];
+                c = col_count(lane);
                 if (c) atomicAdd(&hh[(int64_t)slot * kLpCap + lane], c);
             }
             const int32_t same = __shfl(c, lv >= 0 && lv < E ? lv : 0, kWave);
@@ -373,7 +382,7 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
         for (int64_t e = e0 + lane; e < e1; e += kWave) {
             const int32_t u = g.col[e];
             const int32_t b = L8[u];
-            if (b < E) row[b] += 1;
+            if (b < E) row[b >> 1] += 1u << ((b & 1) * 16);
             const int32_t lu = exact8 ? (b == kLabel8None ? -1 : b) : L[u];
             if (lv == -1 || lu == -1) ++invalid;
             else if (lv != lu) { ++cut2; ++dcount; }
@@ -383,8 +392,7 @@ __global__ __launch_bounds__(kThreads) void k_init_heavy(const P2Job *JS, const 
         intra = wave_sum(intra);
         wave_sync();
         if (lane < E) {
-            int32_t c = 0;
-            for (int r = 0; r < kWave; ++r) c += s_h[w][r * kHistStride + lane];
+            const int32_t c = col_count(lane);
             if (c) atomicAdd(&hh[(int64_t)slot * kLpCap + lane], c);
         }
         wave_sync();
@@ -1107,7 +1115,7 @@ __device__ __forceinline__ void d_flush(const ListDev &D, const int32_t *s_dq, i
 // only the vertex's own row and the round's penalties, so it can follow its
 // take decision directly.  The rows are read as ascending addresses, which
 // keeps the DRAM pages open: the rows table is far larger than the caches.
-__global__ __launch_bounds__(kThreads) void k_lp_light_take(const P2Job *JS, const SelArgs *SA,
+__global__ __launch_bounds__(kThreads, 6) void k_lp_light_take(const P2Job *JS, const SelArgs *SA,
                                                             const int32_t *bstart, int jobs) {
     P2_JOB();
     const SelArgs a = gload(SA[job_]);
@@ -1654,18 +1662,37 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
     const Ctl *ctl = J.ctl;
     if (is_done(ctl)) return;
     const int64_t nl = (int64_t)D.cnt[LC_LIGHT], ns = (int64_t)D.cnt[LC_SLOTS];
-    for (int64_t i = vthread(vg); i < nl + ns;
-         i += vthreads(vg)) {
-        const int32_t v = i < nl ? D.light[i] : D.hv[i - nl];
-        const int32_t nl_v = chg[v];
-        L[v] = nl_v;
-        const uint32_t ob = L8[v];
-        const uint32_t nb = (nl_v >= 0 && nl_v < kLabel8None) ? (uint32_t)nl_v : (uint32_t)kLabel8None;
-        ON[v] = (uint16_t)(ob | (nb << 8));
-        L8[v] = (uint8_t)nb;
-        LN[v] = (uint16_t)(nb | (kLabel8None << 8));
-        chg[v] = -1;
-        atomicOr(&cbits[v >> 5], 1u << (v & 31));
+    const int64_t total = nl + ns;
+    constexpr int kApplyItems = 4;  // changed vertices per thread per step, loads issued together
+    const int64_t step = vthreads(vg);
+    for (int64_t i0 = vthread(vg); i0 < total; i0 += kApplyItems * step) {
+        int32_t v[kApplyItems], nv[kApplyItems];
+        uint32_t ob[kApplyItems];
+#pragma unroll
+        for (int j = 0; j < kApplyItems; ++j) {
+            const int64_t i = i0 + j * step;
+            const int64_t ic = i < total ? i : total - 1;  // in bounds: the load issues unconditionally
+            const int32_t a = D.light[ic < nl ? ic : 0], b = D.hv[ic < nl ? 0 : ic - nl];
+            v[j] = ic < nl ? a : b;
+        }
+#pragma unroll
+        for (int j = 0; j < kApplyItems; ++j) {
+            nv[j] = chg[v[j]];
+            ob[j] = L8[v[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < kApplyItems; ++j) {
+            if (i0 + j * step >= total) continue;
+            const int32_t u = v[j];
+            const int32_t nl_v = nv[j];
+            L[u] = nl_v;
+            const uint32_t nb = (nl_v >= 0 && nl_v < kLabel8None) ? (uint32_t)nl_v : (uint32_t)kLabel8None;
+            ON[u] = (uint16_t)(ob[j] | (nb << 8));
+            L8[u] = (uint8_t)nb;
+            LN[u] = (uint16_t)(nb | (kLabel8None << 8));
+            chg[u] = -1;
+            atomicOr(&cbits[u >> 5], 1u << (u & 31));
+        }
     }
 }
 
@@ -1770,15 +1797,20 @@ __global__ __launch_bounds__(kThreads) void k_hist_recount(const P2Job *JS, cons
         chunk_bounds(g, all, all.hc[j], chunk_edges, slot, v, e0, e1, deg);
         for (int l = 0; l < E; ++l) row[l] = 0;
         for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
+            // In-bounds indices: the loads issue unconditionally.
             int32_t u[kHeavyUnroll];
 #pragma unroll
             for (int q = 0; q < kHeavyUnroll; ++q) {
                 const int64_t e = eb + q * kWave + lane;
-                u[q] = e < e1 ? __builtin_nontemporal_load(&g.col[e]) : -1;
+                const int32_t cu = __builtin_nontemporal_load(&g.col[e < e1 ? e : e1 - 1]);
+                u[q] = e < e1 ? cu : -1;
             }
             int32_t lab[kHeavyUnroll];
 #pragma unroll
-            for (int q = 0; q < kHeavyUnroll; ++q) lab[q] = u[q] >= 0 ? (int32_t)L8[u[q]] : kLabel8None;
+            for (int q = 0; q < kHeavyUnroll; ++q) {
+                const int32_t lb = L8[u[q] >= 0 ? u[q] : 0];
+                lab[q] = u[q] >= 0 ? lb : kLabel8None;
+            }
 #pragma unroll
             for (int q = 0; q < kHeavyUnroll; ++q)
                 if (lab[q] < E) row[lab[q]] += 1;
@@ -1822,19 +1854,32 @@ __global__ __launch_bounds__(kThreads) void k_hist_update(const P2Job *JS, const
         chunk_bounds(g, all, all.hc[j], chunk_edges, slot, v, e0, e1, deg);
         bool touched = false;
         for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
+            // In-bounds indices: the loads issue unconditionally.
             int32_t u[kHeavyUnroll];
 #pragma unroll
             for (int q = 0; q < kHeavyUnroll; ++q) {
                 const int64_t e = eb + q * kWave + lane;
-                u[q] = e < e1 ? __builtin_nontemporal_load(&g.col[e]) : -1;
+                const int32_t cu = __builtin_nontemporal_load(&g.col[e < e1 ? e : e1 - 1]);
+                u[q] = e < e1 ? cu : -1;
             }
             uint32_t word[kHeavyUnroll];
 #pragma unroll
-            for (int q = 0; q < kHeavyUnroll; ++q) word[q] = u[q] >= 0 ? cbits[u[q] >> 5] : 0u;
+            for (int q = 0; q < kHeavyUnroll; ++q) {
+                const uint32_t cw = cbits[(u[q] >= 0 ? u[q] : 0) >> 5];
+                word[q] = u[q] >= 0 ? cw : 0u;
+            }
+            // Unchanged entries read ON[0] (one shared line), so these loads
+            // too issue unconditionally.
+            uint32_t onw[kHeavyUnroll];
 #pragma unroll
             for (int q = 0; q < kHeavyUnroll; ++q) {
-                if (!((word[q] >> (u[q] & 31)) & 1u)) continue;
-                const uint32_t on = ON[u[q]];
+                const bool ch = u[q] >= 0 && ((word[q] >> (u[q] & 31)) & 1u);
+                onw[q] = ON[ch ? u[q] : 0];
+            }
+#pragma unroll
+            for (int q = 0; q < kHeavyUnroll; ++q) {
+                if (u[q] < 0 || !((word[q] >> (u[q] & 31)) & 1u)) continue;
+                const uint32_t on = onw[q];
                 const int32_t o = (int32_t)(on & 0xffu), nw = (int32_t)(on >> 8);
                 if (o < E) atomicAdd(&wrow[o], -1);
                 if (nw < E) atomicAdd(&wrow[nw], 1);

@@ -290,22 +290,37 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(const SelArgs *A, con
     const int nb = a.groups * kDigitBins;
     const bool lds = nb <= kHist2LdsInts;
     __shared__ int32_t s_h[kHist2LdsInts];
-    if (lds) {
+    __shared__ SelState s_st[kSelLdsGroups];  // the first groups' states
+    for (int i = threadIdx.x; i < a.groups && i < kSelLdsGroups; i += blockDim.x) s_st[i] = st[i];
+    if (lds)
         for (int i = threadIdx.x; i < nb; i += blockDim.x) s_h[i] = 0;
-        __syncthreads();
-    }
-    for (int64_t i = vthread(vg); i < count; i += vthreads(vg)) {
-        const int32_t gg = in.g[i];
-        const SelState s = st[gg];
-        if (s.mode != SEL_REFINE) continue;
-        const int32_t d = in.d[i];
-        if (bin1_of(d, b1_bins) != s.b1) continue;
-        const uint64_t K = sel_key(in.x[i], d, s.desc, id_bits, deg_bits);
-        if ((K >> s.r) != (s.prefix >> s.r)) continue;
-        const int w = s.r < kDigitBits ? s.r : kDigitBits;
-        const int digit = (int)((K >> (s.r - w)) & ((1ull << w) - 1));
-        if (lds) atomicAdd(&s_h[gg * kDigitBins + digit], 1);
-        else atomicAdd(&h2[(int64_t)gg * kDigitBins + digit], 1);
+    __syncthreads();
+    constexpr int kH2Items = 4;  // records per thread per step, loads issued together
+    const int64_t step = vthreads(vg);
+    for (int64_t i0 = vthread(vg); i0 < count; i0 += kH2Items * step) {
+        int32_t gg[kH2Items], dd[kH2Items], xx[kH2Items];
+#pragma unroll
+        for (int j = 0; j < kH2Items; ++j) {
+            const int64_t i = i0 + j * step;
+            const int64_t ic = i < count ? i : count - 1;  // in bounds: loads issue unconditionally
+            gg[j] = in.g[ic];
+            dd[j] = in.d[ic];
+            xx[j] = in.x[ic];
+        }
+#pragma unroll
+        for (int j = 0; j < kH2Items; ++j) {
+            if (i0 + j * step >= count) continue;
+            const SelState s = gg[j] < kSelLdsGroups ? s_st[gg[j]] : st[gg[j]];
+            if (s.mode != SEL_REFINE) continue;
+            const int32_t d = dd[j];
+            if (bin1_of(d, b1_bins) != s.b1) continue;
+            const uint64_t K = sel_key(xx[j], d, s.desc, id_bits, deg_bits);
+            if ((K >> s.r) != (s.prefix >> s.r)) continue;
+            const int w = s.r < kDigitBits ? s.r : kDigitBits;
+            const int digit = (int)((K >> (s.r - w)) & ((1ull << w) - 1));
+            if (lds) atomicAdd(&s_h[gg[j] * kDigitBins + digit], 1);
+            else atomicAdd(&h2[(int64_t)gg[j] * kDigitBins + digit], 1);
+        }
     }
     if (lds) {
         __syncthreads();

@@ -99,7 +99,7 @@ int dmolp_ctx_round_times(dmolp_ctx *ctx, int group, double *ms_out, int cap, in
 /* Launch parameters the autotuner chooses; 0 means "keep". */
 typedef struct {
     int32_t heavy_degree;  /* rows longer than this are split into chunks */
-    int32_t chunk_edges;   /* edges per chunk of a heavy row */
+    int32_t chunk_edges;   /* edges per chunk of a heavy row (64 .. 2^21) */
     int32_t block_threads; /* threads per workgroup for the row kernels */
     int32_t grid_blocks;   /* workgroups of the persistent row kernels */
     int32_t pull_per_mille; /* find candidates from the boundary side when B_{t-1}

@@ -137,3 +137,17 @@ def test_autotune_keeps_results(ctx):
             g.free()
         ctx.set_tuning(heavy_degree=old.heavy_degree, chunk_edges=old.chunk_edges,
                        batch_work_per_block=old.batch_work_per_block)
+
+
+def test_tuning_rejects_out_of_range_values(ctx):
+    # chunk_edges is bounded so a lane's 16-bit chunk counters cannot wrap;
+    # heavy_degree so light rows' 16-bit kept counts cannot.
+    old = ctx.tuning()
+    for bad in (dict(chunk_edges=(1 << 21) + 1), dict(heavy_degree=65536), dict(light_hist=3)):
+        with pytest.raises(api.DmolpError):
+            ctx.set_tuning(**bad)
+    now = ctx.tuning()
+    assert (now.chunk_edges, now.heavy_degree) == (old.chunk_edges, old.heavy_degree)
+    ctx.set_tuning(chunk_edges=1 << 21)
+    assert ctx.tuning().chunk_edges == 1 << 21
+    ctx.set_tuning(chunk_edges=old.chunk_edges)
