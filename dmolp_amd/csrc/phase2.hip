@@ -109,6 +109,7 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.D = gload(v.D);
     r.L = gload(v.L);
     r.L8 = gload(v.L8);
+    r.L5 = gload(v.L5);
     r.chg = gload(v.chg);
     r.diff = gload(v.diff);
     r.stamp = gload(v.stamp);
@@ -147,6 +148,45 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
 
 // ---------------------------------------------------------------- init sweep
 
+// Label of vertex u for the initial sweep's gathers: the byte mirror (B = 8)
+// or the packed mirror, B bits a label, 32 / B labels a word.  A smaller
+// array keeps more of the gathered labels in L2: the sweep's gathers are
+// bound by the lines their L2 misses move.
+template <int B>
+__device__ __forceinline__ int32_t sweep_label(const uint8_t *L8, const uint32_t *L5, uint32_t u) {
+    if constexpr (B == 8) {
+        return L8[u];
+    } else {
+        constexpr uint32_t per = 32 / B;
+        return (int32_t)((L5[u / per] >> ((u % per) * B)) & ((1u << B) - 1u));
+    }
+}
+
+// Packed label mirror (lbits bits a label) and the count of vertices whose
+// label lies outside [0, k).  The sweep reads the packed mirror only when
+// that count is zero: then no entry has an invalid label and the packed
+// value is the label itself.
+__global__ void k_pack_labels(const P2Job *JS, const int32_t *bstart, int jobs) {
+    P2_JOB();
+    if (J.lbits == 0) return;  // uniform per job
+    const int64_t n = J.g.n;
+    const int B = J.lbits, per = 32 / B;
+    const int64_t nw = (n + per - 1) / per;
+    int64_t bad = 0;
+    for (int64_t i = vthread(vg); i < nw; i += vthreads(vg)) {
+        uint32_t word = 0;
+        for (int j = 0; j < per; ++j) {
+            const int64_t v = i * per + j;
+            if (v >= n) break;
+            const int32_t l = J.L[v];
+            bad += (l < 0 || l >= J.k) ? 1 : 0;
+            word |= ((uint32_t)l & ((1u << B) - 1u)) << (j * B);
+        }
+        J.L5[i] = word;
+    }
+    block_count_add(&J.ctl->v[C_BADLAB], bad);
+}
+
 // phase2.cpp:55-117 and 214-253 over all vertices, plus the per-vertex count
 // of differing neighbours used as the boundary test, and (lh_on) each light
 // row's label counts for the kept histograms.
@@ -169,6 +209,8 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
     // With kept histograms and every label counted (k <= 32), the row's
     // differing, intra and invalid counts follow from its histogram.
     const bool from_hist = lh_on && k <= kLpCap;
+    // Packed gathers when every label is valid (see k_pack_labels).
+    const int packed = (J.lbits != 0 && ctl->v[C_BADLAB] == 0) ? J.lbits : 0;
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ int32_t s_cnt[kWavesPerBlock][kWave];
     __shared__ uint32_t s_h[kWavesPerBlock][kWave * kPackedStride];
@@ -187,9 +229,10 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
         if (lh_on)
             for (int l = 0; l < 4 * nvec; ++l) h[lane * kPackedStride + l] = 0u;
         wave_sync();
-        if (from_hist) {
-            // Counting only: kLightUnroll entries per lane per step, their
-            // slot searches, column loads and label gathers issued together.
+        // Counting only: kLightUnroll entries per lane per step, their slot
+        // searches, column loads and label gathers issued together.
+        const auto count_rows = [&](auto bits) {
+            constexpr int B = decltype(bits)::value;
             for (int32_t q0 = 0; q0 < total; q0 += kWave * kLightUnroll) {
                 int slot[kLightUnroll];
                 int32_t u[kLightUnroll];
@@ -210,7 +253,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
                 int32_t b[kLightUnroll];
 #pragma unroll
                 for (int j = 0; j < kLightUnroll; ++j) {
-                    const int32_t lb = L8[u[j] >= 0 ? u[j] : 0];
+                    const int32_t lb = sweep_label<B>(L8, J.L5, (uint32_t)(u[j] >= 0 ? u[j] : 0));
                     b[j] = u[j] >= 0 ? lb : -1;
                 }
 #pragma unroll
@@ -220,6 +263,11 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
                     else atomicAdd(&s_cnt[w][slot[j]], b[j] == kLabel8None ? 1 : 0x10000);
                 }
             }
+        };
+        if (from_hist) {
+            if (packed == 5) count_rows(std::integral_constant<int, 5>{});
+            else if (packed == 4) count_rows(std::integral_constant<int, 4>{});
+            else count_rows(std::integral_constant<int, 8>{});
         }
         for (int32_t q0 = 0; q0 < (from_hist ? 0 : total); q0 += kWave) {
             const int32_t q = q0 + lane;
@@ -307,6 +355,8 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_heavy(const P2Job *JS, con
     const int w = threadIdx.x / kWave, lane = lane_id();
     uint32_t *row = &s_h[w][lane * kPackedStride];
     const int nw = (E + 1) / 2;  // words of a row in use
+    // Packed gathers when every label is valid (see k_pack_labels).
+    const int packed = (J.lbits != 0 && ctl->v[C_BADLAB] == 0) ? J.lbits : 0;
     // Column lane of the chunk's label-lane count (lane < E).
     const auto col_count = [&](int l) {
         int32_t c = 0;
@@ -330,6 +380,8 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_heavy(const P2Job *JS, con
             // Labels in [k, 255) count as differing, as the loop below
             // would; their vertices raise the kernel error anyway.
             int32_t inv = 0;
+            const auto count_chunk = [&](auto bits) {
+            constexpr int B = decltype(bits)::value;
             for (int64_t eb = e0; eb < e1; eb += kWave * kHeavyUnroll) {
                 // In-bounds indices: the loads issue unconditionally.
                 int32_t u[kHeavyUnroll];
@@ -342,7 +394,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_heavy(const P2Job *JS, con
                 int32_t lab[kHeavyUnroll];
 #pragma unroll
                 for (int q = 0; q < kHeavyUnroll; ++q) {
-                    const int32_t lb = L8[u[q] >= 0 ? u[q] : 0];
+                    const int32_t lb = sweep_label<B>(L8, J.L5, (uint32_t)(u[q] >= 0 ? u[q] : 0));
                     lab[q] = u[q] >= 0 ? lb : -1;
                 }
 #pragma unroll
@@ -351,6 +403,10 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_heavy(const P2Job *JS, con
                     inv += lab[q] == kLabel8None ? 1 : 0;
                 }
             }
+            };
+            if (packed == 5) count_chunk(std::integral_constant<int, 5>{});
+            else if (packed == 4) count_chunk(std::integral_constant<int, 4>{});
+            else count_chunk(std::integral_constant<int, 8>{});
             inv = (int32_t)wave_sum((int64_t)inv);
             wave_sync();
             int32_t c = 0;
@@ -2079,6 +2135,7 @@ Phase2Work &phase2_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.wit.alloc(sizeof(int32_t) * (size_t)n);
         w.bbits.alloc(sizeof(uint32_t) * 2 * (size_t)((n + 31) / 32));
         w.L8.alloc((size_t)n);
+        w.L5.alloc(sizeof(uint32_t) * (size_t)((n + 5) / 6));  // 6 labels a word at 5 bits, 8 at 4
         w.ON.alloc(sizeof(uint16_t) * (size_t)n);
         w.LN.alloc(sizeof(uint16_t) * (size_t)n);
         w.cbits.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
@@ -2139,6 +2196,8 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.D = w.D.dev;
     j.L = g->labels.as<int32_t>();
     j.L8 = w.L8.as<uint8_t>();
+    j.L5 = w.L5.as<uint32_t>();
+    j.lbits = w.k <= 16 ? 4 : (w.k <= kLpCap ? 5 : 0);
     j.chg = w.chg.as<int32_t>();
     j.diff = w.diff.as<int32_t>();
     j.stamp = w.stamp.as<int32_t>();
@@ -2246,6 +2305,8 @@ void phase2_init_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     hipLaunchKernelGGL(k_init_zero, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), J);
     hipLaunchKernelGGL(k_mirror8, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
+                       ls.bs(P_GRID), J);
+    hipLaunchKernelGGL(k_pack_labels, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), J);
     size_t i0 = 0;
     if (ctx->profiling) {

@@ -19,6 +19,7 @@ enum CtlIndex {
     C_KERNEL_ERR,    // a kernel found an impossible state
     C_DEFERRED,      // heavy rows left to the pull kernel's second pass
     C_PENDING,       // rows the pull pass left to k_cand_pull_rest
+    C_BADLAB,        // vertices whose label lies outside [0, k) (packing pass)
     C_NCTL = 16
 };
 
@@ -60,6 +61,7 @@ struct P2Job {
     ListDev all, B, D;
     int32_t *L;
     uint8_t *L8;
+    uint32_t *L5;   // packed labels (lbits bits, 32 / lbits a word) for the initial sweep
     int32_t *chg, *diff, *stamp, *hub_hist;
     int64_t *deferred;  // long rows the pull pass left undecided: (flags << 32) | v
     int32_t *rdeg;    // dense selection records: degree of each record vertex, else -1
@@ -96,7 +98,7 @@ struct P2Job {
     int32_t *remaining;  // unfinished jobs of the set (shared by the set)
     int32_t trace_cap, k, heavy_degree, chunk_edges;
     int32_t epoch_base, max_rounds;
-    int32_t lh_on, pad0;
+    int32_t lh_on, lbits;  // lbits: bits of a packed label, 0 without the packed mirror
     double epsilon;
     double pull_frac;
     double recount_share, push_share;  // heavy-histogram modes (see k_hist_mode)
@@ -119,6 +121,7 @@ struct Phase2Work {
     int32_t epoch = 0;  // stamp value of the current candidate pass
     DevMem chg, diff, stamp, rdeg, bbits, deferred, wit;
     DevMem L8;  // byte mirror of the labels for the LP gathers
+    DevMem L5;  // packed labels for the initial sweep (k <= 32)
     DevMem hh, hhc, cbits, ON, LN;
     DevMem lh;  // uint32 [n][kLpCap / 2]: kept light-row label counts
     List B, D;          // boundary list (B_{t-1} read, then B_t written), Delta

@@ -75,12 +75,30 @@ def test_stats_parity(tuned, name):
     off, col = TINY[name]
     g = tuned.upload(off, col)
     rng = np.random.default_rng(1)
-    for k in (1, 3, 8, 40):
+    # k <= 16 and 17..32 take the packed label mirror (4 and 5 bits), 40 the bytes.
+    for k in (1, 3, 8, 16, 20, 32, 40):
         labels = rng.integers(0, k, len(off) - 1).astype(np.int32)
         rc, vc, ic, cut = O.stats(off, col, k, labels)
         assert rc == 0
         dvc, dic, dcut = g.stats(k, labels)
         assert np.array_equal(vc, dvc) and np.array_equal(ic, dic) and cut == dcut
+
+
+@pytest.mark.parametrize("name", ["grid5x7", "plaw500"])
+def test_stats_invalid_label_status(ctx, name):
+    # A -1 label that reaches the cut fails the sweep with the oracle's
+    # status; the packed mirror is skipped when any label is invalid.
+    off, col = TINY[name]
+    g = ctx.upload(off, col)
+    rng = np.random.default_rng(3)
+    for k in (4, 20, 40):
+        labels = rng.integers(0, k, len(off) - 1).astype(np.int32)
+        labels[int(col[0])] = -1
+        rc = O.stats(off, col, k, labels)[0]
+        assert rc == -3
+        with pytest.raises(api.DmolpError) as e:
+            g.stats(k, labels)
+        assert e.value.status == rc
 
 
 def test_penalties_bit_exact(ctx):
