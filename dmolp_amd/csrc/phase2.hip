@@ -1200,6 +1200,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_lp_light_take(const P2Job *JS, 
     __shared__ int32_t s_av[kWavesPerBlock][kWave * kDenseItems];  // light members: id,
     __shared__ int32_t s_ao[kWavesPerBlock][kWave * kDenseItems];  // own label,
     __shared__ int32_t s_ad[kWavesPerBlock][kWave * kDenseItems];  // degree
+    __shared__ int32_t s_wc[kWavesPerBlock];
     const int E = k < kLpCap ? k : kLpCap;
     const int nvec = (E + 7) / 8;
     if ((int)threadIdx.x < E) {
@@ -1226,14 +1227,20 @@ __global__ __launch_bounds__(kThreads, 6) void k_lp_light_take(const P2Job *JS, 
         int32_t d[kDenseItems], gg[kDenseItems];
         if (a.rlab8) dense_load<true>(a, g0 * kWave + lane, kWave, d, gg);
         else dense_load<false>(a, g0 * kWave + lane, kWave, d, gg);
+        bool tk[kDenseItems];
+#pragma unroll
+        for (int j = 0; j < kDenseItems; ++j) {
+            const int64_t v = (g0 + j) * kWave + lane;
+            tk[j] = d[j] >= 0 && dense_take(sel_get(s_st, st, gg[j]), (int32_t)v, d[j], b1_bins, id_bits, deg_bits);
+        }
+        const unsigned long long mark = a.rank_mark[base / per_block];  // this chunk's rank cuts
+        if (mark) rank_cut(s_st, mark, d, gg, tk, s_wc);
         uint32_t act = 0;
 #pragma unroll
         for (int j = 0; j < kDenseItems; ++j) {
             const int64_t gi = g0 + j;
             if (gi >= ng) break;  // wave-uniform
-            const int64_t v = gi * kWave + lane;
-            const bool take =
-                d[j] >= 0 && dense_take(sel_get(s_st, st, gg[j]), (int32_t)v, d[j], b1_bins, id_bits, deg_bits);
+            const bool take = tk[j];
             const bool light = take && d[j] <= heavy_degree;
             const uint64_t m = __ballot(take);  // the bitmap holds heavy members too
             if (lane == 0) half[2 * gi] = (uint32_t)m;

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist1(const SelArgs *A, con
     const int b1_bins = a.b1_bins;
     const int nb = a.groups * b1_bins;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) s_h[i] = 0;
+    if (vg.block == 0 && threadIdx.x == 0) a.refining[1] = 0;  // rank-cut groups of this selection
     __syncthreads();
     const bool dense = a.dense != 0;
     const int64_t count = dense ? a.rn : (int64_t)*in.cnt;
@@ -151,6 +152,12 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_pick1(const SelArgs *A, con
         s.b1 = b;
         if (before + bins[b] == m) {
             s.mode = SEL_THROUGH;
+        } else if (a.dense && b < b1_bins - 1 && g < kSelLdsGroups) {
+            // Dense records, cut inside one exact degree: the first `need`
+            // of that degree by id, found by counts per id chunk.
+            s.mode = SEL_RANK;
+            s.need = m - before;
+            if (threadIdx.x == 0) atomicAdd(&a.refining[1], 1);
         } else {
             s.mode = SEL_REFINE;
             s.need = m - before;
@@ -271,6 +278,115 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_collect_dense(const SelArgs
     else run(std::false_type{});
     __syncthreads();
     if (s_n > 0) ref_flush(a, s_x, s_n, s_base);
+}
+
+// Rank cuts, pass 1: per 1024-id chunk, the count of each rank group's
+// records in its cut bin (b1, an exact degree), and the chunk's cut marks
+// cleared.  One block iteration per chunk, in the dense passes' layout.
+__global__ __launch_bounds__(kSelThreads) void k_sel_rank_count(const SelArgs *A, const int32_t *bstart,
+                                                                int jobs) {
+    SEL_JOB();
+    if ((a.done && *a.done) || !a.dense) return;
+    if (a.refining[1] == 0) return;
+    __shared__ SelLite s_st[kSelLdsGroups];
+    __shared__ int32_t s_c[kSelLdsGroups];
+    sel_lite_load(s_st, a.st, a.groups);  // ends with a barrier
+    const int G = a.groups < kSelLdsGroups ? a.groups : kSelLdsGroups;
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    const int64_t nc = a.rank_chunks;
+    for (int64_t c = vg.block; c < nc; c += vg.blocks) {
+        if ((int)threadIdx.x < G) s_c[threadIdx.x] = 0;
+        __syncthreads();
+        int32_t d[kDenseItems], gg[kDenseItems];
+        const int64_t i0 = (c << kRankChunkLog) + (int64_t)w * kDenseItems * kWave + lane;
+        if (a.rlab8) dense_load<true>(a, i0, kWave, d, gg);
+        else dense_load<false>(a, i0, kWave, d, gg);
+#pragma unroll
+        for (int j = 0; j < kDenseItems; ++j) {
+            if (d[j] < 0 || gg[j] >= G) continue;
+            const SelLite s = s_st[gg[j]];
+            if (s.mode == SEL_RANK && d[j] == s.b1) atomicAdd(&s_c[gg[j]], 1);
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < G) a.rank_cnt[(int64_t)threadIdx.x * rank_stride(nc) + c] = s_c[threadIdx.x];
+        if (threadIdx.x == 0) a.rank_mark[c] = 0ull;
+        __syncthreads();
+    }
+}
+
+// Rank cuts, pass 2, one block per group: the chunk where the running count
+// of the group's cut-bin records reaches its need, and the need left inside
+// that chunk; the chunk gets the group's mark.
+__global__ __launch_bounds__(kSelThreads) void k_sel_rank_scan(const SelArgs *A, const int32_t *bstart,
+                                                               int jobs) {
+    SEL_JOB();
+    if ((a.done && *a.done) || !a.dense) return;
+    const int g = (int)vg.block;
+    if (g >= kSelLdsGroups || a.st[g].mode != SEL_RANK) return;  // uniform per block
+    __shared__ int64_t s_sum[kSelThreads];
+    const int64_t nc = a.rank_chunks;
+    const int32_t *cnt = a.rank_cnt + (int64_t)g * rank_stride(nc);  // 16-byte aligned rows
+    // Each thread's contiguous range of chunks, a multiple of 4 long, read
+    // as 16-byte vectors, four in flight.
+    const int64_t per = ((nc + blockDim.x - 1) / blockDim.x + 3) & ~3ll;
+    const int64_t c0 = (int64_t)threadIdx.x * per;
+    const int64_t c1 = c0 + per < nc ? c0 + per : nc;
+    int64_t sum = 0;
+    int64_t c = c0;
+    for (; c + 16 <= c1; c += 16) {
+        int4 x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] = *reinterpret_cast<const int4 *>(cnt + c + 4 * q);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sum += (int64_t)x[q].x + x[q].y + x[q].z + x[q].w;
+    }
+    for (; c < c1; ++c) sum += cnt[c];
+    s_sum[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive prefix of the threads' ranges, in chunk order
+        int64_t run = 0;
+        for (int t = 0; t < (int)blockDim.x; ++t) {
+            const int64_t x = s_sum[t];
+            s_sum[t] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    const int64_t need = a.st[g].need;
+    __shared__ int s_owner;
+    __shared__ int64_t s_run;
+    const int64_t mine = s_sum[threadIdx.x];
+    if (mine < need && need <= mine + sum) {  // exactly one thread: its range holds the cut
+        s_owner = (int)threadIdx.x;
+        s_run = mine;
+    }
+    __syncthreads();
+    // The owner's range, walked by the whole block kSelThreads chunks a step
+    // (an inclusive scan per step) instead of one chunk a load.
+    const int64_t o0 = (int64_t)s_owner * per;
+    const int64_t o1 = o0 + per < nc ? o0 + per : nc;
+    int64_t run = s_run;
+    for (int64_t b = o0; b < o1; b += blockDim.x) {  // uniform trip count
+        const int64_t c = b + threadIdx.x;
+        const int64_t x = c < o1 ? cnt[c] : 0;
+        s_sum[threadIdx.x] = x;
+        __syncthreads();
+        for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+            const int64_t y = (int)threadIdx.x >= o ? s_sum[threadIdx.x - o] : 0;
+            __syncthreads();
+            s_sum[threadIdx.x] += y;
+            __syncthreads();
+        }
+        const int64_t incl = run + s_sum[threadIdx.x];
+        if (c < o1 && incl - x < need && need <= incl) {  // the first chunk reaching the need
+            a.st[g].prefix = (uint64_t)c;
+            a.st[g].need = need - (incl - x);
+            atomicOr(&a.rank_mark[c], 1ull << g);
+        }
+        run += s_sum[blockDim.x - 1];
+        __syncthreads();
+        if (run >= need) break;  // uniform
+    }
 }
 
 // Digit histogram of the records still being refined.  With few enough
@@ -465,21 +581,31 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs
     __shared__ SelLite s_st[kSelLdsGroups];
     __shared__ unsigned long long s_sum[4];
     if (threadIdx.x < 4) s_sum[threadIdx.x] = 0;
+    __shared__ int32_t s_wc[kSelThreads / kWave];
     sel_lite_load(s_st, st, a.groups);
-    const int lane = lane_id();
+    const int w = threadIdx.x / kWave, lane = lane_id();
     const int64_t ng = (rn + kWave - 1) / kWave;
+    const int64_t per_block = (int64_t)(blockDim.x / kWave) * kDenseItems;  // one rank chunk
     int64_t n_light = 0, deg_sum = 0, n_heavy = 0, heavy_deg = 0;
-    for (int64_t g0 = vwave(vg) * kDenseItems; g0 < ng; g0 += vwaves(vg) * kDenseItems) {
+    // Block-uniform loop over 1024-id chunks (rank cuts rank across the block).
+    for (int64_t base = vg.block * per_block; base < ng; base += vg.blocks * per_block) {
+        const int64_t g0 = base + (int64_t)w * kDenseItems;
         int32_t d[kDenseItems], gg[kDenseItems];
         if (a.rlab8) dense_load<true>(a, g0 * kWave + lane, kWave, d, gg);
         else dense_load<false>(a, g0 * kWave + lane, kWave, d, gg);
+        bool tk[kDenseItems];
+#pragma unroll
+        for (int j = 0; j < kDenseItems; ++j) {
+            const int64_t v = (g0 + j) * kWave + lane;
+            tk[j] = d[j] >= 0 && dense_take(sel_get(s_st, st, gg[j]), (int32_t)v, d[j], b1_bins, id_bits, deg_bits);
+        }
+        const unsigned long long mark = a.rank_mark[base / per_block];
+        if (mark) rank_cut(s_st, mark, d, gg, tk, s_wc);
 #pragma unroll
         for (int j = 0; j < kDenseItems; ++j) {
             const int64_t gi = g0 + j;
             if (gi >= ng) break;  // wave-uniform
-            const int64_t v = gi * kWave + lane;
-            const bool take =
-                d[j] >= 0 && dense_take(sel_get(s_st, st, gg[j]), (int32_t)v, d[j], b1_bins, id_bits, deg_bits);
+            const bool take = tk[j];
             const bool light = take && d[j] <= heavy_degree;
             const uint64_t m = __ballot(take);  // the bitmap holds heavy members too
             if (lane == 0) half[2 * gi] = (uint32_t)m;
@@ -525,6 +651,14 @@ void SelectWork::prepare(dmolp_ctx *ctx, int ngroups, int64_t n, int32_t max_deg
         ref.alloc(n);
         ref_cap = n;
     }
+    const int64_t chunks = (n + (1 << kRankChunkLog) - 1) >> kRankChunkLog;
+    const int rg = ngroups < kSelLdsGroups ? ngroups : kSelLdsGroups;
+    if (rank_chunks != chunks || (int64_t)rank_cnt.bytes < (int64_t)sizeof(int32_t) * rg * rank_stride(chunks)) {
+        rank_cnt.alloc(sizeof(int32_t) * (size_t)rg * (size_t)rank_stride(chunks > 0 ? chunks : 1));
+        rank_mark.alloc(sizeof(unsigned long long) * (size_t)(chunks > 0 ? chunks : 1));
+        DMOLP_HIP_CHECK(hipMemsetAsync(rank_mark.p, 0, rank_mark.bytes, ctx->stream));
+        rank_chunks = chunks;
+    }
     if (groups == ngroups && id_bits == ib && deg_bits == db) return;
     groups = ngroups;
     id_bits = ib;
@@ -556,6 +690,9 @@ SelArgs select_args(dmolp_ctx *ctx, SelectWork &w, const RecDev &in, const TakeR
     a.h2 = w.h2.as<int32_t>();
     a.st = w.st.as<SelState>();
     a.refining = w.refining.as<int32_t>();
+    a.rank_cnt = w.rank_cnt.as<int32_t>();
+    a.rank_mark = w.rank_mark.as<unsigned long long>();
+    a.rank_chunks = w.rank_chunks;
     a.done = done;
     a.mark = mark;
     a.mark_val = mark_val;
@@ -588,6 +725,10 @@ void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, in
     hipLaunchKernelGGL(k_sel_hist1, dim3(ls.total[P_HIST1]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_HIST1), J);
     hipLaunchKernelGGL(k_sel_pick1, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
+                       ls.bs(P_GROUPS), J);
+    hipLaunchKernelGGL(k_sel_rank_count, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                       ls.bs(P_GRID), J);
+    hipLaunchKernelGGL(k_sel_rank_scan, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_GROUPS), J);
     hipLaunchKernelGGL(k_sel_collect, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_GRID), J);

@@ -53,7 +53,10 @@ enum SelMode : int32_t {
     SEL_ALL = 1,      // take the whole group
     SEL_THROUGH = 2,  // take every record up to and including level-1 bin b1
     SEL_REFINE = 3,   // threshold inside bin b1 still being refined
-    SEL_KEY = 4       // take records before bin b1, and in b1 those with K <= prefix|(2^r-1)
+    SEL_KEY = 4,      // take records before bin b1, and in b1 those with K <= prefix|(2^r-1)
+    SEL_RANK = 5      // dense, b1 an exact degree: records before bin b1, and of b1 the
+                      // first `need` by id (id chunks before chunk `prefix`, and in that
+                      // chunk those ranked below its remaining `need`)
 };
 
 struct SelState {
@@ -85,6 +88,11 @@ struct SelectWork {
     DevMem h1, h2, st, refining;
     Recs ref;  // records still being refined after the level-1 pick
     int64_t ref_cap = 0;
+    // Dense rank cuts: per group (the first kSelLdsGroups) and 1024-id chunk,
+    // the count of records in the group's cut bin; per chunk, the groups
+    // whose cut falls in it.
+    DevMem rank_cnt, rank_mark;
+    int64_t rank_chunks = 0;
     void prepare(dmolp_ctx *ctx, int groups, int64_t n, int32_t max_degree);
 };
 
@@ -116,6 +124,9 @@ struct SelArgs {
     int64_t rn;
     uint32_t *bits;
     int64_t bwords;
+    int32_t *rank_cnt;             // [rank groups][rank_stride(rank_chunks)]
+    unsigned long long *rank_mark; // [rank_chunks]: bit g when group g's cut chunk is this one
+    int64_t rank_chunks;
 };
 
 // Device helpers shared by the selection kernels and phase 2's fused
@@ -144,7 +155,9 @@ __device__ __forceinline__ SelLite sel_lite(const SelState &s) {
     l.b1 = s.b1;
     l.desc = s.desc;
     l.pad = 0;
-    l.kmax = s.prefix | ((s.r >= 64) ? ~0ull : ((1ull << s.r) - 1));
+    // SEL_RANK: the cut chunk in the high word, the need inside it below.
+    l.kmax = s.mode == SEL_RANK ? ((s.prefix << 32) | (uint64_t)(uint32_t)s.need)
+                                : s.prefix | ((s.r >= 64) ? ~0ull : ((1ull << s.r) - 1));
     return l;
 }
 __device__ __forceinline__ void sel_lite_load(SelLite *lds, const SelState *st, int groups) {
@@ -176,6 +189,8 @@ __device__ __forceinline__ SelArgs gload(const SelArgs &v) {
     r.rlab = gload(v.rlab);
     r.rlab8 = gload(v.rlab8);
     r.bits = gload(v.bits);
+    r.rank_cnt = gload(v.rank_cnt);
+    r.rank_mark = gload(v.rank_mark);
     return r;
 }
 
@@ -185,6 +200,12 @@ __device__ __forceinline__ int32_t dense_group(const SelArgs &a, int64_t x) {
 }
 
 constexpr int kDenseItems = 4;  // dense passes: 64-id groups per wave per iteration
+// Dense passes over ids run block-uniform iterations of 1024 ids (4 waves of
+// kDenseItems 64-id groups): the rank chunks of SEL_RANK.
+constexpr int kRankChunkLog = 10;
+// Row stride (in counts) of a group's chunk counts: a multiple of 4, so rows
+// start 16-byte aligned.
+__host__ __device__ __forceinline__ int64_t rank_stride(int64_t chunks) { return (chunks + 3) & ~3ll; }
 
 // Degree and group of dense records i0 + j * stride (j < N), d = -1 past rn
 // or for a non-record.  The indices are clamped into [0, rn), so all 2N
@@ -213,7 +234,46 @@ __device__ __forceinline__ bool dense_take(const SelLite &s, int32_t v, int32_t 
         case SEL_ALL: return true;
         case SEL_THROUGH: return earlier || b == s.b1;
         case SEL_KEY: return earlier || (b == s.b1 && sel_key(v, d, s.desc, id_bits, deg_bits) <= s.kmax);
+        // In the cut chunk itself the records are ranked by rank_cut.
+        case SEL_RANK: return earlier || (b == s.b1 && (uint32_t)(v >> kRankChunkLog) < (uint32_t)(s.kmax >> 32));
         default: return false;
+    }
+}
+
+// Take decisions of one block iteration over the 1024 ids of chunk c
+// (thread layout of the dense passes: wave w, item j, lane: id
+// (c << 10) + (w * kDenseItems + j) * 64 + lane) for the groups whose rank
+// cut falls in this chunk (bits of `mark`): a record of such a group in its
+// cut bin is taken when fewer than the chunk's need of the group's cut-bin
+// records precede it in id order.  Block-uniform call (4 waves).
+__device__ __forceinline__ void rank_cut(const SelLite *s_st, unsigned long long mark,
+                                         const int32_t (&d)[kDenseItems], const int32_t (&gg)[kDenseItems],
+                                         bool (&take)[kDenseItems], int32_t *s_wc) {
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    while (mark) {
+        const int g = __ffsll((long long)mark) - 1;
+        mark &= mark - 1;
+        const SelLite s = s_st[g];
+        if (s.mode != SEL_RANK) continue;  // uniform: a mark left from another round
+        const int32_t need = (int32_t)(uint32_t)s.kmax;
+        uint64_t m[kDenseItems];
+        int32_t tot = 0;
+#pragma unroll
+        for (int j = 0; j < kDenseItems; ++j) {
+            m[j] = __ballot(d[j] >= 0 && gg[j] == g && d[j] == s.b1);
+            tot += __popcll(m[j]);
+        }
+        if (lane == 0) s_wc[w] = tot;
+        __syncthreads();
+        int32_t run = 0;
+        for (int x = 0; x < w; ++x) run += s_wc[x];
+#pragma unroll
+        for (int j = 0; j < kDenseItems; ++j) {
+            if ((m[j] >> lane) & 1ull) take[j] = run + __popcll(m[j] & lt) < need;
+            run += __popcll(m[j]);
+        }
+        __syncthreads();
     }
 }
 
