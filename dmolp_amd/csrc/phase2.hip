@@ -589,23 +589,24 @@ __global__ void k_round_begin(const P2Job *JS, const int32_t *bstart, int jobs) 
 
 // ---------------------------------------------------------------- candidates
 
+
 // Round 1: B_0 is every vertex, so the candidates are every vertex.
-__global__ __launch_bounds__(kThreads) void k_cand_all(const P2Job *JS, const int32_t *bstart, int jobs) {
-    P2_JOB();
+__device__ __forceinline__ void cand_all(const P2Job &J, const VGrid &vg) {
     const GraphDev g = J.g;
     const int32_t *diff = J.diff;
     int32_t *rdeg = J.rdeg;
-    Ctl *ctl = J.ctl;
-    if (is_done(ctl) || ctl->cand_mode != CAND_ALL) return;
     int64_t nrec = 0;
     const int64_t span = (int64_t)blockDim.x * kScanItems;
     for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            const int64_t vi = v < g.n ? v : g.n - 1;  // in bounds: the loads issue together
+            const int32_t dv = diff[vi];
+            const int64_t s = g.off[vi], e = g.off[vi + 1];
             if (v >= g.n) continue;
-            const bool rec = diff[v] > 0;
-            rdeg[v] = rec ? (int32_t)(g.off[v + 1] - g.off[v]) : -1;
+            const bool rec = dv > 0;
+            rdeg[v] = rec ? (int32_t)(e - s) : -1;
             nrec += rec ? 1 : 0;
         }
     }
@@ -684,7 +685,12 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
     int64_t *deferred = J.deferred;
     const int heavy_degree = J.heavy_degree;
     Ctl *ctl = J.ctl;
-    if (is_done(ctl) || ctl->cand_mode != CAND_PULL) return;
+    if (is_done(ctl)) return;
+    if (ctl->cand_mode == CAND_ALL) {  // round 1, in the same launch
+        cand_all(J, vg);
+        return;
+    }
+    if (ctl->cand_mode != CAND_PULL) return;
     const uint32_t *inb = J.bbits + (int64_t)((ctl->t - 1) & 1) * J.bwords;  // B_{t-1}
     // Once the differing counts are kept (see on_boundary), interior vertices
     // are skipped outright and boundary rows are scanned for B_{t-1} only.
@@ -1713,8 +1719,14 @@ __global__ void k_mirror8(const P2Job *JS, const int32_t *bstart, int jobs) {
     }
 }
 
-__global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs) {
+__global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs, int zero_hh) {
     P2_JOB();
+    // Recount rounds: the heavy histograms are cleared here, in the same
+    // launch (nothing reads them between the Delta pass and the recount).
+    if (zero_hh && !is_done(J.ctl) && J.ctl->hist_mode == HIST_RECOUNT) {
+        const int64_t cnt = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
+        for (int64_t i = vthread(vg); i < cnt; i += vthreads(vg)) J.hh[i] = 0;
+    }
     const ListDev D = J.D;
     int32_t *L = J.L;
     uint8_t *L8 = J.L8;
@@ -1814,29 +1826,6 @@ __global__ void k_hist_mode(const P2Job *JS, const int32_t *bstart, int jobs) {
                      : (chg_deg <= J.push_share * nnz ? HIST_PUSH : HIST_SCAN);
 }
 
-// Push mode: add the copies of this round's moves into hh and clear them.
-__global__ void k_hist_fold(const P2Job *JS, const int32_t *bstart, int jobs) {
-    P2_JOB();
-    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_PUSH) return;
-    const int64_t nhh = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
-    for (int64_t i = vthread(vg); i < nhh; i += vthreads(vg)) {
-        int32_t sum = 0;
-#pragma unroll
-        for (int c = 0; c < kHhCopies; ++c) {
-            int32_t *p = &J.hhc[(int64_t)c * nhh + i];
-            sum += *p;
-            *p = 0;
-        }
-        if (sum) J.hh[i] += sum;
-    }
-}
-
-__global__ void k_hist_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
-    P2_JOB();
-    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_RECOUNT) return;
-    const int64_t cnt = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
-    for (int64_t i = vthread(vg); i < cnt; i += vthreads(vg)) J.hh[i] = 0;
-}
 
 // Recount mode (after rounds where many vertices changed): every heavy
 // row's label histogram from scratch, one chunk per wave, per-lane LDS rows.
@@ -1888,6 +1877,21 @@ __global__ __launch_bounds__(kThreads) void k_hist_recount(const P2Job *JS, cons
     }
 }
 
+// Push mode: add the copies of this round's moves into hh and clear them.
+__device__ __forceinline__ void hist_fold(const P2Job &J, const VGrid &vg) {
+    const int64_t nhh = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
+    for (int64_t i = vthread(vg); i < nhh; i += vthreads(vg)) {
+        int32_t sum = 0;
+#pragma unroll
+        for (int c = 0; c < kHhCopies; ++c) {
+            int32_t *p = &J.hhc[(int64_t)c * nhh + i];
+            sum += *p;
+            *p = 0;
+        }
+        if (sum) J.hh[i] += sum;
+    }
+}
+
 // Update mode: one chunk of a heavy row per wave.  Each neighbour's bit in
 // the changed-vertex bitmap (n bits: a small, well-cached array) is tested;
 // a changed neighbour moves one count from its old to its new label (one
@@ -1902,7 +1906,12 @@ __global__ __launch_bounds__(kThreads) void k_hist_update(const P2Job *JS, const
     const uint16_t *ON = J.ON;
     const uint32_t *cbits = J.cbits;
     int32_t *hh = J.hh;
-    if (is_done(J.ctl) || J.ctl->hist_mode != HIST_SCAN) return;
+    if (is_done(J.ctl)) return;
+    if (J.ctl->hist_mode == HIST_PUSH) {  // push rounds fold in the same launch
+        hist_fold(J, vg);
+        return;
+    }
+    if (J.ctl->hist_mode != HIST_SCAN) return;
     const int E = J.k < kLpCap ? J.k : kLpCap;
     const int chunk_edges = J.chunk_edges;
     __shared__ int32_t s_w[kWavesPerBlock][kLpCap];
@@ -2078,6 +2087,10 @@ __global__ void k_b_list(const P2Job *JS, const int32_t *bstart, int jobs, int s
     if (is_done(ctl) || (only_push && ctl->cand_mode != CAND_PUSH)) return;
     const ListDev B = J.B;
     const int64_t bwords = J.bwords, n = J.g.n;
+    // Push rounds start from clear record entries (the candidate kernels
+    // write the records they find); one launch does both.
+    if (only_push)
+        for (int64_t v = vthread(vg); v < n; v += vthreads(vg)) J.rdeg[v] = -1;
     const uint32_t *half = J.bbits + (int64_t)((ctl->t - shift) & 1) * bwords;
     const int lane = lane_id();
     for (int64_t base = vwave(vg) * kWave; base < n; base += vwaves(vg) * kWave) {
@@ -2353,12 +2366,10 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     hipLaunchKernelGGL(k_round_begin, dim3(ls.total[P_ONE]), dim3(64), 0, s, jobs, ls.bs(P_ONE), J);
     // Round 1 takes every vertex; later rounds pull or push (ctl->cand_mode).
     // Each writes one dense record entry per vertex (push: after a clear).
-    hipLaunchKernelGGL(k_cand_all, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
-    hipLaunchKernelGGL(k_cand_pull, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_cand_pull, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);  // and round 1's
     hipLaunchKernelGGL(k_cand_pull_rest, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     if (set.any_heavy)
         hipLaunchKernelGGL(k_cand_pull_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
-    hipLaunchKernelGGL(k_rdeg_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     hipLaunchKernelGGL(k_b_list, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 1, 1, 0);  // B_{t-1} list
     hipLaunchKernelGGL(k_cand_light, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     if (set.any_heavy)
@@ -2366,7 +2377,7 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     const size_t m1 = mark();
     // B_t: the bitmap and counts.  With kept histograms the selection's last
     // pass is fused with the light LP below.
-    select_enqueue(ctx, set.sel, ls, set.passes, !set.lh_on);
+    select_enqueue(ctx, set.sel, ls, set.passes, !set.lh_on, true);
     if (!set.lh_on)  // the gathering LP walks B_t as a list
         hipLaunchKernelGGL(k_b_list, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 0, 0, 0);
     const size_t m2 = mark();
@@ -2387,11 +2398,9 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     const size_t m4b = mark();
     if (set.any_heavy)
         hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
-    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, set.any_heavy ? 1 : 0);
     const size_t m5 = mark();
-    if (set.any_heavy) {  // fold the pushed moves, or recount, or scan
-        hipLaunchKernelGGL(k_hist_fold, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
-        hipLaunchKernelGGL(k_hist_zero, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    if (set.any_heavy) {  // recount, or fold the pushed moves or scan (one launch)
         hipLaunchKernelGGL(k_hist_recount, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_update, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     }
@@ -2500,7 +2509,7 @@ void phase2_select_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int t)
     if (set.any_heavy)
         hipLaunchKernelGGL(k_cand_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                            ls.bs(P_GRID), ls.jobs);
-    select_enqueue(ctx, set.sel, ls, set.passes);
+    select_enqueue(ctx, set.sel, ls, set.passes, true, true);
     hipLaunchKernelGGL(k_b_list, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), ls.jobs, 0, 0, 1);
     DMOLP_HIP_CHECK(hipGetLastError());
@@ -2521,7 +2530,7 @@ void phase2_lp_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
         hipLaunchKernelGGL(k_lp_heavy_decide, dim3(ls.total[P_HEAVY]), dim3(kThreads), 0, s,
                            set.jobs, ls.bs(P_HEAVY), ls.jobs);
     }
-    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs);
+    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs, 0);
     hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs);
     DMOLP_HIP_CHECK(hipGetLastError());
 }

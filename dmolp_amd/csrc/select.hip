@@ -719,20 +719,34 @@ void select_args_dense(SelArgs &a, const int32_t *rdeg, const int32_t *rlab, con
 }
 
 void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, int passes,
-                    bool compact) {
+                    bool compact, bool dense) {
     hipStream_t s = ctx->stream;
     const int J = ls.jobs;
     hipLaunchKernelGGL(k_sel_hist1, dim3(ls.total[P_HIST1]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_HIST1), J);
     hipLaunchKernelGGL(k_sel_pick1, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_GROUPS), J);
-    hipLaunchKernelGGL(k_sel_rank_count, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
-                       ls.bs(P_GRID), J);
-    hipLaunchKernelGGL(k_sel_rank_scan, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
-                       ls.bs(P_GROUPS), J);
+    if (dense) {
+        // Dense jobs only: rank cuts, then digit passes for cuts in the last bin.
+        hipLaunchKernelGGL(k_sel_rank_count, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                           ls.bs(P_GRID), J);
+        hipLaunchKernelGGL(k_sel_rank_scan, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
+                           ls.bs(P_GROUPS), J);
+        hipLaunchKernelGGL(k_sel_collect_dense, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                           ls.bs(P_GRID), J);
+        for (int p = 0; p < passes; ++p) {
+            hipLaunchKernelGGL(k_sel_hist2, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                               ls.bs(P_GRID), J);
+            hipLaunchKernelGGL(k_sel_pick2, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
+                               ls.bs(P_GROUPS), J);
+        }
+        if (compact)
+            hipLaunchKernelGGL(k_sel_compact_dense, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
+                               ls.bs(P_GRID), J);
+        DMOLP_HIP_CHECK(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_sel_collect, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
-                       ls.bs(P_GRID), J);
-    hipLaunchKernelGGL(k_sel_collect_dense, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_GRID), J);
     for (int p = 0; p < passes; ++p) {
         hipLaunchKernelGGL(k_sel_hist2, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
@@ -742,9 +756,6 @@ void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, in
     }
     hipLaunchKernelGGL(k_sel_compact, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
                        ls.bs(P_GRID), J);
-    if (compact)
-        hipLaunchKernelGGL(k_sel_compact_dense, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
-                           ls.bs(P_GRID), J);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 

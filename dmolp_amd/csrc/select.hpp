@@ -287,10 +287,12 @@ void select_args_dense(SelArgs &a, const int32_t *rdeg, const int32_t *rlab, con
 // Enqueue the whole selection for a set of jobs (args: device array, one
 // SelArgs per job): level-1 histogram, pick, digit passes, compaction.
 // `passes` is the largest digit-pass count over the jobs.
-// `compact` false leaves the dense jobs' final take pass (bitmap and counts)
-// to the caller, which must then run dense_take over every record itself.
+// `dense`: every job's records are dense (select_args_dense), so only the
+// dense kernels are launched.  `compact` false leaves the dense jobs' final take pass (bitmap and
+// counts) to the caller, which must then run dense_take and rank_cut over
+// every record itself.
 void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, int passes,
-                    bool compact = true);
+                    bool compact = true, bool dense = false);
 
 // Block counts of one job for the launch policies (jobs with n vertices
 // and nnz entries, `groups` selection groups, `heavy` heavy rows).
