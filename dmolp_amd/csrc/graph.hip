@@ -93,11 +93,12 @@ __global__ void k_heavy_ids(ListDev all, int32_t *hid, uint32_t *hbits) {
     }
 }
 
-__global__ void k_first_nb(GraphDev g, int32_t *first) {
+__global__ void k_first_nb(GraphDev g, int32_t *first, int32_t *deg) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
          v += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = g.off[v];
-        first[v] = s < g.off[v + 1] ? g.col[s] : -1;
+        const int64_t s = g.off[v], e = g.off[v + 1];
+        first[v] = s < e ? g.col[s] : -1;
+        deg[v] = (int32_t)(e - s);
     }
 }
 
@@ -167,9 +168,10 @@ void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g) {
 void graph_prepare_first(dmolp_ctx *ctx, dmolp_graph *g) {
     if (g->first_valid) return;
     g->first.grow(sizeof(int32_t) * (size_t)(g->n > 0 ? g->n : 1));
+    g->deg.grow(sizeof(int32_t) * (size_t)(g->n > 0 ? g->n : 1));
     if (g->n > 0) {
         hipLaunchKernelGGL(k_first_nb, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, g->dev(),
-                           g->first.as<int32_t>());
+                           g->first.as<int32_t>(), g->deg.as<int32_t>());
         DMOLP_HIP_CHECK(hipGetLastError());
     }
     g->first_valid = true;

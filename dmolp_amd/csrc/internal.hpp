@@ -206,7 +206,8 @@ struct dmolp_graph {
     dmolp::DevMem hid;    // int32[n]: heavy slot of the all list, -1 for light rows
     dmolp::DevMem hbits;  // uint32[(n + 31) / 32]: bit v set for the heavy rows
     dmolp::DevMem first;  // int32[n]: v's first (smallest) neighbour, -1 for an empty row
-    bool first_valid = false;
+    dmolp::DevMem deg;    // int32[n]: v's degree (rows shorter than 2^31 entries)
+    bool first_valid = false;  // first and deg
     std::unique_ptr<dmolp::Phase2Work> p2;
     std::unique_ptr<dmolp::Phase1Work> p1;
     dmolp::GraphDev dev() const {
@@ -218,7 +219,7 @@ struct dmolp_graph {
 namespace dmolp {
 // Implemented in the .hip files.
 void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g);  // builds g->all for ctx->tune
-void graph_prepare_first(dmolp_ctx *ctx, dmolp_graph *g);  // g->first
+void graph_prepare_first(dmolp_ctx *ctx, dmolp_graph *g);  // g->first, g->deg
 void graph_validate(dmolp_ctx *ctx, dmolp_graph *g);       // offsets, sorted rows, no self-loops
 int64_t bits_for(int64_t v);                                // bits needed to hold 0..v
 // After enqueueing round t of a loop: queue a copy of the device flag behind

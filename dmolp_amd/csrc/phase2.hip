@@ -126,6 +126,7 @@ __device__ __forceinline__ P2Job gload(const P2Job &v) {
     r.lh = gload(v.lh);
     r.hbits = gload(v.hbits);
     r.first = gload(v.first);
+    r.deg = gload(v.deg);
     r.wit = gload(v.wit);
     r.bd = gload(v.bd);
     r.ctl = gload(v.ctl);
@@ -603,10 +604,10 @@ __device__ __forceinline__ void cand_all(const P2Job &J, const VGrid &vg) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             const int64_t vi = v < g.n ? v : g.n - 1;  // in bounds: the loads issue together
             const int32_t dv = diff[vi];
-            const int64_t s = g.off[vi], e = g.off[vi + 1];
+            const int32_t d = J.deg[vi];
             if (v >= g.n) continue;
             const bool rec = dv > 0;
-            rdeg[v] = rec ? (int32_t)(e - s) : -1;
+            rdeg[v] = rec ? d : -1;
             nrec += rec ? 1 : 0;
         }
     }
@@ -710,7 +711,6 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
     for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
         uint32_t rec = 0, pen = 0;
         int32_t deg[kScanItems], fnb[kScanItems];
-        int64_t xs[kScanItems], end[kScanItems];
         bool heavy_v[kScanItems], cand[kScanItems], known[kScanItems], hit[kScanItems];
         // Two stages of loads, each stage's loads for all items issued
         // together (a branch on one item's load before the next item's loads
@@ -726,8 +726,7 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
             const int64_t vi = in ? v : g.n - 1;
             const int32_t dv = diff[vi];
             const uint32_t hw = J.hbits[vi >> 5], bw = inb[vi >> 5];
-            xs[j] = g.off[vi];
-            end[j] = g.off[vi + 1];
+            deg[j] = J.deg[vi];
             fnb[j] = J.wit[vi];
             heavy_v[j] = in && use_diff && ((hw >> (vi & 31)) & 1u);
             hit[j] = in && ((bw >> (vi & 31)) & 1u);
@@ -737,9 +736,8 @@ __global__ __launch_bounds__(kThreads) void k_cand_pull(const P2Job *JS, const i
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
-            deg[j] = (int32_t)(end[j] - xs[j]);
             if (cand[j] && heavy_v[j] && J.k <= kLpCap) {  // every label counted in the kept histogram
-                if ((int64_t)J.hh[(int64_t)J.hid[v] * kLpCap + L[v]] == end[j] - xs[j]) cand[j] = false;
+                if (J.hh[(int64_t)J.hid[v] * kLpCap + L[v]] == deg[j]) cand[j] = false;
                 known[j] = true;
             }
         }
@@ -2234,6 +2232,7 @@ P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, 
     j.lh = j.lh_on ? w.lh.as<uint32_t>() : nullptr;
     j.hbits = g->hbits.as<uint32_t>();
     j.first = g->first.as<int32_t>();
+    j.deg = g->deg.as<int32_t>();
     j.wit = w.wit.as<int32_t>();
     j.deferred = w.deferred.as<int64_t>();
     j.hub_hist = w.hub_hist.as<int32_t>();

@@ -88,6 +88,7 @@ struct P2Job {
     uint32_t *lh;
     const uint32_t *hbits;  // heavy-row bitmap of the graph
     const int32_t *first;   // each row's first (smallest) neighbour, -1 if empty
+    const int32_t *deg;     // each row's degree
     int32_t *wit;           // per run: a neighbour that last put the row in the candidates
     RecDev bd;
     Ctl *ctl;

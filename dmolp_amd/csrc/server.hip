@@ -66,11 +66,11 @@ void spec_size(const dmolp_gen_spec &s, int64_t &n, int64_t &entries) {
 }
 
 // Device bytes a job holds while admitted: CSR, labels and the phase-1 and
-// phase-2 work arrays (about 272 bytes per vertex, 64 of them the kept
+// phase-2 work arrays (about 276 bytes per vertex, 64 of them the kept
 // light-row histograms), plus the generator's key buffers for generated
 // graphs.
 int64_t job_footprint(const dmolp_request &r) {
-    const int64_t per_vertex = 272;
+    const int64_t per_vertex = 276;
     if (r.graph) return per_vertex * r.graph->n;
     int64_t n = 0, e = 0;
     spec_size(r.spec, n, e);
