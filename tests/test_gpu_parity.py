@@ -294,3 +294,40 @@ def test_trace_light_parts_and_profile_groups(ctx):
     finally:
         ctx.set_profiling(False)
         ctx.set_tuning(**DEFAULT_TUNING)
+
+
+# Graphs whose rows reach the selection's last level-1 bin (degree 255 and
+# up), so label cuts fall there too and take the digit passes, next to cuts
+# inside one exact degree (ranked by id chunks); k = 70 adds labels beyond
+# the first 64, which always take the digit passes.
+HIGH_DEGREE = {
+    "dense700": graphs.random_graph(700, 700 * 150, 5),
+    "plaw3000": graphs.powerlaw(3000, 60000, 7),
+}
+
+
+@pytest.mark.parametrize("name", list(HIGH_DEGREE))
+def test_select_boundary_parity_high_degree(ctx, name):
+    off, col = HIGH_DEGREE[name]
+    n = len(off) - 1
+    assert int(np.diff(off).max()) >= 255
+    g = ctx.upload(off, col)
+    rng = np.random.default_rng(17)
+    for k in (2, 4, 9, 70):
+        for t in (1, 2, 5, 13):
+            labels = rng.integers(0, k, n).astype(np.int32)
+            prev = np.sort(rng.choice(n, size=max(1, n // 2), replace=False)).astype(np.int32)
+            re = rng.uniform(0.5, 1.5, k)
+            want = O.select_boundary(off, col, labels, prev, re, k, t)
+            got = g.select_boundary(labels, prev, re, k, t)
+            assert np.array_equal(want, got), (k, t)
+
+
+@pytest.mark.parametrize("name", list(HIGH_DEGREE))
+def test_phase2_parity_high_degree(ctx, name):
+    off, col = HIGH_DEGREE[name]
+    g = ctx.upload(off, col)
+    rng = np.random.default_rng(19)
+    for k in (4, 9, 70):
+        labels0 = rng.integers(0, k, len(off) - 1).astype(np.int32)
+        _check_phase2(g, off, col, k, labels0, 15, 0.0)
