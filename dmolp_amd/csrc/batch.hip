@@ -148,7 +148,6 @@ void p2_collect(dmolp_ctx *ctx, std::vector<P2Item *> &items) {
 
 void phase2_run_set(dmolp_ctx *ctx, std::vector<P2Item> &items, BatchState &bs) {
     const int G = (int)items.size();
-    hipStream_t s = ctx->stream;
     std::vector<P2Item *> ip((size_t)G);
     int Rmax = 0;
     for (int j = 0; j < G; ++j) {

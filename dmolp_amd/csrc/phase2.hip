@@ -1192,7 +1192,6 @@ __global__ __launch_bounds__(kThreads, 6) void k_lp_light_take(const P2Job *JS, 
     const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
     const int heavy_degree = a.heavy_degree;
     const int64_t rn = a.rn, bwords = a.bwords;
-    const int32_t *rdeg = a.rdeg;
     uint32_t *half = a.bits + (int64_t)(ctl->t & 1) * bwords;  // B_t
     __shared__ double s_pen[kLpCap], s_fac[kLpCap];
     __shared__ double s_fmax;

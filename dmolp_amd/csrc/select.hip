@@ -576,7 +576,6 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_compact_dense(const SelArgs
     const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
     const int heavy_degree = a.heavy_degree;
     const int64_t rn = a.rn, bwords = a.bwords;
-    const int32_t *rdeg = a.rdeg;
     uint32_t *half = a.bits + (int64_t)(*a.rule.t_ptr & 1) * bwords;
     __shared__ SelLite s_st[kSelLdsGroups];
     __shared__ unsigned long long s_sum[4];
