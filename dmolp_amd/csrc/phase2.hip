@@ -1716,11 +1716,17 @@ __global__ void k_mirror8(const P2Job *JS, const int32_t *bstart, int jobs) {
     }
 }
 
-__global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs, int zero_hh) {
+// `round` (the LP round, graph with heavy rows when heavy != 0): the
+// changed-vertex state the round's later kernels read is written only when
+// they read it: the old|new pairs and changed bits only for a scanning
+// heavy-histogram update, the new-label reset only where the word labels
+// (k above the byte range) tell changed vertices apart.  The test hook
+// passes 0 and gets every write.
+__global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs, int round, int heavy) {
     P2_JOB();
     // Recount rounds: the heavy histograms are cleared here, in the same
     // launch (nothing reads them between the Delta pass and the recount).
-    if (zero_hh && !is_done(J.ctl) && J.ctl->hist_mode == HIST_RECOUNT) {
+    if (round && heavy && !is_done(J.ctl) && J.ctl->hist_mode == HIST_RECOUNT) {
         const int64_t cnt = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
         for (int64_t i = vthread(vg); i < cnt; i += vthreads(vg)) J.hh[i] = 0;
     }
@@ -1735,6 +1741,8 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs, int ze
     if (is_done(ctl)) return;
     const int64_t nl = (int64_t)D.cnt[LC_LIGHT], ns = (int64_t)D.cnt[LC_SLOTS];
     const int64_t total = nl + ns;
+    const bool scan = !round || (heavy && ctl->hist_mode == HIST_SCAN);
+    const bool reset = !round || J.k > kExact8;
     constexpr int kApplyItems = 4;  // changed vertices per thread per step, loads issued together
     const int64_t step = vthreads(vg);
     for (int64_t i0 = vthread(vg); i0 < total; i0 += kApplyItems * step) {
@@ -1759,11 +1767,13 @@ __global__ void k_apply(const P2Job *JS, const int32_t *bstart, int jobs, int ze
             const int32_t nl_v = nv[j];
             L[u] = nl_v;
             const uint32_t nb = (nl_v >= 0 && nl_v < kLabel8None) ? (uint32_t)nl_v : (uint32_t)kLabel8None;
-            ON[u] = (uint16_t)(ob[j] | (nb << 8));
             L8[u] = (uint8_t)nb;
             LN[u] = (uint16_t)(nb | (kLabel8None << 8));
-            chg[u] = -1;
-            atomicOr(&cbits[u >> 5], 1u << (u & 31));
+            if (reset) chg[u] = -1;
+            if (scan) {
+                ON[u] = (uint16_t)(ob[j] | (nb << 8));
+                atomicOr(&cbits[u >> 5], 1u << (u & 31));
+            }
         }
     }
 }
@@ -1967,10 +1977,12 @@ __global__ __launch_bounds__(kThreads) void k_hist_update(const P2Job *JS, const
 }
 
 // Clear the changed-vertex bits of this round (every writer stores zero).
-__global__ void k_cbits_clear(const P2Job *JS, const int32_t *bstart, int jobs) {
+// `round`: the LP round, which sets the bits only in scanning rounds.
+__global__ void k_cbits_clear(const P2Job *JS, const int32_t *bstart, int jobs, int round, int heavy) {
     P2_JOB();
     const ListDev D = J.D;
     if (is_done(J.ctl)) return;
+    if (round && !(heavy && J.ctl->hist_mode == HIST_SCAN)) return;  // uniform per job
     const int64_t nl = (int64_t)D.cnt[LC_LIGHT], ns = (int64_t)D.cnt[LC_SLOTS];
     for (int64_t i = vthread(vg); i < nl + ns; i += vthreads(vg)) {
         const int32_t v = i < nl ? D.light[i] : D.hv[i - nl];
@@ -2045,6 +2057,10 @@ __global__ void k_init_zero(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const int64_t n = J.g.n;
     for (int64_t v = vthread(vg); v < n; v += vthreads(vg)) J.diff[v] = 0;
+    // Word labels (k above the byte range) tell changed neighbours apart by
+    // chg >= 0; byte-label rounds leave chg unreset (k_apply), so start clean.
+    if (J.k > kExact8)
+        for (int64_t v = vthread(vg); v < n; v += vthreads(vg)) J.chg[v] = -1;
     for (int64_t i = vthread(vg); i < 2 * J.bwords; i += vthreads(vg)) J.bbits[i] = 0u;
     for (int64_t i = vthread(vg); i < J.bwords; i += vthreads(vg)) J.cbits[i] = 0u;
     const int64_t nhh = (int64_t)J.all.cnt[LC_SLOTS] * kLpCap;
@@ -2396,13 +2412,13 @@ void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set) {
     const size_t m4b = mark();
     if (set.any_heavy)
         hipLaunchKernelGGL(k_delta_heavy, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
-    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, set.any_heavy ? 1 : 0);
+    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 1, set.any_heavy ? 1 : 0);
     const size_t m5 = mark();
     if (set.any_heavy) {  // recount, or fold the pushed moves or scan (one launch)
         hipLaunchKernelGGL(k_hist_recount, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
         hipLaunchKernelGGL(k_hist_update, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
     }
-    hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J);
+    hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, jobs, bg, J, 1, set.any_heavy ? 1 : 0);
     const size_t m6 = mark();
     hipLaunchKernelGGL(k_round_end, dim3(ls.total[P_ONE]), dim3(1), 0, s, jobs, ls.bs(P_ONE), J);
     DMOLP_HIP_CHECK(hipGetLastError());
@@ -2528,8 +2544,8 @@ void phase2_lp_enqueue(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
         hipLaunchKernelGGL(k_lp_heavy_decide, dim3(ls.total[P_HEAVY]), dim3(kThreads), 0, s,
                            set.jobs, ls.bs(P_HEAVY), ls.jobs);
     }
-    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs, 0);
-    hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs);
+    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs, 0, 0);
+    hipLaunchKernelGGL(k_cbits_clear, dim3(grid), dim3(kThreads), 0, s, set.jobs, bg, ls.jobs, 0, 0);
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 

@@ -331,3 +331,14 @@ def test_phase2_parity_high_degree(ctx, name):
     for k in (4, 9, 70):
         labels0 = rng.integers(0, k, len(off) - 1).astype(np.int32)
         _check_phase2(g, off, col, k, labels0, 15, 0.0)
+
+
+def test_phase2_byte_then_word_labels_same_graph(tuned):
+    # Runs on one graph keep their work arrays: a run with byte labels
+    # (k <= 254) followed by runs with word labels (k > 254) and back.
+    off, col = api.gen_host(api.rmat_spec(12, 8, seed=23))
+    g = tuned.upload(off, col)
+    rng = np.random.default_rng(29)
+    for k in (16, 300, 32, 1024):
+        labels0 = rng.integers(0, k, len(off) - 1).astype(np.int32)
+        _check_phase2(g, off, col, k, labels0, 8, 0.01)
