@@ -733,9 +733,11 @@ void select_enqueue(dmolp_ctx *ctx, const SelArgs *args, const LaunchSet &ls, in
                            ls.bs(P_GROUPS), J);
         hipLaunchKernelGGL(k_sel_collect_dense, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
                            ls.bs(P_GRID), J);
+        // The refine list holds only cuts in the last bin here (few records):
+        // the level-1 histogram's smaller grid suffices and launches faster.
         for (int p = 0; p < passes; ++p) {
-            hipLaunchKernelGGL(k_sel_hist2, dim3(ls.total[P_GRID]), dim3(kSelThreads), 0, s, args,
-                               ls.bs(P_GRID), J);
+            hipLaunchKernelGGL(k_sel_hist2, dim3(ls.total[P_HIST1]), dim3(kSelThreads), 0, s, args,
+                               ls.bs(P_HIST1), J);
             hipLaunchKernelGGL(k_sel_pick2, dim3(ls.total[P_GROUPS]), dim3(kSelThreads), 0, s, args,
                                ls.bs(P_GROUPS), J);
         }
