@@ -355,12 +355,17 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_rank_scan(const SelArgs *A,
     const int64_t need = a.st[g].need;
     __shared__ int s_owner;
     __shared__ int64_t s_run;
+    if (threadIdx.x == 0) s_owner = -1;
+    __syncthreads();
     const int64_t mine = s_sum[threadIdx.x];
     if (mine < need && need <= mine + sum) {  // exactly one thread: its range holds the cut
         s_owner = (int)threadIdx.x;
         s_run = mine;
     }
     __syncthreads();
+    // The counts sum to the cut bin's size, which holds the need; a group
+    // without an owner (no such chunk) would be a counting fault: stop.
+    if (s_owner < 0) return;  // uniform
     // The owner's range, walked by the whole block kSelThreads chunks a step
     // (an inclusive scan per step) instead of one chunk a load.
     const int64_t o0 = (int64_t)s_owner * per;
