@@ -49,6 +49,29 @@ __device__ __forceinline__ int64_t wave_sum(int64_t x) {
     return x;
 }
 
+// Segmented inclusive sum over the 64 lanes (all lanes must participate).
+// Each lane holds key << 24 | count; lanes whose keys are equal must be
+// consecutive, and every run's count sum must stay below 2^24.  A lane ends
+// with its key and the sum of the counts of its run's lanes up to itself.
+// Hillis-Steele steps of 1, 2, 4 and 8 within each row of 16 lanes (DPP
+// row_shr), then the end of each row is carried into the next rows (DPP
+// row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and 3).  A lane
+// with no source keeps kNone, a key no lane holds.
+__device__ __forceinline__ uint32_t wave_seg_incl(uint32_t x) {
+    constexpr int kNone = (int)0xfe000000u;
+    auto step = [](uint32_t x, int p) {
+        const uint32_t y = (uint32_t)p;
+        return (y >> 24) == (x >> 24) ? x + (y & 0xffffffu) : x;
+    };
+    x = step(x, __builtin_amdgcn_update_dpp(kNone, (int)x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = step(x, __builtin_amdgcn_update_dpp(kNone, (int)x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = step(x, __builtin_amdgcn_update_dpp(kNone, (int)x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = step(x, __builtin_amdgcn_update_dpp(kNone, (int)x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = step(x, __builtin_amdgcn_update_dpp(kNone, (int)x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = step(x, __builtin_amdgcn_update_dpp(kNone, (int)x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
+
 // Append the lanes with `pred` to list[*count ...]: one atomic per wave.
 // Returns this lane's slot (or -1).  All lanes must call it.
 __device__ __forceinline__ int64_t wave_append(bool pred, unsigned long long *count) {

@@ -1580,8 +1580,14 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
                     ln[j] = exact8 ? (uint32_t)LN[uc] : 0u;
                     hs[j] = Push ? (uint32_t)J.hid[uc] : 0u;  // unconditional within the version
                 }
+                // Each entry's counts for its row (old label hits, new
+                // label hits, same-label neighbours), summed over the lanes
+                // of the row after the loop: one LDS add per row instead of
+                // one per entry (the lanes of a row hit one LDS word).
+                uint32_t sx[kLpUnroll];
     #pragma unroll
                 for (int j = 0; j < kLpUnroll; ++j) {
+                    sx[j] = 0xff000000u;
                     if (slot[j] < 0) continue;
                     const int32_t vv = s_b[w].v[slot[j]];
                     const int32_t a = s_b[w].aux[slot[j]], c = s_new[w][slot[j]];
@@ -1598,8 +1604,8 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
                     }
                     const int32_t io = pair_effect_of(vv, a, c, u[j], lu, lu2, uchg, keep_diff && !uh, diff, dd, same);
                     cut_delta += dd;
-                    if (io) atomicAdd(&s_io[w][slot[j]], io);
-                    if (same) atomicAdd(&s_same[w][slot[j]], 1);
+                    sx[j] = (uint32_t)slot[j] << 24 | (uint32_t)(io & 1) | (uint32_t)(io >> 16) << 7 |
+                            (same ? 1u << 14 : 0u);
                     if (uh) {
                         if (Push) {
                             int32_t *row = J.hhc + ((int64_t)(blockIdx.x % kHhCopies) * nheavy + (int64_t)hs[j]) * kLpCap;
@@ -1608,6 +1614,17 @@ __global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const
                         }
                     } else if (lh_on) {
                         lh_move(J.lh, u[j], a, c, E);
+                    }
+                }
+                // Fields of 7 bits: at most 64 lanes add a 0 or 1 to each.
+    #pragma unroll
+                for (int j = 0; j < kLpUnroll; ++j) {
+                    const uint32_t t = wave_seg_incl(sx[j]);
+                    const int32_t q = q0 + j * kWave + lane;
+                    if (slot[j] >= 0 && (lane == kWave - 1 || q + 1 == s_b[w].incl[slot[j]])) {
+                        const uint32_t io = (t & 0x7fu) | ((t >> 7) & 0x7fu) << 16, same = (t >> 14) & 0x7fu;
+                        if (io) atomicAdd(&s_io[w][slot[j]], (int32_t)io);
+                        if (same) atomicAdd(&s_same[w][slot[j]], (int32_t)same);
                     }
                 }
             }
