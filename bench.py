@@ -359,8 +359,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; how the path would shard over "
-                                   "several is designed in DESIGN.md but not built",
+        print(json.dumps({"error": "the library runs on one GPU; sharding the path over several "
+                                   "GPUs is not built",
                           "n_gpus": a.gpus}))
         return 2
     if a.config == "c4":

@@ -40,7 +40,7 @@ void check(int rc, dmolp_ctx *ctx, const char *what) {
 
 void require_single_rank(int mpi_rank, int mpi_size) {
     if (mpi_rank != 0 || mpi_size != 1)
-        throw std::runtime_error("this build partitions on one GPU in one process (mpi_size 1)");
+        throw std::runtime_error("the library partitions on one GPU in one process (mpi_size 1)");
 }
 
 // Upload a reference Graph (int offsets; ghost ids would be >= num_vertices,

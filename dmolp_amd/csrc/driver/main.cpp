@@ -59,7 +59,7 @@ bool parse(int argc, char **argv, Options &o) {
     o.num_partitions = std::atoi(pos[1]);
     o.theta = std::atoi(pos[2]);
     // The reference accepts theta = 0 (main.cpp:188), which never finishes
-    // (partition.cpp:427); this build rejects it.
+    // (partition.cpp:427); it is rejected here.
     if (o.num_partitions <= 0 || o.theta <= 0) {
         std::cerr << "error: num_partitions > 0 and theta > 0 required\n";
         return false;

@@ -41,7 +41,7 @@ typedef enum {
     DMOLP_E_ODD_CUT = -4,       /* odd cut: asymmetric CSR (phase2.cpp:246-250) */
     DMOLP_E_HIP = -5,           /* HIP runtime failure */
     DMOLP_E_NOMEM = -6,         /* device or host allocation failed */
-    DMOLP_E_UNSUPPORTED = -7,   /* valid request outside this build's limits */
+    DMOLP_E_UNSUPPORTED = -7,   /* valid request outside the library's limits */
     DMOLP_E_STATE = -8,         /* call out of order (e.g. no labels resident) */
     DMOLP_E_GRAPH = -9          /* CSR fails validation (self-loop, unsorted row, bad offset) */
 } dmolp_status;
@@ -55,7 +55,7 @@ typedef enum {
 /* Labels at or above this value are invisible to the LP decision
  * (gpu_lp_boundary.cu:360). */
 #define DMOLP_LP_LABEL_CAP 32
-/* Largest k this build accepts. */
+/* Largest k the library accepts. */
 #define DMOLP_MAX_K 1024
 
 typedef struct dmolp_ctx dmolp_ctx;
