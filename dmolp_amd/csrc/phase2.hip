@@ -1515,7 +1515,7 @@ __device__ __forceinline__ void row_upkeep(const P2Job &J, bool lh_on, bool hh_p
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_delta_light(const P2Job *JS, const int32_t *bstart, int jobs) {
+__global__ __launch_bounds__(kThreads, 6) void k_delta_light(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
     const ListDev D = J.D;
