@@ -179,6 +179,7 @@ const char *dmolp_status_string(int status) {
         case DMOLP_E_UNSUPPORTED: return "unsupported request";
         case DMOLP_E_STATE: return "call out of order";
         case DMOLP_E_GRAPH: return "CSR failed validation";
+        case DMOLP_E_INTERNAL: return "device consistency check failed";
         default: return "unknown status";
     }
 }
@@ -547,7 +548,9 @@ int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, 
                                            hipMemcpyDeviceToHost, s));
         DMOLP_HIP_CHECK(hipEventSynchronize(ctx->ev1));
         DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-        DMOLP_REQUIRE(c.err == 0, c.err, "phase 2 rejected the labels (see status)");
+        DMOLP_REQUIRE(c.err == 0, c.err,
+                      c.err == DMOLP_E_INTERNAL ? "a phase-2 kernel found an impossible state"
+                                                : "phase 2 rejected the labels (see status)");
         float ms = 0.f;
         DMOLP_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         if (ctx->profiling) {

@@ -2032,7 +2032,10 @@ __global__ void k_round_end(const P2Job *JS, const int32_t *bstart, int jobs) {
     ctl->rounds = t;
     ctl->v[C_LP_EDGES] += (unsigned long long)bdeg;
     int64_t cut_rec = -1;
-    if (nb == 0) {
+    if (ctl->v[C_KERNEL_ERR] != 0) {  // a kernel of this round found an impossible state
+        ctl->done = 1;
+        ctl->err = DMOLP_E_INTERNAL;
+    } else if (nb == 0) {
         ctl->done = 1;
         ctl->reason = DMOLP_EXIT_EMPTY_BOUNDARY;
     } else if (nchg == 0) {
@@ -2304,6 +2307,7 @@ SelArgs phase2_sel_args(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     // straight to the B_t bitmap.  Records have valid labels: a run with
     // any label outside [0, k) stops after the initial sweep.
     SelArgs a = select_args(ctx, w.sel, w.bd.dev, rule, &ctl->done, w.B.dev, nullptr);
+    a.fault = &ctl->v[C_KERNEL_ERR];
     select_args_dense(a, w.rdeg.as<int32_t>(), g->labels.as<int32_t>(),
                       w.k <= kExact8 ? w.L8.as<uint8_t>() : nullptr, g->n, w.bbits.as<uint32_t>(),
                       (g->n + 31) / 32);

@@ -364,8 +364,12 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_rank_scan(const SelArgs *A,
     }
     __syncthreads();
     // The counts sum to the cut bin's size, which holds the need; a group
-    // without an owner (no such chunk) would be a counting fault: stop.
-    if (s_owner < 0) return;  // uniform
+    // without an owner (no such chunk) is a counting fault: report it (the
+    // run stops with DMOLP_E_INTERNAL at the round's end) and read nothing.
+    if (s_owner < 0) {  // uniform
+        if (threadIdx.x == 0 && a.fault) atomicAdd(a.fault, 1ull);
+        return;
+    }
     // The owner's range, walked by the whole block kSelThreads chunks a step
     // (an inclusive scan per step) instead of one chunk a load.
     const int64_t o0 = (int64_t)s_owner * per;

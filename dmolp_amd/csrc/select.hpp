@@ -106,6 +106,7 @@ struct SelArgs {
     SelState *st;
     int32_t *refining;
     const int32_t *done;      // the job's stop flag (nullable)
+    unsigned long long *fault;  // counted when a pass finds an impossible state (nullable)
     int32_t *mark;            // selected vertices get mark[x] = *mark_val (nullable)
     const int32_t *mark_val;
     TakeRule rule;

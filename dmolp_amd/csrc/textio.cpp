@@ -1,15 +1,24 @@
 // textio.cpp -- the reference's text adjacency format.
 //
 // Format (init.cpp:25-100, written by tool/convert.sh:132-176): the first
-// line holds V; line i+1 lists the neighbours of vertex i.  Differences from
-// the reference's loader (init.cpp:15-23,53-82), all on malformed input:
+// line holds V; line i+1 lists the neighbours of vertex i.  As in the
+// reference, the vertex count is the number of adjacency lines: the header
+// is read but its value is replaced (init.cpp:96 overwrites V with the
+// summed line count).  Differences from the reference's loader
+// (init.cpp:15-23,53-82), on malformed input only:
 //  - a last line without '\n' is kept (the reference drops it);
 //  - a line longer than the 16 MB read buffer is kept whole (the reference
 //    keeps only its tail, init.cpp:81);
-//  - a line starting with a non-digit does not yield a phantom neighbour 0.
+//  - a line starting with a non-digit does not yield a phantom neighbour 0;
+//  - a neighbour id above 2^31-1 (which the reference's int parser wraps)
+//    or outside [0, V) fails the load with DMOLP_E_GRAPH.
 // Rows are sorted and deduplicated as build_local_graph_and_ghosts does
-// (partition.cpp:231-232); self-loops are dropped (the device path requires
-// none; random.cpp:90 drops them too).
+// (partition.cpp:231-232) and self-loops are dropped (the device path
+// requires none; random.cpp:90 drops them too).  tool/convert.sh writes
+// files with neither (convert.sh:76-79,106-116); for a file that holds
+// duplicate neighbours or self-loops, the reference's phase 1 takes degrees
+// from the raw line lengths (init.cpp:86-90) and keeps self-loops in its
+// CSR, whereas here every degree is the cleaned row's length.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -32,9 +41,14 @@ extern "C" int dmolp_read_adjacency_text(const char *path, dmolp_csr *out) {
     std::vector<int32_t> row;
     int64_t declared = -1, line = 0;
     int64_t cur = 0;
-    bool in_num = false, have_header = false;
+    bool in_num = false, have_header = false, too_big = false;
+    constexpr int64_t kIdMax = 0x7fffffffll;
     auto end_line = [&]() {
-        if (in_num) { if (!have_header) declared = cur; else row.push_back((int32_t)cur); }
+        if (in_num) {
+            if (!have_header) declared = cur;
+            else if (cur > kIdMax) too_big = true;
+            else row.push_back((int32_t)cur);
+        }
         in_num = false;
         cur = 0;
         if (!have_header) { have_header = true; return; }
@@ -54,10 +68,14 @@ extern "C" int dmolp_read_adjacency_text(const char *path, dmolp_csr *out) {
             if (ch == '\n') { end_line(); pending = false; continue; }
             pending = true;
             if (ch >= '0' && ch <= '9') {
-                cur = cur * 10 + (ch - '0');
+                // Saturate above the id range (the header may be larger; a
+                // saturated neighbour id fails the load below).
+                cur = cur > kIdMax ? cur : cur * 10 + (ch - '0');
                 in_num = true;
             } else if (in_num) {
-                if (!have_header) declared = cur; else row.push_back((int32_t)cur);
+                if (!have_header) declared = cur;
+                else if (cur > kIdMax) too_big = true;
+                else row.push_back((int32_t)cur);
                 in_num = false;
                 cur = 0;
             }
@@ -65,9 +83,9 @@ extern "C" int dmolp_read_adjacency_text(const char *path, dmolp_csr *out) {
     }
     std::fclose(fp);
     if (pending) end_line();
-    if (!have_header || declared < 0) return DMOLP_E_GRAPH;
-    const int64_t n = line;
-    if (n != declared) return DMOLP_E_GRAPH;
+    if (!have_header || declared < 0 || too_big) return DMOLP_E_GRAPH;
+    const int64_t n = line;  // the header's V is replaced by the line count (init.cpp:96)
+    if (n > kIdMax) return DMOLP_E_GRAPH;
     for (int32_t u : col)
         if (u < 0 || u >= n) return DMOLP_E_GRAPH;
     out->n = n;

@@ -43,7 +43,8 @@ typedef enum {
     DMOLP_E_NOMEM = -6,         /* device or host allocation failed */
     DMOLP_E_UNSUPPORTED = -7,   /* valid request outside the library's limits */
     DMOLP_E_STATE = -8,         /* call out of order (e.g. no labels resident) */
-    DMOLP_E_GRAPH = -9          /* CSR fails validation (self-loop, unsorted row, bad offset) */
+    DMOLP_E_GRAPH = -9,         /* CSR fails validation (self-loop, unsorted row, bad offset) */
+    DMOLP_E_INTERNAL = -10      /* a device-side consistency check failed (a library fault) */
 } dmolp_status;
 
 /* Phase-2 exit reasons (phase2.cpp:309-312, 387-390, 409-415). */

@@ -2,8 +2,17 @@
 //
 // TEST INFRASTRUCTURE ONLY: the checker the HIP path is compared against.
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
-// it.  It is built with -O2 -fno-fast-math -ffp-contract=off so every double
+// it.  It is built with -O3 -fno-fast-math -ffp-contract=off so every double
 // operation is a correctly rounded IEEE operation in source order.
+//
+// OpenMP: the integer sweeps (statistics, cut, candidate marking, boundary
+// test, LP decisions, expansion claims, depth-2 balls) run on
+// ref_set_threads() threads (default: OMP_NUM_THREADS).  Every parallel
+// step either sums integers or produces a set that is sorted or consumed as
+// a set afterwards, and the claim step keeps the lowest claiming part with
+// an atomic minimum, so results do not depend on the thread count.  The
+// floating-point steps (hub tangent, penalties, selection ratio, balances)
+// stay sequential in source order.
 //
 // Each function names the reference lines it restates.  The restatement is
 // written for one rank: owner(v) = 0, no ghosts, every MPI collective is the
@@ -30,7 +39,10 @@
 //                                gpu_lp_boundary.cu:393-421 in Python.
 #include "ref_cpu.h"
 
+#include <omp.h>
+
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -59,20 +71,53 @@ inline int bitrev5(int x) {
 // 0, 1, 2; seed.cpp:33-98), OR-ed into `mark`.  The ball of s is computed on
 // its own (compute_bfs starts from empty bitsets, seed.cpp:21-24), so level
 // 1 is every neighbour of s whatever `mark` already holds.
+// Concurrent writers only ever store 1, so the marks are relaxed atomic
+// byte stores.
+inline void set_mark(uint8_t *m) { __atomic_store_n(m, (uint8_t)1, __ATOMIC_RELAXED); }
+
 void ball2(int64_t n, const int64_t *off, const int32_t *col, int32_t s,
            std::vector<uint8_t> &mark) {
     (void)n;
-    mark[s] = 1;
-    for (int64_t e = off[s]; e < off[s + 1]; ++e) {
+    uint8_t *mk = mark.data();
+    mk[s] = 1;
+    const int64_t e0 = off[s], e1 = off[s + 1];
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t e = e0; e < e1; ++e) {
         const int32_t u = col[e];
-        mark[u] = 1;
-        for (int64_t f = off[u]; f < off[u + 1]; ++f) mark[col[f]] = 1;
+        set_mark(mk + u);
+        for (int64_t f = off[u]; f < off[u + 1]; ++f) set_mark(mk + col[f]);
+    }
+}
+
+// Ids v in [0, n) with mark[v] set, ascending; the marks are cleared.
+void collect_marked(int64_t n, std::vector<uint8_t> &mark, std::vector<int32_t> &out) {
+    const int T = omp_get_max_threads();
+    std::vector<int64_t> cnt((size_t)T + 1, 0);
+    uint8_t *mk = mark.data();
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        const int64_t a = n * t / nt, b = n * (t + 1) / nt;
+        int64_t c = 0;
+        for (int64_t v = a; v < b; ++v) c += mk[v] != 0;
+        cnt[(size_t)t + 1] = c;
+#pragma omp barrier
+#pragma omp single
+        for (int i = 0; i < nt; ++i) cnt[(size_t)i + 1] += cnt[(size_t)i];
+        int64_t w = cnt[(size_t)t];
+        if (t == 0) out.resize((size_t)cnt[(size_t)nt]);
+#pragma omp barrier
+        for (int64_t v = a; v < b; ++v)
+            if (mk[v]) { out[(size_t)w++] = (int32_t)v; mk[v] = 0; }
     }
 }
 
 }  // namespace
 
 extern "C" {
+
+void ref_set_threads(int32_t t) { omp_set_num_threads(t > 0 ? t : 1); }
+int32_t ref_get_threads(void) { return omp_get_max_threads(); }
 
 // init.cpp:43-44,68,90-93 (adj and local_degree built with reserve(2V) in
 // ascending source order) and init.cpp:123-135 (global_degree with
@@ -232,7 +277,9 @@ int ref_seeds(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
 // order against one shared visited set (partition.cpp:451-476) and at one
 // rank each vertex then has a single claimant (partition.cpp:530-538);
 // frontiers become carry + newly labelled (partition.cpp:588-597); sizes are
-// recounted (partition.cpp:586).
+// recounted (partition.cpp:586).  A frontier is used only as a set (the
+// selection takes its theta smallest keys, which are unique), so the order
+// in which claims join it does not matter.
 int ref_expand(int64_t n, const int64_t *off, const int32_t *col, int32_t k, int32_t theta,
                const int32_t *seeds, int64_t *rounds_out, int32_t *fill_round_out,
                int32_t *labels_out) {
@@ -249,12 +296,30 @@ int ref_expand(int64_t n, const int64_t *off, const int32_t *col, int32_t k, int
     }
     std::vector<int64_t> size((size_t)k, 0);
     int64_t labelled = 0;
-    for (int64_t v = 0; v < n; ++v)
-        if (L[v] >= 0) { ++size[(size_t)L[v]]; ++labelled; }
-    std::vector<uint8_t> visited((size_t)n, 0);
-    std::vector<std::vector<int32_t>> fresh((size_t)k);
+    auto recount = [&]() {
+        std::fill(size.begin(), size.end(), 0);
+        labelled = 0;
+#pragma omp parallel
+        {
+            std::vector<int64_t> mine((size_t)k, 0);
+            int64_t lab = 0;
+#pragma omp for schedule(static) nowait
+            for (int64_t v = 0; v < n; ++v)
+                if (L[v] >= 0) { ++mine[(size_t)L[v]]; ++lab; }
+#pragma omp critical
+            {
+                for (int32_t p = 0; p < k; ++p) size[(size_t)p] += mine[(size_t)p];
+                labelled += lab;
+            }
+        }
+    };
+    recount();
+    // claim[v]: lowest part whose selected vertices reach v this round
+    // (INT32_MAX: none).
+    std::vector<int32_t> claim((size_t)n, INT32_MAX);
     int64_t round = 0;
     int32_t fill_round = -1;
+    std::vector<std::vector<uint64_t>> sel((size_t)k);
     while (labelled < n) {
         bool all_empty = true;
         for (int32_t p = 0; p < k; ++p)
@@ -268,9 +333,10 @@ int ref_expand(int64_t n, const int64_t *off, const int32_t *col, int32_t k, int
             if (fill_round < 0) fill_round = (int32_t)round;
         }
         // Selection: the theta smallest keys of each frontier.
-        std::vector<std::vector<uint64_t>> sel((size_t)k);
+#pragma omp parallel for schedule(dynamic, 1)
         for (int32_t p = 0; p < k; ++p) {
             auto &f = F[(size_t)p];
+            sel[(size_t)p].clear();
             if ((int64_t)f.size() > theta) {
                 std::nth_element(f.begin(), f.begin() + theta, f.end());
                 sel[(size_t)p].assign(f.begin(), f.begin() + theta);
@@ -280,34 +346,42 @@ int ref_expand(int64_t n, const int64_t *off, const int32_t *col, int32_t k, int
                 f.clear();
             }
         }
-        // Claims: ascending partition order, shared visited set; the label
-        // test reads labels as they were before this round's claims.
+        // Claims: the lowest part reaching v wins; the label test reads labels
+        // as they were before this round's claims.
+        std::vector<std::pair<int32_t, int64_t>> work;  // (part, index into sel[part])
+        for (int32_t p = 0; p < k; ++p)
+            for (size_t i = 0; i < sel[(size_t)p].size(); ++i) work.emplace_back(p, (int64_t)i);
         std::vector<int32_t> touched;
-        for (int32_t p = 0; p < k; ++p) {
-            for (uint64_t kk : sel[(size_t)p]) {
-                int32_t u = (int32_t)(kk & 0xffffffffu);
+#pragma omp parallel
+        {
+            std::vector<int32_t> mine;
+#pragma omp for schedule(dynamic, 16) nowait
+            for (int64_t w = 0; w < (int64_t)work.size(); ++w) {
+                const int32_t p = work[(size_t)w].first;
+                const int32_t u = (int32_t)(sel[(size_t)p][(size_t)work[(size_t)w].second] & 0xffffffffu);
                 for (int64_t e = off[u]; e < off[u + 1]; ++e) {
-                    int32_t v = col[e];
-                    if (L[v] == -1 && !visited[(size_t)v]) {
-                        visited[(size_t)v] = 1;
-                        touched.push_back(v);
-                        fresh[(size_t)p].push_back(v);
+                    const int32_t v = col[e];
+                    if (L[v] != -1) continue;
+                    int32_t cur = __atomic_load_n(&claim[(size_t)v], __ATOMIC_RELAXED);
+                    while (p < cur) {
+                        if (__atomic_compare_exchange_n(&claim[(size_t)v], &cur, p, false,
+                                                        __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                            if (cur == INT32_MAX) mine.push_back(v);
+                            break;
+                        }
                     }
                 }
             }
+#pragma omp critical
+            touched.insert(touched.end(), mine.begin(), mine.end());
         }
-        for (int32_t v : touched) visited[(size_t)v] = 0;
-        for (int32_t p = 0; p < k; ++p) {
-            for (int32_t v : fresh[(size_t)p]) {
-                L[v] = p;
-                F[(size_t)p].push_back(key(v));
-            }
-            fresh[(size_t)p].clear();
+        for (int32_t v : touched) {
+            const int32_t p = claim[(size_t)v];
+            L[v] = p;
+            F[(size_t)p].push_back(key(v));
+            claim[(size_t)v] = INT32_MAX;
         }
-        std::fill(size.begin(), size.end(), 0);
-        labelled = 0;
-        for (int64_t v = 0; v < n; ++v)
-            if (L[v] >= 0) { ++size[(size_t)L[v]]; ++labelled; }
+        recount();
         ++round;
     }
     if (rounds_out) *rounds_out = round;
@@ -321,13 +395,24 @@ int ref_phase1_metrics(int64_t n, const int64_t *off, const int32_t *col, int32_
     if (n <= 0 || k <= 0 || !labels || !m) return REF_E_ARG;
     std::vector<int64_t> cv((size_t)k, 0), ce((size_t)k, 0);
     int64_t cut2 = 0;
-    for (int64_t u = 0; u < n; ++u) {
-        int32_t lu = labels[u];
-        if (lu >= 0) ++cv[(size_t)lu];
-        for (int64_t e = off[u]; e < off[u + 1]; ++e) {
-            int32_t lv = labels[col[e]];
-            if (lu != -1 && lv != -1 && lu != lv) ++cut2;
-            if (lu >= 0 && lv == lu) ++ce[(size_t)lu];
+#pragma omp parallel
+    {
+        std::vector<int64_t> mv((size_t)k, 0), me((size_t)k, 0);
+        int64_t mc = 0;
+#pragma omp for schedule(dynamic, 1024) nowait
+        for (int64_t u = 0; u < n; ++u) {
+            int32_t lu = labels[u];
+            if (lu >= 0) ++mv[(size_t)lu];
+            for (int64_t e = off[u]; e < off[u + 1]; ++e) {
+                int32_t lv = labels[col[e]];
+                if (lu != -1 && lv != -1 && lu != lv) ++mc;
+                if (lu >= 0 && lv == lu) ++me[(size_t)lu];
+            }
+        }
+#pragma omp critical
+        {
+            for (int32_t p = 0; p < k; ++p) { cv[(size_t)p] += mv[(size_t)p]; ce[(size_t)p] += me[(size_t)p]; }
+            cut2 += mc;
         }
     }
     for (int32_t p = 0; p < k; ++p) ce[(size_t)p] /= 2;
@@ -388,15 +473,27 @@ int ref_stats(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
     if (n <= 0 || k <= 0 || !labels) return REF_E_ARG;
     std::vector<int64_t> cv((size_t)k, 0), ce((size_t)k, 0);
     int64_t cut2 = 0, invalid = 0;
-    for (int64_t u = 0; u < n; ++u) {
-        int32_t lu = labels[u];
-        bool lu_ok = lu >= 0 && lu < k;
-        if (lu_ok) ++cv[(size_t)lu];
-        for (int64_t e = off[u]; e < off[u + 1]; ++e) {
-            int32_t lv = labels[col[e]];
-            if (lu_ok && lv == lu) ++ce[(size_t)lu];
-            if (lu == -1 || lv == -1) { ++invalid; continue; }
-            if (lu != lv) ++cut2;
+#pragma omp parallel
+    {
+        std::vector<int64_t> mv((size_t)k, 0), me((size_t)k, 0);
+        int64_t mc = 0, mi = 0;
+#pragma omp for schedule(dynamic, 1024) nowait
+        for (int64_t u = 0; u < n; ++u) {
+            int32_t lu = labels[u];
+            bool lu_ok = lu >= 0 && lu < k;
+            if (lu_ok) ++mv[(size_t)lu];
+            for (int64_t e = off[u]; e < off[u + 1]; ++e) {
+                int32_t lv = labels[col[e]];
+                if (lu_ok && lv == lu) ++me[(size_t)lu];
+                if (lu == -1 || lv == -1) { ++mi; continue; }
+                if (lu != lv) ++mc;
+            }
+        }
+#pragma omp critical
+        {
+            for (int32_t p = 0; p < k; ++p) { cv[(size_t)p] += mv[(size_t)p]; ce[(size_t)p] += me[(size_t)p]; }
+            cut2 += mc;
+            invalid += mi;
         }
     }
     if (invalid) return REF_E_INVALID_LABEL;
@@ -453,42 +550,57 @@ struct SelectCounts {
 void select_boundary(int64_t n, const int64_t *off, const int32_t *col, const int32_t *L,
                      const std::vector<int32_t> &prev, const double *re, int32_t k, int32_t t,
                      std::vector<int32_t> &out, std::vector<uint8_t> &mark, SelectCounts &sc) {
-    // Candidates: previous boundary and its neighbours, unique
-    // (gpu_lp_boundary.cu:207-240).
+    // Candidates: previous boundary and its neighbours, unique and sorted
+    // (gpu_lp_boundary.cu:207-240): marked, then collected in id order.
     std::vector<int32_t> cand;
-    for (int32_t v : prev) {
+    uint8_t *mk = mark.data();
+    int64_t prev_deg = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : prev_deg)
+    for (int64_t i = 0; i < (int64_t)prev.size(); ++i) {
+        const int32_t v = prev[(size_t)i];
         if (v < 0 || v >= n) continue;
-        sc.prev_deg += deg_of(off, v);
-        if (!mark[(size_t)v]) { mark[(size_t)v] = 1; cand.push_back(v); }
-        for (int64_t e = off[v]; e < off[v + 1]; ++e) {
-            int32_t u = col[e];
-            if (!mark[(size_t)u]) { mark[(size_t)u] = 1; cand.push_back(u); }
-        }
+        prev_deg += deg_of(off, v);
+        set_mark(mk + v);
+        for (int64_t e = off[v]; e < off[v + 1]; ++e) set_mark(mk + col[e]);
     }
-    for (int32_t v : cand) mark[(size_t)v] = 0;
-    std::sort(cand.begin(), cand.end());
+    sc.prev_deg += prev_deg;
+    collect_marked(n, mark, cand);
     sc.cand = (int64_t)cand.size();
     // Boundary test with early exit (gpu_lp_boundary.cu:251-264), grouped by
-    // label (gpu_lp_boundary.cu:266-269).
+    // label (gpu_lp_boundary.cu:266-269).  Each group is sorted by a total
+    // order below, so the order in which threads fill it does not matter.
     std::vector<std::vector<std::pair<int32_t, int64_t>>> groups((size_t)k);
-    for (int32_t x : cand) {
-        int32_t lx = L[x];
-        bool bd = false;
-        int64_t scanned = 0;
-        for (int64_t e = off[x]; e < off[x + 1]; ++e) {
-            ++scanned;
-            if (L[col[e]] != lx) { bd = true; break; }
+    int64_t scan = 0, nbd = 0;
+#pragma omp parallel reduction(+ : scan, nbd)
+    {
+        std::vector<std::vector<std::pair<int32_t, int64_t>>> mine((size_t)k);
+#pragma omp for schedule(dynamic, 1024) nowait
+        for (int64_t i = 0; i < (int64_t)cand.size(); ++i) {
+            const int32_t x = cand[(size_t)i];
+            int32_t lx = L[x];
+            bool bd = false;
+            int64_t scanned = 0;
+            for (int64_t e = off[x]; e < off[x + 1]; ++e) {
+                ++scanned;
+                if (L[col[e]] != lx) { bd = true; break; }
+            }
+            scan += scanned;
+            if (bd) {
+                mine[(size_t)lx].emplace_back(x, deg_of(off, x));
+                ++nbd;
+            }
         }
-        sc.scan += scanned;
-        if (bd) {
-            groups[(size_t)lx].emplace_back(x, deg_of(off, x));
-            ++sc.bd;
-        }
+#pragma omp critical
+        for (int32_t p = 0; p < k; ++p)
+            groups[(size_t)p].insert(groups[(size_t)p].end(), mine[(size_t)p].begin(), mine[(size_t)p].end());
     }
+    sc.scan += scan;
+    sc.bd += nbd;
     // Per label, the first ceil(ratio * size) by (degree desc, id) when
     // RE > 1.0, else (degree asc, id) (gpu_lp_boundary.cu:244,277-302).
     double ratio = exp((-t) / (5.0));
-    out.clear();
+    std::vector<int64_t> take_of((size_t)k, 0);
+#pragma omp parallel for schedule(dynamic, 1)
     for (int32_t p = 0; p < k; ++p) {
         auto &g = groups[(size_t)p];
         if (g.empty()) continue;
@@ -505,9 +617,12 @@ void select_boundary(int64_t n, const int64_t *off, const int32_t *col, const in
         }
         int take = static_cast<int>(std::ceil(ratio * static_cast<double>(g.size())));
         if (take > (int)g.size()) take = (int)g.size();
-        for (int i = 0; i < take; ++i) out.push_back(g[(size_t)i].first);
+        take_of[(size_t)p] = take;
     }
-    std::sort(out.begin(), out.end());
+    out.clear();
+    for (int32_t p = 0; p < k; ++p)
+        for (int64_t i = 0; i < take_of[(size_t)p]; ++i) set_mark(mk + groups[(size_t)p][(size_t)i].first);
+    collect_marked(n, mark, out);
 }
 
 // gpu_lp_boundary.cu:354-426 for one boundary vertex: histogram over labels
@@ -555,11 +670,14 @@ int ref_select_boundary(int64_t n, const int64_t *off, const int32_t *col,
 int ref_lp_round(int64_t n, const int64_t *off, const int32_t *col, const int32_t *labels,
                  const int32_t *b, int64_t nb, const double *pen, int32_t k, int32_t *labels_out) {
     if (n <= 0 || k <= 0 || !labels || !pen || !labels_out || nb < 0) return REF_E_ARG;
-    std::vector<int64_t> hist((size_t)kLpLabelCap);
-    for (int64_t i = 0; i < nb; ++i) {
-        int32_t v = b[i];
-        if (v < 0 || v >= n) return REF_E_ARG;
-        labels_out[v] = lp_decide(off, col, labels, v, pen, k, hist.data());
+    for (int64_t i = 0; i < nb; ++i)
+        if (b[i] < 0 || b[i] >= n) return REF_E_ARG;
+#pragma omp parallel
+    {
+        std::vector<int64_t> hist((size_t)kLpLabelCap);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t i = 0; i < nb; ++i)
+            labels_out[b[i]] = lp_decide(off, col, labels, b[i], pen, k, hist.data());
     }
     return REF_OK;
 }
@@ -582,7 +700,6 @@ int ref_phase2(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
     std::vector<uint8_t> mark((size_t)n, 0);
     std::vector<double> pen((size_t)k), re((size_t)k);
     std::vector<int32_t> newL(labels, labels + n);
-    std::vector<int64_t> hist((size_t)kLpLabelCap);
     std::vector<int32_t> changed;
     int32_t rounds = 0, reason = 0;
     int64_t lp_edges = 0;
@@ -612,9 +729,18 @@ int ref_phase2(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
         if (B.empty()) { reason = 1; break; }
         // Jacobi update: every decision reads the labels of round t-1.
         changed.clear();
-        for (int32_t b : B) {
-            int32_t nl = lp_decide(off, col, labels, b, pen.data(), k, hist.data());
-            if (nl != labels[b]) { newL[(size_t)b] = nl; changed.push_back(b); }
+#pragma omp parallel
+        {
+            std::vector<int64_t> h((size_t)kLpLabelCap);
+            std::vector<int32_t> mine;
+#pragma omp for schedule(dynamic, 256) nowait
+            for (int64_t i = 0; i < (int64_t)B.size(); ++i) {
+                const int32_t b = B[(size_t)i];
+                int32_t nl = lp_decide(off, col, labels, b, pen.data(), k, h.data());
+                if (nl != labels[b]) { newL[(size_t)b] = nl; mine.push_back(b); }
+            }
+#pragma omp critical
+            changed.insert(changed.end(), mine.begin(), mine.end());
         }
         int64_t cdeg = 0;
         for (int32_t v : changed) { labels[v] = newL[(size_t)v]; cdeg += deg_of(off, v); }

@@ -79,6 +79,10 @@ typedef struct {
     int64_t *cut;          /* cut after round t (or -1 when the round stopped before stats) */
 } ref_p2_trace;
 
+/* OpenMP threads for the parallel sweeps (results do not depend on it). */
+void ref_set_threads(int32_t t);
+int32_t ref_get_threads(void);
+
 /* ---- Phase 1 pieces (init.cpp, seed.cpp, partition.cpp, phase1.cpp) ---- */
 
 /* Iteration order of the degree map the reference builds at one rank

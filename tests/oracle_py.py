@@ -73,11 +73,24 @@ def lib():
             fn = getattr(L, name)
             fn.restype = C.c_int
             fn.argtypes = args
+        L.ref_set_threads.restype = None
+        L.ref_set_threads.argtypes = [i32]
+        L.ref_get_threads.restype = i32
+        L.ref_get_threads.argtypes = []
         L.ref_penalties.restype = None
         L.ref_penalties.argtypes = [i32, P(i64), P(i64), i32, P(f64), P(f64)]
         _ = VP
         _lib = L
     return _lib
+
+
+def set_threads(t: int):
+    """OpenMP threads of the oracle's sweeps (results do not change)."""
+    lib().ref_set_threads(int(t))
+
+
+def get_threads() -> int:
+    return int(lib().ref_get_threads())
 
 
 def _p(a, ct):

@@ -162,6 +162,33 @@ def test_text_reader_normalises_rows(tmp_path):
     assert list(col) == [1, 3, 0, 2, 1, 0]
 
 
+def test_text_reader_vertex_count_is_the_line_count(tmp_path):
+    """The header's V is replaced by the number of adjacency lines, as the
+    reference's Allreduce of the line count does (init.cpp:96)."""
+    from dmolp_amd import api
+    p = tmp_path / "g.txt"
+    p.write_text("7\n1\n0 2\n1\n")
+    off, col = api.read_adjacency_text(p)
+    assert list(off) == [0, 1, 3, 4] and list(col) == [1, 0, 2, 1]
+    p.write_text("2\n1\n0 2\n1\n")
+    off, col = api.read_adjacency_text(p)
+    assert len(off) == 4
+
+
+@pytest.mark.parametrize("body", [
+    "2\n4294967297\n0\n",            # wraps to 1 in a 32-bit parser
+    "2\n1\n99999999999999999999999\n",  # overflows 64 bits
+    "2\n1\n2\n",                      # id outside [0, V)
+])
+def test_text_reader_rejects_ids_out_of_range(tmp_path, body):
+    from dmolp_amd import api
+    p = tmp_path / "g.txt"
+    p.write_text(body)
+    with pytest.raises(api.DmolpError) as e:
+        api.read_adjacency_text(p)
+    assert e.value.status == -9
+
+
 def test_cli_rejects_bad_arguments_before_touching_the_device():
     exe = ROOT / "dmolp_amd" / "hpc_partitioning"
     if not exe.exists():
