@@ -28,66 +28,109 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 
-CONFIGS = {
-    "c1": dict(workload="rmat-s16-ef16-seed1", gen=("rmat", 16, 16, 1), k=8, rounds=10, eps=0.03),
-    "c2": dict(workload="rmat-s22-ef16-seed2", gen=("rmat", 22, 16, 2), k=16, rounds=20, eps=0.03),
-    "c3": dict(workload="chunglu-n41652230-m1468365182-g2.0-dmax3e6-seed3",
-               gen=("chunglu", 41652230, 1468365182, 3), k=32, rounds=30, eps=0.05),
-}
-THETA = 10000
+from dmolp_amd.workloads import (C4_WORKLOAD, C5_WORKLOAD, CONFIGS, THETA,  # noqa: E402
+                                 c4_specs, c5_jobs, spec_for)
+
 # Algorithmic bytes of the LP decision per boundary vertex b (int64 offsets,
 # int32 columns and labels): 2 offsets + own label read + label write, and a
 # column plus a label per neighbour.
 O_BYTES, C_BYTES, L_BYTES = 8, 4, 4
-
-
-def spec_for(cfg):
-    from dmolp_amd import api
-    g = cfg["gen"]
-    if g[0] == "rmat":
-        return api.rmat_spec(g[1], g[2], seed=g[3])
-    return api.chung_lu_spec(g[1], g[2], gamma=2.0, dmax=3.0e6, seed=g[3])
+FIXTURES = ROOT / "tests" / "golden"
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(off, col, k, labels0, eps, rounds_cap):
-    """The CPU oracle's phase 2 (single thread) on the same graph and start
-    labels, for the first rounds_cap round(s)."""
+def sha(a) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def host_info():
+    """The host the CPU baseline ran on: logical CPUs, the CPUs this process
+    may use, physical cores and the CPU model (lscpu)."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=10).stdout
+        info["physical_cores"] = len({ln for ln in out.splitlines() if ln and ln[0] != "#"})
+        for ln in subprocess.run(["lscpu"], capture_output=True, text=True,
+                                 timeout=10).stdout.splitlines():
+            if ln.startswith("Model name:"):
+                info["model"] = ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_threads():
+    """Threads for the multi-threaded CPU baseline: the physical cores this
+    process may use, at most 16 (the host share of one GPU)."""
+    info = host_info()
+    return max(1, min(16, info["affinity"], info.get("physical_cores") or info["affinity"])), info
+
+
+def oracle():
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_py as O
-    t0 = time.perf_counter()
-    res = O.phase2(off, col, k, labels0, rounds_cap, eps)
-    elapsed = time.perf_counter() - t0
+    return O
+
+
+def timed_median(fn, reps):
+    ts, out = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), out
+
+
+def cpu_baseline_full(off, col, k, R, eps, threads, reps=3):
+    """The CPU oracle's phase 1 and phase 2 (R rounds) on the same graph,
+    timed separately, median of `reps` runs, on `threads` threads and on one
+    thread.  Returns the baseline record and the oracle's results."""
+    O = oracle()
+    rec = {}
+    p1 = p2 = None
+    for T in sorted({threads, 1}, reverse=True):
+        O.set_threads(T)
+        t1, p1 = timed_median(lambda: O.phase1(off, col, k, THETA), reps)
+        t2, p2 = timed_median(lambda: O.phase2(off, col, k, p1.labels, R, eps), reps)
+        rec[T] = {"phase1_s": t1, "phase2_s": t2, "lp_edges_per_s": p2.m.lp_edges / t2}
+    top = rec[threads]
     return {
-        "value": res.m.lp_edges / elapsed,
-        "unit": "LP edges/s",
-        "cores": 1,
+        "value": top["lp_edges_per_s"], "unit": "LP edges/s", "cores": threads, "kind": "port",
+        "sample": f"the whole workload: oracle phase 1 and phase 2 ({p2.m.rounds} rounds, "
+                  f"{p2.m.lp_edges} LP edges), median of {reps} runs each",
+        "phase1_s": top["phase1_s"], "phase2_s": top["phase2_s"],
+        "one_thread": rec[1],
+    }, p1, p2
+
+
+def cpu_baseline_round1(off, col, k, labels0, eps, threads):
+    """The CPU oracle's first phase-2 round (initial statistics, round 1,
+    final statistics) from the device's phase-1 labels, on `threads` threads
+    and on one thread (one run each).  Returns the record and the oracle's
+    one-round result."""
+    O = oracle()
+    rec, res = {}, None
+    for T in sorted({threads, 1}, reverse=True):
+        O.set_threads(T)
+        t0 = time.perf_counter()
+        res = O.phase2(off, col, k, labels0, 1, eps)
+        el = time.perf_counter() - t0
+        rec[T] = {"phase2_round1_s": el, "lp_edges_per_s": res.m.lp_edges / el}
+    return {
+        "value": rec[threads]["lp_edges_per_s"], "unit": "LP edges/s", "cores": threads,
         "kind": "port",
-        "sample": f"oracle phase 2 (stats sweep + {res.m.rounds} round(s) + final stats) on the "
-                  f"same graph and phase-1 labels: {res.m.lp_edges} LP edges in {elapsed:.1f} s",
-    }
-
-
-C4_WORKLOAD = "batch256-rmat-s10to18-grid64to1024-seed4"
-
-
-def c4_specs(seed=4, count=256):
-    """256 jobs: half R-MAT (scale uniform in 10..18, edge factor 16), half 2-D
-    5-point grids (sides log-uniform in 64..1024); k from {2,4,8,16,64}."""
-    from dmolp_amd import api
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for i in range(count):
-        k = int(rng.choice([2, 4, 8, 16, 64]))
-        if i % 2 == 0:
-            jobs.append((api.rmat_spec(int(rng.integers(10, 19)), 16, seed=1000 + i), k))
-        else:
-            r, c = (int(round(x)) for x in np.exp(rng.uniform(np.log(64), np.log(1024), 2)))
-            jobs.append((api.grid_spec(r, c), k))
-    return jobs
+        "sample": f"oracle phase 2 limited to round 1 (initial statistics sweep, boundary "
+                  f"selection over all vertices, {res.m.lp_edges} LP edges, final statistics) "
+                  f"on the same graph and phase-1 labels, one run per thread count",
+        "phase2_round1_s": rec[threads]["phase2_round1_s"],
+        "one_thread": rec[1],
+    }, res
 
 
 def run_batched(a):
@@ -170,23 +213,6 @@ def run_batched(a):
     for g in graphs:
         g.free()
     ctx.close()
-    return out
-
-
-C5_WORKLOAD = "served-rmat-n1e3to1e7-ef8to32-seed5"
-
-
-def c5_jobs(count, seed=5):
-    """Requests: R-MAT, vertices log-uniform in 1e3..1e7 (scale = nearest
-    power of two), edge factor uniform in 8..32, k from {4..128}."""
-    from dmolp_amd import api
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        n = float(np.exp(rng.uniform(np.log(1e3), np.log(1e7))))
-        scale = int(min(23, max(10, round(np.log2(n)))))
-        out.append((api.rmat_spec(scale, int(rng.integers(8, 33)), seed=5000 + i),
-                    int(rng.choice([4, 8, 16, 32, 64, 128]))))
     return out
 
 
@@ -377,12 +403,22 @@ def main(argv=None):
     g = ctx.generate(spec_for(cfg))
     log(f"[bench] generated {cfg['workload']}: n={g.n} nnz={g.nnz} in "
         f"{time.perf_counter() - t0:.2f} s")
+    fx_path = FIXTURES / f"large_{a.config}.json"
+    fx = json.loads(fx_path.read_text()) if fx_path.exists() else None
+    parity = {"fixture": str(fx_path.relative_to(ROOT)) if fx else None}
     t0 = time.perf_counter()
-    p1 = g.phase1(k, THETA, want_labels=False)
+    p1 = g.phase1(k, THETA, want_labels=True)
     p1_wall = time.perf_counter() - t0
+    labels0 = p1.labels
     log(f"[bench] phase 1: {p1_wall:.2f} s wall, device {p1.metrics.device_ms:.1f} ms, "
         f"hub sort {p1.metrics.host_ms:.1f} ms, rounds {p1.metrics.rounds}, "
         f"cut {p1.metrics.initial_edge_cut}")
+    if fx:
+        w1 = fx["phase1"]
+        parity["phase1"] = bool(sha(labels0) == w1["labels_sha256"]
+                                and p1.metrics.initial_edge_cut == w1["cut"]
+                                and p1.seeds.tolist() == w1["seeds"]
+                                and p1.vc.tolist() == w1["vc"] and p1.ec.tolist() == w1["ec"])
     g.checkpoint_labels()
 
     for _ in range(a.warmup):
@@ -429,14 +465,51 @@ def main(argv=None):
         f"{json.dumps({kk: round(v, 3) for kk, v in groups.items()})}; dominant {dom}; "
         f"kernel ms {json.dumps({kk: round(float(v[0].sum()), 3) for kk, v in kern.items()})}")
 
+    # Parity of the timed run's final state: its labels against the
+    # oracle's fixture, and its incrementally kept counts against a
+    # from-scratch sweep of those labels.
+    final = g.get_labels()
+    svc, sic, scut = g.stats(k, final)
+    parity["counts_vs_sweep"] = bool(scut == res.metrics.edge_cut and
+                                     svc.tolist() == pr.vc.tolist() and
+                                     sic.tolist() == pr.ic.tolist())
+    if fx:
+        w2 = fx["phase2"]
+        parity["phase2"] = bool(sha(final) == w2["labels_sha256"]
+                                and res.metrics.edge_cut == w2["cut"]
+                                and res.metrics.rounds == w2["rounds"]
+                                and res.metrics.exit_reason == w2["exit_reason"]
+                                and res.metrics.lp_edges == w2["lp_edges"]
+                                and pr.vc.tolist() == w2["vc"] and pr.ic.tolist() == w2["ic"])
+    del final
+
     cpu = None
     if not a.no_cpu_baseline:
+        T, hinfo = cpu_threads()
         off, col = g.download()
-        g.checkpoint_labels(restore=True)
-        labels0 = g.get_labels()
-        cpu = cpu_baseline(off, col, k, labels0, eps, 1 if a.config == "c3" else 3)
-        log(f"[bench] cpu baseline {cpu['value']:.3e} LP edges/s ({cpu['sample']})")
+        if a.config == "c3":
+            # Bounded sample: round 1 alone, from the device's phase-1 labels,
+            # which also checks the device's first round against the oracle's.
+            cpu, o1 = cpu_baseline_round1(off, col, k, labels0, eps, T)
+            g.set_labels(labels0)
+            d1 = g.phase2(k, 1, eps, want_labels=True, want_trace=False)
+            parity["round1_vs_oracle"] = bool(np.array_equal(d1.labels, o1.labels)
+                                              and d1.metrics.edge_cut == o1.m.edge_cut
+                                              and d1.vc.tolist() == o1.vc.tolist()
+                                              and d1.ic.tolist() == o1.ic.tolist())
+        else:
+            cpu, o1, o2 = cpu_baseline_full(off, col, k, R, eps, T)
+            parity["phase1_vs_oracle"] = bool(np.array_equal(o1.labels, labels0))
+            g.set_labels(labels0)
+            d2 = g.phase2(k, R, eps, want_labels=True, want_trace=False)
+            parity["phase2_vs_oracle"] = bool(np.array_equal(o2.labels, d2.labels)
+                                              and o2.m.edge_cut == d2.metrics.edge_cut)
+        cpu["host"] = hinfo
+        log(f"[bench] cpu baseline {cpu['value']:.3e} LP edges/s on {T} threads "
+            f"({cpu['sample']})")
         del off, col
+    parity["ok"] = all(v for kk, v in parity.items() if kk != "fixture")
+    log(f"[bench] parity {json.dumps(parity)}")
 
     traffic = measured_traffic(cfg["workload"], dom)
     out = {
@@ -464,6 +537,10 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this "
+                                       "kernel on this workload, copy-calibrated "
+                                       "(profiles/traffic.json), not measured in this run"
+                                       if traffic is not None else None,
                      "algorithmic_bytes_per_launch": float(bytes_r[live].mean()),
                      "avg_launch_ms": float(ms_r[live].mean()),
                      "launches": int(live.sum())},
@@ -474,12 +551,13 @@ def main(argv=None):
                               "frac": p2eq_gbs / HBM_PEAK_GBS,
                               "device_ms": float(pr.metrics.device_ms)},
         "cpu_baseline": cpu,
+        "parity": parity,
         "kernel_groups_ms": groups,
     }
     print(json.dumps(out))
     g.free()
     ctx.close()
-    return 0
+    return 0 if parity["ok"] else 1
 
 
 if __name__ == "__main__":

@@ -365,6 +365,17 @@ class Graph:
             _p(vc, C.c_int64), _p(ec, C.c_int64), C.byref(m)), self.ctx.handle)
         return Phase1Result(labels, seeds, vc, ec, m)
 
+    def phase1_random(self, k: int, want_labels=True) -> Phase1Result:
+        """Random initial mode: label = vertex id mod k (random.cpp:71-74)."""
+        labels = np.zeros(self.n, np.int32) if want_labels else None
+        vc = np.zeros(k, np.int64)
+        ec = np.zeros(k, np.int64)
+        m = _abi.Phase1Metrics()
+        _check(_abi.lib().dmolp_phase1_random(
+            self.ctx.handle, self._h, k, _p(labels, C.c_int32) if want_labels else None,
+            _p(vc, C.c_int64), _p(ec, C.c_int64), C.byref(m)), self.ctx.handle)
+        return Phase1Result(labels, np.zeros(0, np.int32), vc, ec, m)
+
     def phase2(self, k: int, max_rounds: int, epsilon: float, want_labels=True,
                want_trace=True) -> Phase2Result:
         vc = np.zeros(k, np.int64)

@@ -31,6 +31,7 @@ SOURCES = [
     "gen_host.cpp",
     "textio.cpp",
     "driver/driver.cpp",
+    "driver/report.cpp",
 ]
 CLI = PKG / "hpc_partitioning"
 

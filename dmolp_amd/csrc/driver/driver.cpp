@@ -43,32 +43,45 @@ void require_single_rank(int mpi_rank, int mpi_size) {
         throw std::runtime_error("the library partitions on one GPU in one process (mpi_size 1)");
 }
 
-// Upload a reference Graph (int offsets; ghost ids would be >= num_vertices,
-// and a single rank has none).
-dmolp_graph *upload(dmolp_ctx *ctx, const Graph &g) {
+// Offsets of a graph as the ABI takes them (offset_bytes 4 or 8).
+inline const void *offsets_of(const Graph &g) { return g.row_ptr.data(); }
+inline const void *offsets_of(const GraphWide &g) { return g.row_ptr.data(); }
+inline int offset_bytes(const Graph &) { return 4; }
+inline int offset_bytes(const GraphWide &) { return 8; }
+
+// Upload a graph (ghost ids would be >= num_vertices; a single rank has
+// none).
+template <typename G>
+dmolp_graph *upload(dmolp_ctx *ctx, const G &g) {
     dmolp_graph *dg = nullptr;
-    if ((int)g.row_ptr.size() != g.num_vertices + 1)
-        throw std::runtime_error("Graph.row_ptr must hold num_vertices + 1 entries");
-    check(dmolp_graph_upload(ctx, g.num_vertices, (int64_t)g.col_indices.size(), g.row_ptr.data(),
-                             4, g.col_indices.data(), &dg),
+    if ((int64_t)g.row_ptr.size() != (int64_t)g.num_vertices + 1)
+        throw std::runtime_error("row_ptr must hold num_vertices + 1 entries");
+    check(dmolp_graph_upload(ctx, g.num_vertices, (int64_t)g.col_indices.size(), offsets_of(g),
+                             offset_bytes(g), g.col_indices.data(), &dg),
           ctx, "dmolp_graph_upload");
     return dg;
 }
 
-}  // namespace
+// Whether a CSR of n vertices and nnz entries fits the graph type.
+inline bool fits(const Graph *, int64_t n, int64_t nnz) {
+    return n < (1ll << 31) && nnz < (1ll << 31);
+}
+inline bool fits(const GraphWide *, int64_t n, int64_t) { return n < (1ll << 31); }
 
-Phase1Metrics run_phase1(int mpi_rank, int mpi_size, const char *graph_file, int num_parts,
-                         int theta, bool mode, bool verbose, Graph &local_graph,
-                         GhostNodes &ghost_nodes) {
+template <typename G>
+Phase1Metrics phase1_impl(int mpi_rank, int mpi_size, const char *graph_file, int num_parts,
+                          int theta, bool mode, bool verbose, G &local_graph,
+                          GhostNodes &ghost_nodes) {
     require_single_rank(mpi_rank, mpi_size);
     dmolp_ctx *ctx = context_for(0);
     const auto t0 = std::chrono::high_resolution_clock::now();
     dmolp_csr csr{};
     check(dmolp_read_adjacency_text(graph_file, &csr), ctx, "dmolp_read_adjacency_text");
     const auto t1 = std::chrono::high_resolution_clock::now();
-    if (csr.nnz >= (1ll << 31) || csr.n >= (1ll << 31)) {
+    if (!fits(&local_graph, csr.n, csr.nnz)) {
         dmolp_csr_free(&csr);
-        throw std::runtime_error("graph too large for the int32 Graph layout; use the C ABI");
+        throw std::runtime_error("graph has 2^31 or more CSR entries: the int32 Graph cannot hold "
+                                 "it; call the GraphWide overload of run_phase1");
     }
     dmolp_graph *g = nullptr;
     int rc = dmolp_graph_upload(ctx, csr.n, csr.nnz, csr.off, 8, csr.col, &g);
@@ -83,10 +96,10 @@ Phase1Metrics run_phase1(int mpi_rank, int mpi_size, const char *graph_file, int
     if (rc != DMOLP_OK) { dmolp_csr_free(&csr); check(rc, ctx, "dmolp_phase1"); }
     const auto t2 = std::chrono::high_resolution_clock::now();
 
-    local_graph.clear();
+    local_graph = G{};
     ghost_nodes.clear();
-    local_graph.num_vertices = (int)csr.n;
-    local_graph.num_edges = (int)csr.nnz;
+    local_graph.num_vertices = (decltype(local_graph.num_vertices))csr.n;
+    local_graph.num_edges = (decltype(local_graph.num_edges))csr.nnz;
     local_graph.row_ptr.assign(csr.off, csr.off + csr.n + 1);
     local_graph.col_indices.assign(csr.col, csr.col + csr.nnz);
     local_graph.global_ids.resize((size_t)csr.n);
@@ -110,12 +123,15 @@ Phase1Metrics run_phase1(int mpi_rank, int mpi_size, const char *graph_file, int
     return out;
 }
 
-PartitioningMetrics run_phase2_ex(int mpi_rank, int mpi_size, int num_partitions,
-                                  Graph &local_graph, GhostNodes &ghost_nodes, int gpu_id,
-                                  int max_rounds, double epsilon) {
+template <typename G>
+PartitioningMetrics phase2_impl(int mpi_rank, int mpi_size, int num_partitions, G &local_graph,
+                                GhostNodes &ghost_nodes, int gpu_id, int max_rounds,
+                                double epsilon) {
     require_single_rank(mpi_rank, mpi_size);
     if (!ghost_nodes.global_ids.empty())
         throw std::runtime_error("a single rank has no ghost vertices");
+    if ((int64_t)local_graph.vertex_labels.size() != (int64_t)local_graph.num_vertices)
+        throw std::runtime_error("vertex_labels must hold num_vertices entries");
     dmolp_ctx *ctx = context_for(gpu_id);
     const auto t0 = std::chrono::high_resolution_clock::now();
     dmolp_graph *g = upload(ctx, local_graph);
@@ -139,8 +155,45 @@ PartitioningMetrics run_phase2_ex(int mpi_rank, int mpi_size, int num_partitions
     return out;
 }
 
+}  // namespace
+
+Phase1Metrics run_phase1(int mpi_rank, int mpi_size, const char *graph_file, int num_parts,
+                         int theta, bool mode, bool verbose, Graph &local_graph,
+                         GhostNodes &ghost_nodes) {
+    return phase1_impl(mpi_rank, mpi_size, graph_file, num_parts, theta, mode, verbose, local_graph,
+                       ghost_nodes);
+}
+
+Phase1Metrics run_phase1(int mpi_rank, int mpi_size, const char *graph_file, int num_parts,
+                         int theta, bool mode, bool verbose, GraphWide &local_graph,
+                         GhostNodes &ghost_nodes) {
+    return phase1_impl(mpi_rank, mpi_size, graph_file, num_parts, theta, mode, verbose, local_graph,
+                       ghost_nodes);
+}
+
+PartitioningMetrics run_phase2_ex(int mpi_rank, int mpi_size, int num_partitions,
+                                  Graph &local_graph, GhostNodes &ghost_nodes, int gpu_id,
+                                  int max_rounds, double epsilon) {
+    return phase2_impl(mpi_rank, mpi_size, num_partitions, local_graph, ghost_nodes, gpu_id,
+                       max_rounds, epsilon);
+}
+
+PartitioningMetrics run_phase2_ex(int mpi_rank, int mpi_size, int num_partitions,
+                                  GraphWide &local_graph, GhostNodes &ghost_nodes, int gpu_id,
+                                  int max_rounds, double epsilon) {
+    return phase2_impl(mpi_rank, mpi_size, num_partitions, local_graph, ghost_nodes, gpu_id,
+                       max_rounds, epsilon);
+}
+
+// The reference's constants (phase2.cpp:263-264): 500 rounds, epsilon 0.03.
 PartitioningMetrics run_phase2(int mpi_rank, int mpi_size, int num_partitions, Graph &local_graph,
                                GhostNodes &ghost_nodes, int gpu_id) {
+    return run_phase2_ex(mpi_rank, mpi_size, num_partitions, local_graph, ghost_nodes, gpu_id, 500,
+                         0.03);
+}
+
+PartitioningMetrics run_phase2(int mpi_rank, int mpi_size, int num_partitions,
+                               GraphWide &local_graph, GhostNodes &ghost_nodes, int gpu_id) {
     return run_phase2_ex(mpi_rank, mpi_size, num_partitions, local_graph, ghost_nodes, gpu_id, 500,
                          0.03);
 }

@@ -1,8 +1,13 @@
 // main.cpp -- the reference's command line (main.cpp:123-205) over the
 // drop-in run_phase1 / run_phase2:
 //   hpc_partitioning [--verbose] -m <bfs|random> <graph_file> <k> <theta>
-// One process on GPU 0.  Extra options: --rounds R and --epsilon E (the
-// reference fixes 500 and 0.03, phase2.cpp:263-264).
+// One process on GPU 0.  Output as the reference's main.cpp:280-283:
+// Phase1Metrics::print, then printComparisonReport.  Extra options:
+//   --rounds R, --epsilon E  the round limit and convergence epsilon (the
+//                            reference fixes 500 and 0.03, phase2.cpp:263-264)
+//   --labels-out FILE        write the final labels (int32, native order)
+//   --layout graph|wide      run on the reference's int32 Graph or on
+//                            GraphWide (the default; any size)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,6 +30,8 @@ struct Options {
     int theta = 0;
     int rounds = 500;
     double epsilon = 0.03;
+    const char *labels_out = nullptr;
+    bool int32_graph = false;
 };
 
 bool parse(int argc, char **argv, Options &o) {
@@ -51,6 +58,14 @@ bool parse(int argc, char **argv, Options &o) {
         }
         if (!std::strcmp(a, "--rounds") && i + 1 < argc) { o.rounds = std::atoi(argv[++i]); continue; }
         if (!std::strcmp(a, "--epsilon") && i + 1 < argc) { o.epsilon = std::atof(argv[++i]); continue; }
+        if (!std::strcmp(a, "--labels-out") && i + 1 < argc) { o.labels_out = argv[++i]; continue; }
+        if (!std::strcmp(a, "--layout") && i + 1 < argc) {
+            const char *l = argv[++i];
+            if (!std::strcmp(l, "graph")) o.int32_graph = true;
+            else if (!std::strcmp(l, "wide")) o.int32_graph = false;
+            else { std::cerr << "error: layout must be graph or wide: " << l << "\n"; return false; }
+            continue;
+        }
         if (a[0] == '-') { std::cerr << "error: unknown option: " << a << "\n"; return false; }
         pos.push_back(a);
     }
@@ -67,18 +82,26 @@ bool parse(int argc, char **argv, Options &o) {
     return true;
 }
 
-void comparison_report(const PartitioningMetrics &p1, const PartitioningMetrics &p2) {
-    auto pct = [](double a, double b) { return a != 0.0 ? (a - b) / a * 100.0 : 0.0; };
-    std::printf("\n=== Phase 1 vs Phase 2 ===\n");
-    std::printf("%-16s %14s %14s %12s\n", "metric", "phase 1", "phase 2", "improvement");
-    std::printf("%-16s %14d %14d %11.2f%%\n", "edge cut", p1.edge_cut, p2.edge_cut,
-                pct(p1.edge_cut, p2.edge_cut));
-    std::printf("%-16s %14.4f %14.4f %11.2f%%\n", "vertex balance", p1.vertex_balance,
-                p2.vertex_balance, pct(p1.vertex_balance, p2.vertex_balance));
-    std::printf("%-16s %14.4f %14.4f %11.2f%%\n", "edge balance", p1.edge_balance, p2.edge_balance,
-                pct(p1.edge_balance, p2.edge_balance));
-    std::printf("%-16s %14ld %14ld\n", "time (ms)", p1.loading_time_ms + p1.distribution_time_ms,
-                p2.loading_time_ms);
+template <typename G>
+int run(const Options &o) {
+    G g;
+    GhostNodes ghosts;
+    std::cout << "\n=== Phase 1: Initial Partitioning & Distribution ===\n";
+    Phase1Metrics m1 = run_phase1(0, 1, o.graph_file, o.num_partitions, o.theta, o.random_mode,
+                                  o.verbose, g, ghosts);
+    PartitioningMetrics p1(m1, o.num_partitions);
+    std::cout << "\n=== Phase 2: Label Propagation ===\n";
+    PartitioningMetrics p2 = run_phase2_ex(0, 1, o.num_partitions, g, ghosts, 0, o.rounds, o.epsilon);
+    m1.print();
+    printComparisonReport(p1, p2);
+    if (o.labels_out) {
+        FILE *fp = std::fopen(o.labels_out, "wb");
+        const size_t n = g.vertex_labels.size();
+        const bool ok = fp && std::fwrite(g.vertex_labels.data(), sizeof(int32_t), n, fp) == n;
+        if (fp) std::fclose(fp);
+        if (!ok) { std::cerr << "error: cannot write " << o.labels_out << "\n"; return 2; }
+    }
+    return 0;
 }
 
 }  // namespace
@@ -87,22 +110,12 @@ int main(int argc, char **argv) {
     Options o;
     if (!parse(argc, argv, o)) {
         std::cout << "Usage: hpc_partitioning [--verbose] -m <bfs|random> <graph_file> "
-                     "<num_partitions> <theta> [--rounds R] [--epsilon E]\n";
+                     "<num_partitions> <theta> [--rounds R] [--epsilon E] "
+                     "[--labels-out FILE] [--layout graph|wide]\n";
         return 1;
     }
     try {
-        Graph g;
-        GhostNodes ghosts;
-        std::cout << "\n=== Phase 1: Initial Partitioning ===\n";
-        Phase1Metrics m1 = run_phase1(0, 1, o.graph_file, o.num_partitions, o.theta, o.random_mode,
-                                      o.verbose, g, ghosts);
-        PartitioningMetrics p1(m1, o.num_partitions);
-        std::cout << "\n=== Phase 2: Label Propagation ===\n";
-        PartitioningMetrics p2 =
-            run_phase2_ex(0, 1, o.num_partitions, g, ghosts, 0, o.rounds, o.epsilon);
-        m1.print();
-        p2.print();
-        comparison_report(p1, p2);
+        return o.int32_graph ? run<Graph>(o) : run<GraphWide>(o);
     } catch (const std::exception &e) {
         std::cerr << "error: " << e.what() << "\n";
         return 2;

@@ -11,3 +11,15 @@ PartitioningMetrics run_phase2(int mpi_rank, int mpi_size, int num_partitions, G
 PartitioningMetrics run_phase2_ex(int mpi_rank, int mpi_size, int num_partitions,
                                   Graph &local_graph, GhostNodes &ghost_nodes, int gpu_id,
                                   int max_rounds, double epsilon);
+
+// The same on a GraphWide (int64 offsets, any size).
+PartitioningMetrics run_phase2(int mpi_rank, int mpi_size, int num_partitions,
+                               GraphWide &local_graph, GhostNodes &ghost_nodes, int gpu_id);
+PartitioningMetrics run_phase2_ex(int mpi_rank, int mpi_size, int num_partitions,
+                                  GraphWide &local_graph, GhostNodes &ghost_nodes, int gpu_id,
+                                  int max_rounds, double epsilon);
+
+// printComparisonReport (utils.cpp:337-394, declared in utils.hpp): the
+// phase-1 versus phase-2 table the reference prints at the end of a run,
+// line for line in the reference's format so its log parsers keep working.
+void printComparisonReport(const PartitioningMetrics &m1, const PartitioningMetrics &m2);

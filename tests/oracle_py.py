@@ -177,6 +177,18 @@ def phase1(off, col, k, theta) -> Phase1Out:
     return Phase1Out(labels, sd, vc, ec, m)
 
 
+def phase1_metrics(off, col, k, labels):
+    """ref_phase1_metrics: (vc, ec, P1Metrics) of a labelling."""
+    off, col = csr(off, col)
+    labels = np.ascontiguousarray(labels, np.int32)
+    vc = np.zeros(k, np.int64)
+    ec = np.zeros(k, np.int64)
+    m = P1Metrics()
+    _ok(lib().ref_phase1_metrics(len(off) - 1, _p(off, i64), _p(col, i32), k, _p(labels, i32),
+                                 _p(vc, i64), _p(ec, i64), C.byref(m)))
+    return vc, ec, m
+
+
 def stats(off, col, k, labels):
     off, col = csr(off, col)
     labels = np.ascontiguousarray(labels, np.int32)
