@@ -531,7 +531,7 @@ int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, 
         Ctl c{};
         const int32_t *d_done = &w.ctl.as<Ctl>()->done;
         for (int t = 1; t <= max_rounds; ++t) {
-            phase2_round_launch(ctx, set, &w.round_exec, &w.round_key, key);
+            phase2_round_launch(ctx, set, w.round, key);
             if (t < max_rounds && round_poll(ctx, d_done, t, false)) break;
         }
         DMOLP_HIP_CHECK(hipEventRecord(ctx->ev1, s));

@@ -159,7 +159,7 @@ void phase2_run_set(dmolp_ctx *ctx, std::vector<P2Item> &items, BatchState &bs) 
     const P2Set set = p2_set_build(ctx, ip, bs, &key);
     phase2_init_enqueue(ctx, set);
     for (int t = 1; t <= Rmax; ++t) {
-        phase2_round_launch(ctx, set, &bs.exec, &bs.exec_key, key);
+        phase2_round_launch(ctx, set, bs.round, key);
         if (t < Rmax && round_poll(ctx, set.remaining, t, true)) break;
     }
     p2_collect(ctx, ip);

@@ -148,18 +148,35 @@ struct P1SetDev {
     DevMem jobs, sel, done, remaining;
 };
 
+// A captured phase-2 round: the graph, its executable and the launch shape
+// (`key`) it was captured for.  The graph is kept as long as the executable.
+// Owners release it only after the stream has drained (graph free and
+// context teardown synchronise first; a re-capture waits for the stream).
+struct RoundGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t key = 0;
+    RoundGraph() = default;
+    RoundGraph(const RoundGraph &) = delete;
+    RoundGraph &operator=(const RoundGraph &) = delete;
+    void release() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key = 0;
+    }
+    ~RoundGraph() { release(); }
+};
+
 // Device records and the captured round of the last phase-2 job set.
 struct BatchState {
     DevMem jobs, sel, remaining, done;
     LaunchSet ls;
-    hipGraphExec_t exec = nullptr;
-    uint64_t exec_key = 0;
+    RoundGraph round;
     BatchState() = default;
     BatchState(const BatchState &) = delete;
     BatchState &operator=(const BatchState &) = delete;
-    ~BatchState() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-    }
 };
 
 }  // namespace dmolp

@@ -2237,7 +2237,7 @@ void phase2_round_table(dmolp_ctx *ctx, Phase2Work &w, int R) {
                                    ctx->stream));
     DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     w.rtab_rounds = R;
-    w.round_key = 0;  // the captured round reads rtab: capture again
+    w.round.key = 0;  // the captured round reads rtab: capture again
 }
 
 P2Job phase2_job(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w, int max_rounds, double epsilon,
@@ -2492,16 +2492,17 @@ uint64_t phase2_single_key(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
     return key;
 }
 
-void phase2_round_launch(dmolp_ctx *ctx, const P2Set &set, hipGraphExec_t *exec, uint64_t *exec_key,
-                         uint64_t key) {
+void phase2_round_launch(dmolp_ctx *ctx, const P2Set &set, RoundGraph &rg, uint64_t key) {
     if (ctx->profiling) {  // events between kernels: launch directly
         phase2_round_enqueue(ctx, set);
         return;
     }
-    if (!*exec || *exec_key != key) {
-        if (*exec) {
-            DMOLP_HIP_CHECK(hipGraphExecDestroy(*exec));
-            *exec = nullptr;
+    if (!rg.exec || rg.key != key) {
+        if (rg.exec || rg.graph) {
+            // Launches of the old executable may still be queued: let the
+            // stream drain before it is destroyed.
+            DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            rg.release();
         }
         hipGraph_t graph = nullptr;
         DMOLP_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -2513,12 +2514,11 @@ void phase2_round_launch(dmolp_ctx *ctx, const P2Set &set, hipGraphExec_t *exec,
             throw;
         }
         DMOLP_HIP_CHECK(hipStreamEndCapture(ctx->stream, &graph));
-        const hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        DMOLP_HIP_CHECK(e);
-        *exec_key = key;
+        rg.graph = graph;
+        DMOLP_HIP_CHECK(hipGraphInstantiate(&rg.exec, rg.graph, nullptr, nullptr, 0));
+        rg.key = key;
     }
-    DMOLP_HIP_CHECK(hipGraphLaunch(*exec, ctx->stream));
+    DMOLP_HIP_CHECK(hipGraphLaunch(rg.exec, ctx->stream));
 }
 
 // Test hook: the candidate + selection steps of round t from an explicit

@@ -128,16 +128,13 @@ struct Phase2Work {
     List B, D;          // boundary list (B_{t-1} read, then B_t written), Delta
     DevMem rtab;        // per-round constants: [0..R] 1 + log(t), [R+1..2R+1] exp(-t/5)
     int rtab_rounds = -1;
-    hipGraphExec_t round_exec = nullptr;  // one captured round
-    uint64_t round_key = 0;               // launch shape the capture was made for
+    RoundGraph round;  // one captured round of this graph as a job set of one
+    ~Phase2Work() { round.release(); }  // before any buffer its kernels read is freed
     // This graph as a job set of one (the plain path).
     DevMem job_dev, sel_dev, remaining;
     LaunchSet ls;
     P2Job job_host{};
     SelArgs sel_host{};
-    ~Phase2Work() {
-        if (round_exec) (void)hipGraphExecDestroy(round_exec);
-    }
     Recs bd;  // record counter of the round (the records themselves are dense, in rdeg)
     SelectWork sel;
     DevMem hub_hist, hub_done;
@@ -170,8 +167,7 @@ void phase2_init_enqueue(dmolp_ctx *ctx, const P2Set &set);
 void phase2_round_enqueue(dmolp_ctx *ctx, const P2Set &set);
 // Launch one round through a captured HIP graph (captured on first use,
 // cached in *exec under `key`).
-void phase2_round_launch(dmolp_ctx *ctx, const P2Set &set, hipGraphExec_t *exec, uint64_t *exec_key,
-                         uint64_t key);
+void phase2_round_launch(dmolp_ctx *ctx, const P2Set &set, RoundGraph &rg, uint64_t key);
 // Every job's done flag (d_out: device scratch of one int per job).
 void phase2_done_flags(dmolp_ctx *ctx, const P2Set &set, int32_t *d_out, std::vector<int32_t> &out);
 // Shape key of the single-graph set for phase2_round_launch.

@@ -16,7 +16,9 @@ box.  The oracle runs on the host's OpenMP threads; its results do not
 depend on the thread count.
 
 Run from the repository root:
-    python tests/golden/make_large_golden.py c2 [c3 c4 c5]
+    python tests/golden/make_large_golden.py [--csr-cache DIR] c2 [c3 c4 c5]
+--csr-cache keeps each generated plain graph as DIR/<workload>.{off,col}.npy
+and reuses it (the host generator takes minutes on the 2.4e9-entry graph).
 """
 from __future__ import annotations
 
@@ -42,13 +44,28 @@ def sha(*arrays):
     return h.hexdigest()
 
 
-def plain_case(name):
-    import oracle_py as O
+def plain_graph(cfg, cache):
     from dmolp_amd import api, workloads as W
+    if cache:
+        po = Path(cache) / f"{cfg['workload']}.off.npy"
+        pc = Path(cache) / f"{cfg['workload']}.col.npy"
+        if po.exists() and pc.exists():
+            return np.load(po, mmap_mode="r"), np.load(pc, mmap_mode="r")
+    off, col = api.gen_host(W.spec_for(cfg))
+    if cache:
+        Path(cache).mkdir(parents=True, exist_ok=True)
+        np.save(po, off)
+        np.save(pc, col)
+    return off, col
+
+
+def plain_case(name, cache=None):
+    import oracle_py as O
+    from dmolp_amd import workloads as W
     cfg = W.CONFIGS[name]
     k, R, eps = cfg["k"], cfg["rounds"], cfg["eps"]
     t0 = time.time()
-    off, col = api.gen_host(W.spec_for(cfg))
+    off, col = plain_graph(cfg, cache)
     print(f"[{name}] generated n={len(off) - 1} nnz={len(col)} in {time.time() - t0:.1f} s",
           flush=True)
     rec = {"workload": cfg["workload"], "generator": W.spec_key(W.spec_for(cfg)), "k": k,
@@ -119,8 +136,11 @@ def jobs_case(name):
 
 
 def main(argv):
+    cache = None
+    if argv[:1] == ["--csr-cache"]:
+        cache, argv = argv[1], argv[2:]
     for name in argv or ["c2"]:
-        rec = plain_case(name) if name in ("c1", "c2", "c3") else jobs_case(name)
+        rec = plain_case(name, cache) if name in ("c1", "c2", "c3") else jobs_case(name)
         (HERE / f"large_{name}.json").write_text(json.dumps(rec, indent=1) + "\n")
         print(f"[{name}] written", flush=True)
 
