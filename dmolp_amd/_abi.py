@@ -93,6 +93,9 @@ SIGNATURES = {
     "dmolp_csr_free": (None, [P(Csr)]),
     "dmolp_read_adjacency_text": (C.c_int, [C.c_char_p, P(Csr)]),
     "dmolp_write_adjacency_text": (C.c_int, [C.c_char_p, i64, P(i64), P(i32)]),
+    "dmolp_write_csr_binary": (C.c_int, [C.c_char_p, i64, P(i64), P(i32)]),
+    "dmolp_read_csr_binary": (C.c_int, [C.c_char_p, P(Csr)]),
+    "dmolp_load_adjacency_cached": (C.c_int, [C.c_char_p, P(Csr), P(i32)]),
     "dmolp_graph_upload": (C.c_int, [VP, i64, i64, VP, C.c_int, P(i32), P(VP)]),
     "dmolp_graph_generate": (C.c_int, [VP, P(GenSpec), P(VP)]),
     "dmolp_graph_info": (C.c_int, [VP, P(i64), P(i64)]),

@@ -74,6 +74,29 @@ def write_adjacency_text(path: str, off: np.ndarray, col: np.ndarray):
                                                  _p(off, C.c_int64), _p(col, C.c_int32)))
 
 
+def write_csr_binary(path: str, off: np.ndarray, col: np.ndarray):
+    off = np.ascontiguousarray(off, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    _check(_abi.lib().dmolp_write_csr_binary(str(path).encode(), len(off) - 1,
+                                             _p(off, C.c_int64), _p(col, C.c_int32)))
+
+
+def read_csr_binary(path: str):
+    csr = _abi.Csr()
+    _check(_abi.lib().dmolp_read_csr_binary(str(path).encode(), C.byref(csr)))
+    return _take_csr(csr)
+
+
+def load_adjacency_cached(path: str):
+    """(off, col, from_cache): the text file through its <path>.csr cache."""
+    csr = _abi.Csr()
+    hit = C.c_int32()
+    _check(_abi.lib().dmolp_load_adjacency_cached(str(path).encode(), C.byref(csr),
+                                                  C.byref(hit)))
+    off, col = _take_csr(csr)
+    return off, col, bool(hit.value)
+
+
 @dataclass
 class Phase1Result:
     labels: np.ndarray

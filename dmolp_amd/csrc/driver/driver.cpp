@@ -76,7 +76,8 @@ Phase1Metrics phase1_impl(int mpi_rank, int mpi_size, const char *graph_file, in
     dmolp_ctx *ctx = context_for(0);
     const auto t0 = std::chrono::high_resolution_clock::now();
     dmolp_csr csr{};
-    check(dmolp_read_adjacency_text(graph_file, &csr), ctx, "dmolp_read_adjacency_text");
+    int32_t cached = 0;
+    check(dmolp_load_adjacency_cached(graph_file, &csr, &cached), ctx, "dmolp_load_adjacency_cached");
     const auto t1 = std::chrono::high_resolution_clock::now();
     if (!fits(&local_graph, csr.n, csr.nnz)) {
         dmolp_csr_free(&csr);
@@ -118,7 +119,8 @@ Phase1Metrics phase1_impl(int mpi_rank, int mpi_size, const char *graph_file, in
     out.partition_vertex_counts.assign(vc.begin(), vc.end());
     out.partition_edge_counts.assign(ec.begin(), ec.end());
     if (verbose)
-        std::printf("[phase1] seeds found %d, fallback %d, rounds %lld\n", m.n_seeds_found,
+        std::printf("[phase1] graph %s (%s), seeds found %d, fallback %d, rounds %lld\n",
+                    graph_file, cached ? "binary cache" : "text", m.n_seeds_found,
                     m.n_seeds_fallback, (long long)m.rounds);
     return out;
 }

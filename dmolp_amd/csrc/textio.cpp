@@ -19,6 +19,8 @@
 // duplicate neighbours or self-loops, the reference's phase 1 takes degrees
 // from the raw line lengths (init.cpp:86-90) and keeps self-loops in its
 // CSR, whereas here every degree is the cleaned row's length.
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -120,4 +122,118 @@ extern "C" int dmolp_write_adjacency_text(const char *path, int64_t n, const int
         std::fwrite(line.data(), 1, line.size(), fp);
     }
     return std::fclose(fp) == 0 ? DMOLP_OK : DMOLP_E_ARG;
+}
+
+// ---------------------------------------------------------------- binary CSR cache
+
+namespace {
+
+constexpr char kCsrMagic[8] = {'D', 'M', 'O', 'L', 'P', 'C', 'S', 'R'};
+constexpr uint32_t kCsrVersion = 1;
+
+struct CsrHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t offset_bytes;
+    int64_t n;
+    int64_t nnz;
+};
+
+bool write_all(FILE *fp, const void *p, size_t bytes) {
+    return bytes == 0 || std::fwrite(p, 1, bytes, fp) == bytes;
+}
+
+bool read_all(FILE *fp, void *p, size_t bytes) {
+    return bytes == 0 || std::fread(p, 1, bytes, fp) == bytes;
+}
+
+}  // namespace
+
+extern "C" int dmolp_write_csr_binary(const char *path, int64_t n, const int64_t *off,
+                                      const int32_t *col) {
+    if (!path || n < 0 || !off || (off[n] > 0 && !col)) return DMOLP_E_ARG;
+    // Written to a temporary name and renamed, so a reader never sees a
+    // half-written cache.
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *fp = std::fopen(tmp.c_str(), "wb");
+    if (!fp) return DMOLP_E_ARG;
+    CsrHeader h{};
+    std::memcpy(h.magic, kCsrMagic, sizeof(h.magic));
+    h.version = kCsrVersion;
+    h.offset_bytes = 8;
+    h.n = n;
+    h.nnz = off[n];
+    bool ok = write_all(fp, &h, sizeof(h)) &&
+              write_all(fp, off, sizeof(int64_t) * (size_t)(n + 1)) &&
+              write_all(fp, col, sizeof(int32_t) * (size_t)h.nnz);
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return DMOLP_E_ARG;
+    }
+    return DMOLP_OK;
+}
+
+extern "C" int dmolp_read_csr_binary(const char *path, dmolp_csr *out) {
+    if (!path || !out) return DMOLP_E_ARG;
+    std::memset(out, 0, sizeof(*out));
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return DMOLP_E_ARG;
+    CsrHeader h{};
+    int rc = DMOLP_OK;
+    long long length = -1;
+    if (std::fseek(fp, 0, SEEK_END) == 0) length = std::ftell(fp);
+    std::rewind(fp);
+    if (!read_all(fp, &h, sizeof(h)) || std::memcmp(h.magic, kCsrMagic, sizeof(h.magic)) != 0 ||
+        h.version != kCsrVersion || h.offset_bytes != 8 || h.n < 0 || h.n > 0x7fffffffll ||
+        h.nnz < 0 || h.nnz > (1ll << 40) ||
+        (long long)(sizeof(h) + 8 * (uint64_t)(h.n + 1) + 4 * (uint64_t)h.nnz) != length) {
+        std::fclose(fp);
+        return DMOLP_E_GRAPH;
+    }
+    out->off = (int64_t *)std::malloc(sizeof(int64_t) * (size_t)(h.n + 1));
+    out->col = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)(h.nnz > 0 ? h.nnz : 1));
+    if (!out->off || !out->col) rc = DMOLP_E_NOMEM;
+    if (rc == DMOLP_OK && (!read_all(fp, out->off, sizeof(int64_t) * (size_t)(h.n + 1)) ||
+                           !read_all(fp, out->col, sizeof(int32_t) * (size_t)h.nnz)))
+        rc = DMOLP_E_GRAPH;
+    std::fclose(fp);
+    if (rc == DMOLP_OK) {
+        // Offsets must run from 0 to nnz without decreasing, columns stay in
+        // [0, n) (row order and self-loops are checked on upload).
+        bool good = out->off[0] == 0 && out->off[h.n] == h.nnz;
+        for (int64_t v = 0; good && v < h.n; ++v) good = out->off[v] <= out->off[v + 1];
+        for (int64_t e = 0; good && e < h.nnz; ++e) good = out->col[e] >= 0 && out->col[e] < h.n;
+        if (!good) rc = DMOLP_E_GRAPH;
+    }
+    if (rc != DMOLP_OK) {
+        std::free(out->off);
+        std::free(out->col);
+        std::memset(out, 0, sizeof(*out));
+        return rc;
+    }
+    out->n = h.n;
+    out->nnz = h.nnz;
+    return DMOLP_OK;
+}
+
+extern "C" int dmolp_load_adjacency_cached(const char *path, dmolp_csr *out, int32_t *from_cache) {
+    if (!path || !out) return DMOLP_E_ARG;
+    if (from_cache) *from_cache = 0;
+    const std::string cache = std::string(path) + ".csr";
+    struct stat st_text {}, st_cache {};
+    const bool have_text = ::stat(path, &st_text) == 0;
+    const bool have_cache = ::stat(cache.c_str(), &st_cache) == 0;
+    if (have_text && have_cache &&
+        (st_cache.st_mtim.tv_sec > st_text.st_mtim.tv_sec ||
+         (st_cache.st_mtim.tv_sec == st_text.st_mtim.tv_sec &&
+          st_cache.st_mtim.tv_nsec >= st_text.st_mtim.tv_nsec)) &&
+        dmolp_read_csr_binary(cache.c_str(), out) == DMOLP_OK) {
+        if (from_cache) *from_cache = 1;
+        return DMOLP_OK;
+    }
+    const int rc = dmolp_read_adjacency_text(path, out);
+    if (rc != DMOLP_OK) return rc;
+    (void)dmolp_write_csr_binary(cache.c_str(), out->n, out->off, out->col);
+    return DMOLP_OK;
 }

@@ -158,6 +158,20 @@ void dmolp_csr_free(dmolp_csr *csr);
 int dmolp_read_adjacency_text(const char *path, dmolp_csr *out);
 int dmolp_write_adjacency_text(const char *path, int64_t n, const int64_t *off, const int32_t *col);
 
+/* Binary CSR cache (no counterpart in the reference, whose loader parses the
+ * text file on every run, init.cpp:25-100).  Layout, native byte order: the
+ * 8 bytes "DMOLPCSR", uint32 version (1), uint32 offset width (8), int64 n,
+ * int64 nnz, then off[n + 1] (int64) and col[nnz] (int32).  The reader
+ * checks the header, the sizes against the file length and the offsets'
+ * bounds; it returns DMOLP_E_GRAPH for a file that fails them. */
+int dmolp_write_csr_binary(const char *path, int64_t n, const int64_t *off, const int32_t *col);
+int dmolp_read_csr_binary(const char *path, dmolp_csr *out);
+/* The text file through its cache: reads <path>.csr when that file exists,
+ * is at least as new as the text file and passes the reader's checks;
+ * otherwise parses the text and writes <path>.csr for the next run (a cache
+ * that cannot be written is skipped).  *from_cache (nullable) tells which. */
+int dmolp_load_adjacency_cached(const char *path, dmolp_csr *out, int32_t *from_cache);
+
 /* Make a CSR device-resident (replaces the per-round H2D copies of
  * gpu_lp_boundary.cu:456-471).  Validates offsets, sorted rows and the
  * absence of self-loops on the device. */

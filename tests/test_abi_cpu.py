@@ -189,6 +189,54 @@ def test_text_reader_rejects_ids_out_of_range(tmp_path, body):
     assert e.value.status == -9
 
 
+def test_binary_csr_round_trip_and_checks(tmp_path):
+    from dmolp_amd import api
+    off, col = random_graph(500, 3000, seed=8)
+    p = tmp_path / "g.csr"
+    api.write_csr_binary(p, off, col)
+    assert p.stat().st_size == 32 + 8 * len(off) + 4 * len(col)
+    off2, col2 = api.read_csr_binary(p)
+    assert (off2 == off).all() and (col2 == col).all()
+    raw = bytearray(p.read_bytes())
+    for bad in (raw[:-4],                                # truncated
+                raw[:3] + b"X" + raw[4:],                # magic
+                raw[:32 + 8] + (10 ** 9).to_bytes(8, "little") + raw[32 + 16:]):  # offset
+        q = tmp_path / "bad.csr"
+        q.write_bytes(bytes(bad))
+        with pytest.raises(api.DmolpError) as e:
+            api.read_csr_binary(q)
+        assert e.value.status == -9
+    col3 = col.copy()
+    col3[7] = len(off)  # a column outside [0, n)
+    api.write_csr_binary(tmp_path / "c.csr", off, col3)
+    with pytest.raises(api.DmolpError):
+        api.read_csr_binary(tmp_path / "c.csr")
+
+
+def test_cached_loader_writes_then_reads_the_cache(tmp_path):
+    import os
+    from dmolp_amd import api
+    off, col = random_graph(400, 2000, seed=9)
+    p = tmp_path / "g.txt"
+    api.write_adjacency_text(p, off, col)
+    o1, c1, hit1 = api.load_adjacency_cached(p)
+    assert not hit1 and (tmp_path / "g.txt.csr").exists()
+    o2, c2, hit2 = api.load_adjacency_cached(p)
+    assert hit2 and (o2 == off).all() and (c2 == col).all() and (o1 == off).all()
+    # A text file newer than its cache is parsed again (and re-cached).
+    off4, col4 = random_graph(300, 900, seed=10)
+    api.write_adjacency_text(p, off4, col4)
+    st = (tmp_path / "g.txt.csr").stat()
+    os.utime(p, ns=(st.st_atime_ns, st.st_mtime_ns + 10 ** 9))
+    o3, c3, hit3 = api.load_adjacency_cached(p)
+    assert not hit3 and (o3 == off4).all() and (c3 == col4).all()
+    # A damaged cache falls back to the text file.
+    (tmp_path / "g.txt.csr").write_bytes(b"garbage")
+    os.utime(p, ns=(st.st_atime_ns, st.st_mtime_ns - 10 ** 9))
+    o5, c5, hit5 = api.load_adjacency_cached(p)
+    assert not hit5 and (o5 == off4).all()
+
+
 def test_cli_rejects_bad_arguments_before_touching_the_device():
     exe = ROOT / "dmolp_amd" / "hpc_partitioning"
     if not exe.exists():

@@ -21,10 +21,12 @@ pytestmark = pytest.mark.gpu
 CLI = ROOT / "dmolp_amd" / "hpc_partitioning"
 
 
-def run_cli(tmp_path, graph, mode, k, theta, rounds, eps, layout):
+def run_cli(tmp_path, graph, mode, k, theta, rounds, eps, layout, verbose=False):
     out = tmp_path / f"labels-{mode}-{layout}.bin"
     cmd = [str(CLI), "-m", mode, str(graph), str(k), str(theta), "--rounds", str(rounds),
            "--epsilon", str(eps), "--labels-out", str(out), "--layout", layout]
+    if verbose:
+        cmd.insert(1, "--verbose")
     print("[dropin]", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -77,6 +79,21 @@ def test_cli_random_mode_matches_oracle(tmp_path, graph_file, layout):
     assert int(field(text, r"^Initial edge cut: (\d+)")) == m.initial_edge_cut
     assert field(text, r"^Partition vertex counts: ([\d ]+)").split() == [str(x) for x in vc]
     assert int(field(text, r"Phase 2 \(최종\):\s+(\d+)")) == w2.m.edge_cut
+
+
+def test_cli_second_run_reads_the_binary_cache(tmp_path):
+    """The first run parses the text file and leaves <file>.csr; the next
+    run loads that cache and gives the same partition."""
+    off, col = api.gen_host(api.rmat_spec(11, 8, seed=33))
+    p = tmp_path / "rmat11.txt"
+    api.write_adjacency_text(p, off, col)
+    t1, l1 = run_cli(tmp_path, p, "bfs", 4, 100, 8, 0.03, "wide", verbose=True)
+    assert "(text)" in t1 and (tmp_path / "rmat11.txt.csr").exists()
+    t2, l2 = run_cli(tmp_path, p, "bfs", 4, 100, 8, 0.03, "wide", verbose=True)
+    assert "(binary cache)" in t2
+    assert np.array_equal(l1, l2)
+    w1 = O.phase1(off, col, 4, 100)
+    assert np.array_equal(l2, O.phase2(off, col, 4, w1.labels, 8, 0.03).labels)
 
 
 def test_phase1_random_abi_labels(ctx, graph_file):
