@@ -91,6 +91,13 @@ SIGNATURES = {
     "dmolp_ctx_set_tuning": (C.c_int, [VP, P(Tuning)]),
     "dmolp_gen_host": (C.c_int, [P(GenSpec), P(Csr)]),
     "dmolp_csr_free": (None, [P(Csr)]),
+    "dmolp_shares_create": (C.c_int, [VP, VP, i32, P(VP)]),
+    "dmolp_shares_destroy": (C.c_int, [VP, VP]),
+    "dmolp_shares_info": (C.c_int, [VP, i32, P(i64), P(i64), P(i64)]),
+    "dmolp_shares_download": (C.c_int, [VP, VP, i32, P(i64), P(i32), P(i32), P(i32)]),
+    "dmolp_halo_refresh": (C.c_int, [VP, VP, i32, P(i32), P(i32), i64, P(i64)]),
+    "dmolp_phase2_shares": (C.c_int, [VP, VP, i32, i32, f64, i32, P(i32), P(i64), P(i64),
+                                      P(Phase2Metrics), P(Trace), P(i64)]),
     "dmolp_read_adjacency_text": (C.c_int, [C.c_char_p, P(Csr)]),
     "dmolp_write_adjacency_text": (C.c_int, [C.c_char_p, i64, P(i64), P(i32)]),
     "dmolp_write_csr_binary": (C.c_int, [C.c_char_p, i64, P(i64), P(i32)]),

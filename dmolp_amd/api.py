@@ -450,3 +450,75 @@ class Graph:
                                          _p(b, C.c_int32), len(b), _p(pen, C.c_double), k,
                                          _p(out, C.c_int32)), self.ctx.handle)
         return out
+
+
+class Shares:
+    """P ranks' shares of a graph with on-device ghost labels (the halo);
+    built from the graph's resident labels."""
+
+    def __init__(self, g: "Graph", nranks: int):
+        self.g = g
+        self.ctx = g.ctx
+        self.P = nranks
+        self._h = C.c_void_p()
+        _check(_abi.lib().dmolp_shares_create(self.ctx.handle, g.handle, nranks,
+                                              C.byref(self._h)), self.ctx.handle)
+
+    def close(self):
+        if self._h:
+            _abi.lib().dmolp_shares_destroy(self.ctx.handle, self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.ctx.handle:
+                self.close()
+        except Exception:
+            pass
+
+    def info(self, r):
+        nl, ng, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(_abi.lib().dmolp_shares_info(self._h, r, C.byref(nl), C.byref(ng), C.byref(nnz)))
+        return nl.value, ng.value, nnz.value
+
+    def download(self, r):
+        """(off, col in unified ids, ghost global ids in slot order, labels
+        of the owned vertices then of the ghosts) of share r."""
+        nl, ng, nnz = self.info(r)
+        off = np.zeros(nl + 1, np.int64)
+        col = np.zeros(max(nnz, 1), np.int32)
+        gg = np.zeros(max(ng, 1), np.int32)
+        lab = np.zeros(max(nl + ng, 1), np.int32)
+        _check(_abi.lib().dmolp_shares_download(self.ctx.handle, self._h, r, _p(off, C.c_int64),
+                                                _p(col, C.c_int32), _p(gg, C.c_int32),
+                                                _p(lab, C.c_int32)), self.ctx.handle)
+        return off, col[:nnz], gg[:ng], lab[:nl + ng]
+
+    def refresh(self, r, gid, label) -> int:
+        gid = np.ascontiguousarray(gid, np.int32)
+        label = np.ascontiguousarray(label, np.int32)
+        u = C.c_int64()
+        _check(_abi.lib().dmolp_halo_refresh(self.ctx.handle, self._h, r, _p(gid, C.c_int32),
+                                             _p(label, C.c_int32), len(gid), C.byref(u)),
+               self.ctx.handle)
+        return u.value
+
+    def phase2(self, k, max_rounds, epsilon, scope) -> Phase2Result:
+        labels = np.zeros(self.g.n, np.int32)
+        vc = np.zeros(k, np.int64)
+        ic = np.zeros(k, np.int64)
+        m = _abi.Phase2Metrics()
+        tr = _abi.Trace()
+        arrays = {}
+        for name, _ in _abi.Trace._fields_:
+            a = np.full(max(max_rounds, 1), -7, np.int64)
+            arrays[name] = a
+            setattr(tr, name, _p(a, C.c_int64))
+        gu = np.full(max(max_rounds, 1), -7, np.int64)
+        _check(_abi.lib().dmolp_phase2_shares(self.ctx.handle, self._h, k, max_rounds, epsilon,
+                                              scope, _p(labels, C.c_int32), _p(vc, C.c_int64),
+                                              _p(ic, C.c_int64), C.byref(m), C.byref(tr),
+                                              _p(gu, C.c_int64)), self.ctx.handle)
+        trace = {name: a[:m.rounds].copy() for name, a in arrays.items()}
+        trace["ghost_updates"] = gu[:m.rounds].copy()
+        return Phase2Result(labels, vc, ic, m, trace)

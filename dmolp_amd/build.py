@@ -28,6 +28,7 @@ SOURCES = [
     "api.hip",
     "batch.hip",
     "server.hip",
+    "halo.hip",
     "gen_host.cpp",
     "textio.cpp",
     "driver/driver.cpp",

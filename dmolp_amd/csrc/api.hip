@@ -16,6 +16,7 @@
 #include "phase1.hpp"
 #include "phase2.hpp"
 #include "batch.hpp"
+#include "halo.hpp"
 #include "rows.hpp"
 
 namespace dmolp {
@@ -729,6 +730,82 @@ int dmolp_server_take(dmolp_server *srv, int64_t ticket, dmolp_served *out, int3
     int ready = 0;
     const int rc = guarded(nullptr, [&] { ready = server_take(srv, ticket, out, labels_out); });
     return rc != DMOLP_OK ? rc : ready;
+}
+
+// ---------------------------------------------------------------- rank shares
+
+int dmolp_shares_create(dmolp_ctx *ctx, dmolp_graph *g, int32_t P, dmolp_shares **out) {
+    if (!ctx || !g || !out) return fail(ctx, DMOLP_E_ARG, "null argument");
+    *out = nullptr;
+    return guarded(ctx, [&] { *out = shares_create(ctx, g, P); });
+}
+
+int dmolp_shares_destroy(dmolp_ctx *ctx, dmolp_shares *s) {
+    if (!s) return DMOLP_OK;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    delete s;
+    return DMOLP_OK;
+}
+
+int dmolp_shares_info(const dmolp_shares *s, int32_t r, int64_t *n_local, int64_t *n_ghost,
+                      int64_t *nnz) {
+    if (!s || r < 0 || r >= s->P) return DMOLP_E_ARG;
+    const Share &x = s->sh[(size_t)r];
+    if (n_local) *n_local = x.nl;
+    if (n_ghost) *n_ghost = x.ng;
+    if (nnz) *nnz = x.nnz;
+    return DMOLP_OK;
+}
+
+int dmolp_shares_download(dmolp_ctx *ctx, const dmolp_shares *s, int32_t r, int64_t *off,
+                          int32_t *col, int32_t *ghost_gid, int32_t *labels) {
+    if (!ctx || !s || r < 0 || r >= s->P) return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    return guarded(ctx, [&] {
+        const Share &x = s->sh[(size_t)r];
+        hipStream_t st = ctx->stream;
+        if (off)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(off, x.off.p, sizeof(int64_t) * (size_t)(x.nl + 1),
+                                           hipMemcpyDeviceToHost, st));
+        if (col && x.nnz > 0)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(col, x.col.p, sizeof(int32_t) * (size_t)x.nnz,
+                                           hipMemcpyDeviceToHost, st));
+        if (ghost_gid && x.ng > 0)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(ghost_gid, x.ghost_gid.p, sizeof(int32_t) * (size_t)x.ng,
+                                           hipMemcpyDeviceToHost, st));
+        if (labels && x.nl + x.ng > 0)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(labels, x.lab.p, sizeof(int32_t) * (size_t)(x.nl + x.ng),
+                                           hipMemcpyDeviceToHost, st));
+        DMOLP_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int dmolp_halo_refresh(dmolp_ctx *ctx, dmolp_shares *s, int32_t r, const int32_t *gid,
+                       const int32_t *label, int64_t count, int64_t *updated) {
+    if (!ctx || !s || r < 0 || r >= s->P || count < 0 || (count > 0 && (!gid || !label)))
+        return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    return guarded(ctx, [&] {
+        DevMem dg(sizeof(int32_t) * (size_t)(count > 0 ? count : 1)),
+            dl(sizeof(int32_t) * (size_t)(count > 0 ? count : 1));
+        if (count > 0) {
+            DMOLP_HIP_CHECK(hipMemcpyAsync(dg.p, gid, sizeof(int32_t) * (size_t)count,
+                                           hipMemcpyHostToDevice, ctx->stream));
+            DMOLP_HIP_CHECK(hipMemcpyAsync(dl.p, label, sizeof(int32_t) * (size_t)count,
+                                           hipMemcpyHostToDevice, ctx->stream));
+        }
+        const int64_t u = halo_refresh(ctx, s, r, dg.as<int32_t>(), dl.as<int32_t>(), count);
+        if (updated) *updated = u;
+    });
+}
+
+int dmolp_phase2_shares(dmolp_ctx *ctx, dmolp_shares *s, int32_t k, int32_t max_rounds,
+                        double epsilon, int32_t scope, int32_t *labels_out, int64_t *vc,
+                        int64_t *ic, dmolp_phase2_metrics *m, dmolp_trace *trace,
+                        int64_t *ghost_updates) {
+    if (!ctx || !s) return fail(ctx, DMOLP_E_ARG, "null argument");
+    return guarded(ctx, [&] {
+        shares_phase2(ctx, s, k, max_rounds, epsilon, scope, labels_out, vc, ic, m, trace,
+                      ghost_updates);
+    });
 }
 
 }  // extern "C"

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <vector>
 
+#include "lp_rule.hpp"
 #include "phase2.hpp"
 #include "rows.hpp"
 
@@ -996,26 +997,7 @@ __global__ __launch_bounds__(kThreads) void k_cand_heavy(const P2Job *JS, const 
 }
 
 // ---------------------------------------------------------------- LP decision
-
-// gpu_lp_boundary.cu:388-421 for one vertex: scores h*(1+pen) over labels
-// below E; among equal best scores the smallest 5-bit bit-reversed label
-// wins (the order the 32-lane shuffle-down tree yields); the winner must
-// beat the raw own-label count.
-template <class Count>
-__device__ __forceinline__ int32_t lp_decide_by(Count count, int E, const double *pen, int32_t own) {
-    const double base = (own >= 0 && own < E) ? (double)count(own) : -1e300;
-    int best_l = 0;
-    double best_s = 0.0;
-    for (int l = 0; l < E; ++l) {
-        double s = (double)count(l);
-        if (s > 0.0) s *= (1.0 + pen[l]);
-        if (l == 0 || s > best_s || (s == best_s && bitrev5(l) < bitrev5(best_l))) {
-            best_l = l;
-            best_s = s;
-        }
-    }
-    return (best_s > base) ? best_l : own;
-}
+// (lp_decide_by, the rule of gpu_lp_boundary.cu:388-421: lp_rule.hpp)
 
 __device__ __forceinline__ int32_t lp_decide(const int32_t *h, int stride, int E, const double *pen,
                                              int32_t own) {

@@ -290,6 +290,47 @@ int dmolp_select_boundary(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels,
 int dmolp_lp_round(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels, const int32_t *b,
                    int64_t nb, const double *pen, int32_t k, int32_t *labels_out);
 
+/* ---------------- rank shares with an on-device halo ---------------- */
+
+/* P ranks' shares of one graph on one GPU (P in [1, 64]).  Rank r owns
+ * the vertices v with v mod P == r (partition.cpp:55-57); its share holds
+ * their rows with neighbours as local ids (v / P) or ghost ids nl + slot,
+ * slots in order of first appearance over the rows (partition.cpp:182-262),
+ * and its ghosts' labels fetched from their owners (partition.cpp:264-318),
+ * all in device memory.  Built from g's resident labels. */
+typedef struct dmolp_shares dmolp_shares;
+int dmolp_shares_create(dmolp_ctx *ctx, dmolp_graph *g, int32_t P, dmolp_shares **out);
+int dmolp_shares_destroy(dmolp_ctx *ctx, dmolp_shares *s);
+/* Owned rows, ghosts and entries of share r. */
+int dmolp_shares_info(const dmolp_shares *s, int32_t r, int64_t *n_local, int64_t *n_ghost,
+                      int64_t *nnz);
+/* Share r's layout and halo (nullable outputs): off (n_local + 1), col
+ * (nnz, unified ids), ghost_gid (n_ghost, slot order), labels (n_local +
+ * n_ghost: owned, then ghosts). */
+int dmolp_shares_download(dmolp_ctx *ctx, const dmolp_shares *s, int32_t r, int64_t *off,
+                          int32_t *col, int32_t *ghost_gid, int32_t *labels);
+/* The halo refresh of share r (phase2.cpp:368-376): every Delta{gid, label}
+ * (host arrays of count entries) whose vertex is one of r's ghosts rewrites
+ * that ghost's label on the device; *updated (nullable) receives how many. */
+int dmolp_halo_refresh(dmolp_ctx *ctx, dmolp_shares *s, int32_t r, const int32_t *gid,
+                       const int32_t *label, int64_t count, int64_t *updated);
+/* Phase 2 (phase2.cpp:256-449) run rank by rank on the shares: partial
+ * counts summed (the Allreduce of phase2.cpp:100-117, 229-236), each
+ * share's selection and decisions on its owned rows over its unified
+ * labels, and every round the shares' deltas concatenated in rank order
+ * (the Allgatherv of phase2.cpp:122-154) and applied to every share's
+ * ghost labels.  scope 0: each rank selects over its own boundary vertices
+ * (gpu_lp_boundary.cu:274-302 on the rank's vectors); scope 1: selection
+ * over all ranks' boundary vertices (the run then equals dmolp_phase2).
+ * labels_out (n, nullable); the final labels also become g's resident
+ * labels.  ghost_updates (max_rounds, nullable): ghost labels rewritten by
+ * each round's refresh, summed over the shares.  The trace's candidate and
+ * light-part columns are -1. */
+int dmolp_phase2_shares(dmolp_ctx *ctx, dmolp_shares *s, int32_t k, int32_t max_rounds,
+                        double epsilon, int32_t scope, int32_t *labels_out, int64_t *vc,
+                        int64_t *ic, dmolp_phase2_metrics *m, dmolp_trace *trace,
+                        int64_t *ghost_updates);
+
 /* ---------------- batched and served modes ---------------- */
 
 /* One job of a batch: a graph, its parameters, and where results go. */

@@ -786,4 +786,328 @@ int ref_phase2(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
     return REF_OK;
 }
 
+
+// ---- Phase 2 as P ranks (the reference's multi-rank path at one process) ----
+
+namespace {
+
+// One rank's share: owned rows in ascending global id (partition.cpp:193-196),
+// columns as local ids for owned neighbours and nl + slot for ghosts, slots
+// given in order of first appearance over the rows (partition.cpp:225-257).
+struct RankShare {
+    int64_t nl = 0;
+    std::vector<int32_t> gid;        // local id -> global id
+    std::vector<int64_t> off;        // nl + 1
+    std::vector<int32_t> col;        // unified ids
+    std::vector<int32_t> ghost_gid;  // slot -> global id
+    std::unordered_map<int32_t, int32_t> slot_of;  // ghost global id -> slot
+    std::vector<int32_t> lab;        // unified labels: owned, then ghosts
+    std::vector<int32_t> B;          // this rank's boundary (local ids)
+    std::vector<uint8_t> ghost_in_b; // ghosts in some rank's boundary (global scope)
+};
+
+inline int32_t owner_of(int64_t v, int32_t P) { return (int32_t)(v % P); }
+
+void build_share(int64_t n, const int64_t *off, const int32_t *col, const int32_t *labels,
+                 int32_t P, int32_t r, RankShare &s) {
+    for (int64_t v = r; v < n; v += P) s.gid.push_back((int32_t)v);
+    s.nl = (int64_t)s.gid.size();
+    s.off.assign(1, 0);
+    for (int64_t i = 0; i < s.nl; ++i) {
+        const int32_t v = s.gid[(size_t)i];
+        for (int64_t e = off[v]; e < off[v + 1]; ++e) {
+            const int32_t u = col[e];
+            if (owner_of(u, P) == r) {
+                s.col.push_back((int32_t)(u / P));  // rank of u among the owned ids
+            } else {
+                auto it = s.slot_of.find(u);
+                int32_t slot;
+                if (it == s.slot_of.end()) {
+                    slot = (int32_t)s.ghost_gid.size();
+                    s.slot_of.emplace(u, slot);
+                    s.ghost_gid.push_back(u);
+                } else {
+                    slot = it->second;
+                }
+                s.col.push_back((int32_t)(s.nl + slot));
+            }
+        }
+        s.off.push_back((int64_t)s.col.size());
+    }
+    // Owned labels, then each ghost's label as its owner reports it
+    // (partition.cpp:264-318).
+    s.lab.resize((size_t)(s.nl + (int64_t)s.ghost_gid.size()));
+    for (int64_t i = 0; i < s.nl; ++i) s.lab[(size_t)i] = labels[s.gid[(size_t)i]];
+    for (size_t g = 0; g < s.ghost_gid.size(); ++g) s.lab[(size_t)s.nl + g] = labels[s.ghost_gid[g]];
+    s.ghost_in_b.assign(s.ghost_gid.size(), 0);
+}
+
+// Partial counts of one rank (phase2.cpp:55-98, 214-236 before the
+// Allreduce): owned labels, intra entries and differing entries of its rows.
+int share_counts(const RankShare &s, int32_t k, std::vector<int64_t> &vc, std::vector<int64_t> &ic,
+                 int64_t &cut2) {
+    for (int64_t i = 0; i < s.nl; ++i) {
+        const int32_t li = s.lab[(size_t)i];
+        const bool ok = li >= 0 && li < k;
+        if (ok) ++vc[(size_t)li];
+        for (int64_t e = s.off[(size_t)i]; e < s.off[(size_t)i + 1]; ++e) {
+            const int32_t lj = s.lab[(size_t)s.col[(size_t)e]];
+            if (ok && lj == li) ++ic[(size_t)li];
+            if (li == -1 || lj == -1) return REF_E_INVALID_LABEL;
+            if (li != lj) ++cut2;
+        }
+    }
+    return REF_OK;
+}
+
+// The Allreduce of every rank's partial counts.
+int global_counts(const std::vector<RankShare> &R, int32_t k, std::vector<int64_t> &vc,
+                  std::vector<int64_t> &ic, int64_t &cut) {
+    std::fill(vc.begin(), vc.end(), 0);
+    std::fill(ic.begin(), ic.end(), 0);
+    int64_t cut2 = 0;
+    for (const RankShare &s : R) {
+        const int rc = share_counts(s, k, vc, ic, cut2);
+        if (rc != REF_OK) return rc;
+    }
+    if (cut2 & 1) return REF_E_ODD_CUT;
+    for (auto &x : ic) x /= 2;  // phase2.cpp:110
+    cut = cut2 >> 1;
+    return REF_OK;
+}
+
+// One boundary vertex of a rank: local id, label, degree, global id.
+struct BdRec {
+    int32_t i, label;
+    int64_t deg;
+    int32_t gid;
+};
+
+}  // namespace
+
+// phase2.cpp:256-449 run by P ranks, each on its share of the graph
+// (owner(v) = v mod P, partition.cpp:55-57), with ghost labels
+// (partition.cpp:182-318) refreshed every round from the allgathered deltas
+// (phase2.cpp:122-154, 349-381).  Statistics and cut are the ranks' partial
+// counts summed (the Allreduce of phase2.cpp:100-117, 229-236).
+//
+// Selection scope 0 follows the reference rank by rank: candidates are the
+// rank's previous boundary and its neighbours, of which only owned vertices
+// are tested (gpu_lp_boundary.cu:251-252 skips ids past the owned rows), and
+// each label's ceil(ratio * c) is taken over the rank's own boundary vertices
+// (gpu_lp_boundary.cu:274-302 on the rank's vectors).  The boundary test
+// reads a ghost neighbour's label from the ghost labels (the reference
+// indexes its owned-label vector with the ghost id there, out of bounds,
+// gpu_lp_boundary.cu:259-260; the LP kernel reads the same ghost labels).
+// Scope 1 selects over all ranks together (candidates from every rank's
+// previous boundary, each label's take over all boundary vertices, ties by
+// global id), so the run equals the one-rank run for any P.
+//
+// labels: global, in/out.  ghost_updates (nullable, max_rounds entries):
+// ghost labels rewritten by each round's refresh, summed over ranks.
+int ref_phase2_ranks(int64_t n, const int64_t *off, const int32_t *col, int32_t k, int32_t P,
+                     int32_t scope, int32_t max_rounds, double epsilon, int32_t *labels,
+                     int64_t *vc_out, int64_t *ic_out, ref_p2_metrics *m, ref_p2_trace *tr,
+                     int64_t *ghost_updates) {
+    if (n <= 0 || k <= 0 || P <= 0 || P > n || (scope != 0 && scope != 1) || max_rounds < 0 ||
+        !labels || !m)
+        return REF_E_ARG;
+    std::memset(m, 0, sizeof(*m));
+    for (int64_t v = 0; v < n; ++v)
+        if (labels[v] < 0 || labels[v] >= k) return REF_E_INVALID_LABEL;
+    std::vector<RankShare> R((size_t)P);
+    for (int32_t r = 0; r < P; ++r) build_share(n, off, col, labels, P, r, R[(size_t)r]);
+    std::vector<int64_t> vc((size_t)k), ic((size_t)k);
+    int64_t prev_cut = 0;
+    int rc = global_counts(R, k, vc, ic, prev_cut);
+    if (rc != REF_OK) return rc;
+    m->initial_cut = prev_cut;
+    for (RankShare &s : R) {  // phase2.cpp:285-287: every owned vertex
+        s.B.resize((size_t)s.nl);
+        for (int64_t i = 0; i < s.nl; ++i) s.B[(size_t)i] = (int32_t)i;
+        std::fill(s.ghost_in_b.begin(), s.ghost_in_b.end(), 1);
+    }
+    std::vector<double> pen((size_t)k), re((size_t)k);
+    int cc = 0;
+    int32_t rounds = 0, reason = 0;
+    int64_t lp_edges = 0;
+    for (int32_t t = 1; t <= max_rounds; ++t) {
+        rounds = t;
+        ref_penalties(k, vc.data(), ic.data(), t, pen.data(), re.data());
+        const double ratio = exp((-t) / (5.0));
+        // Boundary records of every rank (candidates, then the test).
+        std::vector<std::vector<BdRec>> bd((size_t)P);
+        int64_t nprev = 0, dprev = 0;
+        for (int32_t r = 0; r < P; ++r) {
+            RankShare &s = R[(size_t)r];
+            std::vector<uint8_t> cand((size_t)s.nl, 0), in_b((size_t)s.nl, 0);
+            for (int32_t i : s.B) {
+                in_b[(size_t)i] = 1;
+                ++nprev;
+                dprev += s.off[(size_t)i + 1] - s.off[(size_t)i];
+            }
+            for (int64_t i = 0; i < s.nl; ++i) {
+                bool c = in_b[(size_t)i] != 0;
+                for (int64_t e = s.off[(size_t)i]; !c && e < s.off[(size_t)i + 1]; ++e) {
+                    const int32_t j = s.col[(size_t)e];
+                    c = j < s.nl ? in_b[(size_t)j] != 0
+                                 : (scope == 1 && s.ghost_in_b[(size_t)(j - s.nl)] != 0);
+                }
+                cand[(size_t)i] = c;
+            }
+            for (int64_t i = 0; i < s.nl; ++i) {
+                if (!cand[(size_t)i]) continue;
+                const int32_t li = s.lab[(size_t)i];
+                bool b = false;
+                for (int64_t e = s.off[(size_t)i]; !b && e < s.off[(size_t)i + 1]; ++e)
+                    b = s.lab[(size_t)s.col[(size_t)e]] != li;
+                if (b)
+                    bd[(size_t)r].push_back({(int32_t)i, li, s.off[(size_t)i + 1] - s.off[(size_t)i],
+                                             s.gid[(size_t)i]});
+            }
+        }
+        // Per label, the first ceil(ratio * size) by (degree desc if RE > 1
+        // else asc, id): per rank (scope 0) or over all ranks (scope 1).
+        auto take_groups = [&](std::vector<BdRec> &recs, std::vector<uint8_t> &pick) {
+            std::vector<std::vector<size_t>> groups((size_t)k);
+            for (size_t x = 0; x < recs.size(); ++x) groups[(size_t)recs[x].label].push_back(x);
+            pick.assign(recs.size(), 0);
+            for (int32_t p = 0; p < k; ++p) {
+                auto &g = groups[(size_t)p];
+                if (g.empty()) continue;
+                const bool desc = re[p] > 1.0;
+                std::sort(g.begin(), g.end(), [&](size_t a, size_t b) {
+                    const BdRec &x = recs[a], &y = recs[b];
+                    if (x.deg != y.deg) return desc ? x.deg > y.deg : x.deg < y.deg;
+                    return x.gid < y.gid;
+                });
+                int take = static_cast<int>(std::ceil(ratio * static_cast<double>(g.size())));
+                if (take > (int)g.size()) take = (int)g.size();
+                for (int q = 0; q < take; ++q) pick[g[(size_t)q]] = 1;
+            }
+        };
+        int64_t nbd = 0, nb = 0, bdeg = 0;
+        if (scope == 0) {
+            for (int32_t r = 0; r < P; ++r) {
+                std::vector<uint8_t> pick;
+                take_groups(bd[(size_t)r], pick);
+                RankShare &s = R[(size_t)r];
+                s.B.clear();
+                for (size_t x = 0; x < pick.size(); ++x)
+                    if (pick[x]) s.B.push_back(bd[(size_t)r][x].i);
+                nbd += (int64_t)bd[(size_t)r].size();
+            }
+        } else {
+            std::vector<BdRec> all;
+            std::vector<int32_t> rank_of;
+            for (int32_t r = 0; r < P; ++r)
+                for (const BdRec &x : bd[(size_t)r]) { all.push_back(x); rank_of.push_back(r); }
+            std::vector<uint8_t> pick;
+            take_groups(all, pick);
+            for (RankShare &s : R) s.B.clear();
+            for (size_t x = 0; x < pick.size(); ++x)
+                if (pick[x]) R[(size_t)rank_of[x]].B.push_back(all[x].i);
+            for (RankShare &s : R) std::sort(s.B.begin(), s.B.end());
+            nbd = (int64_t)all.size();
+        }
+        for (const RankShare &s : R) {
+            nb += (int64_t)s.B.size();
+            for (int32_t i : s.B) bdeg += s.off[(size_t)i + 1] - s.off[(size_t)i];
+        }
+        lp_edges += bdeg;
+        if (tr) {
+            const int q = t - 1;
+            if (tr->prev_count) tr->prev_count[q] = nprev;
+            if (tr->prev_deg) tr->prev_deg[q] = dprev;
+            if (tr->cand_count) tr->cand_count[q] = -1;
+            if (tr->scan_len) tr->scan_len[q] = -1;
+            if (tr->bd_count) tr->bd_count[q] = nbd;
+            if (tr->b_count) tr->b_count[q] = nb;
+            if (tr->b_deg) tr->b_deg[q] = bdeg;
+            if (tr->chg_count) tr->chg_count[q] = 0;
+            if (tr->chg_deg) tr->chg_deg[q] = 0;
+            if (tr->cut) tr->cut[q] = -1;
+        }
+        if (ghost_updates) ghost_updates[t - 1] = 0;
+        if (nb == 0) { reason = 1; break; }  // every rank's boundary is empty
+        // Each rank decides on its boundary from its unified labels (Jacobi),
+        // then builds its delta list (phase2.cpp:349-366).
+        std::vector<std::pair<int32_t, int32_t>> deltas;  // (global id, new label), rank order
+        int64_t cdeg = 0;
+        std::vector<int64_t> hist((size_t)kLpLabelCap);
+        for (RankShare &s : R) {
+            std::vector<std::pair<int32_t, int32_t>> mine;
+            for (int32_t i : s.B) {
+                const int32_t nl = lp_decide(s.off.data(), s.col.data(), s.lab.data(), i, pen.data(),
+                                             k, hist.data());
+                if (nl != s.lab[(size_t)i]) mine.emplace_back(i, nl);
+            }
+            for (const auto &d : mine) {
+                s.lab[(size_t)d.first] = d.second;
+                deltas.emplace_back(s.gid[(size_t)d.first], d.second);
+                cdeg += s.off[(size_t)d.first + 1] - s.off[(size_t)d.first];
+            }
+        }
+        // Allgather of the deltas and the ghost refresh (phase2.cpp:122-154,
+        // 368-376); in scope 1 the ranks also share their new boundary.
+        int64_t refreshed = 0;
+        for (RankShare &s : R) {
+            for (const auto &d : deltas) {
+                auto it = s.slot_of.find(d.first);
+                if (it != s.slot_of.end()) {
+                    s.lab[(size_t)(s.nl + it->second)] = d.second;
+                    ++refreshed;
+                }
+            }
+            if (scope == 1) {
+                std::fill(s.ghost_in_b.begin(), s.ghost_in_b.end(), 0);
+                for (const RankShare &o : R)
+                    for (int32_t i : o.B) {
+                        auto it = s.slot_of.find(o.gid[(size_t)i]);
+                        if (it != s.slot_of.end()) s.ghost_in_b[(size_t)it->second] = 1;
+                    }
+            }
+        }
+        if (ghost_updates) ghost_updates[t - 1] = refreshed;
+        if (tr) {
+            if (tr->chg_count) tr->chg_count[t - 1] = (int64_t)deltas.size();
+            if (tr->chg_deg) tr->chg_deg[t - 1] = cdeg;
+        }
+        if (deltas.empty()) { reason = 2; break; }  // phase2.cpp:384-390
+        int64_t cur = 0;
+        rc = global_counts(R, k, vc, ic, cur);
+        if (rc != REF_OK) return rc;
+        if (tr && tr->cut) tr->cut[t - 1] = cur;
+        double delta = (prev_cut > 0) ? std::abs((double)(cur - prev_cut) / (double)prev_cut) : 1.0;
+        if (delta < epsilon) ++cc; else cc = 0;
+        prev_cut = cur;
+        if (cc >= kConvergeRounds) { reason = 3; break; }
+    }
+    int64_t dummy = 0;
+    rc = global_counts(R, k, vc, ic, dummy);
+    if (rc != REF_OK) return rc;
+    int64_t tv = 0, te = 0;
+    for (int32_t i = 0; i < k; ++i) { tv += vc[(size_t)i]; te += ic[(size_t)i]; }
+    double exp_v = static_cast<double>(tv) / k;
+    double exp_e = (te > 0) ? static_cast<double>(te) / k : 1.0;
+    double max_v = 0.0, max_e = 0.0;
+    for (int32_t i = 0; i < k; ++i) {
+        max_v = std::max(max_v, (exp_v > 0) ? static_cast<double>(vc[(size_t)i]) / exp_v : 1.0);
+        max_e = std::max(max_e, (exp_e > 0) ? static_cast<double>(ic[(size_t)i]) / exp_e : 1.0);
+    }
+    for (const RankShare &s : R)
+        for (int64_t i = 0; i < s.nl; ++i) labels[s.gid[(size_t)i]] = s.lab[(size_t)i];
+    m->edge_cut = prev_cut;
+    m->vertex_balance = max_v;
+    m->edge_balance = max_e;
+    m->total_vertices = tv;
+    m->total_edges = te;
+    m->rounds = rounds;
+    m->exit_reason = reason;
+    m->lp_edges = lp_edges;
+    if (vc_out) std::copy(vc.begin(), vc.end(), vc_out);
+    if (ic_out) std::copy(ic.begin(), ic.end(), ic_out);
+    return REF_OK;
+}
+
 }  // extern "C"

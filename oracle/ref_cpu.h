@@ -145,6 +145,18 @@ int ref_phase2(int64_t n, const int64_t *off, const int32_t *col, int32_t k,
                int32_t max_rounds, double epsilon, int32_t *labels,
                int64_t *vc, int64_t *ic, ref_p2_metrics *m, ref_p2_trace *trace);
 
+/* Phase 2 run as P ranks on their shares of the graph (owner(v) = v mod P),
+ * ghost labels refreshed from the allgathered deltas each round.  scope 0:
+ * each rank selects over its own boundary vertices (the reference);
+ * scope 1: selection over all ranks (equals ref_phase2 for every P).
+ * labels: global, in/out.  ghost_updates (nullable): per round, the ghost
+ * labels the refresh rewrote, summed over the ranks.  The trace's
+ * cand_count and scan_len are -1. */
+int ref_phase2_ranks(int64_t n, const int64_t *off, const int32_t *col, int32_t k, int32_t P,
+                     int32_t scope, int32_t max_rounds, double epsilon, int32_t *labels,
+                     int64_t *vc, int64_t *ic, ref_p2_metrics *m, ref_p2_trace *trace,
+                     int64_t *ghost_updates);
+
 #ifdef __cplusplus
 }
 #endif

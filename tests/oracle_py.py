@@ -68,6 +68,8 @@ def lib():
             "ref_lp_round": [i64, P(i64), P(i32), P(i32), P(i32), i64, P(f64), i32, P(i32)],
             "ref_phase2": [i64, P(i64), P(i32), i32, i32, f64, P(i32), P(i64), P(i64),
                            P(P2Metrics), P(P2Trace)],
+            "ref_phase2_ranks": [i64, P(i64), P(i32), i32, i32, i32, i32, f64, P(i32), P(i64),
+                                 P(i64), P(P2Metrics), P(P2Trace), P(i64)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -253,4 +255,24 @@ def phase2(off, col, k, labels, max_rounds, epsilon) -> Phase2Out:
     _ok(lib().ref_phase2(len(off) - 1, _p(off, i64), _p(col, i32), k, max_rounds, epsilon,
                          _p(labels, i32), _p(vc, i64), _p(ic, i64), C.byref(m), C.byref(tr)))
     trace = {f: a[:m.rounds].copy() for f, a in arrays.items()}
+    return Phase2Out(labels, vc, ic, m, trace)
+
+
+def phase2_ranks(off, col, k, labels, nranks, scope, max_rounds, epsilon) -> Phase2Out:
+    """Phase 2 as nranks ranks on their shares (scope 0: per-rank selection
+    as the reference; 1: selection over all ranks).  The trace gains
+    'ghost_updates'."""
+    off, col = csr(off, col)
+    labels = np.array(labels, np.int32, copy=True)
+    vc = np.zeros(k, np.int64)
+    ic = np.zeros(k, np.int64)
+    m = P2Metrics()
+    arrays = {f: np.full(max(max_rounds, 1), -7, np.int64) for f in TRACE_FIELDS}
+    tr = P2Trace(**{f: _p(a, i64) for f, a in arrays.items()})
+    gu = np.full(max(max_rounds, 1), -7, np.int64)
+    _ok(lib().ref_phase2_ranks(len(off) - 1, _p(off, i64), _p(col, i32), k, nranks, scope,
+                               max_rounds, epsilon, _p(labels, i32), _p(vc, i64), _p(ic, i64),
+                               C.byref(m), C.byref(tr), _p(gu, i64)))
+    trace = {f: a[:m.rounds].copy() for f, a in arrays.items()}
+    trace["ghost_updates"] = gu[:m.rounds].copy()
     return Phase2Out(labels, vc, ic, m, trace)
