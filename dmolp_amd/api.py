@@ -74,6 +74,16 @@ def write_adjacency_text(path: str, off: np.ndarray, col: np.ndarray):
                                                  _p(off, C.c_int64), _p(col, C.c_int32)))
 
 
+def hubs_host(deg: np.ndarray) -> np.ndarray:
+    """The library's hub list of a degree sequence (host, no device)."""
+    deg = np.ascontiguousarray(deg, np.int32)
+    out = np.zeros(max(len(deg), 1), np.int32)
+    nh = C.c_int64()
+    _check(_abi.lib().dmolp_hubs_host(_p(deg, C.c_int32), len(deg), _p(out, C.c_int32),
+                                      C.byref(nh)))
+    return out[:nh.value].copy()
+
+
 def write_csr_binary(path: str, off: np.ndarray, col: np.ndarray):
     off = np.ascontiguousarray(off, np.int64)
     col = np.ascontiguousarray(col, np.int32)

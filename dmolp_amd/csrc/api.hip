@@ -296,6 +296,16 @@ int dmolp_ctx_set_tuning(dmolp_ctx *ctx, const dmolp_tuning *t) {
     return DMOLP_OK;
 }
 
+int dmolp_hubs_host(const int32_t *deg, int64_t n, int32_t *hubs_out, int64_t *n_hubs) {
+    if (!n_hubs || n < 0 || n > 0x7fffffffll || (n > 0 && !deg)) return DMOLP_E_ARG;
+    return guarded(nullptr, [&] {
+        const std::vector<int32_t> d(deg, deg + n);
+        const std::vector<int32_t> h = find_hubs_host(d);
+        if (hubs_out) std::copy(h.begin(), h.end(), hubs_out);
+        *n_hubs = (int64_t)h.size();
+    });
+}
+
 int dmolp_gen_host(const dmolp_gen_spec *spec, dmolp_csr *out) {
     return guarded(nullptr, [&] { gen_host(spec, out); });
 }

@@ -445,40 +445,129 @@ void phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int k) {
 
 // ---------------------------------------------------------------- host side
 
+namespace {
+
+using DegPair = std::pair<int, int>;  // (vertex id, degree)
+struct ByDegree {
+    bool operator()(const DegPair &a, const DegPair &b) const { return a.second < b.second; }
+};
+
+// The positions [keep, n) of a[0, n) exactly as std::sort(a, a + n, ByDegree)
+// leaves them with libstdc++ (bits/stl_algo.h: __introsort_loop with
+// median-of-three pivots and unguarded partitions, depth limit 2 lg n, heap
+// sort past it, then one insertion sort).  The comparison is not a total
+// order (ties in degree), so the tie order at the top is that algorithm's,
+// and it is reproduced step for step on every range that reaches into
+// [keep, n); ranges wholly below `keep` are left as they are, since no step
+// moves an element across a partition cut and the final insertion sort
+// only reorders within the ranges the partitions leave (every element of a
+// range is no greater than every element of the next).  Cost: the
+// partitions along the path to the top, about 2n, instead of n lg n.
+void introsort_top(DegPair *a, int64_t n, int64_t keep) {
+    const ByDegree less;
+    std::vector<std::pair<int64_t, int64_t>> leaves;  // ranges the final insertion sort orders
+    // __introsort_loop(first, last, depth), right parts first (recursion),
+    // then the loop over the left part; ranges below `keep` are skipped.
+    struct Frame { int64_t first, last; int depth; };
+    std::vector<Frame> stack{{0, n, 0}};
+    int lg = 0;
+    while ((int64_t(1) << (lg + 1)) <= n) ++lg;
+    stack[0].depth = 2 * lg;
+    while (!stack.empty()) {
+        Frame f = stack.back();
+        stack.pop_back();
+        int64_t first = f.first, last = f.last;
+        int depth = f.depth;
+        while (last - first > 16) {
+            if (last <= keep) { first = last; break; }  // no effect on [keep, n)
+            if (depth == 0) {  // __partial_sort(first, last, last): heap sort
+                std::make_heap(a + first, a + last, less);
+                std::sort_heap(a + first, a + last, less);
+                first = last;  // sorted: the insertion sort leaves it as it is
+                break;
+            }
+            --depth;
+            // __unguarded_partition_pivot
+            const int64_t mid = first + (last - first) / 2;
+            {
+                DegPair *r = a + first, *x = a + first + 1, *y = a + mid, *z = a + last - 1;
+                if (less(*x, *y)) {
+                    if (less(*y, *z)) std::iter_swap(r, y);
+                    else if (less(*x, *z)) std::iter_swap(r, z);
+                    else std::iter_swap(r, x);
+                } else if (less(*x, *z)) std::iter_swap(r, x);
+                else if (less(*y, *z)) std::iter_swap(r, z);
+                else std::iter_swap(r, y);
+            }
+            int64_t lo = first + 1, hi = last;
+            const DegPair *pivot = a + first;
+            for (;;) {
+                while (less(a[lo], *pivot)) ++lo;
+                --hi;
+                while (less(*pivot, a[hi])) --hi;
+                if (!(lo < hi)) break;
+                std::iter_swap(a + lo, a + hi);
+                ++lo;
+            }
+            const int64_t cut = lo;
+            // Recursion on [cut, last) runs before the loop goes on with
+            // [first, cut): push the left part, process the right part now.
+            stack.push_back({first, cut, depth});
+            first = cut;
+        }
+        if (last - first > 0 && last > keep && last - first <= 16) leaves.push_back({first, last});
+    }
+    // __final_insertion_sort restricted to the ranges that reach [keep, n).
+    for (const auto &lf : leaves) {
+        for (int64_t i = lf.first + 1; i < lf.second; ++i) {
+            const DegPair val = a[i];
+            int64_t j = i;
+            while (j > lf.first && less(val, a[j - 1])) { a[j] = a[j - 1]; --j; }
+            a[j] = val;
+        }
+    }
+}
+
+}  // namespace
+
 // init.cpp:139-177 at one rank: (id, degree) pairs in the degree map's
 // iteration order (descending id), libstdc++ std::sort by degree, Lorenz
-// tangent cut.  Double arithmetic in source order.
+// tangent cut.  Double arithmetic in source order.  The tangent needs only
+// the last two cumulative shares, which are exact integer sums divided by
+// the total whatever the order (every partial sum is an integer below
+// 2^53), so the number of hubs is known before sorting and only the top of
+// the sorted order is produced (introsort_top).
 std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg) {
     const int64_t n = (int64_t)deg.size();
-    std::vector<std::pair<int, int>> sd;
-    sd.reserve((size_t)n);
-    for (int64_t v = n - 1; v >= 0; --v) sd.emplace_back((int)v, deg[(size_t)v]);
-    std::sort(sd.begin(), sd.end(),
-              [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.second < b.second; });
     std::vector<int32_t> hubs;
-    const int N = (int)sd.size();
+    const int N = (int)n;
     if (N < 2) return hubs;
-    std::vector<double> cum((size_t)N);
-    double total = 0.0, run = 0.0;
-    for (int i = 0; i < N; ++i) {
-        const int d = sd[(size_t)i].second;
-        total += d;
-        run += d;
-        cum[(size_t)i] = run;
+    double total = 0.0;
+    int dmax = 0;
+    for (int64_t v = n - 1; v >= 0; --v) {
+        total += deg[(size_t)v];
+        dmax = std::max(dmax, deg[(size_t)v]);
     }
-    for (int i = 0; i < N; ++i) cum[(size_t)i] /= total;
     const double inv_N = 1.0 / N;
     const double x1 = N * inv_N;
-    const double y1 = cum[(size_t)N - 1];
+    const double y1 = total / total;            // cum[N-1]
     const double x0 = (N - 1) * inv_N;
-    const double y0 = cum[(size_t)N - 2];
+    const double y0 = (total - dmax) / total;   // cum[N-2]: all but the largest degree
     const double slope = (y1 - y0) / (x1 - x0);
     const double xi = x1 - y1 / slope;
+    int64_t M = 0;
     for (int i = N - 1; i >= 0; --i) {
         const double x = (i + 1) * inv_N;
-        if (x > xi) hubs.push_back(sd[(size_t)i].first);
+        if (x > xi) ++M;
         else break;
     }
+    if (M == 0) return hubs;
+    std::vector<DegPair> sd;
+    sd.reserve((size_t)n);
+    for (int64_t v = n - 1; v >= 0; --v) sd.emplace_back((int)v, deg[(size_t)v]);
+    introsort_top(sd.data(), n, n - M);
+    hubs.reserve((size_t)M);
+    for (int64_t i = n - 1; i >= n - M; --i) hubs.push_back(sd[(size_t)i].first);
     return hubs;
 }
 

@@ -197,6 +197,12 @@ int dmolp_graph_labels_checkpoint(dmolp_ctx *ctx, dmolp_graph *g, int op);
 
 /* ---------------- phase 1 ---------------- */
 
+/* find_hub_nodes (init.cpp:139-177) as the library runs it on the host
+ * for dmolp_phase1: the hubs of a degree sequence deg[0, n) in the
+ * reference's order (its std::sort's order of equal degrees included).
+ * hubs_out has capacity n (nullable); no device work. */
+int dmolp_hubs_host(const int32_t *deg, int64_t n, int32_t *hubs_out, int64_t *n_hubs);
+
 /* Phase1Metrics (graph_types.h:276-309) with int64 counters. */
 typedef struct {
     int64_t initial_edge_cut;
