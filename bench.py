@@ -226,18 +226,29 @@ def run_served(a):
     jobs = c5_jobs(a.jobs)
     ctx = api.Context(0)
     # Plain service time of every job (generate + phase 1 + phase 2 + free).
-    service = []
+    service, dev_ms, host_ms = [], [], []
     for it in range(2):
-        service = []
+        service, dev_ms, host_ms = [], [], []
         for sp, k in jobs:
             t0 = time.perf_counter()
             g = ctx.generate(sp)
-            g.phase1(k, theta, want_labels=False)
-            g.phase2(k, rounds, eps, want_labels=False)
+            p1 = g.phase1(k, theta, want_labels=False)
+            p2 = g.phase2(k, rounds, eps, want_labels=False, want_trace=False)
             g.free()
             service.append(time.perf_counter() - t0)
+            dev_ms.append(p1.metrics.device_ms + p2.metrics.device_ms)
+            host_ms.append(p1.metrics.host_ms)
+            if it == 1 and service[-1] > 0.05:
+                log(f"[bench]   job n={g.n if g.handle else 0} scale={sp.scale} ef={sp.edge_factor} "
+                    f"k={k}: service {service[-1] * 1e3:.0f} ms, phase 1 device "
+                    f"{p1.metrics.device_ms:.1f} ms ({p1.metrics.rounds} rounds, hub sort "
+                    f"{p1.metrics.host_ms:.1f} ms), phase 2 device {p2.metrics.device_ms:.1f} ms "
+                    f"({p2.metrics.rounds} rounds)")
+    order = np.argsort(service)[::-1][:5]
     log(f"[bench] plain service: total {sum(service):.2f} s over {len(jobs)} jobs, "
-        f"max {max(service) * 1e3:.1f} ms")
+        f"max {max(service) * 1e3:.1f} ms; phase 1 + 2 device time {sum(dev_ms) / 1e3:.2f} s, "
+        f"hub sorts {sum(host_ms) / 1e3:.2f} s; slowest jobs (ms service/device/hub): " +
+        ", ".join(f"{service[i] * 1e3:.0f}/{dev_ms[i]:.0f}/{host_ms[i]:.0f}" for i in order))
     srv = api.Server(ctx, 200 << 30, a.max_active)
     per_rate = {}
     rng = np.random.default_rng(5)

@@ -296,14 +296,45 @@ __global__ void k_p1_fill(const P1Job *JS, const int32_t *bstart, int jobs, int 
     }
 }
 
-__device__ __forceinline__ void claim(int32_t v, int32_t p, int32_t *claims, int32_t *touched,
-                                      int64_t *touched_cnt, const int32_t *L, bool pred) {
-    bool first = false;
-    if (pred && L[v] == -1) {
-        if (claims[v] > p) first = atomicMin(&claims[v], p) == kNoClaim;
-    }
-    const int64_t pos = wave_append(first, (unsigned long long *)touched_cnt);
-    if (first) touched[pos] = v;
+// Claims of one round (partition.cpp:451-476): an entry (u selected by part
+// p, neighbour v) with v unlabelled lowers claims[v] to p; the thread whose
+// atomicMin replaced "no claim" records v as touched.  Touched vertices are
+// staged per wave in LDS and reserved in the global list once per
+// kTouchStage - kWave entries (one counter hit by every wave each step
+// serialises at the L2).
+constexpr int kTouchStage = 1024;
+constexpr int kExpUnroll = 4;  // entries per lane per step, loads issued together
+
+struct TouchLds {
+    int32_t q[kTouchStage];
+    int32_t n;
+};
+
+// Wave-level: every lane calls it.
+__device__ __forceinline__ void touch_flush(TouchLds &t, int32_t *touched, int64_t *touched_cnt) {
+    wave_sync();
+    const int32_t c = t.n;
+    unsigned long long base = 0;
+    if (lane_id() == 0 && c) base = atomicAdd((unsigned long long *)touched_cnt, (unsigned long long)c);
+    base = __shfl(base, 0, kWave);
+    for (int i = lane_id(); i < c; i += kWave) touched[base + (unsigned long long)i] = t.q[i];
+    wave_sync();
+    if (lane_id() == 0) t.n = 0;
+    wave_sync();
+}
+
+// Stage the lanes with `first` set (every lane calls it); flushes when the
+// next step could overflow the stage.
+__device__ __forceinline__ void touch_stage(bool first, int32_t v, TouchLds &t, int32_t *touched,
+                                            int64_t *touched_cnt) {
+    const uint64_t m = __ballot(first);
+    if (m == 0) return;  // wave-uniform
+    const int32_t pos = t.n;
+    if (first) t.q[pos + lanes_below(m)] = v;
+    wave_sync();
+    if (lane_id() == 0) t.n = pos + __popcll(m);
+    wave_sync();
+    if (t.n > kTouchStage - kWave) touch_flush(t, touched, touched_cnt);
 }
 
 // partition.cpp:451-476: claims by the selected frontier members.
@@ -318,7 +349,10 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
     const P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
+    __shared__ TouchLds s_t[kWavesPerBlock];
     const int w = threadIdx.x / kWave, lane = lane_id();
+    if (lane == 0) s_t[w].n = 0;
+    wave_sync();
     const int64_t count = (int64_t)sel.cnt[LC_LIGHT];
     for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
         int32_t v, deg;
@@ -326,22 +360,39 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
         const int32_t total = batch_load(g, sel.light, count, b0, s_b[w], v, deg, valid);
         s_b[w].aux[lane] = valid ? L[v] : 0;
         wave_sync();
-        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
-            const int32_t q = q0 + lane;
-            const bool on = q < total;
-            int32_t u = 0, p = 0;
-            if (on) {
-                const int slot = batch_slot(s_b[w], q);
-                u = g.col[s_b[w].base[slot] + q];
-                p = s_b[w].aux[slot];
+        for (int32_t q0 = 0; q0 < total; q0 += kWave * kExpUnroll) {
+            // Columns, then the neighbours' labels and claims, each stage's
+            // loads from in-bounds indices so they issue together.
+            int32_t u[kExpUnroll], p[kExpUnroll], lv[kExpUnroll], cl[kExpUnroll];
+            bool on[kExpUnroll];
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) {
+                const int32_t q = q0 + j * kWave + lane;
+                on[j] = q < total;
+                const int slot = on[j] ? batch_slot(s_b[w], q) : 0;
+                const int64_t e = s_b[w].base[slot] + q;
+                u[j] = g.col[on[j] ? e : 0];
+                p[j] = s_b[w].aux[slot];
             }
-            claim(u, p, claims, touched, touched_cnt, L, on);
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) {
+                lv[j] = L[u[j]];
+                cl[j] = claims[u[j]];
+            }
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) {
+                bool first = false;
+                if (on[j] && lv[j] == -1 && cl[j] > p[j])
+                    first = atomicMin(&claims[u[j]], p[j]) == kNoClaim;
+                touch_stage(first, u[j], s_t[w], touched, touched_cnt);
+            }
         }
         wave_sync();
     }
+    touch_flush(s_t[w], touched, touched_cnt);
 }
 
-__global__ void k_expand_heavy(const P1Job *JS, const int32_t *bstart, int jobs) {
+__global__ __launch_bounds__(kThreads) void k_expand_heavy(const P1Job *JS, const int32_t *bstart, int jobs) {
     P1_JOB();
     const GraphDev g = J.g;
     const ListDev sel = J.sel_list;
@@ -352,18 +403,39 @@ __global__ void k_expand_heavy(const P1Job *JS, const int32_t *bstart, int jobs)
     int64_t *touched_cnt = J.touched_cnt;
     const P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
+    __shared__ TouchLds s_t[kWavesPerBlock];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    if (lane == 0) s_t[w].n = 0;
+    wave_sync();
     const int64_t nch = (int64_t)sel.cnt[LC_CHUNKS];
-    for (int64_t j = vwave(vg); j < nch; j += vwaves(vg)) {
+    for (int64_t j0 = vwave(vg); j0 < nch; j0 += vwaves(vg)) {
         int32_t slot, v;
         int64_t e0, e1, deg;
-        chunk_bounds(g, sel, sel.hc[j], chunk_edges, slot, v, e0, e1, deg);
+        chunk_bounds(g, sel, sel.hc[j0], chunk_edges, slot, v, e0, e1, deg);
         const int32_t p = L[v];
-        for (int64_t eb = e0; eb < e1; eb += kWave) {
-            const int64_t e = eb + lane_id();
-            const bool on = e < e1;
-            claim(on ? g.col[e] : 0, p, claims, touched, touched_cnt, L, on);
+        for (int64_t eb = e0; eb < e1; eb += (int64_t)kWave * kExpUnroll) {
+            int32_t u[kExpUnroll], lv[kExpUnroll], cl[kExpUnroll];
+            bool on[kExpUnroll];
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) {
+                const int64_t e = eb + j * kWave + lane;
+                on[j] = e < e1;
+                u[j] = g.col[on[j] ? e : e0];
+            }
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) {
+                lv[j] = L[u[j]];
+                cl[j] = claims[u[j]];
+            }
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) {
+                bool first = false;
+                if (on[j] && lv[j] == -1 && cl[j] > p) first = atomicMin(&claims[u[j]], p) == kNoClaim;
+                touch_stage(first, u[j], s_t[w], touched, touched_cnt);
+            }
         }
     }
+    touch_flush(s_t[w], touched, touched_cnt);
 }
 
 // partition.cpp:539-597: label the claimed vertices, extend the frontiers,
