@@ -440,6 +440,17 @@ class Graph:
                self.ctx.handle)
         return vc, ic, cut.value
 
+    def stats_noalloc(self, k, labels):
+        """stats() through the entry point that allocates no device memory."""
+        labels = np.ascontiguousarray(labels, np.int32)
+        vc = np.zeros(k, np.int64)
+        ic = np.zeros(k, np.int64)
+        cut = C.c_int64()
+        _check(_abi.lib().dmolp_stats_noalloc(self.ctx.handle, self._h, k, _p(labels, C.c_int32),
+                                              _p(vc, C.c_int64), _p(ic, C.c_int64),
+                                              C.byref(cut)), self.ctx.handle)
+        return vc, ic, cut.value
+
     def select_boundary(self, labels, prev, re, k, t):
         labels = np.ascontiguousarray(labels, np.int32)
         prev = np.ascontiguousarray(prev, np.int32)

@@ -29,6 +29,7 @@ SOURCES = [
     "batch.hip",
     "server.hip",
     "halo.hip",
+    "noalloc.hip",
     "gen_host.cpp",
     "textio.cpp",
     "driver/driver.cpp",

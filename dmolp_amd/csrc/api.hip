@@ -17,6 +17,7 @@
 #include "phase2.hpp"
 #include "batch.hpp"
 #include "halo.hpp"
+#include "noalloc.hpp"
 #include "rows.hpp"
 
 namespace dmolp {
@@ -209,6 +210,7 @@ int dmolp_ctx_create(int device, dmolp_ctx **out) {
         DMOLP_HIP_CHECK(hipEventCreate(&ctx->ev1));
         DMOLP_HIP_CHECK(hipHostMalloc((void **)&ctx->poll_host, 64, hipHostMallocDefault));
         for (hipEvent_t &e : ctx->poll_ev) DMOLP_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        noalloc_init_device(device);  // the no-allocation scratch's address on this device
     });
     if (rc != DMOLP_OK) {
         g_tls_error = ctx->last_error;
@@ -634,6 +636,12 @@ int dmolp_stats(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, const int32_t *labels
         if (ic) std::copy(ci.begin(), ci.end(), ic);
         if (cut) *cut = c;
     });
+}
+
+int dmolp_stats_noalloc(dmolp_ctx *ctx, const dmolp_graph *g, int32_t k, const int32_t *labels,
+                        int64_t *vc, int64_t *ic, int64_t *cut) {
+    if (!ctx || !g || (!labels && g->n > 0)) return fail(ctx, DMOLP_E_ARG, "null argument");
+    return guarded(ctx, [&] { stats_noalloc(ctx, g, k, labels, vc, ic, cut); });
 }
 
 int dmolp_penalties(dmolp_ctx *ctx, int32_t k, const int64_t *vc, const int64_t *ic, int32_t t,

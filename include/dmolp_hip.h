@@ -282,6 +282,16 @@ int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, 
  * host labels. */
 int dmolp_stats(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, const int32_t *labels, int64_t *vc,
                 int64_t *ic, int64_t *cut);
+/* No counterpart in the reference: dmolp_stats for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * statuses as dmolp_stats; its device memory beyond g is one static array
+ * per GPU (the labels and the counters), reused by every call, so calls
+ * that overlap on one GPU race.  n above DMOLP_NOALLOC_MAX_N fails with
+ * DMOLP_E_UNSUPPORTED. */
+#define DMOLP_NOALLOC_MAX_N (1 << 24)
+int dmolp_stats_noalloc(dmolp_ctx *ctx, const dmolp_graph *g, int32_t k, const int32_t *labels,
+                        int64_t *vc, int64_t *ic, int64_t *cut);
 /* calculatePenalties (phase2.cpp:158-209) evaluated on the device. */
 int dmolp_penalties(dmolp_ctx *ctx, int32_t k, const int64_t *vc, const int64_t *ic, int32_t t,
                     double *pen, double *re);
