@@ -133,6 +133,45 @@ def cpu_baseline_round1(off, col, k, labels0, eps, threads):
     }, res
 
 
+def oracle_jobs_baseline(specs, rounds, eps, budget_s, unit):
+    """The CPU oracle's phase 1 + phase 2 on jobs taken in order until about
+    budget_s seconds of work, on all usable cores (generation excluded)."""
+    from dmolp_amd import api
+    T, hinfo = cpu_threads()
+    O = oracle()
+    O.set_threads(T)
+    done, el = 0, 0.0
+    for spec, k in specs:
+        off, col = api.gen_host(spec)
+        t0 = time.perf_counter()
+        p1 = O.phase1(off, col, k, THETA)
+        O.phase2(off, col, k, p1.labels, rounds, eps)
+        el += time.perf_counter() - t0
+        done += 1
+        if el >= budget_s:
+            break
+    return {"value": done / el, "unit": unit, "cores": T, "kind": "port",
+            "sample": f"oracle phase 1 + phase 2 on the first {done} jobs of the same list "
+                      f"(generation excluded), {el:.1f} s on {T} threads", "host": hinfo}
+
+
+def job_parity(fixture, results):
+    """Every job's labels (SHA-256) and metrics against the oracle fixture."""
+    if fixture is None or len(fixture["jobs"]) != len(results):
+        return None
+    bad = []
+    for i, (w, r) in enumerate(zip(fixture["jobs"], results)):
+        res, labels = r
+        ok = (res.status == 0 and sha(labels) == w["labels_sha256"] and res.edge_cut == w["cut"]
+              and res.phase1_cut == w["phase1_cut"] and res.rounds == w["rounds"]
+              and res.exit_reason == w["exit_reason"] and res.lp_edges == w["lp_edges"]
+              and res.vertex_balance == w["vertex_balance"]
+              and res.edge_balance == w["edge_balance"])
+        if not ok:
+            bad.append(i)
+    return {"jobs": len(results), "mismatched": bad, "ok": not bad}
+
+
 def run_batched(a):
     """Batched mode: every graph's phase 1 + 10 LP rounds in one job set,
     against the same graphs partitioned one at a time (plain mode)."""
@@ -173,20 +212,18 @@ def run_batched(a):
     same_labels = all(np.array_equal(b.labels, p.labels) for b, p in zip(bres, plain_res))
     log(f"[bench] batched: {len(jobs)} graphs in {batch_s:.3f} s (device {np.mean(dev):.1f} ms), "
         f"speedup {plain_s / batch_s:.2f}x, identical cut {same_cut}, identical labels {same_labels}")
+    fx_path = FIXTURES / "large_c4.json"
+    fx = json.loads(fx_path.read_text()) if fx_path.exists() else None
+    parity = job_parity(fx, [(b, b.labels) for b in bres])
+    if parity is not None:
+        parity["fixture"] = str(fx_path.relative_to(ROOT))
+    parity = parity or {"ok": None}
+    parity["batched_equals_plain"] = bool(same_cut and same_labels)
+    log(f"[bench] parity {json.dumps(parity)}")
     cpu = None
     if not a.no_cpu_baseline:
-        sys.path.insert(0, str(ROOT / "tests"))
-        import oracle_py as O
-        sample = list(range(0, 16))
-        t0 = time.perf_counter()
-        for i in sample:
-            off, col = api.gen_host(specs[i][0])
-            p1 = O.phase1(off, col, specs[i][1], theta)
-            O.phase2(off, col, specs[i][1], p1.labels, rounds, eps)
-        el = time.perf_counter() - t0
-        cpu = {"value": len(sample) / el, "unit": "graphs/s", "cores": 1, "kind": "port",
-               "sample": f"oracle phase 1 + phase 2 on the first {len(sample)} jobs of the same "
-                         f"batch (generation excluded): {el:.1f} s"}
+        cpu = oracle_jobs_baseline(specs, rounds, eps, 15.0, "graphs/s")
+        log(f"[bench] cpu baseline {cpu['value']:.2f} graphs/s ({cpu['sample']})")
     out = {
         "metric": "batched graphs/sec (phase 1 + 10 LP rounds per graph)",
         "value": len(jobs) / batch_s,
@@ -209,6 +246,7 @@ def run_batched(a):
         "batch_device_ms": float(np.mean(dev)),
         "roofline": None,
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     for g in graphs:
         g.free()
@@ -252,17 +290,20 @@ def run_served(a):
     srv = api.Server(ctx, 200 << 30, a.max_active)
     per_rate = {}
     rng = np.random.default_rng(5)
+    last_results = None
     for lam in rates:
+        keep = lam == rates[-1]  # the last rate's results are checked against the oracle
         gaps = rng.exponential(1.0 / lam, len(jobs))
         arr_rel = np.cumsum(gaps) - gaps[0]
         base = srv.clock() + 0.01
         arr = base + arr_rel
-        tickets, done, nxt = [], {}, 0
+        tickets, done, kept, nxt = [], {}, {}, 0
         while len(done) < len(jobs):
             now = srv.clock()
             while nxt < len(jobs) and arr[nxt] <= now:
                 sp, k = jobs[nxt]
-                tickets.append(srv.submit(k, theta, rounds, eps, spec=sp, arrival_s=float(arr[nxt])))
+                tickets.append(srv.submit(k, theta, rounds, eps, spec=sp, arrival_s=float(arr[nxt]),
+                                          want_labels=keep))
                 nxt += 1
             active, queued = srv.step()
             for t in tickets:
@@ -270,10 +311,14 @@ def run_served(a):
                     r = srv.take(t)
                     if r is not None:
                         done[t] = r[0]
+                        if keep:
+                            kept[t] = r[1]
             if active == 0 and queued == 0 and nxt < len(jobs) and len(done) == nxt:
                 time.sleep(max(0.0, min(0.05, arr[nxt] - srv.clock())))
         lat = np.array([done[t].done_s - done[t].arrival_s for t in tickets])
-        span = max(d.done_s for d in done.values()) - float(arr[0])
+        span = max(done[t].done_s for t in tickets) - float(arr[0])
+        if keep:
+            last_results = [(done[t].result, kept[t]) for t in tickets]
         bad = [t for t in tickets if done[t].result.status != 0]
         # Plain server: FIFO, one job at a time.
         fin, plat = 0.0, []
@@ -291,6 +336,17 @@ def run_served(a):
             f"plain {per_rate[str(lam)]['plain_jobs_per_s']:.1f} jobs/s "
             f"p50 {per_rate[str(lam)]['plain_p50_ms']:.1f} ms p99 {per_rate[str(lam)]['plain_p99_ms']:.1f} ms")
     srv.close()
+    fx_path = FIXTURES / "large_c5.json"
+    fx = json.loads(fx_path.read_text()) if fx_path.exists() else None
+    parity = job_parity(fx, last_results) if last_results else None
+    if parity is not None:
+        parity["fixture"] = str(fx_path.relative_to(ROOT))
+    parity = parity or {"ok": None}
+    log(f"[bench] parity {json.dumps(parity)}")
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = oracle_jobs_baseline(jobs, rounds, eps, 15.0, "jobs/s")
+        log(f"[bench] cpu baseline {cpu['value']:.2f} jobs/s ({cpu['sample']})")
     top = per_rate[str(rates[-1])]
     out = {
         "metric": "served jobs/sec at saturation (Poisson arrivals), with p50/p99 latency",
@@ -310,8 +366,11 @@ def run_served(a):
                    "max_active": a.max_active, "budget_gib": 200},
         "per_rate": per_rate,
         "speedup_vs_plain_at_saturation": top["jobs_per_s"] / top["plain_jobs_per_s"],
+        "plain_device_s": sum(dev_ms) / 1e3,
+        "plain_service_s": sum(service),
         "roofline": None,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
+        "parity": parity,
     }
     ctx.close()
     return out
@@ -400,12 +459,10 @@ def main(argv=None):
                                    "GPUs is not built",
                           "n_gpus": a.gpus}))
         return 2
-    if a.config == "c4":
-        print(json.dumps(run_batched(a)))
-        return 0
-    if a.config == "c5":
-        print(json.dumps(run_served(a)))
-        return 0
+    if a.config in ("c4", "c5"):
+        out = run_batched(a) if a.config == "c4" else run_served(a)
+        print(json.dumps(out))
+        return 0 if out["parity"].get("ok") is not False else 1
     from dmolp_amd import api
     cfg = CONFIGS[a.config]
     k, R, eps = cfg["k"], cfg["rounds"], cfg["eps"]
