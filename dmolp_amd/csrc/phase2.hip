@@ -1482,6 +1482,41 @@ __device__ __forceinline__ void lh_move(uint32_t *lh, int32_t u, int32_t a, int3
     if (da) atomicAdd(&row[a >> 1], 0u - ma);
     if (dc) atomicAdd(&row[c >> 1], mc);
 }
+// The same moves for one entry per lane of a full wave, two lanes an entry:
+// in each of two instructions lane 2m moves entry m's count out of the old
+// label's word and lane 2m+1 into the new label's word (entries of lanes
+// 0-31, then 32-63), so each instruction touches a row's line once.  The
+// rate of scattered atomics follows the number of lines an instruction
+// touches, so this halves the cost of moves whose labels sit in different
+// words.  u: the lane's neighbour, or -1; ac: old | new << 8, each 0xff
+// when not counted.  Called by every lane of the wave.
+__device__ __forceinline__ void lh_move_paired(uint32_t *lh, int32_t u, uint32_t ac, int lane) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int src = r * (kWave / 2) + (lane >> 1);
+        const int32_t su = __shfl(u, src, kWave);
+        const uint32_t sac = (uint32_t)__shfl((int32_t)ac, src, kWave);
+        const uint32_t a = sac & 0xffu, c = (sac >> 8) & 0xffu;
+        const bool da = a != 0xffu, dc = c != 0xffu;
+        const bool odd = (lane & 1) != 0;
+        uint32_t word, val;
+        bool act;
+        if (da && dc && (a >> 1) == (c >> 1)) {  // one add moves both halves
+            act = !odd;
+            word = a >> 1;
+            val = (1u << ((c & 1) * 16)) - (1u << ((a & 1) * 16));
+        } else if (!odd) {
+            act = da;
+            word = a >> 1;
+            val = 0u - (1u << ((a & 1) * 16));
+        } else {
+            act = dc;
+            word = c >> 1;
+            val = 1u << ((c & 1) * 16);
+        }
+        if (su >= 0 && act) atomicAdd(lh + (int64_t)su * kPackedWords + word, val);
+    }
+}
 __device__ __forceinline__ void hh_move(const int32_t *hid, int32_t *hhc, int64_t nheavy, int32_t u,
                                         int32_t a, int32_t c, int E) {
     int32_t *row = hhc + ((int64_t)(blockIdx.x % kHhCopies) * nheavy + hid[u]) * kLpCap;
@@ -1497,7 +1532,7 @@ __device__ __forceinline__ void row_upkeep(const P2Job &J, bool lh_on, bool hh_p
     }
 }
 
-__global__ __launch_bounds__(kThreads, 6) void k_delta_light(const P2Job *JS, const int32_t *bstart, int jobs) {
+__global__ __launch_bounds__(kThreads, 4) void k_delta_light(const P2Job *JS, const int32_t *bstart, int jobs) {
     P2_JOB();
     const GraphDev g = J.g;
     const ListDev D = J.D;
@@ -1570,7 +1605,9 @@ __global__ __launch_bounds__(kThreads, 6) void k_delta_light(const P2Job *JS, co
     #pragma unroll
                 for (int j = 0; j < kLpUnroll; ++j) {
                     sx[j] = 0xff000000u;
-                    if (slot[j] < 0) continue;
+                    int32_t mu = -1;        // kept-row move of the entry: neighbour or -1
+                    uint32_t mac = 0xffffu; // and its old | new << 8 (0xff: not counted)
+                    if (slot[j] >= 0) {
                     const int32_t vv = s_b[w].v[slot[j]];
                     const int32_t a = s_b[w].aux[slot[j]], c = s_new[w][slot[j]];
                     int32_t dd, lu, lu2;
@@ -1595,8 +1632,12 @@ __global__ __launch_bounds__(kThreads, 6) void k_delta_light(const P2Job *JS, co
                             if (c >= 0 && c < E) atomicAdd(&row[c], 1);
                         }
                     } else if (lh_on) {
-                        lh_move(J.lh, u[j], a, c, E);
+                        mu = u[j];
+                        mac = (a >= 0 && a < E ? (uint32_t)a : 0xffu) |
+                              (c >= 0 && c < E ? (uint32_t)c : 0xffu) << 8;
                     }
+                    }
+                    if (lh_on) lh_move_paired(J.lh, mu, mac, lane);  // every lane, convergent
                 }
                 // Fields of 7 bits: at most 64 lanes add a 0 or 1 to each.
     #pragma unroll
