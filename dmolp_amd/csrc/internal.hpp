@@ -138,20 +138,11 @@ struct LaunchSet {
     const int32_t *bs(int pol) const { return bstart.as<int32_t>() + (size_t)pol * (jobs + 1); }
 };
 
-// Device records of a phase-1 job set.
-struct P1SetDev {
-    int G = 0;
-    int passes = 0;
-    bool any_heavy = false;
-    int64_t max_n = 0;
-    LaunchSet ls;
-    DevMem jobs, sel, done, remaining;
-};
-
-// A captured phase-2 round: the graph, its executable and the launch shape
-// (`key`) it was captured for.  The graph is kept as long as the executable.
-// Owners release it only after the stream has drained (graph free and
-// context teardown synchronise first; a re-capture waits for the stream).
+// A captured phase-2 round (or batch of phase-1 rounds): the graph, its
+// executable and the launch shape (`key`) it was captured for.  The graph is
+// kept as long as the executable.  Owners release it only after the stream
+// has drained (graph free and context teardown synchronise first; a
+// re-capture waits for the stream).
 struct RoundGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -167,6 +158,24 @@ struct RoundGraph {
         key = 0;
     }
     ~RoundGraph() { release(); }
+};
+
+// Device records of a phase-1 job set, and its captured batch of expansion
+// rounds: captured the second time the same launch shape is enqueued (a
+// shape used once is launched directly, so job sets that change every call
+// pay no capture).
+struct P1SetDev {
+    int G = 0;
+    int passes = 0;
+    bool any_heavy = false;
+    int64_t max_n = 0;
+    LaunchSet ls;
+    DevMem jobs, sel, done, remaining;
+    RoundGraph rounds;
+    uint64_t seen_key = 0;
+    P1SetDev() = default;
+    P1SetDev(const P1SetDev &) = delete;
+    P1SetDev &operator=(const P1SetDev &) = delete;
 };
 
 // Device records and the captured round of the last phase-2 job set.

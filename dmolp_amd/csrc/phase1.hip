@@ -943,7 +943,9 @@ void p1_expand_begin(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     DMOLP_HIP_CHECK(hipGetLastError());
 }
 
-void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds) {
+namespace {
+
+void p1_expand_enqueue(dmolp_ctx *ctx, P1SetDev &sd, int rounds) {
     hipStream_t s = ctx->stream;
     const LaunchSet &ls = sd.ls;
     const int G = sd.G;
@@ -963,6 +965,60 @@ void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds) {
         hipLaunchKernelGGL(k_p1_round_end, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G);
     }
     DMOLP_HIP_CHECK(hipGetLastError());
+}
+
+// Everything the captured launches bake in: pointers, grid sizes, pass
+// count, job count and the number of rounds.
+uint64_t p1_rounds_key(const P1SetDev &sd, int rounds) {
+    uint64_t h = 0x6a09e667f3bcc909ull;
+    const auto mix = [&h](uint64_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
+    mix((uint64_t)(uintptr_t)sd.jobs.p);
+    mix((uint64_t)(uintptr_t)sd.sel.p);
+    mix((uint64_t)(uintptr_t)sd.ls.bstart.p);
+    mix((uint64_t)sd.G);
+    mix((uint64_t)sd.ls.jobs);
+    mix((uint64_t)sd.passes);
+    mix(sd.any_heavy ? 1u : 0u);
+    mix((uint64_t)rounds);
+    for (int p = 0; p < P_NPOL; ++p) mix((uint64_t)sd.ls.total[p]);
+    return h;
+}
+
+}  // namespace
+
+// `rounds` expansion rounds of the set.  Each round is about twenty short
+// launches, so a batch launched kernel by kernel spends as long between
+// kernels as in them; a launch shape seen before runs as one captured graph
+// (kernels only: no memset or copy is recorded).
+void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds) {
+    const uint64_t key = p1_rounds_key(sd, rounds);
+    RoundGraph &rg = sd.rounds;
+    if (ctx->profiling || (!(rg.exec && rg.key == key) && sd.seen_key != key)) {
+        sd.seen_key = key;
+        p1_expand_enqueue(ctx, sd, rounds);
+        return;
+    }
+    if (!rg.exec || rg.key != key) {
+        if (rg.exec || rg.graph) {
+            // Launches of the old executable may still be queued.
+            DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            rg.release();
+        }
+        hipGraph_t graph = nullptr;
+        DMOLP_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            p1_expand_enqueue(ctx, sd, rounds);
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        DMOLP_HIP_CHECK(hipStreamEndCapture(ctx->stream, &graph));
+        rg.graph = graph;
+        DMOLP_HIP_CHECK(hipGraphInstantiate(&rg.exec, rg.graph, nullptr, nullptr, 0));
+        rg.key = key;
+    }
+    DMOLP_HIP_CHECK(hipGraphLaunch(rg.exec, ctx->stream));
 }
 
 int32_t p1_remaining(dmolp_ctx *ctx, P1SetDev &sd) {
