@@ -617,7 +617,13 @@ def main(argv=None):
         # streaming implementation of that algorithm would have to move.
         "phase2_equivalent": {"bytes_per_step": p2eq, "achieved": p2eq_gbs, "unit": "GB/s",
                               "frac": p2eq_gbs / HBM_PEAK_GBS,
-                              "device_ms": float(pr.metrics.device_ms)},
+                              "device_ms": float(pr.metrics.device_ms),
+                              "basis": "bytes the reference algorithm would stream per step "
+                                       "(LP histograms recounted from the rows, candidates "
+                                       "pushed from the previous boundary, boundary test, "
+                                       "selection keys, statistics, initial sweep); not this "
+                                       "build's traffic, which the roofline object and the "
+                                       "rocprofv3 PMC tables report"},
         "cpu_baseline": cpu,
         "parity": parity,
         "kernel_groups_ms": groups,

@@ -153,6 +153,14 @@ hipEvent_t dmolp_ctx::prof_event() {
 }
 
 namespace dmolp {
+bool graphs_enabled() {
+    static const bool on = [] {
+        const char *e = std::getenv("DMOLP_NO_GRAPHS");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
 bool round_poll(dmolp_ctx *ctx, const int32_t *d_flag, int t, bool stop_at_zero) {
     const int slot = t & 1;
     DMOLP_HIP_CHECK(hipMemcpyAsync(&ctx->poll_host[slot], d_flag, sizeof(int32_t),
@@ -302,7 +310,7 @@ int dmolp_hubs_host(const int32_t *deg, int64_t n, int32_t *hubs_out, int64_t *n
     if (!n_hubs || n < 0 || n > 0x7fffffffll || (n > 0 && !deg)) return DMOLP_E_ARG;
     return guarded(nullptr, [&] {
         const std::vector<int32_t> d(deg, deg + n);
-        const std::vector<int32_t> h = find_hubs_host(d);
+        const std::vector<int32_t> h = find_hubs_host(d, std::min(16, host_threads()));
         if (hubs_out) std::copy(h.begin(), h.end(), hubs_out);
         *n_hubs = (int64_t)h.size();
     });

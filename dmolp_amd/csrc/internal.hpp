@@ -253,4 +253,10 @@ int64_t bits_for(int64_t v);                                // bits needed to ho
 // ahead, so at most one round runs past the stop).  True when that earlier
 // flag says stop: nonzero (stop_at_zero false) or <= 0 (stop_at_zero true).
 bool round_poll(dmolp_ctx *ctx, const int32_t *d_flag, int t, bool stop_at_zero);
+// False when DMOLP_NO_GRAPHS=1 is set: the phase-1 and phase-2 rounds are
+// then launched kernel by kernel instead of replayed as captured graphs
+// (same kernels, same results).  For profilers whose graph-launch
+// interception misbehaves: rocprofv3 --kernel-trace has ended processes
+// with a host SIGSEGV inside its callback under hipGraphLaunch.
+bool graphs_enabled();
 }  // namespace dmolp

@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstring>
 #include <array>
+#include <functional>
+#include <memory>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -25,6 +27,7 @@
 #include "phase2.hpp"
 #include "rows.hpp"
 #include "select.hpp"
+#include "gen_host.hpp"
 
 namespace dmolp {
 namespace {
@@ -519,10 +522,144 @@ void phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int k) {
 
 namespace {
 
-using DegPair = std::pair<int, int>;  // (vertex id, degree)
+struct DegPair {  // (vertex id, degree); trivial, so large arrays start uninitialised
+    int first, second;
+};
 struct ByDegree {
     bool operator()(const DegPair &a, const DegPair &b) const { return a.second < b.second; }
 };
+
+// body(t) for t in [0, T) on T threads (the caller runs t = 0).
+template <class F>
+void run_threads(int T, F &&body) {
+    if (T <= 1) { body(0); return; }
+    std::vector<std::thread> th;
+    th.reserve((size_t)T - 1);
+    for (int t = 1; t < T; ++t) th.emplace_back([&body, t] { body(t); });
+    body(0);
+    for (auto &x : th) x.join();
+}
+
+// Ranges at least this long are partitioned by several threads.
+constexpr int64_t kParPartitionMin = int64_t(1) << 20;
+
+// The partition step of libstdc++'s __introsort_loop on a[first, last)
+// after the median has been moved to a[first] (__unguarded_partition of
+// [first + 1, last) around a[first]), sequential.  Returns the cut.
+int64_t partition_seq(DegPair *a, int64_t first, int64_t last) {
+    const ByDegree less;
+    int64_t lo = first + 1, hi = last;
+    const DegPair *pivot = a + first;
+    for (;;) {
+        while (less(a[lo], *pivot)) ++lo;
+        --hi;
+        while (less(*pivot, a[hi])) --hi;
+        if (!(lo < hi)) break;
+        std::swap(a[lo], a[hi]);
+        ++lo;
+    }
+    return lo;
+}
+
+// The same step on T threads, with the same result.  The sequential scans
+// stop, from the left, at elements not below the pivot (L_1 < L_2 < ...)
+// and, from the right, at elements not above it (R_1 > R_2 > ...), and
+// swap the j-th pair while L_j < R_j.  A swap changes only the two
+// positions it names, and the scans have already passed them, so L and R
+// can be read off the array before any swap; the pivot at `first` is never
+// swapped (the right scan can stop there only once the scans have crossed).
+// With m swaps (L_j < R_j holds exactly for j <= m: L rises, R falls), the
+// left scan then stops at min(L_{m+1}, R_m): the next original candidate,
+// or position R_m, which now holds the old L_m, not below the pivot; with
+// no swap it stops at L_1.  Every L_j with j <= m lies below every such
+// R_j, so the swaps touch disjoint positions and the threads' reads during
+// the swaps (their own slice's left candidates, their own run of right
+// candidates) never meet another thread's writes.
+int64_t partition_par(DegPair *a, int64_t first, int64_t last, int T) {
+    const int P = a[first].second;
+    const int64_t len = last - first;
+    // Fine slices, so that locating the j-th candidate scans a short range.
+    const int S = (int)std::min<int64_t>(4096, std::max<int64_t>(1, len / 4096));
+    const auto lo_of = [&](int64_t s) { return first + len * s / S; };
+    const auto is_l = [&](int64_t p) { return p != first && a[p].second >= P; };
+    const auto is_r = [&](int64_t p) { return a[p].second <= P; };
+    // Candidates per slice, the slices dealt to the threads in blocks.
+    std::vector<int64_t> cl((size_t)S), cr((size_t)S);
+    run_threads(T, [&](int t) {
+        for (int64_t s = (int64_t)S * t / T; s < (int64_t)S * (t + 1) / T; ++s) {
+            int64_t l = 0, r = 0;
+            for (int64_t p = lo_of(s); p < lo_of(s + 1); ++p) {
+                const int d = a[p].second;
+                l += d >= P ? 1 : 0;
+                r += d <= P ? 1 : 0;
+            }
+            cl[(size_t)s] = l - (s == 0 ? 1 : 0);  // the pivot's position is not a left candidate
+            cr[(size_t)s] = r;
+        }
+    });
+    // bl[s]: left candidates (0-based index) before slice s; br[s]: right
+    // candidates in slices s and above (R is numbered from the right end).
+    std::vector<int64_t> bl((size_t)S + 1, 0), br((size_t)S + 1, 0);
+    for (int s = 0; s < S; ++s) bl[(size_t)s + 1] = bl[(size_t)s] + cl[(size_t)s];
+    for (int s = S - 1; s >= 0; --s) br[(size_t)s] = br[(size_t)s + 1] + cr[(size_t)s];
+    const int64_t nl = bl[(size_t)S], nr = br[0];
+    if (nl == 0) return partition_seq(a, first, last);  // not reached: the median's sample holds one
+    // Slices holding L_j and R_j, then their positions by a scan of that
+    // slice, on the array as it is before any swap.
+    const auto slice_l = [&](int64_t j) {  // bl[s] <= j < bl[s + 1]
+        return (int)(std::upper_bound(bl.begin(), bl.end(), j) - bl.begin()) - 1;
+    };
+    const auto slice_r = [&](int64_t j) {  // br[s + 1] <= j < br[s]; br falls with s
+        return (int)(std::lower_bound(br.begin(), br.end(), j, std::greater<int64_t>()) - br.begin()) - 1;
+    };
+    const auto pos_l = [&](int64_t j) {
+        const int s = slice_l(j);
+        int64_t c = bl[(size_t)s];
+        for (int64_t p = lo_of(s);; ++p)
+            if (is_l(p) && c++ == j) return p;
+    };
+    const auto pos_r = [&](int64_t j) {
+        const int s = slice_r(j);
+        int64_t c = br[(size_t)s + 1];
+        for (int64_t p = lo_of(s + 1) - 1;; --p)
+            if (is_r(p) && c++ == j) return p;
+    };
+    // m: the first j with L_j >= R_j.
+    int64_t lo = 0, hi = std::min(nl, nr);
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        const int sl = slice_l(mid), sr = slice_r(mid);
+        const bool lt = sl != sr ? sl < sr : pos_l(mid) < pos_r(mid);
+        if (lt) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t m = lo;
+    if (m == 0) return pos_l(0);
+    const int64_t cut_l = m < nl ? pos_l(m) : last, cut_r = pos_r(m - 1);
+    // Swaps: thread t takes the pairs j in [m t / T, m (t + 1) / T), walking
+    // up from L at the first and down from R at the first.
+    std::vector<int64_t> ls((size_t)T, -1), rs((size_t)T, -1);
+    run_threads(T, [&](int t) {
+        const int64_t j0 = m * t / T;
+        if (j0 < m * (t + 1) / T) {
+            ls[(size_t)t] = pos_l(j0);
+            rs[(size_t)t] = pos_r(j0);
+        }
+    });
+    run_threads(T, [&](int t) {
+        int64_t j = m * t / T;
+        const int64_t j1 = m * (t + 1) / T;
+        if (j >= j1) return;
+        int64_t l = ls[(size_t)t], r = rs[(size_t)t];
+        for (;;) {
+            std::swap(a[l], a[r]);
+            if (++j == j1) break;
+            for (++l; !is_l(l); ++l) {}
+            for (--r; !is_r(r); --r) {}
+        }
+    });
+    return std::min(cut_l, cut_r);
+}
 
 // The positions [keep, n) of a[0, n) exactly as std::sort(a, a + n, ByDegree)
 // leaves them with libstdc++ (bits/stl_algo.h: __introsort_loop with
@@ -534,8 +671,9 @@ struct ByDegree {
 // moves an element across a partition cut and the final insertion sort
 // only reorders within the ranges the partitions leave (every element of a
 // range is no greater than every element of the next).  Cost: the
-// partitions along the path to the top, about 2n, instead of n lg n.
-void introsort_top(DegPair *a, int64_t n, int64_t keep) {
+// partitions along the path to the top, about 2n, instead of n lg n; the
+// long ones run on `threads` threads (partition_par).
+void introsort_top(DegPair *a, int64_t n, int64_t keep, int threads) {
     const ByDegree less;
     std::vector<std::pair<int64_t, int64_t>> leaves;  // ranges the final insertion sort orders
     // __introsort_loop(first, last, depth), right parts first (recursion),
@@ -571,17 +709,12 @@ void introsort_top(DegPair *a, int64_t n, int64_t keep) {
                 else if (less(*y, *z)) std::iter_swap(r, z);
                 else std::iter_swap(r, y);
             }
-            int64_t lo = first + 1, hi = last;
-            const DegPair *pivot = a + first;
-            for (;;) {
-                while (less(a[lo], *pivot)) ++lo;
-                --hi;
-                while (less(*pivot, a[hi])) --hi;
-                if (!(lo < hi)) break;
-                std::iter_swap(a + lo, a + hi);
-                ++lo;
+            int64_t cut;
+            if (threads > 1 && last - first >= kParPartitionMin) {
+                cut = partition_par(a, first, last, threads);
+            } else {
+                cut = partition_seq(a, first, last);
             }
-            const int64_t cut = lo;
             // Recursion on [cut, last) runs before the loop goes on with
             // [first, cut): push the left part, process the right part now.
             stack.push_back({first, cut, depth});
@@ -609,17 +742,33 @@ void introsort_top(DegPair *a, int64_t n, int64_t keep) {
 // the total whatever the order (every partial sum is an integer below
 // 2^53), so the number of hubs is known before sorting and only the top of
 // the sorted order is produced (introsort_top).
-std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg) {
+std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads) {
     const int64_t n = (int64_t)deg.size();
     std::vector<int32_t> hubs;
     const int N = (int)n;
     if (N < 2) return hubs;
-    double total = 0.0;
+    const int T = n >= kParPartitionMin ? std::max(1, threads) : 1;
+    // Degree total (every partial sum is an integer below 2^53, so the
+    // reference's double accumulation is this exact sum) and maximum.
+    std::vector<int64_t> tsum((size_t)T, 0);
+    std::vector<int> tmax((size_t)T, 0);
+    run_threads(T, [&](int t) {
+        int64_t s = 0;
+        int mx = 0;
+        for (int64_t v = n * t / T; v < n * (t + 1) / T; ++v) {
+            s += deg[(size_t)v];
+            mx = std::max(mx, deg[(size_t)v]);
+        }
+        tsum[(size_t)t] = s;
+        tmax[(size_t)t] = mx;
+    });
+    int64_t isum = 0;
     int dmax = 0;
-    for (int64_t v = n - 1; v >= 0; --v) {
-        total += deg[(size_t)v];
-        dmax = std::max(dmax, deg[(size_t)v]);
+    for (int t = 0; t < T; ++t) {
+        isum += tsum[(size_t)t];
+        dmax = std::max(dmax, tmax[(size_t)t]);
     }
+    const double total = (double)isum;
     const double inv_N = 1.0 / N;
     const double x1 = N * inv_N;
     const double y1 = total / total;            // cum[N-1]
@@ -634,10 +783,14 @@ std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg) {
         else break;
     }
     if (M == 0) return hubs;
-    std::vector<DegPair> sd;
-    sd.reserve((size_t)n);
-    for (int64_t v = n - 1; v >= 0; --v) sd.emplace_back((int)v, deg[(size_t)v]);
-    introsort_top(sd.data(), n, n - M);
+    std::unique_ptr<DegPair[]> sd(new DegPair[(size_t)n]);  // (id, degree), descending id
+    run_threads(T, [&](int t) {
+        for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i) {
+            const int64_t v = n - 1 - i;
+            sd[(size_t)i] = DegPair{(int)v, deg[(size_t)v]};
+        }
+    });
+    introsort_top(sd.get(), n, n - M, T);
     hubs.reserve((size_t)M);
     for (int64_t i = n - 1; i >= n - M; --i) hubs.push_back(sd[(size_t)i].first);
     return hubs;
@@ -726,10 +879,10 @@ P1Job p1_job(dmolp_ctx *ctx, dmolp_graph *g, Phase1Work &w, int32_t *remaining) 
     return j;
 }
 
-void p1_hub_search(P1Stage &h) {
+void p1_hub_search(P1Stage &h, int threads) {
     const auto t0 = std::chrono::steady_clock::now();
     h.fs = first_seed_host(h.deg);
-    h.hubs = find_hubs_host(h.deg);
+    h.hubs = find_hubs_host(h.deg, threads);
     const auto t1 = std::chrono::steady_clock::now();
     h.host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     // Candidates of the seed fallback: the first 2k vertices by (degree
@@ -742,21 +895,30 @@ void p1_hub_search(P1Stage &h) {
         if (deg[(size_t)a] != deg[(size_t)b]) return deg[(size_t)a] > deg[(size_t)b];
         return a < b;
     };
-    // Bounded heap with the worst kept candidate on top: one pass over n.
-    std::vector<int32_t> heap;
-    heap.reserve(m);
-    for (int64_t v = 0; v < n; ++v) {
-        if (heap.size() < m) {
-            heap.push_back((int32_t)v);
-            std::push_heap(heap.begin(), heap.end(), better);
-        } else if (m > 0 && better((int32_t)v, heap.front())) {
-            std::pop_heap(heap.begin(), heap.end(), better);
-            heap.back() = (int32_t)v;
-            std::push_heap(heap.begin(), heap.end(), better);
+    // Bounded heaps with the worst kept candidate on top, one pass over a
+    // slice of the ids each; the order is total (ids break ties), so the
+    // first m of the slices' candidates together are the first m overall.
+    const int T = n >= (int64_t(1) << 20) ? std::max(1, threads) : 1;
+    std::vector<std::vector<int32_t>> part((size_t)T);
+    run_threads(T, [&](int t) {
+        std::vector<int32_t> &heap = part[(size_t)t];
+        heap.reserve(m);
+        for (int64_t v = n * t / T; v < n * (t + 1) / T; ++v) {
+            if (heap.size() < m) {
+                heap.push_back((int32_t)v);
+                std::push_heap(heap.begin(), heap.end(), better);
+            } else if (m > 0 && better((int32_t)v, heap.front())) {
+                std::pop_heap(heap.begin(), heap.end(), better);
+                heap.back() = (int32_t)v;
+                std::push_heap(heap.begin(), heap.end(), better);
+            }
         }
-    }
-    std::sort_heap(heap.begin(), heap.end(), better);
-    h.top = std::move(heap);
+    });
+    std::vector<int32_t> all;
+    for (const auto &p : part) all.insert(all.end(), p.begin(), p.end());
+    std::sort(all.begin(), all.end(), better);
+    if (all.size() > m) all.resize(m);
+    h.top = std::move(all);
 }
 
 namespace {
@@ -764,11 +926,10 @@ namespace {
 // Hubs and first seed of every job on the host; jobs in parallel threads.
 void host_hub_search(std::vector<P1Stage> &H) {
     const size_t G = H.size();
-    size_t T = std::thread::hardware_concurrency();
-    if (T > 16) T = 16;
+    size_t T = (size_t)std::min(16, host_threads());
     if (T > G) T = G;
-    if (T <= 1) {
-        for (size_t j = 0; j < G; ++j) p1_hub_search(H[j]);
+    if (T <= 1) {  // one job: its passes and long partitions use the threads
+        for (size_t j = 0; j < G; ++j) p1_hub_search(H[j], std::min(16, host_threads()));
         return;
     }
     std::vector<std::thread> pool;
@@ -993,7 +1154,7 @@ uint64_t p1_rounds_key(const P1SetDev &sd, int rounds) {
 void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds) {
     const uint64_t key = p1_rounds_key(sd, rounds);
     RoundGraph &rg = sd.rounds;
-    if (ctx->profiling || (!(rg.exec && rg.key == key) && sd.seen_key != key)) {
+    if (ctx->profiling || !graphs_enabled() || (!(rg.exec && rg.key == key) && sd.seen_key != key)) {
         sd.seen_key = key;
         p1_expand_enqueue(ctx, sd, rounds);
         return;

@@ -79,7 +79,7 @@ struct P1Stage {
 void p1_stage_init(dmolp_ctx *ctx, P1Item &it, P1Stage &h);
 void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);
 void p1_degrees(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);  // degrees to the host
-void p1_hub_search(P1Stage &h);                                           // host: hubs, first seed
+void p1_hub_search(P1Stage &h, int threads = 1);                                           // host: hubs, first seed
 void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);    // hub-ball seeds + fallback
 void p1_expand_begin(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);
 void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds);          // rounds: even
@@ -88,7 +88,9 @@ void p1_done_flags(dmolp_ctx *ctx, P1SetDev &sd, std::vector<int32_t> &out);
 void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double device_ms);
 
 Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
-std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg);
+// threads: host threads for the long partition steps and passes (1: all
+// on the calling thread); the result does not depend on it.
+std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads = 1);
 std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg);
 // Seeds + expansion; labels become resident in g.  Metrics fields about
 // seeds, rounds and timing are filled (cut and counts come from the stats

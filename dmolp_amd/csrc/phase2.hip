@@ -2516,7 +2516,7 @@ uint64_t phase2_single_key(dmolp_ctx *ctx, dmolp_graph *g, Phase2Work &w) {
 }
 
 void phase2_round_launch(dmolp_ctx *ctx, const P2Set &set, RoundGraph &rg, uint64_t key) {
-    if (ctx->profiling) {  // events between kernels: launch directly
+    if (ctx->profiling || !graphs_enabled()) {  // events between kernels, or no graphs: launch directly
         phase2_round_enqueue(ctx, set);
         return;
     }

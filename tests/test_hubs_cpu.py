@@ -53,6 +53,31 @@ def test_host_hubs_match_full_sort(name, deg):
     assert np.array_equal(want, got), (name, len(want), len(got))
 
 
+def long_sequences():
+    """Above 2^20 entries the long partition steps run on several threads
+    (the test process's host threads): the result must not change."""
+    rng = np.random.default_rng(77)
+    n = 1_300_000
+    pl = np.floor(3000.0 / (np.arange(n) + 1.0) ** 0.7).astype(np.int64)
+    return [
+        ("power-law-ids", pl),  # hubs at low ids: the descending-id order ends with them
+        ("power-law-shuffled", rng.permutation(pl)),
+        ("zipf", np.minimum(rng.zipf(1.6, 2_400_000), 1_000_000)),
+        ("few-values", rng.choice([1, 2, 2, 3, 40, 40, 900], 1_700_000)),
+        ("descending", (1_100_000 - np.arange(1_100_000)) // 3),
+        ("organ-pipe", np.concatenate([np.arange(700_000), np.arange(700_000)[::-1]]) % 5003),
+    ]
+
+
+@pytest.mark.parametrize("name,deg", long_sequences(), ids=lambda x: x if isinstance(x, str) else "")
+def test_host_hubs_long_sequences(name, deg):
+    deg = np.asarray(deg, np.int32)
+    want = oracle_hubs(deg)
+    got = api.hubs_host(deg)
+    assert len(want) > 0
+    assert np.array_equal(want, got), (name, len(want), len(got))
+
+
 def test_host_hubs_random_sweep():
     rng = np.random.default_rng(2024)
     for trial in range(300):
