@@ -30,6 +30,7 @@ SOURCES = [
     "server.hip",
     "halo.hip",
     "noalloc.hip",
+    "opbatch.hip",
     "gen_host.cpp",
     "textio.cpp",
     "driver/driver.cpp",

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "opbatch.hpp"
 #include "phase1.hpp"
 #include "phase2.hpp"
 
@@ -113,6 +114,7 @@ void p2_collect(dmolp_ctx *ctx, std::vector<P2Item *> &items) {
     hipStream_t s = ctx->stream;
     const int G = (int)items.size();
     std::vector<Ctl> cs((size_t)G);
+    OpBatch ob;  // the jobs' results in one copy
     std::vector<std::vector<int64_t>> cv((size_t)G), ci((size_t)G);
     for (int j = 0; j < G; ++j) {
         P2Item &it = *items[(size_t)j];
@@ -120,13 +122,11 @@ void p2_collect(dmolp_ctx *ctx, std::vector<P2Item *> &items) {
         const int k = it.k;
         cv[(size_t)j].assign((size_t)k, 0);
         ci[(size_t)j].assign((size_t)k, 0);
-        DMOLP_HIP_CHECK(hipMemcpyAsync(&cs[(size_t)j], w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));
-        DMOLP_HIP_CHECK(hipMemcpyAsync(cv[(size_t)j].data(), w.vc.p, sizeof(int64_t) * (size_t)k,
-                                       hipMemcpyDeviceToHost, s));
-        DMOLP_HIP_CHECK(hipMemcpyAsync(ci[(size_t)j].data(), w.ic.p, sizeof(int64_t) * (size_t)k,
-                                       hipMemcpyDeviceToHost, s));
+        ob.download(&cs[(size_t)j], w.ctl.p, sizeof(Ctl));
+        ob.download(cv[(size_t)j].data(), w.vc.p, sizeof(int64_t) * (size_t)k);
+        ob.download(ci[(size_t)j].data(), w.ic.p, sizeof(int64_t) * (size_t)k);
     }
-    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    ob.run(ctx);
     for (int j = 0; j < G; ++j) {
         P2Item &it = *items[(size_t)j];
         const Ctl &c = cs[(size_t)j];

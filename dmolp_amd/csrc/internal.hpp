@@ -173,6 +173,8 @@ struct P1SetDev {
     DevMem jobs, sel, done, remaining;
     RoundGraph rounds;
     uint64_t seen_key = 0;
+    int kmax = 0;
+    DevMem run_out;  // results of p1_rounds_wg
     P1SetDev() = default;
     P1SetDev(const P1SetDev &) = delete;
     P1SetDev &operator=(const P1SetDev &) = delete;
@@ -197,6 +199,7 @@ struct dmolp_ctx {
     std::string last_error;
     dmolp::Tuning tune;
     dmolp::DevMem scratch;  // small per-call scratch (host<->device staging)
+    dmolp::DevMem ops_dev;  // staging of OpBatch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Kernel-group timing of the last phase-2 run (profiling on): events
     // recorded around each group every round.

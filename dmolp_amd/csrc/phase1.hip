@@ -28,6 +28,7 @@
 #include "rows.hpp"
 #include "select.hpp"
 #include "gen_host.hpp"
+#include "opbatch.hpp"
 
 namespace dmolp {
 namespace {
@@ -151,9 +152,15 @@ __global__ void k_ball2_heavy(const P1Job *JS, const int32_t *bstart, int jobs, 
     }
 }
 
-// seed.cpp:158-170: over unused, uncovered hubs, the largest degree, first
-// in hub order; key = degree << 32 | ~index.
-__global__ void k_next_seed(const P1Job *JS, const int32_t *bstart, int jobs, int step) {
+// The next seed (seed.cpp:154-169): the unused hub outside every ball with
+// the largest degree, ties to the earliest in hub order (key = degree << 32
+// | ~index).  The hub list
+// holds the top of the degree order read downward, so its degrees never
+// rise: that hub is the first unused, uncovered one.  Used and covered
+// hubs stay so, so the search resumes where the last one stopped.  One
+// block per job; the key (degree, position) goes to k_take_seed.
+__global__ __launch_bounds__(kThreads) void k_next_seed(const P1Job *JS, const int32_t *bstart, int jobs,
+                                                        int step) {
     P1_JOB();
     const GraphDev g = J.g;
     const int32_t *hubs = J.hubs;
@@ -161,25 +168,32 @@ __global__ void k_next_seed(const P1Job *JS, const int32_t *bstart, int jobs, in
     const uint8_t *used = J.used;
     const uint32_t *covered = J.covered;
     unsigned long long *best = J.best;
-    const P1Ctl *ctl = J.ctl;
+    P1Ctl *ctl = J.ctl;
     if (step >= J.seed_steps) return;
     if (ctl->seed_stop) return;
-    unsigned long long mine = 0;
-    for (int64_t i = vthread(vg); i < nh;
-         i += vthreads(vg)) {
-        if (used[i]) continue;
-        const int32_t h = hubs[i];
-        if (covered[h >> 5] & (1u << (h & 31))) continue;
-        const unsigned long long d = (unsigned long long)(g.off[h + 1] - g.off[h]);
-        const unsigned long long key = (d << 32) | (0xffffffffull - (unsigned long long)i);
-        if (key > mine) mine = key;
+    if (vg.block != 0) return;
+    __shared__ int64_t s_first;
+    for (int64_t i0 = ctl->seed_cursor; i0 < nh; i0 += blockDim.x) {
+        if (threadIdx.x == 0) s_first = nh;
+        __syncthreads();
+        const int64_t i = i0 + threadIdx.x;
+        if (i < nh && !used[i]) {
+            const int32_t h = hubs[i];
+            if (!(covered[h >> 5] & (1u << (h & 31)))) atomicMin((unsigned long long *)&s_first, (unsigned long long)i);
+        }
+        __syncthreads();
+        const int64_t f = s_first;
+        if (f < nh) {
+            if (threadIdx.x == 0) {
+                const int32_t h = hubs[f];
+                const unsigned long long d = (unsigned long long)(g.off[h + 1] - g.off[h]);
+                *best = (d << 32) | (0xffffffffull - (unsigned long long)f);
+                ctl->seed_cursor = (int32_t)f;
+            }
+            return;
+        }
+        __syncthreads();  // s_first is reset by thread 0 next
     }
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const unsigned long long other = __shfl_xor(mine, o, kWave);
-        if (other > mine) mine = other;
-    }
-    if (lane_id() == 0 && mine) atomicMax(best, mine);
 }
 
 __global__ void k_take_seed(const P1Job *JS, const int32_t *bstart, int jobs, int step) {
@@ -489,6 +503,219 @@ __global__ void k_p1_round_end(const P1Job *JS, const int32_t *bstart, int jobs)
     P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
     ctl->rounds += 1;
+}
+
+// ------------------------------------------------------- rounds in one workgroup
+
+// Rounds in which a job's every part keeps its whole frontier (at most theta
+// members per part, so the per-part selection takes them all,
+// partition.cpp:419-425), every member's row is light and the frontier has
+// at most kWgMaxMembers members, run for that job by one workgroup: its
+// begin step, fill, claims and their application, separated by workgroup
+// barriers only (a grid-wide barrier costs more than such a round).  Jobs
+// are independent, one block each.  A job stops before the first round that
+// needs a real selection, a heavy row or more row entries than one
+// workgroup gets through faster than the separate launches, and leaves it, with
+// its frontier moved back to F[0], to the launches of p1_expand_rounds,
+// which start every batch there.  The state left after every round is what
+// those launches leave.  Claims and labels, which other waves change within
+// the launch, are read and written with device-scope atomic operations
+// (they bypass the vector L1).
+constexpr int kWgThreads = 1024;
+constexpr int kWgWaves = kWgThreads / kWave;
+constexpr int64_t kWgMaxMembers = 16384;  // frontier members of a round run here
+constexpr int64_t kWgMaxEntries = 32768;  // and their row entries
+
+__device__ __forceinline__ int32_t ld_rel(const int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(int32_t *p, int32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// out[0]: largest number of rounds a job ran here; out[1]: jobs that stopped
+// at a round these blocks cannot run.
+__global__ __launch_bounds__(kWgThreads) void k_p1_rounds_wg(const P1Job *JS, int max_rounds, int32_t *out) {
+    __shared__ P1Job s_job;
+    __shared__ int32_t s_cnt[2][DMOLP_MAX_K];   // per-part frontier counts by parity
+    __shared__ unsigned long long s_sz[DMOLP_MAX_K];
+    __shared__ BatchLds s_b[kWgWaves];
+    __shared__ int32_t s_maxdeg[2];
+    __shared__ unsigned long long s_entries[2];  // row entries of the frontier by parity
+    __shared__ int32_t s_state;
+    __shared__ unsigned long long s_touched, s_next, s_added;
+    const int tid = threadIdx.x, w = tid / kWave, lane = lane_id();
+    {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(JS + blockIdx.x);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(&s_job);
+        for (int i = tid; i < (int)(sizeof(P1Job) / 4); i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    const P1Job &J = s_job;
+    P1Ctl *ctl = J.ctl;
+    const int k = J.k;
+    const int64_t n = J.g.n;
+    if (ctl->done) return;  // uniform: nothing writes it before this point
+    if ((int64_t)*J.F[0].cnt > kWgMaxMembers) {  // too large to count here
+        if (tid == 0) atomicAdd(&out[1], 1);
+        return;
+    }
+    // The entry frontier (F[0]): per-part counts and largest degree.
+    for (int p = tid; p < k; p += blockDim.x) {
+        s_cnt[0][p] = 0;
+        s_cnt[1][p] = 0;
+        s_sz[p] = J.sizes[p];
+    }
+    if (tid < 2) {
+        s_maxdeg[tid] = 0;
+        s_entries[tid] = 0;
+    }
+    __syncthreads();
+    {
+        const RecDev f = J.F[0];
+        const int64_t c = (int64_t)*f.cnt;
+        for (int64_t i = tid; i < c; i += blockDim.x) {
+            atomicAdd(&s_cnt[0][f.g[i]], 1);
+            atomicMax(&s_maxdeg[0], f.d[i]);
+            atomicAdd(&s_entries[0], (unsigned long long)f.d[i]);
+        }
+    }
+    __syncthreads();
+    int t = 0, st = 0;
+    for (; t < max_rounds; ++t) {
+        const int par = t & 1;
+        const RecDev cur = J.F[par], nxt = J.F[par ^ 1];
+        // Begin (k_p1_round_begin) and whether the round runs here:
+        // 0 runs, 1 stops (the round is left to the separate launches),
+        // 2 the job is finished, 3 fill round.
+        if (tid == 0) {
+            int s = 0;
+            int b = -1;
+            if (ctl->labelled >= n) {
+                ctl->done = 1;
+                atomicSub(J.remaining, 1);
+                s = 2;
+            } else {
+                const int64_t c = (int64_t)*cur.cnt;
+                if (c == 0) {
+                    b = 0;
+                    for (int p = 0; p < k; ++p)
+                        if (s_sz[p] < s_sz[b] || (s_sz[p] == s_sz[b] && p < b)) b = p;
+                    s = 3;
+                } else if (c > kWgMaxMembers || s_entries[par] > (unsigned long long)kWgMaxEntries ||
+                           s_maxdeg[par] > J.heavy_degree) {
+                    s = 1;
+                } else {
+                    for (int p = 0; p < k; ++p)
+                        if (s_cnt[par][p] > J.theta) s = 1;
+                }
+                if (s != 1) {
+                    *nxt.cnt = 0;
+                    for (int i = 0; i < LC_N; ++i) J.sel_list.cnt[i] = 0;
+                    *J.touched_cnt = 0;
+                    ctl->fill_part = b;
+                    if (s == 3 && ctl->fill_round < 0) ctl->fill_round = (int32_t)ctl->rounds;
+                }
+            }
+            s_state = s;
+            s_touched = 0;
+            s_next = 0;
+            s_added = 0;
+            s_maxdeg[par ^ 1] = 0;
+            s_entries[par ^ 1] = 0;
+        }
+        for (int p = tid; p < k; p += blockDim.x) s_cnt[par ^ 1][p] = 0;
+        __syncthreads();
+        st = s_state;
+        if (st == 1 || st == 2) break;
+        if (st == 3) {
+            // Fill (k_p1_fill): every unlabelled vertex joins part b; nothing
+            // is left to claim this round.
+            const int b = ctl->fill_part;
+            for (int64_t b0 = (int64_t)w * kWave; b0 < n; b0 += (int64_t)kWgWaves * kWave) {
+                const int64_t v = b0 + lane;
+                const bool un = v < n && ld_rel(J.L + v) == -1;
+                const int64_t pos = wave_append(un, cur.cnt);
+                if (un) {
+                    st_rel(J.L + v, b);
+                    cur.x[pos] = (int32_t)v;
+                    cur.g[pos] = b;
+                    cur.d[pos] = (int32_t)(J.g.off[v + 1] - J.g.off[v]);
+                }
+                const uint64_t m = __ballot(un);
+                if (lane == 0 && m) atomicAdd(&s_added, (unsigned long long)__popcll(m));
+            }
+            __syncthreads();
+            if (tid == 0) {
+                s_sz[b] += s_added;
+                atomicAdd((unsigned long long *)&ctl->labelled, s_added);
+            }
+        } else {
+            // Claims (k_expand_light) of every member.
+            const int64_t c = (int64_t)*cur.cnt;
+            for (int64_t b0 = (int64_t)w * kWave; b0 < c; b0 += (int64_t)kWgWaves * kWave) {
+                int32_t v, deg;
+                bool valid;
+                const int32_t total = batch_load(J.g, cur.x, c, b0, s_b[w], v, deg, valid);
+                s_b[w].aux[lane] = valid ? ld_rel(J.L + v) : 0;
+                wave_sync();
+                for (int32_t q0 = 0; q0 < total; q0 += kWave) {
+                    const int32_t q = q0 + lane;
+                    if (q < total) {
+                        const int slot = batch_slot(s_b[w], q);
+                        const int32_t u = J.g.col[s_b[w].base[slot] + q];
+                        const int32_t p = s_b[w].aux[slot];
+                        if (ld_rel(J.L + u) == -1 && ld_rel(J.claims + u) > p &&
+                            atomicMin(J.claims + u, p) == kNoClaim)
+                            J.touched[atomicAdd(&s_touched, 1ull)] = u;
+                    }
+                }
+                wave_sync();
+            }
+            __syncthreads();
+            // Application (k_p1_apply).
+            const int64_t nt = (int64_t)s_touched;
+            for (int64_t i = tid; i < nt; i += blockDim.x) {
+                const int32_t v = J.touched[i];
+                const int32_t p = ld_rel(J.claims + v);
+                st_rel(J.claims + v, kNoClaim);
+                st_rel(J.L + v, p);
+                const int32_t d = (int32_t)(J.g.off[v + 1] - J.g.off[v]);
+                const unsigned long long pos = atomicAdd(&s_next, 1ull);
+                nxt.x[pos] = v;
+                nxt.g[pos] = p;
+                nxt.d[pos] = d;
+                atomicAdd(&s_cnt[par ^ 1][p], 1);
+                atomicAdd(&s_sz[p], 1ull);
+                atomicMax(&s_maxdeg[par ^ 1], d);
+                atomicAdd(&s_entries[par ^ 1], (unsigned long long)d);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                *nxt.cnt = s_next;
+                *J.touched_cnt = nt;
+                atomicAdd((unsigned long long *)&ctl->labelled, (unsigned long long)nt);
+            }
+        }
+        if (tid == 0) ctl->rounds += 1;  // k_p1_round_end
+    }
+    __syncthreads();
+    // Leave the frontier in F[0] for the separate launches.
+    if ((t & 1) && st != 2) {
+        const RecDev src = J.F[1], dst = J.F[0];
+        const int64_t c = (int64_t)*src.cnt;
+        for (int64_t i = tid; i < c; i += blockDim.x) {
+            dst.x[i] = src.x[i];
+            dst.g[i] = src.g[i];
+            dst.d[i] = src.d[i];
+        }
+        if (tid == 0) *dst.cnt = (unsigned long long)c;
+    }
+    for (int p = tid; p < k; p += blockDim.x) J.sizes[p] = s_sz[p];
+    if (tid == 0) {
+        atomicMax(&out[0], t);
+        if (st == 1) atomicAdd(&out[1], 1);
+    }
 }
 
 // random.cpp:71-74: label = global id mod k.
@@ -956,6 +1183,7 @@ void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     const int G = (int)H.size();
     hipStream_t s = ctx->stream;
     sd.G = G;
+    sd.kmax = 0;
     sd.any_heavy = false;
     sd.passes = 0;
     sd.max_n = 0;
@@ -971,6 +1199,7 @@ void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
             h.have_job = true;
         }
         h.job.remaining = sd.remaining.as<int32_t>();
+        h.job.theta = h.theta;
         jobs[(size_t)j] = h.job;
         P1Ctl *ctl = w.ctl.as<P1Ctl>();
         TakeRule rule{};
@@ -982,6 +1211,7 @@ void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
         sd.any_heavy = sd.any_heavy || h.g->n_heavy > 0;
         sd.passes = std::max(sd.passes, w.sel.passes);
         sd.max_n = std::max(sd.max_n, h.g->n);
+        sd.kmax = std::max(sd.kmax, h.k);
     }
     launch_set_build(ctx, sd.ls, blocks);
     sd.jobs.grow(sizeof(P1Job) * (size_t)G);
@@ -1012,26 +1242,29 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     hipStream_t s = ctx->stream;
     const int G = (int)H.size();
     int steps = 1;
+    OpBatch ob;  // the jobs' clears and uploads together
     for (P1Stage *hp : H) {
         P1Stage &h = *hp;
         Phase1Work &w = *h.w;
         const int k = h.k;
         const int64_t nh = (int64_t)h.hubs.size();
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.ctl.p, 0, sizeof(P1Ctl), s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.covered.p, 0, w.covered.bytes, s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.best.p, 0, w.best.bytes, s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(w.seeds.p, 0xff, w.seeds.bytes, s));
-        DMOLP_HIP_CHECK(hipMemcpyAsync(w.seeds.p, &h.fs.first, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        ob.fill(w.ctl.p, 0, sizeof(P1Ctl));
+        ob.fill(w.covered.p, 0, w.covered.bytes);
+        ob.fill(w.best.p, 0, w.best.bytes);
+        ob.fill(w.seeds.p, 0xff, w.seeds.bytes);
+        ob.upload(w.seeds.p, &h.fs.first, sizeof(int32_t));
+        for (int64_t i = 1; i < nh; ++i)  // k_next_seed relies on the order
+            DMOLP_REQUIRE(h.deg[(size_t)h.hubs[(size_t)i]] <= h.deg[(size_t)h.hubs[(size_t)i - 1]],
+                          DMOLP_E_INTERNAL, "hub degrees out of order");
         h.used.assign((size_t)(nh > 0 ? nh : 1), 0);
         for (int64_t i = 0; i < nh; ++i)
             if (h.hubs[(size_t)i] == h.fs.first) { h.used[(size_t)i] = 1; break; }
         w.hubs.grow(sizeof(int32_t) * (size_t)(nh > 0 ? nh : 1));
         w.used.grow((size_t)(nh > 0 ? nh : 1));
         if (nh > 0) {
-            DMOLP_HIP_CHECK(hipMemcpyAsync(w.hubs.p, h.hubs.data(), sizeof(int32_t) * (size_t)nh,
-                                           hipMemcpyHostToDevice, s));
-            DMOLP_HIP_CHECK(hipMemcpyAsync(w.used.p, h.used.data(), (size_t)nh, hipMemcpyHostToDevice, s));
-            DMOLP_HIP_CHECK(hipMemsetAsync(w.lvl1.cnt.p, 0, w.lvl1.cnt.bytes, s));
+            ob.upload(w.hubs.p, h.hubs.data(), sizeof(int32_t) * (size_t)nh);
+            ob.upload(w.used.p, h.used.data(), (size_t)nh);
+            ob.fill(w.lvl1.cnt.p, 0, w.lvl1.cnt.bytes);
         }
         h.job.hubs = w.hubs.as<int32_t>();
         h.job.used = w.used.as<uint8_t>();
@@ -1039,6 +1272,7 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
         h.job.seed_steps = nh > 0 ? k : 1;
         steps = std::max(steps, h.job.seed_steps);
     }
+    ob.run(ctx);
     p1_set_build(ctx, H, sd);  // uploads the hub fields
     const LaunchSet &ls = sd.ls;
     P1Job *d_jobs = sd.jobs.as<P1Job>();
@@ -1048,7 +1282,7 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
         hipLaunchKernelGGL(k_ball2_light, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
         if (sd.any_heavy)
             hipLaunchKernelGGL(k_ball2_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);
-        hipLaunchKernelGGL(k_next_seed, dim3(ls.total[P_HIST1]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_HIST1), G, step);
+        hipLaunchKernelGGL(k_next_seed, dim3(ls.total[P_ONE]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_ONE), G, step);
         hipLaunchKernelGGL(k_take_seed, dim3(ls.total[P_ONE]), dim3(1), 0, s, d_jobs, ls.bs(P_ONE), G, step);
     }
     DMOLP_HIP_CHECK(hipGetLastError());
@@ -1056,11 +1290,10 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     for (int j = 0; j < G; ++j) {
         P1Stage &h = *H[(size_t)j];
         h.seeds.assign((size_t)h.k, -1);
-        DMOLP_HIP_CHECK(hipMemcpyAsync(h.seeds.data(), h.w->seeds.p, sizeof(int32_t) * h.seeds.size(),
-                                       hipMemcpyDeviceToHost, s));
-        DMOLP_HIP_CHECK(hipMemcpyAsync(&hc[(size_t)j], h.w->ctl.p, sizeof(P1Ctl), hipMemcpyDeviceToHost, s));
+        ob.download(h.seeds.data(), h.w->seeds.p, sizeof(int32_t) * h.seeds.size());
+        ob.download(&hc[(size_t)j], h.w->ctl.p, sizeof(P1Ctl));
     }
-    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    ob.run(ctx);
     for (int j = 0; j < G; ++j) {
         P1Stage &h = *H[(size_t)j];
         const int k = h.k;
@@ -1084,20 +1317,21 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
             }
             DMOLP_REQUIRE(at == k, DMOLP_E_HIP, "seed fallback ran out of candidates");
             h.fallback = k - found;
-            DMOLP_HIP_CHECK(hipMemcpyAsync(h.w->seeds.p, h.seeds.data(), sizeof(int32_t) * (size_t)k,
-                                           hipMemcpyHostToDevice, s));
+            ob.upload(h.w->seeds.p, h.seeds.data(), sizeof(int32_t) * (size_t)k);
         }
     }
-    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    ob.run(ctx);
 }
 
 void p1_expand_begin(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     hipStream_t s = ctx->stream;
+    OpBatch ob;
     for (P1Stage *h : H) {
-        DMOLP_HIP_CHECK(hipMemsetAsync(h->g->labels.p, 0xff, sizeof(int32_t) * (size_t)h->g->n, s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(h->w->sizes.p, 0, h->w->sizes.bytes, s));
-        DMOLP_HIP_CHECK(hipMemsetAsync(h->w->ctl.p, 0, sizeof(P1Ctl), s));
+        ob.fill(h->g->labels.p, 0xff, sizeof(int32_t) * (size_t)h->g->n);
+        ob.fill(h->w->sizes.p, 0, h->w->sizes.bytes);
+        ob.fill(h->w->ctl.p, 0, sizeof(P1Ctl));
     }
+    ob.run(ctx);
     hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, sd.remaining.as<int32_t>(), (int32_t)sd.G);
     hipLaunchKernelGGL(k_expand_init, dim3(sd.ls.total[P_ONE]), dim3(256), 0, s, sd.jobs.as<P1Job>(),
                        sd.ls.bs(P_ONE), sd.G);
@@ -1182,6 +1416,25 @@ void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds) {
     DMOLP_HIP_CHECK(hipGraphLaunch(rg.exec, ctx->stream));
 }
 
+// Up to max_rounds (even) rounds of every job of the set that k_p1_rounds_wg
+// can run, one block per job.  ran: the most rounds a job ran; stopped: jobs
+// that stopped at a round the separate launches must run.
+void p1_rounds_wg(dmolp_ctx *ctx, P1SetDev &sd, int max_rounds, int &ran, int &stopped) {
+    ran = 0;
+    stopped = 0;
+    if (sd.G < 1) return;
+    sd.run_out.grow(sizeof(int32_t) * 2);
+    DMOLP_HIP_CHECK(hipMemsetAsync(sd.run_out.p, 0, sizeof(int32_t) * 2, ctx->stream));
+    hipLaunchKernelGGL(k_p1_rounds_wg, dim3(sd.G), dim3(kWgThreads), 0, ctx->stream, sd.jobs.as<P1Job>(),
+                       max_rounds, sd.run_out.as<int32_t>());
+    DMOLP_HIP_CHECK(hipGetLastError());
+    int32_t out[2] = {0, 0};
+    DMOLP_HIP_CHECK(hipMemcpyAsync(out, sd.run_out.p, sizeof(out), hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ran = out[0];
+    stopped = out[1];
+}
+
 int32_t p1_remaining(dmolp_ctx *ctx, P1SetDev &sd) {
     int32_t left = 0;
     DMOLP_HIP_CHECK(hipMemcpyAsync(&left, sd.remaining.p, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -1198,10 +1451,14 @@ void p1_done_flags(dmolp_ctx *ctx, P1SetDev &sd, std::vector<int32_t> &out) {
     DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
-void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double device_ms) {
+void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double device_ms, const P1Ctl *ctl) {
     P1Ctl hc{};
-    DMOLP_HIP_CHECK(hipMemcpyAsync(&hc, h.w->ctl.p, sizeof(P1Ctl), hipMemcpyDeviceToHost, ctx->stream));
-    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ctl) {
+        hc = *ctl;
+    } else {
+        DMOLP_HIP_CHECK(hipMemcpyAsync(&hc, h.w->ctl.p, sizeof(P1Ctl), hipMemcpyDeviceToHost, ctx->stream));
+        DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
     std::memset(&m, 0, sizeof(m));
     m.first_seed = h.fs.first;
     m.first_seed_degree = h.fs.second;
@@ -1244,19 +1501,65 @@ void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items) {
     p1_seeds(ctx, HP, sd);
     p1_expand_begin(ctx, HP, sd);
     int64_t enq = 0;
-    const int batch = 16;
+    const int batch = 16;        // even: every batch starts on the same frontier parity
+    const int run_rounds = 4096;  // rounds of one k_p1_rounds_wg launch (even, for the same reason)
+    std::vector<P1Stage *> act = HP;
+    std::vector<int32_t> done;
+    int skip = 0, backoff = 1;
     for (;;) {
-        p1_expand_rounds(ctx, sd, batch);
-        enq += batch;
-        if (p1_remaining(ctx, sd) <= 0) break;
+        // Rounds that need no selection run in one workgroup per job; a round
+        // that does (and those after it, for a batch) runs as separate
+        // launches.  After an attempt in which no job ran a round, the next
+        // ones wait for 1, 2, 4, up to 16 batches.
+        int ran = 0, stopped = 1;
+        if (skip > 0) {
+            --skip;
+        } else {
+            p1_rounds_wg(ctx, sd, run_rounds, ran, stopped);
+            if (ran == 0) {
+                skip = backoff;
+                backoff = std::min(2 * backoff, 16);
+            } else {
+                backoff = 1;
+            }
+        }
+        enq += ran;
+        int32_t left = p1_remaining(ctx, sd);
+        if (left <= 0) break;
+        if (stopped > 0) {
+            p1_expand_rounds(ctx, sd, batch);
+            enq += batch;
+            left = p1_remaining(ctx, sd);
+            if (left <= 0) break;
+        }
         DMOLP_REQUIRE(enq < 64ll * (sd.max_n + 16), DMOLP_E_HIP, "phase-1 expansion did not finish");
+        // A finished job keeps its blocks in every launch (they return at
+        // once) and its selection pass count: once half of the set has
+        // finished, the set is rebuilt from the jobs still expanding.
+        if (act.size() > 1 && (size_t)left * 2 <= act.size()) {
+            p1_done_flags(ctx, sd, done);
+            std::vector<P1Stage *> keep;
+            for (size_t j = 0; j < act.size(); ++j)
+                if (!done[j]) keep.push_back(act[j]);
+            DMOLP_REQUIRE((int32_t)keep.size() == left, DMOLP_E_INTERNAL,
+                          "phase-1 job count disagrees with the done flags");
+            act.swap(keep);
+            p1_set_build(ctx, act, sd);
+            hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, sd.remaining.as<int32_t>(),
+                               (int32_t)act.size());
+            DMOLP_HIP_CHECK(hipGetLastError());
+        }
     }
     DMOLP_HIP_CHECK(hipEventRecord(ctx->ev1, s));
     DMOLP_HIP_CHECK(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     DMOLP_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    std::vector<P1Ctl> hc((size_t)G);
+    OpBatch ob;
+    for (int j = 0; j < G; ++j) ob.download(&hc[(size_t)j], H[(size_t)j].w->ctl.p, sizeof(P1Ctl));
+    ob.run(ctx);
     for (int j = 0; j < G; ++j) {
-        p1_finish(ctx, H[(size_t)j], items[(size_t)j].m, ms);
+        p1_finish(ctx, H[(size_t)j], items[(size_t)j].m, ms, &hc[(size_t)j]);
         items[(size_t)j].seeds = H[(size_t)j].seeds;
     }
 }

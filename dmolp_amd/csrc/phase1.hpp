@@ -16,7 +16,7 @@ struct P1Ctl {
     int64_t labelled;
     int64_t rounds;
     int32_t fill_round;
-    int32_t pad;
+    int32_t seed_cursor;  // hubs before it are used or covered
 };
 
 struct Phase1Work {
@@ -46,6 +46,7 @@ struct P1Job {
     P1Ctl *ctl;
     int32_t *remaining;  // unfinished jobs of the set (shared by the set)
     int32_t k;
+    int32_t theta;       // frontier members a part expands per round
     int32_t seed_steps;  // steps of the hub search (k, or 1 when there are no hubs)
     int32_t heavy_degree, chunk_edges;
 };
@@ -83,9 +84,13 @@ void p1_hub_search(P1Stage &h, int threads = 1);                                
 void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);    // hub-ball seeds + fallback
 void p1_expand_begin(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);
 void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds);          // rounds: even
+// Rounds that need no selection, one workgroup per job (k_p1_rounds_wg).
+void p1_rounds_wg(dmolp_ctx *ctx, P1SetDev &sd, int max_rounds, int &ran, int &stopped);
 int32_t p1_remaining(dmolp_ctx *ctx, P1SetDev &sd);
 void p1_done_flags(dmolp_ctx *ctx, P1SetDev &sd, std::vector<int32_t> &out);
-void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double device_ms);
+// ctl: the job's control block when the caller has already downloaded it.
+void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double device_ms,
+               const P1Ctl *ctl = nullptr);
 
 Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
 // threads: host threads for the long partition steps and passes (1: all
