@@ -273,7 +273,12 @@ void server_step(dmolp_server *s, int32_t *active_out, int32_t *queued_out) {
         if (j->st == JS_EXPAND) { exp.push_back(&j->p1); exp_jobs.push_back(j); }
     if (!exp.empty()) {
         p1_set_build(ctx, exp, s->expand_set);
-        p1_expand_rounds(ctx, s->expand_set, s->p1_rounds_per_step);
+        // Rounds that need no selection, one workgroup per job (a bounded
+        // number per step, so every step stays short); then, if a job
+        // stopped at a round that needs one, rounds as separate launches.
+        int ran = 0, stopped = 0;
+        p1_rounds_wg(ctx, s->expand_set, 256, ran, stopped);
+        if (stopped > 0) p1_expand_rounds(ctx, s->expand_set, s->p1_rounds_per_step);
         std::vector<int32_t> done;
         p1_done_flags(ctx, s->expand_set, done);
         for (size_t i = 0; i < exp_jobs.size(); ++i) {
