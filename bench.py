@@ -443,6 +443,47 @@ def measured_traffic(workload, kernel):
         return None
 
 
+def measured_atomics(workload, kernel):
+    """L2 atomic requests per launch of `kernel` (rocprofv3 --pmc TCC_ATOMIC_sum)
+    from the committed table, if one exists."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text())[workload][kernel]["tcc_atomic_per_launch"]
+    except (KeyError, ValueError):
+        return None
+
+
+def atomic_ceiling(ctx, rows):
+    """The device's rate of scattered 32-bit atomic adds, measured now: groups
+    of 1 and of 2 lanes add into random 64-byte rows of `rows` rows (the kept
+    light-row array's size).  requests = adds / lanes per row."""
+    out = {}
+    for share in (1, 2):
+        ms, adds = ctx.probe_atomics(rows, share)
+        out[f"lanes_per_row_{share}"] = {"adds_per_s": adds / (ms * 1e-3),
+                                         "requests_per_s": adds / share / (ms * 1e-3),
+                                         "ms": ms, "adds": adds}
+    return out
+
+
+def atomic_roofline(ctx, rows, workload, kernel, avg_ms):
+    """The Delta pass is bound by scattered atomic requests, not bytes: its L2
+    atomic requests per launch against the request rate the probe sustains."""
+    probe = atomic_ceiling(ctx, rows)
+    ceiling = max(v["requests_per_s"] for v in probe.values())
+    out = {"bound": "scattered atomic requests", "ceiling_requests_per_s": ceiling, "probe": probe}
+    req = measured_atomics(workload, kernel)
+    if req is not None:
+        achieved = req / (avg_ms * 1e-3)
+        out.update({"kernel_requests_per_launch": req, "achieved_requests_per_s": achieved,
+                    "frac": achieved / ceiling,
+                    "requests_source": "rocprofv3 --pmc TCC_ATOMIC_sum of this kernel on this "
+                                       "workload (profiles/traffic.json), not measured in this run"})
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -453,12 +494,22 @@ def main(argv=None):
     ap.add_argument("--jobs", type=int, default=96, help="served mode: jobs per arrival rate")
     ap.add_argument("--max-active", type=int, default=32, help="served mode: admitted jobs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe-atomics", action="store_true",
+                    help="only measure the scattered-atomic ceiling at the config's vertex count")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; sharding the path over several "
                                    "GPUs is not built",
                           "n_gpus": a.gpus}))
         return 2
+    if a.probe_atomics:
+        from dmolp_amd import api
+        with api.Context(0) as ctx:
+            n = CONFIGS[a.config if a.config in CONFIGS else "c3"]["gen"]
+            rows = n[1] if n[0] == "chunglu" else 1 << n[1]
+            print(json.dumps({"probe": "scattered atomic adds", "rows": rows,
+                              "result": atomic_ceiling(ctx, rows)}))
+        return 0
     if a.config in ("c4", "c5"):
         out = run_batched(a) if a.config == "c4" else run_served(a)
         print(json.dumps(out))
@@ -580,6 +631,8 @@ def main(argv=None):
     log(f"[bench] parity {json.dumps(parity)}")
 
     traffic = measured_traffic(cfg["workload"], dom)
+    atomic = atomic_roofline(ctx, g.n, cfg["workload"], dom, float(ms_r[live].mean())) \
+        if dom == "k_delta_light" else None
     out = {
         "metric": "LP edges/sec (phase-2 boundary label propagation)",
         "value": value,
@@ -611,7 +664,8 @@ def main(argv=None):
                                        if traffic is not None else None,
                      "algorithmic_bytes_per_launch": float(bytes_r[live].mean()),
                      "avg_launch_ms": float(ms_r[live].mean()),
-                     "launches": int(live.sum())},
+                     "launches": int(live.sum()),
+                     "atomic": atomic},
         # The whole phase-2 step priced with the reference algorithm's bytes
         # (recounted LP histograms, candidates pushed from B_{t-1}): what a
         # streaming implementation of that algorithm would have to move.

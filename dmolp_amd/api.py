@@ -178,6 +178,15 @@ class Context:
                                                     C.byref(n)), self._h)
         return out
 
+    def probe_atomics(self, rows: int, share: int, reps: int = 32) -> tuple[float, int]:
+        """(ms, adds) of scattered 32-bit adds into `rows` 64-byte rows, groups
+        of `share` lanes sharing a row (dmolp_probe_atomics)."""
+        ms = C.c_double()
+        adds = C.c_int64()
+        _check(_abi.lib().dmolp_probe_atomics(self._h, rows, share, reps, C.byref(ms), C.byref(adds)),
+               self._h)
+        return ms.value, adds.value
+
     def upload(self, off: np.ndarray, col: np.ndarray) -> "Graph":
         off = np.ascontiguousarray(off)
         col = np.ascontiguousarray(col, np.int32)

@@ -31,6 +31,7 @@ SOURCES = [
     "halo.hip",
     "noalloc.hip",
     "opbatch.hip",
+    "probe.hip",
     "gen_host.cpp",
     "textio.cpp",
     "driver/driver.cpp",

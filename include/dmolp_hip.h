@@ -97,6 +97,16 @@ void *dmolp_ctx_stream(dmolp_ctx *ctx);
 int dmolp_ctx_set_profiling(dmolp_ctx *ctx, int on);
 int dmolp_ctx_round_times(dmolp_ctx *ctx, int group, double *ms_out, int cap, int *n_out);
 
+/* Measurement hook, no counterpart in the reference: the device's rate of
+ * scattered 32-bit atomic adds, the ceiling of the Delta pass (which moves
+ * counts in the 64-byte kept rows of random neighbours).  Groups of `share`
+ * lanes (a divisor of 64) pick a random row of `rows` (1 to 2^32) 64-byte rows and each
+ * lane adds 1 to a word of it, `reps` times per lane (reps <= 4096);
+ * ms_out: the fastest of two timed launches; adds_out (nullable): adds per
+ * launch.  Allocates and frees rows * 64 bytes. */
+int dmolp_probe_atomics(dmolp_ctx *ctx, int64_t rows, int32_t share, int32_t reps, double *ms_out,
+                        int64_t *adds_out);
+
 /* Launch parameters the autotuner chooses; 0 means "keep". */
 typedef struct {
     int32_t heavy_degree;  /* rows longer than this are split into chunks */

@@ -1,10 +1,10 @@
-"""Phase-1 rounds that need no selection run in one persistent launch
-(k_p1_rounds_all) and hand over to the per-kernel launches at the first round
-that does.  Every run here must equal the oracle (labels, seeds, rounds, fill
-round): small theta makes runs switch between the two ways round by round
-(and leave the persistent launch after odd and even round counts), graphs
-with many components make the fill run inside it, and a batch mixes jobs that
-need selections with jobs that never do."""
+"""Phase-1 rounds that need no selection run for each job in one workgroup
+(k_p1_rounds_wg, block barriers only) and hand over to the per-kernel launches
+at the first round that does.  Every run here must equal the oracle (labels,
+seeds, rounds, fill round): small theta makes runs switch between the two ways
+round by round (and leave the workgroup rounds after odd and even round
+counts), graphs with many components make the fill run inside them, and a
+batch mixes jobs that need selections with jobs that never do."""
 from __future__ import annotations
 
 import numpy as np

@@ -342,3 +342,14 @@ def test_phase2_byte_then_word_labels_same_graph(tuned):
     for k in (16, 300, 32, 1024):
         labels0 = rng.integers(0, k, len(off) - 1).astype(np.int32)
         _check_phase2(g, off, col, k, labels0, 8, 0.01)
+
+
+def test_probe_atomics_counts_and_rejects(ctx):
+    """The atomic-rate probe reports its adds per launch (blocks x 256 lanes x
+    reps) and a positive time, and rejects shares that do not divide 64."""
+    ms, adds = ctx.probe_atomics(1 << 16, 1, reps=4)
+    assert ms > 0 and adds > 0 and adds % (256 * 4) == 0
+    ms2, adds2 = ctx.probe_atomics(1 << 16, 8, reps=4)
+    assert ms2 > 0 and adds2 == adds
+    with pytest.raises(api.DmolpError):
+        ctx.probe_atomics(1 << 16, 3, reps=4)
