@@ -52,6 +52,8 @@ int fail(dmolp_ctx *ctx, int status, const std::string &msg) {
 
 template <typename F>
 int guarded(dmolp_ctx *ctx, F &&f) {
+    // Device allocations of the call come from (and go back to) ctx's pool.
+    PoolScope scope(ctx ? &ctx->pool : tls_pool(), ctx ? ctx->stream : tls_stream());
     try {
         f();
         return DMOLP_OK;
@@ -153,6 +155,14 @@ hipEvent_t dmolp_ctx::prof_event() {
 }
 
 namespace dmolp {
+bool pool_poison() {
+    static const bool on = [] {
+        const char *e = std::getenv("DMOLP_POOL_POISON");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
 bool graphs_enabled() {
     static const bool on = [] {
         const char *e = std::getenv("DMOLP_NO_GRAPHS");
@@ -213,6 +223,8 @@ int dmolp_ctx_create(int device, dmolp_ctx **out) {
         DMOLP_HIP_CHECK(hipGetDeviceProperties(&prop, device));
         ctx->device = device;
         ctx->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        // Up to a quarter of the device's memory stays cached for reuse.
+        ctx->pool = std::make_shared<DevPool>((size_t)prop.totalGlobalMem / 4);
         DMOLP_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         DMOLP_HIP_CHECK(hipEventCreate(&ctx->ev0));
         DMOLP_HIP_CHECK(hipEventCreate(&ctx->ev1));
@@ -233,6 +245,7 @@ int dmolp_ctx_destroy(dmolp_ctx *ctx) {
     if (!ctx) return DMOLP_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->pool) ctx->pool->close();  // memory still held elsewhere is freed when released
     ctx->scratch.release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -340,6 +353,8 @@ int dmolp_graph_upload(dmolp_ctx *ctx, int64_t n, int64_t nnz, const void *offse
         return fail(ctx, DMOLP_E_ARG, "int32 offsets cannot address nnz >= 2^31");
     dmolp_graph *g = new (std::nothrow) dmolp_graph();
     if (!g) return fail(ctx, DMOLP_E_NOMEM, "host allocation failed");
+    g->pool = ctx->pool.get();
+    g->device = ctx->device;
     int rc = guarded(ctx, [&] {
         DMOLP_HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
@@ -377,6 +392,8 @@ int dmolp_graph_generate(dmolp_ctx *ctx, const dmolp_gen_spec *spec, dmolp_graph
     *out = nullptr;
     dmolp_graph *g = new (std::nothrow) dmolp_graph();
     if (!g) return fail(ctx, DMOLP_E_NOMEM, "host allocation failed");
+    g->pool = ctx->pool.get();
+    g->device = ctx->device;
     int rc = guarded(ctx, [&] {
         DMOLP_HIP_CHECK(hipSetDevice(ctx->device));
         gen_device(ctx, spec, g);
@@ -396,6 +413,7 @@ int dmolp_graph_info(const dmolp_graph *g, int64_t *n, int64_t *nnz) {
 
 int dmolp_graph_bins(dmolp_ctx *ctx, dmolp_graph *g, int64_t *heavy_rows, int64_t *heavy_entries) {
     if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     return guarded(ctx, [&] {
         graph_prepare_bins(ctx, g);
         unsigned long long c[LC_N] = {};
@@ -408,6 +426,7 @@ int dmolp_graph_bins(dmolp_ctx *ctx, dmolp_graph *g, int64_t *heavy_rows, int64_
 
 int dmolp_graph_download(dmolp_ctx *ctx, const dmolp_graph *g, int64_t *off, int32_t *col) {
     if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     return guarded(ctx, [&] {
         if (off)
             DMOLP_HIP_CHECK(hipMemcpyAsync(off, g->off.p, sizeof(int64_t) * (size_t)(g->n + 1),
@@ -421,13 +440,25 @@ int dmolp_graph_download(dmolp_ctx *ctx, const dmolp_graph *g, int64_t *off, int
 
 int dmolp_graph_free(dmolp_ctx *ctx, dmolp_graph *g) {
     if (!g) return DMOLP_OK;
-    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    // The graph's memory goes back to its pool for reuse in stream order
+    // (or is freed, if that pool's context is gone).  Work another context
+    // queued on it must be finished first: then wait for the whole device.
+    if (ctx && !g->mixed && g->pool == ctx->pool.get()) {
+        (void)hipStreamSynchronize(ctx->stream);
+    } else {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(g->device);
+        (void)hipDeviceSynchronize();
+        (void)hipSetDevice(cur);
+    }
     delete g;
     return DMOLP_OK;
 }
 
 int dmolp_graph_set_labels(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels) {
     if (!ctx || !g || !labels) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     return guarded(ctx, [&] {
         set_labels(ctx, g, labels);
         DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -436,11 +467,13 @@ int dmolp_graph_set_labels(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels
 
 int dmolp_graph_get_labels(dmolp_ctx *ctx, const dmolp_graph *g, int32_t *labels) {
     if (!ctx || !g || !labels) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     return guarded(ctx, [&] { get_labels(ctx, g, labels); });
 }
 
 int dmolp_graph_labels_checkpoint(dmolp_ctx *ctx, dmolp_graph *g, int op) {
     if (!ctx || !g || (op != 0 && op != 1)) return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    note_use(ctx, g);
     return guarded(ctx, [&] {
         const size_t bytes = sizeof(int32_t) * (size_t)g->n;
         if (op == 0) {
@@ -460,6 +493,7 @@ int dmolp_graph_labels_checkpoint(dmolp_ctx *ctx, dmolp_graph *g, int op) {
 int dmolp_phase1(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t theta, int32_t *labels_out,
                  int32_t *seeds_out, int64_t *vc, int64_t *ec, dmolp_phase1_metrics *m) {
     if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
     if (theta <= 0) return fail(ctx, DMOLP_E_ARG, "theta must be positive");
     if ((int64_t)k > g->n) return fail(ctx, DMOLP_E_SEEDS, "k larger than the vertex count");
@@ -496,6 +530,7 @@ int dmolp_phase1(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t theta, int32
 int dmolp_phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t *labels_out,
                         int64_t *vc, int64_t *ec, dmolp_phase1_metrics *m) {
     if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
     return guarded(ctx, [&] {
         DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
@@ -531,6 +566,7 @@ int dmolp_phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t *labe
 int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, double epsilon,
                  int64_t *vc, int64_t *ic, dmolp_phase2_metrics *m, dmolp_trace *trace) {
     if (!ctx || !g) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
     if (max_rounds < 0) return fail(ctx, DMOLP_E_ARG, "max_rounds must be >= 0");
     if (!g->labels_valid) return fail(ctx, DMOLP_E_STATE, "graph has no resident labels");
@@ -633,6 +669,7 @@ int dmolp_phase2(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, int32_t max_rounds, 
 int dmolp_stats(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, const int32_t *labels, int64_t *vc,
                 int64_t *ic, int64_t *cut) {
     if (!ctx || !g || !labels) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
     return guarded(ctx, [&] {
         set_labels(ctx, g, labels);
@@ -649,6 +686,7 @@ int dmolp_stats(dmolp_ctx *ctx, dmolp_graph *g, int32_t k, const int32_t *labels
 int dmolp_stats_noalloc(dmolp_ctx *ctx, const dmolp_graph *g, int32_t k, const int32_t *labels,
                         int64_t *vc, int64_t *ic, int64_t *cut) {
     if (!ctx || !g || (!labels && g->n > 0)) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     return guarded(ctx, [&] { stats_noalloc(ctx, g, k, labels, vc, ic, cut); });
 }
 
@@ -675,6 +713,7 @@ int dmolp_select_boundary(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels,
                           int32_t t, int32_t *out, int64_t *n_out) {
     if (!ctx || !g || !labels || !re || !out || !n_out || n_prev < 0 || (n_prev > 0 && !prev))
         return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     if (k <= 0 || k > DMOLP_MAX_K || t < 1) return fail(ctx, DMOLP_E_ARG, "bad k or t");
     return guarded(ctx, [&] {
         set_labels(ctx, g, labels);
@@ -698,6 +737,7 @@ int dmolp_lp_round(dmolp_ctx *ctx, dmolp_graph *g, const int32_t *labels, const 
                    int64_t nb, const double *pen, int32_t k, int32_t *labels_out) {
     if (!ctx || !g || !labels || !pen || !labels_out || nb < 0 || (nb > 0 && !b))
         return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     if (k <= 0 || k > DMOLP_MAX_K) return fail(ctx, DMOLP_E_ARG, "k outside [1, 1024]");
     return guarded(ctx, [&] {
         set_labels(ctx, g, labels);
@@ -719,12 +759,14 @@ int dmolp_batch_run(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, dmolp
                     double *device_ms) {
     if (!ctx || n_jobs < 0 || (n_jobs > 0 && (!jobs || !results)))
         return fail(ctx, DMOLP_E_ARG, "null argument");
+    for (int32_t i = 0; i < n_jobs; ++i) note_use(ctx, jobs[i].graph);
     return guarded(ctx, [&] { batch_run(ctx, n_jobs, jobs, results, device_ms); });
 }
 
 int dmolp_batch_autotune(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, int32_t max_trials,
                          dmolp_tuning *best) {
     if (!ctx || n_jobs <= 0 || !jobs) return fail(ctx, DMOLP_E_ARG, "null argument");
+    for (int32_t i = 0; i < n_jobs; ++i) note_use(ctx, jobs[i].graph);
     return guarded(ctx, [&] { batch_autotune(ctx, n_jobs, jobs, max_trials, best); });
 }
 
@@ -762,6 +804,7 @@ int dmolp_server_take(dmolp_server *srv, int64_t ticket, dmolp_served *out, int3
 
 int dmolp_shares_create(dmolp_ctx *ctx, dmolp_graph *g, int32_t P, dmolp_shares **out) {
     if (!ctx || !g || !out) return fail(ctx, DMOLP_E_ARG, "null argument");
+    note_use(ctx, g);
     *out = nullptr;
     return guarded(ctx, [&] { *out = shares_create(ctx, g, P); });
 }
@@ -769,6 +812,7 @@ int dmolp_shares_create(dmolp_ctx *ctx, dmolp_graph *g, int32_t P, dmolp_shares 
 int dmolp_shares_destroy(dmolp_ctx *ctx, dmolp_shares *s) {
     if (!s) return DMOLP_OK;
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    else (void)hipDeviceSynchronize();
     delete s;
     return DMOLP_OK;
 }

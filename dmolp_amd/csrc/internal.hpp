@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -36,31 +38,176 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::dmolp::Error((status), std::string(msg)); \
     } while (0)
 
-// Owning device allocation.
+// Device memory recycling for one context.  A released block is kept and
+// handed out again to a later allocation of the same context, so that a
+// steady stream of graphs and calls makes no hipMalloc or hipFree (hipFree
+// waits for the whole device, which would serialise served jobs).  Reuse is
+// safe in stream order: every kernel and copy that touches a context's
+// memory is queued on that context's stream, so a block's next owner's work
+// runs after its previous owner's.  Graphs used with a second context are
+// freed behind a device-wide synchronisation (dmolp_graph_free).  Blocks are
+// rounded to size classes (powers of two below 1 MiB, then whole MiB) and a
+// cached block serves a request it exceeds by at most a quarter.  At most
+// `limit` bytes stay cached; past it a released block is freed at once.
+class DevPool {
+  public:
+    explicit DevPool(size_t limit) : limit_(limit) {}
+    DevPool(const DevPool &) = delete;
+    DevPool &operator=(const DevPool &) = delete;
+    ~DevPool() { trim(); }
+    static size_t round(size_t b) {
+        if (b < (1u << 20)) {
+            size_t r = 512;
+            while (r < b) r <<= 1;
+            return r;
+        }
+        return (b + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+    }
+    // A cached block of at least `want` bytes (a rounded size), or nullptr.
+    void *take(size_t want, size_t &cap) {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.lower_bound(want);
+        const size_t slack = want < (1u << 20) ? 0 : want / 4;
+        if (it == free_.end() || it->first > want + slack) return nullptr;
+        void *p = it->second;
+        cap = it->first;
+        cached_ -= cap;
+        free_.erase(it);
+        return p;
+    }
+    void give(void *p, size_t cap) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (closed_ || cached_ + cap > limit_) {
+            (void)hipFree(p);
+            return;
+        }
+        free_.emplace(cap, p);
+        cached_ += cap;
+    }
+    // Frees every cached block (hipFree waits for the device first).
+    void trim() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto &kv : free_) (void)hipFree(kv.second);
+        free_.clear();
+        cached_ = 0;
+    }
+    // The owning context is going away: free the cache, and from now on free
+    // blocks as they come back.
+    void close() {
+        trim();
+        std::lock_guard<std::mutex> lk(mu_);
+        closed_ = true;
+    }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    size_t cached_ = 0;
+    size_t limit_;
+    bool closed_ = false;
+};
+
+// The pool that allocations on this thread draw from, and its context's
+// stream: set for the duration of an ABI call (PoolScope); none outside
+// calls.
+inline const std::shared_ptr<DevPool> *&tls_pool() {
+    static thread_local const std::shared_ptr<DevPool> *p = nullptr;
+    return p;
+}
+inline hipStream_t &tls_stream() {
+    static thread_local hipStream_t s = nullptr;
+    return s;
+}
+struct PoolScope {
+    const std::shared_ptr<DevPool> *prev;
+    hipStream_t prev_stream;
+    PoolScope(const std::shared_ptr<DevPool> *p, hipStream_t s)
+        : prev(tls_pool()), prev_stream(tls_stream()) {
+        tls_pool() = p;
+        tls_stream() = s;
+    }
+    ~PoolScope() {
+        tls_pool() = prev;
+        tls_stream() = prev_stream;
+    }
+    PoolScope(const PoolScope &) = delete;
+    PoolScope &operator=(const PoolScope &) = delete;
+};
+// DMOLP_POOL_POISON=1 (read once): every block an allocation hands out,
+// fresh or reused, is first filled with 0xa5 bytes on the call's stream, so
+// that code reading memory it never wrote fails in tests instead of passing
+// on the zeros of fresh pages.
+bool pool_poison();
+
+// Owning device allocation: drawn from the calling context's pool when
+// there is one, and given back to that pool when released.
 struct DevMem {
     void *p = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;  // requested size
+    size_t cap = 0;    // size of the block behind p
+    std::shared_ptr<DevPool> pool;
     DevMem() = default;
     explicit DevMem(size_t b) { alloc(b); }
     DevMem(const DevMem &) = delete;
     DevMem &operator=(const DevMem &) = delete;
-    DevMem(DevMem &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevMem(DevMem &&o) noexcept : p(o.p), bytes(o.bytes), cap(o.cap), pool(std::move(o.pool)) {
+        o.p = nullptr;
+        o.bytes = o.cap = 0;
+    }
     DevMem &operator=(DevMem &&o) noexcept {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            cap = o.cap;
+            pool = std::move(o.pool);
+            o.p = nullptr;
+            o.bytes = o.cap = 0;
+        }
         return *this;
     }
     ~DevMem() { release(); }
     void alloc(size_t b) {
         release();
         if (b == 0) b = 256;
-        DMOLP_HIP_CHECK(hipMalloc(&p, b));
+        const size_t r = DevPool::round(b);
+        const std::shared_ptr<DevPool> *tp = tls_pool();
+        if (tp && *tp) {
+            size_t c = 0;
+            if (void *q = (*tp)->take(r, c)) {
+                p = q;
+                cap = c;
+                bytes = b;
+                pool = *tp;
+                if (pool_poison()) DMOLP_HIP_CHECK(hipMemsetAsync(p, 0xa5, cap, tls_stream()));
+                return;
+            }
+        }
+        hipError_t e = hipMalloc(&p, r);
+        if (e == hipErrorOutOfMemory && tp && *tp) {  // give the cache back and retry once
+            (void)hipGetLastError();
+            (*tp)->trim();
+            e = hipMalloc(&p, r);
+        }
+        if (e != hipSuccess) {
+            p = nullptr;
+            DMOLP_HIP_CHECK(e);
+        }
         bytes = b;
+        cap = r;
+        if (tp) pool = *tp;
+        if (pool_poison()) DMOLP_HIP_CHECK(hipMemsetAsync(p, 0xa5, cap, tls_stream()));
     }
     void grow(size_t b) {
         if (!p || bytes < b) alloc(b);
     }
     void release() {
-        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (!p) return;
+        if (pool) pool->give(p, cap);
+        else (void)hipFree(p);
+        p = nullptr;
+        bytes = cap = 0;
+        pool.reset();
     }
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
@@ -198,6 +345,7 @@ struct dmolp_ctx {
     hipStream_t stream = nullptr;
     std::string last_error;
     dmolp::Tuning tune;
+    std::shared_ptr<dmolp::DevPool> pool;  // recycled device memory of this context
     dmolp::DevMem scratch;  // small per-call scratch (host<->device staging)
     dmolp::DevMem ops_dev;  // staging of OpBatch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -219,6 +367,12 @@ struct dmolp_ctx {
 };
 
 struct dmolp_graph {
+    // The pool of the context that created the graph; `mixed` is set when a
+    // call of another context has queued work on it (freeing it then waits
+    // for the whole device, not one stream).
+    const dmolp::DevPool *pool = nullptr;
+    mutable bool mixed = false;
+    int device = 0;
     int64_t n = 0;
     int64_t nnz = 0;
     int32_t max_degree = 0;
@@ -262,4 +416,9 @@ bool round_poll(dmolp_ctx *ctx, const int32_t *d_flag, int t, bool stop_at_zero)
 // interception misbehaves: rocprofv3 --kernel-trace has ended processes
 // with a host SIGSEGV inside its callback under hipGraphLaunch.
 bool graphs_enabled();
+// Records that ctx queues work on g: a graph made by another context is then
+// freed behind a device-wide synchronisation.
+inline void note_use(const dmolp_ctx *ctx, const dmolp_graph *g) {
+    if (ctx && g && g->pool != ctx->pool.get()) g->mixed = true;
+}
 }  // namespace dmolp

@@ -201,6 +201,7 @@ void finish_job(dmolp_server *s, SrvJob *j, int status) {
 void server_step(dmolp_server *s, int32_t *active_out, int32_t *queued_out) {
     dmolp_ctx *ctx = s->ctx;
     hipStream_t st = ctx->stream;
+    PoolScope scope(&ctx->pool, ctx->stream);
     DMOLP_HIP_CHECK(hipSetDevice(ctx->device));
     const double now = s->clock();
     // 1. Admission: first come, first served, while a slot and the memory
@@ -225,8 +226,11 @@ void server_step(dmolp_server *s, int32_t *active_out, int32_t *queued_out) {
         try {
             if (j->req.graph) {
                 j->g = j->req.graph;
+                note_use(ctx, j->g);
             } else {
                 j->g = new dmolp_graph();
+                j->g->pool = ctx->pool.get();
+                j->g->device = ctx->device;
                 j->own_graph = true;
                 gen_device(ctx, &j->req.spec, j->g);
                 graph_validate(ctx, j->g);
