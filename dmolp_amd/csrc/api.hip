@@ -397,7 +397,7 @@ int dmolp_graph_generate(dmolp_ctx *ctx, const dmolp_gen_spec *spec, dmolp_graph
     int rc = guarded(ctx, [&] {
         DMOLP_HIP_CHECK(hipSetDevice(ctx->device));
         gen_device(ctx, spec, g);
-        graph_validate(ctx, g);
+        graph_max_degree(ctx, g);
     });
     if (rc != DMOLP_OK) { delete g; return rc; }
     *out = g;

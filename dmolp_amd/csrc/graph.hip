@@ -74,6 +74,20 @@ __global__ void k_validate_long(GraphDev g, unsigned long long *flags, const int
     if (loops) atomicAdd(&flags[3], loops);
 }
 
+__global__ void k_max_degree(GraphDev g, unsigned long long *out) {
+    unsigned long long m = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long d = (unsigned long long)(g.off[v + 1] - g.off[v]);
+        m = d > m ? d : m;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const unsigned long long x = __shfl_down(m, o, kWave);
+        m = x > m ? x : m;
+    }
+    if (lane_id() == 0 && m) atomicMax(out, m);
+}
+
 __global__ void k_bin_all(GraphDev g, ListDev out, int heavy_degree, int chunk_edges) {
     for (int64_t b0 = global_wave() * kWave; b0 < g.n; b0 += wave_count() * kWave) {
         const int64_t v = b0 + lane_id();
@@ -135,6 +149,19 @@ void graph_validate(dmolp_ctx *ctx, dmolp_graph *g) {
     DMOLP_REQUIRE(h[2] == 0, DMOLP_E_GRAPH, "CSR row not strictly ascending (unsorted or duplicate)");
     DMOLP_REQUIRE(h[3] == 0, DMOLP_E_GRAPH, "CSR has a self-loop");
     g->max_degree = (int32_t)h[4];
+}
+
+void graph_max_degree(dmolp_ctx *ctx, dmolp_graph *g) {
+    DevMem acc(sizeof(unsigned long long));
+    DMOLP_HIP_CHECK(hipMemsetAsync(acc.p, 0, acc.bytes, ctx->stream));
+    if (g->n > 0)
+        hipLaunchKernelGGL(k_max_degree, dim3(ctx->cu_count * 4), dim3(256), 0, ctx->stream, g->dev(),
+                           acc.as<unsigned long long>());
+    DMOLP_HIP_CHECK(hipGetLastError());
+    unsigned long long h = 0;
+    DMOLP_HIP_CHECK(hipMemcpyAsync(&h, acc.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    g->max_degree = (int32_t)h;
 }
 
 void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g) {

@@ -404,6 +404,9 @@ namespace dmolp {
 void graph_prepare_bins(dmolp_ctx *ctx, dmolp_graph *g);  // builds g->all for ctx->tune
 void graph_prepare_first(dmolp_ctx *ctx, dmolp_graph *g);  // g->first, g->deg
 void graph_validate(dmolp_ctx *ctx, dmolp_graph *g);       // offsets, sorted rows, no self-loops
+// The largest degree only, for graphs the library generated itself (sorted,
+// deduplicated and loop-free by construction).
+void graph_max_degree(dmolp_ctx *ctx, dmolp_graph *g);
 int64_t bits_for(int64_t v);                                // bits needed to hold 0..v
 // After enqueueing round t of a loop: queue a copy of the device flag behind
 // it and read the copy queued after round t-1 (the host stays one round

@@ -10,6 +10,7 @@ namespace {
 constexpr int kSelThreads = 256;
 constexpr int kHist1LdsInts = 8192;  // 32 KB of LDS for the level-1 histogram
 constexpr int kCompactItems = 8;     // records per thread per block iteration
+constexpr int kHist2LdsInts = 16384; // 64 KB of LDS for the digit histograms
 
 // Running count over bins in scan order (reverse: from the last bin down);
 // finds the first bin where it reaches target (1 <= target <= total).
@@ -398,10 +399,11 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_rank_scan(const SelArgs *A,
     }
 }
 
-// Digit histogram of the records still being refined.  With few enough
-// groups the block accumulates in LDS and adds its counts once; otherwise
-// every record adds to the global histogram.
-constexpr int kHist2LdsInts = 16384;
+// Digit histogram of the records still being refined.  The block
+// accumulates in LDS and adds its counts once: the digit width is chosen so
+// that every group's bins fit (adding every record to the global histogram
+// instead serialises on the hot bins: the frontier records of one part often
+// share their top id digits).
 __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(const SelArgs *A, const int32_t *bstart,
                                                            int jobs) {
     SEL_JOB();
@@ -412,7 +414,8 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(const SelArgs *A, con
     const SelState *st = a.st;
     const int b1_bins = a.b1_bins, id_bits = a.id_bits, deg_bits = a.deg_bits;
     const int64_t count = (int64_t)*in.cnt;
-    const int nb = a.groups * kDigitBins;
+    const int db = a.digit_bits;
+    const int nb = a.groups << db;
     const bool lds = nb <= kHist2LdsInts;
     __shared__ int32_t s_h[kHist2LdsInts];
     __shared__ SelState s_st[kSelLdsGroups];  // the first groups' states
@@ -441,10 +444,10 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(const SelArgs *A, con
             if (bin1_of(d, b1_bins) != s.b1) continue;
             const uint64_t K = sel_key(xx[j], d, s.desc, id_bits, deg_bits);
             if ((K >> s.r) != (s.prefix >> s.r)) continue;
-            const int w = s.r < kDigitBits ? s.r : kDigitBits;
+            const int w = s.r < db ? s.r : db;
             const int digit = (int)((K >> (s.r - w)) & ((1ull << w) - 1));
-            if (lds) atomicAdd(&s_h[gg[j] * kDigitBins + digit], 1);
-            else atomicAdd(&h2[(int64_t)gg[j] * kDigitBins + digit], 1);
+            if (lds) atomicAdd(&s_h[(gg[j] << db) + digit], 1);
+            else atomicAdd(&h2[((int64_t)gg[j] << db) + digit], 1);
         }
     }
     if (lds) {
@@ -464,9 +467,9 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_pick2(const SelArgs *A, con
     const int g = (int)vg.block;
     SelState s = a.st[g];
     if (s.mode != SEL_REFINE) return;  // uniform per block
-    const int w = s.r < kDigitBits ? s.r : kDigitBits;
+    const int w = s.r < a.digit_bits ? s.r : a.digit_bits;
     const int nb = 1 << w;
-    int32_t *bins = a.h2 + (int64_t)g * kDigitBins;
+    int32_t *bins = a.h2 + ((int64_t)g << a.digit_bits);
     block_crossing(bins, nb, false, s.need, &s_bin, &s_before, s_tmp);
     const int digit = s_bin;
     const int64_t cnt = bins[digit];
@@ -675,9 +678,13 @@ void SelectWork::prepare(dmolp_ctx *ctx, int ngroups, int64_t n, int32_t max_deg
     if (b1 > kSelBins) b1 = kSelBins;
     DMOLP_REQUIRE(b1 >= 2, DMOLP_E_UNSUPPORTED, "too many groups for the selection histogram");
     b1_bins = b1;
-    passes = (id_bits + deg_bits + kDigitBits - 1) / kDigitBits;
+    digit_bits = kDigitBits;
+    while (digit_bits > 1 && ((int64_t)groups << digit_bits) > kHist2LdsInts) --digit_bits;
+    DMOLP_REQUIRE(((int64_t)groups << digit_bits) <= kHist2LdsInts, DMOLP_E_UNSUPPORTED,
+                  "too many groups for the digit histogram");
+    passes = (id_bits + deg_bits + digit_bits - 1) / digit_bits;
     h1.alloc(sizeof(int32_t) * (size_t)groups * b1_bins);
-    h2.alloc(sizeof(int32_t) * (size_t)groups * kDigitBins);
+    h2.alloc(sizeof(int32_t) * ((size_t)groups << digit_bits));
     st.alloc(sizeof(SelState) * (size_t)groups);
     refining.alloc(sizeof(int32_t) * 4);
     DMOLP_HIP_CHECK(hipMemsetAsync(h1.p, 0, h1.bytes, ctx->stream));
@@ -709,6 +716,7 @@ SelArgs select_args(dmolp_ctx *ctx, SelectWork &w, const RecDev &in, const TakeR
     a.b1_bins = w.b1_bins;
     a.id_bits = w.id_bits;
     a.deg_bits = w.deg_bits;
+    a.digit_bits = w.digit_bits;
     a.heavy_degree = ctx->tune.heavy_degree;
     a.chunk_edges = ctx->tune.chunk_edges;
     a.dense = 0;

@@ -6,7 +6,8 @@
 // partition.cpp:400-431 (per partition, the theta smallest (degree, id)).
 //
 // No sort: a level-1 histogram over min(degree, B1-1) finds the degree bin
-// where the cut falls; if that bin must be split, 8-bit MSD digit passes
+// where the cut falls; if that bin must be split, MSD digit passes (8 bits
+// wide, narrower when there are more than 64 groups)
 // over the key K = (dk << IB) | id (dk = degree, or maxdeg - degree for the
 // descending order) find the exact threshold key.  Every pass is a
 // histogram, so the selected set does not depend on record order.
@@ -84,6 +85,7 @@ struct SelectWork {
     int b1_bins = 0;
     int id_bits = 0;
     int deg_bits = 0;
+    int digit_bits = 0;
     int passes = 0;
     DevMem h1, h2, st, refining;
     Recs ref;  // records still being refined after the level-1 pick
@@ -111,6 +113,7 @@ struct SelArgs {
     const int32_t *mark_val;
     TakeRule rule;
     int32_t groups, b1_bins, id_bits, deg_bits;
+    int32_t digit_bits;  // width of a digit pass: groups << digit_bits bins fit the LDS histogram
     int32_t has_carry, heavy_degree, chunk_edges, dense;
     // Dense records (dense != 0): record v exists for 0 <= v < rn iff
     // rdeg[v] >= 0, with id v, group rlab[v] (rlab8[v] when rlab8 is set:

@@ -233,7 +233,7 @@ void server_step(dmolp_server *s, int32_t *active_out, int32_t *queued_out) {
                 j->g->device = ctx->device;
                 j->own_graph = true;
                 gen_device(ctx, &j->req.spec, j->g);
-                graph_validate(ctx, j->g);
+                graph_max_degree(ctx, j->g);
             }
             P1Item it;
             it.g = j->g;
