@@ -88,12 +88,44 @@ __global__ void k_max_degree(GraphDev g, unsigned long long *out) {
     if (lane_id() == 0 && m) atomicMax(out, m);
 }
 
-__global__ void k_bin_all(GraphDev g, ListDev out, int heavy_degree, int chunk_edges) {
-    for (int64_t b0 = global_wave() * kWave; b0 < g.n; b0 += wave_count() * kWave) {
-        const int64_t v = b0 + lane_id();
-        const bool valid = v < g.n;
-        const int64_t d = valid ? g.off[v + 1] - g.off[v] : 0;
-        list_append(out, valid, (int32_t)v, d, heavy_degree, chunk_edges);
+// Every vertex into the light list or a heavy slot.  Each block takes
+// kBinItems vertices a thread per step and reserves its light entries with
+// one atomic (a counter hit once per wave by every wave serialises).
+constexpr int kBinItems = 8;
+__global__ __launch_bounds__(256) void k_bin_all(GraphDev g, ListDev out, int heavy_degree,
+                                                 int chunk_edges) {
+    __shared__ BlockAppendLds s_app;
+    const int64_t span = (int64_t)blockDim.x * kBinItems;
+    for (int64_t b0 = (int64_t)blockIdx.x * span; b0 < g.n; b0 += (int64_t)gridDim.x * span) {
+        int64_t d[kBinItems];
+        int32_t nl = 0;
+        int64_t dsum = 0;
+#pragma unroll
+        for (int j = 0; j < kBinItems; ++j) {
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            const int64_t vc = v < g.n ? v : g.n - 1;  // loads issued together
+            const int64_t dv = g.off[vc + 1] - g.off[vc];
+            d[j] = v < g.n ? dv : -1;
+            nl += (d[j] >= 0 && d[j] <= heavy_degree) ? 1 : 0;
+            dsum += d[j] >= 0 ? d[j] : 0;
+        }
+        int64_t pos = block_reserve(nl, &out.cnt[LC_LIGHT], s_app, dsum, &out.cnt[LC_DEG]);
+#pragma unroll
+        for (int j = 0; j < kBinItems; ++j) {
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            if (d[j] < 0) continue;
+            if (d[j] <= heavy_degree) {
+                out.light[pos++] = (int32_t)v;
+            } else {
+                const int64_t slot = (int64_t)atomicAdd(&out.cnt[LC_SLOTS], 1ull);
+                const int64_t nch = (d[j] + chunk_edges - 1) / chunk_edges;
+                const int64_t base = (int64_t)atomicAdd(&out.cnt[LC_CHUNKS], (unsigned long long)nch);
+                out.hv[slot] = (int32_t)v;
+                out.hn[slot] = (int32_t)nch;
+                atomicAdd(&out.cnt[LC_HDEG], (unsigned long long)d[j]);
+                for (int64_t c = 0; c < nch; ++c) out.hc[base + c] = (slot << 32) | c;
+            }
+        }
     }
 }
 

@@ -292,17 +292,33 @@ __global__ void k_p1_fill(const P1Job *JS, const int32_t *bstart, int jobs, int 
     P1Ctl *ctl = J.ctl;
     if (p1_done(ctl)) return;
     const int b = ctl->fill_part;
-    if (b < 0) return;
+    if (b < 0) return;  // uniform per job, so per block
+    // kFillItems vertices a thread per step, the block's new members
+    // reserved with one atomic (one per wave serialises on the counter).
+    constexpr int kFillItems = 8;
+    __shared__ BlockAppendLds s_app;
+    const int64_t span = (int64_t)blockDim.x * kFillItems;
     int64_t added = 0;
-    for (int64_t b0 = vwave(vg) * kWave; b0 < g.n; b0 += vwaves(vg) * kWave) {
-        const int64_t v = b0 + lane_id();
-        const bool un = v < g.n && L[v] == -1;
-        const int64_t pos = wave_append(un, fcur.cnt);
-        if (un) {
+    for (int64_t b0 = vg.block * span; b0 < g.n; b0 += vg.blocks * span) {
+        bool un[kFillItems];
+        int32_t nu = 0;
+#pragma unroll
+        for (int j = 0; j < kFillItems; ++j) {
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+            const int32_t lv = L[v < g.n ? v : 0];
+            un[j] = v < g.n && lv == -1;
+            nu += un[j] ? 1 : 0;
+        }
+        int64_t pos = block_reserve(nu, fcur.cnt, s_app);
+#pragma unroll
+        for (int j = 0; j < kFillItems; ++j) {
+            if (!un[j]) continue;
+            const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             L[v] = b;
             fcur.x[pos] = (int32_t)v;
             fcur.g[pos] = b;
             fcur.d[pos] = (int32_t)(g.off[v + 1] - g.off[v]);
+            ++pos;
             ++added;
         }
     }
@@ -392,10 +408,12 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
                 p[j] = s_b[w].aux[slot];
             }
 #pragma unroll
-            for (int j = 0; j < kExpUnroll; ++j) {
-                lv[j] = L[u[j]];
-                cl[j] = claims[u[j]];
-            }
+            for (int j = 0; j < kExpUnroll; ++j) lv[j] = L[u[j]];
+            // Claims only of unlabelled neighbours: a labelled one's lane
+            // reads claims[0] (one line shared by the wave), so most of a
+            // late round's entries cost one gather, not two.
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) cl[j] = claims[lv[j] == -1 ? u[j] : 0];
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) {
                 bool first = false;
@@ -440,10 +458,9 @@ __global__ __launch_bounds__(kThreads) void k_expand_heavy(const P1Job *JS, cons
                 u[j] = g.col[on[j] ? e : e0];
             }
 #pragma unroll
-            for (int j = 0; j < kExpUnroll; ++j) {
-                lv[j] = L[u[j]];
-                cl[j] = claims[u[j]];
-            }
+            for (int j = 0; j < kExpUnroll; ++j) lv[j] = L[u[j]];
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) cl[j] = claims[lv[j] == -1 ? u[j] : 0];
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) {
                 bool first = false;
