@@ -188,7 +188,14 @@ void batch_run(dmolp_ctx *ctx, int32_t n_jobs, const dmolp_job *jobs, dmolp_resu
         p1[(size_t)i].k = jobs[i].k;
         p1[(size_t)i].theta = jobs[i].theta;
     }
-    phase1_run_set(ctx, p1);
+    // The phase-2 work arrays and degree bins depend on the graphs only: they
+    // are set up while the host searches the hubs.
+    phase1_run_set(ctx, p1, [&] {
+        for (int32_t i = 0; i < n_jobs; ++i) {
+            Phase2Work &w = phase2_work(ctx, jobs[i].graph, jobs[i].k);
+            phase2_round_table(ctx, w, jobs[i].max_rounds);
+        }
+    });
     std::vector<P2Item> p2((size_t)n_jobs);
     for (int32_t i = 0; i < n_jobs; ++i) {
         p2[(size_t)i].g = jobs[i].graph;

@@ -12,6 +12,7 @@
 //             unlabelled neighbour goes to the lowest claiming partition
 //             (atomicMin); frontiers become carry + newly labelled.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -1168,17 +1169,36 @@ void p1_hub_search(P1Stage &h, int threads) {
 namespace {
 
 // Hubs and first seed of every job on the host; jobs in parallel threads.
-void host_hub_search(std::vector<P1Stage> &H) {
+// `during` (may be empty) runs on the calling thread while the searches run
+// on worker threads.  Jobs are taken largest first from a shared cursor, so
+// the longest search starts at once and the threads finish close together.
+void host_hub_search(std::vector<P1Stage> &H, const std::function<void()> &during) {
     const size_t G = H.size();
     size_t T = (size_t)std::min(16, host_threads());
     if (T > G) T = G;
     if (T <= 1) {  // one job: its passes and long partitions use the threads
-        for (size_t j = 0; j < G; ++j) p1_hub_search(H[j], std::min(16, host_threads()));
+        if (during) {
+            std::thread th([&] {
+                for (size_t j = 0; j < G; ++j) p1_hub_search(H[j], std::min(16, host_threads()));
+            });
+            during();
+            th.join();
+        } else {
+            for (size_t j = 0; j < G; ++j) p1_hub_search(H[j], std::min(16, host_threads()));
+        }
         return;
     }
+    std::vector<size_t> order(G);
+    for (size_t j = 0; j < G; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return H[a].deg.size() > H[b].deg.size(); });
+    std::atomic<size_t> next{0};
     std::vector<std::thread> pool;
     for (size_t t = 0; t < T; ++t)
-        pool.emplace_back([&, t]() { for (size_t j = t; j < G; j += T) p1_hub_search(H[j]); });
+        pool.emplace_back([&]() {
+            for (size_t i = next.fetch_add(1); i < G; i = next.fetch_add(1)) p1_hub_search(H[order[i]]);
+        });
+    if (during) during();
     for (auto &th : pool) th.join();
 }
 
@@ -1495,12 +1515,12 @@ void phase1_run(dmolp_ctx *ctx, dmolp_graph *g, int k, int theta, std::vector<in
     items[0].g = g;
     items[0].k = k;
     items[0].theta = theta;
-    phase1_run_set(ctx, items);
+    phase1_run_set(ctx, items, {});
     seeds_out = items[0].seeds;
     if (m) *m = items[0].m;
 }
 
-void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items) {
+void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items, const std::function<void()> &during_hubs) {
     const int G = (int)items.size();
     DMOLP_REQUIRE(G >= 1, DMOLP_E_ARG, "empty job set");
     hipStream_t s = ctx->stream;
@@ -1514,7 +1534,7 @@ void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items) {
     p1_set_build(ctx, HP, sd);
     DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, s));
     p1_degrees(ctx, HP, sd);
-    host_hub_search(H);
+    host_hub_search(H, during_hubs);
     p1_seeds(ctx, HP, sd);
     p1_expand_begin(ctx, HP, sd);
     int64_t enq = 0;

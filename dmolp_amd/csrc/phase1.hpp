@@ -1,6 +1,7 @@
 // phase1.hpp -- device state of the hub-seed expansion phase.
 #pragma once
 #include <utility>
+#include <functional>
 #include <vector>
 
 #include "internal.hpp"
@@ -105,6 +106,9 @@ void phase1_run(dmolp_ctx *ctx, dmolp_graph *g, int k, int theta, std::vector<in
                 dmolp_phase1_metrics *m);
 // The same for a set of graphs advancing together: one launch per kernel
 // step serves every job.  Each job's result equals its plain run.
-void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items);
+// during_hubs (may be empty) runs on the calling thread while the host hub
+// searches run on worker threads: device work queued there overlaps them.
+void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items,
+                    const std::function<void()> &during_hubs);
 
 }  // namespace dmolp

@@ -417,15 +417,22 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_hist2(const SelArgs *A, con
     const int db = a.digit_bits;
     const int nb = a.groups << db;
     const bool lds = nb <= kHist2LdsInts;
+    constexpr int kH2Items = 4;  // records per thread per step, loads issued together
+    // Only as many blocks as the refine list needs (at least kH2MinRecs
+    // records a block) clear and flush an LDS histogram; the launch is sized
+    // for the job's whole frontier, usually far more than the refine list.
+    constexpr int64_t kH2MinRecs = 8 * kSelThreads * kH2Items;
+    int64_t eff = (count + kH2MinRecs - 1) / kH2MinRecs;
+    eff = eff < 1 ? 1 : (eff > vg.blocks ? vg.blocks : eff);
+    if (vg.block >= eff) return;  // uniform per block
     __shared__ int32_t s_h[kHist2LdsInts];
     __shared__ SelState s_st[kSelLdsGroups];  // the first groups' states
     for (int i = threadIdx.x; i < a.groups && i < kSelLdsGroups; i += blockDim.x) s_st[i] = st[i];
     if (lds)
         for (int i = threadIdx.x; i < nb; i += blockDim.x) s_h[i] = 0;
     __syncthreads();
-    constexpr int kH2Items = 4;  // records per thread per step, loads issued together
-    const int64_t step = vthreads(vg);
-    for (int64_t i0 = vthread(vg); i0 < count; i0 += kH2Items * step) {
+    const int64_t step = eff * (int64_t)blockDim.x;
+    for (int64_t i0 = vg.block * (int64_t)blockDim.x + threadIdx.x; i0 < count; i0 += kH2Items * step) {
         int32_t gg[kH2Items], dd[kH2Items], xx[kH2Items];
 #pragma unroll
         for (int j = 0; j < kH2Items; ++j) {
