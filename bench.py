@@ -254,34 +254,87 @@ def run_batched(a):
     return out
 
 
+def serve_leg(srv, jobs, lam, rng, theta, rounds, eps, graphs=None, keep=False):
+    """One arrival rate: Poisson arrivals of the jobs (resident graphs, or
+    generator specs when graphs is None); returns (per-request records,
+    labels of each request when keep, span in seconds, arrival offsets)."""
+    gaps = rng.exponential(1.0 / lam, len(jobs))
+    arr_rel = np.cumsum(gaps) - gaps[0]
+    arr = srv.clock() + 0.01 + arr_rel
+    tickets, done, kept, nxt = [], {}, {}, 0
+    while len(done) < len(jobs):
+        now = srv.clock()
+        while nxt < len(jobs) and arr[nxt] <= now:
+            sp, k = jobs[nxt]
+            if graphs is None:
+                t = srv.submit(k, theta, rounds, eps, spec=sp, arrival_s=float(arr[nxt]),
+                               want_labels=keep)
+            else:
+                t = srv.submit(k, theta, rounds, eps, graph=graphs[nxt],
+                               arrival_s=float(arr[nxt]), want_labels=keep)
+            tickets.append(t)
+            nxt += 1
+        active, queued = srv.step()
+        for t in tickets:
+            if t not in done:
+                r = srv.take(t)
+                if r is not None:
+                    done[t] = r[0]
+                    if keep:
+                        kept[t] = r[1]
+        if active == 0 and queued == 0 and nxt < len(jobs) and len(done) == nxt:
+            time.sleep(max(0.0, min(0.05, arr[nxt] - srv.clock())))
+    recs = [done[t] for t in tickets]
+    labels = [kept[t] for t in tickets] if keep else None
+    span = max(r.done_s for r in recs) - float(arr[0])
+    return recs, labels, span, arr_rel
+
+
+def plain_fifo(arr_rel, service):
+    """A one-job-at-a-time server over measured service times: (jobs/s,
+    latencies)."""
+    fin, lat = 0.0, []
+    for x, sv in zip(arr_rel, service):
+        fin = max(fin, x) + sv
+        lat.append(fin - x)
+    return len(service) / fin, np.array(lat)
+
+
 def run_served(a):
     """Served mode: Poisson arrivals at several rates; continuous batching
     against a one-job-at-a-time plain server (FIFO over measured plain
-    service times of the same jobs)."""
+    service times of the same jobs).  Requests name CSR graphs already
+    resident in HBM when timing starts (as in the plain and batched lines);
+    a second leg at the highest rate has the server generate each request's
+    graph on admission, against plain service times that include the same
+    generation."""
     from dmolp_amd import api
     theta, rounds, eps = THETA, 20, 0.03
     rates = [10, 30, 100, 300, 1000]
     jobs = c5_jobs(a.jobs)
     ctx = api.Context(0)
-    # Plain service time of every job (generate + phase 1 + phase 2 + free).
-    service, dev_ms, host_ms = [], [], []
-    for it in range(2):
-        service, dev_ms, host_ms = [], [], []
-        for sp, k in jobs:
-            t0 = time.perf_counter()
-            g = ctx.generate(sp)
-            p1 = g.phase1(k, theta, want_labels=False)
-            p2 = g.phase2(k, rounds, eps, want_labels=False, want_trace=False)
-            g.free()
-            service.append(time.perf_counter() - t0)
-            dev_ms.append(p1.metrics.device_ms + p2.metrics.device_ms)
-            host_ms.append(p1.metrics.host_ms)
-            if it == 1 and service[-1] > 0.05:
-                log(f"[bench]   job n={g.n if g.handle else 0} scale={sp.scale} ef={sp.edge_factor} "
-                    f"k={k}: service {service[-1] * 1e3:.0f} ms, phase 1 device "
-                    f"{p1.metrics.device_ms:.1f} ms ({p1.metrics.rounds} rounds, hub sort "
-                    f"{p1.metrics.host_ms:.1f} ms), phase 2 device {p2.metrics.device_ms:.1f} ms "
-                    f"({p2.metrics.rounds} rounds)")
+    t0 = time.perf_counter()
+    graphs = [ctx.generate(sp) for sp, _ in jobs]
+    log(f"[bench] generated {len(graphs)} graphs: n={sum(g.n for g in graphs)} "
+        f"nnz={sum(g.nnz for g in graphs)} in {time.perf_counter() - t0:.2f} s")
+
+    def plain_times(generate):
+        # Second of two passes (the first allocates each graph's work arrays).
+        for _ in range(2):
+            service, dev_ms, host_ms = [], [], []
+            for i, (sp, k) in enumerate(jobs):
+                t0 = time.perf_counter()
+                g = ctx.generate(sp) if generate else graphs[i]
+                p1 = g.phase1(k, theta, want_labels=False)
+                p2 = g.phase2(k, rounds, eps, want_labels=False, want_trace=False)
+                if generate:
+                    g.free()
+                service.append(time.perf_counter() - t0)
+                dev_ms.append(p1.metrics.device_ms + p2.metrics.device_ms)
+                host_ms.append(p1.metrics.host_ms)
+        return service, dev_ms, host_ms
+
+    service, dev_ms, host_ms = plain_times(False)
     order = np.argsort(service)[::-1][:5]
     log(f"[bench] plain service: total {sum(service):.2f} s over {len(jobs)} jobs, "
         f"max {max(service) * 1e3:.1f} ms; phase 1 + 2 device time {sum(dev_ms) / 1e3:.2f} s, "
@@ -290,59 +343,57 @@ def run_served(a):
     srv = api.Server(ctx, 200 << 30, a.max_active)
     per_rate = {}
     rng = np.random.default_rng(5)
-    last_results = None
+    # Timed legs return metrics only, as the plain service times above do;
+    # the labels are checked in untimed legs at the highest rate below.
     for lam in rates:
-        keep = lam == rates[-1]  # the last rate's results are checked against the oracle
-        gaps = rng.exponential(1.0 / lam, len(jobs))
-        arr_rel = np.cumsum(gaps) - gaps[0]
-        base = srv.clock() + 0.01
-        arr = base + arr_rel
-        tickets, done, kept, nxt = [], {}, {}, 0
-        while len(done) < len(jobs):
-            now = srv.clock()
-            while nxt < len(jobs) and arr[nxt] <= now:
-                sp, k = jobs[nxt]
-                tickets.append(srv.submit(k, theta, rounds, eps, spec=sp, arrival_s=float(arr[nxt]),
-                                          want_labels=keep))
-                nxt += 1
-            active, queued = srv.step()
-            for t in tickets:
-                if t not in done:
-                    r = srv.take(t)
-                    if r is not None:
-                        done[t] = r[0]
-                        if keep:
-                            kept[t] = r[1]
-            if active == 0 and queued == 0 and nxt < len(jobs) and len(done) == nxt:
-                time.sleep(max(0.0, min(0.05, arr[nxt] - srv.clock())))
-        lat = np.array([done[t].done_s - done[t].arrival_s for t in tickets])
-        span = max(done[t].done_s for t in tickets) - float(arr[0])
-        if keep:
-            last_results = [(done[t].result, kept[t]) for t in tickets]
-        bad = [t for t in tickets if done[t].result.status != 0]
-        # Plain server: FIFO, one job at a time.
-        fin, plat = 0.0, []
-        for x, sv in zip(arr_rel, service):
-            fin = max(fin, x) + sv
-            plat.append(fin - x)
-        plat = np.array(plat)
+        recs, _, span, arr_rel = serve_leg(srv, jobs, lam, rng, theta, rounds, eps, graphs=graphs)
+        lat = np.array([r.done_s - r.arrival_s for r in recs])
+        bad = [i for i, r in enumerate(recs) if r.result.status != 0]
+        pj, plat = plain_fifo(arr_rel, service)
         per_rate[str(lam)] = {
             "jobs_per_s": len(jobs) / span, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
             "p99_ms": 1e3 * float(np.percentile(lat, 99)),
-            "plain_jobs_per_s": len(jobs) / fin, "plain_p50_ms": 1e3 * float(np.percentile(plat, 50)),
+            "plain_jobs_per_s": pj, "plain_p50_ms": 1e3 * float(np.percentile(plat, 50)),
             "plain_p99_ms": 1e3 * float(np.percentile(plat, 99)), "failed": len(bad)}
         log(f"[bench] rate {lam}/s: served {per_rate[str(lam)]['jobs_per_s']:.1f} jobs/s "
             f"p50 {per_rate[str(lam)]['p50_ms']:.1f} ms p99 {per_rate[str(lam)]['p99_ms']:.1f} ms | "
-            f"plain {per_rate[str(lam)]['plain_jobs_per_s']:.1f} jobs/s "
+            f"plain {pj:.1f} jobs/s "
             f"p50 {per_rate[str(lam)]['plain_p50_ms']:.1f} ms p99 {per_rate[str(lam)]['plain_p99_ms']:.1f} ms")
+    # The same requests with generation on admission, at the highest rate.
+    gservice, _, _ = plain_times(True)
+    recs, _, gspan, garr = serve_leg(srv, jobs, rates[-1], rng, theta, rounds, eps, graphs=None)
+    gpj, gplat = plain_fifo(garr, gservice)
+    glat = np.array([r.done_s - r.arrival_s for r in recs])
+    generated = {
+        "rate_per_s": rates[-1], "jobs_per_s": len(jobs) / gspan,
+        "p50_ms": 1e3 * float(np.percentile(glat, 50)), "p99_ms": 1e3 * float(np.percentile(glat, 99)),
+        "plain_jobs_per_s": gpj, "plain_p50_ms": 1e3 * float(np.percentile(gplat, 50)),
+        "plain_p99_ms": 1e3 * float(np.percentile(gplat, 99)),
+        "plain_service_s": sum(gservice)}
+    generated["speedup_vs_plain"] = generated["jobs_per_s"] / gpj
+    log(f"[bench] with generation on admission, rate {rates[-1]}/s: served "
+        f"{generated['jobs_per_s']:.1f} jobs/s p50 {generated['p50_ms']:.1f} ms p99 "
+        f"{generated['p99_ms']:.1f} ms | plain {gpj:.1f} jobs/s")
+    # Parity legs (untimed): every request's labels and metrics, resident
+    # and generated, against the oracle's fixture.
+    recs, labels, _, _ = serve_leg(srv, jobs, rates[-1], rng, theta, rounds, eps, graphs=graphs,
+                                   keep=True)
+    last_results = [(r.result, lab) for r, lab in zip(recs, labels)]
+    recs, glabels, _, _ = serve_leg(srv, jobs, rates[-1], rng, theta, rounds, eps, graphs=None,
+                                    keep=True)
     srv.close()
     fx_path = FIXTURES / "large_c5.json"
     fx = json.loads(fx_path.read_text()) if fx_path.exists() else None
-    parity = job_parity(fx, last_results) if last_results else None
+    parity = job_parity(fx, last_results)
+    gparity = job_parity(fx, [(r.result, lab) for r, lab in zip(recs, glabels)])
     if parity is not None:
         parity["fixture"] = str(fx_path.relative_to(ROOT))
+        parity["generated_leg_ok"] = gparity["ok"] if gparity else None
+        parity["ok"] = bool(parity["ok"] and parity["generated_leg_ok"])
     parity = parity or {"ok": None}
     log(f"[bench] parity {json.dumps(parity)}")
+    for g in graphs:
+        g.free()
     cpu = None
     if not a.no_cpu_baseline:
         cpu = oracle_jobs_baseline(jobs, rounds, eps, 15.0, "jobs/s")
@@ -363,9 +414,12 @@ def run_served(a):
         "data": "synthetic (project generator, seeded)",
         "config": {"workload": C5_WORKLOAD, "jobs_per_rate": len(jobs), "rates_per_s": rates,
                    "theta": theta, "max_rounds": rounds, "epsilon": eps,
-                   "max_active": a.max_active, "budget_gib": 200},
+                   "max_active": a.max_active, "budget_gib": 200,
+                   "inputs": "CSR resident in HBM before timing (generated_on_admission: the "
+                             "server generates each graph)"},
         "per_rate": per_rate,
         "speedup_vs_plain_at_saturation": top["jobs_per_s"] / top["plain_jobs_per_s"],
+        "generated_on_admission": generated,
         "plain_device_s": sum(dev_ms) / 1e3,
         "plain_service_s": sum(service),
         "roofline": None,
@@ -492,7 +546,7 @@ def main(argv=None):
     ap.add_argument("--config", default=os.environ.get("DMOLP_BENCH_CONFIG", "c3"),
                     choices=sorted(CONFIGS) + ["c4", "c5"])
     ap.add_argument("--jobs", type=int, default=96, help="served mode: jobs per arrival rate")
-    ap.add_argument("--max-active", type=int, default=32, help="served mode: admitted jobs")
+    ap.add_argument("--max-active", type=int, default=96, help="served mode: admitted jobs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe-atomics", action="store_true",
                     help="only measure the scattered-atomic ceiling at the config's vertex count")

@@ -253,6 +253,7 @@ int dmolp_ctx_destroy(dmolp_ctx *ctx) {
     for (hipEvent_t e : ctx->poll_ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->poll_host) (void)hipHostFree(ctx->poll_host);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DMOLP_OK;

@@ -364,6 +364,21 @@ struct dmolp_ctx {
     // behind each round, read one round later (see round_poll).
     int32_t *poll_host = nullptr;
     hipEvent_t poll_ev[2] = {nullptr, nullptr};
+    // Pinned host staging for device-to-host copies of vertex arrays (grows,
+    // freed with the context).
+    int32_t *pin = nullptr;
+    size_t pin_count = 0;
+    int32_t *pinned_i32(size_t count) {
+        if (count > pin_count) {
+            if (pin) (void)hipHostFree(pin);
+            pin = nullptr;
+            pin_count = 0;
+            DMOLP_HIP_CHECK(hipHostMalloc((void **)&pin, sizeof(int32_t) * (count > 0 ? count : 1),
+                                          hipHostMallocDefault));
+            pin_count = count;
+        }
+        return pin;
+    }
 };
 
 struct dmolp_graph {

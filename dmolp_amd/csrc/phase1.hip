@@ -1192,11 +1192,16 @@ void host_hub_search(std::vector<P1Stage> &H, const std::function<void()> &durin
     for (size_t j = 0; j < G; ++j) order[j] = j;
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return H[a].deg.size() > H[b].deg.size(); });
+    // A job of at least 2^22 vertices splits its own long partitions over 4
+    // threads: the largest search is the critical path.
     std::atomic<size_t> next{0};
     std::vector<std::thread> pool;
     for (size_t t = 0; t < T; ++t)
         pool.emplace_back([&]() {
-            for (size_t i = next.fetch_add(1); i < G; i = next.fetch_add(1)) p1_hub_search(H[order[i]]);
+            for (size_t i = next.fetch_add(1); i < G; i = next.fetch_add(1)) {
+                P1Stage &h = H[order[i]];
+                p1_hub_search(h, h.deg.size() >= (size_t(1) << 22) ? 4 : 1);
+            }
         });
     if (during) during();
     for (auto &th : pool) th.join();
@@ -1262,17 +1267,37 @@ void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     DMOLP_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// The degrees go to the host through the context's pinned staging buffer
+// (a copy into pageable memory is staged by the runtime in small pieces, at
+// a fraction of the link's rate), then into each job's vector on threads.
 void p1_degrees(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     hipStream_t s = ctx->stream;
     const LaunchSet &ls = sd.ls;
     hipLaunchKernelGGL(k_degrees, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, sd.jobs.as<P1Job>(),
                        ls.bs(P_GRID), sd.G);
-    for (P1Stage *h : H) {
-        h->deg.resize((size_t)h->g->n);
-        DMOLP_HIP_CHECK(hipMemcpyAsync(h->deg.data(), h->w->deg.p, sizeof(int32_t) * (size_t)h->g->n,
-                                       hipMemcpyDeviceToHost, s));
+    DMOLP_HIP_CHECK(hipGetLastError());
+    size_t total = 0;
+    std::vector<size_t> at(H.size());
+    for (size_t j = 0; j < H.size(); ++j) {
+        at[j] = total;
+        total += (size_t)H[j]->g->n;
     }
+    int32_t *pin = ctx->pinned_i32(total);
+    for (size_t j = 0; j < H.size(); ++j)
+        DMOLP_HIP_CHECK(hipMemcpyAsync(pin + at[j], H[j]->w->deg.p, sizeof(int32_t) * (size_t)H[j]->g->n,
+                                       hipMemcpyDeviceToHost, s));
     DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    for (P1Stage *h : H) h->deg.resize((size_t)h->g->n);
+    // Copy out in slices of about equal size across the jobs.
+    const int T = total >= (size_t(1) << 22) ? std::min(16, host_threads()) : 1;
+    run_threads(T, [&](int t) {
+        const size_t a = total * (size_t)t / (size_t)T, b = total * (size_t)(t + 1) / (size_t)T;
+        for (size_t j = 0; j < H.size(); ++j) {
+            const size_t lo = std::max(a, at[j]), hi = std::min(b, at[j] + H[j]->deg.size());
+            if (lo < hi)
+                std::memcpy(H[j]->deg.data() + (lo - at[j]), pin + lo, sizeof(int32_t) * (hi - lo));
+        }
+    });
 }
 
 void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
