@@ -805,8 +805,21 @@ void job_blocks(const dmolp_ctx *ctx, int64_t n, int64_t nnz, int groups, int64_
     out[P_HEAVY] = (int32_t)(hb < 1 ? 1 : (hb > 1024 ? 1024 : hb));
 }
 
-void launch_set_build(dmolp_ctx *ctx, LaunchSet &ls, const std::vector<std::array<int32_t, P_NPOL>> &blocks) {
-    const int J = (int)blocks.size();
+void launch_set_build(dmolp_ctx *ctx, LaunchSet &ls, const std::vector<std::array<int32_t, P_NPOL>> &blocks_in) {
+    const int J = (int)blocks_in.size();
+    // Grid-stride shares of a set whose jobs together take less than the
+    // full grid (a few jobs of a served mix) are scaled up in proportion,
+    // so the set's launches still fill the CUs; a job's results do not
+    // depend on its share.
+    std::vector<std::array<int32_t, P_NPOL>> blocks = blocks_in;
+    const int64_t full = ctx->grid_blocks();
+    int64_t sum = 0;
+    for (const auto &b : blocks) sum += b[P_GRID];
+    if (J > 1 && sum > 0 && sum < full)
+        for (auto &b : blocks) {
+            const int64_t x = (int64_t)b[P_GRID] * full / sum;
+            b[P_GRID] = (int32_t)(x < 1 ? 1 : (x > full ? full : x));
+        }
     std::vector<int32_t> h((size_t)P_NPOL * (J + 1));
     for (int p = 0; p < P_NPOL; ++p) {
         int64_t run = 0;
