@@ -178,6 +178,11 @@ class Context:
                                                     C.byref(n)), self._h)
         return out
 
+    def set_hub_prefix(self, prefix: int):
+        """Hubs phase 1 orders before its seed search (0: all); results do
+        not depend on it (dmolp_ctx_set_hub_prefix)."""
+        _check(_abi.lib().dmolp_ctx_set_hub_prefix(self._h, int(prefix)), self._h)
+
     def probe_atomics(self, rows: int, share: int, reps: int = 32) -> tuple[float, int]:
         """(ms, adds) of scattered 32-bit adds into `rows` 64-byte rows, groups
         of `share` lanes sharing a row (dmolp_probe_atomics)."""

@@ -330,6 +330,12 @@ int dmolp_hubs_host(const int32_t *deg, int64_t n, int32_t *hubs_out, int64_t *n
     });
 }
 
+int dmolp_ctx_set_hub_prefix(dmolp_ctx *ctx, int64_t prefix) {
+    if (!ctx || prefix < 0) return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    ctx->hub_prefix = prefix;
+    return DMOLP_OK;
+}
+
 int dmolp_gen_host(const dmolp_gen_spec *spec, dmolp_csr *out) {
     return guarded(nullptr, [&] { gen_host(spec, out); });
 }

@@ -368,6 +368,7 @@ struct dmolp_ctx {
     // freed with the context).
     int32_t *pin = nullptr;
     size_t pin_count = 0;
+    int64_t hub_prefix = int64_t(1) << 16;  // hubs a phase-1 host search orders first (0: all)
     int32_t *pinned_i32(size_t count) {
         if (count > pin_count) {
             if (pin) (void)hipHostFree(pin);

@@ -788,6 +788,7 @@ void run_threads(int T, F &&body) {
 // Ranges at least this long are partitioned by several threads.
 constexpr int64_t kParPartitionMin = int64_t(1) << 20;
 
+
 // The partition step of libstdc++'s __introsort_loop on a[first, last)
 // after the median has been moved to a[first] (__unguarded_partition of
 // [first + 1, last) around a[first]), sequential.  Returns the cut.
@@ -987,9 +988,11 @@ void introsort_top(DegPair *a, int64_t n, int64_t keep, int threads) {
 // the total whatever the order (every partial sum is an integer below
 // 2^53), so the number of hubs is known before sorting and only the top of
 // the sorted order is produced (introsort_top).
-std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads) {
+std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads, int64_t limit,
+                                    int64_t *m_out) {
     const int64_t n = (int64_t)deg.size();
     std::vector<int32_t> hubs;
+    if (m_out) *m_out = 0;
     const int N = (int)n;
     if (N < 2) return hubs;
     const int T = n >= kParPartitionMin ? std::max(1, threads) : 1;
@@ -1027,7 +1030,9 @@ std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads
         if (x > xi) ++M;
         else break;
     }
+    if (m_out) *m_out = M;
     if (M == 0) return hubs;
+    if (limit >= 0 && limit < M) M = limit;  // only the first `limit` hubs of the order
     std::unique_ptr<DegPair[]> sd(new DegPair[(size_t)n]);  // (id, degree), descending id
     run_threads(T, [&](int t) {
         for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i) {
@@ -1127,7 +1132,12 @@ P1Job p1_job(dmolp_ctx *ctx, dmolp_graph *g, Phase1Work &w, int32_t *remaining) 
 void p1_hub_search(P1Stage &h, int threads) {
     const auto t0 = std::chrono::steady_clock::now();
     h.fs = first_seed_host(h.deg);
-    h.hubs = find_hubs_host(h.deg, threads);
+    // Only the first hub_prefix hubs are ordered: a seed search walks the
+    // hub list from the top and stops at its k-th seed, which on graphs with
+    // many hubs (near-regular ones, where every vertex is a hub) lies far
+    // above that; a search that runs out of listed hubs is repeated with
+    // the whole list (p1_seeds).
+    h.hubs = find_hubs_host(h.deg, threads, h.hub_prefix > 0 ? h.hub_prefix : -1, &h.n_hubs);
     const auto t1 = std::chrono::steady_clock::now();
     h.host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     // Candidates of the seed fallback: the first 2k vertices by (degree
@@ -1219,6 +1229,7 @@ void p1_stage_init(dmolp_ctx *ctx, P1Item &it, P1Stage &h) {
     h.g = g;
     h.k = it.k;
     h.theta = it.theta;
+    h.hub_prefix = ctx->hub_prefix;
 }
 
 void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
@@ -1356,7 +1367,27 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
         ob.download(&hc[(size_t)j], h.w->ctl.p, sizeof(P1Ctl));
     }
     ob.run(ctx);
+    // A search that ran out of hubs on a shortened hub list would have gone
+    // on into the rest of the list: such jobs are searched again, from the
+    // start, with the whole list (a search that found all its seeds stopped
+    // inside the prefix, where both lists agree).
+    std::vector<P1Stage *> redo;
+    std::vector<uint8_t> redone((size_t)G, 0);
     for (int j = 0; j < G; ++j) {
+        P1Stage &h = *H[(size_t)j];
+        if (hc[(size_t)j].seed_stop && (int64_t)h.hubs.size() < h.n_hubs) {
+            redo.push_back(&h);
+            redone[(size_t)j] = 1;
+        }
+    }
+    if (!redo.empty()) {
+        for (P1Stage *h : redo)
+            h->hubs = find_hubs_host(h->deg, std::min(16, host_threads()), -1, &h->n_hubs);
+        p1_seeds(ctx, redo, sd);
+        p1_set_build(ctx, H, sd);  // the caller's set again
+    }
+    for (int j = 0; j < G; ++j) {
+        if (redone[(size_t)j]) continue;
         P1Stage &h = *H[(size_t)j];
         const int k = h.k;
         const int64_t n = h.g->n;
@@ -1524,7 +1555,7 @@ void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double devic
     std::memset(&m, 0, sizeof(m));
     m.first_seed = h.fs.first;
     m.first_seed_degree = h.fs.second;
-    m.n_hubs = (int64_t)h.hubs.size();
+    m.n_hubs = h.n_hubs;
     m.n_seeds_found = h.found;
     m.n_seeds_fallback = h.fallback;
     m.rounds = hc.rounds;

@@ -66,7 +66,9 @@ struct P1Stage {
     dmolp_graph *g = nullptr;
     Phase1Work *w = nullptr;
     int k = 0, theta = 0;
-    std::vector<int32_t> deg, hubs, seeds;
+    std::vector<int32_t> deg, hubs, seeds;  // hubs: the first of the n_hubs hubs (p1_hub_search)
+    int64_t n_hubs = 0;
+    int64_t hub_prefix = 0;  // hubs ordered by the first search (0: all)
     std::vector<int32_t> top;  // first 2k vertices by (degree desc, id asc)
     std::vector<uint8_t> used;
     std::pair<int32_t, int32_t> fs{-1, -1};
@@ -96,7 +98,10 @@ void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double devic
 Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
 // threads: host threads for the long partition steps and passes (1: all
 // on the calling thread); the result does not depend on it.
-std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads = 1);
+// limit >= 0: only the first min(limit, M) hubs of the order; m_out gets M,
+// the number of hubs.
+std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads = 1,
+                                    int64_t limit = -1, int64_t *m_out = nullptr);
 std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg);
 // Seeds + expansion; labels become resident in g.  Metrics fields about
 // seeds, rounds and timing are filled (cut and counts come from the stats

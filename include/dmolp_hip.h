@@ -212,6 +212,12 @@ int dmolp_graph_labels_checkpoint(dmolp_ctx *ctx, dmolp_graph *g, int op);
  * reference's order (its std::sort's order of equal degrees included).
  * hubs_out has capacity n (nullable); no device work. */
 int dmolp_hubs_host(const int32_t *deg, int64_t n, int32_t *hubs_out, int64_t *n_hubs);
+/* How many hubs dmolp_phase1's host search orders before the seed search
+ * (default 65536; 0: all of them).  A seed search that runs out of the
+ * ordered hubs is repeated with all of them, so the result does not depend
+ * on this; it only bounds the host sort (on near-regular graphs every
+ * vertex is a hub).  Test hook; no counterpart in the reference. */
+int dmolp_ctx_set_hub_prefix(dmolp_ctx *ctx, int64_t prefix);
 
 /* Phase1Metrics (graph_types.h:276-309) with int64 counters. */
 typedef struct {
