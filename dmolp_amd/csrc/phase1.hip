@@ -12,6 +12,8 @@
 //             unlabelled neighbour goes to the lowest claiming partition
 //             (atomicMin); frontiers become carry + newly labelled.
 #include <algorithm>
+#include <mutex>
+#include <exception>
 #include <atomic>
 #include <chrono>
 #include <climits>
@@ -1186,35 +1188,54 @@ void host_hub_search(std::vector<P1Stage> &H, const std::function<void()> &durin
     const size_t G = H.size();
     size_t T = (size_t)std::min(16, host_threads());
     if (T > G) T = G;
-    if (T <= 1) {  // one job: its passes and long partitions use the threads
-        if (during) {
-            std::thread th([&] {
-                for (size_t j = 0; j < G; ++j) p1_hub_search(H[j], std::min(16, host_threads()));
-            });
-            during();
-            th.join();
-        } else {
-            for (size_t j = 0; j < G; ++j) p1_hub_search(H[j], std::min(16, host_threads()));
-        }
-        return;
-    }
-    std::vector<size_t> order(G);
-    for (size_t j = 0; j < G; ++j) order[j] = j;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t a, size_t b) { return H[a].deg.size() > H[b].deg.size(); });
-    // A job of at least 2^22 vertices splits its own long partitions over 4
-    // threads: the largest search is the critical path.
-    std::atomic<size_t> next{0};
+    // The first exception of a worker or of `during` is rethrown once every
+    // thread has been joined.
+    std::exception_ptr err;
+    std::mutex err_mu;
+    const auto keep = [&](std::exception_ptr e) {
+        std::lock_guard<std::mutex> lk(err_mu);
+        if (!err) err = e;
+    };
     std::vector<std::thread> pool;
-    for (size_t t = 0; t < T; ++t)
-        pool.emplace_back([&]() {
-            for (size_t i = next.fetch_add(1); i < G; i = next.fetch_add(1)) {
-                P1Stage &h = H[order[i]];
-                p1_hub_search(h, h.deg.size() >= (size_t(1) << 22) ? 4 : 1);
+    std::vector<size_t> order(G);  // outlives the workers: joined below
+    std::atomic<size_t> next{0};
+    if (T <= 1) {  // one job: its passes and long partitions use the threads
+        const auto all = [&] {
+            try {
+                for (size_t j = 0; j < G; ++j) p1_hub_search(H[j], std::min(16, host_threads()));
+            } catch (...) {
+                keep(std::current_exception());
             }
-        });
-    if (during) during();
+        };
+        if (during) pool.emplace_back(all);
+        else all();
+    } else {
+        for (size_t j = 0; j < G; ++j) order[j] = j;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return H[a].deg.size() > H[b].deg.size(); });
+        // A job of at least 2^22 vertices splits its own long partitions over 4
+        // threads: the largest search is the critical path.
+        for (size_t t = 0; t < T; ++t)
+            pool.emplace_back([&]() {
+                try {
+                    for (size_t i = next.fetch_add(1); i < G; i = next.fetch_add(1)) {
+                        P1Stage &h = H[order[i]];
+                        p1_hub_search(h, h.deg.size() >= (size_t(1) << 22) ? 4 : 1);
+                    }
+                } catch (...) {
+                    keep(std::current_exception());
+                }
+            });
+    }
+    if (during) {
+        try {
+            during();
+        } catch (...) {
+            keep(std::current_exception());
+        }
+    }
     for (auto &th : pool) th.join();
+    if (err) std::rethrow_exception(err);
 }
 
 }  // namespace

@@ -18,6 +18,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -119,8 +120,15 @@ struct dmolp_server {
                 j = hubq.front();
                 hubq.pop_front();
             }
-            p1_hub_search(j->p1);
-            j->hubs_ready.store(1, std::memory_order_release);
+            int ready = 1;
+            try {
+                p1_hub_search(j->p1);
+            } catch (const std::bad_alloc &) {
+                ready = DMOLP_E_NOMEM;  // negative: the job finishes with this status
+            } catch (...) {
+                ready = DMOLP_E_INTERNAL;
+            }
+            j->hubs_ready.store(ready, std::memory_order_release);
         }
     }
     ~dmolp_server() {
@@ -264,11 +272,23 @@ void server_step(dmolp_server *s, int32_t *active_out, int32_t *queued_out) {
     // 2. Seeds for jobs whose hub search is done; they enter expansion.
     std::vector<P1Stage *> seeded;
     std::vector<SrvJob *> seeded_jobs;
-    for (SrvJob *j : s->active)
-        if (j->st == JS_HUBS && j->hubs_ready.load(std::memory_order_acquire)) {
+    std::vector<SrvJob *> failed;
+    for (SrvJob *j : s->active) {
+        if (j->st != JS_HUBS) continue;
+        const int ready = j->hubs_ready.load(std::memory_order_acquire);
+        if (ready > 0) {
             seeded.push_back(&j->p1);
             seeded_jobs.push_back(j);
+        } else if (ready < 0) {
+            failed.push_back(j);
         }
+    }
+    if (!failed.empty()) {
+        for (SrvJob *j : failed) finish_job(s, j, j->hubs_ready.load(std::memory_order_acquire));
+        s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
+                                       [](SrvJob *j) { return j->st == JS_DONE; }),
+                        s->active.end());
+    }
     if (!seeded.empty()) {
         p1_seeds(ctx, seeded, s->seeds_set);
         p1_expand_begin(ctx, seeded, s->seeds_set);
