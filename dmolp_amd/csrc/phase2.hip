@@ -208,9 +208,13 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
     uint32_t *lh = J.lh;
     const int E = k < kLpCap ? k : kLpCap;
     const int nvec = (E + 7) / 8;  // 16-byte vectors of a row that hold labels below E
-    // With kept histograms and every label counted (k <= 32), the row's
-    // differing, intra and invalid counts follow from its histogram.
-    const bool from_hist = lh_on && k <= kLpCap;
+    // Whenever labels fit the byte mirror (k <= kExact8) a row's differing,
+    // intra and invalid counts follow from per-row counts (count_rows): the
+    // kept light row takes the entries whose label is below E, and s_cnt
+    // counts, in its low half, the entries whose label is -1 and, in its
+    // high half, those carrying the row's own label when that label has no
+    // place in the kept row (k > 32, or no kept rows).
+    const bool from_hist = exact8;
     // Packed gathers when every label is valid (see k_pack_labels).
     const int packed = (J.lbits != 0 && ctl->v[C_BADLAB] == 0) ? J.lbits : 0;
     __shared__ BatchLds s_b[kWavesPerBlock];
@@ -228,6 +232,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
         bool valid;
         const int32_t total = batch_load(g, all.light, count, b0, s_b[w], v, deg, valid);
         s_cnt[w][lane] = 0;
+        s_b[w].aux[lane] = valid ? (int32_t)L8[v] : -2;  // the row's own label byte
         if (lh_on)
             for (int l = 0; l < 4 * nvec; ++l) h[lane * kPackedStride + l] = 0u;
         wave_sync();
@@ -261,8 +266,12 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
 #pragma unroll
                 for (int j = 0; j < kLightUnroll; ++j) {
                     if (b[j] < 0) continue;
-                    if (b[j] < E) atomicAdd(&h[slot[j] * kPackedStride + (b[j] >> 1)], 1u << ((b[j] & 1) * 16));
-                    else atomicAdd(&s_cnt[w][slot[j]], b[j] == kLabel8None ? 1 : 0x10000);
+                    if (lh_on && b[j] < E)
+                        atomicAdd(&h[slot[j] * kPackedStride + (b[j] >> 1)], 1u << ((b[j] & 1) * 16));
+                    else if (b[j] == kLabel8None)
+                        atomicAdd(&s_cnt[w][slot[j]], 1);
+                    else if (b[j] == s_b[w].aux[slot[j]])
+                        atomicAdd(&s_cnt[w][slot[j]], 0x10000);
                 }
             }
         };
@@ -303,18 +312,21 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
                     dst[q] = make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
             }
             if (from_hist) {
-                // The row's counts from its histogram: `same` entries share
-                // v's label, `inv` have label -1 (byte 255), the rest differ.
-                // Labels in [k, 255) are counted as differing, as the entry
-                // loop would; their vertices raise the kernel error anyway.
-                const int32_t oc = s_cnt[w][lane];
-                const int32_t inv = oc & 0xffff;
+                // The row's counts: `same` entries share v's label (from the
+                // kept row when the label has a place there, else from
+                // s_cnt's high half), `inv` have label -1 (byte 255), the
+                // rest differ.  Light rows are shorter than 2^16 entries, so
+                // the 16-bit halves do not overflow.
+                const uint32_t oc = (uint32_t)s_cnt[w][lane];
+                const int32_t inv = (int32_t)(oc & 0xffffu);
                 if (lv == -1) {
                     invalid += deg;
                     diff[v] = 0;
                 } else {
                     const int32_t same =
-                        (lv >= 0 && lv < E) ? (int32_t)((r_word(h, lane, lv >> 1) >> ((lv & 1) * 16)) & 0xffffu) : 0;
+                        (lh_on && lv >= 0 && lv < E)
+                            ? (int32_t)((r_word(h, lane, lv >> 1) >> ((lv & 1) * 16)) & 0xffffu)
+                            : (int32_t)(oc >> 16);
                     const int32_t dif = deg - inv - same;
                     invalid += inv;
                     cut2 += dif;

@@ -75,8 +75,10 @@ def test_stats_parity(tuned, name):
     off, col = TINY[name]
     g = tuned.upload(off, col)
     rng = np.random.default_rng(1)
-    # k <= 16 and 17..32 take the packed label mirror (4 and 5 bits), 40 the bytes.
-    for k in (1, 3, 8, 16, 20, 32, 40):
+    # k <= 16 and 17..32 take the packed label mirror (4 and 5 bits), 40-254
+    # the bytes (labels from 32 on have no place in the kept rows), 255 and
+    # 300 the 32-bit labels.
+    for k in (1, 3, 8, 16, 20, 32, 40, 64, 128, 254, 255, 300):
         labels = rng.integers(0, k, len(off) - 1).astype(np.int32)
         rc, vc, ic, cut = O.stats(off, col, k, labels)
         assert rc == 0
