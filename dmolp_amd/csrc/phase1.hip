@@ -679,16 +679,29 @@ __global__ __launch_bounds__(kWgThreads) void k_p1_rounds_wg(const P1Job *JS, in
                 const int32_t total = batch_load(J.g, cur.x, c, b0, s_b[w], v, deg, valid);
                 s_b[w].aux[lane] = valid ? ld_rel(J.L + v) : 0;
                 wave_sync();
-                for (int32_t q0 = 0; q0 < total; q0 += kWave) {
-                    const int32_t q = q0 + lane;
-                    if (q < total) {
-                        const int slot = batch_slot(s_b[w], q);
-                        const int32_t u = J.g.col[s_b[w].base[slot] + q];
-                        const int32_t p = s_b[w].aux[slot];
-                        if (ld_rel(J.L + u) == -1 && ld_rel(J.claims + u) > p &&
-                            atomicMin(J.claims + u, p) == kNoClaim)
-                            J.touched[atomicAdd(&s_touched, 1ull)] = u;
+                // kExpUnroll entries a lane per step, each stage's loads
+                // issued together (columns, then labels, then the claims of
+                // unlabelled neighbours), so a round waits for about three
+                // memory latencies rather than three per entry.
+                for (int32_t q0 = 0; q0 < total; q0 += kWave * kExpUnroll) {
+                    int32_t u[kExpUnroll], p[kExpUnroll], lv[kExpUnroll], cl[kExpUnroll];
+                    bool on[kExpUnroll];
+#pragma unroll
+                    for (int j = 0; j < kExpUnroll; ++j) {
+                        const int32_t q = q0 + j * kWave + lane;
+                        on[j] = q < total;
+                        const int slot = on[j] ? batch_slot(s_b[w], q) : 0;
+                        u[j] = J.g.col[on[j] ? s_b[w].base[slot] + q : 0];
+                        p[j] = s_b[w].aux[slot];
                     }
+#pragma unroll
+                    for (int j = 0; j < kExpUnroll; ++j) lv[j] = ld_rel(J.L + u[j]);
+#pragma unroll
+                    for (int j = 0; j < kExpUnroll; ++j) cl[j] = ld_rel(J.claims + (lv[j] == -1 ? u[j] : 0));
+#pragma unroll
+                    for (int j = 0; j < kExpUnroll; ++j)
+                        if (on[j] && lv[j] == -1 && cl[j] > p[j] && atomicMin(J.claims + u[j], p[j]) == kNoClaim)
+                            J.touched[atomicAdd(&s_touched, 1ull)] = u[j];
                 }
                 wave_sync();
             }
