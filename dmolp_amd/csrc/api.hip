@@ -245,6 +245,7 @@ int dmolp_ctx_destroy(dmolp_ctx *ctx) {
     if (!ctx) return DMOLP_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->pool) ctx->pool->close();  // memory still held elsewhere is freed when released
     ctx->scratch.release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -255,6 +256,8 @@ int dmolp_ctx_destroy(dmolp_ctx *ctx) {
     if (ctx->poll_host) (void)hipHostFree(ctx->poll_host);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     delete ctx;
     return DMOLP_OK;
 }

@@ -359,6 +359,16 @@ struct dmolp_ctx {
     hipEvent_t prof_event();
     int grid_blocks() const { return tune.grid_blocks > 0 ? tune.grid_blocks : cu_count * 8; }
     dmolp::P1SetDev p1_set;   // phase-1 job set of the plain and batched paths
+    // A second stream (default priority) and the set it runs: a batch's
+    // long one-workgroup phase-1 launch beside the main stream's launches.
+    dmolp::P1SetDev p1_side;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr;
+    void ensure_side() {
+        if (side) return;
+        DMOLP_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        DMOLP_HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    }
     dmolp::BatchState batch;  // phase-2 job set of dmolp_batch_run
     // Stop polling of round loops: a device flag copied to pinned memory
     // behind each round, read one round later (see round_poll).

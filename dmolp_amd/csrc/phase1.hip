@@ -542,6 +542,7 @@ __global__ void k_p1_round_end(const P1Job *JS, const int32_t *bstart, int jobs)
 // the launch, are read and written with device-scope atomic operations
 // (they bypass the vector L1).
 constexpr int kWgThreads = 1024;
+constexpr int kWgProbeRounds = 32;  // first one-workgroup attempt of a batch (even)
 constexpr int kWgWaves = kWgThreads / kWave;
 constexpr int64_t kWgMaxMembers = 16384;  // frontier members of a round run here
 constexpr int64_t kWgMaxEntries = 32768;  // and their row entries
@@ -575,9 +576,17 @@ __global__ __launch_bounds__(kWgThreads) void k_p1_rounds_wg(const P1Job *JS, in
     P1Ctl *ctl = J.ctl;
     const int k = J.k;
     const int64_t n = J.g.n;
-    if (ctl->done) return;  // uniform: nothing writes it before this point
+    // out[2 + job]: 0 the job ran max_rounds rounds here, 1 it stopped at a
+    // round the separate launches must run, 2 it is finished.
+    if (ctl->done) {  // uniform: nothing writes it before this point
+        if (tid == 0) out[2 + blockIdx.x] = 2;
+        return;
+    }
     if ((int64_t)*J.F[0].cnt > kWgMaxMembers) {  // too large to count here
-        if (tid == 0) atomicAdd(&out[1], 1);
+        if (tid == 0) {
+            atomicAdd(&out[1], 1);
+            out[2 + blockIdx.x] = 1;
+        }
         return;
     }
     // The entry frontier (F[0]): per-part counts and largest degree.
@@ -748,6 +757,7 @@ __global__ __launch_bounds__(kWgThreads) void k_p1_rounds_wg(const P1Job *JS, in
     if (tid == 0) {
         atomicMax(&out[0], t);
         if (st == 1) atomicAdd(&out[1], 1);
+        out[2 + blockIdx.x] = st == 1 ? 1 : (st == 2 ? 2 : 0);
     }
 }
 
@@ -1546,20 +1556,33 @@ void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds) {
 // Up to max_rounds (even) rounds of every job of the set that k_p1_rounds_wg
 // can run, one block per job.  ran: the most rounds a job ran; stopped: jobs
 // that stopped at a round the separate launches must run.
+namespace {
+void p1_rounds_wg_launch(P1SetDev &sd, int max_rounds, hipStream_t s) {
+    sd.run_out.grow(sizeof(int32_t) * (2 + (size_t)sd.G));
+    DMOLP_HIP_CHECK(hipMemsetAsync(sd.run_out.p, 0, sizeof(int32_t) * (2 + (size_t)sd.G), s));
+    hipLaunchKernelGGL(k_p1_rounds_wg, dim3(sd.G), dim3(kWgThreads), 0, s, sd.jobs.as<P1Job>(),
+                       max_rounds, sd.run_out.as<int32_t>());
+    DMOLP_HIP_CHECK(hipGetLastError());
+}
+// ran, stopped and (states) every job's state after the launch on s.
+void p1_rounds_wg_results(P1SetDev &sd, hipStream_t s, int &ran, int &stopped,
+                          std::vector<int32_t> *states) {
+    std::vector<int32_t> out(2 + (size_t)sd.G, 0);
+    DMOLP_HIP_CHECK(hipMemcpyAsync(out.data(), sd.run_out.p, sizeof(int32_t) * out.size(),
+                                   hipMemcpyDeviceToHost, s));
+    DMOLP_HIP_CHECK(hipStreamSynchronize(s));
+    ran = out[0];
+    stopped = out[1];
+    if (states) states->assign(out.begin() + 2, out.end());
+}
+}  // namespace
+
 void p1_rounds_wg(dmolp_ctx *ctx, P1SetDev &sd, int max_rounds, int &ran, int &stopped) {
     ran = 0;
     stopped = 0;
     if (sd.G < 1) return;
-    sd.run_out.grow(sizeof(int32_t) * 2);
-    DMOLP_HIP_CHECK(hipMemsetAsync(sd.run_out.p, 0, sizeof(int32_t) * 2, ctx->stream));
-    hipLaunchKernelGGL(k_p1_rounds_wg, dim3(sd.G), dim3(kWgThreads), 0, ctx->stream, sd.jobs.as<P1Job>(),
-                       max_rounds, sd.run_out.as<int32_t>());
-    DMOLP_HIP_CHECK(hipGetLastError());
-    int32_t out[2] = {0, 0};
-    DMOLP_HIP_CHECK(hipMemcpyAsync(out, sd.run_out.p, sizeof(out), hipMemcpyDeviceToHost, ctx->stream));
-    DMOLP_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    ran = out[0];
-    stopped = out[1];
+    p1_rounds_wg_launch(sd, max_rounds, ctx->stream);
+    p1_rounds_wg_results(sd, ctx->stream, ran, stopped, nullptr);
 }
 
 int32_t p1_remaining(dmolp_ctx *ctx, P1SetDev &sd) {
@@ -1630,51 +1653,115 @@ void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items, const std::funct
     int64_t enq = 0;
     const int batch = 16;        // even: every batch starts on the same frontier parity
     const int run_rounds = 4096;  // rounds of one k_p1_rounds_wg launch (even, for the same reason)
-    std::vector<P1Stage *> act = HP;
     std::vector<int32_t> done;
-    int skip = 0, backoff = 1;
-    for (;;) {
-        // Rounds that need no selection run in one workgroup per job; a round
-        // that does (and those after it, for a batch) runs as separate
-        // launches.  After an attempt in which no job ran a round, the next
-        // ones wait for 1, 2, 4, up to 16 batches.
-        int ran = 0, stopped = 1;
-        if (skip > 0) {
-            --skip;
-        } else {
-            p1_rounds_wg(ctx, sd, run_rounds, ran, stopped);
-            if (ran == 0) {
-                skip = backoff;
-                backoff = std::min(2 * backoff, 16);
+    // The rounds of the jobs of `act` (the jobs of sd) on the main stream,
+    // until every one has finished.
+    const auto run_main = [&](std::vector<P1Stage *> act) {
+        int skip = 0, backoff = 1;
+        while (!act.empty()) {
+            // Rounds that need no selection run in one workgroup per job; a
+            // round that does (and those after it, for a batch) runs as
+            // separate launches.  After an attempt in which no job ran a
+            // round, the next ones wait for 1, 2, 4, up to 16 batches.
+            int ran = 0, stopped = 1;
+            if (skip > 0) {
+                --skip;
             } else {
-                backoff = 1;
+                p1_rounds_wg(ctx, sd, run_rounds, ran, stopped);
+                if (ran == 0) {
+                    skip = backoff;
+                    backoff = std::min(2 * backoff, 16);
+                } else {
+                    backoff = 1;
+                }
+            }
+            enq += ran;
+            int32_t left = p1_remaining(ctx, sd);
+            if (left <= 0) break;
+            if (stopped > 0) {
+                p1_expand_rounds(ctx, sd, batch);
+                enq += batch;
+                left = p1_remaining(ctx, sd);
+                if (left <= 0) break;
+            }
+            DMOLP_REQUIRE(enq < 64ll * (sd.max_n + 16), DMOLP_E_HIP, "phase-1 expansion did not finish");
+            // A finished job keeps its blocks in every launch (they return at
+            // once) and its selection pass count: once half of the set has
+            // finished, the set is rebuilt from the jobs still expanding.
+            if (act.size() > 1 && (size_t)left * 2 <= act.size()) {
+                p1_done_flags(ctx, sd, done);
+                std::vector<P1Stage *> keep;
+                for (size_t j = 0; j < act.size(); ++j)
+                    if (!done[j]) keep.push_back(act[j]);
+                DMOLP_REQUIRE((int32_t)keep.size() == left, DMOLP_E_INTERNAL,
+                              "phase-1 job count disagrees with the done flags");
+                act.swap(keep);
+                p1_set_build(ctx, act, sd);
+                hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, sd.remaining.as<int32_t>(),
+                                   (int32_t)act.size());
+                DMOLP_HIP_CHECK(hipGetLastError());
             }
         }
+    };
+    // A short first attempt in one workgroup per job sorts the jobs: those
+    // still running rounds that need no selection after it (small frontiers:
+    // grids, long paths) go on in one long one-workgroup launch on a second
+    // stream, while the others advance through the separate launches on
+    // the main stream; the two touch disjoint jobs.  Whatever the long
+    // launch leaves unfinished joins the main stream afterwards.
+    std::vector<P1Stage *> act = HP, later;
+    if (G > 1) {
+        int ran = 0, stopped = 0;
+        std::vector<int32_t> st;
+        p1_rounds_wg_launch(sd, kWgProbeRounds, s);
+        p1_rounds_wg_results(sd, s, ran, stopped, &st);
         enq += ran;
-        int32_t left = p1_remaining(ctx, sd);
-        if (left <= 0) break;
-        if (stopped > 0) {
-            p1_expand_rounds(ctx, sd, batch);
-            enq += batch;
-            left = p1_remaining(ctx, sd);
-            if (left <= 0) break;
+        std::vector<P1Stage *> mainj, sidej;
+        for (int j = 0; j < G; ++j) {
+            if (st[(size_t)j] == 0) sidej.push_back(HP[(size_t)j]);
+            else if (st[(size_t)j] == 1) mainj.push_back(HP[(size_t)j]);
         }
-        DMOLP_REQUIRE(enq < 64ll * (sd.max_n + 16), DMOLP_E_HIP, "phase-1 expansion did not finish");
-        // A finished job keeps its blocks in every launch (they return at
-        // once) and its selection pass count: once half of the set has
-        // finished, the set is rebuilt from the jobs still expanding.
-        if (act.size() > 1 && (size_t)left * 2 <= act.size()) {
-            p1_done_flags(ctx, sd, done);
-            std::vector<P1Stage *> keep;
-            for (size_t j = 0; j < act.size(); ++j)
-                if (!done[j]) keep.push_back(act[j]);
-            DMOLP_REQUIRE((int32_t)keep.size() == left, DMOLP_E_INTERNAL,
-                          "phase-1 job count disagrees with the done flags");
-            act.swap(keep);
-            p1_set_build(ctx, act, sd);
-            hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, sd.remaining.as<int32_t>(),
-                               (int32_t)act.size());
+        if (!sidej.empty()) {
+            ctx->ensure_side();
+            P1SetDev &sw = ctx->p1_side;
+            p1_set_build(ctx, sidej, sw);
+            hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, sw.remaining.as<int32_t>(),
+                               (int32_t)sidej.size());
+            if (!mainj.empty()) {
+                p1_set_build(ctx, mainj, sd);
+                hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, sd.remaining.as<int32_t>(),
+                                   (int32_t)mainj.size());
+            }
             DMOLP_HIP_CHECK(hipGetLastError());
+            // The side stream starts after everything queued so far (the
+            // set uploads, and the work of whatever memory it reuses).
+            DMOLP_HIP_CHECK(hipEventRecord(ctx->ev_fork, s));
+            DMOLP_HIP_CHECK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+            p1_rounds_wg_launch(sw, run_rounds, ctx->side);
+            act = mainj;
+            later = sidej;
+        }
+    }
+    try {
+        run_main(act);
+    } catch (...) {
+        if (!later.empty()) (void)hipStreamSynchronize(ctx->side);  // no work left on its jobs
+        throw;
+    }
+    if (!later.empty()) {
+        int ran = 0, stopped = 0;
+        std::vector<int32_t> st;
+        p1_rounds_wg_results(ctx->p1_side, ctx->side, ran, stopped, &st);  // waits for the side stream
+        enq += ran;
+        std::vector<P1Stage *> rest;
+        for (size_t j = 0; j < later.size(); ++j)
+            if (st[j] != 2) rest.push_back(later[j]);
+        if (!rest.empty()) {
+            p1_set_build(ctx, rest, sd);
+            hipLaunchKernelGGL(k_p1_set_remaining, dim3(1), dim3(1), 0, s, sd.remaining.as<int32_t>(),
+                               (int32_t)rest.size());
+            DMOLP_HIP_CHECK(hipGetLastError());
+            run_main(rest);
         }
     }
     DMOLP_HIP_CHECK(hipEventRecord(ctx->ev1, s));
