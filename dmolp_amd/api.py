@@ -84,6 +84,15 @@ def hubs_host(deg: np.ndarray) -> np.ndarray:
     return out[:nh.value].copy()
 
 
+def first_seed_host(deg: np.ndarray) -> tuple[int, int]:
+    """The library's first seed of a degree sequence and its degree (host,
+    no device)."""
+    deg = np.ascontiguousarray(deg, np.int32)
+    s, d = C.c_int32(), C.c_int32()
+    _check(_abi.lib().dmolp_first_seed_host(_p(deg, C.c_int32), len(deg), C.byref(s), C.byref(d)))
+    return s.value, d.value
+
+
 def write_csr_binary(path: str, off: np.ndarray, col: np.ndarray):
     off = np.ascontiguousarray(off, np.int64)
     col = np.ascontiguousarray(col, np.int32)

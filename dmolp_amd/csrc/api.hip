@@ -333,6 +333,16 @@ int dmolp_hubs_host(const int32_t *deg, int64_t n, int32_t *hubs_out, int64_t *n
     });
 }
 
+int dmolp_first_seed_host(const int32_t *deg, int64_t n, int32_t *seed, int32_t *degree) {
+    if (!seed || !degree || n < 0 || n > 0x7fffffffll || (n > 0 && !deg)) return DMOLP_E_ARG;
+    return guarded(nullptr, [&] {
+        const std::vector<int32_t> d(deg, deg + n);
+        const std::pair<int32_t, int32_t> r = first_seed_host(d, std::min(16, host_threads()));
+        *seed = r.first;
+        *degree = r.second;
+    });
+}
+
 int dmolp_ctx_set_hub_prefix(dmolp_ctx *ctx, int64_t prefix) {
     if (!ctx || prefix < 0) return fail(ctx, DMOLP_E_ARG, "bad arguments");
     ctx->hub_prefix = prefix;

@@ -1071,15 +1071,41 @@ std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads
     return hubs;
 }
 
-// init.cpp:179-187 over the degree map's order (descending id); the tie
-// test compares an id with the running best degree.
-std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg) {
-    int64_t node = -1, best = -1;
-    for (int64_t v = (int64_t)deg.size() - 1; v >= 0; --v) {
-        const int64_t d = deg[(size_t)v];
-        if (d > best || (d == best && v < best)) { node = v; best = d; }
-    }
-    return {(int32_t)node, (int32_t)best};
+// init.cpp:179-187 over the degree map's order (descending id): the scan
+// keeps (node, best) and takes v when deg v > best, or when deg v == best
+// and v < best (the tie test compares an id with the running best degree).
+// Once best reaches the largest degree D (at the largest id v0 of degree D)
+// only the tie rule can move node, to ids of degree D below both v0 and D,
+// and the last such move is the smallest one: the result is the smallest id
+// of degree D when it lies below D and is not v0 itself, else v0.  Three
+// reductions (D, then the smallest and largest ids of degree D) on
+// `threads` threads instead of a sequential scan.
+std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg, int threads) {
+    const int64_t n = (int64_t)deg.size();
+    if (n == 0) return {-1, -1};
+    const int T = n >= kParPartitionMin ? std::max(1, threads) : 1;
+    std::vector<int32_t> tmax((size_t)T, -1);
+    run_threads(T, [&](int t) {
+        int32_t m = -1;
+        for (int64_t v = n * t / T; v < n * (t + 1) / T; ++v) m = std::max(m, deg[(size_t)v]);
+        tmax[(size_t)t] = m;
+    });
+    const int32_t D = *std::max_element(tmax.begin(), tmax.end());
+    std::vector<int64_t> tlo((size_t)T, n), thi((size_t)T, -1);
+    run_threads(T, [&](int t) {
+        int64_t lo = n, hi = -1;
+        for (int64_t v = n * t / T; v < n * (t + 1) / T; ++v)
+            if (deg[(size_t)v] == D) {
+                if (lo == n) lo = v;
+                hi = v;
+            }
+        tlo[(size_t)t] = lo;
+        thi[(size_t)t] = hi;
+    });
+    const int64_t vmin = *std::min_element(tlo.begin(), tlo.end());
+    const int64_t vmax = *std::max_element(thi.begin(), thi.end());
+    const int64_t node = (vmin < vmax && vmin < (int64_t)D) ? vmin : vmax;
+    return {(int32_t)node, D};
 }
 
 Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
@@ -1156,7 +1182,7 @@ P1Job p1_job(dmolp_ctx *ctx, dmolp_graph *g, Phase1Work &w, int32_t *remaining) 
 
 void p1_hub_search(P1Stage &h, int threads) {
     const auto t0 = std::chrono::steady_clock::now();
-    h.fs = first_seed_host(h.deg);
+    h.fs = first_seed_host(h.deg, threads);
     // Only the first hub_prefix hubs are ordered: a seed search walks the
     // hub list from the top and stops at its k-th seed, which on graphs with
     // many hubs (near-regular ones, where every vertex is a hub) lies far

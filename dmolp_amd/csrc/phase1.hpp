@@ -102,7 +102,7 @@ Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
 // the number of hubs.
 std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads = 1,
                                     int64_t limit = -1, int64_t *m_out = nullptr);
-std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg);
+std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg, int threads = 1);
 // Seeds + expansion; labels become resident in g.  Metrics fields about
 // seeds, rounds and timing are filled (cut and counts come from the stats
 // sweep).

@@ -212,6 +212,10 @@ int dmolp_graph_labels_checkpoint(dmolp_ctx *ctx, dmolp_graph *g, int op);
  * reference's order (its std::sort's order of equal degrees included).
  * hubs_out has capacity n (nullable); no device work. */
 int dmolp_hubs_host(const int32_t *deg, int64_t n, int32_t *hubs_out, int64_t *n_hubs);
+/* find_first_seed (init.cpp:179-187) as the library runs it on the host for
+ * dmolp_phase1: the first seed of a degree sequence deg[0, n) and its
+ * degree (-1, -1 when n is 0); no device work. */
+int dmolp_first_seed_host(const int32_t *deg, int64_t n, int32_t *seed, int32_t *degree);
 /* How many hubs dmolp_phase1's host search orders before the seed search
  * (default 65536; 0: all of them).  A seed search that runs out of the
  * ordered hubs is repeated with all of them, so the result does not depend

@@ -94,3 +94,44 @@ def test_host_hubs_random_sweep():
             deg[rng.integers(0, n, 5)] = int(rng.integers(10, 100))
         deg = np.ascontiguousarray(deg, np.int32)
         assert np.array_equal(oracle_hubs(deg), api.hubs_host(deg)), trial
+
+
+def oracle_first_seed(deg):
+    off = np.zeros(len(deg) + 1, np.int64)
+    off[1:] = np.cumsum(deg)
+    return O.first_seed(off)
+
+
+def first_seed_sequences():
+    """Degree sequences for find_first_seed (init.cpp:179-187): one maximum;
+    several maxima with ids below and above the maximum degree (the tie test
+    compares an id with the running best degree); maxima only above it;
+    long sequences that the library splits over threads."""
+    rng = np.random.default_rng(7)
+    out = [("one", np.array([3])), ("pair", np.array([2, 2])), ("zeros", np.zeros(9, np.int64))]
+    for n in (5, 40, 1000):
+        d = rng.integers(0, 10, n)
+        d[n // 2] = 50
+        out.append(("single-max", d))
+        d = rng.integers(0, 30, n)
+        d[[1, n // 3, n - 2]] = 30  # ids below 30 and above
+        out.append(("ties-below-and-above", d))
+        d = rng.integers(0, 5, n)
+        d[[n - 1, n - 3]] = 5  # all maxima at ids >= 5 (n >= 40 case)
+        out.append(("ties-above", d))
+    for n in (1 << 20, (1 << 21) + 17):
+        d = rng.integers(0, 100, n)
+        out.append(("many-ties", d))
+        d = np.minimum(rng.zipf(1.9, n), 5000)
+        d[rng.integers(0, n, 7)] = 5000
+        out.append(("zipf-capped", d))
+        d = rng.integers(0, 100, n)
+        d[[n - 1, n - 5]] = 100000  # maxima only at large ids
+        out.append(("big-max-high-ids", d))
+    return out
+
+
+@pytest.mark.parametrize("name,deg", first_seed_sequences(),
+                         ids=[f"{nm}-{len(d)}" for nm, d in first_seed_sequences()])
+def test_first_seed_equals_oracle_scan(name, deg):
+    assert api.first_seed_host(deg) == oracle_first_seed(deg)
