@@ -1191,9 +1191,13 @@ void p1_hub_search(P1Stage &h, int threads) {
     h.hubs = find_hubs_host(h.deg, threads, h.hub_prefix > 0 ? h.hub_prefix : -1, &h.n_hubs);
     const auto t1 = std::chrono::steady_clock::now();
     h.host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    // Candidates of the seed fallback: the first 2k vertices by (degree
-    // descending, id ascending).  At most k of them can already be seeds,
-    // so the first k - found unchosen vertices of the full order are here.
+}
+
+// Candidates of the seed fallback (p1_seeds, only for a search that found
+// fewer than k seeds): the first 2k vertices by (degree descending, id
+// ascending).  At most k of them can already be seeds, so the first
+// k - found unchosen vertices of the full order are here.
+void p1_fallback_candidates(P1Stage &h, int threads) {
     const int64_t n = (int64_t)h.deg.size();
     const size_t m = (size_t)std::min<int64_t>(n, 2 * (int64_t)h.k);
     const std::vector<int32_t> &deg = h.deg;
@@ -1467,6 +1471,7 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
         h.found = found;
         h.fallback = 0;
         if (found < k) {
+            if (h.top.empty()) p1_fallback_candidates(h, std::min(16, host_threads()));
             // Documented extension for seed.cpp:214-217 followed by the
             // out-of-range seeds[p] of partition.cpp:347: the missing seeds are
             // the remaining vertices by (degree descending, id ascending).
