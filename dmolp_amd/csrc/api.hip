@@ -258,6 +258,7 @@ int dmolp_ctx_destroy(dmolp_ctx *ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     delete ctx;
     return DMOLP_OK;
 }

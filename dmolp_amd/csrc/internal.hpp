@@ -363,11 +363,12 @@ struct dmolp_ctx {
     // long one-workgroup phase-1 launch beside the main stream's launches.
     dmolp::P1SetDev p1_side;
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     void ensure_side() {
         if (side) return;
         DMOLP_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         DMOLP_HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        DMOLP_HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     }
     dmolp::BatchState batch;  // phase-2 job set of dmolp_batch_run
     // Stop polling of round loops: a device flag copied to pinned memory

@@ -2398,6 +2398,17 @@ void phase2_init_enqueue(dmolp_ctx *ctx, const P2Set &set) {
                        ls.bs(P_GRID), J);
     hipLaunchKernelGGL(k_pack_labels, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
                        ls.bs(P_GRID), J);
+    // The heavy rows' sweep runs on the side stream beside the light rows'
+    // (disjoint rows; the counters both add to take atomic adds), and the
+    // finish waits for both.
+    if (set.any_heavy) {
+        ctx->ensure_side();
+        DMOLP_HIP_CHECK(hipEventRecord(ctx->ev_fork, s));
+        DMOLP_HIP_CHECK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        hipLaunchKernelGGL(k_init_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, ctx->side,
+                           set.jobs, ls.bs(P_GRID), J);
+        DMOLP_HIP_CHECK(hipEventRecord(ctx->ev_join, ctx->side));
+    }
     size_t i0 = 0;
     if (ctx->profiling) {
         DMOLP_HIP_CHECK(hipEventRecord(ctx->prof_event(), s));
@@ -2410,9 +2421,7 @@ void phase2_init_enqueue(dmolp_ctx *ctx, const P2Set &set) {
         if (ctx->prof_marks.size() < 9) ctx->prof_marks.resize(9);
         ctx->prof_marks[8].push_back({i0, ctx->ev_used - 1});
     }
-    if (set.any_heavy)
-        hipLaunchKernelGGL(k_init_heavy, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, set.jobs,
-                           ls.bs(P_GRID), J);
+    if (set.any_heavy) DMOLP_HIP_CHECK(hipStreamWaitEvent(s, ctx->ev_join, 0));
     hipLaunchKernelGGL(k_init_finish, dim3(ls.total[P_ONE]), dim3(256), 0, s, set.jobs, ls.bs(P_ONE),
                        J);
     DMOLP_HIP_CHECK(hipGetLastError());
