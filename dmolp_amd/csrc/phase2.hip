@@ -30,8 +30,11 @@ constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr int kHistStride = kLpCap + 1;  // odd stride: no LDS bank conflicts across slots
 // Per-block label counters (vc, ic) live in LDS for labels below kLdsK;
-// larger labels (k above kLdsK) add to the global counters directly.
-constexpr int kLdsK = 64;
+// larger labels (k above kLdsK) add to the global counters directly.  Every
+// k the byte mirror covers fits: a global counter takes the adds of rows
+// that share its label one after another (consecutive rows mostly carry
+// one label after phase 1), at the memory-side atomic rate.
+constexpr int kLdsK = 256;
 constexpr int kLpUnroll = 4;     // light rows: gathers in flight per lane
 constexpr int kHeavyUnroll = 8;  // heavy rows: gathers in flight per lane
 constexpr int kLightUnroll = 4;  // light rows in the initial sweep: gathers in flight per lane
@@ -48,15 +51,18 @@ constexpr int kPackedStride = kPackedWords + 1;  // odd: rows of a wave fall in 
 
 
 __device__ __forceinline__ bool is_done(const Ctl *c) { return c->done != 0; }
-__device__ __forceinline__ void label_add(unsigned long long *lds, unsigned long long *glob, int32_t l,
-                                          unsigned long long x) {
-    atomicAdd(l < kLdsK ? &lds[l] : &glob[l], x);
+// T is the LDS counter type: 64-bit, or 32-bit where a block's count cannot
+// reach 2^32 (one add per vertex the block handles).
+template <typename T>
+__device__ __forceinline__ void label_add(T *lds, unsigned long long *glob, int32_t l, T x) {
+    if (l < kLdsK) atomicAdd(&lds[l], x);
+    else atomicAdd(&glob[l], (unsigned long long)x);
 }
-__device__ __forceinline__ void label_flush(const unsigned long long *lds, unsigned long long *glob,
-                                            int k) {
+template <typename T>
+__device__ __forceinline__ void label_flush(const T *lds, unsigned long long *glob, int k) {
     const int kk = k < kLdsK ? k : kLdsK;
     for (int i = threadIdx.x; i < kk; i += blockDim.x)
-        if (lds[i]) atomicAdd(&glob[i], lds[i]);
+        if (lds[i]) atomicAdd(&glob[i], (unsigned long long)lds[i]);
 }
 __device__ __forceinline__ bool bit_of(const uint32_t *bits, int32_t v) {
     return (bits[v >> 5] >> (v & 31)) & 1u;
@@ -220,7 +226,10 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ int32_t s_cnt[kWavesPerBlock][kWave];
     __shared__ uint32_t s_h[kWavesPerBlock][kWave * kPackedStride];
-    __shared__ unsigned long long s_vc[kLdsK], s_ic[kLdsK];
+    // One vertex count a row: 32-bit counters keep the block within the LDS
+    // budget of six blocks a CU.
+    __shared__ uint32_t s_vc[kLdsK];
+    __shared__ unsigned long long s_ic[kLdsK];
     for (int i = threadIdx.x; i < kLdsK; i += blockDim.x) { s_vc[i] = 0; s_ic[i] = 0; }
     __syncthreads();
     const int w = threadIdx.x / kWave, lane = lane_id();
@@ -303,7 +312,7 @@ __global__ __launch_bounds__(kThreads, 6) void k_init_light(const P2Job *JS, con
         wave_sync();
         if (valid) {
             const int32_t lv = L[v];
-            if (lv >= 0 && lv < k) label_add(s_vc, vc, lv, 1ull);
+            if (lv >= 0 && lv < k) label_add(s_vc, vc, lv, 1u);
             else if (lv != -1) atomicAdd(&ctl->v[C_KERNEL_ERR], 1ull);
             if (lh_on) {
                 const uint32_t *r = &h[lane * kPackedStride];
