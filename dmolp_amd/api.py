@@ -192,6 +192,24 @@ class Context:
         not depend on it (dmolp_ctx_set_hub_prefix)."""
         _check(_abi.lib().dmolp_ctx_set_hub_prefix(self._h, int(prefix)), self._h)
 
+    def set_hub_device(self, dev_min: int, host_max: int):
+        """Phase 1 searches the hubs of graphs with at least dev_min vertices
+        on the device, finishing ranges shorter than host_max on the host;
+        results do not depend on either (dmolp_ctx_set_hub_device)."""
+        _check(_abi.lib().dmolp_ctx_set_hub_device(self._h, int(dev_min), int(host_max)), self._h)
+
+    def hubs_device(self, deg: np.ndarray, limit: int = -1, host_max: int = 1 << 15):
+        """(hubs, n_hubs, first seed, its degree) of a degree sequence by the
+        device hub search (dmolp_hubs_device)."""
+        deg = np.ascontiguousarray(deg, np.int32)
+        out = np.zeros(max(len(deg), 1), np.int32)
+        nh, s, d = C.c_int64(), C.c_int32(), C.c_int32()
+        _check(_abi.lib().dmolp_hubs_device(self._h, _p(deg, C.c_int32), len(deg), int(limit),
+                                            int(host_max), _p(out, C.c_int32), C.byref(nh),
+                                            C.byref(s), C.byref(d)), self._h)
+        m = nh.value if limit < 0 else min(nh.value, limit)
+        return out[:m].copy(), nh.value, s.value, d.value
+
     def probe_atomics(self, rows: int, share: int, reps: int = 32) -> tuple[float, int]:
         """(ms, adds) of scattered 32-bit adds into `rows` 64-byte rows, groups
         of `share` lanes sharing a row (dmolp_probe_atomics)."""

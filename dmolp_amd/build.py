@@ -29,6 +29,7 @@ SOURCES = [
     "batch.hip",
     "server.hip",
     "halo.hip",
+    "hubs.hip",
     "noalloc.hip",
     "opbatch.hip",
     "probe.hip",

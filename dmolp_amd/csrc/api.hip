@@ -350,6 +350,36 @@ int dmolp_ctx_set_hub_prefix(dmolp_ctx *ctx, int64_t prefix) {
     return DMOLP_OK;
 }
 
+int dmolp_hubs_device(dmolp_ctx *ctx, const int32_t *deg, int64_t n, int64_t limit, int64_t host_max,
+                      int32_t *hubs_out, int64_t *n_hubs, int32_t *seed, int32_t *degree) {
+    if (!ctx || !n_hubs || !seed || !degree || n < 0 || n > 0x7fffffffll || (n > 0 && !deg) ||
+        host_max < 0)
+        return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    return guarded(ctx, [&] {
+        DevMem d(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+        if (n > 0)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(d.p, deg, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice,
+                                           ctx->stream));
+        HubSearch hs;
+        hs.n = n;
+        hs.deg = d.as<int32_t>();
+        hs.limit = limit < 0 ? -1 : limit;
+        std::vector<HubSearch *> one{&hs};
+        hub_search_device(ctx, one, host_max, std::min(16, host_threads()));
+        if (hubs_out) std::copy(hs.hubs.begin(), hs.hubs.end(), hubs_out);
+        *n_hubs = hs.n_hubs;
+        *seed = hs.fs.first;
+        *degree = hs.fs.second;
+    });
+}
+
+int dmolp_ctx_set_hub_device(dmolp_ctx *ctx, int64_t dev_min, int64_t host_max) {
+    if (!ctx || dev_min < 0 || host_max < 0) return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    ctx->hub_dev_min = dev_min;
+    ctx->hub_host_max = host_max;
+    return DMOLP_OK;
+}
+
 int dmolp_gen_host(const dmolp_gen_spec *spec, dmolp_csr *out) {
     return guarded(nullptr, [&] { gen_host(spec, out); });
 }

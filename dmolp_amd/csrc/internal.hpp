@@ -380,6 +380,11 @@ struct dmolp_ctx {
     int32_t *pin = nullptr;
     size_t pin_count = 0;
     int64_t hub_prefix = int64_t(1) << 16;  // hubs a phase-1 host search orders first (0: all)
+    // Phase-1 hub searches of graphs with at least hub_dev_min vertices run
+    // their partition steps on the device; ranges shorter than hub_host_max
+    // are finished on the host.
+    int64_t hub_dev_min = int64_t(1) << 18;
+    int64_t hub_host_max = int64_t(1) << 15;
     int32_t *pinned_i32(size_t count) {
         if (count > pin_count) {
             if (pin) (void)hipHostFree(pin);

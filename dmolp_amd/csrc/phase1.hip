@@ -790,324 +790,6 @@ void phase1_random(dmolp_ctx *ctx, dmolp_graph *g, int k) {
 
 // ---------------------------------------------------------------- host side
 
-namespace {
-
-struct DegPair {  // (vertex id, degree); trivial, so large arrays start uninitialised
-    int first, second;
-};
-struct ByDegree {
-    bool operator()(const DegPair &a, const DegPair &b) const { return a.second < b.second; }
-};
-
-// body(t) for t in [0, T) on T threads (the caller runs t = 0).
-template <class F>
-void run_threads(int T, F &&body) {
-    if (T <= 1) { body(0); return; }
-    std::vector<std::thread> th;
-    th.reserve((size_t)T - 1);
-    for (int t = 1; t < T; ++t) th.emplace_back([&body, t] { body(t); });
-    body(0);
-    for (auto &x : th) x.join();
-}
-
-// Ranges at least this long are partitioned by several threads.
-constexpr int64_t kParPartitionMin = int64_t(1) << 20;
-
-
-// The partition step of libstdc++'s __introsort_loop on a[first, last)
-// after the median has been moved to a[first] (__unguarded_partition of
-// [first + 1, last) around a[first]), sequential.  Returns the cut.
-int64_t partition_seq(DegPair *a, int64_t first, int64_t last) {
-    const ByDegree less;
-    int64_t lo = first + 1, hi = last;
-    const DegPair *pivot = a + first;
-    for (;;) {
-        while (less(a[lo], *pivot)) ++lo;
-        --hi;
-        while (less(*pivot, a[hi])) --hi;
-        if (!(lo < hi)) break;
-        std::swap(a[lo], a[hi]);
-        ++lo;
-    }
-    return lo;
-}
-
-// The same step on T threads, with the same result.  The sequential scans
-// stop, from the left, at elements not below the pivot (L_1 < L_2 < ...)
-// and, from the right, at elements not above it (R_1 > R_2 > ...), and
-// swap the j-th pair while L_j < R_j.  A swap changes only the two
-// positions it names, and the scans have already passed them, so L and R
-// can be read off the array before any swap; the pivot at `first` is never
-// swapped (the right scan can stop there only once the scans have crossed).
-// With m swaps (L_j < R_j holds exactly for j <= m: L rises, R falls), the
-// left scan then stops at min(L_{m+1}, R_m): the next original candidate,
-// or position R_m, which now holds the old L_m, not below the pivot; with
-// no swap it stops at L_1.  Every L_j with j <= m lies below every such
-// R_j, so the swaps touch disjoint positions and the threads' reads during
-// the swaps (their own slice's left candidates, their own run of right
-// candidates) never meet another thread's writes.
-int64_t partition_par(DegPair *a, int64_t first, int64_t last, int T) {
-    const int P = a[first].second;
-    const int64_t len = last - first;
-    // Fine slices, so that locating the j-th candidate scans a short range.
-    const int S = (int)std::min<int64_t>(4096, std::max<int64_t>(1, len / 4096));
-    const auto lo_of = [&](int64_t s) { return first + len * s / S; };
-    const auto is_l = [&](int64_t p) { return p != first && a[p].second >= P; };
-    const auto is_r = [&](int64_t p) { return a[p].second <= P; };
-    // Candidates per slice, the slices dealt to the threads in blocks.
-    std::vector<int64_t> cl((size_t)S), cr((size_t)S);
-    run_threads(T, [&](int t) {
-        for (int64_t s = (int64_t)S * t / T; s < (int64_t)S * (t + 1) / T; ++s) {
-            int64_t l = 0, r = 0;
-            for (int64_t p = lo_of(s); p < lo_of(s + 1); ++p) {
-                const int d = a[p].second;
-                l += d >= P ? 1 : 0;
-                r += d <= P ? 1 : 0;
-            }
-            cl[(size_t)s] = l - (s == 0 ? 1 : 0);  // the pivot's position is not a left candidate
-            cr[(size_t)s] = r;
-        }
-    });
-    // bl[s]: left candidates (0-based index) before slice s; br[s]: right
-    // candidates in slices s and above (R is numbered from the right end).
-    std::vector<int64_t> bl((size_t)S + 1, 0), br((size_t)S + 1, 0);
-    for (int s = 0; s < S; ++s) bl[(size_t)s + 1] = bl[(size_t)s] + cl[(size_t)s];
-    for (int s = S - 1; s >= 0; --s) br[(size_t)s] = br[(size_t)s + 1] + cr[(size_t)s];
-    const int64_t nl = bl[(size_t)S], nr = br[0];
-    if (nl == 0) return partition_seq(a, first, last);  // not reached: the median's sample holds one
-    // Slices holding L_j and R_j, then their positions by a scan of that
-    // slice, on the array as it is before any swap.
-    const auto slice_l = [&](int64_t j) {  // bl[s] <= j < bl[s + 1]
-        return (int)(std::upper_bound(bl.begin(), bl.end(), j) - bl.begin()) - 1;
-    };
-    const auto slice_r = [&](int64_t j) {  // br[s + 1] <= j < br[s]; br falls with s
-        return (int)(std::lower_bound(br.begin(), br.end(), j, std::greater<int64_t>()) - br.begin()) - 1;
-    };
-    const auto pos_l = [&](int64_t j) {
-        const int s = slice_l(j);
-        int64_t c = bl[(size_t)s];
-        for (int64_t p = lo_of(s);; ++p)
-            if (is_l(p) && c++ == j) return p;
-    };
-    const auto pos_r = [&](int64_t j) {
-        const int s = slice_r(j);
-        int64_t c = br[(size_t)s + 1];
-        for (int64_t p = lo_of(s + 1) - 1;; --p)
-            if (is_r(p) && c++ == j) return p;
-    };
-    // m: the first j with L_j >= R_j.
-    int64_t lo = 0, hi = std::min(nl, nr);
-    while (lo < hi) {
-        const int64_t mid = lo + (hi - lo) / 2;
-        const int sl = slice_l(mid), sr = slice_r(mid);
-        const bool lt = sl != sr ? sl < sr : pos_l(mid) < pos_r(mid);
-        if (lt) lo = mid + 1;
-        else hi = mid;
-    }
-    const int64_t m = lo;
-    if (m == 0) return pos_l(0);
-    const int64_t cut_l = m < nl ? pos_l(m) : last, cut_r = pos_r(m - 1);
-    // Swaps: thread t takes the pairs j in [m t / T, m (t + 1) / T), walking
-    // up from L at the first and down from R at the first.
-    std::vector<int64_t> ls((size_t)T, -1), rs((size_t)T, -1);
-    run_threads(T, [&](int t) {
-        const int64_t j0 = m * t / T;
-        if (j0 < m * (t + 1) / T) {
-            ls[(size_t)t] = pos_l(j0);
-            rs[(size_t)t] = pos_r(j0);
-        }
-    });
-    run_threads(T, [&](int t) {
-        int64_t j = m * t / T;
-        const int64_t j1 = m * (t + 1) / T;
-        if (j >= j1) return;
-        int64_t l = ls[(size_t)t], r = rs[(size_t)t];
-        for (;;) {
-            std::swap(a[l], a[r]);
-            if (++j == j1) break;
-            for (++l; !is_l(l); ++l) {}
-            for (--r; !is_r(r); --r) {}
-        }
-    });
-    return std::min(cut_l, cut_r);
-}
-
-// The positions [keep, n) of a[0, n) exactly as std::sort(a, a + n, ByDegree)
-// leaves them with libstdc++ (bits/stl_algo.h: __introsort_loop with
-// median-of-three pivots and unguarded partitions, depth limit 2 lg n, heap
-// sort past it, then one insertion sort).  The comparison is not a total
-// order (ties in degree), so the tie order at the top is that algorithm's,
-// and it is reproduced step for step on every range that reaches into
-// [keep, n); ranges wholly below `keep` are left as they are, since no step
-// moves an element across a partition cut and the final insertion sort
-// only reorders within the ranges the partitions leave (every element of a
-// range is no greater than every element of the next).  Cost: the
-// partitions along the path to the top, about 2n, instead of n lg n; the
-// long ones run on `threads` threads (partition_par).
-void introsort_top(DegPair *a, int64_t n, int64_t keep, int threads) {
-    const ByDegree less;
-    std::vector<std::pair<int64_t, int64_t>> leaves;  // ranges the final insertion sort orders
-    // __introsort_loop(first, last, depth), right parts first (recursion),
-    // then the loop over the left part; ranges below `keep` are skipped.
-    struct Frame { int64_t first, last; int depth; };
-    std::vector<Frame> stack{{0, n, 0}};
-    int lg = 0;
-    while ((int64_t(1) << (lg + 1)) <= n) ++lg;
-    stack[0].depth = 2 * lg;
-    while (!stack.empty()) {
-        Frame f = stack.back();
-        stack.pop_back();
-        int64_t first = f.first, last = f.last;
-        int depth = f.depth;
-        while (last - first > 16) {
-            if (last <= keep) { first = last; break; }  // no effect on [keep, n)
-            if (depth == 0) {  // __partial_sort(first, last, last): heap sort
-                std::make_heap(a + first, a + last, less);
-                std::sort_heap(a + first, a + last, less);
-                first = last;  // sorted: the insertion sort leaves it as it is
-                break;
-            }
-            --depth;
-            // __unguarded_partition_pivot
-            const int64_t mid = first + (last - first) / 2;
-            {
-                DegPair *r = a + first, *x = a + first + 1, *y = a + mid, *z = a + last - 1;
-                if (less(*x, *y)) {
-                    if (less(*y, *z)) std::iter_swap(r, y);
-                    else if (less(*x, *z)) std::iter_swap(r, z);
-                    else std::iter_swap(r, x);
-                } else if (less(*x, *z)) std::iter_swap(r, x);
-                else if (less(*y, *z)) std::iter_swap(r, z);
-                else std::iter_swap(r, y);
-            }
-            int64_t cut;
-            if (threads > 1 && last - first >= kParPartitionMin) {
-                cut = partition_par(a, first, last, threads);
-            } else {
-                cut = partition_seq(a, first, last);
-            }
-            // Recursion on [cut, last) runs before the loop goes on with
-            // [first, cut): push the left part, process the right part now.
-            stack.push_back({first, cut, depth});
-            first = cut;
-        }
-        if (last - first > 0 && last > keep && last - first <= 16) leaves.push_back({first, last});
-    }
-    // __final_insertion_sort restricted to the ranges that reach [keep, n).
-    for (const auto &lf : leaves) {
-        for (int64_t i = lf.first + 1; i < lf.second; ++i) {
-            const DegPair val = a[i];
-            int64_t j = i;
-            while (j > lf.first && less(val, a[j - 1])) { a[j] = a[j - 1]; --j; }
-            a[j] = val;
-        }
-    }
-}
-
-}  // namespace
-
-// init.cpp:139-177 at one rank: (id, degree) pairs in the degree map's
-// iteration order (descending id), libstdc++ std::sort by degree, Lorenz
-// tangent cut.  Double arithmetic in source order.  The tangent needs only
-// the last two cumulative shares, which are exact integer sums divided by
-// the total whatever the order (every partial sum is an integer below
-// 2^53), so the number of hubs is known before sorting and only the top of
-// the sorted order is produced (introsort_top).
-std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads, int64_t limit,
-                                    int64_t *m_out) {
-    const int64_t n = (int64_t)deg.size();
-    std::vector<int32_t> hubs;
-    if (m_out) *m_out = 0;
-    const int N = (int)n;
-    if (N < 2) return hubs;
-    const int T = n >= kParPartitionMin ? std::max(1, threads) : 1;
-    // Degree total (every partial sum is an integer below 2^53, so the
-    // reference's double accumulation is this exact sum) and maximum.
-    std::vector<int64_t> tsum((size_t)T, 0);
-    std::vector<int> tmax((size_t)T, 0);
-    run_threads(T, [&](int t) {
-        int64_t s = 0;
-        int mx = 0;
-        for (int64_t v = n * t / T; v < n * (t + 1) / T; ++v) {
-            s += deg[(size_t)v];
-            mx = std::max(mx, deg[(size_t)v]);
-        }
-        tsum[(size_t)t] = s;
-        tmax[(size_t)t] = mx;
-    });
-    int64_t isum = 0;
-    int dmax = 0;
-    for (int t = 0; t < T; ++t) {
-        isum += tsum[(size_t)t];
-        dmax = std::max(dmax, tmax[(size_t)t]);
-    }
-    const double total = (double)isum;
-    const double inv_N = 1.0 / N;
-    const double x1 = N * inv_N;
-    const double y1 = total / total;            // cum[N-1]
-    const double x0 = (N - 1) * inv_N;
-    const double y0 = (total - dmax) / total;   // cum[N-2]: all but the largest degree
-    const double slope = (y1 - y0) / (x1 - x0);
-    const double xi = x1 - y1 / slope;
-    int64_t M = 0;
-    for (int i = N - 1; i >= 0; --i) {
-        const double x = (i + 1) * inv_N;
-        if (x > xi) ++M;
-        else break;
-    }
-    if (m_out) *m_out = M;
-    if (M == 0) return hubs;
-    if (limit >= 0 && limit < M) M = limit;  // only the first `limit` hubs of the order
-    std::unique_ptr<DegPair[]> sd(new DegPair[(size_t)n]);  // (id, degree), descending id
-    run_threads(T, [&](int t) {
-        for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i) {
-            const int64_t v = n - 1 - i;
-            sd[(size_t)i] = DegPair{(int)v, deg[(size_t)v]};
-        }
-    });
-    introsort_top(sd.get(), n, n - M, T);
-    hubs.reserve((size_t)M);
-    for (int64_t i = n - 1; i >= n - M; --i) hubs.push_back(sd[(size_t)i].first);
-    return hubs;
-}
-
-// init.cpp:179-187 over the degree map's order (descending id): the scan
-// keeps (node, best) and takes v when deg v > best, or when deg v == best
-// and v < best (the tie test compares an id with the running best degree).
-// Once best reaches the largest degree D (at the largest id v0 of degree D)
-// only the tie rule can move node, to ids of degree D below both v0 and D,
-// and the last such move is the smallest one: the result is the smallest id
-// of degree D when it lies below D and is not v0 itself, else v0.  Three
-// reductions (D, then the smallest and largest ids of degree D) on
-// `threads` threads instead of a sequential scan.
-std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg, int threads) {
-    const int64_t n = (int64_t)deg.size();
-    if (n == 0) return {-1, -1};
-    const int T = n >= kParPartitionMin ? std::max(1, threads) : 1;
-    std::vector<int32_t> tmax((size_t)T, -1);
-    run_threads(T, [&](int t) {
-        int32_t m = -1;
-        for (int64_t v = n * t / T; v < n * (t + 1) / T; ++v) m = std::max(m, deg[(size_t)v]);
-        tmax[(size_t)t] = m;
-    });
-    const int32_t D = *std::max_element(tmax.begin(), tmax.end());
-    std::vector<int64_t> tlo((size_t)T, n), thi((size_t)T, -1);
-    run_threads(T, [&](int t) {
-        int64_t lo = n, hi = -1;
-        for (int64_t v = n * t / T; v < n * (t + 1) / T; ++v)
-            if (deg[(size_t)v] == D) {
-                if (lo == n) lo = v;
-                hi = v;
-            }
-        tlo[(size_t)t] = lo;
-        thi[(size_t)t] = hi;
-    });
-    const int64_t vmin = *std::min_element(tlo.begin(), tlo.end());
-    const int64_t vmax = *std::max_element(thi.begin(), thi.end());
-    const int64_t node = (vmin < vmax && vmin < (int64_t)D) ? vmin : vmax;
-    return {(int32_t)node, D};
-}
-
 Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
     graph_prepare_bins(ctx, g);
     if (!g->p1) g->p1.reset(new Phase1Work());
@@ -1181,6 +863,7 @@ P1Job p1_job(dmolp_ctx *ctx, dmolp_graph *g, Phase1Work &w, int32_t *remaining) 
 }
 
 void p1_hub_search(P1Stage &h, int threads) {
+    if (h.dev_hubs) return;  // p1_hub_search_device
     const auto t0 = std::chrono::steady_clock::now();
     h.fs = first_seed_host(h.deg, threads);
     // Only the first hub_prefix hubs are ordered: a seed search walks the
@@ -1189,8 +872,49 @@ void p1_hub_search(P1Stage &h, int threads) {
     // above that; a search that runs out of listed hubs is repeated with
     // the whole list (p1_seeds).
     h.hubs = find_hubs_host(h.deg, threads, h.hub_prefix > 0 ? h.hub_prefix : -1, &h.n_hubs);
+    h.hub_deg.resize(h.hubs.size());
+    for (size_t i = 0; i < h.hubs.size(); ++i) h.hub_deg[i] = h.deg[(size_t)h.hubs[i]];
     const auto t1 = std::chrono::steady_clock::now();
     h.host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+void p1_hub_search_device(dmolp_ctx *ctx, std::vector<P1Stage *> &H) {
+    std::vector<HubSearch> hs;
+    std::vector<P1Stage *> on;
+    for (P1Stage *h : H)
+        if (h->dev_hubs) on.push_back(h);
+    if (on.empty()) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    hs.resize(on.size());
+    std::vector<HubSearch *> ptr;
+    for (size_t i = 0; i < on.size(); ++i) {
+        hs[i].n = on[i]->g->n;
+        hs[i].deg = on[i]->w->deg.as<int32_t>();
+        hs[i].limit = on[i]->hub_prefix > 0 ? on[i]->hub_prefix : -1;  // as p1_hub_search
+        ptr.push_back(&hs[i]);
+    }
+    hub_search_device(ctx, ptr, ctx->hub_host_max, std::min(16, host_threads()));
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (size_t i = 0; i < on.size(); ++i) {
+        P1Stage &h = *on[i];
+        h.fs = hs[i].fs;
+        h.hubs = std::move(hs[i].hubs);
+        h.hub_deg = std::move(hs[i].hub_deg);
+        h.n_hubs = hs[i].n_hubs;
+        h.host_ms = ms;
+    }
+}
+
+// The degrees of device-search jobs on the host, for the seed fallback.
+void p1_host_degrees(dmolp_ctx *ctx, const std::vector<P1Stage *> &H) {
+    OpBatch ob;
+    for (P1Stage *h : H)
+        if ((int64_t)h->deg.size() != h->g->n) {
+            h->deg.resize((size_t)h->g->n);
+            ob.download(h->deg.data(), h->w->deg.p, sizeof(int32_t) * (size_t)h->g->n);
+        }
+    ob.run(ctx);
 }
 
 // Candidates of the seed fallback (p1_seeds, only for a search that found
@@ -1233,9 +957,9 @@ void p1_fallback_candidates(P1Stage &h, int threads) {
 
 namespace {
 
-// Hubs and first seed of every job on the host; jobs in parallel threads.
-// `during` (may be empty) runs on the calling thread while the searches run
-// on worker threads.  Jobs are taken largest first from a shared cursor, so
+// Hubs and first seed of every job searched on the host; jobs in parallel
+// threads.  `during` (may be empty) runs on the calling thread while the
+// searches run on worker threads.  Jobs are taken largest first from a shared cursor, so
 // the longest search starts at once and the threads finish close together.
 void host_hub_search(std::vector<P1Stage> &H, const std::function<void()> &during) {
     const size_t G = H.size();
@@ -1361,26 +1085,34 @@ void p1_degrees(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     hipLaunchKernelGGL(k_degrees, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, sd.jobs.as<P1Job>(),
                        ls.bs(P_GRID), sd.G);
     DMOLP_HIP_CHECK(hipGetLastError());
+    // Jobs searched on the device keep their degrees there.
+    std::vector<P1Stage *> Hh;
+    for (P1Stage *h : H) {
+        h->dev_hubs = h->g->n >= ctx->hub_dev_min;
+        h->deg.clear();
+        if (!h->dev_hubs) Hh.push_back(h);
+    }
+    if (Hh.empty()) return;
     size_t total = 0;
-    std::vector<size_t> at(H.size());
-    for (size_t j = 0; j < H.size(); ++j) {
+    std::vector<size_t> at(Hh.size());
+    for (size_t j = 0; j < Hh.size(); ++j) {
         at[j] = total;
-        total += (size_t)H[j]->g->n;
+        total += (size_t)Hh[j]->g->n;
     }
     int32_t *pin = ctx->pinned_i32(total);
-    for (size_t j = 0; j < H.size(); ++j)
-        DMOLP_HIP_CHECK(hipMemcpyAsync(pin + at[j], H[j]->w->deg.p, sizeof(int32_t) * (size_t)H[j]->g->n,
+    for (size_t j = 0; j < Hh.size(); ++j)
+        DMOLP_HIP_CHECK(hipMemcpyAsync(pin + at[j], Hh[j]->w->deg.p, sizeof(int32_t) * (size_t)Hh[j]->g->n,
                                        hipMemcpyDeviceToHost, s));
     DMOLP_HIP_CHECK(hipStreamSynchronize(s));
-    for (P1Stage *h : H) h->deg.resize((size_t)h->g->n);
+    for (P1Stage *h : Hh) h->deg.resize((size_t)h->g->n);
     // Copy out in slices of about equal size across the jobs.
     const int T = total >= (size_t(1) << 22) ? std::min(16, host_threads()) : 1;
     run_threads(T, [&](int t) {
         const size_t a = total * (size_t)t / (size_t)T, b = total * (size_t)(t + 1) / (size_t)T;
-        for (size_t j = 0; j < H.size(); ++j) {
-            const size_t lo = std::max(a, at[j]), hi = std::min(b, at[j] + H[j]->deg.size());
+        for (size_t j = 0; j < Hh.size(); ++j) {
+            const size_t lo = std::max(a, at[j]), hi = std::min(b, at[j] + Hh[j]->deg.size());
             if (lo < hi)
-                std::memcpy(H[j]->deg.data() + (lo - at[j]), pin + lo, sizeof(int32_t) * (hi - lo));
+                std::memcpy(Hh[j]->deg.data() + (lo - at[j]), pin + lo, sizeof(int32_t) * (hi - lo));
         }
     });
 }
@@ -1401,8 +1133,8 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
         ob.fill(w.seeds.p, 0xff, w.seeds.bytes);
         ob.upload(w.seeds.p, &h.fs.first, sizeof(int32_t));
         for (int64_t i = 1; i < nh; ++i)  // k_next_seed relies on the order
-            DMOLP_REQUIRE(h.deg[(size_t)h.hubs[(size_t)i]] <= h.deg[(size_t)h.hubs[(size_t)i - 1]],
-                          DMOLP_E_INTERNAL, "hub degrees out of order");
+            DMOLP_REQUIRE(h.hub_deg[(size_t)i] <= h.hub_deg[(size_t)i - 1], DMOLP_E_INTERNAL,
+                          "hub degrees out of order");
         h.used.assign((size_t)(nh > 0 ? nh : 1), 0);
         for (int64_t i = 0; i < nh; ++i)
             if (h.hubs[(size_t)i] == h.fs.first) { h.used[(size_t)i] = 1; break; }
@@ -1455,27 +1187,48 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
         }
     }
     if (!redo.empty()) {
-        for (P1Stage *h : redo)
-            h->hubs = find_hubs_host(h->deg, std::min(16, host_threads()), -1, &h->n_hubs);
+        for (P1Stage *h : redo) h->hub_prefix = 0;
+        p1_hub_search_device(ctx, redo);
+        for (P1Stage *h : redo) p1_hub_search(*h, std::min(16, host_threads()));
         p1_seeds(ctx, redo, sd);
         p1_set_build(ctx, H, sd);  // the caller's set again
     }
+    // Seed fallback candidates of the jobs that found fewer than k seeds, the
+    // jobs on host threads (a lone job splits its own pass over them).
+    std::vector<P1Stage *> short_of;
     for (int j = 0; j < G; ++j) {
         if (redone[(size_t)j]) continue;
         P1Stage &h = *H[(size_t)j];
         const int k = h.k;
-        const int64_t n = h.g->n;
         const int64_t nh = (int64_t)h.hubs.size();
         int32_t found = (nh > 0) ? (hc[(size_t)j].seed_stop ? hc[(size_t)j].seeds_found : k) : 1;
         if (k == 1) found = 1;
         h.found = found;
         h.fallback = 0;
+        if (found < k && h.top.empty()) short_of.push_back(&h);
+    }
+    if (!short_of.empty()) {
+        p1_host_degrees(ctx, short_of);
+        const int T = std::min(16, host_threads());
+        if (short_of.size() == 1) {
+            p1_fallback_candidates(*short_of[0], T);
+        } else {
+            std::atomic<size_t> next{0};
+            run_threads((int)std::min<size_t>((size_t)T, short_of.size()), [&](int) {
+                for (size_t i = next.fetch_add(1); i < short_of.size(); i = next.fetch_add(1))
+                    p1_fallback_candidates(*short_of[i], 1);
+            });
+        }
+    }
+    for (int j = 0; j < G; ++j) {
+        if (redone[(size_t)j]) continue;
+        P1Stage &h = *H[(size_t)j];
+        const int k = h.k;
+        const int32_t found = h.found;
         if (found < k) {
-            if (h.top.empty()) p1_fallback_candidates(h, std::min(16, host_threads()));
             // Documented extension for seed.cpp:214-217 followed by the
             // out-of-range seeds[p] of partition.cpp:347: the missing seeds are
             // the remaining vertices by (degree descending, id ascending).
-            (void)n;
             std::vector<int32_t> chosen(h.seeds.begin(), h.seeds.begin() + found);
             std::sort(chosen.begin(), chosen.end());
             int32_t at = found;
@@ -1678,7 +1431,12 @@ void phase1_run_set(dmolp_ctx *ctx, std::vector<P1Item> &items, const std::funct
     p1_set_build(ctx, HP, sd);
     DMOLP_HIP_CHECK(hipEventRecord(ctx->ev0, s));
     p1_degrees(ctx, HP, sd);
-    host_hub_search(H, during_hubs);
+    // The device searches run on this thread while host threads search the
+    // short graphs; then the caller's work.
+    host_hub_search(H, [&] {
+        p1_hub_search_device(ctx, HP);
+        if (during_hubs) during_hubs();
+    });
     p1_seeds(ctx, HP, sd);
     p1_expand_begin(ctx, HP, sd);
     int64_t enq = 0;

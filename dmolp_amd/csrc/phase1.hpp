@@ -4,6 +4,7 @@
 #include <functional>
 #include <vector>
 
+#include "hubs.hpp"
 #include "internal.hpp"
 #include "select.hpp"
 
@@ -66,9 +67,12 @@ struct P1Stage {
     dmolp_graph *g = nullptr;
     Phase1Work *w = nullptr;
     int k = 0, theta = 0;
-    std::vector<int32_t> deg, hubs, seeds;  // hubs: the first of the n_hubs hubs (p1_hub_search)
+    // deg: the degrees on the host (host hub searches, the seed fallback);
+    // hubs: the first of the n_hubs hubs, hub_deg their degrees.
+    std::vector<int32_t> deg, hubs, hub_deg, seeds;
     int64_t n_hubs = 0;
     int64_t hub_prefix = 0;  // hubs ordered by the first search (0: all)
+    bool dev_hubs = false;   // hub search on the device (p1_hub_search_device)
     std::vector<int32_t> top;  // first 2k vertices by (degree desc, id asc)
     std::vector<uint8_t> used;
     std::pair<int32_t, int32_t> fs{-1, -1};
@@ -83,7 +87,9 @@ struct P1Stage {
 void p1_stage_init(dmolp_ctx *ctx, P1Item &it, P1Stage &h);
 void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);
 void p1_degrees(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);  // degrees to the host
-void p1_hub_search(P1Stage &h, int threads = 1);                                           // host: hubs, first seed
+void p1_hub_search(P1Stage &h, int threads = 1);       // host: hubs, first seed (dev_hubs: none)
+void p1_hub_search_device(dmolp_ctx *ctx, std::vector<P1Stage *> &H);  // the jobs with dev_hubs
+void p1_host_degrees(dmolp_ctx *ctx, const std::vector<P1Stage *> &H);  // deg of device-search jobs
 void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);    // hub-ball seeds + fallback
 void p1_expand_begin(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);
 void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds);          // rounds: even
@@ -96,13 +102,6 @@ void p1_finish(dmolp_ctx *ctx, P1Stage &h, dmolp_phase1_metrics &m, double devic
                const P1Ctl *ctl = nullptr);
 
 Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k);
-// threads: host threads for the long partition steps and passes (1: all
-// on the calling thread); the result does not depend on it.
-// limit >= 0: only the first min(limit, M) hubs of the order; m_out gets M,
-// the number of hubs.
-std::vector<int32_t> find_hubs_host(const std::vector<int32_t> &deg, int threads = 1,
-                                    int64_t limit = -1, int64_t *m_out = nullptr);
-std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg, int threads = 1);
 // Seeds + expansion; labels become resident in g.  Metrics fields about
 // seeds, rounds and timing are filled (cut and counts come from the stats
 // sweep).

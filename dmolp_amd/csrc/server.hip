@@ -263,9 +263,22 @@ void server_step(dmolp_server *s, int32_t *active_out, int32_t *queued_out) {
     if (!fresh.empty()) {
         p1_set_build(ctx, fresh, s->seeds_set);
         p1_degrees(ctx, fresh, s->seeds_set);
+        // Long graphs: partition steps on the device, now (a few ms); short
+        // ones: host sorts on the workers while the device goes on.
+        int dev_status = 1;
+        try {
+            p1_hub_search_device(ctx, fresh);
+        } catch (const Error &e) {
+            dev_status = e.status < 0 ? e.status : DMOLP_E_INTERNAL;
+        } catch (const std::bad_alloc &) {
+            dev_status = DMOLP_E_NOMEM;
+        }
         {
             std::lock_guard<std::mutex> lk(s->hmu);
-            for (SrvJob *j : fresh_jobs) s->hubq.push_back(j);
+            for (SrvJob *j : fresh_jobs) {
+                if (j->p1.dev_hubs) j->hubs_ready.store(dev_status, std::memory_order_release);
+                else s->hubq.push_back(j);
+            }
         }
         s->hcv.notify_all();
     }

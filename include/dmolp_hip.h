@@ -222,6 +222,19 @@ int dmolp_first_seed_host(const int32_t *deg, int64_t n, int32_t *seed, int32_t 
  * on this; it only bounds the host sort (on near-regular graphs every
  * vertex is a hub).  Test hook; no counterpart in the reference. */
 int dmolp_ctx_set_hub_prefix(dmolp_ctx *ctx, int64_t prefix);
+/* find_hub_nodes and find_first_seed (init.cpp:139-187) as dmolp_phase1
+ * runs them for graphs of at least dev_min vertices: the introsort's
+ * partition steps on the device for ranges of at least host_max entries,
+ * shorter ranges finished on the host.  The same results as
+ * dmolp_hubs_host and dmolp_first_seed_host; limit >= 0 lists only the
+ * first min(limit, n_hubs) hubs (hubs_out capacity n, nullable). */
+int dmolp_hubs_device(dmolp_ctx *ctx, const int32_t *deg, int64_t n, int64_t limit, int64_t host_max,
+                      int32_t *hubs_out, int64_t *n_hubs, int32_t *seed, int32_t *degree);
+/* Graphs of at least dev_min vertices get dmolp_phase1's hub search on the
+ * device, finishing ranges shorter than host_max on the host (defaults
+ * 2^18 and 2^15).  The results do not depend on either.  Test hook; no
+ * counterpart in the reference. */
+int dmolp_ctx_set_hub_device(dmolp_ctx *ctx, int64_t dev_min, int64_t host_max);
 
 /* Phase1Metrics (graph_types.h:276-309) with int64 counters. */
 typedef struct {
