@@ -96,6 +96,7 @@ SIGNATURES = {
     "dmolp_first_seed_host": (C.c_int, [P(i32), i64, P(i32), P(i32)]),
     "dmolp_hubs_device": (C.c_int, [VP, P(i32), i64, i64, i64, P(i32), P(i64), P(i32), P(i32)]),
     "dmolp_ctx_set_hub_device": (C.c_int, [VP, i64, i64]),
+    "dmolp_top_degree_device": (C.c_int, [VP, P(i32), i64, i64, i64, P(i32)]),
     "dmolp_probe_atomics": (C.c_int, [VP, i64, i32, i32, P(f64), P(i64)]),
     "dmolp_stats_noalloc": (C.c_int, [VP, VP, i32, P(i32), P(i64), P(i64), P(i64)]),
     "dmolp_shares_create": (C.c_int, [VP, VP, i32, P(VP)]),

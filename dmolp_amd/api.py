@@ -198,6 +198,15 @@ class Context:
         results do not depend on either (dmolp_ctx_set_hub_device)."""
         _check(_abi.lib().dmolp_ctx_set_hub_device(self._h, int(dev_min), int(host_max)), self._h)
 
+    def top_degree_device(self, deg: np.ndarray, m: int, host_max: int = 1 << 15) -> np.ndarray:
+        """The first min(m, n) vertices by (degree descending, id ascending),
+        selected on the device (dmolp_top_degree_device)."""
+        deg = np.ascontiguousarray(deg, np.int32)
+        out = np.zeros(max(min(m, len(deg)), 1), np.int32)
+        _check(_abi.lib().dmolp_top_degree_device(self._h, _p(deg, C.c_int32), len(deg), int(m),
+                                                  int(host_max), _p(out, C.c_int32)), self._h)
+        return out[:min(m, len(deg))].copy()
+
     def hubs_device(self, deg: np.ndarray, limit: int = -1, host_max: int = 1 << 15):
         """(hubs, n_hubs, first seed, its degree) of a degree sequence by the
         device hub search (dmolp_hubs_device)."""

@@ -373,6 +373,25 @@ int dmolp_hubs_device(dmolp_ctx *ctx, const int32_t *deg, int64_t n, int64_t lim
     });
 }
 
+int dmolp_top_degree_device(dmolp_ctx *ctx, const int32_t *deg, int64_t n, int64_t m, int64_t host_max,
+                            int32_t *out) {
+    if (!ctx || n < 0 || n > 0x7fffffffll || (n > 0 && (!deg || !out)) || m < 0 || host_max < 0)
+        return fail(ctx, DMOLP_E_ARG, "bad arguments");
+    return guarded(ctx, [&] {
+        DevMem d(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+        if (n > 0)
+            DMOLP_HIP_CHECK(hipMemcpyAsync(d.p, deg, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice,
+                                           ctx->stream));
+        TopSearch ts;
+        ts.n = n;
+        ts.deg = d.as<int32_t>();
+        ts.m = m;
+        std::vector<TopSearch *> one{&ts};
+        top_by_degree_device(ctx, one, host_max, std::min(16, host_threads()));
+        std::copy(ts.top.begin(), ts.top.end(), out);
+    });
+}
+
 int dmolp_ctx_set_hub_device(dmolp_ctx *ctx, int64_t dev_min, int64_t host_max) {
     if (!ctx || dev_min < 0 || host_max < 0) return fail(ctx, DMOLP_E_ARG, "bad arguments");
     ctx->hub_dev_min = dev_min;

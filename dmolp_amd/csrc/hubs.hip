@@ -356,6 +356,11 @@ std::pair<int32_t, int32_t> first_seed_host(const std::vector<int32_t> &deg, int
 // rpos[m - 1]) (lpos[0] without swaps).  Every range of a step, of every
 // job, is one entry of the step's frame table and its chunks are blocks of
 // the same launches.
+//
+// The same steps under a total order (degree, then id descending) select
+// the first vertices by (degree descending, id ascending): the seed
+// fallback's candidates.  There only the set at the top matters, so the
+// host sorts each short range outright.
 
 namespace {
 
@@ -371,7 +376,8 @@ struct HsFrame {
     int64_t nchunks;
 };
 struct HsOut {
-    int64_t P, nl, nr;
+    DegPair P;  // the pivot
+    int64_t nl, nr;
     unsigned long long m;
     int64_t cut;  // -1: no left candidate (the median-of-three sample rules it out)
 };
@@ -383,6 +389,17 @@ struct HsInit {
     int64_t n;
     int64_t chunk0, nchunks;
     unsigned long long *stat;  // [0] max (deg, ~id), [1] max (deg, id), [2] degree total
+};
+
+// ByDegree, or the total order (degree, then id descending).
+template <bool Total>
+__device__ __forceinline__ bool hs_less(const DegPair &x, const DegPair &y) {
+    return x.second < y.second || (Total && x.second == y.second && x.first > y.first);
+}
+struct ByDegreeThenId {  // the total order on the host
+    bool operator()(const DegPair &x, const DegPair &y) const {
+        return x.second < y.second || (x.second == y.second && x.first > y.first);
+    }
 };
 
 template <class T>
@@ -440,39 +457,40 @@ __global__ __launch_bounds__(kHsThreads) void k_hs_init(const HsInit *J, int nj)
 }
 
 // __move_median_to_first(first, first + 1, mid, last - 1): one thread a frame.
+template <bool Total>
 __global__ void k_hs_pivot(const HsFrame *F, HsOut *O) {
     if (threadIdx.x != 0) return;
     const HsFrame f = F[blockIdx.x];
     DegPair *a = f.a;
-    const ByDegree less;
     const int64_t r = f.first, x = f.first + 1, y = f.first + (f.last - f.first) / 2, z = f.last - 1;
     int64_t pick;
-    if (less(a[x], a[y])) {
-        if (less(a[y], a[z])) pick = y;
-        else if (less(a[x], a[z])) pick = z;
+    if (hs_less<Total>(a[x], a[y])) {
+        if (hs_less<Total>(a[y], a[z])) pick = y;
+        else if (hs_less<Total>(a[x], a[z])) pick = z;
         else pick = x;
-    } else if (less(a[x], a[z])) pick = x;
-    else if (less(a[y], a[z])) pick = z;
+    } else if (hs_less<Total>(a[x], a[z])) pick = x;
+    else if (hs_less<Total>(a[y], a[z])) pick = z;
     else pick = y;
     const DegPair t = a[r];
     a[r] = a[pick];
     a[pick] = t;
-    O[blockIdx.x].P = a[r].second;
+    O[blockIdx.x].P = a[r];
 }
 
+template <bool Total>
 __global__ __launch_bounds__(kHsThreads) void k_hs_count(const HsFrame *F, int nf, const HsOut *O,
                                                         int32_t *cl, int32_t *cr) {
     __shared__ int64_t s[kHsThreads / kWave];
     const int fi = frame_of(F, nf, blockIdx.x);
     const HsFrame f = F[fi];
-    const int P = (int)O[fi].P;
+    const DegPair P = O[fi].P;
     const int64_t lo = f.first + ((int64_t)blockIdx.x - f.chunk0) * kHsChunk;
     const int64_t hi = lo + kHsChunk < f.last ? lo + kHsChunk : f.last;
     int64_t l = 0, r = 0;
     for (int64_t p = lo + threadIdx.x; p < hi; p += kHsThreads) {
-        const int d = f.a[p].second;
-        l += (p != f.first && d >= P) ? 1 : 0;
-        r += d <= P ? 1 : 0;
+        const DegPair e = f.a[p];
+        l += (p != f.first && !hs_less<Total>(e, P)) ? 1 : 0;
+        r += !hs_less<Total>(P, e) ? 1 : 0;
     }
     l = block_sum(l, s);
     r = block_sum(r, s);
@@ -523,13 +541,14 @@ __global__ __launch_bounds__(kHsThreads) void k_hs_scan(const HsFrame *F, const 
     }
 }
 
+template <bool Total>
 __global__ __launch_bounds__(kHsThreads) void k_hs_scatter(const HsFrame *F, int nf, const HsOut *O,
                                                           const int32_t *cr, const int64_t *bl,
                                                           const int64_t *br) {
     __shared__ int32_t sl[kHsThreads / kWave], sr[kHsThreads / kWave];
     const int fi = frame_of(F, nf, blockIdx.x);
     const HsFrame f = F[fi];
-    const int P = (int)O[fi].P;
+    const DegPair P = O[fi].P;
     const int64_t lo = f.first + ((int64_t)blockIdx.x - f.chunk0) * kHsChunk;
     const int64_t hi = lo + kHsChunk < f.last ? lo + kHsChunk : f.last;
     const int64_t basel = bl[blockIdx.x], baser = br[blockIdx.x];
@@ -539,9 +558,9 @@ __global__ __launch_bounds__(kHsThreads) void k_hs_scatter(const HsFrame *F, int
     for (int64_t s0 = lo; s0 < hi; s0 += kHsThreads) {
         const int64_t p = s0 + threadIdx.x;
         const bool valid = p < hi;
-        const int d = f.a[valid ? p : lo].second;
-        const bool isl = valid && p != f.first && d >= P;
-        const bool isr = valid && d <= P;
+        const DegPair e = f.a[valid ? p : lo];
+        const bool isl = valid && p != f.first && !hs_less<Total>(e, P);
+        const bool isr = valid && !hs_less<Total>(P, e);
         const uint64_t ml = __ballot(isl), mr = __ballot(isr);
         if (lane_id() == 0) {
             sl[w] = __popcll(ml);
@@ -624,36 +643,31 @@ struct HsRange {
     int depth;
 };
 
+// One job's sequence and ranges.  keep: positions [keep, n) are the top the
+// caller wants.
 struct HsJob {
-    HubSearch *s;
+    int64_t n = 0;
+    const int32_t *deg = nullptr;
     DevMem a, lpos, rpos;
-    int64_t keep = 0;  // positions [keep, n) are the hubs kept
+    int64_t keep = 0;
     std::vector<HsRange> dev, host;
     std::vector<std::vector<DegPair>> host_data;
+    std::vector<DegPair> top;  // positions n - 1 down to keep
 };
 
-}  // namespace
-
-void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int64_t host_max,
-                       int threads) {
+// The sequences of every job (jobs with n > 0) and their reductions:
+// stats[3 j ..] = (max (deg, ~id), max (deg, id), degree total).
+void hs_setup(dmolp_ctx *ctx, std::vector<HsJob> &J, std::vector<unsigned long long> &stats) {
     hipStream_t st = ctx->stream;
-    std::vector<HsJob> J(jobs.size());
-    // Set-up: the sequences and their reductions, all jobs in one launch.
-    std::vector<HsInit> init;
-    DevMem stat(sizeof(unsigned long long) * 3 * (jobs.size() ? jobs.size() : 1));
+    DevMem stat(sizeof(unsigned long long) * 3 * (J.size() ? J.size() : 1));
     DMOLP_HIP_CHECK(hipMemsetAsync(stat.p, 0, stat.bytes, st));
+    std::vector<HsInit> init;
     int64_t chunks = 0;
-    for (size_t i = 0; i < jobs.size(); ++i) {
+    for (size_t i = 0; i < J.size(); ++i) {
         HsJob &j = J[i];
-        j.s = jobs[i];
-        j.s->hubs.clear();
-        j.s->hub_deg.clear();
-        j.s->n_hubs = 0;
-        j.s->fs = {-1, -1};
-        const int64_t n = j.s->n;
-        if (n == 0) continue;
-        j.a.alloc(sizeof(DegPair) * (size_t)n);
-        HsInit h{j.s->deg, j.a.as<DegPair>(), n, chunks, (n + kHsChunk - 1) / kHsChunk,
+        if (j.n == 0) continue;
+        j.a.alloc(sizeof(DegPair) * (size_t)j.n);
+        HsInit h{j.deg, j.a.as<DegPair>(), j.n, chunks, (j.n + kHsChunk - 1) / kHsChunk,
                  stat.as<unsigned long long>() + 3 * i};
         chunks += h.nchunks;
         init.push_back(h);
@@ -666,37 +680,37 @@ void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int
                            (int)init.size());
         DMOLP_HIP_CHECK(hipGetLastError());
     }
-    std::vector<unsigned long long> hs(3 * jobs.size());
-    if (!jobs.empty())
-        DMOLP_HIP_CHECK(hipMemcpyAsync(hs.data(), stat.p, sizeof(unsigned long long) * hs.size(),
+    stats.assign(3 * J.size(), 0);
+    if (!J.empty())
+        DMOLP_HIP_CHECK(hipMemcpyAsync(stats.data(), stat.p, sizeof(unsigned long long) * stats.size(),
                                        hipMemcpyDeviceToHost, st));
     DMOLP_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// Partition steps from the range [0, n) of every job with keep < n, until
+// every range that reaches [keep, n) is shorter than host_max (or a leaf,
+// or at its depth limit); then those ranges are copied out and finished on
+// the host (Total: sorted outright), and j.top holds positions n - 1 down
+// to keep.
+template <bool Total>
+void hs_run(dmolp_ctx *ctx, std::vector<HsJob> &J, int64_t host_max, int threads) {
+    hipStream_t st = ctx->stream;
     const auto place = [&](HsJob &j, const HsRange &r) {
         if (r.last <= j.keep || r.last <= r.first) return;  // no effect on [keep, n)
         if (r.last - r.first > 16 && r.depth > 0 && r.last - r.first >= host_max) j.dev.push_back(r);
         else j.host.push_back(r);
     };
-    for (size_t i = 0; i < J.size(); ++i) {
-        HsJob &j = J[i];
-        const int64_t n = j.s->n;
-        if (n == 0) continue;
-        const unsigned long long *q = &hs[3 * i];
-        const int32_t D = (int32_t)(q[0] >> 32);
-        const int64_t vmin = (int64_t)(0xffffffffull - (q[0] & 0xffffffffull));
-        const int64_t vmax = (int64_t)(q[1] & 0xffffffffull);
-        j.s->fs = first_seed_from(D, vmin, vmax);
-        int64_t M = hub_count(n, (int64_t)q[2], D);
-        j.s->n_hubs = M;
-        if (j.s->limit >= 0 && j.s->limit < M) M = j.s->limit;
-        if (M == 0) continue;
-        j.keep = n - M;
-        place(j, HsRange{0, n, depth_limit(n)});
+    for (HsJob &j : J) {
+        j.dev.clear();
+        j.host.clear();
+        j.top.clear();
+        if (j.n == 0 || j.keep >= j.n) continue;
+        place(j, HsRange{0, j.n, depth_limit(j.n)});
         if (!j.dev.empty()) {
-            j.lpos.alloc(sizeof(int32_t) * (size_t)n);
-            j.rpos.alloc(sizeof(int32_t) * (size_t)n);
+            j.lpos.alloc(sizeof(int32_t) * (size_t)j.n);
+            j.rpos.alloc(sizeof(int32_t) * (size_t)j.n);
         }
     }
-    // Partition steps: every device range of every job in one set of launches.
     DevMem ftab, otab, cl, cr, bl, br;
     std::vector<HsFrame> fr;
     std::vector<std::pair<size_t, HsRange>> owner;
@@ -704,7 +718,7 @@ void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int
     for (;;) {
         fr.clear();
         owner.clear();
-        chunks = 0;
+        int64_t chunks = 0;
         for (size_t i = 0; i < J.size(); ++i) {
             for (const HsRange &r : J[i].dev) {
                 const int64_t nc = (r.last - r.first + kHsChunk - 1) / kHsChunk;
@@ -727,12 +741,14 @@ void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int
         const HsFrame *F = ftab.as<HsFrame>();
         HsOut *O = otab.as<HsOut>();
         const dim3 g((unsigned)chunks), b(kHsThreads);
-        hipLaunchKernelGGL(k_hs_pivot, dim3((unsigned)nf), dim3(kWave), 0, st, F, O);
-        hipLaunchKernelGGL(k_hs_count, g, b, 0, st, F, nf, (const HsOut *)O, cl.as<int32_t>(), cr.as<int32_t>());
+        hipLaunchKernelGGL(k_hs_pivot<Total>, dim3((unsigned)nf), dim3(kWave), 0, st, F, O);
+        hipLaunchKernelGGL(k_hs_count<Total>, g, b, 0, st, F, nf, (const HsOut *)O, cl.as<int32_t>(),
+                           cr.as<int32_t>());
         hipLaunchKernelGGL(k_hs_scan, dim3((unsigned)nf), b, 0, st, F, (const int32_t *)cl.as<int32_t>(),
                            (const int32_t *)cr.as<int32_t>(), bl.as<int64_t>(), br.as<int64_t>(), O);
-        hipLaunchKernelGGL(k_hs_scatter, g, b, 0, st, F, nf, (const HsOut *)O, (const int32_t *)cr.as<int32_t>(),
-                           (const int64_t *)bl.as<int64_t>(), (const int64_t *)br.as<int64_t>());
+        hipLaunchKernelGGL(k_hs_scatter<Total>, g, b, 0, st, F, nf, (const HsOut *)O,
+                           (const int32_t *)cr.as<int32_t>(), (const int64_t *)bl.as<int64_t>(),
+                           (const int64_t *)br.as<int64_t>());
         hipLaunchKernelGGL(k_hs_find_m, g, b, 0, st, F, nf, O);
         hipLaunchKernelGGL(k_hs_swap, g, b, 0, st, F, nf, O);
         DMOLP_HIP_CHECK(hipGetLastError());
@@ -749,9 +765,9 @@ void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int
             place(j, HsRange{r.first, cut, r.depth - 1});
         }
     }
-    // The short ranges on the host: copied out, finished by the host loop
-    // (with its depth limit, heap sort and final insertion sort), read off
-    // from the top.
+    // The short ranges: copied out, finished by the host loop (with its
+    // depth limit, heap sort and final insertion sort) or, for the total
+    // order, sorted; then read off from the top.
     OpBatch ob;
     std::vector<std::pair<size_t, size_t>> items;
     for (size_t i = 0; i < J.size(); ++i) {
@@ -760,7 +776,8 @@ void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int
         for (size_t h = 0; h < j.host.size(); ++h) {
             const HsRange &r = j.host[h];
             j.host_data[h].resize((size_t)(r.last - r.first));
-            ob.download(j.host_data[h].data(), j.a.as<DegPair>() + r.first, sizeof(DegPair) * (size_t)(r.last - r.first));
+            ob.download(j.host_data[h].data(), j.a.as<DegPair>() + r.first,
+                        sizeof(DegPair) * (size_t)(r.last - r.first));
             items.push_back({i, h});
         }
     }
@@ -771,9 +788,13 @@ void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int
         for (size_t t = next.fetch_add(1); t < items.size(); t = next.fetch_add(1)) {
             HsJob &j = J[items[t].first];
             const HsRange &r = j.host[items[t].second];
-            const int64_t len = r.last - r.first;
-            const int64_t keep = j.keep > r.first ? j.keep - r.first : 0;
-            introsort_from(j.host_data[items[t].second].data(), 0, len, r.depth, keep, 1);
+            std::vector<DegPair> &d = j.host_data[items[t].second];
+            if (Total) {
+                std::sort(d.begin(), d.end(), ByDegreeThenId());
+            } else {
+                const int64_t keep = j.keep > r.first ? j.keep - r.first : 0;
+                introsort_from(d.data(), 0, r.last - r.first, r.depth, keep, 1);
+            }
         }
     });
     for (HsJob &j : J) {
@@ -781,22 +802,77 @@ void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int
         std::vector<size_t> ord(j.host.size());
         for (size_t h = 0; h < ord.size(); ++h) ord[h] = h;
         std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return j.host[x].first > j.host[y].first; });
-        const int64_t M = j.s->n - j.keep;
-        j.s->hubs.reserve((size_t)M);
-        j.s->hub_deg.reserve((size_t)M);
-        int64_t at = j.s->n;  // the ranges tile [keep, n) from the top down
+        j.top.reserve((size_t)(j.n - j.keep));
+        int64_t at = j.n;  // the ranges tile [keep, n) from the top down
         for (size_t h : ord) {
             const HsRange &r = j.host[h];
             DMOLP_REQUIRE(r.last == at, DMOLP_E_INTERNAL, "hub ranges do not tile the top");
             const std::vector<DegPair> &d = j.host_data[h];
-            for (int64_t p = r.last - 1; p >= r.first && p >= j.keep; --p) {
-                j.s->hubs.push_back(d[(size_t)(p - r.first)].first);
-                j.s->hub_deg.push_back(d[(size_t)(p - r.first)].second);
-            }
+            for (int64_t p = r.last - 1; p >= r.first && p >= j.keep; --p) j.top.push_back(d[(size_t)(p - r.first)]);
             at = r.first;
         }
-        DMOLP_REQUIRE(at <= j.keep && (int64_t)j.s->hubs.size() == M, DMOLP_E_INTERNAL,
+        DMOLP_REQUIRE(at <= j.keep && (int64_t)j.top.size() == j.n - j.keep, DMOLP_E_INTERNAL,
                       "hub ranges do not cover the top");
+        j.host_data.clear();
+    }
+}
+
+}  // namespace
+
+void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int64_t host_max,
+                       int threads) {
+    std::vector<HsJob> J(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        HubSearch &s = *jobs[i];
+        s.hubs.clear();
+        s.hub_deg.clear();
+        s.n_hubs = 0;
+        s.fs = {-1, -1};
+        J[i].n = s.n;
+        J[i].deg = s.deg;
+    }
+    std::vector<unsigned long long> q;
+    hs_setup(ctx, J, q);
+    for (size_t i = 0; i < J.size(); ++i) {
+        HsJob &j = J[i];
+        HubSearch &s = *jobs[i];
+        j.keep = j.n;
+        if (j.n == 0) continue;
+        const int32_t D = (int32_t)(q[3 * i] >> 32);
+        const int64_t vmin = (int64_t)(0xffffffffull - (q[3 * i] & 0xffffffffull));
+        const int64_t vmax = (int64_t)(q[3 * i + 1] & 0xffffffffull);
+        s.fs = first_seed_from(D, vmin, vmax);
+        int64_t M = hub_count(j.n, (int64_t)q[3 * i + 2], D);
+        s.n_hubs = M;
+        if (s.limit >= 0 && s.limit < M) M = s.limit;
+        j.keep = j.n - M;
+    }
+    hs_run<false>(ctx, J, host_max, threads);
+    for (size_t i = 0; i < J.size(); ++i) {
+        HubSearch &s = *jobs[i];
+        s.hubs.reserve(J[i].top.size());
+        s.hub_deg.reserve(J[i].top.size());
+        for (const DegPair &e : J[i].top) {
+            s.hubs.push_back(e.first);
+            s.hub_deg.push_back(e.second);
+        }
+    }
+}
+
+void top_by_degree_device(dmolp_ctx *ctx, const std::vector<TopSearch *> &jobs, int64_t host_max,
+                          int threads) {
+    std::vector<HsJob> J(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        J[i].n = jobs[i]->n;
+        J[i].deg = jobs[i]->deg;
+        J[i].keep = J[i].n - std::min<int64_t>(std::max<int64_t>(jobs[i]->m, 0), J[i].n);
+    }
+    std::vector<unsigned long long> q;
+    hs_setup(ctx, J, q);
+    hs_run<true>(ctx, J, host_max, threads);
+    for (size_t i = 0; i < J.size(); ++i) {
+        jobs[i]->top.clear();
+        for (const DegPair &e : J[i].top) jobs[i]->top.push_back(e.first);
     }
 }
 

@@ -64,4 +64,16 @@ struct HubSearch {
 void hub_search_device(dmolp_ctx *ctx, const std::vector<HubSearch *> &jobs, int64_t host_max,
                        int threads);
 
+// The first m vertices by (degree descending, id ascending), with the same
+// partition steps under that total order (the device does the long ranges,
+// the host sorts the short ones).
+struct TopSearch {
+    int64_t n = 0;
+    const int32_t *deg = nullptr;  // device, n entries
+    int64_t m = 0;
+    std::vector<int32_t> top;      // min(m, n) ids in that order
+};
+void top_by_degree_device(dmolp_ctx *ctx, const std::vector<TopSearch *> &jobs, int64_t host_max,
+                          int threads);
+
 }  // namespace dmolp

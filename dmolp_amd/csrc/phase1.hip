@@ -906,17 +906,6 @@ void p1_hub_search_device(dmolp_ctx *ctx, std::vector<P1Stage *> &H) {
     }
 }
 
-// The degrees of device-search jobs on the host, for the seed fallback.
-void p1_host_degrees(dmolp_ctx *ctx, const std::vector<P1Stage *> &H) {
-    OpBatch ob;
-    for (P1Stage *h : H)
-        if ((int64_t)h->deg.size() != h->g->n) {
-            h->deg.resize((size_t)h->g->n);
-            ob.download(h->deg.data(), h->w->deg.p, sizeof(int32_t) * (size_t)h->g->n);
-        }
-    ob.run(ctx);
-}
-
 // Candidates of the seed fallback (p1_seeds, only for a search that found
 // fewer than k seeds): the first 2k vertices by (degree descending, id
 // ascending).  At most k of them can already be seeds, so the first
@@ -1208,15 +1197,30 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
         if (found < k && h.top.empty()) short_of.push_back(&h);
     }
     if (!short_of.empty()) {
-        p1_host_degrees(ctx, short_of);
+        // Graphs searched on the device select their candidates there.
+        std::vector<TopSearch> ts;
+        std::vector<P1Stage *> on_dev, on_host;
+        for (P1Stage *h : short_of) (h->dev_hubs ? on_dev : on_host).push_back(h);
+        if (!on_dev.empty()) {
+            ts.resize(on_dev.size());
+            std::vector<TopSearch *> tp;
+            for (size_t i = 0; i < on_dev.size(); ++i) {
+                ts[i].n = on_dev[i]->g->n;
+                ts[i].deg = on_dev[i]->w->deg.as<int32_t>();
+                ts[i].m = std::min<int64_t>(ts[i].n, 2 * (int64_t)on_dev[i]->k);
+                tp.push_back(&ts[i]);
+            }
+            top_by_degree_device(ctx, tp, ctx->hub_host_max, std::min(16, host_threads()));
+            for (size_t i = 0; i < on_dev.size(); ++i) on_dev[i]->top = std::move(ts[i].top);
+        }
         const int T = std::min(16, host_threads());
-        if (short_of.size() == 1) {
-            p1_fallback_candidates(*short_of[0], T);
-        } else {
+        if (on_host.size() == 1) {
+            p1_fallback_candidates(*on_host[0], T);
+        } else if (!on_host.empty()) {
             std::atomic<size_t> next{0};
-            run_threads((int)std::min<size_t>((size_t)T, short_of.size()), [&](int) {
-                for (size_t i = next.fetch_add(1); i < short_of.size(); i = next.fetch_add(1))
-                    p1_fallback_candidates(*short_of[i], 1);
+            run_threads((int)std::min<size_t>((size_t)T, on_host.size()), [&](int) {
+                for (size_t i = next.fetch_add(1); i < on_host.size(); i = next.fetch_add(1))
+                    p1_fallback_candidates(*on_host[i], 1);
             });
         }
     }

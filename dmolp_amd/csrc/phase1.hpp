@@ -89,7 +89,6 @@ void p1_set_build(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);
 void p1_degrees(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);  // degrees to the host
 void p1_hub_search(P1Stage &h, int threads = 1);       // host: hubs, first seed (dev_hubs: none)
 void p1_hub_search_device(dmolp_ctx *ctx, std::vector<P1Stage *> &H);  // the jobs with dev_hubs
-void p1_host_degrees(dmolp_ctx *ctx, const std::vector<P1Stage *> &H);  // deg of device-search jobs
 void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);    // hub-ball seeds + fallback
 void p1_expand_begin(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd);
 void p1_expand_rounds(dmolp_ctx *ctx, P1SetDev &sd, int rounds);          // rounds: even

@@ -235,6 +235,13 @@ int dmolp_hubs_device(dmolp_ctx *ctx, const int32_t *deg, int64_t n, int64_t lim
  * 2^18 and 2^15).  The results do not depend on either.  Test hook; no
  * counterpart in the reference. */
 int dmolp_ctx_set_hub_device(dmolp_ctx *ctx, int64_t dev_min, int64_t host_max);
+/* The seed fallback's candidates as dmolp_phase1 selects them on the device
+ * for graphs it searches there: the first min(m, n) vertices of deg[0, n)
+ * by (degree descending, id ascending), in that order, into out (capacity
+ * min(m, n)).  Ranges shorter than host_max are sorted on the host.  The
+ * fallback extends seed.cpp:214-217 (fewer than k seeds); test hook. */
+int dmolp_top_degree_device(dmolp_ctx *ctx, const int32_t *deg, int64_t n, int64_t m, int64_t host_max,
+                            int32_t *out);
 
 /* Phase1Metrics (graph_types.h:276-309) with int64 counters. */
 typedef struct {

@@ -186,3 +186,19 @@ def test_served_with_device_hubs_equals_host_hubs(dev):
         assert served.result.status == 0
         assert np.array_equal(labels, w.labels)
         assert served.result.edge_cut == w.metrics.edge_cut
+
+
+@pytest.mark.parametrize("host_max", [17, 1000, 1 << 15])
+def test_device_top_by_degree_equals_sorted_order(ctx, host_max):
+    """The seed fallback's candidates: the first m vertices by (degree
+    descending, id ascending), against numpy's lexicographic sort, on
+    sequences with many ties at the cut."""
+    rng = np.random.default_rng(host_max)
+    seqs = [np.full(5000, 7), rng.integers(0, 4, 30000), np.minimum(rng.zipf(1.7, 300_000), 10**5),
+            rng.integers(0, 3, 1_200_000), (np.arange(100_000) % 97)[::-1].copy()]
+    for deg in seqs:
+        deg = np.asarray(deg, np.int32)
+        order = np.lexsort((np.arange(len(deg)), -deg.astype(np.int64)))
+        for m in (1, 2, 33, 256, 2048):
+            got = ctx.top_degree_device(deg, m, host_max)
+            assert np.array_equal(got, order[:m]), (len(deg), m)
