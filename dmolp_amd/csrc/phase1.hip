@@ -57,6 +57,7 @@ __device__ __forceinline__ P1Job gload(const P1Job &v) {
     r.touched = gload(v.touched);
     r.touched_cnt = gload(v.touched_cnt);
     r.covered = gload(v.covered);
+    r.lab = gload(v.lab);
     r.deg = gload(v.deg);
     r.best = gload(v.best);
     r.sizes = gload(v.sizes);
@@ -78,6 +79,17 @@ __device__ __forceinline__ P1Job gload(const P1Job &v) {
 __device__ __forceinline__ void mark_bit(uint32_t *bits, int32_t v) {
     const uint32_t m = 1u << (v & 31);
     if (!(bits[v >> 5] & m)) atomicOr(&bits[v >> 5], m);
+}
+
+// The labelled bitmap: the expansion tests bits (n / 8 bytes, which stay in
+// L2) instead of gathering 4-byte labels.  Every phase-1 kernel that writes
+// labels sets their bits in the same kernel (k_p1_rounds_wg rebuilds the
+// bitmap from the labels at its end).
+__device__ __forceinline__ void set_labelled(uint32_t *lab, int32_t v) {
+    atomicOr(&lab[v >> 5], 1u << (v & 31));
+}
+__device__ __forceinline__ bool is_labelled(const uint32_t *lab, int32_t v) {
+    return (lab[v >> 5] >> (v & 31)) & 1u;
 }
 
 // ---------------------------------------------------------------- seeds
@@ -238,6 +250,7 @@ __global__ void k_expand_init(const P1Job *JS, const int32_t *bstart, int jobs) 
     for (int p = threadIdx.x; p < k; p += blockDim.x) {
         const int32_t s = seeds[p];
         L[s] = p;  // partition.cpp:346-352
+        set_labelled(J.lab, s);
         f.x[p] = s;
         f.g[p] = p;
         f.d[p] = (int32_t)(g.off[s + 1] - g.off[s]);
@@ -318,6 +331,7 @@ __global__ void k_p1_fill(const P1Job *JS, const int32_t *bstart, int jobs, int 
             if (!un[j]) continue;
             const int64_t v = b0 + (int64_t)j * blockDim.x + threadIdx.x;
             L[v] = b;
+            set_labelled(J.lab, (int32_t)v);
             fcur.x[pos] = (int32_t)v;
             fcur.g[pos] = b;
             fcur.d[pos] = (int32_t)(g.off[v + 1] - g.off[v]);
@@ -379,6 +393,7 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
     const GraphDev g = J.g;
     const ListDev sel = J.sel_list;
     const int32_t *L = J.L;
+    const uint32_t *lab = J.lab;
     int32_t *claims = J.claims;
     int32_t *touched = J.touched;
     int64_t *touched_cnt = J.touched_cnt;
@@ -399,8 +414,8 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
         for (int32_t q0 = 0; q0 < total; q0 += kWave * kExpUnroll) {
             // Columns, then the neighbours' labels and claims, each stage's
             // loads from in-bounds indices so they issue together.
-            int32_t u[kExpUnroll], p[kExpUnroll], lv[kExpUnroll], cl[kExpUnroll];
-            bool on[kExpUnroll];
+            int32_t u[kExpUnroll], p[kExpUnroll], cl[kExpUnroll];
+            bool on[kExpUnroll], lb[kExpUnroll];
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) {
                 const int32_t q = q0 + j * kWave + lane;
@@ -411,16 +426,16 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
                 p[j] = s_b[w].aux[slot];
             }
 #pragma unroll
-            for (int j = 0; j < kExpUnroll; ++j) lv[j] = L[u[j]];
+            for (int j = 0; j < kExpUnroll; ++j) lb[j] = is_labelled(lab, u[j]);
             // Claims only of unlabelled neighbours: a labelled one's lane
             // reads claims[0] (one line shared by the wave), so most of a
-            // late round's entries cost one gather, not two.
+            // late round's entries cost one bitmap read, not a gather.
 #pragma unroll
-            for (int j = 0; j < kExpUnroll; ++j) cl[j] = claims[lv[j] == -1 ? u[j] : 0];
+            for (int j = 0; j < kExpUnroll; ++j) cl[j] = claims[lb[j] ? 0 : u[j]];
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) {
                 bool first = false;
-                if (on[j] && lv[j] == -1 && cl[j] > p[j])
+                if (on[j] && !lb[j] && cl[j] > p[j])
                     first = atomicMin(&claims[u[j]], p[j]) == kNoClaim;
                 touch_stage(first, u[j], s_t[w], touched, touched_cnt);
             }
@@ -436,6 +451,7 @@ __global__ __launch_bounds__(kThreads) void k_expand_heavy(const P1Job *JS, cons
     const ListDev sel = J.sel_list;
     const int chunk_edges = J.chunk_edges;
     const int32_t *L = J.L;
+    const uint32_t *lab = J.lab;
     int32_t *claims = J.claims;
     int32_t *touched = J.touched;
     int64_t *touched_cnt = J.touched_cnt;
@@ -452,8 +468,8 @@ __global__ __launch_bounds__(kThreads) void k_expand_heavy(const P1Job *JS, cons
         chunk_bounds(g, sel, sel.hc[j0], chunk_edges, slot, v, e0, e1, deg);
         const int32_t p = L[v];
         for (int64_t eb = e0; eb < e1; eb += (int64_t)kWave * kExpUnroll) {
-            int32_t u[kExpUnroll], lv[kExpUnroll], cl[kExpUnroll];
-            bool on[kExpUnroll];
+            int32_t u[kExpUnroll], cl[kExpUnroll];
+            bool on[kExpUnroll], lb[kExpUnroll];
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) {
                 const int64_t e = eb + j * kWave + lane;
@@ -461,13 +477,13 @@ __global__ __launch_bounds__(kThreads) void k_expand_heavy(const P1Job *JS, cons
                 u[j] = g.col[on[j] ? e : e0];
             }
 #pragma unroll
-            for (int j = 0; j < kExpUnroll; ++j) lv[j] = L[u[j]];
+            for (int j = 0; j < kExpUnroll; ++j) lb[j] = is_labelled(lab, u[j]);
 #pragma unroll
-            for (int j = 0; j < kExpUnroll; ++j) cl[j] = claims[lv[j] == -1 ? u[j] : 0];
+            for (int j = 0; j < kExpUnroll; ++j) cl[j] = claims[lb[j] ? 0 : u[j]];
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) {
                 bool first = false;
-                if (on[j] && lv[j] == -1 && cl[j] > p) first = atomicMin(&claims[u[j]], p) == kNoClaim;
+                if (on[j] && !lb[j] && cl[j] > p) first = atomicMin(&claims[u[j]], p) == kNoClaim;
                 touch_stage(first, u[j], s_t[w], touched, touched_cnt);
             }
         }
@@ -502,6 +518,7 @@ __global__ __launch_bounds__(kThreads) void k_p1_apply(const P1Job *JS, const in
             p = claims[v];
             claims[v] = kNoClaim;
             L[v] = p;
+            set_labelled(J.lab, v);
             atomicAdd(&s_sz[p], 1ull);
         }
         const int64_t pos = wave_append(on, fnext.cnt);
@@ -742,6 +759,20 @@ __global__ __launch_bounds__(kWgThreads) void k_p1_rounds_wg(const P1Job *JS, in
         if (tid == 0) ctl->rounds += 1;  // k_p1_round_end
     }
     __syncthreads();
+    // The labelled bitmap the separate launches test, rebuilt from the labels
+    // once the rounds here have changed them (one atomic a labelled vertex
+    // inside the rounds cost more than this pass).
+    if (t > 0 && st != 2) {
+        const int64_t nw = (n + 31) / 32;
+        for (int64_t i = tid; i < nw; i += blockDim.x) {
+            uint32_t word = 0;
+            for (int b = 0; b < 32; ++b) {
+                const int64_t v = i * 32 + b;
+                if (v < n && ld_rel(J.L + v) != -1) word |= 1u << b;
+            }
+            J.lab[i] = word;
+        }
+    }
     // Leave the frontier in F[0] for the separate launches.
     if ((t & 1) && st != 2) {
         const RecDev src = J.F[1], dst = J.F[0];
@@ -803,6 +834,7 @@ Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.F[0].alloc(n);
         w.F[1].alloc(n);
         w.covered.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
+        w.lab.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
         w.deg.alloc(sizeof(int32_t) * (size_t)n);
         w.ctl.alloc(sizeof(P1Ctl));
         w.best.alloc(sizeof(unsigned long long));
@@ -846,6 +878,7 @@ P1Job p1_job(dmolp_ctx *ctx, dmolp_graph *g, Phase1Work &w, int32_t *remaining) 
     j.touched = w.touched.as<int32_t>();
     j.touched_cnt = w.touched_cnt.as<int64_t>();
     j.covered = w.covered.as<uint32_t>();
+    j.lab = w.lab.as<uint32_t>();
     j.deg = w.deg.as<int32_t>();
     j.best = w.best.as<unsigned long long>();
     j.sizes = w.sizes.as<unsigned long long>();
@@ -1253,6 +1286,7 @@ void p1_expand_begin(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     OpBatch ob;
     for (P1Stage *h : H) {
         ob.fill(h->g->labels.p, 0xff, sizeof(int32_t) * (size_t)h->g->n);
+        ob.fill(h->w->lab.p, 0, h->w->lab.bytes);
         ob.fill(h->w->sizes.p, 0, h->w->sizes.bytes);
         ob.fill(h->w->ctl.p, 0, sizeof(P1Ctl));
     }

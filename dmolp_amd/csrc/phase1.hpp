@@ -25,6 +25,7 @@ struct Phase1Work {
     int k = 0;
     int64_t cap_heavy = -1, cap_chunks = -1;
     DevMem claims, touched, touched_cnt, covered, deg, ctl, best, sizes, seeds;
+    DevMem lab;  // bit v: vertex v is labelled (the expansion's test, L2-resident)
     DevMem hubs, used;  // hub list of the current run and its used flags
     Recs F[2];
     List sel_list, lvl1;
@@ -39,6 +40,7 @@ struct P1Job {
     int32_t *L, *claims, *touched;
     int64_t *touched_cnt;
     uint32_t *covered;
+    uint32_t *lab;  // labelled-vertex bitmap, kept in step with L
     int32_t *deg;
     unsigned long long *best, *sizes;
     int32_t *seeds;
