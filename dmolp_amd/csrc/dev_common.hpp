@@ -72,6 +72,20 @@ __device__ __forceinline__ uint32_t wave_seg_incl(uint32_t x) {
     return x;
 }
 
+// Inclusive max-scan over the 64 lanes for values >= -1 (all lanes must
+// participate): the same DPP steps as wave_seg_incl, with -1 for lanes that
+// have no source.
+__device__ __forceinline__ int32_t wave_max_incl(int32_t x) {
+    const auto step = [](int32_t x, int32_t y) { return y > x ? y : x; };
+    x = step(x, __builtin_amdgcn_update_dpp(-1, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = step(x, __builtin_amdgcn_update_dpp(-1, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = step(x, __builtin_amdgcn_update_dpp(-1, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = step(x, __builtin_amdgcn_update_dpp(-1, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = step(x, __builtin_amdgcn_update_dpp(-1, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = step(x, __builtin_amdgcn_update_dpp(-1, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
+
 // Append the lanes with `pred` to list[*count ...]: one atomic per wave.
 // Returns this lane's slot (or -1).  All lanes must call it.
 __device__ __forceinline__ int64_t wave_append(bool pred, unsigned long long *count) {

@@ -401,6 +401,7 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
     if (p1_done(ctl)) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
     __shared__ TouchLds s_t[kWavesPerBlock];
+    __shared__ int32_t s_mark[kWavesPerBlock][kWave];
     const int w = threadIdx.x / kWave, lane = lane_id();
     if (lane == 0) s_t[w].n = 0;
     wave_sync();
@@ -410,6 +411,8 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
         bool valid;
         const int32_t total = batch_load(g, sel.light, count, b0, s_b[w], v, deg, valid);
         s_b[w].aux[lane] = valid ? L[v] : 0;
+        s_mark[w][lane] = -1;
+        const int32_t rend = s_b[w].incl[lane], rstart = rend - deg;  // this lane's row
         wave_sync();
         for (int32_t q0 = 0; q0 < total; q0 += kWave * kExpUnroll) {
             // Columns, then the neighbours' labels and claims, each stage's
@@ -420,7 +423,8 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
             for (int j = 0; j < kExpUnroll; ++j) {
                 const int32_t q = q0 + j * kWave + lane;
                 on[j] = q < total;
-                const int slot = on[j] ? batch_slot(s_b[w], q) : 0;
+                const int ws = batch_slot_window(s_mark[w], q0 + j * kWave, rstart, rend);
+                const int slot = on[j] ? ws : 0;
                 const int64_t e = s_b[w].base[slot] + q;
                 u[j] = g.col[on[j] ? e : 0];
                 p[j] = s_b[w].aux[slot];
@@ -432,13 +436,21 @@ __global__ __launch_bounds__(kThreads) void k_expand_light(const P1Job *JS, cons
             // late round's entries cost one bitmap read, not a gather.
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) cl[j] = claims[lb[j] ? 0 : u[j]];
+            // The claims' atomics issued together, then their results.
+            bool want[kExpUnroll];
+            int32_t prev[kExpUnroll];
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) {
-                bool first = false;
-                if (on[j] && !lb[j] && cl[j] > p[j])
-                    first = atomicMin(&claims[u[j]], p[j]) == kNoClaim;
-                touch_stage(first, u[j], s_t[w], touched, touched_cnt);
+                want[j] = on[j] && !lb[j] && cl[j] > p[j];
+                prev[j] = want[j] ? atomicMin(&claims[u[j]], p[j]) : 0;
             }
+            // (Keeps each result's compare out of its atomic's branch, where
+            // it would wait for that atomic before the next one issues.)
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) asm volatile("" : "+v"(prev[j]));
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j)
+                touch_stage(want[j] && prev[j] == kNoClaim, u[j], s_t[w], touched, touched_cnt);
         }
         wave_sync();
     }
@@ -480,12 +492,18 @@ __global__ __launch_bounds__(kThreads) void k_expand_heavy(const P1Job *JS, cons
             for (int j = 0; j < kExpUnroll; ++j) lb[j] = is_labelled(lab, u[j]);
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) cl[j] = claims[lb[j] ? 0 : u[j]];
+            bool want[kExpUnroll];
+            int32_t prev[kExpUnroll];
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) {
-                bool first = false;
-                if (on[j] && !lb[j] && cl[j] > p) first = atomicMin(&claims[u[j]], p) == kNoClaim;
-                touch_stage(first, u[j], s_t[w], touched, touched_cnt);
+                want[j] = on[j] && !lb[j] && cl[j] > p;
+                prev[j] = want[j] ? atomicMin(&claims[u[j]], p) : 0;
             }
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) asm volatile("" : "+v"(prev[j]));
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j)
+                touch_stage(want[j] && prev[j] == kNoClaim, u[j], s_t[w], touched, touched_cnt);
         }
     }
     touch_flush(s_t[w], touched, touched_cnt);

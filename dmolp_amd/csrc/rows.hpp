@@ -66,6 +66,24 @@ __device__ __forceinline__ int batch_slot(const BatchLds &lds, int32_t q) {
     return lo;
 }
 
+// Slots owning the 64 positions w0 + lane of a batch, all lanes at once
+// (every lane must call; positions at or past the batch's total get some
+// slot below 64).  Lane r holds its row's concatenated range [start, end);
+// the row covering w0 and every row starting inside the window mark their
+// first position in `mark` (kWave ints of LDS, set to -1 at each batch
+// start), and a max-scan over the window carries each mark forward.  Marks
+// an earlier window of the same batch left name earlier rows, so they never
+// win the max.  One LDS write and read and six DPP steps instead of a
+// six-step binary search of dependent LDS reads per position.
+__device__ __forceinline__ int batch_slot_window(int32_t *mark, int32_t w0, int32_t start, int32_t end) {
+    const int lane = lane_id();
+    const int32_t p = start > w0 ? start : w0;
+    if (p < w0 + kWave && end > p) mark[p - w0] = lane;
+    wave_sync();
+    const int32_t x = wave_max_incl(mark[lane]);
+    return x < 0 ? 0 : x;
+}
+
 // Append v to a list (all lanes of the wave must call; pred selects lanes).
 __device__ __forceinline__ void list_append(const ListDev &L, bool pred, int32_t v, int64_t deg,
                                             int heavy_degree, int chunk_edges) {
