@@ -39,6 +39,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr int32_t kNoClaim = 0x7f7f7f7f;  // claims are reset with a 0x7f byte fill
+constexpr int kExpUnroll = 4;  // row entries per lane per step, loads issued together
 
 __device__ __forceinline__ bool p1_done(const P1Ctl *c) { return c->done != 0; }
 
@@ -135,15 +136,37 @@ __global__ __launch_bounds__(kThreads) void k_ball2_light(const P1Job *JS, const
     if (step >= J.seed_steps) return;
     if (ctl->seed_stop) return;
     __shared__ BatchLds s_b[kWavesPerBlock];
+    __shared__ int32_t s_mark[kWavesPerBlock][kWave];
     const int w = threadIdx.x / kWave, lane = lane_id();
     const int64_t count = (int64_t)lvl1.cnt[LC_LIGHT];
     for (int64_t b0 = vwave(vg) * kWave; b0 < count; b0 += vwaves(vg) * kWave) {
         int32_t v, deg;
         bool valid;
         const int32_t total = batch_load(g, lvl1.light, count, b0, s_b[w], v, deg, valid);
-        for (int32_t q0 = 0; q0 < total; q0 += kWave) {
-            const int32_t q = q0 + lane;
-            if (q < total) mark_bit(covered, g.col[s_b[w].base[batch_slot(s_b[w], q)] + q]);
+        s_mark[w][lane] = -1;
+        const int32_t rend = s_b[w].incl[lane], rstart = rend - deg;  // this lane's row
+        wave_sync();
+        // kExpUnroll entries a lane per step: columns, then the bitmap words,
+        // each stage's loads issued together; the marks go out without
+        // waiting for their results.
+        for (int32_t q0 = 0; q0 < total; q0 += kWave * kExpUnroll) {
+            int32_t u[kExpUnroll];
+            bool on[kExpUnroll];
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) {
+                const int32_t q = q0 + j * kWave + lane;
+                on[j] = q < total;
+                const int ws = batch_slot_window(s_mark[w], q0 + j * kWave, rstart, rend);
+                u[j] = g.col[on[j] ? s_b[w].base[on[j] ? ws : 0] + q : 0];
+            }
+            uint32_t word[kExpUnroll];
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) word[j] = covered[u[j] >> 5];
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) {
+                const uint32_t m = 1u << (u[j] & 31);
+                if (on[j] && !(word[j] & m)) atomicOr(&covered[u[j] >> 5], m);
+            }
         }
         wave_sync();
     }
@@ -163,7 +186,24 @@ __global__ void k_ball2_heavy(const P1Job *JS, const int32_t *bstart, int jobs, 
         int32_t slot, v;
         int64_t e0, e1, deg;
         chunk_bounds(g, lvl1, lvl1.hc[j], chunk_edges, slot, v, e0, e1, deg);
-        for (int64_t e = e0 + lane_id(); e < e1; e += kWave) mark_bit(covered, g.col[e]);
+        for (int64_t eb = e0; eb < e1; eb += (int64_t)kWave * kExpUnroll) {
+            int32_t u[kExpUnroll];
+            bool on[kExpUnroll];
+#pragma unroll
+            for (int q = 0; q < kExpUnroll; ++q) {
+                const int64_t e = eb + q * kWave + lane_id();
+                on[q] = e < e1;
+                u[q] = g.col[on[q] ? e : e0];
+            }
+            uint32_t word[kExpUnroll];
+#pragma unroll
+            for (int q = 0; q < kExpUnroll; ++q) word[q] = covered[u[q] >> 5];
+#pragma unroll
+            for (int q = 0; q < kExpUnroll; ++q) {
+                const uint32_t m = 1u << (u[q] & 31);
+                if (on[q] && !(word[q] & m)) atomicOr(&covered[u[q] >> 5], m);
+            }
+        }
     }
 }
 
@@ -353,7 +393,6 @@ __global__ void k_p1_fill(const P1Job *JS, const int32_t *bstart, int jobs, int 
 // kTouchStage - kWave entries (one counter hit by every wave each step
 // serialises at the L2).
 constexpr int kTouchStage = 1024;
-constexpr int kExpUnroll = 4;  // entries per lane per step, loads issued together
 
 struct TouchLds {
     int32_t q[kTouchStage];
