@@ -59,6 +59,7 @@ __device__ __forceinline__ P1Job gload(const P1Job &v) {
     r.touched_cnt = gload(v.touched_cnt);
     r.covered = gload(v.covered);
     r.lab = gload(v.lab);
+    r.hubbits = gload(v.hubbits);
     r.deg = gload(v.deg);
     r.best = gload(v.best);
     r.sizes = gload(v.sizes);
@@ -95,6 +96,21 @@ __device__ __forceinline__ bool is_labelled(const uint32_t *lab, int32_t v) {
 
 // ---------------------------------------------------------------- seeds
 
+// The seed search asks only whether a listed hub lies in a ball (seed.cpp:
+// 154-169), so the balls mark listed hubs only: their other members cost a
+// read of the hub bitmap (n / 8 bytes, L2-resident) instead of an atomic,
+// and the atomics left fall on the hubs' few bitmap words.
+__device__ __forceinline__ bool is_hub(const uint32_t *hubbits, int32_t v) {
+    return (hubbits[v >> 5] >> (v & 31)) & 1u;
+}
+__global__ void k_hub_bits(const P1Job *JS, const int32_t *bstart, int jobs) {
+    P1_JOB();
+    for (int64_t i = vthread(vg); i < J.nh; i += vthreads(vg)) {
+        const int32_t h = J.hubs[i];
+        atomicOr(&J.hubbits[h >> 5], 1u << (h & 31));
+    }
+}
+
 // Level 0 and 1 of the ball of seeds[j] (seed.cpp:26-29, 39-95).
 __global__ void k_ball1(const P1Job *JS, const int32_t *bstart, int jobs, int step) {
     P1_JOB();
@@ -119,7 +135,7 @@ __global__ void k_ball1(const P1Job *JS, const int32_t *bstart, int jobs, int st
         int64_t d = 0;
         if (on) {
             u = g.col[e];
-            mark_bit(covered, u);
+            if (is_hub(J.hubbits, u)) mark_bit(covered, u);
             d = g.off[u + 1] - g.off[u];
         }
         list_append(lvl1, on, u, d, heavy_degree, chunk_edges);
@@ -132,6 +148,7 @@ __global__ __launch_bounds__(kThreads) void k_ball2_light(const P1Job *JS, const
     const GraphDev g = J.g;
     const ListDev lvl1 = J.lvl1;
     uint32_t *covered = J.covered;
+    const uint32_t *hubbits = J.hubbits;
     const P1Ctl *ctl = J.ctl;
     if (step >= J.seed_steps) return;
     if (ctl->seed_stop) return;
@@ -159,13 +176,16 @@ __global__ __launch_bounds__(kThreads) void k_ball2_light(const P1Job *JS, const
                 const int ws = batch_slot_window(s_mark[w], q0 + j * kWave, rstart, rend);
                 u[j] = g.col[on[j] ? s_b[w].base[on[j] ? ws : 0] + q : 0];
             }
+            bool hub[kExpUnroll];
+#pragma unroll
+            for (int j = 0; j < kExpUnroll; ++j) hub[j] = on[j] && is_hub(hubbits, u[j]);
             uint32_t word[kExpUnroll];
 #pragma unroll
-            for (int j = 0; j < kExpUnroll; ++j) word[j] = covered[u[j] >> 5];
+            for (int j = 0; j < kExpUnroll; ++j) word[j] = covered[hub[j] ? u[j] >> 5 : 0];
 #pragma unroll
             for (int j = 0; j < kExpUnroll; ++j) {
                 const uint32_t m = 1u << (u[j] & 31);
-                if (on[j] && !(word[j] & m)) atomicOr(&covered[u[j] >> 5], m);
+                if (hub[j] && !(word[j] & m)) atomicOr(&covered[u[j] >> 5], m);
             }
         }
         wave_sync();
@@ -178,6 +198,7 @@ __global__ void k_ball2_heavy(const P1Job *JS, const int32_t *bstart, int jobs, 
     const ListDev lvl1 = J.lvl1;
     const int chunk_edges = J.chunk_edges;
     uint32_t *covered = J.covered;
+    const uint32_t *hubbits = J.hubbits;
     const P1Ctl *ctl = J.ctl;
     if (step >= J.seed_steps) return;
     if (ctl->seed_stop) return;
@@ -195,13 +216,16 @@ __global__ void k_ball2_heavy(const P1Job *JS, const int32_t *bstart, int jobs, 
                 on[q] = e < e1;
                 u[q] = g.col[on[q] ? e : e0];
             }
+            bool hub[kExpUnroll];
+#pragma unroll
+            for (int q = 0; q < kExpUnroll; ++q) hub[q] = on[q] && is_hub(hubbits, u[q]);
             uint32_t word[kExpUnroll];
 #pragma unroll
-            for (int q = 0; q < kExpUnroll; ++q) word[q] = covered[u[q] >> 5];
+            for (int q = 0; q < kExpUnroll; ++q) word[q] = covered[hub[q] ? u[q] >> 5 : 0];
 #pragma unroll
             for (int q = 0; q < kExpUnroll; ++q) {
                 const uint32_t m = 1u << (u[q] & 31);
-                if (on[q] && !(word[q] & m)) atomicOr(&covered[u[q] >> 5], m);
+                if (hub[q] && !(word[q] & m)) atomicOr(&covered[u[q] >> 5], m);
             }
         }
     }
@@ -892,6 +916,7 @@ Phase1Work &phase1_work(dmolp_ctx *ctx, dmolp_graph *g, int k) {
         w.F[1].alloc(n);
         w.covered.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
         w.lab.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
+        w.hubbits.alloc(sizeof(uint32_t) * (size_t)((n + 31) / 32));
         w.deg.alloc(sizeof(int32_t) * (size_t)n);
         w.ctl.alloc(sizeof(P1Ctl));
         w.best.alloc(sizeof(unsigned long long));
@@ -936,6 +961,7 @@ P1Job p1_job(dmolp_ctx *ctx, dmolp_graph *g, Phase1Work &w, int32_t *remaining) 
     j.touched_cnt = w.touched_cnt.as<int64_t>();
     j.covered = w.covered.as<uint32_t>();
     j.lab = w.lab.as<uint32_t>();
+    j.hubbits = w.hubbits.as<uint32_t>();
     j.deg = w.deg.as<int32_t>();
     j.best = w.best.as<unsigned long long>();
     j.sizes = w.sizes.as<unsigned long long>();
@@ -1208,6 +1234,7 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
         const int64_t nh = (int64_t)h.hubs.size();
         ob.fill(w.ctl.p, 0, sizeof(P1Ctl));
         ob.fill(w.covered.p, 0, w.covered.bytes);
+        ob.fill(w.hubbits.p, 0, w.hubbits.bytes);
         ob.fill(w.best.p, 0, w.best.bytes);
         ob.fill(w.seeds.p, 0xff, w.seeds.bytes);
         ob.upload(w.seeds.p, &h.fs.first, sizeof(int32_t));
@@ -1234,6 +1261,7 @@ void p1_seeds(dmolp_ctx *ctx, std::vector<P1Stage *> &H, P1SetDev &sd) {
     p1_set_build(ctx, H, sd);  // uploads the hub fields
     const LaunchSet &ls = sd.ls;
     P1Job *d_jobs = sd.jobs.as<P1Job>();
+    hipLaunchKernelGGL(k_hub_bits, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G);
     for (int step = 1; step < steps; ++step) {
         // Ball of seed step-1, then the next seed.
         hipLaunchKernelGGL(k_ball1, dim3(ls.total[P_GRID]), dim3(kThreads), 0, s, d_jobs, ls.bs(P_GRID), G, step);

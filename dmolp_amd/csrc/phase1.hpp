@@ -26,6 +26,7 @@ struct Phase1Work {
     int64_t cap_heavy = -1, cap_chunks = -1;
     DevMem claims, touched, touched_cnt, covered, deg, ctl, best, sizes, seeds;
     DevMem lab;  // bit v: vertex v is labelled (the expansion's test, L2-resident)
+    DevMem hubbits;  // bit h: h is a listed hub (the only vertices the balls mark)
     DevMem hubs, used;  // hub list of the current run and its used flags
     Recs F[2];
     List sel_list, lvl1;
@@ -41,6 +42,7 @@ struct P1Job {
     int64_t *touched_cnt;
     uint32_t *covered;
     uint32_t *lab;  // labelled-vertex bitmap, kept in step with L
+    uint32_t *hubbits;  // listed hubs
     int32_t *deg;
     unsigned long long *best, *sizes;
     int32_t *seeds;
